@@ -1,0 +1,1126 @@
+// C API of the MI355X triangle ray-query library.
+//
+// Object model and error semantics follow the reference's API layer
+// (kernels/common/rtcore.cpp, rtcore.h:28-122): opaque reference-counted
+// handles, every entry point converts failures into an error code stored in
+// a per-device, per-thread first-error slot (kernels/common/device.cpp:258-285,
+// state.h:162-175) and reported to the optional error callback. The compute
+// behind rtcCommitScene and the ray queries runs on the GPU (builder.hip,
+// trace.hip); there is no CPU execution path.
+#define RTC_EXPORT_API
+#include <hip/hip_runtime.h>
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "builder.h"
+#include "embree3/rtcore.h"
+#include "embree3/rtcore_mi355x.h"
+#include "trace.h"
+
+namespace {
+
+struct rtc_error {
+  RTCError code;
+  std::string msg;
+};
+#define RTC_THROW(c, m) throw rtc_error{c, m}
+
+struct RefObj {
+  std::atomic<long> refs{1};
+  virtual ~RefObj() {}
+};
+inline void ref_inc(RefObj* o) { o->refs.fetch_add(1); }
+inline void ref_dec(RefObj* o) {
+  if (o->refs.fetch_sub(1) == 1) delete o;
+}
+
+// ---------------------------------------------------------------- device --
+
+struct Device;
+
+struct ThreadDev {
+  hipStream_t stream = nullptr;
+  rtg::DevBuf staging;
+  rtg::DevBuf errword;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+struct Device : RefObj {
+  int ordinal = 0;
+  int verbose = 0;
+  rtg::BuildParams build;
+  RTCErrorFunction errfn = nullptr;
+  void* errptr = nullptr;
+  RTCMemoryMonitorFunction memfn = nullptr;
+  void* memptr = nullptr;
+  std::mutex build_mtx;
+  rtg::BuildScratch scratch;
+  std::mutex tls_mtx;
+  std::unordered_map<std::thread::id, ThreadDev*> threads;
+  ~Device() override {
+    (void)hipSetDevice(ordinal);
+    for (auto& kv : threads) {
+      ThreadDev* t = kv.second;
+      if (t->stream) (void)hipStreamDestroy(t->stream);
+      if (t->ev0) (void)hipEventDestroy(t->ev0);
+      if (t->ev1) (void)hipEventDestroy(t->ev1);
+      delete t;
+    }
+  }
+  void activate() { RTG_HIP(hipSetDevice(ordinal)); }
+  ThreadDev& td() {
+    std::lock_guard<std::mutex> g(tls_mtx);
+    auto it = threads.find(std::this_thread::get_id());
+    if (it != threads.end()) return *it->second;
+    ThreadDev* t = new ThreadDev();
+    RTG_HIP(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
+    RTG_HIP(hipEventCreate(&t->ev0));
+    RTG_HIP(hipEventCreate(&t->ev1));
+    t->errword.reserve(256);
+    RTG_HIP(hipMemsetAsync(t->errword.ptr, 0, 4, t->stream));
+    threads[std::this_thread::get_id()] = t;
+    return *t;
+  }
+  void monitor(ssize_t bytes) {
+    if (memfn && !memfn(memptr, bytes, false)) RTC_THROW(RTC_ERROR_OUT_OF_MEMORY, "memory monitor forced termination");
+  }
+};
+
+// per-thread first-error slots (reference: device.cpp:258-285)
+thread_local std::unordered_map<const Device*, RTCError> tls_errors;
+thread_local RTCError tls_null_error = RTC_ERROR_NONE;
+
+void process_error(Device* d, RTCError code, const char* msg) {
+  if (!d) {
+    if (tls_null_error == RTC_ERROR_NONE) tls_null_error = code;
+    return;
+  }
+  RTCError& e = tls_errors[d];
+  if (e == RTC_ERROR_NONE) e = code;
+  if (d->verbose) fprintf(stderr, "Embree-MI355X: error %d: %s\n", (int)code, msg);
+  if (d->errfn) d->errfn(d->errptr, code, msg);
+}
+
+// ---------------------------------------------------------------- buffer --
+
+struct Buffer : RefObj {
+  Device* dev;
+  char* data;
+  size_t size;
+  bool shared;
+  Buffer(Device* d, void* p, size_t n, bool sh) : dev(d), data((char*)p), size(n), shared(sh) { ref_inc(d); }
+  ~Buffer() override {
+    if (!shared) free(data);
+    ref_dec(dev);
+  }
+};
+
+// -------------------------------------------------------------- geometry --
+
+struct Slot {
+  Buffer* buf = nullptr;  // retained when set through a buffer object
+  const char* ptr = nullptr;
+  size_t stride = 0, count = 0;
+  RTCFormat format = RTC_FORMAT_UNDEFINED;
+  bool set = false;
+  void clear() {
+    if (buf) ref_dec(buf);
+    buf = nullptr;
+    ptr = nullptr;
+    set = false;
+  }
+};
+
+struct Geometry : RefObj {
+  Device* dev;
+  RTCGeometryType type;
+  Slot vertex, index;
+  bool enabled = true;
+  bool committed = false;
+  unsigned mask = 0xffffffffu;
+  RTCBuildQuality quality = RTC_BUILD_QUALITY_MEDIUM;
+  void* user = nullptr;
+  Geometry(Device* d, RTCGeometryType t) : dev(d), type(t) { ref_inc(d); }
+  ~Geometry() override {
+    vertex.clear();
+    index.clear();
+    ref_dec(dev);
+  }
+};
+
+// ----------------------------------------------------------------- scene --
+
+struct Scene : RefObj {
+  Device* dev;
+  std::vector<Geometry*> geoms;
+  RTCSceneFlags flags = RTC_SCENE_FLAG_NONE;
+  RTCBuildQuality quality = RTC_BUILD_QUALITY_MEDIUM;
+  RTCProgressMonitorFunction progress = nullptr;
+  void* progress_ptr = nullptr;
+  bool committed = false;
+  std::mutex mtx;
+  rtg::DeviceBVH bvh;
+  std::vector<rtg::DevBuf*> gdata;
+  std::vector<rtg::GeomDesc> descs;
+  explicit Scene(Device* d) : dev(d) { ref_inc(d); }
+  ~Scene() override {
+    for (Geometry* g : geoms)
+      if (g) ref_dec(g);
+    for (auto* b : gdata) delete b;
+    ref_dec(dev);
+  }
+};
+
+Device* dev_of(Device* d) { return d; }
+Device* dev_of(Buffer* b) { return b ? b->dev : nullptr; }
+Device* dev_of(Geometry* g) { return g ? g->dev : nullptr; }
+Device* dev_of(Scene* s) { return s ? s->dev : nullptr; }
+
+#define API_BEGIN try {
+#define API_END(obj)                                                                                 \
+  }                                                                                                  \
+  catch (const rtc_error& e) {                                                                       \
+    process_error(dev_of(obj), e.code, e.msg.c_str());                                               \
+  }                                                                                                  \
+  catch (const rtg::hip_error& e) {                                                                  \
+    process_error(dev_of(obj), e.code == hipErrorOutOfMemory ? RTC_ERROR_OUT_OF_MEMORY : RTC_ERROR_UNKNOWN, \
+                  e.what());                                                                         \
+  }                                                                                                  \
+  catch (const std::bad_alloc&) {                                                                    \
+    process_error(dev_of(obj), RTC_ERROR_OUT_OF_MEMORY, "out of memory");                            \
+  }                                                                                                  \
+  catch (const std::exception& e) {                                                                  \
+    process_error(dev_of(obj), RTC_ERROR_UNKNOWN, e.what());                                         \
+  }
+
+#define VERIFY_HANDLE(h) \
+  if ((h) == nullptr) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+
+std::mutex g_api_mutex;  // device creation / retain / release (reference rtcore.cpp:20-38)
+
+void parse_config(Device* d, const char* cfg) {
+  if (!cfg) return;
+  std::string s(cfg);
+  size_t pos = 0;
+  while (pos < s.size()) {
+    size_t end = s.find(',', pos);
+    if (end == std::string::npos) end = s.size();
+    std::string kv = s.substr(pos, end - pos);
+    pos = end + 1;
+    size_t eq = kv.find('=');
+    std::string k = kv.substr(0, eq), v = eq == std::string::npos ? "" : kv.substr(eq + 1);
+    while (!k.empty() && k.front() == ' ') k.erase(0, 1);
+    while (!k.empty() && k.back() == ' ') k.pop_back();
+    if (k == "verbose") d->verbose = atoi(v.c_str());
+    else if (k == "gpu") d->ordinal = atoi(v.c_str());
+    else if (k == "bvh_leaf" || k == "max_leaf") d->build.max_leaf = (uint32_t)atoi(v.c_str());
+    else if (k == "cost_node") d->build.cost_node = (float)atof(v.c_str());
+    else if (k == "cost_tri") d->build.cost_tri = (float)atof(v.c_str());
+    // threads, isa, tri_accel, tri_builder, set_affinity, ... are CPU settings: accepted, ignored
+  }
+}
+
+Scene* SC(RTCScene s) { return (Scene*)s; }
+Geometry* GE(RTCGeometry g) { return (Geometry*)g; }
+Device* DE(RTCDevice d) { return (Device*)d; }
+Buffer* BU(RTCBuffer b) { return (Buffer*)b; }
+
+size_t format_size(RTCFormat f) {
+  switch (f) {
+    case RTC_FORMAT_FLOAT3: return 12;
+    case RTC_FORMAT_UINT3: return 12;
+    case RTC_FORMAT_FLOAT: case RTC_FORMAT_UINT: case RTC_FORMAT_INT: return 4;
+    case RTC_FORMAT_FLOAT2: case RTC_FORMAT_UINT2: case RTC_FORMAT_INT2: return 8;
+    case RTC_FORMAT_FLOAT4: case RTC_FORMAT_UINT4: case RTC_FORMAT_INT4: return 16;
+    default: return 0;
+  }
+}
+
+// Validates and binds a triangle-mesh buffer (reference scene_triangle_mesh.cpp:35-80).
+void bind_slot(Geometry* g, RTCBufferType type, unsigned slot, RTCFormat format, Buffer* buf, const void* ptr,
+               size_t offset, size_t stride, size_t count) {
+  if (g->type != RTC_GEOMETRY_TYPE_TRIANGLE) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "unsupported geometry type");
+  if (count > 0xffffffffull) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "buffer too large");
+  const char* p = (const char*)ptr + offset;
+  if (((size_t)p & 3) || (stride & 3)) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "data must be 4 bytes aligned");
+  Slot* s = nullptr;
+  if (type == RTC_BUFFER_TYPE_INDEX) {
+    if (slot != 0) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid index buffer slot");
+    if (format != RTC_FORMAT_UINT3) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "invalid index buffer format");
+    s = &g->index;
+  } else if (type == RTC_BUFFER_TYPE_VERTEX) {
+    if (slot != 0) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid vertex buffer slot");
+    if (format != RTC_FORMAT_FLOAT3) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "invalid vertex buffer format");
+    s = &g->vertex;
+  } else if (type == RTC_BUFFER_TYPE_VERTEX_ATTRIBUTE) {
+    return;  // accepted; interpolation is outside the ray-query path
+  } else {
+    RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "unknown buffer type");
+  }
+  if (stride < format_size(format)) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "stride smaller than element");
+  if (buf) ref_inc(buf);
+  s->clear();
+  s->buf = buf;
+  s->ptr = p;
+  s->stride = stride;
+  s->count = count;
+  s->format = format;
+  s->set = true;
+  g->committed = false;
+}
+
+// Uploads geometry buffers and builds the device BVH.
+void commit_scene(Scene* sc) {
+  Device* d = sc->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  std::lock_guard<std::mutex> lk(sc->mtx);
+  if (sc->progress && !sc->progress(sc->progress_ptr, 0.0)) RTC_THROW(RTC_ERROR_CANCELLED, "progress monitor forced termination");
+  for (auto* b : sc->gdata) delete b;
+  sc->gdata.clear();
+  sc->descs.clear();
+  uint32_t prim_offset = 0;
+  for (size_t id = 0; id < sc->geoms.size(); id++) {
+    Geometry* g = sc->geoms[id];
+    if (!g || !g->enabled || g->type != RTC_GEOMETRY_TYPE_TRIANGLE) continue;
+    if (!g->vertex.set || !g->index.set || g->index.count == 0) continue;
+    const size_t vbytes = g->vertex.count ? (g->vertex.count - 1) * g->vertex.stride + 12 : 0;
+    const size_t ibytes = (g->index.count - 1) * g->index.stride + 12;
+    if ((uint64_t)prim_offset + g->index.count > 0xffffffffull) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "too many primitives");
+    d->monitor((ssize_t)(vbytes + ibytes));
+    auto* vb = new rtg::DevBuf();
+    sc->gdata.push_back(vb);
+    auto* ib = new rtg::DevBuf();
+    sc->gdata.push_back(ib);
+    vb->reserve(vbytes ? vbytes : 16);
+    ib->reserve(ibytes);
+    if (vbytes) RTG_HIP(hipMemcpyAsync(vb->ptr, g->vertex.ptr, vbytes, hipMemcpyHostToDevice, t.stream));
+    RTG_HIP(hipMemcpyAsync(ib->ptr, g->index.ptr, ibytes, hipMemcpyHostToDevice, t.stream));
+    rtg::GeomDesc gd;
+    gd.vertices = (const char*)vb->ptr;
+    gd.indices = (const char*)ib->ptr;
+    gd.vstride = g->vertex.stride;
+    gd.istride = g->index.stride;
+    gd.num_vertices = (uint32_t)g->vertex.count;
+    gd.num_tris = (uint32_t)g->index.count;
+    gd.prim_offset = prim_offset;
+    gd.geomID = (uint32_t)id;
+    prim_offset += gd.num_tris;
+    sc->descs.push_back(gd);
+  }
+  {
+    std::lock_guard<std::mutex> bl(d->build_mtx);
+    rtg::BuildParams p = d->build;
+    p.quality = sc->quality;
+    d->monitor((ssize_t)prim_offset * (80 + 48));
+    rtg::build_bvh(sc->descs, p, d->scratch, sc->bvh, t.stream);
+  }
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  if (sc->bvh.stats.max_depth > 36) RTC_THROW(RTC_ERROR_UNKNOWN, "BVH deeper than the traversal stack supports");
+  sc->committed = true;
+  if (sc->progress && !sc->progress(sc->progress_ptr, 1.0)) RTC_THROW(RTC_ERROR_CANCELLED, "progress monitor forced termination");
+}
+
+bool is_device_ptr(Device* d, const void* p) {
+  hipPointerAttribute_t attr;
+  hipError_t e = hipPointerGetAttributes(&attr, p);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return false;
+  }
+  return attr.type == hipMemoryTypeDevice && attr.device == d->ordinal;
+}
+
+void check_trace_error(ThreadDev& t) {
+  uint32_t err = 0;
+  RTG_HIP(hipMemcpyAsync(&err, t.errword.ptr, 4, hipMemcpyDeviceToHost, t.stream));
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  if (err) {
+    RTG_HIP(hipMemsetAsync(t.errword.ptr, 0, 4, t.stream));
+    RTG_HIP(hipStreamSynchronize(t.stream));
+    RTC_THROW(RTC_ERROR_UNKNOWN, "traversal stack overflow");
+  }
+}
+
+// Stream query over host or device records (reference rtcore.cpp:595-623, 848-874).
+void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigned M, size_t stride, bool occluded) {
+  if (!sc->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  if (ctx && ctx->filter) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "filter functions are not supported");
+  if (M == 0) return;
+  if (!rays) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  const size_t rec = occluded ? sizeof(RTCRay) : sizeof(RTCRayHit);
+  if (stride < rec && M > 1) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "stride smaller than the ray record");
+  Device* d = sc->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  const uint32_t inst = ctx ? ctx->instID[0] : RTC_INVALID_GEOMETRY_ID;
+  if (sc->bvh.stats.num_nodes == 0) return;  // empty scene: every ray misses
+  if (is_device_ptr(d, rays)) {
+    rtg::launch_trace(sc->bvh.view(), rays, M, stride, occluded, inst, nullptr, nullptr, t.errword.as<uint32_t>(),
+                      t.stream);
+    check_trace_error(t);
+    return;
+  }
+  // host records: pack, upload, trace, download, unpack
+  t.staging.reserve((size_t)M * rec);
+  if (stride == rec) {
+    RTG_HIP(hipMemcpyAsync(t.staging.ptr, rays, (size_t)M * rec, hipMemcpyHostToDevice, t.stream));
+  } else {
+    std::vector<char> pack((size_t)M * rec);
+    for (size_t i = 0; i < M; i++) memcpy(&pack[i * rec], (const char*)rays + i * stride, rec);
+    RTG_HIP(hipMemcpyAsync(t.staging.ptr, pack.data(), pack.size(), hipMemcpyHostToDevice, t.stream));
+    RTG_HIP(hipStreamSynchronize(t.stream));
+  }
+  rtg::launch_trace(sc->bvh.view(), t.staging.ptr, M, rec, occluded, inst, nullptr, nullptr,
+                    t.errword.as<uint32_t>(), t.stream);
+  if (stride == rec) {
+    RTG_HIP(hipMemcpyAsync(rays, t.staging.ptr, (size_t)M * rec, hipMemcpyDeviceToHost, t.stream));
+    RTG_HIP(hipStreamSynchronize(t.stream));
+  } else {
+    std::vector<char> pack((size_t)M * rec);
+    RTG_HIP(hipMemcpyAsync(pack.data(), t.staging.ptr, pack.size(), hipMemcpyDeviceToHost, t.stream));
+    RTG_HIP(hipStreamSynchronize(t.stream));
+    for (size_t i = 0; i < M; i++) memcpy((char*)rays + i * stride, &pack[i * rec], rec);
+  }
+  check_trace_error(t);
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------------ device --
+
+RTC_API RTCDevice rtcNewDevice(const char* config) {
+  Device* d = nullptr;
+  API_BEGIN
+  std::lock_guard<std::mutex> g(g_api_mutex);
+  d = new Device();
+  parse_config(d, config);
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n <= d->ordinal) {
+    (void)hipGetLastError();
+    delete d;
+    d = nullptr;
+    RTC_THROW(RTC_ERROR_UNSUPPORTED_CPU, "no usable GPU");
+  }
+  d->activate();
+  if (d->verbose) {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d->ordinal) == hipSuccess)
+      fprintf(stderr, "Embree-MI355X %s on %s (%s), %d CUs\n", RTC_VERSION_STRING, prop.name, prop.gcnArchName,
+              prop.multiProcessorCount);
+  }
+  return (RTCDevice)d;
+  API_END((Device*)nullptr)
+  return nullptr;
+}
+
+RTC_API void rtcRetainDevice(RTCDevice device) {
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  std::lock_guard<std::mutex> g(g_api_mutex);
+  ref_inc(DE(device));
+  API_END((Device*)nullptr)
+}
+
+RTC_API void rtcReleaseDevice(RTCDevice device) {
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  std::lock_guard<std::mutex> g(g_api_mutex);
+  ref_dec(DE(device));
+  API_END((Device*)nullptr)
+}
+
+RTC_API ssize_t rtcGetDeviceProperty(RTCDevice device, RTCDeviceProperty prop) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  switch (prop) {
+    case RTC_DEVICE_PROPERTY_VERSION: return RTC_VERSION;
+    case RTC_DEVICE_PROPERTY_VERSION_MAJOR: return RTC_VERSION_MAJOR;
+    case RTC_DEVICE_PROPERTY_VERSION_MINOR: return RTC_VERSION_MINOR;
+    case RTC_DEVICE_PROPERTY_VERSION_PATCH: return RTC_VERSION_PATCH;
+    case RTC_DEVICE_PROPERTY_NATIVE_RAY4_SUPPORTED:
+    case RTC_DEVICE_PROPERTY_NATIVE_RAY8_SUPPORTED:
+    case RTC_DEVICE_PROPERTY_NATIVE_RAY16_SUPPORTED: return 0;
+    case RTC_DEVICE_PROPERTY_RAY_STREAM_SUPPORTED: return 1;
+    case RTC_DEVICE_PROPERTY_BACKFACE_CULLING_CURVES_ENABLED:
+    case RTC_DEVICE_PROPERTY_RAY_MASK_SUPPORTED:
+    case RTC_DEVICE_PROPERTY_BACKFACE_CULLING_ENABLED:
+    case RTC_DEVICE_PROPERTY_FILTER_FUNCTION_SUPPORTED:
+    case RTC_DEVICE_PROPERTY_IGNORE_INVALID_RAYS_ENABLED:
+    case RTC_DEVICE_PROPERTY_COMPACT_POLYS_ENABLED: return 0;
+    case RTC_DEVICE_PROPERTY_TRIANGLE_GEOMETRY_SUPPORTED: return 1;
+    case RTC_DEVICE_PROPERTY_QUAD_GEOMETRY_SUPPORTED:
+    case RTC_DEVICE_PROPERTY_SUBDIVISION_GEOMETRY_SUPPORTED:
+    case RTC_DEVICE_PROPERTY_CURVE_GEOMETRY_SUPPORTED:
+    case RTC_DEVICE_PROPERTY_USER_GEOMETRY_SUPPORTED:
+    case RTC_DEVICE_PROPERTY_POINT_GEOMETRY_SUPPORTED: return 0;
+    case RTC_DEVICE_PROPERTY_TASKING_SYSTEM: return 0;
+    case RTC_DEVICE_PROPERTY_JOIN_COMMIT_SUPPORTED: return 1;
+    case RTC_DEVICE_PROPERTY_PARALLEL_COMMIT_SUPPORTED: return 1;
+    default: RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "unknown readable property");
+  }
+  API_END(d)
+  return 0;
+}
+
+RTC_API void rtcSetDeviceProperty(RTCDevice device, const RTCDeviceProperty prop, ssize_t value) {
+  Device* d = DE(device);
+  API_BEGIN
+  (void)prop;
+  (void)value;
+  VERIFY_HANDLE(device);
+  RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "unknown writable property");
+  API_END(d)
+}
+
+RTC_API RTCError rtcGetDeviceError(RTCDevice device) {
+  if (!device) {
+    RTCError e = tls_null_error;
+    tls_null_error = RTC_ERROR_NONE;
+    return e;
+  }
+  auto it = tls_errors.find(DE(device));
+  if (it == tls_errors.end()) return RTC_ERROR_NONE;
+  RTCError e = it->second;
+  it->second = RTC_ERROR_NONE;
+  return e;
+}
+
+RTC_API void rtcSetDeviceErrorFunction(RTCDevice device, RTCErrorFunction error, void* userPtr) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  d->errfn = error;
+  d->errptr = userPtr;
+  API_END(d)
+}
+
+RTC_API void rtcSetDeviceMemoryMonitorFunction(RTCDevice device, RTCMemoryMonitorFunction fn, void* userPtr) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  d->memfn = fn;
+  d->memptr = userPtr;
+  API_END(d)
+}
+
+// ------------------------------------------------------------------ buffer --
+
+RTC_API RTCBuffer rtcNewBuffer(RTCDevice device, size_t byteSize) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  void* p = malloc(byteSize + 16);
+  if (!p) RTC_THROW(RTC_ERROR_OUT_OF_MEMORY, "out of memory");
+  return (RTCBuffer) new Buffer(d, p, byteSize, false);
+  API_END(d)
+  return nullptr;
+}
+
+RTC_API RTCBuffer rtcNewSharedBuffer(RTCDevice device, void* ptr, size_t byteSize) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  return (RTCBuffer) new Buffer(d, ptr, byteSize, true);
+  API_END(d)
+  return nullptr;
+}
+
+RTC_API void* rtcGetBufferData(RTCBuffer buffer) {
+  Buffer* b = BU(buffer);
+  API_BEGIN
+  VERIFY_HANDLE(buffer);
+  return b->data;
+  API_END(b)
+  return nullptr;
+}
+
+RTC_API void rtcRetainBuffer(RTCBuffer buffer) {
+  Buffer* b = BU(buffer);
+  API_BEGIN
+  VERIFY_HANDLE(buffer);
+  ref_inc(b);
+  API_END(b)
+}
+
+RTC_API void rtcReleaseBuffer(RTCBuffer buffer) {
+  Buffer* b = BU(buffer);
+  API_BEGIN
+  VERIFY_HANDLE(buffer);
+  ref_dec(b);
+  API_END((Buffer*)nullptr)
+}
+
+// ---------------------------------------------------------------- geometry --
+
+RTC_API RTCGeometry rtcNewGeometry(RTCDevice device, RTCGeometryType type) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  if (type != RTC_GEOMETRY_TYPE_TRIANGLE) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "geometry type not supported");
+  return (RTCGeometry) new Geometry(d, type);
+  API_END(d)
+  return nullptr;
+}
+
+RTC_API void rtcRetainGeometry(RTCGeometry geometry) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  ref_inc(g);
+  API_END(g)
+}
+
+RTC_API void rtcReleaseGeometry(RTCGeometry geometry) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  ref_dec(g);
+  API_END((Geometry*)nullptr)
+}
+
+RTC_API void rtcCommitGeometry(RTCGeometry geometry) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  g->committed = true;
+  API_END(g)
+}
+
+RTC_API void rtcEnableGeometry(RTCGeometry geometry) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  g->enabled = true;
+  API_END(g)
+}
+
+RTC_API void rtcDisableGeometry(RTCGeometry geometry) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  g->enabled = false;
+  API_END(g)
+}
+
+RTC_API void rtcSetGeometryTimeStepCount(RTCGeometry geometry, unsigned int timeStepCount) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  if (timeStepCount != 1) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "motion blur is not supported");
+  API_END(g)
+}
+
+RTC_API void rtcSetGeometryMask(RTCGeometry geometry, unsigned int mask) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  g->mask = mask;
+  API_END(g)
+}
+
+RTC_API void rtcSetGeometryBuildQuality(RTCGeometry geometry, RTCBuildQuality quality) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  if (quality > RTC_BUILD_QUALITY_REFIT) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid build quality");
+  g->quality = quality;
+  API_END(g)
+}
+
+RTC_API void rtcSetGeometryBuffer(RTCGeometry geometry, RTCBufferType type, unsigned int slot, RTCFormat format,
+                                  RTCBuffer buffer, size_t byteOffset, size_t byteStride, size_t itemCount) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  VERIFY_HANDLE(buffer);
+  Buffer* b = BU(buffer);
+  if (b->dev != g->dev) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "inputs are from different devices");
+  if (itemCount > 0 && byteOffset + (itemCount - 1) * byteStride + format_size(format) > b->size)
+    RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "buffer region out of range");
+  bind_slot(g, type, slot, format, b, b->data, byteOffset, byteStride, itemCount);
+  API_END(g)
+}
+
+RTC_API void rtcSetSharedGeometryBuffer(RTCGeometry geometry, RTCBufferType type, unsigned int slot, RTCFormat format,
+                                        const void* ptr, size_t byteOffset, size_t byteStride, size_t itemCount) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  bind_slot(g, type, slot, format, nullptr, ptr, byteOffset, byteStride, itemCount);
+  API_END(g)
+}
+
+RTC_API void* rtcSetNewGeometryBuffer(RTCGeometry geometry, RTCBufferType type, unsigned int slot, RTCFormat format,
+                                      size_t byteStride, size_t itemCount) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  size_t bytes = byteStride * itemCount;
+  if (type == RTC_BUFFER_TYPE_VERTEX || type == RTC_BUFFER_TYPE_VERTEX_ATTRIBUTE) bytes += 16;
+  void* p = calloc(1, bytes + 16);
+  if (!p) RTC_THROW(RTC_ERROR_OUT_OF_MEMORY, "out of memory");
+  Buffer* b = new Buffer(g->dev, p, bytes, false);
+  try {
+    bind_slot(g, type, slot, format, b, p, 0, byteStride, itemCount);
+  } catch (...) {
+    ref_dec(b);
+    throw;
+  }
+  ref_dec(b);  // the slot holds the only reference
+  return p;
+  API_END(g)
+  return nullptr;
+}
+
+RTC_API void* rtcGetGeometryBufferData(RTCGeometry geometry, RTCBufferType type, unsigned int slot) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  if (slot != 0) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid buffer slot");
+  const Slot* s = type == RTC_BUFFER_TYPE_INDEX ? &g->index : type == RTC_BUFFER_TYPE_VERTEX ? &g->vertex : nullptr;
+  if (!s || !s->set) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "buffer not set");
+  return (void*)s->ptr;
+  API_END(g)
+  return nullptr;
+}
+
+RTC_API void rtcUpdateGeometryBuffer(RTCGeometry geometry, RTCBufferType type, unsigned int slot) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  (void)type;
+  (void)slot;
+  g->committed = false;
+  API_END(g)
+}
+
+RTC_API void rtcSetGeometryIntersectFilterFunction(RTCGeometry geometry, RTCFilterFunctionN filter) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  if (filter) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "filter functions are not supported");
+  API_END(g)
+}
+
+RTC_API void rtcSetGeometryOccludedFilterFunction(RTCGeometry geometry, RTCFilterFunctionN filter) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  if (filter) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "filter functions are not supported");
+  API_END(g)
+}
+
+RTC_API void rtcSetGeometryUserData(RTCGeometry geometry, void* ptr) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  g->user = ptr;
+  API_END(g)
+}
+
+RTC_API void* rtcGetGeometryUserData(RTCGeometry geometry) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  return g->user;
+  API_END(g)
+  return nullptr;
+}
+
+// ------------------------------------------------------------------- scene --
+
+RTC_API RTCScene rtcNewScene(RTCDevice device) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  return (RTCScene) new Scene(d);
+  API_END(d)
+  return nullptr;
+}
+
+RTC_API RTCDevice rtcGetSceneDevice(RTCScene scene) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  ref_inc(s->dev);
+  return (RTCDevice)s->dev;
+  API_END(s)
+  return nullptr;
+}
+
+RTC_API void rtcRetainScene(RTCScene scene) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  ref_inc(s);
+  API_END(s)
+}
+
+RTC_API void rtcReleaseScene(RTCScene scene) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  ref_dec(s);
+  API_END((Scene*)nullptr)
+}
+
+RTC_API unsigned int rtcAttachGeometry(RTCScene scene, RTCGeometry geometry) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  VERIFY_HANDLE(geometry);
+  Geometry* g = GE(geometry);
+  if (g->dev != s->dev) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "inputs are from different devices");
+  std::lock_guard<std::mutex> lk(s->mtx);
+  size_t id = 0;
+  while (id < s->geoms.size() && s->geoms[id]) id++;  // smallest free ID
+  if (id == s->geoms.size()) s->geoms.push_back(nullptr);
+  ref_inc(g);
+  s->geoms[id] = g;
+  s->committed = false;
+  return (unsigned)id;
+  API_END(s)
+  return RTC_INVALID_GEOMETRY_ID;
+}
+
+RTC_API void rtcAttachGeometryByID(RTCScene scene, RTCGeometry geometry, unsigned int geomID) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  VERIFY_HANDLE(geometry);
+  if (geomID == RTC_INVALID_GEOMETRY_ID) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "invalid geometry identifier");
+  Geometry* g = GE(geometry);
+  if (g->dev != s->dev) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "inputs are from different devices");
+  std::lock_guard<std::mutex> lk(s->mtx);
+  if (geomID >= s->geoms.size()) s->geoms.resize((size_t)geomID + 1, nullptr);
+  if (s->geoms[geomID]) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "geometry identifier already used");
+  ref_inc(g);
+  s->geoms[geomID] = g;
+  s->committed = false;
+  API_END(s)
+}
+
+RTC_API void rtcDetachGeometry(RTCScene scene, unsigned int geomID) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  std::lock_guard<std::mutex> lk(s->mtx);
+  if (geomID >= s->geoms.size() || !s->geoms[geomID]) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid geometry identifier");
+  ref_dec(s->geoms[geomID]);
+  s->geoms[geomID] = nullptr;
+  s->committed = false;
+  API_END(s)
+}
+
+RTC_API RTCGeometry rtcGetGeometry(RTCScene scene, unsigned int geomID) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  if (geomID >= s->geoms.size() || !s->geoms[geomID]) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid geometry identifier");
+  return (RTCGeometry)s->geoms[geomID];
+  API_END(s)
+  return nullptr;
+}
+
+RTC_API void rtcCommitScene(RTCScene scene) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  commit_scene(s);
+  API_END(s)
+}
+
+RTC_API void rtcJoinCommitScene(RTCScene scene) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  commit_scene(s);
+  API_END(s)
+}
+
+RTC_API void rtcSetSceneProgressMonitorFunction(RTCScene scene, RTCProgressMonitorFunction progress, void* ptr) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  s->progress = progress;
+  s->progress_ptr = ptr;
+  API_END(s)
+}
+
+RTC_API void rtcSetSceneBuildQuality(RTCScene scene, RTCBuildQuality quality) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  if (quality != RTC_BUILD_QUALITY_LOW && quality != RTC_BUILD_QUALITY_MEDIUM && quality != RTC_BUILD_QUALITY_HIGH)
+    RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid build quality");
+  s->quality = quality;
+  API_END(s)
+}
+
+RTC_API void rtcSetSceneFlags(RTCScene scene, RTCSceneFlags flags) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  if (flags & RTC_SCENE_FLAG_CONTEXT_FILTER_FUNCTION)
+    RTC_THROW(RTC_ERROR_INVALID_OPERATION, "filter functions are not supported");
+  s->flags = flags;
+  API_END(s)
+}
+
+RTC_API RTCSceneFlags rtcGetSceneFlags(RTCScene scene) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  return s->flags;
+  API_END(s)
+  return RTC_SCENE_FLAG_NONE;
+}
+
+RTC_API void rtcGetSceneBounds(RTCScene scene, RTCBounds* b) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  const rtg::BuildStats& st = s->bvh.stats;
+  b->lower_x = st.bounds_lo[0]; b->lower_y = st.bounds_lo[1]; b->lower_z = st.bounds_lo[2];
+  b->upper_x = st.bounds_hi[0]; b->upper_y = st.bounds_hi[1]; b->upper_z = st.bounds_hi[2];
+  b->align0 = b->align1 = 0.0f;
+  if (st.num_prims == 0) {
+    b->lower_x = b->lower_y = b->lower_z = INFINITY;
+    b->upper_x = b->upper_y = b->upper_z = -INFINITY;
+  }
+  API_END(s)
+}
+
+RTC_API void rtcIntersect1(RTCScene scene, RTCIntersectContext* context, RTCRayHit* rayhit) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  VERIFY_HANDLE(rayhit);
+  if (!(rayhit->ray.tnear <= rayhit->ray.tfar)) return;
+  trace_stream(s, context, rayhit, 1, sizeof(RTCRayHit), false);
+  API_END(s)
+}
+
+RTC_API void rtcIntersect1M(RTCScene scene, RTCIntersectContext* context, RTCRayHit* rayhit, unsigned int M,
+                            size_t byteStride) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  trace_stream(s, context, rayhit, M, byteStride, false);
+  API_END(s)
+}
+
+RTC_API void rtcOccluded1(RTCScene scene, RTCIntersectContext* context, RTCRay* ray) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  VERIFY_HANDLE(ray);
+  if (!(ray->tnear <= ray->tfar)) return;
+  trace_stream(s, context, ray, 1, sizeof(RTCRay), true);
+  API_END(s)
+}
+
+RTC_API void rtcOccluded1M(RTCScene scene, RTCIntersectContext* context, RTCRay* ray, unsigned int M,
+                           size_t byteStride) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  trace_stream(s, context, ray, M, byteStride, true);
+  API_END(s)
+}
+
+// -------------------------------------------------------------- extensions --
+
+RTC_API void* rtcgDeviceMalloc(RTCDevice device, size_t bytes) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  d->activate();
+  void* p = nullptr;
+  RTG_HIP(hipMalloc(&p, bytes ? bytes : 16));
+  return p;
+  API_END(d)
+  return nullptr;
+}
+
+RTC_API void rtcgDeviceFree(RTCDevice device, void* ptr) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  d->activate();
+  if (ptr) RTG_HIP(hipFree(ptr));
+  API_END(d)
+}
+
+RTC_API void rtcgMemcpy(RTCDevice device, void* dst, const void* src, size_t bytes) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  d->activate();
+  ThreadDev& t = d->td();
+  RTG_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, t.stream));
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  API_END(d)
+}
+
+static void async_trace(Scene* s, const RTCIntersectContext* ctx, void* rays, unsigned M, size_t stride,
+                        bool occluded, void* stream) {
+  if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  if (ctx && ctx->filter) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "filter functions are not supported");
+  Device* d = s->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  const uint32_t inst = ctx ? ctx->instID[0] : RTC_INVALID_GEOMETRY_ID;
+  rtg::launch_trace(s->bvh.view(), rays, M, stride, occluded, inst, nullptr, nullptr, t.errword.as<uint32_t>(),
+                    stream ? (hipStream_t)stream : t.stream);
+}
+
+RTC_API void rtcgIntersect1MAsync(RTCScene scene, const RTCIntersectContext* context, RTCRayHit* d_rayhit,
+                                  unsigned int M, size_t byteStride, void* hipStream) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  async_trace(s, context, d_rayhit, M, byteStride, false, hipStream);
+  API_END(s)
+}
+
+RTC_API void rtcgOccluded1MAsync(RTCScene scene, const RTCIntersectContext* context, RTCRay* d_ray, unsigned int M,
+                                 size_t byteStride, void* hipStream) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  async_trace(s, context, d_ray, M, byteStride, true, hipStream);
+  API_END(s)
+}
+
+RTC_API void rtcgSynchronize(RTCDevice device) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  d->activate();
+  ThreadDev& t = d->td();
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  check_trace_error(t);
+  API_END(d)
+}
+
+RTC_API void rtcgTimeTrace(RTCScene scene, void* d_rays, const void* d_src, unsigned int M, size_t byteStride,
+                           int occluded, int steps, float* ms_out) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  Device* d = s->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  for (int i = 0; i < steps; i++) {
+    if (d_src) RTG_HIP(hipMemcpyAsync(d_rays, d_src, (size_t)M * byteStride, hipMemcpyDeviceToDevice, t.stream));
+    RTG_HIP(hipEventRecord(t.ev0, t.stream));
+    rtg::launch_trace(s->bvh.view(), d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, nullptr, nullptr,
+                      t.errword.as<uint32_t>(), t.stream);
+    RTG_HIP(hipEventRecord(t.ev1, t.stream));
+    RTG_HIP(hipEventSynchronize(t.ev1));
+    float ms = 0.0f;
+    RTG_HIP(hipEventElapsedTime(&ms, t.ev0, t.ev1));
+    if (ms_out) ms_out[i] = ms;
+  }
+  check_trace_error(t);
+  API_END(s)
+}
+
+RTC_API void rtcgTimeBuild(RTCScene scene, int steps, float* ms_out) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  Device* d = s->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  std::lock_guard<std::mutex> lk(s->mtx);
+  for (int i = 0; i < steps; i++) {
+    std::lock_guard<std::mutex> bl(d->build_mtx);
+    RTG_HIP(hipStreamSynchronize(t.stream));
+    auto t0 = std::chrono::steady_clock::now();
+    rtg::BuildParams p = d->build;
+    p.quality = s->quality;
+    rtg::build_bvh(s->descs, p, d->scratch, s->bvh, t.stream);
+    RTG_HIP(hipStreamSynchronize(t.stream));
+    if (ms_out) ms_out[i] = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  API_END(s)
+}
+
+RTC_API void rtcgTraceCounted(RTCScene scene, void* d_rays, unsigned int M, size_t byteStride, int occluded,
+                              unsigned int* d_nodes, unsigned int* d_tris) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  if (!d_nodes || !d_tris) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  Device* d = s->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  if (s->bvh.stats.num_nodes == 0) {
+    RTG_HIP(hipMemsetAsync(d_nodes, 0, (size_t)M * 4, t.stream));
+    RTG_HIP(hipMemsetAsync(d_tris, 0, (size_t)M * 4, t.stream));
+  } else {
+    rtg::launch_trace(s->bvh.view(), d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, d_nodes, d_tris,
+                      t.errword.as<uint32_t>(), t.stream);
+  }
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  check_trace_error(t);
+  API_END(s)
+}
+
+RTC_API void rtcgGetSceneStats(RTCScene scene, RTCGSceneStats* out) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  VERIFY_HANDLE(out);
+  const rtg::BuildStats& st = s->bvh.stats;
+  memset(out, 0, sizeof(*out));
+  out->num_prims = st.num_prims;
+  out->num_nodes = st.num_nodes;
+  out->max_depth = st.max_depth;
+  out->node_bytes = sizeof(rtg::QNode8);
+  out->tri_bytes = sizeof(rtg::TriRec);
+  out->sah = st.sah;
+  out->sah_tri = st.sah_tri;
+  out->build_ms = st.ms_total;
+  API_END(s)
+}
+
+RTC_API void rtcgReadBVH(RTCScene scene, void* nodes, void* tris) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  Device* d = s->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  const rtg::BuildStats& st = s->bvh.stats;
+  if (nodes && st.num_nodes)
+    RTG_HIP(hipMemcpyAsync(nodes, s->bvh.nodes.ptr, (size_t)st.num_nodes * sizeof(rtg::QNode8), hipMemcpyDeviceToHost,
+                           t.stream));
+  if (tris && st.num_prims)
+    RTG_HIP(hipMemcpyAsync(tris, s->bvh.tris.ptr, (size_t)st.num_prims * sizeof(rtg::TriRec), hipMemcpyDeviceToHost,
+                           t.stream));
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  API_END(s)
+}
+
+}  // extern "C"

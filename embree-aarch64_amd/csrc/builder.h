@@ -1,0 +1,59 @@
+// GPU BVH8 builder: primitive references -> Morton codes -> LDS radix sort ->
+// binary radix tree (LBVH) -> bottom-up refit with an SAH leaf/inner cost ->
+// top-down collapse into quantised 8-wide nodes and Moeller-ready triangle
+// records.
+#pragma once
+#include <vector>
+
+#include "bvh_types.h"
+#include "gpu_util.h"
+#include "sort.h"
+
+namespace rtg {
+
+struct BuildParams {
+  uint32_t max_leaf = 8;       // max triangles per leaf (<= 15)
+  float cost_node = 1.0f;      // SAH cost of one binary inner node per unit half area
+  float cost_tri = 1.0f;       // SAH cost of one triangle test per unit half area
+  int quality = 1;             // RTCBuildQuality (LOW uses the same LBVH; MEDIUM/HIGH add SAH binning)
+};
+
+struct BuildStats {
+  uint32_t num_prims = 0;      // valid triangles
+  uint32_t num_nodes = 0;
+  uint32_t max_depth = 0;
+  float bounds_lo[3] = {0, 0, 0};
+  float bounds_hi[3] = {0, 0, 0};
+  double sah = 0;              // reference SAH statistic (leaf cost per block of 4)
+  double sah_tri = 0;          // same with leaf cost per triangle
+  double ms_total = 0;
+  double ms_phase[8] = {0};    // primrefs, morton, sort, tree, refit, collapse
+};
+
+// Owns the device arrays of one built scene.
+struct DeviceBVH {
+  DevBuf nodes, tris;
+  DevScene view() const {
+    DevScene s;
+    s.nodes = nodes.as<QNode8>();
+    s.tris = tris.as<TriRec>();
+    s.num_nodes = stats.num_nodes;
+    s.num_tris = stats.num_prims;
+    return s;
+  }
+  BuildStats stats;
+};
+
+// Scratch reused across builds on one device.
+struct BuildScratch {
+  DevBuf geoms, lo4, hi4, keys, vals, slo, shi, child, parent_int, parent_leaf, range, nlo, nhi, cost,
+      isleaf, visit, queue, counters;
+  RadixSorter sorter;
+};
+
+// Builds over the given geometries (device pointers in GeomDesc).
+// Throws hip_error on a HIP failure.
+void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& p, BuildScratch& scratch,
+               DeviceBVH& out, hipStream_t s);
+
+}  // namespace rtg

@@ -1,0 +1,638 @@
+// GPU BVH8 builder (see builder.h for the pipeline).
+//
+// Reference anchors for what each phase computes:
+//   primitive references and validity: kernels/common/scene_triangle_mesh.h:127-147,220-231,
+//     kernels/builders/primrefgen.cpp:36-56 (invalid triangles are skipped);
+//   Morton codes on 2x centroids, 10 bits per axis: kernels/builders/bvh_builder_morton.h:70-101;
+//   radix sort: common/algorithms/parallel_sort.h:437-456 (semantics only);
+//   leaf/inner decision by SAH cost and the 8-wide collapse that repeatedly
+//     opens the child with the largest half area: kernels/builders/bvh_builder_sah.h:222-330;
+//   leaf fill e1 = v0-v1, e2 = v2-v0: kernels/geometry/triangle.h:40-41,98-120.
+#include <chrono>
+
+#include "builder.h"
+
+namespace rtg {
+
+struct BuildCounters {
+  uint32_t glo[3], ghi[3];
+  uint32_t clo[3], chi[3];
+  uint32_t maxabs;
+  uint32_t valid;
+  uint32_t node_count;
+  uint32_t tri_count;
+  uint32_t qcount[2];
+  uint32_t max_depth;
+  uint32_t pad[3];
+  double sah_nodes, sah_leaves, sah_leaves_tri;
+};
+
+constexpr uint32_t NONE = 0xffffffffu;
+constexpr int BT = 256;
+
+struct QItem {
+  uint32_t bref;  // binary ref: REF_LEAF | sorted prim index, or inner node index
+  uint32_t node;  // QNode8 slot
+  uint32_t depth;
+};
+
+// ------------------------------------------------------------ reductions --
+
+__device__ inline float wave_min(float v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = fminf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ inline float wave_max(float v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = fmaxf(v, __shfl_xor(v, d, 64));
+  return v;
+}
+__device__ inline uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+
+__global__ void k_init_counters(BuildCounters* c) {
+  for (int k = 0; k < 3; k++) {
+    c->glo[k] = f2o(INFINITY);
+    c->ghi[k] = f2o(-INFINITY);
+    c->clo[k] = f2o(INFINITY);
+    c->chi[k] = f2o(-INFINITY);
+  }
+  c->maxabs = f2o(0.0f);
+  c->valid = 0;
+  c->node_count = 1;
+  c->tri_count = 0;
+  c->qcount[0] = c->qcount[1] = 0;
+  c->max_depth = 0;
+  c->sah_nodes = c->sah_leaves = c->sah_leaves_tri = 0.0;
+}
+
+// ------------------------------------------------------------ primrefs --
+
+__device__ inline bool finite_v(float x) { return (x > -FLT_LARGE_) & (x < FLT_LARGE_); }
+
+__global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ geoms, uint32_t ngeoms,
+                                                 uint32_t total, float4* __restrict__ lo4,
+                                                 float4* __restrict__ hi4, uint32_t* __restrict__ keys,
+                                                 BuildCounters* __restrict__ cnt) {
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  float b[13];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    b[k] = INFINITY;        // geom lo
+    b[3 + k] = -INFINITY;   // geom hi
+    b[6 + k] = INFINITY;    // cent lo
+    b[9 + k] = -INFINITY;   // cent hi
+  }
+  b[12] = 0.0f;
+  uint32_t valid = 0;
+  if (i < total) {
+    uint32_t lo = 0, hi = ngeoms - 1;
+    while (lo < hi) {  // last geometry with prim_offset <= i
+      uint32_t mid = (lo + hi + 1) >> 1;
+      if (geoms[mid].prim_offset <= i) lo = mid; else hi = mid - 1;
+    }
+    const GeomDesc g = geoms[lo];
+    const uint32_t prim = i - g.prim_offset;
+    const uint32_t* ix = (const uint32_t*)(g.indices + (uint64_t)prim * g.istride);
+    const uint32_t i0 = ix[0], i1 = ix[1], i2 = ix[2];
+    bool ok = (i0 < g.num_vertices) & (i1 < g.num_vertices) & (i2 < g.num_vertices);
+    float v[9];
+    if (ok) {
+      const float* p0 = (const float*)(g.vertices + (uint64_t)i0 * g.vstride);
+      const float* p1 = (const float*)(g.vertices + (uint64_t)i1 * g.vstride);
+      const float* p2 = (const float*)(g.vertices + (uint64_t)i2 * g.vstride);
+      v[0] = p0[0]; v[1] = p0[1]; v[2] = p0[2];
+      v[3] = p1[0]; v[4] = p1[1]; v[5] = p1[2];
+      v[6] = p2[0]; v[7] = p2[1]; v[8] = p2[2];
+#pragma unroll
+      for (int k = 0; k < 9; k++) ok &= finite_v(v[k]);
+    }
+    if (ok) {
+      float l[3], h[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        l[k] = fminf(v[k], fminf(v[3 + k], v[6 + k]));
+        h[k] = fmaxf(v[k], fmaxf(v[3 + k], v[6 + k]));
+        const float c2 = l[k] + h[k];
+        b[k] = l[k];
+        b[3 + k] = h[k];
+        b[6 + k] = c2;
+        b[9 + k] = c2;
+        b[12] = fmaxf(b[12], fmaxf(fabsf(l[k]), fabsf(h[k])));
+      }
+      lo4[i] = make_float4(l[0], l[1], l[2], __uint_as_float(lo));
+      hi4[i] = make_float4(h[0], h[1], h[2], __uint_as_float(prim));
+      keys[i] = 0u;
+      valid = 1;
+    } else {
+      keys[i] = 0xffffffffu;
+    }
+  }
+  // block reduction
+  __shared__ float red[BT / 64][13];
+  __shared__ uint32_t redc[BT / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int k = 0; k < 13; k++) {
+    const bool mx = (k >= 3 && k < 6) || (k >= 9);
+    float x = mx ? wave_max(b[k]) : wave_min(b[k]);
+    if (lane == 0) red[wave][k] = x;
+  }
+  uint32_t vc = wave_sum(valid);
+  if (lane == 0) redc[wave] = vc;
+  __syncthreads();
+  if (threadIdx.x < 13) {
+    const int k = threadIdx.x;
+    const bool mx = (k >= 3 && k < 6) || (k >= 9);
+    float x = red[0][k];
+    for (int w = 1; w < BT / 64; w++) x = mx ? fmaxf(x, red[w][k]) : fminf(x, red[w][k]);
+    uint32_t* dst = (k < 3) ? &cnt->glo[k] : (k < 6) ? &cnt->ghi[k - 3] : (k < 9) ? &cnt->clo[k - 6]
+                    : (k < 12) ? &cnt->chi[k - 9] : &cnt->maxabs;
+    if (mx) atomicMax(dst, f2o(x)); else atomicMin(dst, f2o(x));
+  }
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int w = 0; w < BT / 64; w++) s += redc[w];
+    if (s) atomicAdd(&cnt->valid, s);
+  }
+}
+
+// ---------------------------------------------------------------- morton --
+
+// bitInterleave (reference common/math/math.h:390-409)
+__device__ inline uint32_t interleave3(uint32_t x) {
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
+}
+
+__global__ __launch_bounds__(BT) void k_morton(const float4* __restrict__ lo4, const float4* __restrict__ hi4,
+                                               uint32_t total, uint32_t* __restrict__ keys,
+                                               uint32_t* __restrict__ vals, const BuildCounters* __restrict__ cnt) {
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  if (i >= total) return;
+  vals[i] = i;
+  if (keys[i] == 0xffffffffu) return;
+  const float4 l = lo4[i], h = hi4[i];
+  const float c[3] = {l.x + h.x, l.y + h.y, l.z + h.z};
+  uint32_t q[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float base = o2f(cnt->clo[k]);
+    const float diag = o2f(cnt->chi[k]) - base;
+    const float scale = diag > 1e-19f ? (1024.0f * 0.99f) / diag : 0.0f;
+    float x = (c[k] - base) * scale;
+    x = fminf(fmaxf(x, 0.0f), 1023.0f);
+    q[k] = (uint32_t)x;
+  }
+  keys[i] = interleave3(q[0]) | (interleave3(q[1]) << 1) | (interleave3(q[2]) << 2);
+}
+
+__global__ __launch_bounds__(BT) void k_gather_sorted(const uint32_t* __restrict__ vals,
+                                                      const float4* __restrict__ lo4,
+                                                      const float4* __restrict__ hi4, uint32_t n,
+                                                      float4* __restrict__ slo, float4* __restrict__ shi) {
+  const uint32_t j = blockIdx.x * BT + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t o = vals[j];
+  slo[j] = lo4[o];
+  shi[j] = hi4[o];
+}
+
+// ------------------------------------------------------- radix tree (LBVH) --
+
+__device__ inline int delta(const uint32_t* __restrict__ keys, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  const uint32_t a = keys[i], b = keys[j];
+  if (a != b) return __clz(a ^ b);
+  return 32 + __clz((uint32_t)i ^ (uint32_t)j);
+}
+
+// Binary radix tree over sorted keys (Karras, HPG 2012): inner node i covers
+// a contiguous range of sorted primitives and splits it where the highest
+// differing key bit changes.
+__global__ __launch_bounds__(BT) void k_radix_tree(const uint32_t* __restrict__ keys, int n,
+                                                   uint32_t* __restrict__ child, uint32_t* __restrict__ parent_int,
+                                                   uint32_t* __restrict__ parent_leaf, uint2* __restrict__ range) {
+  const int i = blockIdx.x * BT + threadIdx.x;
+  if (i >= n - 1) return;
+  const int d = (delta(keys, n, i, i + 1) - delta(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  const int dmin = delta(keys, n, i, i - d);
+  int lmax = 2;
+  while (delta(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (delta(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = delta(keys, n, i, j);
+  int s = 0, t = l;
+  while (true) {
+    t = (t + 1) >> 1;
+    if (delta(keys, n, i, i + (s + t) * d) > dnode) s += t;
+    if (t <= 1) break;
+  }
+  const int gamma = i + s * d + (d < 0 ? -1 : 0);
+  const int first = i < j ? i : j, last = i < j ? j : i;
+  const uint32_t left = (first == gamma) ? ((uint32_t)gamma | REF_LEAF) : (uint32_t)gamma;
+  const uint32_t right = (last == gamma + 1) ? ((uint32_t)(gamma + 1) | REF_LEAF) : (uint32_t)(gamma + 1);
+  child[2 * i] = left;
+  child[2 * i + 1] = right;
+  if (left & REF_LEAF) parent_leaf[left & ~REF_LEAF] = i; else parent_int[left] = i;
+  if (right & REF_LEAF) parent_leaf[right & ~REF_LEAF] = i; else parent_int[right] = i;
+  range[i] = make_uint2((uint32_t)first, (uint32_t)last);
+  if (i == 0) parent_int[0] = NONE;
+}
+
+__device__ inline float box_half_area(float4 l, float4 h) { return half_area(h.x - l.x, h.y - l.y, h.z - l.z); }
+
+// Bottom-up bounds and SAH cost: the second child to arrive at a node
+// (agent-scope acquire/release counter) finishes it and moves up.
+__global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __restrict__ child,
+                                              const uint32_t* __restrict__ parent_int,
+                                              const uint32_t* __restrict__ parent_leaf,
+                                              const uint2* __restrict__ range, const float4* __restrict__ slo,
+                                              const float4* __restrict__ shi, float4* nlo, float4* nhi,
+                                              float* cost, uint8_t* isleaf, uint32_t* visit, float cost_node,
+                                              float cost_tri, uint32_t max_leaf) {
+  const uint32_t j = blockIdx.x * BT + threadIdx.x;
+  if (j >= n) return;
+  uint32_t node = parent_leaf[j];
+  while (node != NONE) {
+    const uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    float4 bl[2], bh[2];
+    float cc[2];
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const uint32_t c = child[2 * node + k];
+      if (c & REF_LEAF) {
+        bl[k] = slo[c & ~REF_LEAF];
+        bh[k] = shi[c & ~REF_LEAF];
+        cc[k] = cost_tri * box_half_area(bl[k], bh[k]);
+      } else {
+        bl[k] = nlo[c];
+        bh[k] = nhi[c];
+        cc[k] = cost[c];
+      }
+    }
+    const float4 l = make_float4(fminf(bl[0].x, bl[1].x), fminf(bl[0].y, bl[1].y), fminf(bl[0].z, bl[1].z), 0.0f);
+    const float4 h = make_float4(fmaxf(bh[0].x, bh[1].x), fmaxf(bh[0].y, bh[1].y), fmaxf(bh[0].z, bh[1].z), 0.0f);
+    const float A = box_half_area(l, h);
+    const uint2 r = range[node];
+    const uint32_t count = r.y - r.x + 1;
+    const float inner = cost_node * A + cc[0] + cc[1];
+    const float leafc = count <= max_leaf ? cost_tri * (float)count * A : INFINITY;
+    nlo[node] = l;
+    nhi[node] = h;
+    cost[node] = fminf(inner, leafc);
+    isleaf[node] = leafc <= inner ? 1 : 0;
+    node = parent_int[node];
+  }
+}
+
+// ---------------------------------------------------------------- collapse --
+
+struct CollapseArgs {
+  const QItem* qin;
+  QItem* qout;
+  uint32_t nin;
+  uint32_t qout_slot;
+  const uint32_t* child;
+  const uint2* range;
+  const float4* nlo;
+  const float4* nhi;
+  const uint8_t* isleaf;
+  const float4* slo;
+  const float4* shi;
+  const uint32_t* vals;
+  const float4* lo4;
+  const float4* hi4;
+  const GeomDesc* geoms;
+  QNode8* nodes;
+  TriRec* tris;
+  BuildCounters* cnt;
+};
+
+__device__ inline bool is_inner(uint32_t ref, const uint8_t* isleaf) {
+  return !(ref & REF_LEAF) && !isleaf[ref];
+}
+
+// Quantises one axis: returns the biased exponent byte of the grid step.
+__device__ inline uint32_t grid_exponent(float p, float hi) {
+  const float ext = hi - p;
+  int eb = (int)((__float_as_uint(ext) >> 23) & 0xff) - 7;
+  if (eb < 1) eb = 1;
+  for (int it = 0; it < 64 && eb < 254; it++) {
+    const float s = __uint_as_float((uint32_t)eb << 23);
+    if (__builtin_fmaf(s, 254.0f, p) >= hi) break;
+    eb++;
+  }
+  return (uint32_t)eb;
+}
+
+__device__ inline uint32_t quant_lo(float v, float p, float s) {
+  float q = floorf((v - p) / s);
+  q = fminf(fmaxf(q, 0.0f), 255.0f);
+  uint32_t qi = (uint32_t)q;
+  while (qi > 0 && __builtin_fmaf(s, (float)qi, p) > v) qi--;
+  return qi;
+}
+__device__ inline uint32_t quant_hi(float v, float p, float s) {
+  float q = ceilf((v - p) / s);
+  q = fminf(fmaxf(q, 0.0f), 255.0f);
+  uint32_t qi = (uint32_t)q;
+  while (qi < 255 && __builtin_fmaf(s, (float)qi, p) < v) qi++;
+  return qi;
+}
+
+__global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
+  const uint32_t t = blockIdx.x * BT + threadIdx.x;
+  if (t >= a.nin) return;
+  const QItem item = a.qin[t];
+  const float eps = o2f(a.cnt->maxabs) * BOX_EPS_REL;
+
+  uint32_t ch[8];
+  int n;
+  if (is_inner(item.bref, a.isleaf)) {
+    ch[0] = a.child[2 * item.bref];
+    ch[1] = a.child[2 * item.bref + 1];
+    n = 2;
+  } else {
+    ch[0] = item.bref;
+    n = 1;
+  }
+  // open the inner child with the largest half area until 8 children
+  while (n < 8) {
+    int best = -1;
+    float bestA = -1.0f;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      if (k < n && is_inner(ch[k], a.isleaf)) {
+        const float A = box_half_area(a.nlo[ch[k]], a.nhi[ch[k]]);
+        if (A > bestA) { bestA = A; best = k; }
+      }
+    }
+    if (best < 0) break;
+    const uint32_t r = ch[best];
+    uint32_t c0 = a.child[2 * r], c1 = a.child[2 * r + 1];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (k == best) ch[k] = c0;
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (k == n) ch[k] = c1;
+    n++;
+  }
+
+  // boxes, ranges, allocation sizes
+  float bl[8][3], bh[8][3];
+  uint32_t first[8], cntp[8];
+  uint32_t ni = 0, nt = 0;
+  float ul[3] = {INFINITY, INFINITY, INFINITY}, uh[3] = {-INFINITY, -INFINITY, -INFINITY};
+  double sah_leaf = 0.0, sah_leaf_tri = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (k < n) {
+      const uint32_t c = ch[k];
+      float4 l, h;
+      if (c & REF_LEAF) {
+        l = a.slo[c & ~REF_LEAF];
+        h = a.shi[c & ~REF_LEAF];
+        first[k] = c & ~REF_LEAF;
+        cntp[k] = 1;
+      } else {
+        l = a.nlo[c];
+        h = a.nhi[c];
+        const uint2 r = a.range[c];
+        first[k] = r.x;
+        cntp[k] = r.y - r.x + 1;
+      }
+      bl[k][0] = l.x; bl[k][1] = l.y; bl[k][2] = l.z;
+      bh[k][0] = h.x; bh[k][1] = h.y; bh[k][2] = h.z;
+#pragma unroll
+      for (int d = 0; d < 3; d++) {
+        ul[d] = fminf(ul[d], bl[k][d]);
+        uh[d] = fmaxf(uh[d], bh[k][d]);
+      }
+      if (is_inner(c, a.isleaf)) {
+        ni++;
+      } else {
+        nt += cntp[k];
+        const float A = box_half_area(l, h);
+        sah_leaf += (double)A * (double)((cntp[k] + 3) / 4);
+        sah_leaf_tri += (double)A * (double)cntp[k];
+      }
+    }
+  }
+  const uint32_t child_base = ni ? atomicAdd(&a.cnt->node_count, ni) : 0u;
+  const uint32_t tri_base = nt ? atomicAdd(&a.cnt->tri_count, nt) : 0u;
+  const uint32_t qbase = ni ? atomicAdd(&a.cnt->qcount[a.qout_slot], ni) : 0u;
+  atomicMax(&a.cnt->max_depth, item.depth);
+  atomicAdd(&a.cnt->sah_nodes, (double)half_area(uh[0] - ul[0], uh[1] - ul[1], uh[2] - ul[2]));
+  if (nt) {
+    atomicAdd(&a.cnt->sah_leaves, sah_leaf);
+    atomicAdd(&a.cnt->sah_leaves_tri, sah_leaf_tri);
+  }
+
+  // quantisation grid over the widened union box
+  QNode8 q;
+  float p[3], s[3];
+  uint32_t e[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    p[d] = ul[d] - eps;
+    e[d] = grid_exponent(p[d], uh[d] + eps);
+    s[d] = __uint_as_float(e[d] << 23);
+  }
+  q.px = p[0]; q.py = p[1]; q.pz = p[2];
+  q.ex = (uint8_t)e[0]; q.ey = (uint8_t)e[1]; q.ez = (uint8_t)e[2];
+  q.child_base = child_base;
+  q.tri_base = tri_base;
+  uint32_t imask = 0, rank_i = 0, off_t = 0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    if (k < n) {
+      const uint32_t c = ch[k];
+      const uint32_t qlx = quant_lo(bl[k][0] - eps, p[0], s[0]), qhx = quant_hi(bh[k][0] + eps, p[0], s[0]);
+      const uint32_t qly = quant_lo(bl[k][1] - eps, p[1], s[1]), qhy = quant_hi(bh[k][1] + eps, p[1], s[1]);
+      const uint32_t qlz = quant_lo(bl[k][2] - eps, p[2], s[2]), qhz = quant_hi(bh[k][2] + eps, p[2], s[2]);
+      q.qlox[k] = (uint8_t)qlx; q.qhix[k] = (uint8_t)qhx;
+      q.qloy[k] = (uint8_t)qly; q.qhiy[k] = (uint8_t)qhy;
+      q.qloz[k] = (uint8_t)qlz; q.qhiz[k] = (uint8_t)qhz;
+      if (is_inner(c, a.isleaf)) {
+        imask |= 1u << k;
+        q.meta[k] = (uint8_t)(0x80u | rank_i);
+        a.qout[qbase + rank_i] = QItem{c, child_base + rank_i, item.depth + 1};
+        rank_i++;
+      } else {
+        q.meta[k] = (uint8_t)(0x80u | off_t);
+        // leaf triangles
+        for (uint32_t j = 0; j < cntp[k]; j++) {
+          const uint32_t sj = first[k] + j;
+          const uint32_t o = a.vals[sj];
+          const uint32_t gi = __float_as_uint(a.lo4[o].w);
+          const uint32_t prim = __float_as_uint(a.hi4[o].w);
+          const GeomDesc g = a.geoms[gi];
+          const uint32_t* ix = (const uint32_t*)(g.indices + (uint64_t)prim * g.istride);
+          const float* p0 = (const float*)(g.vertices + (uint64_t)ix[0] * g.vstride);
+          const float* p1 = (const float*)(g.vertices + (uint64_t)ix[1] * g.vstride);
+          const float* p2 = (const float*)(g.vertices + (uint64_t)ix[2] * g.vstride);
+          TriRec tr;
+          tr.v0x = p0[0]; tr.v0y = p0[1]; tr.v0z = p0[2];
+          tr.e1x = p0[0] - p1[0]; tr.e1y = p0[1] - p1[1]; tr.e1z = p0[2] - p1[2];
+          tr.e2x = p2[0] - p0[0]; tr.e2y = p2[1] - p0[1]; tr.e2z = p2[2] - p0[2];
+          tr.primID = prim;
+          tr.geomID = g.geomID;
+          tr.flags = (j + 1 == cntp[k]) ? TRI_LAST : 0u;
+          a.tris[tri_base + off_t + j] = tr;
+        }
+        off_t += cntp[k];
+      }
+    } else {
+      q.meta[k] = 0;
+      q.qlox[k] = q.qloy[k] = q.qloz[k] = 255;
+      q.qhix[k] = q.qhiy[k] = q.qhiz[k] = 0;
+    }
+  }
+  q.imask = (uint8_t)imask;
+  a.nodes[item.node] = q;
+}
+
+// ------------------------------------------------------------------ host --
+
+void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, BuildScratch& S, DeviceBVH& out,
+               hipStream_t s) {
+  auto t0 = std::chrono::steady_clock::now();
+  out.stats = BuildStats();
+  uint32_t total = 0;
+  for (const auto& g : geoms) total += g.num_tris;
+  if (geoms.empty() || total == 0) {
+    out.stats.num_prims = 0;
+    out.stats.num_nodes = 0;
+    return;
+  }
+  const uint32_t max_leaf = prm.max_leaf < 1 ? 1 : (prm.max_leaf > 15 ? 15 : prm.max_leaf);
+
+  S.geoms.reserve(geoms.size() * sizeof(GeomDesc));
+  RTG_HIP(hipMemcpyAsync(S.geoms.ptr, geoms.data(), geoms.size() * sizeof(GeomDesc), hipMemcpyHostToDevice, s));
+  S.counters.reserve(sizeof(BuildCounters));
+  S.lo4.reserve((size_t)total * 16);
+  S.hi4.reserve((size_t)total * 16);
+  S.keys.reserve((size_t)total * 4);
+  S.vals.reserve((size_t)total * 4);
+  BuildCounters* cnt = S.counters.as<BuildCounters>();
+
+  hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(1), 0, s, cnt);
+  hipLaunchKernelGGL(k_primrefs, dim3(div_up(total, BT)), dim3(BT), 0, s, S.geoms.as<GeomDesc>(),
+                     (uint32_t)geoms.size(), total, S.lo4.as<float4>(), S.hi4.as<float4>(), S.keys.as<uint32_t>(), cnt);
+  hipLaunchKernelGGL(k_morton, dim3(div_up(total, BT)), dim3(BT), 0, s, S.lo4.as<float4>(), S.hi4.as<float4>(),
+                     total, S.keys.as<uint32_t>(), S.vals.as<uint32_t>(), cnt);
+  RTG_HIP(hipGetLastError());
+  S.sorter.sort_pairs(S.keys.as<uint32_t>(), S.vals.as<uint32_t>(), total, 0, 32, s);
+
+  BuildCounters hc;
+  RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+  RTG_HIP(hipStreamSynchronize(s));
+  const uint32_t n = hc.valid;
+  out.stats.num_prims = n;
+  for (int k = 0; k < 3; k++) {
+    out.stats.bounds_lo[k] = o2f(hc.glo[k]);
+    out.stats.bounds_hi[k] = o2f(hc.ghi[k]);
+  }
+  if (n == 0) {
+    out.stats.num_nodes = 0;
+    return;
+  }
+
+  S.slo.reserve((size_t)n * 16);
+  S.shi.reserve((size_t)n * 16);
+  hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(n, BT)), dim3(BT), 0, s, S.vals.as<uint32_t>(), S.lo4.as<float4>(),
+                     S.hi4.as<float4>(), n, S.slo.as<float4>(), S.shi.as<float4>());
+
+  const uint32_t ninner = n > 1 ? n - 1 : 1;
+  S.child.reserve((size_t)ninner * 8);
+  S.parent_int.reserve((size_t)ninner * 4);
+  S.parent_leaf.reserve((size_t)n * 4);
+  S.range.reserve((size_t)ninner * 8);
+  S.nlo.reserve((size_t)ninner * 16);
+  S.nhi.reserve((size_t)ninner * 16);
+  S.cost.reserve((size_t)ninner * 4);
+  S.isleaf.reserve((size_t)ninner);
+  S.visit.reserve((size_t)ninner * 4);
+  if (n > 1) {
+    RTG_HIP(hipMemsetAsync(S.visit.ptr, 0, (size_t)ninner * 4, s));
+    hipLaunchKernelGGL(k_radix_tree, dim3(div_up(n - 1, BT)), dim3(BT), 0, s, S.keys.as<uint32_t>(), (int)n,
+                       S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
+                       S.range.as<uint2>());
+    hipLaunchKernelGGL(k_refit, dim3(div_up(n, BT)), dim3(BT), 0, s, n, S.child.as<uint32_t>(),
+                       S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(),
+                       S.slo.as<float4>(), S.shi.as<float4>(), S.nlo.as<float4>(), S.nhi.as<float4>(),
+                       S.cost.as<float>(), S.isleaf.as<uint8_t>(), S.visit.as<uint32_t>(), prm.cost_node,
+                       prm.cost_tri, max_leaf);
+    RTG_HIP(hipGetLastError());
+  }
+
+  // outputs: at most n nodes, exactly n triangle records
+  out.nodes.reserve((size_t)n * sizeof(QNode8));
+  out.tris.reserve((size_t)n * sizeof(TriRec));
+  S.queue.reserve((size_t)2 * n * sizeof(QItem));
+  QItem* q0 = S.queue.as<QItem>();
+  QItem* q1 = q0 + n;
+  QItem root{n > 1 ? 0u : (0u | REF_LEAF), 0u, 1u};
+  RTG_HIP(hipMemcpyAsync(q0, &root, sizeof(root), hipMemcpyHostToDevice, s));
+
+  CollapseArgs ca;
+  ca.child = S.child.as<uint32_t>();
+  ca.range = S.range.as<uint2>();
+  ca.nlo = S.nlo.as<float4>();
+  ca.nhi = S.nhi.as<float4>();
+  ca.isleaf = S.isleaf.as<uint8_t>();
+  ca.slo = S.slo.as<float4>();
+  ca.shi = S.shi.as<float4>();
+  ca.vals = S.vals.as<uint32_t>();
+  ca.lo4 = S.lo4.as<float4>();
+  ca.hi4 = S.hi4.as<float4>();
+  ca.geoms = S.geoms.as<GeomDesc>();
+  ca.nodes = out.nodes.as<QNode8>();
+  ca.tris = out.tris.as<TriRec>();
+  ca.cnt = cnt;
+  uint32_t nin = 1;
+  int slot = 0;
+  QItem* qin = q0;
+  QItem* qout = q1;
+  while (nin > 0) {
+    uint32_t zero = 0;
+    RTG_HIP(hipMemcpyAsync(&cnt->qcount[slot], &zero, 4, hipMemcpyHostToDevice, s));
+    ca.qin = qin;
+    ca.qout = qout;
+    ca.nin = nin;
+    ca.qout_slot = (uint32_t)slot;
+    hipLaunchKernelGGL(k_collapse, dim3(div_up(nin, BT)), dim3(BT), 0, s, ca);
+    RTG_HIP(hipGetLastError());
+    RTG_HIP(hipMemcpyAsync(&nin, &cnt->qcount[slot], 4, hipMemcpyDeviceToHost, s));
+    RTG_HIP(hipStreamSynchronize(s));
+    std::swap(qin, qout);
+    slot ^= 1;
+  }
+  RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+  RTG_HIP(hipStreamSynchronize(s));
+  out.stats.num_nodes = hc.node_count;
+  out.stats.max_depth = hc.max_depth;
+  const double root_area = half_area(out.stats.bounds_hi[0] - out.stats.bounds_lo[0],
+                                     out.stats.bounds_hi[1] - out.stats.bounds_lo[1],
+                                     out.stats.bounds_hi[2] - out.stats.bounds_lo[2]);
+  if (root_area > 0) {
+    out.stats.sah = (hc.sah_nodes + hc.sah_leaves) / root_area;
+    out.stats.sah_tri = (hc.sah_nodes + hc.sah_leaves_tri) / root_area;
+  }
+  if (hc.tri_count != n) throw hip_error(hipErrorUnknown, "BVH build lost triangles");
+  out.stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+}  // namespace rtg

@@ -1,0 +1,84 @@
+// Device-resident acceleration structure shared by the builder and the
+// traversal kernels (host and device compile this header).
+//
+// Layout in HBM (one scene):
+//   nodes : QNode8[num_nodes], 80 B each, breadth-first; node 0 is the root;
+//           the internal children of a node are contiguous from child_base.
+//   tris  : TriRec[num_tris], 48 B each; the leaf children of a node are
+//           contiguous from tri_base; a leaf ends at the record whose
+//           TRI_LAST flag is set.
+// A ray visit reads one QNode8 per node and one TriRec per triangle test,
+// so the algorithmic bytes per ray are 48 (ray) + 36 (hit) + 80 n_node + 48 n_tri.
+#pragma once
+#include <stdint.h>
+
+namespace rtg {
+
+// 8-wide node with child boxes quantised to 8 bits per plane on a per-node
+// power-of-two grid. Plane k of axis a decodes as fmaf(2^(e_a-127), q, p_a),
+// and the builder checks with that exact expression that every decoded
+// child box contains the child's widened float box, so the slab test is
+// conservative. Slot c is empty when meta[c] == 0; otherwise it is an
+// internal child (imask bit c) at child_base + (meta[c] & 0x7f) or a leaf
+// starting at tri_base + (meta[c] & 0x7f).
+struct alignas(16) QNode8 {
+  float px, py, pz;
+  uint8_t ex, ey, ez;
+  uint8_t imask;
+  uint32_t child_base;
+  uint32_t tri_base;
+  uint8_t meta[8];
+  uint8_t qlox[8], qhix[8];
+  uint8_t qloy[8], qhiy[8];
+  uint8_t qloz[8], qhiz[8];
+};
+static_assert(sizeof(QNode8) == 80, "QNode8 must be 80 bytes");
+
+// Triangle record in Moeller-ready form, e1 = v0 - v1 and e2 = v2 - v0
+// formed once in FP32 as the reference's leaf fill does
+// (kernels/geometry/triangle.h:40-41,98-120).
+//   w0 = (v0.x, v0.y, v0.z, primID bits)
+//   w1 = (e1.x, e1.y, e1.z, geomID bits)
+//   w2 = (e2.x, e2.y, e2.z, flags)
+struct alignas(16) TriRec {
+  float v0x, v0y, v0z;
+  uint32_t primID;
+  float e1x, e1y, e1z;
+  uint32_t geomID;
+  float e2x, e2y, e2z;
+  uint32_t flags;
+};
+static_assert(sizeof(TriRec) == 48, "TriRec must be 48 bytes");
+
+constexpr uint32_t TRI_LAST = 1u;
+
+// Stack / child reference: bit 31 set = leaf (triangle record index).
+constexpr uint32_t REF_LEAF = 0x80000000u;
+
+// One triangle geometry as the builder reads it (device pointers).
+struct GeomDesc {
+  const char* vertices;
+  const char* indices;
+  uint64_t vstride;
+  uint64_t istride;
+  uint32_t num_vertices;
+  uint32_t num_tris;
+  uint32_t prim_offset;  // first global primitive index of this geometry
+  uint32_t geomID;
+};
+
+// What the traversal kernels need.
+struct DevScene {
+  const QNode8* nodes;
+  const TriRec* tris;
+  uint32_t num_nodes;
+  uint32_t num_tris;
+};
+
+// Conservative-culling constants shared with the oracle's rules.
+constexpr float CULL_MARGIN = 1.0f / 4096.0f;       // relative margin on ray interval ends
+constexpr float BOX_EPS_REL = 1.0f / 1048576.0f;    // absolute box widening / max |coord|
+constexpr float MIN_RCP_INPUT = 1e-18f;             // zero_fix threshold (reference constants.h:31)
+constexpr float FLT_LARGE_ = 1.844e18f;             // valid vertex bound (reference isvalid)
+
+}  // namespace rtg

@@ -1,0 +1,316 @@
+// Ray traversal kernels: one ray per lane over the quantised BVH8.
+//
+// Reference anchors:
+//   stream entry rules (rays with !(tnear <= tfar) are skipped, occluded rays
+//     get tfar = -inf): kernels/common/rtcore.cpp:595-623, :848-874,
+//     kernels/bvh/bvh_intersector_stream_filters.cpp:135-151, bvh_intersector1.cpp:191-193;
+//   per-ray stack traversal, nearest child first, far siblings pushed with
+//     their entry distance and skipped on pop once beyond the current hit:
+//     kernels/bvh/bvh_intersector1.cpp:31-119, bvh_traverser1.h:519-643;
+//   robust slab test (rdir scaled by 1 -/+ 3 ulp, zero_fix):
+//     kernels/bvh/node_intersector1.h:122-190, common/math/vec3fa.h:172-176;
+//   Moeller-Trumbore triangle test, same FP32/FMA sequence:
+//     kernels/geometry/triangle_intersector_moeller.h:62-114, common/math/vec3.h:204,209,
+//     common/math/math.h:367-372;
+//   hit write-back: kernels/geometry/intersector_epilog.h:280-292.
+// This file is compiled with -ffp-contract=off: every FMA is explicit.
+//
+// Differences from the reference, shared with the CPU oracle so results are
+// traversal-order independent: t/u/v use IEEE division; the closest hit is
+// the lexicographic minimum of (t, geomID, primID) over all valid hits of
+// the original [tnear, tfar]; culling is conservative (robust slab test on
+// boxes that contain the widened triangle boxes, interval ends widened by
+// 2^-12 relative).
+#include "trace.h"
+
+namespace rtg {
+
+constexpr int TB = 256;
+constexpr int STACK = 256;
+
+__device__ inline float widen_up(float x) { return __builtin_isinf(x) ? x : x + fabsf(x) * CULL_MARGIN; }
+__device__ inline float widen_down(float x) { return __builtin_isinf(x) ? x : x - fabsf(x) * CULL_MARGIN; }
+
+struct TRay {
+  float ox, oy, oz, dx, dy, dz;
+  float rnx, rny, rnz, rfx, rfy, rfz;
+  bool negx, negy, negz;
+  float tnear, tfar, tlo;
+};
+
+__device__ inline float zero_fix(float d) { return fabsf(d) < MIN_RCP_INPUT ? MIN_RCP_INPUT : d; }
+
+__device__ inline void setup_ray(TRay& r, float4 a, float4 b, float tfar) {
+  const float round_down = 1.0f - 3.0f * 1.1920928955078125e-7f;
+  const float round_up = 1.0f + 3.0f * 1.1920928955078125e-7f;
+  r.ox = a.x; r.oy = a.y; r.oz = a.z; r.tnear = a.w;
+  r.dx = b.x; r.dy = b.y; r.dz = b.z;
+  r.tfar = tfar;
+  const float rx = 1.0f / zero_fix(b.x), ry = 1.0f / zero_fix(b.y), rz = 1.0f / zero_fix(b.z);
+  r.rnx = round_down * rx; r.rny = round_down * ry; r.rnz = round_down * rz;
+  r.rfx = round_up * rx; r.rfy = round_up * ry; r.rfz = round_up * rz;
+  r.negx = !(r.rnx >= 0.0f);
+  r.negy = !(r.rny >= 0.0f);
+  r.negz = !(r.rnz >= 0.0f);
+  r.tlo = widen_down(a.w);
+}
+
+__device__ inline float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
+
+struct NodeHit {
+  uint32_t mask;
+  uint32_t imask;
+  uint32_t child_base;
+  uint32_t tri_base;
+  uint32_t meta_lo, meta_hi;
+  float tn[8];
+};
+
+// Slab test of the 8 children; decoded planes are fmaf(2^(e-127), q, p), the
+// exact expression the builder verified against every child box.
+__device__ inline void intersect_node(const QNode8* __restrict__ node, const TRay& r, float thi, NodeHit& h) {
+  const uint4* np = reinterpret_cast<const uint4*>(node);
+  const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
+  const float px = __uint_as_float(w0.x), py = __uint_as_float(w0.y), pz = __uint_as_float(w0.z);
+  const float sx = __uint_as_float((w0.w & 0xffu) << 23);
+  const float sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23);
+  const float sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
+  h.imask = w0.w >> 24;
+  h.child_base = w1.x;
+  h.tri_base = w1.y;
+  h.meta_lo = w1.z;
+  h.meta_hi = w1.w;
+  // near / far plane bytes per axis (lo plane is near for a positive direction)
+  const uint32_t nx0 = r.negx ? w2.z : w2.x, nx1 = r.negx ? w2.w : w2.y;
+  const uint32_t fx0 = r.negx ? w2.x : w2.z, fx1 = r.negx ? w2.y : w2.w;
+  const uint32_t ny0 = r.negy ? w3.z : w3.x, ny1 = r.negy ? w3.w : w3.y;
+  const uint32_t fy0 = r.negy ? w3.x : w3.z, fy1 = r.negy ? w3.y : w3.w;
+  const uint32_t nz0 = r.negz ? w4.z : w4.x, nz1 = r.negz ? w4.w : w4.y;
+  const uint32_t fz0 = r.negz ? w4.x : w4.z, fz1 = r.negz ? w4.y : w4.w;
+  uint32_t mask = 0;
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const int k = c & 3;
+    const uint32_t mb = ((c < 4 ? h.meta_lo : h.meta_hi) >> (8 * k)) & 0xffu;
+    const float qnx = byte_f(c < 4 ? nx0 : nx1, k), qfx = byte_f(c < 4 ? fx0 : fx1, k);
+    const float qny = byte_f(c < 4 ? ny0 : ny1, k), qfy = byte_f(c < 4 ? fy0 : fy1, k);
+    const float qnz = byte_f(c < 4 ? nz0 : nz1, k), qfz = byte_f(c < 4 ? fz0 : fz1, k);
+    const float tnx = (__builtin_fmaf(sx, qnx, px) - r.ox) * r.rnx;
+    const float tfx = (__builtin_fmaf(sx, qfx, px) - r.ox) * r.rfx;
+    const float tny = (__builtin_fmaf(sy, qny, py) - r.oy) * r.rny;
+    const float tfy = (__builtin_fmaf(sy, qfy, py) - r.oy) * r.rfy;
+    const float tnz = (__builtin_fmaf(sz, qnz, pz) - r.oz) * r.rnz;
+    const float tfz = (__builtin_fmaf(sz, qfz, pz) - r.oz) * r.rfz;
+    const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tlo));
+    const float tf = fminf(fminf(tfx, tfy), fminf(tfz, thi));
+    h.tn[c] = tn;
+    mask |= ((tn <= tf) && (mb != 0u)) ? (1u << c) : 0u;
+  }
+  h.mask = mask;
+}
+
+__device__ inline uint32_t child_ref(const NodeHit& h, int c) {
+  const uint32_t mb = ((c < 4 ? h.meta_lo : h.meta_hi) >> (8 * (c & 3))) & 0x7fu;
+  return ((h.imask >> c) & 1u) ? (h.child_base + mb) : ((h.tri_base + mb) | REF_LEAF);
+}
+
+struct Hit {
+  float t, u, v, ngx, ngy, ngz;
+  uint32_t prim, geom;
+  bool found;
+};
+
+__device__ inline float prod_diff(float a, float b, float c, float d) { return __builtin_fmaf(a, b, -(c * d)); }
+__device__ inline float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+  return __builtin_fmaf(ax, bx, __builtin_fmaf(ay, by, az * bz));
+}
+
+// Moeller test; returns validity and the unnormalised quantities.
+__device__ inline bool moeller(const TRay& r, const TriRec* __restrict__ tp, float& T, float& absDen, float& U,
+                               float& V, float& ngx, float& ngy, float& ngz, uint32_t& prim, uint32_t& geom,
+                               uint32_t& flags) {
+  const float4* p4 = reinterpret_cast<const float4*>(tp);
+  const float4 a = p4[0], b = p4[1], c = p4[2];
+  prim = __float_as_uint(a.w);
+  geom = __float_as_uint(b.w);
+  flags = __float_as_uint(c.w);
+  const float Cx = a.x - r.ox, Cy = a.y - r.oy, Cz = a.z - r.oz;
+  // R = cross(C, D)
+  const float Rx = prod_diff(Cy, r.dz, Cz, r.dy);
+  const float Ry = prod_diff(Cz, r.dx, Cx, r.dz);
+  const float Rz = prod_diff(Cx, r.dy, Cy, r.dx);
+  // Ng = cross(e2, e1)
+  ngx = prod_diff(c.y, b.z, c.z, b.y);
+  ngy = prod_diff(c.z, b.x, c.x, b.z);
+  ngz = prod_diff(c.x, b.y, c.y, b.x);
+  const float den = dot3(ngx, ngy, ngz, r.dx, r.dy, r.dz);
+  absDen = fabsf(den);
+  const uint32_t sgn = __float_as_uint(den) & 0x80000000u;
+  U = __uint_as_float(__float_as_uint(dot3(Rx, Ry, Rz, c.x, c.y, c.z)) ^ sgn);
+  V = __uint_as_float(__float_as_uint(dot3(Rx, Ry, Rz, b.x, b.y, b.z)) ^ sgn);
+  bool valid = (den != 0.0f) & (U >= 0.0f) & (V >= 0.0f) & (U + V <= absDen);
+  T = __uint_as_float(__float_as_uint(dot3(ngx, ngy, ngz, Cx, Cy, Cz)) ^ sgn);
+  valid &= (absDen * r.tnear < T) & (T <= absDen * r.tfar);
+  return valid;
+}
+
+template <bool ALIGNED>
+__device__ inline void load_ray(const char* rp, float4& a, float4& b, float& tfar) {
+  if (ALIGNED) {
+    a = *reinterpret_cast<const float4*>(rp);
+    b = *reinterpret_cast<const float4*>(rp + 16);
+  } else {
+    const float* f = reinterpret_cast<const float*>(rp);
+    a = make_float4(f[0], f[1], f[2], f[3]);
+    b = make_float4(f[4], f[5], f[6], f[7]);
+  }
+  tfar = *reinterpret_cast<const float*>(rp + 32);
+}
+
+template <bool OCCLUDED, bool ALIGNED, bool COUNT>
+__global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
+                                              uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
+                                              uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err) {
+  const uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i >= M) return;
+  char* rp = rays + (uint64_t)i * stride;
+  float4 a, b;
+  float tfar;
+  load_ray<ALIGNED>(rp, a, b, tfar);
+  uint32_t nvis = 0, ntri = 0;
+  if (!(a.w <= tfar)) {
+    if (COUNT) { cnt_nodes[i] = 0; cnt_tris[i] = 0; }
+    return;
+  }
+  TRay r;
+  setup_ray(r, a, b, tfar);
+  Hit best;
+  best.found = false;
+  best.t = INFINITY;
+  best.prim = best.geom = 0xffffffffu;
+  best.u = best.v = best.ngx = best.ngy = best.ngz = 0.0f;
+  float lim = widen_up(tfar);
+
+  uint2 stack[STACK];
+  int sp = 0;
+  uint32_t cur = 0;  // root node
+  bool occluded = false;
+  while (true) {
+    if (!(cur & REF_LEAF)) {
+      NodeHit h;
+      intersect_node(sc.nodes + cur, r, lim, h);
+      if (COUNT) nvis++;
+      if (h.mask) {
+        if (OCCLUDED) {
+          const int c0 = __builtin_ctz(h.mask);
+          uint32_t rem = h.mask & (h.mask - 1u);
+          while (rem) {
+            const int c = __builtin_ctz(rem);
+            rem &= rem - 1u;
+            if (sp < STACK) stack[sp++] = make_uint2(child_ref(h, c), 0u);
+            else atomicOr(err, 1u);
+          }
+          cur = child_ref(h, c0);
+          continue;
+        } else {
+          int bi = -1;
+          float bv = 0.0f;
+#pragma unroll
+          for (int c = 0; c < 8; c++)
+            if (((h.mask >> c) & 1u) && (bi < 0 || h.tn[c] < bv)) { bi = c; bv = h.tn[c]; }
+          uint32_t rem = h.mask & ~(1u << bi);
+          while (rem) {
+            int mc = -1;
+            float mv = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+              if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
+            rem &= ~(1u << mc);
+            if (sp < STACK) stack[sp++] = make_uint2(child_ref(h, mc), __float_as_uint(mv));
+            else atomicOr(err, 1u);
+          }
+          cur = child_ref(h, bi);
+          continue;
+        }
+      }
+    } else {
+      uint32_t ti = cur & ~REF_LEAF;
+      while (true) {
+        float T, absDen, U, V, ngx, ngy, ngz;
+        uint32_t prim, geom, flags;
+        const bool valid = moeller(r, sc.tris + ti, T, absDen, U, V, ngx, ngy, ngz, prim, geom, flags);
+        if (COUNT) ntri++;
+        if (valid) {
+          if (OCCLUDED) {
+            occluded = true;
+            break;
+          }
+          const float t = T / absDen;
+          const bool better = !best.found || t < best.t ||
+                              (t == best.t && (geom < best.geom || (geom == best.geom && prim < best.prim)));
+          if (better) {
+            best.found = true;
+            best.t = t;
+            best.u = U / absDen;
+            best.v = V / absDen;
+            best.ngx = ngx; best.ngy = ngy; best.ngz = ngz;
+            best.prim = prim; best.geom = geom;
+          }
+        }
+        if (flags & TRI_LAST) break;
+        ti++;
+      }
+      if (OCCLUDED && occluded) break;
+      if (!OCCLUDED && best.found) lim = widen_up(fminf(best.t, tfar));
+    }
+    // pop the next entry that can still hold a closer hit
+    bool got = false;
+    while (sp > 0) {
+      const uint2 e = stack[--sp];
+      if (OCCLUDED || __uint_as_float(e.y) <= lim) {
+        cur = e.x;
+        got = true;
+        break;
+      }
+    }
+    if (!got) break;
+  }
+  if (COUNT) { cnt_nodes[i] = nvis; cnt_tris[i] = ntri; }
+  if (OCCLUDED) {
+    if (occluded) *reinterpret_cast<float*>(rp + 32) = -INFINITY;
+  } else if (best.found) {
+    float* f = reinterpret_cast<float*>(rp);
+    uint32_t* u = reinterpret_cast<uint32_t*>(rp);
+    f[8] = best.t;
+    f[12] = best.ngx;
+    f[13] = best.ngy;
+    f[14] = best.ngz;
+    f[15] = best.u;
+    f[16] = best.v;
+    u[17] = best.prim;
+    u[18] = best.geom;
+    u[19] = inst_id;
+  }
+}
+
+void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t inst_id,
+                  uint32_t* cnt_nodes, uint32_t* cnt_tris, uint32_t* err, hipStream_t s) {
+  if (M == 0 || sc.num_nodes == 0) return;
+  const bool aligned = ((uintptr_t)rays % 16 == 0) && (stride % 16 == 0);
+  const dim3 grid(div_up(M, TB)), block(TB);
+  char* rp = (char*)rays;
+  const bool count = cnt_nodes != nullptr;
+#define RTG_LAUNCH(OCC, AL, CNT) \
+  hipLaunchKernelGGL((k_trace<OCC, AL, CNT>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes, cnt_tris, err)
+  if (count) {
+    if (occluded) RTG_LAUNCH(true, false, true); else RTG_LAUNCH(false, false, true);
+  } else if (occluded) {
+    if (aligned) RTG_LAUNCH(true, true, false); else RTG_LAUNCH(true, false, false);
+  } else {
+    if (aligned) RTG_LAUNCH(false, true, false); else RTG_LAUNCH(false, false, false);
+  }
+#undef RTG_LAUNCH
+  RTG_HIP(hipGetLastError());
+}
+
+}  // namespace rtg

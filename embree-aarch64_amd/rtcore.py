@@ -1,0 +1,336 @@
+"""Python mirror of the triangle ray-query C API (ctypes over lib/libembree3.so).
+
+The functions keep the reference API's names, argument meaning and error
+behaviour (include/embree3/rtcore_*.h of the reference); numpy structured
+arrays stand in for RTCRayHit / RTCRay records. All compute happens in the
+HIP library: importing this module without a built library raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libembree3.so")
+
+RTC_INVALID_GEOMETRY_ID = 0xFFFFFFFF
+
+# enum values (reference include/embree3/rtcore_*.h)
+RTC_GEOMETRY_TYPE_TRIANGLE = 0
+RTC_GEOMETRY_TYPE_QUAD = 1
+RTC_GEOMETRY_TYPE_INSTANCE = 121
+RTC_BUFFER_TYPE_INDEX = 0
+RTC_BUFFER_TYPE_VERTEX = 1
+RTC_FORMAT_UINT3 = 0x5003
+RTC_FORMAT_FLOAT3 = 0x9003
+RTC_FORMAT_FLOAT4 = 0x9004
+RTC_BUILD_QUALITY_LOW = 0
+RTC_BUILD_QUALITY_MEDIUM = 1
+RTC_BUILD_QUALITY_HIGH = 2
+RTC_SCENE_FLAG_NONE = 0
+RTC_SCENE_FLAG_ROBUST = 4
+RTC_ERROR_NONE = 0
+RTC_ERROR_UNKNOWN = 1
+RTC_ERROR_INVALID_ARGUMENT = 2
+RTC_ERROR_INVALID_OPERATION = 3
+RTC_ERROR_OUT_OF_MEMORY = 4
+RTC_ERROR_UNSUPPORTED_CPU = 5
+RTC_ERROR_CANCELLED = 6
+RTC_DEVICE_PROPERTY_VERSION = 0
+RTC_DEVICE_PROPERTY_RAY_STREAM_SUPPORTED = 35
+RTC_DEVICE_PROPERTY_TRIANGLE_GEOMETRY_SUPPORTED = 96
+
+# RTCRayHit (80 B) and RTCRay (48 B), reference include/embree3/rtcore_ray.h:11-49
+RAY_FIELDS = [("org_x", "<f4"), ("org_y", "<f4"), ("org_z", "<f4"), ("tnear", "<f4"),
+              ("dir_x", "<f4"), ("dir_y", "<f4"), ("dir_z", "<f4"), ("time", "<f4"),
+              ("tfar", "<f4"), ("mask", "<u4"), ("id", "<u4"), ("flags", "<u4")]
+HIT_FIELDS = [("Ng_x", "<f4"), ("Ng_y", "<f4"), ("Ng_z", "<f4"), ("u", "<f4"), ("v", "<f4"),
+              ("primID", "<u4"), ("geomID", "<u4"), ("instID", "<u4")]
+RTCRay = np.dtype(RAY_FIELDS, align=True)
+RTCRayHit = np.dtype(RAY_FIELDS + HIT_FIELDS, align=True)
+assert RTCRay.itemsize == 48 and RTCRayHit.itemsize == 80
+
+
+class RTCIntersectContext(ctypes.Structure):
+    _fields_ = [("flags", ctypes.c_int), ("filter", ctypes.c_void_p), ("instID", ctypes.c_uint * 1)]
+
+
+class RTCBounds(ctypes.Structure):
+    _fields_ = [("lower_x", ctypes.c_float), ("lower_y", ctypes.c_float), ("lower_z", ctypes.c_float),
+                ("align0", ctypes.c_float), ("upper_x", ctypes.c_float), ("upper_y", ctypes.c_float),
+                ("upper_z", ctypes.c_float), ("align1", ctypes.c_float)]
+
+
+class RTCGSceneStats(ctypes.Structure):
+    _fields_ = [("num_prims", ctypes.c_uint), ("num_nodes", ctypes.c_uint), ("max_depth", ctypes.c_uint),
+                ("node_bytes", ctypes.c_uint), ("tri_bytes", ctypes.c_uint), ("pad", ctypes.c_uint),
+                ("sah", ctypes.c_double), ("sah_tri", ctypes.c_double), ("build_ms", ctypes.c_double)]
+
+
+ERROR_FUNC = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p)
+MEMORY_MONITOR_FUNC = ctypes.CFUNCTYPE(ctypes.c_bool, ctypes.c_void_p, ctypes.c_ssize_t, ctypes.c_bool)
+PROGRESS_FUNC = ctypes.CFUNCTYPE(ctypes.c_bool, ctypes.c_void_p, ctypes.c_double)
+
+_vp, _u32, _sz, _i32, _f32 = ctypes.c_void_p, ctypes.c_uint, ctypes.c_size_t, ctypes.c_int, ctypes.c_float
+# name: (restype, argtypes)
+SIGNATURES = {
+    "rtcNewDevice": (_vp, [ctypes.c_char_p]),
+    "rtcRetainDevice": (None, [_vp]),
+    "rtcReleaseDevice": (None, [_vp]),
+    "rtcGetDeviceProperty": (ctypes.c_ssize_t, [_vp, _i32]),
+    "rtcSetDeviceProperty": (None, [_vp, _i32, ctypes.c_ssize_t]),
+    "rtcGetDeviceError": (_i32, [_vp]),
+    "rtcSetDeviceErrorFunction": (None, [_vp, ERROR_FUNC, _vp]),
+    "rtcSetDeviceMemoryMonitorFunction": (None, [_vp, MEMORY_MONITOR_FUNC, _vp]),
+    "rtcNewBuffer": (_vp, [_vp, _sz]),
+    "rtcNewSharedBuffer": (_vp, [_vp, _vp, _sz]),
+    "rtcGetBufferData": (_vp, [_vp]),
+    "rtcRetainBuffer": (None, [_vp]),
+    "rtcReleaseBuffer": (None, [_vp]),
+    "rtcNewGeometry": (_vp, [_vp, _i32]),
+    "rtcRetainGeometry": (None, [_vp]),
+    "rtcReleaseGeometry": (None, [_vp]),
+    "rtcCommitGeometry": (None, [_vp]),
+    "rtcEnableGeometry": (None, [_vp]),
+    "rtcDisableGeometry": (None, [_vp]),
+    "rtcSetGeometryTimeStepCount": (None, [_vp, _u32]),
+    "rtcSetGeometryMask": (None, [_vp, _u32]),
+    "rtcSetGeometryBuildQuality": (None, [_vp, _i32]),
+    "rtcSetGeometryBuffer": (None, [_vp, _i32, _u32, _i32, _vp, _sz, _sz, _sz]),
+    "rtcSetSharedGeometryBuffer": (None, [_vp, _i32, _u32, _i32, _vp, _sz, _sz, _sz]),
+    "rtcSetNewGeometryBuffer": (_vp, [_vp, _i32, _u32, _i32, _sz, _sz]),
+    "rtcGetGeometryBufferData": (_vp, [_vp, _i32, _u32]),
+    "rtcUpdateGeometryBuffer": (None, [_vp, _i32, _u32]),
+    "rtcSetGeometryIntersectFilterFunction": (None, [_vp, _vp]),
+    "rtcSetGeometryOccludedFilterFunction": (None, [_vp, _vp]),
+    "rtcSetGeometryUserData": (None, [_vp, _vp]),
+    "rtcGetGeometryUserData": (_vp, [_vp]),
+    "rtcNewScene": (_vp, [_vp]),
+    "rtcGetSceneDevice": (_vp, [_vp]),
+    "rtcRetainScene": (None, [_vp]),
+    "rtcReleaseScene": (None, [_vp]),
+    "rtcAttachGeometry": (_u32, [_vp, _vp]),
+    "rtcAttachGeometryByID": (None, [_vp, _vp, _u32]),
+    "rtcDetachGeometry": (None, [_vp, _u32]),
+    "rtcGetGeometry": (_vp, [_vp, _u32]),
+    "rtcCommitScene": (None, [_vp]),
+    "rtcJoinCommitScene": (None, [_vp]),
+    "rtcSetSceneProgressMonitorFunction": (None, [_vp, PROGRESS_FUNC, _vp]),
+    "rtcSetSceneBuildQuality": (None, [_vp, _i32]),
+    "rtcSetSceneFlags": (None, [_vp, _i32]),
+    "rtcGetSceneFlags": (_i32, [_vp]),
+    "rtcGetSceneBounds": (None, [_vp, ctypes.POINTER(RTCBounds)]),
+    "rtcIntersect1": (None, [_vp, _vp, _vp]),
+    "rtcIntersect1M": (None, [_vp, _vp, _vp, _u32, _sz]),
+    "rtcOccluded1": (None, [_vp, _vp, _vp]),
+    "rtcOccluded1M": (None, [_vp, _vp, _vp, _u32, _sz]),
+    # extensions (include/embree3/rtcore_mi355x.h)
+    "rtcgDeviceMalloc": (_vp, [_vp, _sz]),
+    "rtcgDeviceFree": (None, [_vp, _vp]),
+    "rtcgMemcpy": (None, [_vp, _vp, _vp, _sz]),
+    "rtcgIntersect1MAsync": (None, [_vp, _vp, _vp, _u32, _sz, _vp]),
+    "rtcgOccluded1MAsync": (None, [_vp, _vp, _vp, _u32, _sz, _vp]),
+    "rtcgSynchronize": (None, [_vp]),
+    "rtcgTimeTrace": (None, [_vp, _vp, _vp, _u32, _sz, _i32, _i32, _vp]),
+    "rtcgTimeBuild": (None, [_vp, _i32, _vp]),
+    "rtcgTraceCounted": (None, [_vp, _vp, _u32, _sz, _i32, _vp, _vp]),
+    "rtcgGetSceneStats": (None, [_vp, ctypes.POINTER(RTCGSceneStats)]),
+    "rtcgReadBVH": (None, [_vp, _vp, _vp]),
+    "rtcgGenSphere": (_u32, [_u32, _f32, _f32, _f32, _f32, _vp, _vp, ctypes.POINTER(_u32)]),
+    "rtcgGenPinholeRays": (None, [_u32, _u32, _f32, _f32, _f32, _f32, _vp]),
+    "rtcgGenSoup": (None, [_u32, ctypes.c_ulonglong, _f32, _vp, _vp]),
+    "rtcgGenRandomRays": (None, [_u32, ctypes.c_ulonglong, _f32, _f32, _f32, _vp]),
+}
+
+_lib = None
+
+
+def lib():
+    """Loads the HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C embree-aarch64_amd`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def __getattr__(name):
+    if name.startswith("rtc"):
+        return getattr(lib(), name)
+    raise AttributeError(name)
+
+
+def new_rayhits(n):
+    r = np.zeros(n, RTCRayHit)
+    r["tfar"] = np.inf
+    r["mask"] = 0xFFFFFFFF
+    r["geomID"] = RTC_INVALID_GEOMETRY_ID
+    r["primID"] = RTC_INVALID_GEOMETRY_ID
+    r["instID"] = RTC_INVALID_GEOMETRY_ID
+    return r
+
+
+def init_context():
+    """rtcInitIntersectContext (reference rtcore_common.h:209-224)."""
+    c = RTCIntersectContext()
+    c.flags = 0
+    c.filter = None
+    c.instID[0] = RTC_INVALID_GEOMETRY_ID
+    return c
+
+
+class RTCError(RuntimeError):
+    def __init__(self, code, where):
+        super().__init__(f"{where}: RTCError {code}")
+        self.code = code
+
+
+def check(device, where=""):
+    e = lib().rtcGetDeviceError(device)
+    if e != RTC_ERROR_NONE:
+        raise RTCError(e, where)
+
+
+# ---------------------------------------------------------------- helpers --
+
+def gen_sphere(num_phi, center=(0.0, 0.0, 0.0), radius=1.0):
+    L = lib()
+    nv = ctypes.c_uint(0)
+    nt = L.rtcgGenSphere(num_phi, center[0], center[1], center[2], radius, None, None, ctypes.byref(nv))
+    v = np.zeros((nv.value, 3), np.float32)
+    ix = np.zeros((nt, 3), np.uint32)
+    L.rtcgGenSphere(num_phi, center[0], center[1], center[2], radius, v.ctypes.data, ix.ctypes.data, None)
+    return v, ix
+
+
+def gen_pinhole(width, height, org=(0.0, 0.0, 3.0), fov=45.0):
+    r = np.zeros(width * height, RTCRayHit)
+    lib().rtcgGenPinholeRays(width, height, org[0], org[1], org[2], fov, r.ctypes.data)
+    return r
+
+
+def gen_soup(n, seed=1, h=0.005):
+    v = np.zeros((3 * n, 3), np.float32)
+    ix = np.zeros((n, 3), np.uint32)
+    lib().rtcgGenSoup(n, seed, h, v.ctypes.data, ix.ctypes.data)
+    return v, ix
+
+
+def gen_random_rays(n, seed=2, lo=0.0, hi=1.0, tfar=float("inf")):
+    r = np.zeros(n, RTCRayHit)
+    lib().rtcgGenRandomRays(n, seed, lo, hi, tfar, r.ctypes.data)
+    return r
+
+
+class Device:
+    def __init__(self, config=""):
+        self.h = lib().rtcNewDevice(config.encode())
+        if not self.h:
+            raise RTCError(lib().rtcGetDeviceError(None), "rtcNewDevice")
+
+    def check(self, where=""):
+        check(self.h, where)
+
+    def release(self):
+        if self.h:
+            lib().rtcReleaseDevice(self.h)
+            self.h = None
+
+    def malloc(self, nbytes):
+        p = lib().rtcgDeviceMalloc(self.h, nbytes)
+        self.check("rtcgDeviceMalloc")
+        return p
+
+    def free(self, p):
+        lib().rtcgDeviceFree(self.h, p)
+
+    def memcpy(self, dst, src, nbytes):
+        lib().rtcgMemcpy(self.h, dst, src, nbytes)
+        self.check("rtcgMemcpy")
+
+
+class Scene:
+    """A scene of triangle meshes; geomIDs follow attach order."""
+
+    def __init__(self, device, meshes=(), quality=RTC_BUILD_QUALITY_MEDIUM, flags=0, shared=False):
+        L = lib()
+        self.device = device
+        self.h = L.rtcNewScene(device.h)
+        self.keep = []
+        if quality != RTC_BUILD_QUALITY_MEDIUM:
+            L.rtcSetSceneBuildQuality(self.h, quality)
+        if flags:
+            L.rtcSetSceneFlags(self.h, flags)
+        for v, ix in meshes:
+            self.add_mesh(v, ix, shared=shared)
+        device.check("Scene")
+
+    def add_mesh(self, v, ix, shared=False):
+        L = lib()
+        v = np.ascontiguousarray(v, np.float32).reshape(-1, 3)
+        ix = np.ascontiguousarray(ix, np.uint32).reshape(-1, 3)
+        g = L.rtcNewGeometry(self.device.h, RTC_GEOMETRY_TYPE_TRIANGLE)
+        if shared:
+            vp = np.zeros((v.shape[0] + 1, 3), np.float32)  # 4 readable bytes past the last vertex
+            vp[:-1] = v
+            self.keep += [vp, ix]
+            L.rtcSetSharedGeometryBuffer(g, RTC_BUFFER_TYPE_VERTEX, 0, RTC_FORMAT_FLOAT3, vp.ctypes.data, 0, 12,
+                                         v.shape[0])
+            L.rtcSetSharedGeometryBuffer(g, RTC_BUFFER_TYPE_INDEX, 0, RTC_FORMAT_UINT3, ix.ctypes.data, 0, 12,
+                                         ix.shape[0])
+        else:
+            pv = L.rtcSetNewGeometryBuffer(g, RTC_BUFFER_TYPE_VERTEX, 0, RTC_FORMAT_FLOAT3, 12, v.shape[0])
+            pi = L.rtcSetNewGeometryBuffer(g, RTC_BUFFER_TYPE_INDEX, 0, RTC_FORMAT_UINT3, 12, ix.shape[0])
+            if v.size:
+                ctypes.memmove(pv, v.ctypes.data, v.nbytes)
+            if ix.size:
+                ctypes.memmove(pi, ix.ctypes.data, ix.nbytes)
+        L.rtcCommitGeometry(g)
+        gid = L.rtcAttachGeometry(self.h, g)
+        L.rtcReleaseGeometry(g)
+        self.device.check("add_mesh")
+        return gid
+
+    def commit(self):
+        lib().rtcCommitScene(self.h)
+        self.device.check("rtcCommitScene")
+        return self
+
+    def stats(self):
+        s = RTCGSceneStats()
+        lib().rtcgGetSceneStats(self.h, ctypes.byref(s))
+        return {k: getattr(s, k) for k, _ in RTCGSceneStats._fields_ if k != "pad"}
+
+    def bounds(self):
+        b = RTCBounds()
+        lib().rtcGetSceneBounds(self.h, ctypes.byref(b))
+        return b
+
+    def intersect1M(self, rays, stride=None, ctx=None):
+        ctx = ctx or init_context()
+        lib().rtcIntersect1M(self.h, ctypes.byref(ctx), rays.ctypes.data, len(rays), stride or rays.itemsize)
+        self.device.check("rtcIntersect1M")
+
+    def occluded1M(self, rays, stride=None, ctx=None):
+        ctx = ctx or init_context()
+        lib().rtcOccluded1M(self.h, ctypes.byref(ctx), rays.ctypes.data, len(rays), stride or rays.itemsize)
+        self.device.check("rtcOccluded1M")
+
+    def read_bvh(self):
+        s = self.stats()
+        nodes = np.zeros(s["num_nodes"] * 80, np.uint8)
+        tris = np.zeros(s["num_prims"] * 48, np.uint8)
+        lib().rtcgReadBVH(self.h, nodes.ctypes.data, tris.ctypes.data)
+        self.device.check("rtcgReadBVH")
+        return nodes, tris
+
+    def release(self):
+        if self.h:
+            lib().rtcReleaseScene(self.h)
+            self.h = None

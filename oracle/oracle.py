@@ -1,0 +1,165 @@
+"""ctypes binding of the CPU restatement (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py as the checker. The product never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+
+class OracleMesh(ctypes.Structure):
+    _fields_ = [("vertices", ctypes.c_void_p), ("num_vertices", ctypes.c_uint),
+                ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle library not built: run make -C oracle")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, u32, sz, i32 = ctypes.c_void_p, ctypes.c_uint, ctypes.c_size_t, ctypes.c_int
+        fp = ctypes.POINTER(ctypes.c_float)
+        L.oracle_moeller.argtypes = [fp, fp, ctypes.c_float, ctypes.c_float, fp, fp, fp, fp]
+        L.oracle_moeller.restype = i32
+        L.oracle_moeller_verts.argtypes = [fp, fp, ctypes.c_float, ctypes.c_float, fp, fp, fp, fp]
+        L.oracle_moeller_verts.restype = i32
+        L.oracle_intersect_brute.argtypes = [vp, u32, vp, u32, sz, u32, i32]
+        L.oracle_occluded_brute.argtypes = [vp, u32, vp, u32, sz, i32]
+        L.oracle_build_sah.argtypes = [vp, u32]
+        L.oracle_build_sah.restype = vp
+        L.oracle_bvh_free.argtypes = [vp]
+        L.oracle_bvh_stats.argtypes = [vp, vp]
+        L.oracle_intersect_bvh.argtypes = [vp, vp, u32, sz, u32, i32]
+        L.oracle_occluded_bvh.argtypes = [vp, vp, u32, sz, i32]
+        L.oracle_intersect_bvh_counted.argtypes = [vp, vp, u32, sz, vp, vp, i32]
+        L.oracle_bit_interleave.argtypes = [u32, u32, u32]
+        L.oracle_bit_interleave.restype = u32
+        L.oracle_morton_codes.argtypes = [vp, vp, u32, vp]
+        L.oracle_half_area.argtypes = [fp, fp]
+        L.oracle_half_area.restype = ctypes.c_float
+        L.oracle_binned_split.argtypes = [vp, vp, u32, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_float),
+                                          ctypes.POINTER(u32)]
+        L.oracle_binned_split.restype = i32
+        _lib = L
+    return _lib
+
+
+def _f3(a):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def moeller(org, dir, tnear, tfar, v0, e1, e2):
+    """Returns (valid, [T, absDen, U, V, Ngx, Ngy, Ngz, t])."""
+    o, po = _f3(org); d, pd = _f3(dir); a, pa = _f3(v0); b, pb = _f3(e1); c, pc = _f3(e2)
+    out = np.zeros(8, np.float32)
+    v = lib().oracle_moeller(po, pd, float(tnear), float(tfar), pa, pb, pc,
+                             out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return bool(v), out
+
+
+def moeller_verts(org, dir, tnear, tfar, p0, p1, p2):
+    o, po = _f3(org); d, pd = _f3(dir); a, pa = _f3(p0); b, pb = _f3(p1); c, pc = _f3(p2)
+    out = np.zeros(8, np.float32)
+    v = lib().oracle_moeller_verts(po, pd, float(tnear), float(tfar), pa, pb, pc,
+                                   out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return bool(v), out
+
+
+class Meshes:
+    """Keeps numpy arrays alive for the meshes passed to the oracle."""
+
+    def __init__(self, meshes):
+        self.arrays = []
+        arr = (OracleMesh * max(1, len(meshes)))()
+        for i, (v, ix) in enumerate(meshes):
+            v = np.ascontiguousarray(v, dtype=np.float32).reshape(-1, 3)
+            ix = np.ascontiguousarray(ix, dtype=np.uint32).reshape(-1, 3)
+            self.arrays += [v, ix]
+            arr[i].vertices = v.ctypes.data
+            arr[i].num_vertices = v.shape[0]
+            arr[i].indices = ix.ctypes.data
+            arr[i].num_triangles = ix.shape[0]
+        self.c = arr
+        self.n = len(meshes)
+
+
+def intersect_brute(meshes, rayhits, inst_id=0xFFFFFFFF, nthreads=0):
+    m = Meshes(meshes)
+    lib().oracle_intersect_brute(ctypes.addressof(m.c), m.n, rayhits.ctypes.data, len(rayhits),
+                                 rayhits.itemsize, inst_id, nthreads)
+
+
+def occluded_brute(meshes, rays, nthreads=0):
+    m = Meshes(meshes)
+    lib().oracle_occluded_brute(ctypes.addressof(m.c), m.n, rays.ctypes.data, len(rays), rays.itemsize,
+                                nthreads)
+
+
+class BVH:
+    """Reference-style binned-SAH BVH8 built by the CPU restatement."""
+
+    def __init__(self, meshes):
+        self.m = Meshes(meshes)
+        self.h = lib().oracle_build_sah(ctypes.addressof(self.m.c), self.m.n)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().oracle_bvh_free(self.h)
+            self.h = None
+
+    def stats(self):
+        out = np.zeros(8, np.float64)
+        lib().oracle_bvh_stats(self.h, out.ctypes.data)
+        return {"inner_nodes": int(out[0]), "leaves": int(out[1]), "prims": int(out[2]),
+                "max_depth": int(out[3]), "sah": float(out[4]), "root_half_area": float(out[5]),
+                "sah_per_tri": float(out[6])}
+
+    def intersect(self, rayhits, inst_id=0xFFFFFFFF, nthreads=0):
+        lib().oracle_intersect_bvh(self.h, rayhits.ctypes.data, len(rayhits), rayhits.itemsize, inst_id,
+                                   nthreads)
+
+    def occluded(self, rays, nthreads=0):
+        lib().oracle_occluded_bvh(self.h, rays.ctypes.data, len(rays), rays.itemsize, nthreads)
+
+    def intersect_counted(self, rayhits, nthreads=0):
+        n = np.zeros(len(rayhits), np.uint32)
+        t = np.zeros(len(rayhits), np.uint32)
+        lib().oracle_intersect_bvh_counted(self.h, rayhits.ctypes.data, len(rayhits), rayhits.itemsize,
+                                           n.ctypes.data, t.ctypes.data, nthreads)
+        return n, t
+
+
+def bit_interleave(x, y, z):
+    return lib().oracle_bit_interleave(x, y, z)
+
+
+def morton_codes(lo, hi):
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    codes = np.zeros(lo.shape[0], np.uint32)
+    lib().oracle_morton_codes(lo.ctypes.data, hi.ctypes.data, lo.shape[0], codes.ctypes.data)
+    return codes
+
+
+def half_area(lo, hi):
+    a, pa = _f3(lo); b, pb = _f3(hi)
+    return lib().oracle_half_area(pa, pb)
+
+
+def binned_split(lo, hi):
+    lo = np.ascontiguousarray(lo, np.float32)
+    hi = np.ascontiguousarray(hi, np.float32)
+    pos = ctypes.c_uint(0); sah = ctypes.c_float(0); nb = ctypes.c_uint(0)
+    dim = lib().oracle_binned_split(lo.ctypes.data, hi.ctypes.data, lo.shape[0], ctypes.byref(pos),
+                                    ctypes.byref(sah), ctypes.byref(nb))
+    return dim, pos.value, sah.value, nb.value
