@@ -347,7 +347,7 @@ void check_trace_error(ThreadDev& t) {
   if (err) {
     RTG_HIP(hipMemsetAsync(t.errword.ptr, 0, 4, t.stream));
     RTG_HIP(hipStreamSynchronize(t.stream));
-    RTC_THROW(RTC_ERROR_UNKNOWN, "traversal stack overflow");
+    RTC_THROW(RTC_ERROR_UNKNOWN, (err & 1u) ? "traversal stack overflow" : "corrupt acceleration structure");
   }
 }
 

@@ -9,6 +9,7 @@
 //     opens the child with the largest half area: kernels/builders/bvh_builder_sah.h:222-330;
 //   leaf fill e1 = v0-v1, e2 = v2-v0: kernels/geometry/triangle.h:40-41,98-120.
 #include <chrono>
+#include <string>
 
 #include "builder.h"
 
@@ -23,7 +24,8 @@ struct BuildCounters {
   uint32_t tri_count;
   uint32_t qcount[2];
   uint32_t max_depth;
-  uint32_t pad[3];
+  uint32_t error;  // bit 0: refit walk too long, bit 1: collapse index out of range, bit 2: bad tree
+  uint32_t pad[2];
   double sah_nodes, sah_leaves, sah_leaves_tri;
 };
 
@@ -67,6 +69,7 @@ __global__ void k_init_counters(BuildCounters* c) {
   c->tri_count = 0;
   c->qcount[0] = c->qcount[1] = 0;
   c->max_depth = 0;
+  c->error = 0;
   c->sah_nodes = c->sah_leaves = c->sah_leaves_tri = 0.0;
 }
 
@@ -238,6 +241,7 @@ __global__ __launch_bounds__(BT) void k_radix_tree(const uint32_t* __restrict__ 
     if (t <= 1) break;
   }
   const int gamma = i + s * d + (d < 0 ? -1 : 0);
+  if (gamma < 0 || gamma + 1 >= n) return;  // impossible for sorted keys; keeps writes in bounds
   const int first = i < j ? i : j, last = i < j ? j : i;
   const uint32_t left = (first == gamma) ? ((uint32_t)gamma | REF_LEAF) : (uint32_t)gamma;
   const uint32_t right = (last == gamma + 1) ? ((uint32_t)(gamma + 1) | REF_LEAF) : (uint32_t)(gamma + 1);
@@ -259,11 +263,15 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
                                               const uint2* __restrict__ range, const float4* __restrict__ slo,
                                               const float4* __restrict__ shi, float4* nlo, float4* nhi,
                                               float* cost, uint8_t* isleaf, uint32_t* visit, float cost_node,
-                                              float cost_tri, uint32_t max_leaf) {
+                                              float cost_tri, uint32_t max_leaf, uint32_t* err) {
   const uint32_t j = blockIdx.x * BT + threadIdx.x;
   if (j >= n) return;
   uint32_t node = parent_leaf[j];
-  while (node != NONE) {
+  for (int steps = 0; node != NONE; steps++) {
+    if (node >= n - 1 || steps > 512) {  // a radix tree over n keys is at most n-1 levels deep
+      atomicOr(err, 1u);
+      return;
+    }
     const uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0) return;
     float4 bl[2], bh[2];
@@ -317,6 +325,7 @@ struct CollapseArgs {
   QNode8* nodes;
   TriRec* tris;
   BuildCounters* cnt;
+  uint32_t n;  // capacity of nodes, tris and each queue
 };
 
 __device__ inline bool is_inner(uint32_t ref, const uint8_t* isleaf) {
@@ -433,6 +442,11 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
   const uint32_t child_base = ni ? atomicAdd(&a.cnt->node_count, ni) : 0u;
   const uint32_t tri_base = nt ? atomicAdd(&a.cnt->tri_count, nt) : 0u;
   const uint32_t qbase = ni ? atomicAdd(&a.cnt->qcount[a.qout_slot], ni) : 0u;
+  if ((uint64_t)child_base + ni > a.n || (uint64_t)tri_base + nt > a.n || (uint64_t)qbase + ni > a.n ||
+      item.node >= a.n) {
+    atomicOr(&a.cnt->error, 2u);
+    return;
+  }
   atomicMax(&a.cnt->max_depth, item.depth);
   atomicAdd(&a.cnt->sah_nodes, (double)half_area(uh[0] - ul[0], uh[1] - ul[1], uh[2] - ul[2]));
   if (nt) {
@@ -574,7 +588,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
                        S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(),
                        S.slo.as<float4>(), S.shi.as<float4>(), S.nlo.as<float4>(), S.nhi.as<float4>(),
                        S.cost.as<float>(), S.isleaf.as<uint8_t>(), S.visit.as<uint32_t>(), prm.cost_node,
-                       prm.cost_tri, max_leaf);
+                       prm.cost_tri, max_leaf, &cnt->error);
     RTG_HIP(hipGetLastError());
   }
 
@@ -602,7 +616,9 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   ca.nodes = out.nodes.as<QNode8>();
   ca.tris = out.tris.as<TriRec>();
   ca.cnt = cnt;
+  ca.n = n;
   uint32_t nin = 1;
+  int levels = 0;
   int slot = 0;
   QItem* qin = q0;
   QItem* qout = q1;
@@ -619,6 +635,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     RTG_HIP(hipStreamSynchronize(s));
     std::swap(qin, qout);
     slot ^= 1;
+    if (++levels > 4096 || nin > n) throw hip_error(hipErrorUnknown, "BVH collapse did not terminate");
   }
   RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));
@@ -631,6 +648,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     out.stats.sah = (hc.sah_nodes + hc.sah_leaves) / root_area;
     out.stats.sah_tri = (hc.sah_nodes + hc.sah_leaves_tri) / root_area;
   }
+  if (hc.error) throw hip_error(hipErrorUnknown, "BVH build consistency check failed (" + std::to_string(hc.error) + ")");
   if (hc.tri_count != n) throw hip_error(hipErrorUnknown, "BVH build lost triangles");
   out.stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }

@@ -171,8 +171,10 @@ __global__ __launch_bounds__(RS_THREADS) void k_rs_scatter(const uint32_t* __res
         prefix += ((v & ~255u) == tag) ? (v & 255u) : 0u;
       }
       const uint32_t dst = run[d] + prefix + wrank;
-      keys_out[dst] = key;
-      vals_out[dst] = val;
+      if (dst < n) {  // holds for a correct scan; guards memory if it does not
+        keys_out[dst] = key;
+        vals_out[dst] = val;
+      }
     }
     __syncthreads();
     {

@@ -195,8 +195,21 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
   int sp = 0;
   uint32_t cur = 0;  // root node
   bool occluded = false;
+  // every node and triangle record is reached at most once per ray, so the
+  // loop runs at most num_nodes + num_tris times; the cap turns a corrupt
+  // structure into an error instead of a hang
+  const uint64_t max_iter = (uint64_t)sc.num_nodes + sc.num_tris + 16;
+  uint64_t iter = 0;
   while (true) {
+    if (++iter > max_iter) {
+      atomicOr(err, 2u);
+      break;
+    }
     if (!(cur & REF_LEAF)) {
+      if (cur >= sc.num_nodes) {
+        atomicOr(err, 4u);
+        break;
+      }
       NodeHit h;
       intersect_node(sc.nodes + cur, r, lim, h);
       if (COUNT) nvis++;
@@ -236,6 +249,10 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
     } else {
       uint32_t ti = cur & ~REF_LEAF;
       while (true) {
+        if (ti >= sc.num_tris) {
+          atomicOr(err, 4u);
+          break;
+        }
         float T, absDen, U, V, ngx, ngy, ngz;
         uint32_t prim, geom, flags;
         const bool valid = moeller(r, sc.tris + ti, T, absDen, U, V, ngx, ngy, ngz, prim, geom, flags);

@@ -650,6 +650,44 @@ void oracle_bvh_stats(const oracle_bvh* b, double out[8]) {
   out[6] = (nodeSAH + leafSAHTri) / root;
 }
 
+static int o_validate_rec(const oracle_bvh* b, int ref, const float* lo, const float* hi, unsigned char* seen) {
+  if (ref >= 0) {
+    if ((size_t)ref >= b->nnodes) return 10;
+    const o_node* n = &b->nodes[ref];
+    if (n->n < 1 || n->n > O_BRANCH) return 11;
+    for (int i = 0; i < n->n; i++) {
+      if (lo && hi)
+        for (int k = 0; k < 3; k++)
+          if (n->lo[i][k] < lo[k] || n->hi[i][k] > hi[k]) return 12;
+      int r = o_validate_rec(b, n->ref[i], n->lo[i], n->hi[i], seen);
+      if (r) return r;
+    }
+    return 0;
+  }
+  if ((size_t)(~ref) >= b->nleaves) return 20;
+  const o_leaf* l = &b->leaves[~ref];
+  if (l->count < 1 || l->count > O_MAX_LEAF) return 21;
+  for (size_t i = l->begin; i < l->begin + l->count; i++) {
+    if (i >= b->nprims) return 22;
+    if (seen[i]) return 23;
+    seen[i] = 1;
+    if (lo && hi)
+      for (int k = 0; k < 3; k++)
+        if (b->prims[i].lo[k] < lo[k] || b->prims[i].hi[k] > hi[k]) return 24;
+  }
+  return 0;
+}
+
+int oracle_bvh_validate(const oracle_bvh* b) {
+  if (b->root == INT32_MIN) return b->nprims == 0 ? 0 : 1;
+  unsigned char* seen = (unsigned char*)calloc(b->nprims ? b->nprims : 1, 1);
+  int r = o_validate_rec(b, b->root, NULL, NULL, seen);
+  for (size_t i = 0; !r && i < b->nprims; i++)
+    if (!seen[i]) r = 30;
+  free(seen);
+  return r;
+}
+
 /* ------------------------------------------------------------ traversal -- */
 
 static inline float widen_up(float x) { return isinf(x) ? x : x + fabsf(x) * O_MARGIN; }

@@ -100,6 +100,11 @@ void oracle_bvh_free(oracle_bvh* bvh);
  * [5] root half area, [6] SAH cost with leaf cost per triangle. */
 void oracle_bvh_stats(const oracle_bvh* bvh, double out[8]);
 
+/* Structural check: every valid primitive in exactly one leaf, leaves of 1..28
+ * primitives, 1..8 children per inner node, stored child boxes containing
+ * their subtree's primitive boxes. Returns 0 when all hold, else a code. */
+int oracle_bvh_validate(const oracle_bvh* bvh);
+
 /* BVH closest hit / occlusion with the conservative rules above. */
 void oracle_intersect_bvh(const oracle_bvh* bvh, void* rayhits, unsigned nrays, size_t stride,
                           unsigned inst_id, int nthreads);

@@ -38,6 +38,8 @@ def lib():
         L.oracle_build_sah.restype = vp
         L.oracle_bvh_free.argtypes = [vp]
         L.oracle_bvh_stats.argtypes = [vp, vp]
+        L.oracle_bvh_validate.argtypes = [vp]
+        L.oracle_bvh_validate.restype = i32
         L.oracle_intersect_bvh.argtypes = [vp, vp, u32, sz, u32, i32]
         L.oracle_occluded_bvh.argtypes = [vp, vp, u32, sz, i32]
         L.oracle_intersect_bvh_counted.argtypes = [vp, vp, u32, sz, vp, vp, i32]
@@ -123,6 +125,9 @@ class BVH:
         return {"inner_nodes": int(out[0]), "leaves": int(out[1]), "prims": int(out[2]),
                 "max_depth": int(out[3]), "sah": float(out[4]), "root_half_area": float(out[5]),
                 "sah_per_tri": float(out[6])}
+
+    def validate(self):
+        return lib().oracle_bvh_validate(self.h)
 
     def intersect(self, rayhits, inst_id=0xFFFFFFFF, nthreads=0):
         lib().oracle_intersect_bvh(self.h, rayhits.ctypes.data, len(rayhits), rayhits.itemsize, inst_id,

@@ -1,0 +1,387 @@
+"""GPU parity: the HIP library against the CPU restatement on the same seeded
+inputs, through the C-ABI (rtcore.py is a ctypes mirror).
+
+Bar: every RTCRayHit / RTCRay record byte-identical to the oracle's
+(geomID/primID exact, t/u/v/Ng bit-exact; both compute t = T/absDen with
+IEEE division and pick the lexicographic minimum of (t, geomID, primID)).
+Against the reference's own recorded answers the bar is the north-star
+tolerance: t within 1e-6 relative, u/v within 1e-5."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import mismatches, records_equal
+
+pytestmark = pytest.mark.gpu
+INV = 0xFFFFFFFF
+
+
+def trace(rtc, scene, rays, occluded=False, stride=None):
+    r = rays.copy()
+    if occluded:
+        scene.occluded1M(r, stride=stride)
+    else:
+        scene.intersect1M(r, stride=stride)
+    return r
+
+
+def oracle_trace(oracle, meshes, rays, occluded=False, brute=False):
+    r = rays.copy()
+    if brute:
+        (oracle.occluded_brute if occluded else oracle.intersect_brute)(meshes, r)
+    else:
+        bvh = oracle.BVH(meshes)
+        assert bvh.validate() == 0
+        (bvh.occluded if occluded else bvh.intersect)(r)
+    return r
+
+
+def assert_same(a, b, what):
+    bad = mismatches(a, b)
+    assert len(bad) == 0, f"{what}: {len(bad)} of {len(a)} records differ, first {bad[:5]}: {a[bad[:2]]} vs {b[bad[:2]]}"
+
+
+# ------------------------------------------------------------------ basics --
+
+def test_device_properties(rtc, device):
+    L = rtc.lib()
+    assert L.rtcGetDeviceProperty(device.h, rtc.RTC_DEVICE_PROPERTY_VERSION) == 31201
+    assert L.rtcGetDeviceProperty(device.h, rtc.RTC_DEVICE_PROPERTY_RAY_STREAM_SUPPORTED) == 1
+    assert L.rtcGetDeviceProperty(device.h, rtc.RTC_DEVICE_PROPERTY_TRIANGLE_GEOMETRY_SUPPORTED) == 1
+    L.rtcGetDeviceProperty(device.h, 12345)
+    assert L.rtcGetDeviceError(device.h) == rtc.RTC_ERROR_INVALID_ARGUMENT
+    assert L.rtcGetDeviceError(device.h) == rtc.RTC_ERROR_NONE
+
+
+def test_single_triangle_reference_answer(rtc, device, oracle, golden_probe):
+    g = golden_probe["single_triangle"]
+    v = np.array(g["vertices"], np.float32)
+    ix = np.array(g["indices"], np.uint32)
+    sc = rtc.Scene(device, [(v, ix)]).commit()
+    r = rtc.new_rayhits(2)
+    r["org_x"], r["org_y"], r["org_z"] = g["ray"]["org"]
+    r["dir_x"], r["dir_y"], r["dir_z"] = g["ray"]["dir"]
+    out = trace(rtc, sc, r)
+    e = g["expected"]
+    assert (out["geomID"] == e["geomID"]).all() and (out["primID"] == e["primID"]).all()
+    assert np.all(np.abs(out["tfar"] - e["t"]) <= 1e-6 * e["t"])
+    assert np.all(np.abs(out["u"] - e["u"]) <= 1e-5) and np.all(np.abs(out["v"] - e["v"]) <= 1e-5)
+    assert np.array([out["Ng_x"][0], out["Ng_y"][0], out["Ng_z"][0]], np.float32).tobytes() == \
+        np.array(e["Ng"], np.float32).tobytes()
+    assert records_equal(out, oracle_trace(oracle, [(v, ix)], r, brute=True))
+    sc.release()
+
+
+def test_rtcIntersect1_and_rtcOccluded1(rtc, device):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    ix = np.array([[0, 1, 2]], np.uint32)
+    sc = rtc.Scene(device, [(v, ix)]).commit()
+    r = rtc.new_rayhits(1)
+    r["org_x"], r["org_y"], r["org_z"] = 0.2, 0.3, -2.0
+    r["dir_z"] = 1.0
+    ctx = rtc.init_context()
+    rtc.lib().rtcIntersect1(sc.h, ctypes.byref(ctx), r.ctypes.data)
+    device.check()
+    assert r["primID"][0] == 0 and r["tfar"][0] == np.float32(2.0)
+    o = np.zeros(1, rtc.RTCRay)
+    o["org_x"], o["org_y"], o["org_z"], o["dir_z"], o["tfar"] = 0.2, 0.3, -2.0, 1.0, 1.5
+    rtc.lib().rtcOccluded1(sc.h, ctypes.byref(ctx), o.ctypes.data)
+    assert o["tfar"][0] == np.float32(1.5)  # hit at 2 is beyond tfar
+    o["tfar"] = 2.5
+    rtc.lib().rtcOccluded1(sc.h, ctypes.byref(ctx), o.ctypes.data)
+    assert np.isneginf(o["tfar"][0])
+    sc.release()
+
+
+# --------------------------------------------------------- C1: sphere 65k --
+
+@pytest.fixture(scope="module")
+def c1(rtc, device):
+    v, ix = rtc.gen_sphere(128)
+    sc = rtc.Scene(device, [(v, ix)]).commit()
+    yield v, ix, sc
+    sc.release()
+
+
+def test_c1_sphere_parity_and_reference_hit_count(rtc, oracle, c1, golden_probe):
+    v, ix, sc = c1
+    rays = rtc.gen_pinhole(1024, 1024)
+    out = trace(rtc, sc, rays)
+    ref = oracle_trace(oracle, [(v, ix)], rays)
+    assert int((out["geomID"] != INV).sum()) == golden_probe["uv_sphere_pinhole"]["expected_hits"]
+    assert_same(out, ref, "C1 closest hit")
+
+
+def test_c1_device_pointer_path(rtc, device, c1):
+    """Records already in HBM are traced in place (same results as host records)."""
+    v, ix, sc = c1
+    rays = rtc.gen_pinhole(256, 256)
+    host = trace(rtc, sc, rays)
+    d = device.malloc(rays.nbytes)
+    try:
+        device.memcpy(d, rays.ctypes.data, rays.nbytes)
+        ctx = rtc.init_context()
+        rtc.lib().rtcIntersect1M(sc.h, ctypes.byref(ctx), d, len(rays), 80)
+        device.check()
+        back = np.zeros_like(rays)
+        device.memcpy(back.ctypes.data, d, rays.nbytes)
+        assert records_equal(back, host)
+        # async extension on the device's stream
+        device.memcpy(d, rays.ctypes.data, rays.nbytes)
+        rtc.lib().rtcgIntersect1MAsync(sc.h, None, d, len(rays), 80, None)
+        rtc.lib().rtcgSynchronize(device.h)
+        device.check()
+        device.memcpy(back.ctypes.data, d, rays.nbytes)
+        assert records_equal(back, host)
+    finally:
+        device.free(d)
+
+
+def test_c1_bvh_structure(rtc, c1):
+    """Every triangle appears in exactly one leaf, every decoded child box
+    contains its subtree, triangle records equal v0, v0-v1, v2-v0."""
+    v, ix, sc = c1
+    nodes, tris = sc.read_bvh()
+    check_bvh(nodes, tris, [(v, ix)])
+
+
+def check_bvh(nodes, tris, meshes):
+    N = nodes.reshape(-1, 80)
+    T = tris.reshape(-1, 48)
+    nn, nt = len(N), len(T)
+    f = N[:, :12].view(np.float32)
+    e = N[:, 12:15].astype(np.uint32)
+    imask = N[:, 15]
+    base = N[:, 16:24].view(np.uint32)
+    meta = N[:, 24:32]
+    q = N[:, 32:80].reshape(nn, 3, 2, 8).astype(np.float32)  # axis, lo/hi, slot
+    scale = (e << 23).view(np.float32)
+    planes = q * scale[:, :, None, None] + f[:, :, None, None]  # fma vs mul+add: conservative check below
+    tf = T.view(np.float32).reshape(-1, 12)
+    tu = T.view(np.uint32).reshape(-1, 12)
+    seen = np.zeros(nt, np.int32)
+    visited = np.zeros(nn, np.int32)
+    stack = [0]
+    while stack:
+        n = stack.pop()
+        visited[n] += 1
+        for c in range(8):
+            m = int(meta[n, c])
+            if m == 0:
+                continue
+            lo = planes[n, :, 0, c] - 1e-6 * np.abs(planes[n, :, 0, c]) - 1e-30
+            hi = planes[n, :, 1, c] + 1e-6 * np.abs(planes[n, :, 1, c]) + 1e-30
+            if (imask[n] >> c) & 1:
+                stack.append(int(base[n, 0]) + (m & 0x7F))
+            else:
+                t = int(base[n, 1]) + (m & 0x7F)
+                while True:
+                    seen[t] += 1
+                    v0 = tf[t, 0:3]
+                    v1 = v0 - tf[t, 4:7]
+                    v2 = v0 + tf[t, 8:11]
+                    for p in (v0, v1, v2):
+                        assert (p >= lo - 1e-6).all() and (p <= hi + 1e-6).all(), (n, c, t)
+                    if tu[t, 11] & 1:
+                        break
+                    t += 1
+    assert (visited == 1).all(), "every node reached exactly once"
+    assert (seen == 1).all(), "every triangle record in exactly one leaf"
+    # records hold the right triangles: (geomID, primID) exactly once each, edges as the reference forms them
+    ids = set()
+    for t in range(nt):
+        g, p = int(tu[t, 7]), int(tu[t, 3])
+        ids.add((g, p))
+        vv, ii = meshes[g]
+        a, b, cc = vv[ii[p]]
+        assert np.array_equal(tf[t, 0:3], a)
+        assert np.array_equal(tf[t, 4:7], (a - b).astype(np.float32))
+        assert np.array_equal(tf[t, 8:11], (cc - a).astype(np.float32))
+    assert len(ids) == nt
+
+
+# ------------------------------------------------------ soup: small cases --
+
+def soup_case(rtc, n, seed, h=0.005):
+    return rtc.gen_soup(n, seed=seed, h=h)
+
+
+@pytest.mark.parametrize("quality", [0, 1, 2])
+def test_soup_parity_with_edge_rays(rtc, device, oracle, quality):
+    v, ix = soup_case(rtc, 20000, 5, h=0.02)
+    sc = rtc.Scene(device, [(v, ix)], quality=quality).commit()
+    rays = rtc.gen_random_rays(100000, seed=6, lo=-0.05, hi=1.05)
+    rays["tfar"][::5] = 0.05
+    rays["tnear"][1::9] = 0.02
+    rays["tnear"][2::13] = 3.0
+    rays["tfar"][2::13] = 1.0          # tnear > tfar: untouched
+    rays["dir_x"][3::17] = np.nan      # NaN direction: never hits
+    rays["org_z"][4::19] = np.inf
+    rays["dir_y"][5::23] = 0.0         # axis-parallel components
+    rays["dir_z"][5::23] = 0.0
+    out = trace(rtc, sc, rays)
+    ref = oracle_trace(oracle, [(v, ix)], rays)
+    assert_same(out, ref, f"soup quality {quality}")
+    untouched = np.zeros(len(rays), bool)
+    untouched[2::13] = True
+    assert records_equal(out[untouched], rays[untouched])
+    assert (out["geomID"] != INV).mean() > 0.2
+    sc.release()
+
+
+def test_soup_occlusion_parity(rtc, device, oracle):
+    v, ix = soup_case(rtc, 20000, 7, h=0.02)
+    sc = rtc.Scene(device, [(v, ix)]).commit()
+    rh = rtc.gen_random_rays(100000, seed=8, tfar=0.1)
+    rays = np.zeros(len(rh), rtc.RTCRay)
+    for k in rtc.RTCRay.names:
+        rays[k] = rh[k]
+    out = trace(rtc, sc, rays, occluded=True)
+    ref = oracle_trace(oracle, [(v, ix)], rays, occluded=True)
+    assert_same(out, ref, "soup occlusion")
+    assert 0.05 < np.isneginf(out["tfar"]).mean() < 0.95
+    sc.release()
+
+
+def test_brute_force_small(rtc, device, oracle):
+    v, ix = soup_case(rtc, 3000, 9, h=0.05)
+    sc = rtc.Scene(device, [(v, ix)]).commit()
+    rays = rtc.gen_random_rays(5000, seed=10)
+    assert_same(trace(rtc, sc, rays), oracle_trace(oracle, [(v, ix)], rays, brute=True), "brute force")
+    sc.release()
+
+
+def test_multi_geometry_ties_and_invalid(rtc, device, oracle):
+    rng = np.random.default_rng(3)
+    v1, i1 = soup_case(rtc, 4000, 11, h=0.03)
+    v2, i2 = rtc.gen_sphere(32, (0.5, 0.5, 0.5), 0.3)
+    # duplicates of some triangles of mesh 1 (exact ties across primIDs and geomIDs)
+    i3 = np.concatenate([i1[:500], i1[:500]])
+    # invalid triangles: index out of range, NaN vertex, huge vertex
+    v4 = np.concatenate([v1[:30], np.array([[np.nan, 0, 0], [3e18, 0, 0]], np.float32)])
+    i4 = np.array([[0, 1, 2], [0, 1, 99], [30, 1, 2], [31, 1, 2], [3, 4, 5]], np.uint32)
+    meshes = [(v1, i1), (v2, i2), (v1, i3), (v4, i4)]
+    sc = rtc.Scene(device, meshes).commit()
+    assert sc.stats()["num_prims"] == len(i1) + len(i2) + len(i3) + 2
+    rays = rtc.gen_random_rays(50000, seed=12)
+    out = trace(rtc, sc, rays)
+    ref = oracle_trace(oracle, meshes, rays)
+    assert_same(out, ref, "multi geometry")
+    assert set(np.unique(out["geomID"])) >= {0, 1, INV}
+    # a disabled / detached geometry disappears after the next commit
+    g1 = rtc.lib().rtcGetGeometry(sc.h, 1)
+    rtc.lib().rtcDisableGeometry(g1)
+    rtc.lib().rtcDetachGeometry(sc.h, 3)
+    sc.commit()
+    out2 = trace(rtc, sc, rays)
+    ref2 = oracle_trace(oracle, [(v1, i1), (v2, i2[:0]), (v1, i3)], rays)
+    assert_same(out2, ref2, "after disable/detach")
+    sc.release()
+
+
+def test_empty_and_degenerate_scenes(rtc, device, oracle):
+    rays = rtc.gen_random_rays(1000, seed=13)
+    empty = rtc.Scene(device, []).commit()
+    assert records_equal(trace(rtc, empty, rays), rays)
+    b = empty.bounds()
+    assert b.lower_x == np.inf and b.upper_x == -np.inf
+    empty.release()
+    # one triangle, all triangles identical, zero-area triangles
+    v = np.array([[0.1, 0.1, 0.1], [0.9, 0.1, 0.1], [0.1, 0.9, 0.5]], np.float32)
+    for ix in (np.array([[0, 1, 2]], np.uint32), np.tile(np.array([[0, 1, 2]], np.uint32), (100, 1)),
+               np.array([[0, 0, 0], [1, 1, 2], [0, 1, 2]], np.uint32)):
+        sc = rtc.Scene(device, [(v, ix)]).commit()
+        assert_same(trace(rtc, sc, rays), oracle_trace(oracle, [(v, ix)], rays, brute=True), f"degenerate {len(ix)}")
+        sc.release()
+
+
+def test_strided_and_shared_buffers(rtc, device, oracle):
+    v, ix = soup_case(rtc, 5000, 14, h=0.03)
+    sc = rtc.Scene(device, [(v, ix)], shared=True).commit()
+    rays = rtc.gen_random_rays(20000, seed=15)
+    ref = oracle_trace(oracle, [(v, ix)], rays)
+    padded = np.dtype({"names": list(rays.dtype.names) + ["pad"],
+                       "formats": [rays.dtype.fields[k][0] for k in rays.dtype.names] + ["<u4"],
+                       "offsets": [rays.dtype.fields[k][1] for k in rays.dtype.names] + [80], "itemsize": 96})
+    pr = np.zeros(len(rays), padded)
+    for k in rays.dtype.names:
+        pr[k] = rays[k]
+    pr["pad"] = 0xDEADBEEF
+    sc.intersect1M(pr, stride=96)
+    got = np.zeros(len(rays), rays.dtype)
+    for k in rays.dtype.names:
+        got[k] = pr[k]
+    assert_same(got, ref, "stride 96")
+    assert (pr["pad"] == 0xDEADBEEF).all()
+    sc.release()
+
+
+def test_error_paths(rtc, device):
+    L = rtc.lib()
+    assert not L.rtcNewGeometry(device.h, rtc.RTC_GEOMETRY_TYPE_QUAD)
+    assert L.rtcGetDeviceError(device.h) == rtc.RTC_ERROR_INVALID_OPERATION
+    g = L.rtcNewGeometry(device.h, rtc.RTC_GEOMETRY_TYPE_TRIANGLE)
+    buf = np.zeros(30, np.float32)
+    L.rtcSetSharedGeometryBuffer(g, rtc.RTC_BUFFER_TYPE_VERTEX, 0, rtc.RTC_FORMAT_FLOAT4, buf.ctypes.data, 0, 16, 2)
+    assert L.rtcGetDeviceError(device.h) == rtc.RTC_ERROR_INVALID_OPERATION
+    L.rtcSetSharedGeometryBuffer(g, rtc.RTC_BUFFER_TYPE_VERTEX, 0, rtc.RTC_FORMAT_FLOAT3, buf.ctypes.data + 2, 0, 12, 2)
+    assert L.rtcGetDeviceError(device.h) == rtc.RTC_ERROR_INVALID_ARGUMENT
+    L.rtcReleaseGeometry(g)
+    sc = L.rtcNewScene(device.h)
+    r = rtc.new_rayhits(4)
+    L.rtcIntersect1M(sc, None, r.ctypes.data, 4, 80)
+    assert L.rtcGetDeviceError(device.h) == rtc.RTC_ERROR_INVALID_OPERATION  # not committed
+    # error callback sees the code
+    seen = []
+    cb = rtc.ERROR_FUNC(lambda p, code, msg: seen.append(code))
+    L.rtcSetDeviceErrorFunction(device.h, cb, None)
+    L.rtcSetSceneBuildQuality(sc, 7)
+    assert seen == [rtc.RTC_ERROR_INVALID_ARGUMENT]
+    L.rtcSetDeviceErrorFunction(device.h, rtc.ERROR_FUNC(0), None)
+    assert L.rtcGetDeviceError(device.h) == rtc.RTC_ERROR_INVALID_ARGUMENT
+    # progress monitor cancellation and memory monitor refusal
+    v, ix = soup_case(rtc, 100, 1)
+    s2 = rtc.Scene(device, [(v, ix)])
+    pm = rtc.PROGRESS_FUNC(lambda p, n: False)
+    L.rtcSetSceneProgressMonitorFunction(s2.h, pm, None)
+    L.rtcCommitScene(s2.h)
+    assert L.rtcGetDeviceError(device.h) == rtc.RTC_ERROR_CANCELLED
+    s2.release()
+    d2 = rtc.Device("")
+    mm = rtc.MEMORY_MONITOR_FUNC(lambda p, b, post: False)
+    L.rtcSetDeviceMemoryMonitorFunction(d2.h, mm, None)
+    s3 = rtc.Scene(d2, [(v, ix)])
+    L.rtcCommitScene(s3.h)
+    assert L.rtcGetDeviceError(d2.h) == rtc.RTC_ERROR_OUT_OF_MEMORY
+    s3.release()
+    d2.release()
+    L.rtcReleaseScene(sc)
+
+
+# -------------------------------------------------- C2: full-size soup --
+
+@pytest.mark.slow
+def test_c2_full_size_parity(rtc, device, oracle):
+    """2^20-triangle soup, 2^24 incoherent rays: closest hit and the tfar=0.1
+    occlusion pass, every record identical to the oracle's."""
+    v, ix = rtc.gen_soup(1 << 20, seed=1, h=0.005)
+    sc = rtc.Scene(device, [(v, ix)]).commit()
+    rays = rtc.gen_random_rays(1 << 24, seed=2)
+    bvh = oracle.BVH([(v, ix)])
+    out = trace(rtc, sc, rays)
+    ref = rays.copy()
+    bvh.intersect(ref)
+    assert_same(out, ref, "C2 closest hit")
+    hitrate = (out["geomID"] != INV).mean()
+    del out, ref
+    occ = np.zeros(len(rays), rtc.RTCRay)
+    for k in rtc.RTCRay.names:
+        occ[k] = rays[k]
+    occ["tfar"] = 0.1
+    del rays
+    o_gpu = trace(rtc, sc, occ, occluded=True)
+    bvh.occluded(occ)
+    assert_same(o_gpu, occ, "C2 occlusion")
+    assert 0.5 < hitrate < 1.0
+    sc.release()
