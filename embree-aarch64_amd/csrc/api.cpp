@@ -98,8 +98,10 @@ struct Device : RefObj {
 // per-thread first-error slots (reference: device.cpp:258-285)
 thread_local std::unordered_map<const Device*, RTCError> tls_errors;
 thread_local RTCError tls_null_error = RTC_ERROR_NONE;
+thread_local std::string tls_last_message;
 
 void process_error(Device* d, RTCError code, const char* msg) {
+  tls_last_message = msg ? msg : "";
   if (!d) {
     if (tls_null_error == RTC_ERROR_NONE) tls_null_error = code;
     return;
@@ -1006,6 +1008,8 @@ RTC_API void rtcgOccluded1MAsync(RTCScene scene, const RTCIntersectContext* cont
   async_trace(s, context, d_ray, M, byteStride, true, hipStream);
   API_END(s)
 }
+
+RTC_API const char* rtcgGetErrorString(void) { return tls_last_message.c_str(); }
 
 RTC_API void rtcgSynchronize(RTCDevice device) {
   Device* d = DE(device);

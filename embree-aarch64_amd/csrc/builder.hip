@@ -304,6 +304,29 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
   }
 }
 
+// Integrity check of the sorted keys and the radix tree (cheap, one pass):
+// bit 3 keys out of order, bit 4 inner node not finished by the refit,
+// bit 5 parent/child links disagree.
+__global__ __launch_bounds__(BT) void k_check_tree(const uint32_t* __restrict__ keys, uint32_t n,
+                                                   const uint32_t* __restrict__ child,
+                                                   const uint32_t* __restrict__ parent_int,
+                                                   const uint32_t* __restrict__ parent_leaf,
+                                                   const uint32_t* __restrict__ visit, uint32_t* err) {
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  if (i + 1 < n && keys[i] > keys[i + 1]) atomicOr(err, 8u);
+  if (i < n) {
+    const uint32_t p = parent_leaf[i];
+    if (p >= n - 1 || (child[2 * p] != (i | REF_LEAF) && child[2 * p + 1] != (i | REF_LEAF))) atomicOr(err, 32u);
+  }
+  if (i + 1 < n) {
+    if (visit[i] != 2u) atomicOr(err, 16u);
+    if (i > 0) {
+      const uint32_t p = parent_int[i];
+      if (p >= n - 1 || (child[2 * p] != i && child[2 * p + 1] != i)) atomicOr(err, 32u);
+    }
+  }
+}
+
 // ---------------------------------------------------------------- collapse --
 
 struct CollapseArgs {
@@ -399,10 +422,12 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
     n++;
   }
 
-  // boxes, ranges, allocation sizes
+  // per-child primitive ranges live in LDS (one row per thread)
+  __shared__ uint32_t s_first[BT][8];
+  __shared__ uint32_t s_count[BT][8];
+  const uint32_t* rng32 = reinterpret_cast<const uint32_t*>(a.range);
   float bl[8][3], bh[8][3];
-  uint32_t first[8], cntp[8];
-  uint32_t ni = 0, nt = 0;
+  uint32_t ni = 0, nt = 0, ninner_prims = 0;
   float ul[3] = {INFINITY, INFINITY, INFINITY}, uh[3] = {-INFINITY, -INFINITY, -INFINITY};
   double sah_leaf = 0.0, sah_leaf_tri = 0.0;
 #pragma unroll
@@ -410,18 +435,20 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
     if (k < n) {
       const uint32_t c = ch[k];
       float4 l, h;
+      uint32_t f, cnt;
       if (c & REF_LEAF) {
         l = a.slo[c & ~REF_LEAF];
         h = a.shi[c & ~REF_LEAF];
-        first[k] = c & ~REF_LEAF;
-        cntp[k] = 1;
+        f = c & ~REF_LEAF;
+        cnt = 1u;
       } else {
         l = a.nlo[c];
         h = a.nhi[c];
-        const uint2 r = a.range[c];
-        first[k] = r.x;
-        cntp[k] = r.y - r.x + 1;
+        f = rng32[2 * c];
+        cnt = rng32[2 * c + 1] - f + 1u;
       }
+      s_first[threadIdx.x][k] = f;
+      s_count[threadIdx.x][k] = cnt;
       bl[k][0] = l.x; bl[k][1] = l.y; bl[k][2] = l.z;
       bh[k][0] = h.x; bh[k][1] = h.y; bh[k][2] = h.z;
 #pragma unroll
@@ -431,13 +458,20 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
       }
       if (is_inner(c, a.isleaf)) {
         ni++;
+        ninner_prims += cnt;
       } else {
-        nt += cntp[k];
+        nt += cnt;
         const float A = box_half_area(l, h);
-        sah_leaf += (double)A * (double)((cntp[k] + 3) / 4);
-        sah_leaf_tri += (double)A * (double)cntp[k];
+        sah_leaf += (double)A * (double)((cnt + 3) / 4);
+        sah_leaf_tri += (double)A * (double)cnt;
       }
     }
+  }
+  // the children partition the parent's range: a second count of the leaf primitives
+  {
+    const uint32_t b = item.bref;
+    const uint32_t total_prims = (b & REF_LEAF) ? 1u : (rng32[2 * b + 1] - rng32[2 * b] + 1u);
+    if (total_prims - ninner_prims != nt) atomicOr(&a.cnt->error, 64u);
   }
   const uint32_t child_base = ni ? atomicAdd(&a.cnt->node_count, ni) : 0u;
   const uint32_t tri_base = nt ? atomicAdd(&a.cnt->tri_count, nt) : 0u;
@@ -487,8 +521,10 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
       } else {
         q.meta[k] = (uint8_t)(0x80u | off_t);
         // leaf triangles
-        for (uint32_t j = 0; j < cntp[k]; j++) {
-          const uint32_t sj = first[k] + j;
+        const uint32_t cntk = s_count[threadIdx.x][k];
+        const uint32_t firstk = s_first[threadIdx.x][k];
+        for (uint32_t j = 0; j < cntk; j++) {
+          const uint32_t sj = firstk + j;
           const uint32_t o = a.vals[sj];
           const uint32_t gi = __float_as_uint(a.lo4[o].w);
           const uint32_t prim = __float_as_uint(a.hi4[o].w);
@@ -503,10 +539,10 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
           tr.e2x = p2[0] - p0[0]; tr.e2y = p2[1] - p0[1]; tr.e2z = p2[2] - p0[2];
           tr.primID = prim;
           tr.geomID = g.geomID;
-          tr.flags = (j + 1 == cntp[k]) ? TRI_LAST : 0u;
+          tr.flags = (j + 1 == cntk) ? TRI_LAST : 0u;
           a.tris[tri_base + off_t + j] = tr;
         }
-        off_t += cntp[k];
+        off_t += cntk;
       }
     } else {
       q.meta[k] = 0;
@@ -589,6 +625,9 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
                        S.slo.as<float4>(), S.shi.as<float4>(), S.nlo.as<float4>(), S.nhi.as<float4>(),
                        S.cost.as<float>(), S.isleaf.as<uint8_t>(), S.visit.as<uint32_t>(), prm.cost_node,
                        prm.cost_tri, max_leaf, &cnt->error);
+    hipLaunchKernelGGL(k_check_tree, dim3(div_up(n, BT)), dim3(BT), 0, s, S.keys.as<uint32_t>(), n,
+                       S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
+                       S.visit.as<uint32_t>(), &cnt->error);
     RTG_HIP(hipGetLastError());
   }
 
@@ -619,6 +658,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   ca.n = n;
   uint32_t nin = 1;
   int levels = 0;
+  std::string level_sizes;
   int slot = 0;
   QItem* qin = q0;
   QItem* qout = q1;
@@ -635,10 +675,16 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     RTG_HIP(hipStreamSynchronize(s));
     std::swap(qin, qout);
     slot ^= 1;
+    level_sizes += " " + std::to_string(nin);
     if (++levels > 4096 || nin > n) throw hip_error(hipErrorUnknown, "BVH collapse did not terminate");
   }
   RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));
+  if (hc.error || hc.tri_count != n)
+    throw hip_error(hipErrorUnknown, "BVH build consistency check failed: error bits " + std::to_string(hc.error) +
+                                         ", primitives " + std::to_string(n) + ", triangle records " +
+                                         std::to_string(hc.tri_count) + ", nodes " + std::to_string(hc.node_count) +
+                                         ", queue sizes per level" + level_sizes);
   out.stats.num_nodes = hc.node_count;
   out.stats.max_depth = hc.max_depth;
   const double root_area = half_area(out.stats.bounds_hi[0] - out.stats.bounds_lo[0],

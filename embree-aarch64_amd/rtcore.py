@@ -131,6 +131,7 @@ SIGNATURES = {
     "rtcgIntersect1MAsync": (None, [_vp, _vp, _vp, _u32, _sz, _vp]),
     "rtcgOccluded1MAsync": (None, [_vp, _vp, _vp, _u32, _sz, _vp]),
     "rtcgSynchronize": (None, [_vp]),
+    "rtcgGetErrorString": (ctypes.c_char_p, []),
     "rtcgTimeTrace": (None, [_vp, _vp, _vp, _u32, _sz, _i32, _i32, _vp]),
     "rtcgTimeBuild": (None, [_vp, _i32, _vp]),
     "rtcgTraceCounted": (None, [_vp, _vp, _u32, _sz, _i32, _vp, _vp]),
@@ -186,15 +187,16 @@ def init_context():
 
 
 class RTCError(RuntimeError):
-    def __init__(self, code, where):
-        super().__init__(f"{where}: RTCError {code}")
+    def __init__(self, code, where, message=""):
+        super().__init__(f"{where}: RTCError {code}: {message}")
         self.code = code
+        self.message = message
 
 
 def check(device, where=""):
     e = lib().rtcGetDeviceError(device)
     if e != RTC_ERROR_NONE:
-        raise RTCError(e, where)
+        raise RTCError(e, where, (lib().rtcgGetErrorString() or b"").decode(errors="replace"))
 
 
 # ---------------------------------------------------------------- helpers --

@@ -26,6 +26,10 @@ RTC_API void rtcgIntersect1MAsync(RTCScene scene, const struct RTCIntersectConte
 RTC_API void rtcgOccluded1MAsync(RTCScene scene, const struct RTCIntersectContext* context, struct RTCRay* d_ray,
                                  unsigned int M, size_t byteStride, void* hipStream);
 
+/* Message of the calling thread's most recent error (any device); the
+ * pointer stays valid until the thread's next error. */
+RTC_API const char* rtcgGetErrorString(void);
+
 /* Waits for the calling thread's stream of this device. */
 RTC_API void rtcgSynchronize(RTCDevice device);
 
