@@ -52,12 +52,14 @@ struct ThreadDev {
   rtg::DevBuf staging;
   rtg::DevBuf errword;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  rtg::RaySorter sorter;
 };
 
 struct Device : RefObj {
   int ordinal = 0;
   int verbose = 0;
   rtg::BuildParams build;
+  rtg::TraceParams trace;
   RTCErrorFunction errfn = nullptr;
   void* errptr = nullptr;
   RTCMemoryMonitorFunction memfn = nullptr;
@@ -227,6 +229,12 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "bvh_leaf" || k == "max_leaf") d->build.max_leaf = (uint32_t)atoi(v.c_str());
     else if (k == "cost_node") d->build.cost_node = (float)atof(v.c_str());
     else if (k == "cost_tri") d->build.cost_tri = (float)atof(v.c_str());
+    else if (k == "trace_stack") d->trace.lds_slots = atoi(v.c_str());
+    else if (k == "ray_sort") d->trace.sort_rays = atoi(v.c_str());
+    else if (k == "ray_sort_min") d->trace.sort_min = (uint32_t)atol(v.c_str());
+    else if (k == "persistent") d->trace.persistent = atoi(v.c_str());
+    else if (k == "refill_min") d->trace.refill_min = atoi(v.c_str());
+    else if (k == "blocks_per_cu") d->trace.blocks_per_cu = atoi(v.c_str());
     // threads, isa, tri_accel, tri_builder, set_affinity, ... are CPU settings: accepted, ignored
   }
 }
@@ -368,7 +376,7 @@ void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigne
   if (sc->bvh.stats.num_nodes == 0) return;  // empty scene: every ray misses
   if (is_device_ptr(d, rays)) {
     rtg::launch_trace(sc->bvh.view(), rays, M, stride, occluded, inst, nullptr, nullptr, t.errword.as<uint32_t>(),
-                      t.stream);
+                      t.stream, d->trace, &t.sorter);
     check_trace_error(t);
     return;
   }
@@ -383,7 +391,7 @@ void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigne
     RTG_HIP(hipStreamSynchronize(t.stream));
   }
   rtg::launch_trace(sc->bvh.view(), t.staging.ptr, M, rec, occluded, inst, nullptr, nullptr,
-                    t.errword.as<uint32_t>(), t.stream);
+                    t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter);
   if (stride == rec) {
     RTG_HIP(hipMemcpyAsync(rays, t.staging.ptr, (size_t)M * rec, hipMemcpyDeviceToHost, t.stream));
     RTG_HIP(hipStreamSynchronize(t.stream));
@@ -416,6 +424,11 @@ RTC_API RTCDevice rtcNewDevice(const char* config) {
     RTC_THROW(RTC_ERROR_UNSUPPORTED_CPU, "no usable GPU");
   }
   d->activate();
+  {
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, d->ordinal) == hipSuccess && prop.multiProcessorCount > 0)
+      d->trace.cus = prop.multiProcessorCount;
+  }
   if (d->verbose) {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, d->ordinal) == hipSuccess)
@@ -988,7 +1001,7 @@ static void async_trace(Scene* s, const RTCIntersectContext* ctx, void* rays, un
   ThreadDev& t = d->td();
   const uint32_t inst = ctx ? ctx->instID[0] : RTC_INVALID_GEOMETRY_ID;
   rtg::launch_trace(s->bvh.view(), rays, M, stride, occluded, inst, nullptr, nullptr, t.errword.as<uint32_t>(),
-                    stream ? (hipStream_t)stream : t.stream);
+                    stream ? (hipStream_t)stream : t.stream, d->trace, &t.sorter);
 }
 
 RTC_API void rtcgIntersect1MAsync(RTCScene scene, const RTCIntersectContext* context, RTCRayHit* d_rayhit,
@@ -1035,7 +1048,7 @@ RTC_API void rtcgTimeTrace(RTCScene scene, void* d_rays, const void* d_src, unsi
     if (d_src) RTG_HIP(hipMemcpyAsync(d_rays, d_src, (size_t)M * byteStride, hipMemcpyDeviceToDevice, t.stream));
     RTG_HIP(hipEventRecord(t.ev0, t.stream));
     rtg::launch_trace(s->bvh.view(), d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, nullptr, nullptr,
-                      t.errword.as<uint32_t>(), t.stream);
+                      t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter);
     RTG_HIP(hipEventRecord(t.ev1, t.stream));
     RTG_HIP(hipEventSynchronize(t.ev1));
     float ms = 0.0f;
@@ -1083,7 +1096,7 @@ RTC_API void rtcgTraceCounted(RTCScene scene, void* d_rays, unsigned int M, size
     RTG_HIP(hipMemsetAsync(d_tris, 0, (size_t)M * 4, t.stream));
   } else {
     rtg::launch_trace(s->bvh.view(), d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, d_nodes, d_tris,
-                      t.errword.as<uint32_t>(), t.stream);
+                      t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter);
   }
   RTG_HIP(hipStreamSynchronize(t.stream));
   check_trace_error(t);

@@ -13,8 +13,8 @@ namespace rtg {
 
 struct BuildParams {
   uint32_t max_leaf = 8;       // max triangles per leaf (<= 15)
-  float cost_node = 1.0f;      // SAH cost of one binary inner node per unit half area
-  float cost_tri = 1.0f;       // SAH cost of one triangle test per unit half area
+  float cost_node = 1.0f;      // SAH cost of visiting one 8-wide node, per unit half area
+  float cost_tri = 0.3f;       // SAH cost of one triangle test, per unit half area
   int quality = 1;             // RTCBuildQuality (LOW uses the same LBVH; MEDIUM/HIGH add SAH binning)
 };
 
@@ -39,6 +39,11 @@ struct DeviceBVH {
     s.tris = tris.as<TriRec>();
     s.num_nodes = stats.num_nodes;
     s.num_tris = stats.num_prims;
+    for (int k = 0; k < 3; k++) {
+      const float ext = stats.bounds_hi[k] - stats.bounds_lo[k];
+      s.lo[k] = stats.bounds_lo[k];
+      s.scale[k] = ext > 0.0f ? 1023.99f / ext : 0.0f;
+    }
     return s;
   }
   BuildStats stats;
@@ -47,7 +52,7 @@ struct DeviceBVH {
 // Scratch reused across builds on one device.
 struct BuildScratch {
   DevBuf geoms, lo4, hi4, keys, vals, slo, shi, child, parent_int, parent_leaf, range, nlo, nhi, cost,
-      isleaf, visit, queue, counters;
+      isleaf, split8, visit, queue, counters;
   RadixSorter sorter;
 };
 

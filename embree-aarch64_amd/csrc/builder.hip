@@ -255,15 +255,24 @@ __global__ __launch_bounds__(BT) void k_radix_tree(const uint32_t* __restrict__ 
 
 __device__ inline float box_half_area(float4 l, float4 h) { return half_area(h.x - l.x, h.y - l.y, h.z - l.z); }
 
-// Bottom-up bounds and SAH cost: the second child to arrive at a node
-// (agent-scope acquire/release counter) finishes it and moves up.
+// Bottom-up bounds and the SAH-optimal 8-wide collapse table. The second
+// child to arrive at a node (agent-scope acquire/release counter) finishes it
+// and moves up. For every binary node n and slot budget s = 1..8, cost8 holds
+// the least SAH cost of covering n's subtree with at most s subtrees:
+//   c(n,1) = min(leaf: cost_tri*count*A(n), node: cost_node*A(n) + d(n,8))
+//   d(n,s) = min_j c(left,j) + c(right,s-j)      (s >= 2)
+//   c(n,s) = min(c(n,s-1), d(n,s))
+// split8 records, per s, the best j (low 3 bits) and whether fewer slots win
+// (bit 7). A binary leaf costs cost_tri*A(prim) whatever the budget.
+// (Collapse formulation after Ylitie, Karras, Laine: "Efficient Incoherent
+// Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017.)
 __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __restrict__ child,
                                               const uint32_t* __restrict__ parent_int,
                                               const uint32_t* __restrict__ parent_leaf,
                                               const uint2* __restrict__ range, const float4* __restrict__ slo,
                                               const float4* __restrict__ shi, float4* nlo, float4* nhi,
-                                              float* cost, uint8_t* isleaf, uint32_t* visit, float cost_node,
-                                              float cost_tri, uint32_t max_leaf, uint32_t* err) {
+                                              float* cost8, uint8_t* split8, uint8_t* isleaf, uint32_t* visit,
+                                              float cost_node, float cost_tri, uint32_t max_leaf, uint32_t* err) {
   const uint32_t j = blockIdx.x * BT + threadIdx.x;
   if (j >= n) return;
   uint32_t node = parent_leaf[j];
@@ -275,18 +284,23 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
     const uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0) return;
     float4 bl[2], bh[2];
-    float cc[2];
+    float cc[2][8];
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       const uint32_t c = child[2 * node + k];
       if (c & REF_LEAF) {
         bl[k] = slo[c & ~REF_LEAF];
         bh[k] = shi[c & ~REF_LEAF];
-        cc[k] = cost_tri * box_half_area(bl[k], bh[k]);
+        const float ct = cost_tri * box_half_area(bl[k], bh[k]);
+#pragma unroll
+        for (int s = 0; s < 8; s++) cc[k][s] = ct;
       } else {
         bl[k] = nlo[c];
         bh[k] = nhi[c];
-        cc[k] = cost[c];
+        const float4* t = reinterpret_cast<const float4*>(cost8 + 8 * (size_t)c);
+        const float4 t0 = t[0], t1 = t[1];
+        cc[k][0] = t0.x; cc[k][1] = t0.y; cc[k][2] = t0.z; cc[k][3] = t0.w;
+        cc[k][4] = t1.x; cc[k][5] = t1.y; cc[k][6] = t1.z; cc[k][7] = t1.w;
       }
     }
     const float4 l = make_float4(fminf(bl[0].x, bl[1].x), fminf(bl[0].y, bl[1].y), fminf(bl[0].z, bl[1].z), 0.0f);
@@ -294,11 +308,39 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
     const float A = box_half_area(l, h);
     const uint2 r = range[node];
     const uint32_t count = r.y - r.x + 1;
-    const float inner = cost_node * A + cc[0] + cc[1];
+    float d[9];
+    uint32_t dj[9];
+#pragma unroll
+    for (int s = 2; s <= 8; s++) {
+      float best = INFINITY;
+      uint32_t bj = 1;
+#pragma unroll
+      for (int jj = 1; jj < s; jj++) {
+        const float v = cc[0][jj - 1] + cc[1][s - jj - 1];
+        if (v < best) { best = v; bj = (uint32_t)jj; }
+      }
+      d[s] = best;
+      dj[s] = bj;
+    }
     const float leafc = count <= max_leaf ? cost_tri * (float)count * A : INFINITY;
+    const float inner = cost_node * A + d[8];
+    float c[8];
+    uint32_t sp[8];
+    c[0] = fminf(leafc, inner);
+    sp[0] = 0;
+#pragma unroll
+    for (int s = 2; s <= 8; s++) {
+      if (d[s] < c[s - 2]) { c[s - 1] = d[s]; sp[s - 1] = dj[s]; }
+      else { c[s - 1] = c[s - 2]; sp[s - 1] = 0x80u | dj[s]; }
+    }
     nlo[node] = l;
     nhi[node] = h;
-    cost[node] = fminf(inner, leafc);
+    float4* t = reinterpret_cast<float4*>(cost8 + 8 * (size_t)node);
+    t[0] = make_float4(c[0], c[1], c[2], c[3]);
+    t[1] = make_float4(c[4], c[5], c[6], c[7]);
+    uint2* sp8 = reinterpret_cast<uint2*>(split8 + 8 * (size_t)node);
+    *sp8 = make_uint2(sp[0] | (sp[1] << 8) | (sp[2] << 16) | (sp[3] << 24),
+                      sp[4] | (sp[5] << 8) | (sp[6] << 16) | (sp[7] << 24));
     isleaf[node] = leafc <= inner ? 1 : 0;
     node = parent_int[node];
   }
@@ -339,6 +381,7 @@ struct CollapseArgs {
   const float4* nlo;
   const float4* nhi;
   const uint8_t* isleaf;
+  const uint8_t* split8;
   const float4* slo;
   const float4* shi;
   const uint32_t* vals;
@@ -389,37 +432,38 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
   const QItem item = a.qin[t];
   const float eps = o2f(a.cnt->maxabs) * BOX_EPS_REL;
 
+  // children: the optimal distribution of 8 slots below item.bref
   uint32_t ch[8];
-  int n;
-  if (is_inner(item.bref, a.isleaf)) {
-    ch[0] = a.child[2 * item.bref];
-    ch[1] = a.child[2 * item.bref + 1];
-    n = 2;
-  } else {
-    ch[0] = item.bref;
+  int n = 0;
+  if (!is_inner(item.bref, a.isleaf)) {
+    ch[0] = item.bref;  // whole (small) scene as one leaf
     n = 1;
-  }
-  // open the inner child with the largest half area until 8 children
-  while (n < 8) {
-    int best = -1;
-    float bestA = -1.0f;
-#pragma unroll
-    for (int k = 0; k < 8; k++) {
-      if (k < n && is_inner(ch[k], a.isleaf)) {
-        const float A = box_half_area(a.nlo[ch[k]], a.nhi[ch[k]]);
-        if (A > bestA) { bestA = A; best = k; }
+  } else {
+    uint32_t st_ref[8], st_s[8];
+    int sp = 0;
+    {
+      const uint32_t j8 = (a.split8[8 * (size_t)item.bref + 7]) & 7u;
+      st_ref[0] = a.child[2 * item.bref + 1]; st_s[0] = 8 - j8;
+      st_ref[1] = a.child[2 * item.bref];     st_s[1] = j8;
+      sp = 2;
+    }
+    while (sp > 0 && n < 8) {
+      sp--;
+      uint32_t c = st_ref[sp], slots = st_s[sp];
+      if (!(c & REF_LEAF)) {
+        while (slots > 1 && (a.split8[8 * (size_t)c + slots - 1] & 0x80u)) slots--;
+      } else {
+        slots = 1;
+      }
+      if (slots <= 1) {
+        ch[n++] = c;
+      } else {
+        const uint32_t jj = a.split8[8 * (size_t)c + slots - 1] & 7u;
+        if (sp + 2 > 8) { atomicOr(&a.cnt->error, 128u); return; }
+        st_ref[sp] = a.child[2 * c + 1]; st_s[sp] = slots - jj; sp++;
+        st_ref[sp] = a.child[2 * c];     st_s[sp] = jj;          sp++;
       }
     }
-    if (best < 0) break;
-    const uint32_t r = ch[best];
-    uint32_t c0 = a.child[2 * r], c1 = a.child[2 * r + 1];
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-      if (k == best) ch[k] = c0;
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-      if (k == n) ch[k] = c1;
-    n++;
   }
 
   // per-child primitive ranges live in LDS (one row per thread)
@@ -612,7 +656,8 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   S.range.reserve((size_t)ninner * 8);
   S.nlo.reserve((size_t)ninner * 16);
   S.nhi.reserve((size_t)ninner * 16);
-  S.cost.reserve((size_t)ninner * 4);
+  S.cost.reserve((size_t)ninner * 32);
+  S.split8.reserve((size_t)ninner * 8);
   S.isleaf.reserve((size_t)ninner);
   S.visit.reserve((size_t)ninner * 4);
   if (n > 1) {
@@ -623,8 +668,8 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     hipLaunchKernelGGL(k_refit, dim3(div_up(n, BT)), dim3(BT), 0, s, n, S.child.as<uint32_t>(),
                        S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(),
                        S.slo.as<float4>(), S.shi.as<float4>(), S.nlo.as<float4>(), S.nhi.as<float4>(),
-                       S.cost.as<float>(), S.isleaf.as<uint8_t>(), S.visit.as<uint32_t>(), prm.cost_node,
-                       prm.cost_tri, max_leaf, &cnt->error);
+                       S.cost.as<float>(), S.split8.as<uint8_t>(), S.isleaf.as<uint8_t>(), S.visit.as<uint32_t>(),
+                       prm.cost_node, prm.cost_tri, max_leaf, &cnt->error);
     hipLaunchKernelGGL(k_check_tree, dim3(div_up(n, BT)), dim3(BT), 0, s, S.keys.as<uint32_t>(), n,
                        S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
                        S.visit.as<uint32_t>(), &cnt->error);
@@ -646,6 +691,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   ca.nlo = S.nlo.as<float4>();
   ca.nhi = S.nhi.as<float4>();
   ca.isleaf = S.isleaf.as<uint8_t>();
+  ca.split8 = S.split8.as<uint8_t>();
   ca.slo = S.slo.as<float4>();
   ca.shi = S.shi.as<float4>();
   ca.vals = S.vals.as<uint32_t>();

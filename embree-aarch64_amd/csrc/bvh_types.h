@@ -73,6 +73,8 @@ struct DevScene {
   const TriRec* tris;
   uint32_t num_nodes;
   uint32_t num_tris;
+  float lo[3];     // scene box, for ray ordering keys
+  float scale[3];  // 1023.99 / extent per axis (0 for a flat axis)
 };
 
 // Conservative-culling constants shared with the oracle's rules.

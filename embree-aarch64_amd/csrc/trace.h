@@ -2,14 +2,33 @@
 #pragma once
 #include "bvh_types.h"
 #include "gpu_util.h"
+#include "sort.h"
 
 namespace rtg {
+
+// Launch-shape parameters of the traversal (chosen per device, tunable).
+struct TraceParams {
+  int lds_slots = 0;        // LDS stack entries per lane (0: private stack, 8 or 16)
+  int sort_rays = 1;        // trace in origin-Morton order (a permutation; records stay in place)
+  uint32_t sort_min = 65536;  // smallest stream that is reordered
+  int persistent = 1;       // persistent waves pulling rays from a counter
+  int refill_min = 16;      // idle lanes of a wave that trigger a refill
+  int cus = 256;            // compute units (set from the device)
+  int blocks_per_cu = 8;    // resident 256-thread blocks per compute unit for the persistent grid
+};
+
+// Per-thread buffers of the ray reordering.
+struct RaySorter {
+  DevBuf keys, vals, counter;
+  RadixSorter sort;
+};
 
 // Traces M ray records (RTCRayHit for closest hit, RTCRay for occlusion)
 // spaced `stride` bytes apart in device memory, in place. cnt_nodes /
 // cnt_tris (optional, M entries each) receive per-ray node visits and
 // triangle tests. err receives bit 0 on a traversal stack overflow.
 void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t inst_id,
-                  uint32_t* cnt_nodes, uint32_t* cnt_tris, uint32_t* err, hipStream_t s);
+                  uint32_t* cnt_nodes, uint32_t* cnt_tris, uint32_t* err, hipStream_t s,
+                  const TraceParams& tp, RaySorter* sorter);
 
 }  // namespace rtg

@@ -26,7 +26,55 @@
 namespace rtg {
 
 constexpr int TB = 256;
-constexpr int STACK = 256;
+constexpr int STACK = 256;  // total entries per ray (builder bounds the depth so 7*depth+1 fits)
+
+// Per-lane traversal stack: the top LDS_SLOTS entries live in LDS (slot-major,
+// so lanes touching the same slot hit distinct banks), older entries spill
+// to a private array only when the ring is full. Entry = (ref, entry distance).
+template <int LDS_SLOTS>
+struct RingStack {
+  uint2* lds;  // this lane's slot 0; slot k is lds + k * TB
+  uint2 spill[STACK];
+  int sp, lo;  // [0, lo) spilled, [lo, sp) in the ring
+  __device__ inline void init(uint2* base) {
+    lds = base;
+    sp = lo = 0;
+  }
+  __device__ inline bool push(uint2 e) {
+    if (sp >= STACK) return false;
+    if (sp - lo == LDS_SLOTS) {
+      spill[lo] = lds[(lo & (LDS_SLOTS - 1)) * TB];
+      lo++;
+    }
+    lds[(sp & (LDS_SLOTS - 1)) * TB] = e;
+    sp++;
+    return true;
+  }
+  __device__ inline uint2 pop() {
+    sp--;
+    if (sp < lo) {
+      lo = sp;
+      return spill[sp];
+    }
+    return lds[(sp & (LDS_SLOTS - 1)) * TB];
+  }
+  __device__ inline bool empty() const { return sp == 0; }
+};
+
+// LDS_SLOTS == 0: the whole stack is a private array.
+template <>
+struct RingStack<0> {
+  uint2 spill[STACK];
+  int sp;
+  __device__ inline void init(uint2*) { sp = 0; }
+  __device__ inline bool push(uint2 e) {
+    if (sp >= STACK) return false;
+    spill[sp++] = e;
+    return true;
+  }
+  __device__ inline uint2 pop() { return spill[--sp]; }
+  __device__ inline bool empty() const { return sp == 0; }
+};
 
 __device__ inline float widen_up(float x) { return __builtin_isinf(x) ? x : x + fabsf(x) * CULL_MARGIN; }
 __device__ inline float widen_down(float x) { return __builtin_isinf(x) ? x : x - fabsf(x) * CULL_MARGIN; }
@@ -53,6 +101,14 @@ __device__ inline void setup_ray(TRay& r, float4 a, float4 b, float tfar) {
   r.negy = !(r.rny >= 0.0f);
   r.negz = !(r.rnz >= 0.0f);
   r.tlo = widen_down(a.w);
+}
+
+__device__ inline uint32_t spread3(uint32_t x) {
+  x = (x | (x << 16)) & 0x030000FFu;
+  x = (x | (x << 8)) & 0x0300F00Fu;
+  x = (x | (x << 4)) & 0x030C30C3u;
+  x = (x | (x << 2)) & 0x09249249u;
+  return x;
 }
 
 __device__ inline float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)) & 0xffu); }
@@ -167,12 +223,14 @@ __device__ inline void load_ray(const char* rp, float4& a, float4& b, float& tfa
   tfar = *reinterpret_cast<const float*>(rp + 32);
 }
 
-template <bool OCCLUDED, bool ALIGNED, bool COUNT>
+template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS>
 __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                               uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
-                                              uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err) {
-  const uint32_t i = blockIdx.x * TB + threadIdx.x;
-  if (i >= M) return;
+                                              uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
+                                              const uint32_t* __restrict__ perm) {
+  const uint32_t slot = blockIdx.x * TB + threadIdx.x;
+  if (slot >= M) return;
+  const uint32_t i = perm ? perm[slot] : slot;  // lanes take rays in spatially sorted order when given
   char* rp = rays + (uint64_t)i * stride;
   float4 a, b;
   float tfar;
@@ -191,8 +249,9 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
   best.u = best.v = best.ngx = best.ngy = best.ngz = 0.0f;
   float lim = widen_up(tfar);
 
-  uint2 stack[STACK];
-  int sp = 0;
+  __shared__ uint2 lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
+  RingStack<LDS_SLOTS> stack;
+  stack.init(lds_stack + threadIdx.x);
   uint32_t cur = 0;  // root node
   bool occluded = false;
   // every node and triangle record is reached at most once per ray, so the
@@ -220,8 +279,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
           while (rem) {
             const int c = __builtin_ctz(rem);
             rem &= rem - 1u;
-            if (sp < STACK) stack[sp++] = make_uint2(child_ref(h, c), 0u);
-            else atomicOr(err, 1u);
+            if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
           }
           cur = child_ref(h, c0);
           continue;
@@ -239,8 +297,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
             for (int c = 0; c < 8; c++)
               if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
             rem &= ~(1u << mc);
-            if (sp < STACK) stack[sp++] = make_uint2(child_ref(h, mc), __float_as_uint(mv));
-            else atomicOr(err, 1u);
+            if (!stack.push(make_uint2(child_ref(h, mc), __float_as_uint(mv)))) atomicOr(err, 1u);
           }
           cur = child_ref(h, bi);
           continue;
@@ -282,8 +339,8 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
     }
     // pop the next entry that can still hold a closer hit
     bool got = false;
-    while (sp > 0) {
-      const uint2 e = stack[--sp];
+    while (!stack.empty()) {
+      const uint2 e = stack.pop();
       if (OCCLUDED || __uint_as_float(e.y) <= lim) {
         cur = e.x;
         got = true;
@@ -310,15 +367,263 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
   }
 }
 
+// Persistent variant: a fixed grid of waves pulls rays from a global counter.
+// A lane whose ray is finished takes a new one as soon as at least
+// `refill_min` lanes of its wave are idle, so incoherent rays of very
+// different lengths keep the SIMD lanes busy (Aila and Laine, "Understanding
+// the efficiency of ray traversal on GPUs", HPG 2009). Each loop iteration is
+// one traversal step (one node, or one leaf plus the pop that follows) for
+// every active lane. Every wave leaves the loop once the counter has passed
+// M and its lanes are idle.
+template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS>
+__global__ __launch_bounds__(TB) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
+                                                uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
+                                                uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
+                                                const uint32_t* __restrict__ perm, uint32_t* __restrict__ next,
+                                                int refill_min) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  __shared__ uint2 lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
+  RingStack<LDS_SLOTS> stack;
+  stack.init(lds_stack + threadIdx.x);
+
+  TRay r;
+  Hit best;
+  float lim = 0.0f;
+  uint32_t cur = 0, ray = 0, nvis = 0, ntri = 0;
+  bool active = false, occluded = false, pool_empty = false;
+  // a ray touches each node and triangle record at most once
+  const uint64_t max_iter = (uint64_t)sc.num_nodes + sc.num_tris + 16;
+  uint64_t iter = 0;
+
+  while (true) {
+    // ---- refill idle lanes (wave-uniform decision)
+    if (!pool_empty) {
+      const uint64_t idle = __ballot(!active);
+      const int nidle = __popcll(idle);
+      if (nidle >= refill_min) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(next, (uint32_t)nidle);
+        base = __shfl(base, 0, 64);
+        if ((uint64_t)base + nidle >= M) pool_empty = true;
+        if (!active) {
+          const uint32_t slot = base + (uint32_t)__popcll(idle & lt);
+          if (slot < M) {
+            ray = perm ? perm[slot] : slot;
+            float4 a, b;
+            float tfar;
+            load_ray<ALIGNED>(rays + (uint64_t)ray * stride, a, b, tfar);
+            if (a.w <= tfar) {
+              setup_ray(r, a, b, tfar);
+              best.found = false;
+              best.t = INFINITY;
+              best.prim = best.geom = 0xffffffffu;
+              best.u = best.v = best.ngx = best.ngy = best.ngz = 0.0f;
+              lim = widen_up(tfar);
+              cur = 0;
+              stack.init(lds_stack + threadIdx.x);
+              occluded = false;
+              nvis = ntri = 0;
+              iter = 0;
+              active = true;
+            } else if (COUNT) {
+              cnt_nodes[ray] = 0;
+              cnt_tris[ray] = 0;
+            }
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (pool_empty) break;
+      continue;
+    }
+    if (!active) continue;
+
+    // ---- one traversal step
+    bool done = false;
+    if (++iter > max_iter) {
+      atomicOr(err, 2u);
+      done = true;
+    } else if (!(cur & REF_LEAF)) {
+      if (cur >= sc.num_nodes) {
+        atomicOr(err, 4u);
+        done = true;
+      } else {
+        NodeHit h;
+        intersect_node(sc.nodes + cur, r, lim, h);
+        if (COUNT) nvis++;
+        if (h.mask) {
+          if (OCCLUDED) {
+            const int c0 = __builtin_ctz(h.mask);
+            uint32_t rem = h.mask & (h.mask - 1u);
+            while (rem) {
+              const int c = __builtin_ctz(rem);
+              rem &= rem - 1u;
+              if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
+            }
+            cur = child_ref(h, c0);
+          } else {
+            int bi = -1;
+            float bv = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+              if (((h.mask >> c) & 1u) && (bi < 0 || h.tn[c] < bv)) { bi = c; bv = h.tn[c]; }
+            uint32_t rem = h.mask & ~(1u << bi);
+            while (rem) {
+              int mc = -1;
+              float mv = 0.0f;
+#pragma unroll
+              for (int c = 0; c < 8; c++)
+                if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
+              rem &= ~(1u << mc);
+              if (!stack.push(make_uint2(child_ref(h, mc), __float_as_uint(mv)))) atomicOr(err, 1u);
+            }
+            cur = child_ref(h, bi);
+          }
+          continue;  // next step descends into cur
+        }
+      }
+    } else {
+      uint32_t ti = cur & ~REF_LEAF;
+      while (true) {
+        if (ti >= sc.num_tris) {
+          atomicOr(err, 4u);
+          done = true;
+          break;
+        }
+        float T, absDen, U, V, ngx, ngy, ngz;
+        uint32_t prim, geom, flags;
+        const bool valid = moeller(r, sc.tris + ti, T, absDen, U, V, ngx, ngy, ngz, prim, geom, flags);
+        if (COUNT) ntri++;
+        if (valid) {
+          if (OCCLUDED) {
+            occluded = true;
+            break;
+          }
+          const float t = T / absDen;
+          const bool better = !best.found || t < best.t ||
+                              (t == best.t && (geom < best.geom || (geom == best.geom && prim < best.prim)));
+          if (better) {
+            best.found = true;
+            best.t = t;
+            best.u = U / absDen;
+            best.v = V / absDen;
+            best.ngx = ngx; best.ngy = ngy; best.ngz = ngz;
+            best.prim = prim; best.geom = geom;
+          }
+        }
+        if (flags & TRI_LAST) break;
+        ti++;
+      }
+      if (OCCLUDED && occluded) done = true;
+      if (!OCCLUDED && best.found) lim = widen_up(fminf(best.t, r.tfar));
+    }
+    if (!done) {
+      // pop the next entry that can still hold a closer hit
+      bool got = false;
+      while (!stack.empty()) {
+        const uint2 e = stack.pop();
+        if (OCCLUDED || __uint_as_float(e.y) <= lim) {
+          cur = e.x;
+          got = true;
+          break;
+        }
+      }
+      done = !got;
+    }
+    if (done) {
+      char* rp = rays + (uint64_t)ray * stride;
+      if (COUNT) { cnt_nodes[ray] = nvis; cnt_tris[ray] = ntri; }
+      if (OCCLUDED) {
+        if (occluded) *reinterpret_cast<float*>(rp + 32) = -INFINITY;
+      } else if (best.found) {
+        float* f = reinterpret_cast<float*>(rp);
+        uint32_t* u = reinterpret_cast<uint32_t*>(rp);
+        f[8] = best.t;
+        f[12] = best.ngx;
+        f[13] = best.ngy;
+        f[14] = best.ngz;
+        f[15] = best.u;
+        f[16] = best.v;
+        u[17] = best.prim;
+        u[18] = best.geom;
+        u[19] = inst_id;
+      }
+      active = false;
+    }
+  }
+}
+
+__global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
+
+// Sort key of a ray: 30-bit Morton code of its origin in the scene box
+// (origins outside are clamped to the box faces); inactive rays sort last.
+__global__ __launch_bounds__(TB) void k_ray_keys(const char* __restrict__ rays, uint32_t M, uint64_t stride, float lx,
+                                                 float ly, float lz, float sx, float sy, float sz,
+                                                 uint32_t* __restrict__ keys, uint32_t* __restrict__ vals) {
+  const uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i >= M) return;
+  const float* f = reinterpret_cast<const float*>(rays + (uint64_t)i * stride);
+  const float ox = f[0], oy = f[1], oz = f[2], tnear = f[3], tfar = f[8];
+  uint32_t key = 0xffffffffu;
+  if (tnear <= tfar) {
+    const uint32_t qx = (uint32_t)fminf(fmaxf((ox - lx) * sx, 0.0f), 1023.0f);
+    const uint32_t qy = (uint32_t)fminf(fmaxf((oy - ly) * sy, 0.0f), 1023.0f);
+    const uint32_t qz = (uint32_t)fminf(fmaxf((oz - lz) * sz, 0.0f), 1023.0f);
+    key = spread3(qx) | (spread3(qy) << 1) | (spread3(qz) << 2);
+  }
+  keys[i] = key;
+  vals[i] = i;
+}
+
 void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t inst_id,
-                  uint32_t* cnt_nodes, uint32_t* cnt_tris, uint32_t* err, hipStream_t s) {
+                  uint32_t* cnt_nodes, uint32_t* cnt_tris, uint32_t* err, hipStream_t s, const TraceParams& tp,
+                  RaySorter* sorter) {
   if (M == 0 || sc.num_nodes == 0) return;
   const bool aligned = ((uintptr_t)rays % 16 == 0) && (stride % 16 == 0);
   const dim3 grid(div_up(M, TB)), block(TB);
   char* rp = (char*)rays;
   const bool count = cnt_nodes != nullptr;
-#define RTG_LAUNCH(OCC, AL, CNT) \
-  hipLaunchKernelGGL((k_trace<OCC, AL, CNT>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes, cnt_tris, err)
+  const uint32_t* perm = nullptr;
+  if (tp.sort_rays && sorter && M >= tp.sort_min) {
+    sorter->keys.reserve((size_t)M * 4);
+    sorter->vals.reserve((size_t)M * 4);
+    hipLaunchKernelGGL(k_ray_keys, grid, block, 0, s, rp, M, stride, sc.lo[0], sc.lo[1], sc.lo[2], sc.scale[0],
+                       sc.scale[1], sc.scale[2], sorter->keys.as<uint32_t>(), sorter->vals.as<uint32_t>());
+    sorter->sort.sort_pairs(sorter->keys.as<uint32_t>(), sorter->vals.as<uint32_t>(), M, 0, 32, s);
+    perm = sorter->vals.as<uint32_t>();
+  }
+  const bool persistent = tp.persistent && sorter;
+  const int refill = tp.refill_min < 1 ? 1 : (tp.refill_min > 64 ? 64 : tp.refill_min);  // 64 idle lanes always refill
+  dim3 pgrid(1);
+  if (persistent) {
+    sorter->counter.reserve(256);
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, sorter->counter.as<uint32_t>());
+    const uint32_t want = (uint32_t)tp.cus * (uint32_t)tp.blocks_per_cu;
+    pgrid = dim3(want < grid.x ? want : grid.x);
+  }
+#define RTG_LAUNCH_P(OCC, AL, CNT, SL)                                                                   \
+  hipLaunchKernelGGL((k_trace_p<OCC, AL, CNT, SL>), pgrid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,      \
+                     cnt_tris, err, perm, sorter->counter.as<uint32_t>(), refill)
+#define RTG_LAUNCH(OCC, AL, CNT)                                                                                 \
+  do {                                                                                                          \
+    if (persistent) {                                                                                           \
+      if (tp.lds_slots <= 0) RTG_LAUNCH_P(OCC, AL, CNT, 0);                                                     \
+      else if (tp.lds_slots <= 8) RTG_LAUNCH_P(OCC, AL, CNT, 8);                                                \
+      else RTG_LAUNCH_P(OCC, AL, CNT, 16);                                                                      \
+      break;                                                                                                    \
+    }                                                                                                           \
+    if (tp.lds_slots <= 0)                                                                                      \
+      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 0>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,   \
+                         cnt_tris, err, perm);                                                                  \
+    else if (tp.lds_slots <= 8)                                                                                 \
+      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 8>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,   \
+                         cnt_tris, err, perm);                                                                  \
+    else                                                                                                        \
+      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 16>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,  \
+                         cnt_tris, err, perm);                                                                  \
+  } while (0)
   if (count) {
     if (occluded) RTG_LAUNCH(true, false, true); else RTG_LAUNCH(false, false, true);
   } else if (occluded) {
@@ -327,6 +632,7 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
     if (aligned) RTG_LAUNCH(false, true, false); else RTG_LAUNCH(false, false, false);
   }
 #undef RTG_LAUNCH
+#undef RTG_LAUNCH_P
   RTG_HIP(hipGetLastError());
 }
 
