@@ -14,7 +14,7 @@ namespace rtg {
 struct BuildParams {
   uint32_t max_leaf = 8;       // max triangles per leaf (<= 15)
   float cost_node = 1.0f;      // SAH cost of visiting one 8-wide node, per unit half area
-  float cost_tri = 0.3f;       // SAH cost of one triangle test, per unit half area
+  float cost_tri = 0.45f;      // SAH cost of one triangle test, per unit half area
   int quality = 1;             // RTCBuildQuality (LOW uses the same LBVH; MEDIUM/HIGH add SAH binning)
 };
 

@@ -8,11 +8,13 @@ namespace rtg {
 
 // Launch-shape parameters of the traversal (chosen per device, tunable).
 struct TraceParams {
-  int lds_slots = 0;        // LDS stack entries per lane (0: private stack, 8 or 16)
-  int sort_rays = 1;        // trace in origin-Morton order (a permutation; records stay in place)
+  int lds_slots = 8;        // LDS stack entries per lane (0: private stack, 8 or 16)
+  int sort_rays = 0;        // trace in origin-Morton order (a permutation; records stay in place)
   uint32_t sort_min = 65536;  // smallest stream that is reordered
   int persistent = 1;       // persistent waves pulling rays from a counter
-  int refill_min = 16;      // idle lanes of a wave that trigger a refill
+  int refill_min = 16;      // idle lanes of a wave that trigger a refill (closest hit)
+  int refill_min_occ = 32;  // same for occlusion rays, which finish sooner
+  int fused_slab = 0;       // slab distances as one FMA per plane (else decode the plane, then subtract and scale)
   int cus = 256;            // compute units (set from the device)
   int blocks_per_cu = 8;    // resident 256-thread blocks per compute unit for the persistent grid
 };

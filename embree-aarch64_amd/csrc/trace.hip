@@ -124,6 +124,7 @@ struct NodeHit {
 
 // Slab test of the 8 children; decoded planes are fmaf(2^(e-127), q, p), the
 // exact expression the builder verified against every child box.
+template <bool FUSED>
 __device__ inline void intersect_node(const QNode8* __restrict__ node, const TRay& r, float thi, NodeHit& h) {
   const uint4* np = reinterpret_cast<const uint4*>(node);
   const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
@@ -143,6 +144,24 @@ __device__ inline void intersect_node(const QNode8* __restrict__ node, const TRa
   const uint32_t fy0 = r.negy ? w3.x : w3.z, fy1 = r.negy ? w3.y : w3.w;
   const uint32_t nz0 = r.negz ? w4.z : w4.x, nz1 = r.negz ? w4.w : w4.y;
   const uint32_t fz0 = r.negz ? w4.x : w4.z, fz1 = r.negz ? w4.y : w4.w;
+  // Slab distances as one FMA per plane: t = q * (s * rd) + (p - o) * rd.
+  // s is a power of two, so s * rd is exact; the constant term carries at
+  // most two roundings and the FMA one more, all bounded by
+  // 2^-22 * (|a| + 255 |b|); twice that is subtracted from near and added to
+  // far distances. The decoded planes of the builder lie outside the child boxes
+  // by the box widening, which covers the half-ulp between the plane's exact
+  // value p + s*q and its rounded decode.
+  const float ax = FUSED ? px - r.ox : 0.0f, ay = FUSED ? py - r.oy : 0.0f, az = FUSED ? pz - r.oz : 0.0f;
+  const float anx = ax * r.rnx, bnx = sx * r.rnx, afx = ax * r.rfx, bfx = sx * r.rfx;
+  const float any = ay * r.rny, bny = sy * r.rny, afy = ay * r.rfy, bfy = sy * r.rfy;
+  const float anz = az * r.rnz, bnz = sz * r.rnz, afz = az * r.rfz, bfz = sz * r.rfz;
+  const float kerr = 4.76837158203125e-7f;  // 2^-21: twice the bound below
+  const float cnx = anx - kerr * (fabsf(anx) + 255.0f * fabsf(bnx));
+  const float cfx = afx + kerr * (fabsf(afx) + 255.0f * fabsf(bfx));
+  const float cny = any - kerr * (fabsf(any) + 255.0f * fabsf(bny));
+  const float cfy = afy + kerr * (fabsf(afy) + 255.0f * fabsf(bfy));
+  const float cnz = anz - kerr * (fabsf(anz) + 255.0f * fabsf(bnz));
+  const float cfz = afz + kerr * (fabsf(afz) + 255.0f * fabsf(bfz));
   uint32_t mask = 0;
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -151,12 +170,22 @@ __device__ inline void intersect_node(const QNode8* __restrict__ node, const TRa
     const float qnx = byte_f(c < 4 ? nx0 : nx1, k), qfx = byte_f(c < 4 ? fx0 : fx1, k);
     const float qny = byte_f(c < 4 ? ny0 : ny1, k), qfy = byte_f(c < 4 ? fy0 : fy1, k);
     const float qnz = byte_f(c < 4 ? nz0 : nz1, k), qfz = byte_f(c < 4 ? fz0 : fz1, k);
-    const float tnx = (__builtin_fmaf(sx, qnx, px) - r.ox) * r.rnx;
-    const float tfx = (__builtin_fmaf(sx, qfx, px) - r.ox) * r.rfx;
-    const float tny = (__builtin_fmaf(sy, qny, py) - r.oy) * r.rny;
-    const float tfy = (__builtin_fmaf(sy, qfy, py) - r.oy) * r.rfy;
-    const float tnz = (__builtin_fmaf(sz, qnz, pz) - r.oz) * r.rnz;
-    const float tfz = (__builtin_fmaf(sz, qfz, pz) - r.oz) * r.rfz;
+    float tnx, tfx, tny, tfy, tnz, tfz;
+    if (FUSED) {
+      tnx = __builtin_fmaf(qnx, bnx, cnx);
+      tfx = __builtin_fmaf(qfx, bfx, cfx);
+      tny = __builtin_fmaf(qny, bny, cny);
+      tfy = __builtin_fmaf(qfy, bfy, cfy);
+      tnz = __builtin_fmaf(qnz, bnz, cnz);
+      tfz = __builtin_fmaf(qfz, bfz, cfz);
+    } else {
+      tnx = (__builtin_fmaf(sx, qnx, px) - r.ox) * r.rnx;
+      tfx = (__builtin_fmaf(sx, qfx, px) - r.ox) * r.rfx;
+      tny = (__builtin_fmaf(sy, qny, py) - r.oy) * r.rny;
+      tfy = (__builtin_fmaf(sy, qfy, py) - r.oy) * r.rfy;
+      tnz = (__builtin_fmaf(sz, qnz, pz) - r.oz) * r.rnz;
+      tfz = (__builtin_fmaf(sz, qfz, pz) - r.oz) * r.rfz;
+    }
     const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tlo));
     const float tf = fminf(fminf(tfx, tfy), fminf(tfz, thi));
     h.tn[c] = tn;
@@ -223,7 +252,7 @@ __device__ inline void load_ray(const char* rp, float4& a, float4& b, float& tfa
   tfar = *reinterpret_cast<const float*>(rp + 32);
 }
 
-template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS>
+template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool FUSED>
 __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                               uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                               uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
@@ -270,7 +299,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
         break;
       }
       NodeHit h;
-      intersect_node(sc.nodes + cur, r, lim, h);
+      intersect_node<FUSED>(sc.nodes + cur, r, lim, h);
       if (COUNT) nvis++;
       if (h.mask) {
         if (OCCLUDED) {
@@ -375,7 +404,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
 // one traversal step (one node, or one leaf plus the pop that follows) for
 // every active lane. Every wave leaves the loop once the counter has passed
 // M and its lanes are idle.
-template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS>
+template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool FUSED>
 __global__ __launch_bounds__(TB) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                                 uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                                 uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
@@ -451,7 +480,7 @@ __global__ __launch_bounds__(TB) void k_trace_p(DevScene sc, char* __restrict__ 
         done = true;
       } else {
         NodeHit h;
-        intersect_node(sc.nodes + cur, r, lim, h);
+        intersect_node<FUSED>(sc.nodes + cur, r, lim, h);
         if (COUNT) nvis++;
         if (h.mask) {
           if (OCCLUDED) {
@@ -595,7 +624,8 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
     perm = sorter->vals.as<uint32_t>();
   }
   const bool persistent = tp.persistent && sorter;
-  const int refill = tp.refill_min < 1 ? 1 : (tp.refill_min > 64 ? 64 : tp.refill_min);  // 64 idle lanes always refill
+  const int rmin = occluded ? tp.refill_min_occ : tp.refill_min;
+  const int refill = rmin < 1 ? 1 : (rmin > 64 ? 64 : rmin);  // 64 idle lanes always refill
   dim3 pgrid(1);
   if (persistent) {
     sorter->counter.reserve(256);
@@ -603,9 +633,13 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
     const uint32_t want = (uint32_t)tp.cus * (uint32_t)tp.blocks_per_cu;
     pgrid = dim3(want < grid.x ? want : grid.x);
   }
-#define RTG_LAUNCH_P(OCC, AL, CNT, SL)                                                                   \
-  hipLaunchKernelGGL((k_trace_p<OCC, AL, CNT, SL>), pgrid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,      \
+#define RTG_LAUNCH_P2(OCC, AL, CNT, SL, FU)                                                              \
+  hipLaunchKernelGGL((k_trace_p<OCC, AL, CNT, SL, FU>), pgrid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,  \
                      cnt_tris, err, perm, sorter->counter.as<uint32_t>(), refill)
+#define RTG_LAUNCH_P(OCC, AL, CNT, SL)                                                                   \
+  do {                                                                                                    \
+    if (tp.fused_slab) RTG_LAUNCH_P2(OCC, AL, CNT, SL, true); else RTG_LAUNCH_P2(OCC, AL, CNT, SL, false); \
+  } while (0)
 #define RTG_LAUNCH(OCC, AL, CNT)                                                                                 \
   do {                                                                                                          \
     if (persistent) {                                                                                           \
@@ -615,13 +649,13 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
       break;                                                                                                    \
     }                                                                                                           \
     if (tp.lds_slots <= 0)                                                                                      \
-      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 0>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,   \
+      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 0, true>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,   \
                          cnt_tris, err, perm);                                                                  \
     else if (tp.lds_slots <= 8)                                                                                 \
-      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 8>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,   \
+      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 8, true>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,   \
                          cnt_tris, err, perm);                                                                  \
     else                                                                                                        \
-      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 16>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,  \
+      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 16, true>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,  \
                          cnt_tris, err, perm);                                                                  \
   } while (0)
   if (count) {
@@ -633,6 +667,7 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
   }
 #undef RTG_LAUNCH
 #undef RTG_LAUNCH_P
+#undef RTG_LAUNCH_P2
   RTG_HIP(hipGetLastError());
 }
 
