@@ -584,6 +584,212 @@ __global__ __launch_bounds__(TB) void k_trace_p(DevScene sc, char* __restrict__ 
   }
 }
 
+constexpr uint32_t NO_REF = 0xffffffffu;
+
+template <int LDS_SLOTS>
+__device__ inline uint32_t pop_live(RingStack<LDS_SLOTS>& stack, float lim, bool occluded) {
+  while (!stack.empty()) {
+    const uint2 e = stack.pop();
+    if (occluded || __uint_as_float(e.y) <= lim) return e.x;
+  }
+  return NO_REF;
+}
+
+// Persistent while-while traversal with one postponed leaf per lane (Aila and
+// Laine, HPG 2009): in the node phase every lane descends; a lane that meets a
+// leaf parks it and keeps descending speculatively until it meets a second
+// leaf or until no lane of the wave is still searching for its first leaf;
+// then all lanes test their parked leaves together. Node-test and
+// triangle-test code thereby run with most lanes of the wave on the same
+// path. The best hit keeps only t, ids and the triangle record; u, v and Ng
+// are recomputed from that record when the ray finishes (same arithmetic,
+// same result). Rays are refilled between phases as in k_trace_p; every wave
+// exits once the counter has passed M and its lanes are idle.
+template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS>
+__global__ __launch_bounds__(TB) void k_trace_ww(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
+                                                 uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
+                                                 uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
+                                                 const uint32_t* __restrict__ perm, uint32_t* __restrict__ next,
+                                                 int refill_min) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  __shared__ uint2 lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
+  RingStack<LDS_SLOTS> stack;
+  stack.init(lds_stack + threadIdx.x);
+
+  TRay r;
+  float lim = 0.0f, best_t = INFINITY;
+  uint32_t best_tri = NO_REF, best_prim = NO_REF, best_geom = NO_REF;
+  uint32_t cur = NO_REF, pleaf = NO_REF, ray = 0, nvis = 0, ntri = 0;
+  bool active = false, pool_empty = false, occluded = false, failed = false;
+  // a ray touches each node and triangle record at most once
+  const uint64_t max_work = (uint64_t)sc.num_nodes + sc.num_tris + 16;
+  uint64_t work = 0;
+
+  while (true) {
+    // ---- refill idle lanes (wave-uniform decision)
+    if (!pool_empty) {
+      const uint64_t idle = __ballot(!active);
+      const int nidle = __popcll(idle);
+      if (nidle >= refill_min) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(next, (uint32_t)nidle);
+        base = __shfl(base, 0, 64);
+        if ((uint64_t)base + nidle >= M) pool_empty = true;
+        if (!active) {
+          const uint32_t slot = base + (uint32_t)__popcll(idle & lt);
+          if (slot < M) {
+            ray = perm ? perm[slot] : slot;
+            float4 a, b;
+            float tfar;
+            load_ray<ALIGNED>(rays + (uint64_t)ray * stride, a, b, tfar);
+            if (a.w <= tfar) {
+              setup_ray(r, a, b, tfar);
+              best_t = INFINITY;
+              best_tri = best_prim = best_geom = NO_REF;
+              lim = widen_up(tfar);
+              cur = 0;
+              pleaf = NO_REF;
+              stack.init(lds_stack + threadIdx.x);
+              occluded = failed = false;
+              nvis = ntri = 0;
+              work = 0;
+              active = true;
+            } else if (COUNT) {
+              cnt_nodes[ray] = 0;
+              cnt_tris[ray] = 0;
+            }
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (pool_empty) break;
+      continue;
+    }
+
+    // ---- node phase
+    while (true) {
+      bool nodework = false;
+      if (active && !failed) {
+        if (cur == NO_REF) cur = pop_live(stack, lim, OCCLUDED);
+        if (cur != NO_REF && (cur & REF_LEAF) && pleaf == NO_REF) {
+          pleaf = cur;
+          cur = pop_live(stack, lim, OCCLUDED);
+        }
+        nodework = cur != NO_REF && !(cur & REF_LEAF);
+      }
+      if (__ballot(nodework && pleaf == NO_REF) == 0) break;  // every searching lane holds a leaf
+      if (nodework) {
+        if (++work > max_work || cur >= sc.num_nodes) {
+          atomicOr(err, cur >= sc.num_nodes ? 4u : 2u);
+          failed = true;
+        } else {
+          NodeHit h;
+          intersect_node<false>(sc.nodes + cur, r, lim, h);
+          if (COUNT) nvis++;
+          cur = NO_REF;
+          if (h.mask) {
+            if (OCCLUDED) {
+              const int c0 = __builtin_ctz(h.mask);
+              uint32_t rem = h.mask & (h.mask - 1u);
+              while (rem) {
+                const int c = __builtin_ctz(rem);
+                rem &= rem - 1u;
+                if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
+              }
+              cur = child_ref(h, c0);
+            } else {
+              int bi = -1;
+              float bv = 0.0f;
+#pragma unroll
+              for (int c = 0; c < 8; c++)
+                if (((h.mask >> c) & 1u) && (bi < 0 || h.tn[c] < bv)) { bi = c; bv = h.tn[c]; }
+              uint32_t rem = h.mask & ~(1u << bi);
+              while (rem) {
+                int mc = -1;
+                float mv = 0.0f;
+#pragma unroll
+                for (int c = 0; c < 8; c++)
+                  if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
+                rem &= ~(1u << mc);
+                if (!stack.push(make_uint2(child_ref(h, mc), __float_as_uint(mv)))) atomicOr(err, 1u);
+              }
+              cur = child_ref(h, bi);
+            }
+          }
+        }
+      }
+    }
+
+    // ---- leaf phase
+    if (active && !failed && pleaf != NO_REF) {
+      uint32_t ti = pleaf & ~REF_LEAF;
+      pleaf = NO_REF;
+      bool improved = false;
+      while (true) {
+        if (ti >= sc.num_tris || ++work > max_work) {
+          atomicOr(err, ti >= sc.num_tris ? 4u : 2u);
+          failed = true;
+          break;
+        }
+        float T, absDen, U, V, ngx, ngy, ngz;
+        uint32_t prim, geom, flags;
+        const bool valid = moeller(r, sc.tris + ti, T, absDen, U, V, ngx, ngy, ngz, prim, geom, flags);
+        if (COUNT) ntri++;
+        if (valid) {
+          if (OCCLUDED) {
+            occluded = true;
+            break;
+          }
+          const float t = T / absDen;
+          const bool better = best_tri == NO_REF || t < best_t ||
+                              (t == best_t && (geom < best_geom || (geom == best_geom && prim < best_prim)));
+          if (better) {
+            best_t = t;
+            best_tri = ti;
+            best_prim = prim;
+            best_geom = geom;
+            improved = true;
+          }
+        }
+        if (flags & TRI_LAST) break;
+        ti++;
+      }
+      if (!OCCLUDED && improved) lim = widen_up(fminf(best_t, r.tfar));
+    }
+
+    // ---- rays that are finished write their record and free the lane
+    if (active) {
+      const bool finished = failed || (OCCLUDED && occluded) ||
+                            (pleaf == NO_REF && cur == NO_REF && stack.empty());
+      if (finished) {
+        char* rp = rays + (uint64_t)ray * stride;
+        if (COUNT) { cnt_nodes[ray] = nvis; cnt_tris[ray] = ntri; }
+        if (OCCLUDED) {
+          if (occluded) *reinterpret_cast<float*>(rp + 32) = -INFINITY;
+        } else if (best_tri != NO_REF && !failed) {
+          float T, absDen, U, V, ngx, ngy, ngz;
+          uint32_t prim, geom, flags;
+          moeller(r, sc.tris + best_tri, T, absDen, U, V, ngx, ngy, ngz, prim, geom, flags);
+          float* f = reinterpret_cast<float*>(rp);
+          uint32_t* u = reinterpret_cast<uint32_t*>(rp);
+          f[8] = best_t;
+          f[12] = ngx;
+          f[13] = ngy;
+          f[14] = ngz;
+          f[15] = U / absDen;
+          f[16] = V / absDen;
+          u[17] = prim;
+          u[18] = geom;
+          u[19] = inst_id;
+        }
+        active = false;
+      }
+    }
+  }
+}
+
 __global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
 
 // Sort key of a ray: 30-bit Morton code of its origin in the scene box
@@ -638,7 +844,11 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
                      cnt_tris, err, perm, sorter->counter.as<uint32_t>(), refill)
 #define RTG_LAUNCH_P(OCC, AL, CNT, SL)                                                                   \
   do {                                                                                                    \
-    if (tp.fused_slab) RTG_LAUNCH_P2(OCC, AL, CNT, SL, true); else RTG_LAUNCH_P2(OCC, AL, CNT, SL, false); \
+    if (tp.while_while)                                                                                   \
+      hipLaunchKernelGGL((k_trace_ww<OCC, AL, CNT, SL>), pgrid, block, 0, s, sc, rp, M, stride, inst_id,   \
+                         cnt_nodes, cnt_tris, err, perm, sorter->counter.as<uint32_t>(), refill);         \
+    else if (tp.fused_slab) RTG_LAUNCH_P2(OCC, AL, CNT, SL, true);                                        \
+    else RTG_LAUNCH_P2(OCC, AL, CNT, SL, false);                                                          \
   } while (0)
 #define RTG_LAUNCH(OCC, AL, CNT)                                                                                 \
   do {                                                                                                          \
