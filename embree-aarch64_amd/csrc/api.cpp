@@ -229,6 +229,7 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "bvh_leaf" || k == "max_leaf") d->build.max_leaf = (uint32_t)atoi(v.c_str());
     else if (k == "cost_node") d->build.cost_node = (float)atof(v.c_str());
     else if (k == "cost_tri") d->build.cost_tri = (float)atof(v.c_str());
+    else if (k == "builder") d->build.builder = (v == "sah") ? rtg::BUILDER_SAH : rtg::BUILDER_MORTON;
     else if (k == "trace_stack") d->trace.lds_slots = atoi(v.c_str());
     else if (k == "ray_sort") d->trace.sort_rays = atoi(v.c_str());
     else if (k == "ray_sort_min") d->trace.sort_min = (uint32_t)atol(v.c_str());
@@ -240,6 +241,16 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "blocks_per_cu") d->trace.blocks_per_cu = atoi(v.c_str());
     // threads, isa, tri_accel, tri_builder, set_affinity, ... are CPU settings: accepted, ignored
   }
+}
+
+// Build settings for a scene: LOW asks for the Morton radix tree, HIGH for
+// the binned-SAH tree, MEDIUM for the device's configured builder.
+rtg::BuildParams build_params(const Device* d, RTCBuildQuality q) {
+  rtg::BuildParams p = d->build;
+  p.quality = q;
+  if (q == RTC_BUILD_QUALITY_LOW) p.builder = rtg::BUILDER_MORTON;
+  else if (q == RTC_BUILD_QUALITY_HIGH) p.builder = rtg::BUILDER_SAH;
+  return p;
 }
 
 Scene* SC(RTCScene s) { return (Scene*)s; }
@@ -332,13 +343,11 @@ void commit_scene(Scene* sc) {
   }
   {
     std::lock_guard<std::mutex> bl(d->build_mtx);
-    rtg::BuildParams p = d->build;
-    p.quality = sc->quality;
+    const rtg::BuildParams p = build_params(d, sc->quality);
     d->monitor((ssize_t)prim_offset * (80 + 48));
     rtg::build_bvh(sc->descs, p, d->scratch, sc->bvh, t.stream);
   }
   RTG_HIP(hipStreamSynchronize(t.stream));
-  if (sc->bvh.stats.max_depth > 36) RTC_THROW(RTC_ERROR_UNKNOWN, "BVH deeper than the traversal stack supports");
   sc->committed = true;
   if (sc->progress && !sc->progress(sc->progress_ptr, 1.0)) RTC_THROW(RTC_ERROR_CANCELLED, "progress monitor forced termination");
 }
@@ -1075,8 +1084,7 @@ RTC_API void rtcgTimeBuild(RTCScene scene, int steps, float* ms_out) {
     std::lock_guard<std::mutex> bl(d->build_mtx);
     RTG_HIP(hipStreamSynchronize(t.stream));
     auto t0 = std::chrono::steady_clock::now();
-    rtg::BuildParams p = d->build;
-    p.quality = s->quality;
+    const rtg::BuildParams p = build_params(d, s->quality);
     rtg::build_bvh(s->descs, p, d->scratch, s->bvh, t.stream);
     RTG_HIP(hipStreamSynchronize(t.stream));
     if (ms_out) ms_out[i] = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1118,6 +1126,7 @@ RTC_API void rtcgGetSceneStats(RTCScene scene, RTCGSceneStats* out) {
   out->max_depth = st.max_depth;
   out->node_bytes = sizeof(rtg::QNode8);
   out->tri_bytes = sizeof(rtg::TriRec);
+  out->max_stack = st.max_stack;
   out->sah = st.sah;
   out->sah_tri = st.sah_tri;
   out->build_ms = st.ms_total;

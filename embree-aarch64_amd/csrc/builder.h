@@ -5,13 +5,17 @@
 #pragma once
 #include <vector>
 
+#include "builder_sah.h"
 #include "bvh_types.h"
 #include "gpu_util.h"
 #include "sort.h"
 
 namespace rtg {
 
+enum BuilderKind { BUILDER_MORTON = 0, BUILDER_SAH = 1 };
+
 struct BuildParams {
+  int builder = BUILDER_MORTON;  // binary tree: Morton radix tree, or top-down binned SAH
   uint32_t max_leaf = 8;       // max triangles per leaf (<= 15)
   float cost_node = 1.0f;      // SAH cost of visiting one 8-wide node, per unit half area
   float cost_tri = 0.45f;      // SAH cost of one triangle test, per unit half area
@@ -22,6 +26,7 @@ struct BuildStats {
   uint32_t num_prims = 0;      // valid triangles
   uint32_t num_nodes = 0;
   uint32_t max_depth = 0;
+  uint32_t max_stack = 0;      // traversal stack entries the deepest path can need
   float bounds_lo[3] = {0, 0, 0};
   float bounds_hi[3] = {0, 0, 0};
   double sah = 0;              // reference SAH statistic (leaf cost per block of 4)
@@ -54,6 +59,7 @@ struct BuildScratch {
   DevBuf geoms, lo4, hi4, keys, vals, slo, shi, child, parent_int, parent_leaf, range, nlo, nhi, cost,
       isleaf, split8, visit, queue, counters;
   RadixSorter sorter;
+  SahScratch sah;
 };
 
 // Builds over the given geometries (device pointers in GeomDesc).

@@ -25,7 +25,8 @@ struct BuildCounters {
   uint32_t qcount[2];
   uint32_t max_depth;
   uint32_t error;  // bit 0: refit walk too long, bit 1: collapse index out of range, bit 2: bad tree
-  uint32_t pad[2];
+  uint32_t max_stack;  // most traversal stack entries any root-to-node path can need
+  uint32_t pad;
   double sah_nodes, sah_leaves, sah_leaves_tri;
 };
 
@@ -36,6 +37,7 @@ struct QItem {
   uint32_t bref;  // binary ref: REF_LEAF | sorted prim index, or inner node index
   uint32_t node;  // QNode8 slot
   uint32_t depth;
+  uint32_t need;  // stack entries the traversal may hold on reaching this node
 };
 
 // ------------------------------------------------------------ reductions --
@@ -69,6 +71,7 @@ __global__ void k_init_counters(BuildCounters* c) {
   c->tri_count = 0;
   c->qcount[0] = c->qcount[1] = 0;
   c->max_depth = 0;
+  c->max_stack = 0;
   c->error = 0;
   c->sah_nodes = c->sah_leaves = c->sah_leaves_tri = 0.0;
 }
@@ -288,6 +291,10 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       const uint32_t c = child[2 * node + k];
+      if ((c & REF_LEAF) ? (c & ~REF_LEAF) >= n : c + 1 >= n) {
+        atomicOr(err, 1u);
+        return;
+      }
       if (c & REF_LEAF) {
         bl[k] = slo[c & ~REF_LEAF];
         bh[k] = shi[c & ~REF_LEAF];
@@ -349,20 +356,27 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
 // Integrity check of the sorted keys and the radix tree (cheap, one pass):
 // bit 3 keys out of order, bit 4 inner node not finished by the refit,
 // bit 5 parent/child links disagree.
-__global__ __launch_bounds__(BT) void k_check_tree(const uint32_t* __restrict__ keys, uint32_t n,
+__global__ __launch_bounds__(BT) void k_check_tree(const uint32_t* __restrict__ keys, uint32_t n, uint32_t root,
                                                    const uint32_t* __restrict__ child,
                                                    const uint32_t* __restrict__ parent_int,
                                                    const uint32_t* __restrict__ parent_leaf,
                                                    const uint32_t* __restrict__ visit, uint32_t* err) {
   const uint32_t i = blockIdx.x * BT + threadIdx.x;
-  if (i + 1 < n && keys[i] > keys[i + 1]) atomicOr(err, 8u);
+  if (keys && i + 1 < n && keys[i] > keys[i + 1]) atomicOr(err, 8u);
   if (i < n) {
     const uint32_t p = parent_leaf[i];
     if (p >= n - 1 || (child[2 * p] != (i | REF_LEAF) && child[2 * p + 1] != (i | REF_LEAF))) atomicOr(err, 32u);
   }
   if (i + 1 < n) {
     if (visit[i] != 2u) atomicOr(err, 16u);
-    if (i > 0) {
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+      const uint32_t c = child[2 * i + k];
+      const bool ok = (c & REF_LEAF) ? ((c & ~REF_LEAF) < n && parent_leaf[c & ~REF_LEAF] == i)
+                                     : (c + 1 < n && parent_int[c] == i);
+      if (!ok) atomicOr(err, 32u);
+    }
+    if (i != root) {
       const uint32_t p = parent_int[i];
       if (p >= n - 1 || (child[2 * p] != i && child[2 * p + 1] != i)) atomicOr(err, 32u);
     }
@@ -526,6 +540,9 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
     return;
   }
   atomicMax(&a.cnt->max_depth, item.depth);
+  // a node pushes at most n-1 of its n children
+  const uint32_t need = item.need + (uint32_t)n - 1u;
+  atomicMax(&a.cnt->max_stack, need);
   atomicAdd(&a.cnt->sah_nodes, (double)half_area(uh[0] - ul[0], uh[1] - ul[1], uh[2] - ul[2]));
   if (nt) {
     atomicAdd(&a.cnt->sah_leaves, sah_leaf);
@@ -560,7 +577,7 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
       if (is_inner(c, a.isleaf)) {
         imask |= 1u << k;
         q.meta[k] = (uint8_t)(0x80u | rank_i);
-        a.qout[qbase + rank_i] = QItem{c, child_base + rank_i, item.depth + 1};
+        a.qout[qbase + rank_i] = QItem{c, child_base + rank_i, item.depth + 1, need};
         rank_i++;
       } else {
         q.meta[k] = (uint8_t)(0x80u | off_t);
@@ -660,20 +677,42 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   S.split8.reserve((size_t)ninner * 8);
   S.isleaf.reserve((size_t)ninner);
   S.visit.reserve((size_t)ninner * 4);
+  uint32_t root_bin = 0;
   if (n > 1) {
     RTG_HIP(hipMemsetAsync(S.visit.ptr, 0, (size_t)ninner * 4, s));
-    hipLaunchKernelGGL(k_radix_tree, dim3(div_up(n - 1, BT)), dim3(BT), 0, s, S.keys.as<uint32_t>(), (int)n,
-                       S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
-                       S.range.as<uint2>());
+    if (prm.builder == BUILDER_SAH) {
+      float clo[3], chi[3];
+      for (int k = 0; k < 3; k++) {
+        clo[k] = o2f(hc.clo[k]);
+        chi[k] = o2f(hc.chi[k]);
+      }
+      SahTreeOut so;
+      build_sah_tree(n, S.vals.as<uint32_t>(), S.slo.as<float4>(), S.shi.as<float4>(), clo, chi,
+                     S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
+                     S.range.as<uint2>(), S.sah, so, s);
+      root_bin = so.root;
+      // boxes in the final primitive order
+      hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(n, BT)), dim3(BT), 0, s, S.vals.as<uint32_t>(),
+                         S.lo4.as<float4>(), S.hi4.as<float4>(), n, S.slo.as<float4>(), S.shi.as<float4>());
+    } else {
+      hipLaunchKernelGGL(k_radix_tree, dim3(div_up(n - 1, BT)), dim3(BT), 0, s, S.keys.as<uint32_t>(), (int)n,
+                         S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
+                         S.range.as<uint2>());
+    }
     hipLaunchKernelGGL(k_refit, dim3(div_up(n, BT)), dim3(BT), 0, s, n, S.child.as<uint32_t>(),
                        S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(),
                        S.slo.as<float4>(), S.shi.as<float4>(), S.nlo.as<float4>(), S.nhi.as<float4>(),
                        S.cost.as<float>(), S.split8.as<uint8_t>(), S.isleaf.as<uint8_t>(), S.visit.as<uint32_t>(),
                        prm.cost_node, prm.cost_tri, max_leaf, &cnt->error);
-    hipLaunchKernelGGL(k_check_tree, dim3(div_up(n, BT)), dim3(BT), 0, s, S.keys.as<uint32_t>(), n,
+    hipLaunchKernelGGL(k_check_tree, dim3(div_up(n, BT)), dim3(BT), 0, s,
+                       prm.builder == BUILDER_SAH ? nullptr : S.keys.as<uint32_t>(), n, root_bin,
                        S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
                        S.visit.as<uint32_t>(), &cnt->error);
     RTG_HIP(hipGetLastError());
+    // the collapse follows child links: stop here if the tree is inconsistent
+    RTG_HIP(hipMemcpyAsync(&hc.error, &cnt->error, 4, hipMemcpyDeviceToHost, s));
+    RTG_HIP(hipStreamSynchronize(s));
+    if (hc.error) throw hip_error(hipErrorUnknown, "binary tree check failed: error bits " + std::to_string(hc.error));
   }
 
   // outputs: at most n nodes, exactly n triangle records
@@ -682,7 +721,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   S.queue.reserve((size_t)2 * n * sizeof(QItem));
   QItem* q0 = S.queue.as<QItem>();
   QItem* q1 = q0 + n;
-  QItem root{n > 1 ? 0u : (0u | REF_LEAF), 0u, 1u};
+  QItem root{n > 1 ? root_bin : (0u | REF_LEAF), 0u, 1u, 0u};
   RTG_HIP(hipMemcpyAsync(q0, &root, sizeof(root), hipMemcpyHostToDevice, s));
 
   CollapseArgs ca;
@@ -731,8 +770,12 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
                                          ", primitives " + std::to_string(n) + ", triangle records " +
                                          std::to_string(hc.tri_count) + ", nodes " + std::to_string(hc.node_count) +
                                          ", queue sizes per level" + level_sizes);
+  if (hc.max_stack + 1 > TRACE_STACK)
+    throw hip_error(hipErrorUnknown, "BVH too deep for the traversal stack: " + std::to_string(hc.max_stack + 1) +
+                                         " entries needed, " + std::to_string(TRACE_STACK) + " available");
   out.stats.num_nodes = hc.node_count;
   out.stats.max_depth = hc.max_depth;
+  out.stats.max_stack = hc.max_stack + 1;
   const double root_area = half_area(out.stats.bounds_hi[0] - out.stats.bounds_lo[0],
                                      out.stats.bounds_hi[1] - out.stats.bounds_lo[1],
                                      out.stats.bounds_hi[2] - out.stats.bounds_lo[2]);

@@ -1,0 +1,33 @@
+// Top-down binned-SAH binary tree over the valid primitive references, built
+// level by level on the GPU. Its output has the same form as the radix tree
+// of the Morton builder (child / parent / range arrays over final primitive
+// positions), so the bottom-up SAH refit and the 8-wide collapse run on
+// either tree unchanged.
+#pragma once
+#include "bvh_types.h"
+#include "gpu_util.h"
+#include "sort.h"
+
+namespace rtg {
+
+struct SahScratch {
+  DevBuf idx[2], blo[2], bhi[2], sid[2], segs[3][2], bins, splits, flags, scan, counters;
+  Scanner scanner;
+};
+
+struct SahTreeOut {
+  uint32_t root = 0;       // inner node id of the root (n >= 2)
+  uint32_t levels = 0;     // top-down levels run
+  uint32_t max_depth = 0;  // binary depth
+};
+
+// Inputs: n >= 2 valid references, `vals` holding their primref indices and
+// slo/shi their boxes in that order. Outputs: vals reordered so that every
+// inner node covers a contiguous position range; child/parent_int/
+// parent_leaf/range filled for inner nodes 0..n-2 (inner node ids are split
+// positions); clo/chi are the bounds of lo+hi over all references.
+void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi, const float clo[3],
+                    const float chi[3], uint32_t* child, uint32_t* parent_int, uint32_t* parent_leaf, uint2* range,
+                    SahScratch& S, SahTreeOut& out, hipStream_t s);
+
+}  // namespace rtg

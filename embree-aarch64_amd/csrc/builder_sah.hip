@@ -1,0 +1,675 @@
+// Top-down binned-SAH binary tree, one level of splits per step.
+//
+// Reference anchors for the heuristic:
+//   bin mapping on doubled centroids lo+hi with scale 0.99*num/diag and the
+//     bin index clamped to [0, num-1]: kernels/builders/heuristic_binning.h:16-112;
+//   per-bin counts and boxes, right-to-left suffix areas, left-to-right
+//     sweep keeping the first minimum, partition by bin < pos:
+//     kernels/builders/heuristic_binning.h:211-258,340-393,
+//     kernels/builders/heuristic_binning_array_aligned.h:37-143;
+//   object-median fallback when no split separates the centroids:
+//     heuristic_binning_array_aligned.h (splitFallback).
+// Differences, none of which can change a hit: 32 bins per axis (min(32, n)
+// for n < 32); the split cost is A_L*n_L + A_R*n_R per primitive (leaf sizes
+// are decided afterwards by the 8-wide collapse); splitting continues down to
+// single primitives; below depth 48 every split is an object median, which
+// bounds the depth.
+//
+// Segments (contiguous position ranges still to split) are processed in three
+// size classes:
+//   big   (> 4096): positions in chunks of 4096 bin into LDS, the chunk bins
+//                   merge into per-segment bins in HBM with atomics, one
+//                   thread per segment picks the split, and a global scan of
+//                   left flags gives every position its stable destination;
+//   mid   (65..4096): one 256-lane workgroup per segment bins, picks and
+//                   partitions in LDS;
+//   small (2..64): one 64-lane workgroup per segment, same code.
+// Every inner node gets the id of its split position (first + n_left - 1),
+// which is unique, so the output does not depend on the order in which
+// segments are processed.
+#include <string>
+
+#include "builder_sah.h"
+
+namespace rtg {
+namespace {
+
+constexpr uint32_t NIL = 0xffffffffu;
+constexpr int NB = 32;                  // bins per axis
+constexpr int BIN_WORDS = 3 * NB * 7;   // counts, then lo xyz, then hi xyz, per (axis, bin)
+constexpr uint32_t SMALL_MAX = 64;
+constexpr uint32_t MID_MAX = 4096;
+constexpr uint32_t CHUNK = 4096;        // positions per block of the big-segment binning
+constexpr uint32_t MEDIAN_DEPTH = 48;   // from this binary depth on, splits are object medians
+constexpr int TBS = 256;
+
+__device__ inline int wc(int a, int b) { return a * NB + b; }
+__device__ inline int wl(int a, int b, int k) { return 3 * NB + (a * NB + b) * 3 + k; }
+__device__ inline int wh(int a, int b, int k) { return 3 * NB + 9 * NB + (a * NB + b) * 3 + k; }
+
+struct Seg {
+  uint32_t first, count, parent, meta;  // meta = depth << 1 | side of the parent
+  float clo[3], chi[3];                 // bounds of lo+hi over the segment
+};
+
+struct Split {
+  float ofs[3], scale[3];
+  int axis;  // 0..2: binned split "bin < k" on this axis; 3: object median by position
+  int k, nb;
+  uint32_t nleft;
+  uint32_t lsid, rsid;  // ids of the children in the next big list (NIL if not big)
+};
+
+struct Counters {
+  uint32_t nseg[3];  // next-level segment counts: big, mid, small
+  uint32_t root;
+  uint32_t max_depth;
+  uint32_t error;    // bit 0: a segment list overflowed, bit 1: bin counts disagree, bit 2: destination out of range
+};
+
+struct Lists {
+  Seg* seg[3];
+  uint32_t cap[3];
+};
+
+struct Tree {
+  uint32_t* child;
+  uint32_t* parent_int;
+  uint32_t* parent_leaf;
+  uint2* range;
+  uint32_t* vals;
+  Counters* cnt;
+};
+
+struct Mapping {
+  float ofs[3], scale[3];
+  int nb;
+};
+
+__device__ inline Mapping make_mapping(const Seg& s) {
+  Mapping m;
+  m.nb = s.count < (uint32_t)NB ? (int)s.count : NB;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const float diag = s.chi[k] - s.clo[k];
+    m.ofs[k] = s.clo[k];
+    m.scale[k] = diag > 1e-34f ? 0.99f * (float)m.nb / diag : 0.0f;
+  }
+  return m;
+}
+
+__device__ inline int bin_of(float c2, float ofs, float scale, int nb) {
+  const int b = (int)floorf((c2 - ofs) * scale);
+  return b < 0 ? 0 : (b > nb - 1 ? nb - 1 : b);
+}
+
+__device__ inline float comp(float4 v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+
+__device__ inline bool goes_left(const Split& sp, float4 l, float4 h, uint32_t rel) {
+  if (sp.axis == 3) return rel < sp.nleft;
+  const float c2 = comp(l, sp.axis) + comp(h, sp.axis);
+  return bin_of(c2, sp.ofs[sp.axis], sp.scale[sp.axis], sp.nb) < sp.k;
+}
+
+__device__ inline void init_bins(uint32_t* w, int tid, int nthreads) {
+  for (int i = tid; i < BIN_WORDS; i += nthreads) w[i] = i < 3 * NB ? 0u : (i < 3 * NB + 9 * NB ? NIL : 0u);
+}
+
+__device__ inline void bin_prim(uint32_t* w, const Mapping& m, float4 l, float4 h) {
+  const float c[3] = {l.x + h.x, l.y + h.y, l.z + h.z};
+  const uint32_t ol[3] = {f2o(l.x), f2o(l.y), f2o(l.z)};
+  const uint32_t oh[3] = {f2o(h.x), f2o(h.y), f2o(h.z)};
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    const int b = bin_of(c[a], m.ofs[a], m.scale[a], m.nb);
+    atomicAdd(&w[wc(a, b)], 1u);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      atomicMin(&w[wl(a, b, k)], ol[k]);
+      atomicMax(&w[wh(a, b, k)], oh[k]);
+    }
+  }
+}
+
+// Best "bin < k" split along axis a from bin words w (LDS or HBM): least
+// A_L*n_L + A_R*n_R, first minimum in k. Returns false when no k separates
+// the primitives.
+__device__ inline bool best_on_axis(const uint32_t* w, const Mapping& m, int a, float& cost, int& kbest,
+                                    uint32_t& nleft) {
+  if (m.scale[a] == 0.0f) return false;
+  float ra[NB];
+  uint32_t rn[NB];
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  uint32_t nn = 0;
+  for (int b = m.nb - 1; b >= 1; b--) {
+    const uint32_t cb = w[wc(a, b)];
+    if (cb) {
+      nn += cb;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        lo[k] = fminf(lo[k], o2f(w[wl(a, b, k)]));
+        hi[k] = fmaxf(hi[k], o2f(w[wh(a, b, k)]));
+      }
+    }
+    rn[b] = nn;
+    ra[b] = nn ? half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) : 0.0f;
+  }
+  bool found = false;
+  cost = INFINITY;
+  nn = 0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+  for (int b = 0; b + 1 < m.nb; b++) {
+    const uint32_t cb = w[wc(a, b)];
+    if (cb) {
+      nn += cb;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        lo[k] = fminf(lo[k], o2f(w[wl(a, b, k)]));
+        hi[k] = fmaxf(hi[k], o2f(w[wh(a, b, k)]));
+      }
+    }
+    const uint32_t nr = rn[b + 1];
+    if (nn == 0 || nr == 0) continue;
+    const float c = half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) * (float)nn + ra[b + 1] * (float)nr;
+    if (c < cost) {
+      cost = c;
+      kbest = b + 1;
+      nleft = nn;
+      found = true;
+    }
+  }
+  return found;
+}
+
+// Box of bins [b0, b1) on axis a.
+__device__ inline void bins_box(const uint32_t* w, int a, int b0, int b1, float lo[3], float hi[3]) {
+#pragma unroll
+  for (int k = 0; k < 3; k++) { lo[k] = INFINITY; hi[k] = -INFINITY; }
+  for (int b = b0; b < b1; b++) {
+    if (!w[wc(a, b)]) continue;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lo[k] = fminf(lo[k], o2f(w[wl(a, b, k)]));
+      hi[k] = fmaxf(hi[k], o2f(w[wh(a, b, k)]));
+    }
+  }
+}
+
+// Records inner node m (the split of segment sg) in the tree.
+__device__ inline void link_inner(const Tree& t, const Seg& sg, uint32_t m) {
+  if (sg.parent == NIL) {
+    t.cnt->root = m;
+    t.parent_int[m] = NIL;
+  } else {
+    t.child[2 * sg.parent + (sg.meta & 1u)] = m;
+    t.parent_int[m] = sg.parent;
+  }
+  t.range[m] = make_uint2(sg.first, sg.first + sg.count - 1);
+  atomicMax(&t.cnt->max_depth, sg.meta >> 1);
+}
+
+// Child segment [first, first+count) of inner node m: a leaf link for a
+// single primitive, else a new segment in the list of its size class.
+// Returns the id in the big list, or NIL.
+__device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m, uint32_t side, uint32_t depth,
+                                      uint32_t first, uint32_t count, const float clo[3], const float chi[3]) {
+  if (count == 1) {
+    t.child[2 * m + side] = first | REF_LEAF;
+    t.parent_leaf[first] = m;
+    return NIL;
+  }
+  const int cls = count > MID_MAX ? 0 : (count > SMALL_MAX ? 1 : 2);
+  const uint32_t id = atomicAdd(&t.cnt->nseg[cls], 1u);
+  if (id >= nx.cap[cls]) {
+    atomicOr(&t.cnt->error, 1u);
+    return NIL;
+  }
+  Seg c;
+  c.first = first;
+  c.count = count;
+  c.parent = m;
+  c.meta = ((depth + 1) << 1) | side;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    c.clo[k] = clo[k];
+    c.chi[k] = chi[k];
+  }
+  nx.seg[cls][id] = c;
+  return cls == 0 ? id : NIL;
+}
+
+// ------------------------------------------------------------ big segments --
+
+__global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
+  const size_t i = (size_t)blockIdx.x * TBS + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_init_gbins(uint32_t* g, uint32_t nseg) {
+  const size_t i = (size_t)blockIdx.x * TBS + threadIdx.x;
+  if (i >= (size_t)nseg * BIN_WORDS) return;
+  const int w = (int)(i % BIN_WORDS);
+  g[i] = w < 3 * NB ? 0u : (w < 3 * NB + 9 * NB ? NIL : 0u);
+}
+
+__global__ void k_zero_counts(Counters* c) {
+  if (threadIdx.x < 3) c->nseg[threadIdx.x] = 0u;
+}
+
+// A chunk of CHUNK positions meets at most two big segments (each is longer
+// than the chunk): they are binned in LDS and merged into HBM; any other
+// segment id (not expected) bins straight into HBM.
+__global__ __launch_bounds__(TBS) void k_big_bins(const uint32_t* __restrict__ sid, const float4* __restrict__ blo,
+                                                  const float4* __restrict__ bhi, uint32_t n,
+                                                  const Seg* __restrict__ segs, uint32_t* gbins) {
+  __shared__ uint32_t sb[2][BIN_WORDS];
+  __shared__ uint32_t s_lo, s_hi;
+  __shared__ Mapping sm[2];
+  const int tid = threadIdx.x;
+  const uint32_t c0 = blockIdx.x * CHUNK;
+  const uint32_t c1 = c0 + CHUNK < n ? c0 + CHUNK : n;
+  if (tid == 0) {
+    s_lo = NIL;
+    s_hi = 0u;
+  }
+  init_bins(sb[0], tid, TBS);
+  init_bins(sb[1], tid, TBS);
+  __syncthreads();
+  uint32_t mn = NIL, mx = 0u;
+  for (uint32_t i = c0 + tid; i < c1; i += TBS) {
+    const uint32_t s = sid[i];
+    if (s != NIL) {
+      mn = s < mn ? s : mn;
+      mx = s > mx ? s : mx;
+    }
+  }
+  if (mn != NIL) {
+    atomicMin(&s_lo, mn);
+    atomicMax(&s_hi, mx);
+  }
+  __syncthreads();
+  const uint32_t slo_id = s_lo, shi_id = s_hi;
+  if (slo_id == NIL) return;  // no big segment here (uniform)
+  if (tid == 0) sm[0] = make_mapping(segs[slo_id]);
+  if (tid == 1) sm[1] = make_mapping(segs[shi_id]);
+  __syncthreads();
+  for (uint32_t i = c0 + tid; i < c1; i += TBS) {
+    const uint32_t s = sid[i];
+    if (s == NIL) continue;
+    const float4 l = blo[i], h = bhi[i];
+    if (s == slo_id) bin_prim(sb[0], sm[0], l, h);
+    else if (s == shi_id) bin_prim(sb[1], sm[1], l, h);
+    else bin_prim(gbins + (size_t)s * BIN_WORDS, make_mapping(segs[s]), l, h);
+  }
+  __syncthreads();
+  for (int slot = 0; slot < 2; slot++) {
+    if (slot == 1 && shi_id == slo_id) break;
+    uint32_t* g = gbins + (size_t)(slot ? shi_id : slo_id) * BIN_WORDS;
+    const uint32_t* w = sb[slot];
+    for (int i = tid; i < BIN_WORDS; i += TBS) {
+      if (i < 3 * NB) {
+        if (w[i]) atomicAdd(&g[i], w[i]);
+      } else if (i < 3 * NB + 9 * NB) {
+        if (w[(i - 3 * NB) / 3]) atomicMin(&g[i], w[i]);
+      } else {
+        if (w[(i - 3 * NB - 9 * NB) / 3]) atomicMax(&g[i], w[i]);
+      }
+    }
+  }
+}
+
+__global__ void k_big_split(const Seg* __restrict__ segs, uint32_t nseg, const uint32_t* __restrict__ gbins,
+                            Split* __restrict__ splits, Lists nx, Tree t) {
+  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
+  if (i >= nseg) return;
+  const Seg sg = segs[i];
+  const uint32_t depth = sg.meta >> 1;
+  const Mapping m = make_mapping(sg);
+  const uint32_t* w = gbins + (size_t)i * BIN_WORDS;
+  Split sp;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    sp.ofs[k] = m.ofs[k];
+    sp.scale[k] = m.scale[k];
+  }
+  sp.nb = m.nb;
+  sp.axis = 3;
+  sp.k = 0;
+  sp.nleft = sg.count / 2;
+  uint32_t total = 0;
+  for (int b = 0; b < m.nb; b++) total += w[wc(0, b)];
+  if (total != sg.count) atomicOr(&t.cnt->error, 2u);
+  if (depth < MEDIAN_DEPTH && total == sg.count) {
+    float best = INFINITY;
+    for (int a = 0; a < 3; a++) {
+      float c;
+      int k;
+      uint32_t nl;
+      if (best_on_axis(w, m, a, c, k, nl) && c < best) {
+        best = c;
+        sp.axis = a;
+        sp.k = k;
+        sp.nleft = nl;
+      }
+    }
+  }
+  // child centroid bounds: lo+hi of a primitive lies in [2 lo, 2 hi] of its
+  // side's box, and in the parent's bounds
+  float lcl[3], lch[3], rcl[3], rch[3];
+  if (sp.axis < 3) {
+    float bl[3], bh[3];
+    bins_box(w, sp.axis, 0, sp.k, bl, bh);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lcl[k] = fmaxf(sg.clo[k], 2.0f * bl[k]);
+      lch[k] = fminf(sg.chi[k], 2.0f * bh[k]);
+    }
+    bins_box(w, sp.axis, sp.k, sp.nb, bl, bh);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      rcl[k] = fmaxf(sg.clo[k], 2.0f * bl[k]);
+      rch[k] = fminf(sg.chi[k], 2.0f * bh[k]);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lcl[k] = rcl[k] = sg.clo[k];
+      lch[k] = rch[k] = sg.chi[k];
+    }
+  }
+  const uint32_t mid = sg.first + sp.nleft - 1;
+  link_inner(t, sg, mid);
+  sp.lsid = emit_child(t, nx, mid, 0, depth, sg.first, sp.nleft, lcl, lch);
+  sp.rsid = emit_child(t, nx, mid, 1, depth, sg.first + sp.nleft, sg.count - sp.nleft, rcl, rch);
+  splits[i] = sp;
+}
+
+__global__ __launch_bounds__(TBS) void k_big_flags(const uint32_t* __restrict__ sid, const float4* __restrict__ blo,
+                                                   const float4* __restrict__ bhi, uint32_t n,
+                                                   const Seg* __restrict__ segs, const Split* __restrict__ splits,
+                                                   uint32_t* __restrict__ flags) {
+  const uint32_t i = blockIdx.x * TBS + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = sid[i];
+  uint32_t f = 0u;
+  if (s != NIL) f = goes_left(splits[s], blo[i], bhi[i], i - segs[s].first) ? 1u : 0u;
+  flags[i] = f;
+}
+
+__global__ __launch_bounds__(TBS) void k_big_scatter(const uint32_t* __restrict__ sid, const uint32_t* __restrict__ idx,
+                                                     const float4* __restrict__ blo, const float4* __restrict__ bhi,
+                                                     uint32_t n, const Seg* __restrict__ segs,
+                                                     const Split* __restrict__ splits,
+                                                     const uint32_t* __restrict__ flags,
+                                                     const uint32_t* __restrict__ scan, uint32_t* __restrict__ sid_o,
+                                                     uint32_t* __restrict__ idx_o, float4* __restrict__ blo_o,
+                                                     float4* __restrict__ bhi_o, uint32_t* __restrict__ vals,
+                                                     Counters* cnt) {
+  const uint32_t i = blockIdx.x * TBS + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t s = sid[i];
+  if (s == NIL) return;
+  const uint32_t first = segs[s].first;
+  const Split& sp = splits[s];
+  const uint32_t lr = scan[i] - scan[first];
+  const bool left = flags[i] != 0u;
+  const uint32_t dst = left ? first + lr : first + sp.nleft + (i - first - lr);
+  if (dst >= first + segs[s].count) {
+    atomicOr(&cnt->error, 4u);
+    return;
+  }
+  const uint32_t v = idx[i];
+  idx_o[dst] = v;
+  blo_o[dst] = blo[i];
+  bhi_o[dst] = bhi[i];
+  vals[dst] = v;
+  sid_o[dst] = left ? sp.lsid : sp.rsid;
+}
+
+// ------------------------------------------------- mid and small segments --
+
+template <int BLOCK, int PER>
+__global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, const uint32_t* __restrict__ idx,
+                                               const float4* __restrict__ blo, const float4* __restrict__ bhi,
+                                               uint32_t* __restrict__ idx_o, float4* __restrict__ blo_o,
+                                               float4* __restrict__ bhi_o, Lists nx, Tree t) {
+  static_assert(PER <= 32, "flags are kept in one 32-bit mask");
+  constexpr int NW = BLOCK / 64;
+  __shared__ uint32_t sb[BIN_WORDS];
+  __shared__ float s_cost[3];
+  __shared__ int s_k[3];
+  __shared__ uint32_t s_nl[3];
+  __shared__ Split s_sp;
+  __shared__ uint32_t s_wsum[NW];
+  __shared__ uint32_t s_cb[12];  // ordered bounds of lo+hi: left lo3, left hi3, right lo3, right hi3
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const Seg sg = segs[blockIdx.x];
+  const uint32_t depth = sg.meta >> 1;
+  const Mapping m = make_mapping(sg);
+  init_bins(sb, tid, BLOCK);
+  if (tid < 12) s_cb[tid] = (tid % 6) < 3 ? NIL : 0u;
+  __syncthreads();
+  const uint32_t end = sg.first + sg.count;
+  const uint32_t p0 = sg.first + (uint32_t)tid * PER;
+#pragma unroll 4
+  for (int j = 0; j < PER; j++) {
+    const uint32_t p = p0 + j;
+    if (p < end) bin_prim(sb, m, blo[p], bhi[p]);
+  }
+  __syncthreads();
+  if (tid < 3) {
+    float c = INFINITY;
+    int k = 0;
+    uint32_t nl = 0;
+    uint32_t total = 0;
+    for (int b = 0; b < m.nb; b++) total += sb[wc(tid, b)];
+    if (total != sg.count) atomicOr(&t.cnt->error, 2u);
+    if (depth < MEDIAN_DEPTH && total == sg.count && !best_on_axis(sb, m, tid, c, k, nl)) c = INFINITY;
+    s_cost[tid] = c;
+    s_k[tid] = k;
+    s_nl[tid] = nl;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    Split sp;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      sp.ofs[k] = m.ofs[k];
+      sp.scale[k] = m.scale[k];
+    }
+    sp.nb = m.nb;
+    sp.axis = 3;
+    sp.k = 0;
+    sp.nleft = sg.count / 2;
+    float best = INFINITY;
+    for (int a = 0; a < 3; a++) {
+      if (s_cost[a] < best) {
+        best = s_cost[a];
+        sp.axis = a;
+        sp.k = s_k[a];
+        sp.nleft = s_nl[a];
+      }
+    }
+    sp.lsid = sp.rsid = NIL;
+    s_sp = sp;
+  }
+  __syncthreads();
+  const Split sp = s_sp;
+  // flags of this thread's positions, and the stable destinations
+  uint32_t mask = 0u, nl = 0u;
+#pragma unroll 4
+  for (int j = 0; j < PER; j++) {
+    const uint32_t p = p0 + j;
+    if (p < end && goes_left(sp, blo[p], bhi[p], p - sg.first)) {
+      mask |= 1u << j;
+      nl++;
+    }
+  }
+  uint32_t x = nl;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) s_wsum[wave] = x;
+  __syncthreads();
+  uint32_t lbase = x - nl;
+  for (int w = 0; w < wave; w++) lbase += s_wsum[w];
+  float bnd[12];
+#pragma unroll
+  for (int k = 0; k < 12; k++) bnd[k] = (k % 6) < 3 ? INFINITY : -INFINITY;
+  uint32_t lr = 0u;
+  for (int j = 0; j < PER; j++) {
+    const uint32_t p = p0 + j;
+    if (p >= end) break;
+    const bool left = (mask >> j) & 1u;
+    const uint32_t rel = p - sg.first;
+    const uint32_t dst = left ? sg.first + lbase + lr : sg.first + sp.nleft + (rel - lbase - lr);
+    if (left) lr++;
+    if (dst >= end) {
+      atomicOr(&t.cnt->error, 4u);
+      continue;
+    }
+    const float4 l = blo[p], h = bhi[p];
+    const uint32_t v = idx[p];
+    idx_o[dst] = v;
+    blo_o[dst] = l;
+    bhi_o[dst] = h;
+    t.vals[dst] = v;
+    const float c[3] = {l.x + h.x, l.y + h.y, l.z + h.z};
+    const int o = left ? 0 : 6;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      bnd[o + k] = fminf(bnd[o + k], c[k]);
+      bnd[o + 3 + k] = fmaxf(bnd[o + 3 + k], c[k]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 12; k++) {
+    float v = bnd[k];
+    const bool mx = (k % 6) >= 3;
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) {
+      const float y = __shfl_xor(v, d, 64);
+      v = mx ? fmaxf(v, y) : fminf(v, y);
+    }
+    if (lane == 0 && !__builtin_isinf(v)) {
+      if (mx) atomicMax(&s_cb[k], f2o(v)); else atomicMin(&s_cb[k], f2o(v));
+    }
+  }
+  __syncthreads();
+  if (tid == 0) {
+    const uint32_t mid = sg.first + sp.nleft - 1;
+    link_inner(t, sg, mid);
+    float lcl[3], lch[3], rcl[3], rch[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lcl[k] = o2f(s_cb[k]);
+      lch[k] = o2f(s_cb[3 + k]);
+      rcl[k] = o2f(s_cb[6 + k]);
+      rch[k] = o2f(s_cb[9 + k]);
+    }
+    emit_child(t, nx, mid, 0, depth, sg.first, sp.nleft, lcl, lch);
+    emit_child(t, nx, mid, 1, depth, sg.first + sp.nleft, sg.count - sp.nleft, rcl, rch);
+  }
+}
+
+}  // namespace
+
+void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi, const float clo[3],
+                    const float chi[3], uint32_t* child, uint32_t* parent_int, uint32_t* parent_leaf, uint2* range,
+                    SahScratch& S, SahTreeOut& out, hipStream_t s) {
+  if (n < 2) throw hip_error(hipErrorInvalidValue, "SAH tree needs at least two primitives");
+  const uint32_t cap[3] = {n / MID_MAX + 2, n / (SMALL_MAX + 1) + 2, n / 2 + 2};
+  for (int p = 0; p < 2; p++) {
+    S.idx[p].reserve((size_t)n * 4);
+    S.blo[p].reserve((size_t)n * 16);
+    S.bhi[p].reserve((size_t)n * 16);
+    S.sid[p].reserve((size_t)n * 4);
+    for (int c = 0; c < 3; c++) S.segs[c][p].reserve((size_t)cap[c] * sizeof(Seg));
+  }
+  S.bins.reserve((size_t)cap[0] * BIN_WORDS * 4);
+  S.splits.reserve((size_t)cap[0] * sizeof(Split));
+  S.flags.reserve((size_t)n * 4);
+  S.scan.reserve((size_t)n * 4);
+  S.counters.reserve(sizeof(Counters));
+  Counters* cnt = S.counters.as<Counters>();
+
+  RTG_HIP(hipMemcpyAsync(S.idx[0].ptr, vals, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
+  RTG_HIP(hipMemcpyAsync(S.blo[0].ptr, slo, (size_t)n * 16, hipMemcpyDeviceToDevice, s));
+  RTG_HIP(hipMemcpyAsync(S.bhi[0].ptr, shi, (size_t)n * 16, hipMemcpyDeviceToDevice, s));
+
+  Seg root;
+  root.first = 0;
+  root.count = n;
+  root.parent = NIL;
+  root.meta = 0;
+  for (int k = 0; k < 3; k++) {
+    root.clo[k] = clo[k];
+    root.chi[k] = chi[k];
+  }
+  const int rcls = n > MID_MAX ? 0 : (n > SMALL_MAX ? 1 : 2);
+  RTG_HIP(hipMemcpyAsync(S.segs[rcls][0].ptr, &root, sizeof(root), hipMemcpyHostToDevice, s));
+  uint32_t cur[3] = {0, 0, 0};
+  cur[rcls] = 1;
+  if (rcls == 0) hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[0].as<uint32_t>(), (size_t)n, 0u);
+  Counters hc{};
+  hc.root = NIL;
+  hc.max_depth = 0;
+  hc.error = 0;
+  RTG_HIP(hipMemcpyAsync(cnt, &hc, sizeof(hc), hipMemcpyHostToDevice, s));
+
+  Tree t{child, parent_int, parent_leaf, range, vals, cnt};
+  int par = 0;
+  uint32_t levels = 0;
+  while (cur[0] + cur[1] + cur[2] > 0) {
+    if (++levels > 4096) throw hip_error(hipErrorUnknown, "SAH build did not terminate");
+    const int in = par, o = par ^ 1;
+    Lists nx;
+    for (int c = 0; c < 3; c++) {
+      nx.seg[c] = S.segs[c][o].as<Seg>();
+      nx.cap[c] = cap[c];
+    }
+    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, cnt);
+    if (cur[0]) {
+      const Seg* bs = S.segs[0][in].as<Seg>();
+      hipLaunchKernelGGL(k_init_gbins, dim3(div_up((size_t)cur[0] * BIN_WORDS, TBS)), dim3(TBS), 0, s,
+                         S.bins.as<uint32_t>(), cur[0]);
+      hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[o].as<uint32_t>(), (size_t)n, NIL);
+      hipLaunchKernelGGL(k_big_bins, dim3(div_up(n, CHUNK)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
+                         S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs, S.bins.as<uint32_t>());
+      hipLaunchKernelGGL(k_big_split, dim3(div_up(cur[0], 64)), dim3(64), 0, s, bs, cur[0], S.bins.as<uint32_t>(),
+                         S.splits.as<Split>(), nx, t);
+      hipLaunchKernelGGL(k_big_flags, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
+                         S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs, S.splits.as<Split>(),
+                         S.flags.as<uint32_t>());
+      S.scanner.exclusive(S.flags.as<uint32_t>(), S.scan.as<uint32_t>(), n, s);
+      hipLaunchKernelGGL(k_big_scatter, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
+                         S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs,
+                         S.splits.as<Split>(), S.flags.as<uint32_t>(), S.scan.as<uint32_t>(), S.sid[o].as<uint32_t>(),
+                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), vals, cnt);
+    }
+    if (cur[1])
+      hipLaunchKernelGGL((k_seg<256, 16>), dim3(cur[1]), dim3(256), 0, s, S.segs[1][in].as<Seg>(),
+                         S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
+                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t);
+    if (cur[2])
+      hipLaunchKernelGGL((k_seg<64, 1>), dim3(cur[2]), dim3(64), 0, s, S.segs[2][in].as<Seg>(),
+                         S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
+                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t);
+    RTG_HIP(hipGetLastError());
+    RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+    RTG_HIP(hipStreamSynchronize(s));
+    if (hc.error) throw hip_error(hipErrorUnknown, "SAH build failed: error bits " + std::to_string(hc.error));
+    for (int c = 0; c < 3; c++) cur[c] = hc.nseg[c];
+    par = o;
+  }
+  if (hc.root >= n - 1) throw hip_error(hipErrorUnknown, "SAH build produced no root");
+  out.root = hc.root;
+  out.levels = levels;
+  out.max_depth = hc.max_depth;
+}
+
+}  // namespace rtg

@@ -77,6 +77,10 @@ struct DevScene {
   float scale[3];  // 1023.99 / extent per axis (0 for a flat axis)
 };
 
+// Entries of the per-ray traversal stack; the builder rejects a tree whose
+// deepest path could need more.
+constexpr uint32_t TRACE_STACK = 256;
+
 // Conservative-culling constants shared with the oracle's rules.
 constexpr float CULL_MARGIN = 1.0f / 4096.0f;       // relative margin on ray interval ends
 constexpr float BOX_EPS_REL = 1.0f / 1048576.0f;    // absolute box widening / max |coord|

@@ -26,7 +26,7 @@
 namespace rtg {
 
 constexpr int TB = 256;
-constexpr int STACK = 256;  // total entries per ray (builder bounds the depth so 7*depth+1 fits)
+constexpr int STACK = (int)TRACE_STACK;  // entries per ray (the builder checks its trees against it)
 
 // Per-lane traversal stack: the top LDS_SLOTS entries live in LDS (slot-major,
 // so lanes touching the same slot hit distinct banks), older entries spill
@@ -404,8 +404,14 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
 // one traversal step (one node, or one leaf plus the pop that follows) for
 // every active lane. Every wave leaves the loop once the counter has passed
 // M and its lanes are idle.
+// Resident 256-lane blocks per compute unit the persistent kernel is
+// compiled for (one wave per SIMD each): the register budget follows from it.
+#ifndef RTG_TRACE_MIN_BLOCKS
+#define RTG_TRACE_MIN_BLOCKS 1
+#endif
+
 template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool FUSED>
-__global__ __launch_bounds__(TB) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
+__global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                                 uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                                 uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
                                                 const uint32_t* __restrict__ perm, uint32_t* __restrict__ next,

@@ -63,7 +63,7 @@ class RTCBounds(ctypes.Structure):
 
 class RTCGSceneStats(ctypes.Structure):
     _fields_ = [("num_prims", ctypes.c_uint), ("num_nodes", ctypes.c_uint), ("max_depth", ctypes.c_uint),
-                ("node_bytes", ctypes.c_uint), ("tri_bytes", ctypes.c_uint), ("pad", ctypes.c_uint),
+                ("node_bytes", ctypes.c_uint), ("tri_bytes", ctypes.c_uint), ("max_stack", ctypes.c_uint),
                 ("sah", ctypes.c_double), ("sah_tri", ctypes.c_double), ("build_ms", ctypes.c_double)]
 
 
@@ -307,7 +307,7 @@ class Scene:
     def stats(self):
         s = RTCGSceneStats()
         lib().rtcgGetSceneStats(self.h, ctypes.byref(s))
-        return {k: getattr(s, k) for k, _ in RTCGSceneStats._fields_ if k != "pad"}
+        return {k: getattr(s, k) for k, _ in RTCGSceneStats._fields_}
 
     def bounds(self):
         b = RTCBounds()

@@ -58,7 +58,7 @@ struct RTCGSceneStats
   unsigned int max_depth;      /* node levels */
   unsigned int node_bytes;     /* bytes per node record */
   unsigned int tri_bytes;      /* bytes per triangle record */
-  unsigned int pad;
+  unsigned int max_stack;      /* traversal stack entries the deepest path can need */
   double sah;                  /* SAH statistic, leaf cost per block of 4 triangles */
   double sah_tri;              /* SAH statistic, leaf cost per triangle */
   double build_ms;             /* last commit's build time */

@@ -230,6 +230,51 @@ def test_soup_parity_with_edge_rays(rtc, device, oracle, quality):
     sc.release()
 
 
+@pytest.fixture(scope="module")
+def sah_device(rtc):
+    d = rtc.Device("builder=sah")
+    yield d
+    d.release()
+
+
+@pytest.mark.parametrize("n", [2, 3, 64, 65, 4096, 4097, 70000])
+def test_sah_builder_sizes(rtc, sah_device, oracle, n):
+    """Binned-SAH tree at the size-class boundaries (one wave per segment up
+    to 64 primitives, one workgroup up to 4096, chunked binning above):
+    structure and closest/any hits identical to the oracle."""
+    v, ix = soup_case(rtc, n, 100 + n, h=0.03 if n > 1000 else 0.2)
+    sc = rtc.Scene(sah_device, [(v, ix)]).commit()
+    st = sc.stats()
+    assert st["num_prims"] == n and st["max_stack"] <= 256
+    nodes, tris = sc.read_bvh()
+    check_bvh(nodes, tris, [(v, ix)])
+    rays = rtc.gen_random_rays(20000, seed=n, lo=-0.05, hi=1.05)
+    assert_same(trace(rtc, sc, rays), oracle_trace(oracle, [(v, ix)], rays), f"SAH n={n}")
+    occ = rays.copy()
+    occ["tfar"] = 0.05
+    assert_same(trace(rtc, sc, occ), oracle_trace(oracle, [(v, ix)], occ), f"SAH n={n} short rays")
+    sc.release()
+
+
+def test_sah_builder_degenerate_and_multi_geometry(rtc, sah_device, oracle):
+    """Identical centroids force the object-median fallback; duplicated
+    triangles across geometries exercise the tie rule on the SAH tree."""
+    v = np.array([[0.1, 0.1, 0.1], [0.9, 0.1, 0.1], [0.1, 0.9, 0.5]], np.float32)
+    same = np.tile(np.array([[0, 1, 2]], np.uint32), (5000, 1))
+    rays = rtc.gen_random_rays(20000, seed=21)
+    sc = rtc.Scene(sah_device, [(v, same)]).commit()
+    assert_same(trace(rtc, sc, rays), oracle_trace(oracle, [(v, same)], rays, brute=True), "SAH identical triangles")
+    sc.release()
+    v1, i1 = soup_case(rtc, 30000, 22, h=0.03)
+    v2, i2 = rtc.gen_sphere(64, (0.5, 0.5, 0.5), 0.3)
+    meshes = [(v1, i1), (v2, i2), (v1, i1[:3000])]
+    sc = rtc.Scene(sah_device, meshes).commit()
+    nodes, tris = sc.read_bvh()
+    check_bvh(nodes, tris, meshes)
+    assert_same(trace(rtc, sc, rays), oracle_trace(oracle, meshes, rays), "SAH multi geometry")
+    sc.release()
+
+
 def test_soup_occlusion_parity(rtc, device, oracle):
     v, ix = soup_case(rtc, 20000, 7, h=0.02)
     sc = rtc.Scene(device, [(v, ix)]).commit()
@@ -374,7 +419,14 @@ def test_c2_full_size_parity(rtc, device, oracle):
     bvh.intersect(ref)
     assert_same(out, ref, "C2 closest hit")
     hitrate = (out["geomID"] != INV).mean()
-    del out, ref
+    del out
+    # the binned-SAH tree gives the same records
+    d2 = rtc.Device("builder=sah")
+    sc2 = rtc.Scene(d2, [(v, ix)]).commit()
+    assert_same(trace(rtc, sc2, rays), ref, "C2 closest hit, SAH tree")
+    sc2.release()
+    d2.release()
+    del ref
     occ = np.zeros(len(rays), rtc.RTCRay)
     for k in rtc.RTCRay.names:
         occ[k] = rays[k]
