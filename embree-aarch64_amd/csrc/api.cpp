@@ -236,8 +236,6 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "persistent") d->trace.persistent = atoi(v.c_str());
     else if (k == "refill_min") d->trace.refill_min = atoi(v.c_str());
     else if (k == "refill_min_occ") d->trace.refill_min_occ = atoi(v.c_str());
-    else if (k == "slab") d->trace.fused_slab = (v == "fused") ? 1 : 0;
-    else if (k == "ww") d->trace.while_while = atoi(v.c_str());
     else if (k == "blocks_per_cu") d->trace.blocks_per_cu = atoi(v.c_str());
     // threads, isa, tri_accel, tri_builder, set_affinity, ... are CPU settings: accepted, ignored
   }
@@ -245,9 +243,10 @@ void parse_config(Device* d, const char* cfg) {
 
 // Build settings for a scene: LOW asks for the Morton radix tree, HIGH for
 // the binned-SAH tree, MEDIUM for the device's configured builder.
-rtg::BuildParams build_params(const Device* d, RTCBuildQuality q) {
+rtg::BuildParams build_params(const Device* d, RTCBuildQuality q, RTCSceneFlags flags) {
   rtg::BuildParams p = d->build;
   p.quality = q;
+  p.robust = (flags & RTC_SCENE_FLAG_ROBUST) ? 1 : 0;
   if (q == RTC_BUILD_QUALITY_LOW) p.builder = rtg::BUILDER_MORTON;
   else if (q == RTC_BUILD_QUALITY_HIGH) p.builder = rtg::BUILDER_SAH;
   return p;
@@ -343,7 +342,7 @@ void commit_scene(Scene* sc) {
   }
   {
     std::lock_guard<std::mutex> bl(d->build_mtx);
-    const rtg::BuildParams p = build_params(d, sc->quality);
+    const rtg::BuildParams p = build_params(d, sc->quality, sc->flags);
     d->monitor((ssize_t)prim_offset * (80 + 48));
     rtg::build_bvh(sc->descs, p, d->scratch, sc->bvh, t.stream);
   }
@@ -1084,7 +1083,7 @@ RTC_API void rtcgTimeBuild(RTCScene scene, int steps, float* ms_out) {
     std::lock_guard<std::mutex> bl(d->build_mtx);
     RTG_HIP(hipStreamSynchronize(t.stream));
     auto t0 = std::chrono::steady_clock::now();
-    const rtg::BuildParams p = build_params(d, s->quality);
+    const rtg::BuildParams p = build_params(d, s->quality, s->flags);
     rtg::build_bvh(s->descs, p, d->scratch, s->bvh, t.stream);
     RTG_HIP(hipStreamSynchronize(t.stream));
     if (ms_out) ms_out[i] = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -19,7 +19,8 @@ struct BuildParams {
   uint32_t max_leaf = 8;       // max triangles per leaf (<= 15)
   float cost_node = 1.0f;      // SAH cost of visiting one 8-wide node, per unit half area
   float cost_tri = 0.45f;      // SAH cost of one triangle test, per unit half area
-  int quality = 1;             // RTCBuildQuality (LOW uses the same LBVH; MEDIUM/HIGH add SAH binning)
+  int quality = 1;             // RTCBuildQuality (LOW: Morton tree, HIGH: binned SAH, MEDIUM: `builder`)
+  int robust = 0;              // leaves hold (v0, v1, v2) for the Pluecker test
 };
 
 struct BuildStats {
@@ -44,6 +45,7 @@ struct DeviceBVH {
     s.tris = tris.as<TriRec>();
     s.num_nodes = stats.num_nodes;
     s.num_tris = stats.num_prims;
+    s.robust = robust;
     for (int k = 0; k < 3; k++) {
       const float ext = stats.bounds_hi[k] - stats.bounds_lo[k];
       s.lo[k] = stats.bounds_lo[k];
@@ -52,6 +54,7 @@ struct DeviceBVH {
     return s;
   }
   BuildStats stats;
+  uint32_t robust = 0;
 };
 
 // Scratch reused across builds on one device.

@@ -406,6 +406,7 @@ struct CollapseArgs {
   TriRec* tris;
   BuildCounters* cnt;
   uint32_t n;  // capacity of nodes, tris and each queue
+  uint32_t robust;
 };
 
 __device__ inline bool is_inner(uint32_t ref, const uint8_t* isleaf) {
@@ -596,8 +597,13 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
           const float* p2 = (const float*)(g.vertices + (uint64_t)ix[2] * g.vstride);
           TriRec tr;
           tr.v0x = p0[0]; tr.v0y = p0[1]; tr.v0z = p0[2];
-          tr.e1x = p0[0] - p1[0]; tr.e1y = p0[1] - p1[1]; tr.e1z = p0[2] - p1[2];
-          tr.e2x = p2[0] - p0[0]; tr.e2y = p2[1] - p0[1]; tr.e2z = p2[2] - p0[2];
+          if (a.robust) {
+            tr.e1x = p1[0]; tr.e1y = p1[1]; tr.e1z = p1[2];
+            tr.e2x = p2[0]; tr.e2y = p2[1]; tr.e2z = p2[2];
+          } else {
+            tr.e1x = p0[0] - p1[0]; tr.e1y = p0[1] - p1[1]; tr.e1z = p0[2] - p1[2];
+            tr.e2x = p2[0] - p0[0]; tr.e2y = p2[1] - p0[1]; tr.e2z = p2[2] - p0[2];
+          }
           tr.primID = prim;
           tr.geomID = g.geomID;
           tr.flags = (j + 1 == cntk) ? TRI_LAST : 0u;
@@ -621,6 +627,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
                hipStream_t s) {
   auto t0 = std::chrono::steady_clock::now();
   out.stats = BuildStats();
+  out.robust = prm.robust ? 1u : 0u;
   uint32_t total = 0;
   for (const auto& g : geoms) total += g.num_tris;
   if (geoms.empty() || total == 0) {
@@ -741,6 +748,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   ca.tris = out.tris.as<TriRec>();
   ca.cnt = cnt;
   ca.n = n;
+  ca.robust = prm.robust ? 1u : 0u;
   uint32_t nin = 1;
   int levels = 0;
   std::string level_sizes;

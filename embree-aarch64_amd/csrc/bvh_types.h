@@ -40,6 +40,8 @@ static_assert(sizeof(QNode8) == 80, "QNode8 must be 80 bytes");
 //   w0 = (v0.x, v0.y, v0.z, primID bits)
 //   w1 = (e1.x, e1.y, e1.z, geomID bits)
 //   w2 = (e2.x, e2.y, e2.z, flags)
+// Robust scenes keep the vertices themselves, as the reference's Triangle4v
+// leaves do (kernels/geometry/trianglev.h): w1 = (v1, geomID), w2 = (v2, flags).
 struct alignas(16) TriRec {
   float v0x, v0y, v0z;
   uint32_t primID;
@@ -75,6 +77,7 @@ struct DevScene {
   uint32_t num_tris;
   float lo[3];     // scene box, for ray ordering keys
   float scale[3];  // 1023.99 / extent per axis (0 for a flat axis)
+  uint32_t robust; // triangle records hold (v0, v1, v2) for the Pluecker test
 };
 
 // Entries of the per-ray traversal stack; the builder rejects a tree whose

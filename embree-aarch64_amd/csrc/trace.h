@@ -14,8 +14,6 @@ struct TraceParams {
   int persistent = 1;       // persistent waves pulling rays from a counter
   int refill_min = 16;      // idle lanes of a wave that trigger a refill (closest hit)
   int refill_min_occ = 32;  // same for occlusion rays, which finish sooner
-  int fused_slab = 0;       // slab distances as one FMA per plane (else decode the plane, then subtract and scale)
-  int while_while = 0;      // node phase / leaf phase traversal with a postponed leaf per lane
   int cus = 256;            // compute units (set from the device)
   int blocks_per_cu = 8;    // resident 256-thread blocks per compute unit for the persistent grid
 };

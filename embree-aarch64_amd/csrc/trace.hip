@@ -123,8 +123,9 @@ struct NodeHit {
 };
 
 // Slab test of the 8 children; decoded planes are fmaf(2^(e-127), q, p), the
-// exact expression the builder verified against every child box.
-template <bool FUSED>
+// exact expression the builder verified against every child box, then the
+// robust per-plane distance (plane - org) * rdir with rdir rounded down for
+// near and up for far planes.
 __device__ inline void intersect_node(const QNode8* __restrict__ node, const TRay& r, float thi, NodeHit& h) {
   const uint4* np = reinterpret_cast<const uint4*>(node);
   const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
@@ -144,24 +145,6 @@ __device__ inline void intersect_node(const QNode8* __restrict__ node, const TRa
   const uint32_t fy0 = r.negy ? w3.x : w3.z, fy1 = r.negy ? w3.y : w3.w;
   const uint32_t nz0 = r.negz ? w4.z : w4.x, nz1 = r.negz ? w4.w : w4.y;
   const uint32_t fz0 = r.negz ? w4.x : w4.z, fz1 = r.negz ? w4.y : w4.w;
-  // Slab distances as one FMA per plane: t = q * (s * rd) + (p - o) * rd.
-  // s is a power of two, so s * rd is exact; the constant term carries at
-  // most two roundings and the FMA one more, all bounded by
-  // 2^-22 * (|a| + 255 |b|); twice that is subtracted from near and added to
-  // far distances. The decoded planes of the builder lie outside the child boxes
-  // by the box widening, which covers the half-ulp between the plane's exact
-  // value p + s*q and its rounded decode.
-  const float ax = FUSED ? px - r.ox : 0.0f, ay = FUSED ? py - r.oy : 0.0f, az = FUSED ? pz - r.oz : 0.0f;
-  const float anx = ax * r.rnx, bnx = sx * r.rnx, afx = ax * r.rfx, bfx = sx * r.rfx;
-  const float any = ay * r.rny, bny = sy * r.rny, afy = ay * r.rfy, bfy = sy * r.rfy;
-  const float anz = az * r.rnz, bnz = sz * r.rnz, afz = az * r.rfz, bfz = sz * r.rfz;
-  const float kerr = 4.76837158203125e-7f;  // 2^-21: twice the bound below
-  const float cnx = anx - kerr * (fabsf(anx) + 255.0f * fabsf(bnx));
-  const float cfx = afx + kerr * (fabsf(afx) + 255.0f * fabsf(bfx));
-  const float cny = any - kerr * (fabsf(any) + 255.0f * fabsf(bny));
-  const float cfy = afy + kerr * (fabsf(afy) + 255.0f * fabsf(bfy));
-  const float cnz = anz - kerr * (fabsf(anz) + 255.0f * fabsf(bnz));
-  const float cfz = afz + kerr * (fabsf(afz) + 255.0f * fabsf(bfz));
   uint32_t mask = 0;
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -170,22 +153,12 @@ __device__ inline void intersect_node(const QNode8* __restrict__ node, const TRa
     const float qnx = byte_f(c < 4 ? nx0 : nx1, k), qfx = byte_f(c < 4 ? fx0 : fx1, k);
     const float qny = byte_f(c < 4 ? ny0 : ny1, k), qfy = byte_f(c < 4 ? fy0 : fy1, k);
     const float qnz = byte_f(c < 4 ? nz0 : nz1, k), qfz = byte_f(c < 4 ? fz0 : fz1, k);
-    float tnx, tfx, tny, tfy, tnz, tfz;
-    if (FUSED) {
-      tnx = __builtin_fmaf(qnx, bnx, cnx);
-      tfx = __builtin_fmaf(qfx, bfx, cfx);
-      tny = __builtin_fmaf(qny, bny, cny);
-      tfy = __builtin_fmaf(qfy, bfy, cfy);
-      tnz = __builtin_fmaf(qnz, bnz, cnz);
-      tfz = __builtin_fmaf(qfz, bfz, cfz);
-    } else {
-      tnx = (__builtin_fmaf(sx, qnx, px) - r.ox) * r.rnx;
-      tfx = (__builtin_fmaf(sx, qfx, px) - r.ox) * r.rfx;
-      tny = (__builtin_fmaf(sy, qny, py) - r.oy) * r.rny;
-      tfy = (__builtin_fmaf(sy, qfy, py) - r.oy) * r.rfy;
-      tnz = (__builtin_fmaf(sz, qnz, pz) - r.oz) * r.rnz;
-      tfz = (__builtin_fmaf(sz, qfz, pz) - r.oz) * r.rfz;
-    }
+    const float tnx = (__builtin_fmaf(sx, qnx, px) - r.ox) * r.rnx;
+    const float tfx = (__builtin_fmaf(sx, qfx, px) - r.ox) * r.rfx;
+    const float tny = (__builtin_fmaf(sy, qny, py) - r.oy) * r.rny;
+    const float tfy = (__builtin_fmaf(sy, qfy, py) - r.oy) * r.rfy;
+    const float tnz = (__builtin_fmaf(sz, qnz, pz) - r.oz) * r.rnz;
+    const float tfz = (__builtin_fmaf(sz, qfz, pz) - r.oz) * r.rfz;
     const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tlo));
     const float tf = fminf(fminf(tfx, tfy), fminf(tfz, thi));
     h.tn[c] = tn;
@@ -210,33 +183,90 @@ __device__ inline float dot3(float ax, float ay, float az, float bx, float by, f
   return __builtin_fmaf(ax, bx, __builtin_fmaf(ay, by, az * bz));
 }
 
-// Moeller test; returns validity and the unnormalised quantities.
-__device__ inline bool moeller(const TRay& r, const TriRec* __restrict__ tp, float& T, float& absDen, float& U,
-                               float& V, float& ngx, float& ngy, float& ngz, uint32_t& prim, uint32_t& geom,
-                               uint32_t& flags) {
+// One triangle test. t is final; the hit's barycentrics are un/dn and vn/dn
+// (divided only when the hit improves the current best).
+struct TriTest {
+  float t, un, vn, dn, ngx, ngy, ngz;
+  uint32_t prim, geom, flags;
+};
+
+__device__ inline float sse_min(float a, float b) { return a < b ? a : b; }
+__device__ inline float sse_max(float a, float b) { return a > b ? a : b; }
+
+// PL = false: Moeller-Trumbore on the record (v0, e1 = v0-v1, e2 = v2-v0).
+// PL = true: Pluecker (robust scenes) on the record (v0, v1, v2),
+//   kernels/geometry/triangle_intersector_pluecker.h:55-110 and finalize
+//   :29-36: edge functions of the origin-shifted vertices, eps = ulp*|U+V+W|,
+//   stable_triangle_normal (common/math/vec3.h:210-222), den and T doubled,
+//   t = T/den, u = U/UVW (U*0 when |UVW| < min_rcp_input), min/max of three
+//   with SSE semantics; valid for tnear <= t <= tfar.
+template <bool PL>
+__device__ inline bool tri_test(const TRay& r, const TriRec* __restrict__ tp, TriTest& o) {
   const float4* p4 = reinterpret_cast<const float4*>(tp);
   const float4 a = p4[0], b = p4[1], c = p4[2];
-  prim = __float_as_uint(a.w);
-  geom = __float_as_uint(b.w);
-  flags = __float_as_uint(c.w);
-  const float Cx = a.x - r.ox, Cy = a.y - r.oy, Cz = a.z - r.oz;
-  // R = cross(C, D)
-  const float Rx = prod_diff(Cy, r.dz, Cz, r.dy);
-  const float Ry = prod_diff(Cz, r.dx, Cx, r.dz);
-  const float Rz = prod_diff(Cx, r.dy, Cy, r.dx);
-  // Ng = cross(e2, e1)
-  ngx = prod_diff(c.y, b.z, c.z, b.y);
-  ngy = prod_diff(c.z, b.x, c.x, b.z);
-  ngz = prod_diff(c.x, b.y, c.y, b.x);
-  const float den = dot3(ngx, ngy, ngz, r.dx, r.dy, r.dz);
-  absDen = fabsf(den);
-  const uint32_t sgn = __float_as_uint(den) & 0x80000000u;
-  U = __uint_as_float(__float_as_uint(dot3(Rx, Ry, Rz, c.x, c.y, c.z)) ^ sgn);
-  V = __uint_as_float(__float_as_uint(dot3(Rx, Ry, Rz, b.x, b.y, b.z)) ^ sgn);
-  bool valid = (den != 0.0f) & (U >= 0.0f) & (V >= 0.0f) & (U + V <= absDen);
-  T = __uint_as_float(__float_as_uint(dot3(ngx, ngy, ngz, Cx, Cy, Cz)) ^ sgn);
-  valid &= (absDen * r.tnear < T) & (T <= absDen * r.tfar);
-  return valid;
+  o.prim = __float_as_uint(a.w);
+  o.geom = __float_as_uint(b.w);
+  o.flags = __float_as_uint(c.w);
+  if (!PL) {
+    const float Cx = a.x - r.ox, Cy = a.y - r.oy, Cz = a.z - r.oz;
+    // R = cross(C, D)
+    const float Rx = prod_diff(Cy, r.dz, Cz, r.dy);
+    const float Ry = prod_diff(Cz, r.dx, Cx, r.dz);
+    const float Rz = prod_diff(Cx, r.dy, Cy, r.dx);
+    // Ng = cross(e2, e1)
+    o.ngx = prod_diff(c.y, b.z, c.z, b.y);
+    o.ngy = prod_diff(c.z, b.x, c.x, b.z);
+    o.ngz = prod_diff(c.x, b.y, c.y, b.x);
+    const float den = dot3(o.ngx, o.ngy, o.ngz, r.dx, r.dy, r.dz);
+    const float absDen = fabsf(den);
+    const uint32_t sgn = __float_as_uint(den) & 0x80000000u;
+    const float U = __uint_as_float(__float_as_uint(dot3(Rx, Ry, Rz, c.x, c.y, c.z)) ^ sgn);
+    const float V = __uint_as_float(__float_as_uint(dot3(Rx, Ry, Rz, b.x, b.y, b.z)) ^ sgn);
+    bool valid = (den != 0.0f) & (U >= 0.0f) & (V >= 0.0f) & (U + V <= absDen);
+    const float T = __uint_as_float(__float_as_uint(dot3(o.ngx, o.ngy, o.ngz, Cx, Cy, Cz)) ^ sgn);
+    valid &= (absDen * r.tnear < T) & (T <= absDen * r.tfar);
+    if (valid) o.t = T / absDen;
+    o.un = U;
+    o.vn = V;
+    o.dn = absDen;
+    return valid;
+  } else {
+    const float v0x = a.x - r.ox, v0y = a.y - r.oy, v0z = a.z - r.oz;
+    const float v1x = b.x - r.ox, v1y = b.y - r.oy, v1z = b.z - r.oz;
+    const float v2x = c.x - r.ox, v2y = c.y - r.oy, v2z = c.z - r.oz;
+    const float e0x = v2x - v0x, e0y = v2y - v0y, e0z = v2z - v0z;
+    const float e1x = v0x - v1x, e1y = v0y - v1y, e1z = v0z - v1z;
+    const float e2x = v1x - v2x, e2y = v1y - v2y, e2z = v1z - v2z;
+    float sx = v2x + v0x, sy = v2y + v0y, sz = v2z + v0z;
+    const float U = dot3(prod_diff(e0y, sz, e0z, sy), prod_diff(e0z, sx, e0x, sz), prod_diff(e0x, sy, e0y, sx),
+                         r.dx, r.dy, r.dz);
+    sx = v0x + v1x; sy = v0y + v1y; sz = v0z + v1z;
+    const float V = dot3(prod_diff(e1y, sz, e1z, sy), prod_diff(e1z, sx, e1x, sz), prod_diff(e1x, sy, e1y, sx),
+                         r.dx, r.dy, r.dz);
+    sx = v1x + v2x; sy = v1y + v2y; sz = v1z + v2z;
+    const float W = dot3(prod_diff(e2y, sz, e2z, sy), prod_diff(e2z, sx, e2x, sz), prod_diff(e2x, sy, e2y, sx),
+                         r.dx, r.dy, r.dz);
+    const float UVW = U + V + W;
+    const float eps = 1.1920928955078125e-7f * fabsf(UVW);
+    bool valid = (sse_min(sse_min(U, V), W) >= -eps) || (sse_max(sse_max(U, V), W) <= eps);
+    // stable_triangle_normal(e0, e1, e2)
+    const float ab_x = e0z * e1y, ab_y = e0x * e1z, ab_z = e0y * e1x;
+    const float bc_x = e1z * e2y, bc_y = e1x * e2z, bc_z = e1y * e2x;
+    o.ngx = fabsf(ab_x) < fabsf(bc_x) ? __builtin_fmaf(e0y, e1z, -ab_x) : __builtin_fmaf(e1y, e2z, -bc_x);
+    o.ngy = fabsf(ab_y) < fabsf(bc_y) ? __builtin_fmaf(e0z, e1x, -ab_y) : __builtin_fmaf(e1z, e2x, -bc_y);
+    o.ngz = fabsf(ab_z) < fabsf(bc_z) ? __builtin_fmaf(e0x, e1y, -ab_z) : __builtin_fmaf(e1x, e2y, -bc_z);
+    const float d = dot3(o.ngx, o.ngy, o.ngz, r.dx, r.dy, r.dz);
+    const float den = d + d;
+    const float tt = dot3(v0x, v0y, v0z, o.ngx, o.ngy, o.ngz);
+    const float T = tt + tt;
+    o.t = T / den;
+    valid &= (r.tnear <= o.t) & (o.t <= r.tfar) & (den != 0.0f);
+    const bool tiny = fabsf(UVW) < MIN_RCP_INPUT;
+    o.un = tiny ? U * 0.0f : U;
+    o.vn = tiny ? V * 0.0f : V;
+    o.dn = tiny ? 1.0f : UVW;
+    return valid;
+  }
 }
 
 template <bool ALIGNED>
@@ -252,7 +282,7 @@ __device__ inline void load_ray(const char* rp, float4& a, float4& b, float& tfa
   tfar = *reinterpret_cast<const float*>(rp + 32);
 }
 
-template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool FUSED>
+template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool PL>
 __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                               uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                               uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
@@ -299,7 +329,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
         break;
       }
       NodeHit h;
-      intersect_node<FUSED>(sc.nodes + cur, r, lim, h);
+      intersect_node(sc.nodes + cur, r, lim, h);
       if (COUNT) nvis++;
       if (h.mask) {
         if (OCCLUDED) {
@@ -339,28 +369,26 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
           atomicOr(err, 4u);
           break;
         }
-        float T, absDen, U, V, ngx, ngy, ngz;
-        uint32_t prim, geom, flags;
-        const bool valid = moeller(r, sc.tris + ti, T, absDen, U, V, ngx, ngy, ngz, prim, geom, flags);
+        TriTest h;
+        const bool valid = tri_test<PL>(r, sc.tris + ti, h);
         if (COUNT) ntri++;
         if (valid) {
           if (OCCLUDED) {
             occluded = true;
             break;
           }
-          const float t = T / absDen;
-          const bool better = !best.found || t < best.t ||
-                              (t == best.t && (geom < best.geom || (geom == best.geom && prim < best.prim)));
+          const bool better = !best.found || h.t < best.t ||
+                              (h.t == best.t && (h.geom < best.geom || (h.geom == best.geom && h.prim < best.prim)));
           if (better) {
             best.found = true;
-            best.t = t;
-            best.u = U / absDen;
-            best.v = V / absDen;
-            best.ngx = ngx; best.ngy = ngy; best.ngz = ngz;
-            best.prim = prim; best.geom = geom;
+            best.t = h.t;
+            best.u = h.un / h.dn;
+            best.v = h.vn / h.dn;
+            best.ngx = h.ngx; best.ngy = h.ngy; best.ngz = h.ngz;
+            best.prim = h.prim; best.geom = h.geom;
           }
         }
-        if (flags & TRI_LAST) break;
+        if (h.flags & TRI_LAST) break;
         ti++;
       }
       if (OCCLUDED && occluded) break;
@@ -410,7 +438,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
 #define RTG_TRACE_MIN_BLOCKS 1
 #endif
 
-template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool FUSED>
+template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool PL>
 __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                                 uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                                 uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
@@ -486,7 +514,7 @@ __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene s
         done = true;
       } else {
         NodeHit h;
-        intersect_node<FUSED>(sc.nodes + cur, r, lim, h);
+        intersect_node(sc.nodes + cur, r, lim, h);
         if (COUNT) nvis++;
         if (h.mask) {
           if (OCCLUDED) {
@@ -527,28 +555,26 @@ __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene s
           done = true;
           break;
         }
-        float T, absDen, U, V, ngx, ngy, ngz;
-        uint32_t prim, geom, flags;
-        const bool valid = moeller(r, sc.tris + ti, T, absDen, U, V, ngx, ngy, ngz, prim, geom, flags);
+        TriTest h;
+        const bool valid = tri_test<PL>(r, sc.tris + ti, h);
         if (COUNT) ntri++;
         if (valid) {
           if (OCCLUDED) {
             occluded = true;
             break;
           }
-          const float t = T / absDen;
-          const bool better = !best.found || t < best.t ||
-                              (t == best.t && (geom < best.geom || (geom == best.geom && prim < best.prim)));
+          const bool better = !best.found || h.t < best.t ||
+                              (h.t == best.t && (h.geom < best.geom || (h.geom == best.geom && h.prim < best.prim)));
           if (better) {
             best.found = true;
-            best.t = t;
-            best.u = U / absDen;
-            best.v = V / absDen;
-            best.ngx = ngx; best.ngy = ngy; best.ngz = ngz;
-            best.prim = prim; best.geom = geom;
+            best.t = h.t;
+            best.u = h.un / h.dn;
+            best.v = h.vn / h.dn;
+            best.ngx = h.ngx; best.ngy = h.ngy; best.ngz = h.ngz;
+            best.prim = h.prim; best.geom = h.geom;
           }
         }
-        if (flags & TRI_LAST) break;
+        if (h.flags & TRI_LAST) break;
         ti++;
       }
       if (OCCLUDED && occluded) done = true;
@@ -590,212 +616,6 @@ __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene s
   }
 }
 
-constexpr uint32_t NO_REF = 0xffffffffu;
-
-template <int LDS_SLOTS>
-__device__ inline uint32_t pop_live(RingStack<LDS_SLOTS>& stack, float lim, bool occluded) {
-  while (!stack.empty()) {
-    const uint2 e = stack.pop();
-    if (occluded || __uint_as_float(e.y) <= lim) return e.x;
-  }
-  return NO_REF;
-}
-
-// Persistent while-while traversal with one postponed leaf per lane (Aila and
-// Laine, HPG 2009): in the node phase every lane descends; a lane that meets a
-// leaf parks it and keeps descending speculatively until it meets a second
-// leaf or until no lane of the wave is still searching for its first leaf;
-// then all lanes test their parked leaves together. Node-test and
-// triangle-test code thereby run with most lanes of the wave on the same
-// path. The best hit keeps only t, ids and the triangle record; u, v and Ng
-// are recomputed from that record when the ray finishes (same arithmetic,
-// same result). Rays are refilled between phases as in k_trace_p; every wave
-// exits once the counter has passed M and its lanes are idle.
-template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS>
-__global__ __launch_bounds__(TB) void k_trace_ww(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
-                                                 uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
-                                                 uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
-                                                 const uint32_t* __restrict__ perm, uint32_t* __restrict__ next,
-                                                 int refill_min) {
-  const int lane = threadIdx.x & 63;
-  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  __shared__ uint2 lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
-  RingStack<LDS_SLOTS> stack;
-  stack.init(lds_stack + threadIdx.x);
-
-  TRay r;
-  float lim = 0.0f, best_t = INFINITY;
-  uint32_t best_tri = NO_REF, best_prim = NO_REF, best_geom = NO_REF;
-  uint32_t cur = NO_REF, pleaf = NO_REF, ray = 0, nvis = 0, ntri = 0;
-  bool active = false, pool_empty = false, occluded = false, failed = false;
-  // a ray touches each node and triangle record at most once
-  const uint64_t max_work = (uint64_t)sc.num_nodes + sc.num_tris + 16;
-  uint64_t work = 0;
-
-  while (true) {
-    // ---- refill idle lanes (wave-uniform decision)
-    if (!pool_empty) {
-      const uint64_t idle = __ballot(!active);
-      const int nidle = __popcll(idle);
-      if (nidle >= refill_min) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(next, (uint32_t)nidle);
-        base = __shfl(base, 0, 64);
-        if ((uint64_t)base + nidle >= M) pool_empty = true;
-        if (!active) {
-          const uint32_t slot = base + (uint32_t)__popcll(idle & lt);
-          if (slot < M) {
-            ray = perm ? perm[slot] : slot;
-            float4 a, b;
-            float tfar;
-            load_ray<ALIGNED>(rays + (uint64_t)ray * stride, a, b, tfar);
-            if (a.w <= tfar) {
-              setup_ray(r, a, b, tfar);
-              best_t = INFINITY;
-              best_tri = best_prim = best_geom = NO_REF;
-              lim = widen_up(tfar);
-              cur = 0;
-              pleaf = NO_REF;
-              stack.init(lds_stack + threadIdx.x);
-              occluded = failed = false;
-              nvis = ntri = 0;
-              work = 0;
-              active = true;
-            } else if (COUNT) {
-              cnt_nodes[ray] = 0;
-              cnt_tris[ray] = 0;
-            }
-          }
-        }
-      }
-    }
-    if (__ballot(active) == 0) {
-      if (pool_empty) break;
-      continue;
-    }
-
-    // ---- node phase
-    while (true) {
-      bool nodework = false;
-      if (active && !failed) {
-        if (cur == NO_REF) cur = pop_live(stack, lim, OCCLUDED);
-        if (cur != NO_REF && (cur & REF_LEAF) && pleaf == NO_REF) {
-          pleaf = cur;
-          cur = pop_live(stack, lim, OCCLUDED);
-        }
-        nodework = cur != NO_REF && !(cur & REF_LEAF);
-      }
-      if (__ballot(nodework && pleaf == NO_REF) == 0) break;  // every searching lane holds a leaf
-      if (nodework) {
-        if (++work > max_work || cur >= sc.num_nodes) {
-          atomicOr(err, cur >= sc.num_nodes ? 4u : 2u);
-          failed = true;
-        } else {
-          NodeHit h;
-          intersect_node<false>(sc.nodes + cur, r, lim, h);
-          if (COUNT) nvis++;
-          cur = NO_REF;
-          if (h.mask) {
-            if (OCCLUDED) {
-              const int c0 = __builtin_ctz(h.mask);
-              uint32_t rem = h.mask & (h.mask - 1u);
-              while (rem) {
-                const int c = __builtin_ctz(rem);
-                rem &= rem - 1u;
-                if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
-              }
-              cur = child_ref(h, c0);
-            } else {
-              int bi = -1;
-              float bv = 0.0f;
-#pragma unroll
-              for (int c = 0; c < 8; c++)
-                if (((h.mask >> c) & 1u) && (bi < 0 || h.tn[c] < bv)) { bi = c; bv = h.tn[c]; }
-              uint32_t rem = h.mask & ~(1u << bi);
-              while (rem) {
-                int mc = -1;
-                float mv = 0.0f;
-#pragma unroll
-                for (int c = 0; c < 8; c++)
-                  if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
-                rem &= ~(1u << mc);
-                if (!stack.push(make_uint2(child_ref(h, mc), __float_as_uint(mv)))) atomicOr(err, 1u);
-              }
-              cur = child_ref(h, bi);
-            }
-          }
-        }
-      }
-    }
-
-    // ---- leaf phase
-    if (active && !failed && pleaf != NO_REF) {
-      uint32_t ti = pleaf & ~REF_LEAF;
-      pleaf = NO_REF;
-      bool improved = false;
-      while (true) {
-        if (ti >= sc.num_tris || ++work > max_work) {
-          atomicOr(err, ti >= sc.num_tris ? 4u : 2u);
-          failed = true;
-          break;
-        }
-        float T, absDen, U, V, ngx, ngy, ngz;
-        uint32_t prim, geom, flags;
-        const bool valid = moeller(r, sc.tris + ti, T, absDen, U, V, ngx, ngy, ngz, prim, geom, flags);
-        if (COUNT) ntri++;
-        if (valid) {
-          if (OCCLUDED) {
-            occluded = true;
-            break;
-          }
-          const float t = T / absDen;
-          const bool better = best_tri == NO_REF || t < best_t ||
-                              (t == best_t && (geom < best_geom || (geom == best_geom && prim < best_prim)));
-          if (better) {
-            best_t = t;
-            best_tri = ti;
-            best_prim = prim;
-            best_geom = geom;
-            improved = true;
-          }
-        }
-        if (flags & TRI_LAST) break;
-        ti++;
-      }
-      if (!OCCLUDED && improved) lim = widen_up(fminf(best_t, r.tfar));
-    }
-
-    // ---- rays that are finished write their record and free the lane
-    if (active) {
-      const bool finished = failed || (OCCLUDED && occluded) ||
-                            (pleaf == NO_REF && cur == NO_REF && stack.empty());
-      if (finished) {
-        char* rp = rays + (uint64_t)ray * stride;
-        if (COUNT) { cnt_nodes[ray] = nvis; cnt_tris[ray] = ntri; }
-        if (OCCLUDED) {
-          if (occluded) *reinterpret_cast<float*>(rp + 32) = -INFINITY;
-        } else if (best_tri != NO_REF && !failed) {
-          float T, absDen, U, V, ngx, ngy, ngz;
-          uint32_t prim, geom, flags;
-          moeller(r, sc.tris + best_tri, T, absDen, U, V, ngx, ngy, ngz, prim, geom, flags);
-          float* f = reinterpret_cast<float*>(rp);
-          uint32_t* u = reinterpret_cast<uint32_t*>(rp);
-          f[8] = best_t;
-          f[12] = ngx;
-          f[13] = ngy;
-          f[14] = ngz;
-          f[15] = U / absDen;
-          f[16] = V / absDen;
-          u[17] = prim;
-          u[18] = geom;
-          u[19] = inst_id;
-        }
-        active = false;
-      }
-    }
-  }
-}
-
 __global__ void k_zero_u32(uint32_t* p) { *p = 0u; }
 
 // Sort key of a ray: 30-bit Morton code of its origin in the scene box
@@ -816,6 +636,53 @@ __global__ __launch_bounds__(TB) void k_ray_keys(const char* __restrict__ rays, 
   }
   keys[i] = key;
   vals[i] = i;
+}
+
+struct LaunchArgs {
+  DevScene sc;
+  char* rays;
+  uint32_t M;
+  uint64_t stride;
+  uint32_t inst_id;
+  uint32_t *cnt_nodes, *cnt_tris, *err;
+  const uint32_t* perm;
+  uint32_t* counter;
+  int refill;
+  bool persistent;
+  dim3 grid, pgrid;
+  int lds_slots;
+  hipStream_t s;
+};
+
+template <bool OCC, bool AL, bool CNT, int SL, bool PL>
+void launch_one(const LaunchArgs& a) {
+  if (a.persistent)
+    hipLaunchKernelGGL((k_trace_p<OCC, AL, CNT, SL, PL>), a.pgrid, dim3(TB), 0, a.s, a.sc, a.rays, a.M, a.stride,
+                       a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm, a.counter, a.refill);
+  else
+    hipLaunchKernelGGL((k_trace<OCC, AL, CNT, SL, PL>), a.grid, dim3(TB), 0, a.s, a.sc, a.rays, a.M, a.stride,
+                       a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm);
+}
+
+template <bool OCC, bool AL, bool CNT, bool PL>
+void launch_slots(const LaunchArgs& a) {
+  if (a.lds_slots <= 0) launch_one<OCC, AL, CNT, 0, PL>(a);
+  else if (a.lds_slots <= 8) launch_one<OCC, AL, CNT, 8, PL>(a);
+  else launch_one<OCC, AL, CNT, 16, PL>(a);
+}
+
+template <bool PL>
+void launch_pl(const LaunchArgs& a, bool count, bool occluded, bool aligned) {
+  if (count) {
+    if (occluded) launch_slots<true, false, true, PL>(a);
+    else launch_slots<false, false, true, PL>(a);
+  } else if (occluded) {
+    if (aligned) launch_slots<true, true, false, PL>(a);
+    else launch_slots<true, false, false, PL>(a);
+  } else {
+    if (aligned) launch_slots<false, true, false, PL>(a);
+    else launch_slots<false, false, false, PL>(a);
+  }
 }
 
 void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t inst_id,
@@ -845,45 +712,10 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
     const uint32_t want = (uint32_t)tp.cus * (uint32_t)tp.blocks_per_cu;
     pgrid = dim3(want < grid.x ? want : grid.x);
   }
-#define RTG_LAUNCH_P2(OCC, AL, CNT, SL, FU)                                                              \
-  hipLaunchKernelGGL((k_trace_p<OCC, AL, CNT, SL, FU>), pgrid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,  \
-                     cnt_tris, err, perm, sorter->counter.as<uint32_t>(), refill)
-#define RTG_LAUNCH_P(OCC, AL, CNT, SL)                                                                   \
-  do {                                                                                                    \
-    if (tp.while_while)                                                                                   \
-      hipLaunchKernelGGL((k_trace_ww<OCC, AL, CNT, SL>), pgrid, block, 0, s, sc, rp, M, stride, inst_id,   \
-                         cnt_nodes, cnt_tris, err, perm, sorter->counter.as<uint32_t>(), refill);         \
-    else if (tp.fused_slab) RTG_LAUNCH_P2(OCC, AL, CNT, SL, true);                                        \
-    else RTG_LAUNCH_P2(OCC, AL, CNT, SL, false);                                                          \
-  } while (0)
-#define RTG_LAUNCH(OCC, AL, CNT)                                                                                 \
-  do {                                                                                                          \
-    if (persistent) {                                                                                           \
-      if (tp.lds_slots <= 0) RTG_LAUNCH_P(OCC, AL, CNT, 0);                                                     \
-      else if (tp.lds_slots <= 8) RTG_LAUNCH_P(OCC, AL, CNT, 8);                                                \
-      else RTG_LAUNCH_P(OCC, AL, CNT, 16);                                                                      \
-      break;                                                                                                    \
-    }                                                                                                           \
-    if (tp.lds_slots <= 0)                                                                                      \
-      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 0, true>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,   \
-                         cnt_tris, err, perm);                                                                  \
-    else if (tp.lds_slots <= 8)                                                                                 \
-      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 8, true>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,   \
-                         cnt_tris, err, perm);                                                                  \
-    else                                                                                                        \
-      hipLaunchKernelGGL((k_trace<OCC, AL, CNT, 16, true>), grid, block, 0, s, sc, rp, M, stride, inst_id, cnt_nodes,  \
-                         cnt_tris, err, perm);                                                                  \
-  } while (0)
-  if (count) {
-    if (occluded) RTG_LAUNCH(true, false, true); else RTG_LAUNCH(false, false, true);
-  } else if (occluded) {
-    if (aligned) RTG_LAUNCH(true, true, false); else RTG_LAUNCH(true, false, false);
-  } else {
-    if (aligned) RTG_LAUNCH(false, true, false); else RTG_LAUNCH(false, false, false);
-  }
-#undef RTG_LAUNCH
-#undef RTG_LAUNCH_P
-#undef RTG_LAUNCH_P2
+  LaunchArgs la{sc, rp, M, stride, inst_id, cnt_nodes, cnt_tris, err, perm,
+                persistent ? sorter->counter.as<uint32_t>() : nullptr, refill, persistent, grid, pgrid, tp.lds_slots, s};
+  if (sc.robust) launch_pl<true>(la, count, occluded, aligned);
+  else launch_pl<false>(la, count, occluded, aligned);
   RTG_HIP(hipGetLastError());
 }
 

@@ -84,6 +84,65 @@ int oracle_moeller_verts(const float org[3], const float dir[3], float tnear, fl
   return oracle_moeller(org, dir, tnear, tfar, p0, e1, e2, out);
 }
 
+/* PlueckerIntersector1::intersect (triangle_intersector_pluecker.h:55-110):
+ * vertices relative to the origin, edge functions U, V, W, the
+ * eps = ulp*|U+V+W| tolerance, stable_triangle_normal (common/math/vec3.h:210-222,
+ * msub = fma(a,b,-c)), den and T doubled (math.h:242 twice). min/max of three
+ * are min(min(a,b),c) with min(a,b) = a < b ? a : b (SSE semantics).
+ * Hit finalisation (PlueckerHitM::finalize, :29-36): u = U/UVW, v = V/UVW,
+ * or U*0, V*0 when |UVW| < min_rcp_input; t = T/den. IEEE division replaces
+ * the reference's rcp for the same reason as in the Moeller test.
+ * out[0..3] = t, U, V, UVW; out[4..6] = Ng. Returns validity for [tnear, tfar]. */
+static inline float o_min(float a, float b) { return a < b ? a : b; }
+static inline float o_max(float a, float b) { return a > b ? a : b; }
+
+int oracle_pluecker(const float org[3], const float dir[3], float tnear, float tfar,
+                    const float p0[3], const float p1[3], const float p2[3], float out[8]) {
+  float v0[3], v1[3], v2[3], e0[3], e1[3], e2[3], s[3], c[3];
+  for (int k = 0; k < 3; k++) {
+    v0[k] = p0[k] - org[k];
+    v1[k] = p1[k] - org[k];
+    v2[k] = p2[k] - org[k];
+  }
+  for (int k = 0; k < 3; k++) {
+    e0[k] = v2[k] - v0[k];
+    e1[k] = v0[k] - v1[k];
+    e2[k] = v1[k] - v2[k];
+  }
+  for (int k = 0; k < 3; k++) s[k] = v2[k] + v0[k];
+  cross3(e0, s, c);
+  const float U = dot3(c, dir);
+  for (int k = 0; k < 3; k++) s[k] = v0[k] + v1[k];
+  cross3(e1, s, c);
+  const float V = dot3(c, dir);
+  for (int k = 0; k < 3; k++) s[k] = v1[k] + v2[k];
+  cross3(e2, s, c);
+  const float W = dot3(c, dir);
+  const float UVW = U + V + W;
+  const float eps = FLT_EPSILON * fabsf(UVW);
+  int valid = (o_min(o_min(U, V), W) >= -eps) | (o_max(o_max(U, V), W) <= eps);
+  /* stable_triangle_normal(a=e0, b=e1, c=e2) */
+  const float ab_x = e0[2] * e1[1], ab_y = e0[0] * e1[2], ab_z = e0[1] * e1[0];
+  const float bc_x = e1[2] * e2[1], bc_y = e1[0] * e2[2], bc_z = e1[1] * e2[0];
+  const float cab[3] = {fmaf(e0[1], e1[2], -ab_x), fmaf(e0[2], e1[0], -ab_y), fmaf(e0[0], e1[1], -ab_z)};
+  const float cbc[3] = {fmaf(e1[1], e2[2], -bc_x), fmaf(e1[2], e2[0], -bc_y), fmaf(e1[0], e2[1], -bc_z)};
+  float Ng[3];
+  Ng[0] = fabsf(ab_x) < fabsf(bc_x) ? cab[0] : cbc[0];
+  Ng[1] = fabsf(ab_y) < fabsf(bc_y) ? cab[1] : cbc[1];
+  Ng[2] = fabsf(ab_z) < fabsf(bc_z) ? cab[2] : cbc[2];
+  const float d = dot3(Ng, dir);
+  const float den = d + d;
+  const float tt = dot3(v0, Ng);
+  const float T = tt + tt;
+  const float t = T / den;
+  valid &= (tnear <= t) & (t <= tfar);
+  valid &= den != 0.0f;
+  out[0] = t; out[1] = U; out[2] = V; out[3] = UVW;
+  out[4] = Ng[0]; out[5] = Ng[1]; out[6] = Ng[2];
+  out[7] = valid ? t : INFINITY;
+  return valid;
+}
+
 /* ---------------------------------------------------- ray record access -- */
 
 typedef struct {
@@ -123,6 +182,22 @@ static inline void o_consider(const float org[3], const float dir[3], float tnea
   b->geomID = g; b->primID = p;
 }
 
+static inline void o_consider_pl(const float org[3], const float dir[3], float tnear, float tfar,
+                                 const float p0[3], const float p1[3], const float p2[3], unsigned g,
+                                 unsigned p, o_best* b) {
+  float h[8];
+  if (!oracle_pluecker(org, dir, tnear, tfar, p0, p1, p2, h)) return;
+  const float t = h[0];
+  if (!o_better(t, g, p, b)) return;
+  const int tiny = fabsf(h[3]) < O_MIN_RCP_INPUT;
+  b->found = 1;
+  b->t = t;
+  b->u = tiny ? h[1] * 0.0f : h[1] / h[3];
+  b->v = tiny ? h[2] * 0.0f : h[2] / h[3];
+  b->Ng[0] = h[4]; b->Ng[1] = h[5]; b->Ng[2] = h[6];
+  b->geomID = g; b->primID = p;
+}
+
 /* Hit write-back (intersector_epilog.h:280-292). */
 static inline void o_write_hit(o_rayhit* r, const o_best* b, unsigned inst_id) {
   r->tfar = b->t;
@@ -157,6 +232,14 @@ static inline void o_tri_edges(const oracle_mesh* m, unsigned prim, float p0[3],
   }
 }
 
+static inline void o_tri_verts(const oracle_mesh* m, unsigned prim, const float** a, const float** b,
+                               const float** c) {
+  const unsigned* ix = m->indices + 3 * (size_t)prim;
+  *a = m->vertices + 3 * (size_t)ix[0];
+  *b = m->vertices + 3 * (size_t)ix[1];
+  *c = m->vertices + 3 * (size_t)ix[2];
+}
+
 static int o_threads(int nthreads) {
 #ifdef _OPENMP
   return nthreads > 0 ? nthreads : omp_get_max_threads();
@@ -168,8 +251,8 @@ static int o_threads(int nthreads) {
 
 /* ---------------------------------------------------------- brute force -- */
 
-void oracle_intersect_brute(const oracle_mesh* meshes, unsigned nmeshes, void* rayhits,
-                            unsigned nrays, size_t stride, unsigned inst_id, int nthreads) {
+static void o_intersect_brute(const oracle_mesh* meshes, unsigned nmeshes, void* rayhits, unsigned nrays,
+                              size_t stride, unsigned inst_id, int nthreads, int robust) {
   int nt = o_threads(nthreads);
   (void)nt;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
@@ -182,17 +265,23 @@ void oracle_intersect_brute(const oracle_mesh* meshes, unsigned nmeshes, void* r
       const oracle_mesh* m = &meshes[g];
       for (unsigned p = 0; p < m->num_triangles; p++) {
         if (!oracle_triangle_valid(m, p)) continue;
-        float p0[3], e1[3], e2[3];
-        o_tri_edges(m, p, p0, e1, e2);
-        o_consider(r->org, r->dir, r->tnear, r->tfar, p0, e1, e2, g, p, &b);
+        if (robust) {
+          const float *a, *bb, *c;
+          o_tri_verts(m, p, &a, &bb, &c);
+          o_consider_pl(r->org, r->dir, r->tnear, r->tfar, a, bb, c, g, p, &b);
+        } else {
+          float p0[3], e1[3], e2[3];
+          o_tri_edges(m, p, p0, e1, e2);
+          o_consider(r->org, r->dir, r->tnear, r->tfar, p0, e1, e2, g, p, &b);
+        }
       }
     }
     if (b.found) o_write_hit(r, &b, inst_id);
   }
 }
 
-void oracle_occluded_brute(const oracle_mesh* meshes, unsigned nmeshes, void* rays,
-                           unsigned nrays, size_t stride, int nthreads) {
+static void o_occluded_brute(const oracle_mesh* meshes, unsigned nmeshes, void* rays, unsigned nrays,
+                             size_t stride, int nthreads, int robust) {
   int nt = o_threads(nthreads);
   (void)nt;
 #pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
@@ -204,13 +293,40 @@ void oracle_occluded_brute(const oracle_mesh* meshes, unsigned nmeshes, void* ra
       const oracle_mesh* m = &meshes[g];
       for (unsigned p = 0; p < m->num_triangles && !hit; p++) {
         if (!oracle_triangle_valid(m, p)) continue;
-        float p0[3], e1[3], e2[3], h[8];
-        o_tri_edges(m, p, p0, e1, e2);
-        hit = oracle_moeller(r->org, r->dir, r->tnear, r->tfar, p0, e1, e2, h);
+        float h[8];
+        if (robust) {
+          const float *a, *bb, *c;
+          o_tri_verts(m, p, &a, &bb, &c);
+          hit = oracle_pluecker(r->org, r->dir, r->tnear, r->tfar, a, bb, c, h);
+        } else {
+          float p0[3], e1[3], e2[3];
+          o_tri_edges(m, p, p0, e1, e2);
+          hit = oracle_moeller(r->org, r->dir, r->tnear, r->tfar, p0, e1, e2, h);
+        }
       }
     }
     if (hit) r->tfar = -INFINITY;
   }
+}
+
+void oracle_intersect_brute(const oracle_mesh* meshes, unsigned nmeshes, void* rayhits,
+                            unsigned nrays, size_t stride, unsigned inst_id, int nthreads) {
+  o_intersect_brute(meshes, nmeshes, rayhits, nrays, stride, inst_id, nthreads, 0);
+}
+
+void oracle_occluded_brute(const oracle_mesh* meshes, unsigned nmeshes, void* rays,
+                           unsigned nrays, size_t stride, int nthreads) {
+  o_occluded_brute(meshes, nmeshes, rays, nrays, stride, nthreads, 0);
+}
+
+void oracle_intersect_brute_robust(const oracle_mesh* meshes, unsigned nmeshes, void* rayhits, unsigned nrays,
+                                   size_t stride, unsigned inst_id, int nthreads) {
+  o_intersect_brute(meshes, nmeshes, rayhits, nrays, stride, inst_id, nthreads, 1);
+}
+
+void oracle_occluded_brute_robust(const oracle_mesh* meshes, unsigned nmeshes, void* rays, unsigned nrays,
+                                  size_t stride, int nthreads) {
+  o_occluded_brute(meshes, nmeshes, rays, nrays, stride, nthreads, 1);
 }
 
 /* ------------------------------------------------------ binned SAH build -- */
@@ -737,7 +853,7 @@ typedef struct {
 #define O_STACK 1024
 
 static void o_trace_one(const oracle_bvh* b, o_rayhit* r, int occl, unsigned inst_id, unsigned* cn,
-                        unsigned* ct) {
+                        unsigned* ct, int robust) {
   if (!(r->tnear <= r->tfar)) return;
   if (b->root == INT32_MIN) return;
   o_tray t;
@@ -780,13 +896,18 @@ static void o_trace_one(const oracle_bvh* b, o_rayhit* r, int occl, unsigned ins
       for (size_t i = l->begin; i < l->begin + l->count; i++) {
         const float* tr = b->tri + 9 * i;
         tris++;
+        const float *pa = NULL, *pb = NULL, *pc = NULL;
+        if (robust) o_tri_verts(&b->meshes[b->prims[i].geomID], b->prims[i].primID, &pa, &pb, &pc);
         if (occl) {
           float h[8];
-          if (oracle_moeller(r->org, r->dir, tnear, tfar, tr, tr + 3, tr + 6, h)) {
+          if (robust ? oracle_pluecker(r->org, r->dir, tnear, tfar, pa, pb, pc, h)
+                     : oracle_moeller(r->org, r->dir, tnear, tfar, tr, tr + 3, tr + 6, h)) {
             r->tfar = -INFINITY;
             if (cn) { *cn = nodes; *ct = tris; }
             return;
           }
+        } else if (robust) {
+          o_consider_pl(r->org, r->dir, tnear, tfar, pa, pb, pc, b->prims[i].geomID, b->prims[i].primID, &best);
         } else {
           o_consider(r->org, r->dir, tnear, tfar, tr, tr + 3, tr + 6, b->prims[i].geomID, b->prims[i].primID,
                      &best);
@@ -805,7 +926,24 @@ void oracle_intersect_bvh(const oracle_bvh* b, void* rayhits, unsigned nrays, si
   (void)nt;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
   for (long i = 0; i < (long)nrays; i++)
-    o_trace_one(b, (o_rayhit*)((char*)rayhits + (size_t)i * stride), 0, inst_id, NULL, NULL);
+    o_trace_one(b, (o_rayhit*)((char*)rayhits + (size_t)i * stride), 0, inst_id, NULL, NULL, 0);
+}
+
+void oracle_intersect_bvh_robust(const oracle_bvh* b, void* rayhits, unsigned nrays, size_t stride,
+                                 unsigned inst_id, int nthreads) {
+  int nt = o_threads(nthreads);
+  (void)nt;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+  for (long i = 0; i < (long)nrays; i++)
+    o_trace_one(b, (o_rayhit*)((char*)rayhits + (size_t)i * stride), 0, inst_id, NULL, NULL, 1);
+}
+
+void oracle_occluded_bvh_robust(const oracle_bvh* b, void* rays, unsigned nrays, size_t stride, int nthreads) {
+  int nt = o_threads(nthreads);
+  (void)nt;
+#pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
+  for (long i = 0; i < (long)nrays; i++)
+    o_trace_one(b, (o_rayhit*)((char*)rays + (size_t)i * stride), 1, 0, NULL, NULL, 1);
 }
 
 void oracle_occluded_bvh(const oracle_bvh* b, void* rays, unsigned nrays, size_t stride, int nthreads) {
@@ -813,7 +951,7 @@ void oracle_occluded_bvh(const oracle_bvh* b, void* rays, unsigned nrays, size_t
   (void)nt;
 #pragma omp parallel for schedule(dynamic, 256) num_threads(nt)
   for (long i = 0; i < (long)nrays; i++)
-    o_trace_one(b, (o_rayhit*)((char*)rays + (size_t)i * stride), 1, 0, NULL, NULL);
+    o_trace_one(b, (o_rayhit*)((char*)rays + (size_t)i * stride), 1, 0, NULL, NULL, 0);
 }
 
 void oracle_intersect_bvh_counted(const oracle_bvh* b, void* rayhits, unsigned nrays, size_t stride,
@@ -824,7 +962,7 @@ void oracle_intersect_bvh_counted(const oracle_bvh* b, void* rayhits, unsigned n
   for (long i = 0; i < (long)nrays; i++) {
     nodes[i] = tris[i] = 0;
     o_trace_one(b, (o_rayhit*)((char*)rayhits + (size_t)i * stride), 0, 0xFFFFFFFFu, &nodes[i],
-                &tris[i]);
+                &tris[i], 0);
   }
 }
 

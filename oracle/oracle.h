@@ -5,6 +5,10 @@
  * the GPU library against. Nothing in the product links, loads or calls it.
  *
  * What it restates (reference files, relative to the reference root):
+ *   - Pluecker triangle test of robust scenes, same operation sequence:
+ *       kernels/geometry/triangle_intersector_pluecker.h:23-110,
+ *       stable_triangle_normal common/math/vec3.h:210-222, ulp = FLT_EPSILON
+ *       (common/math/constants.h:130-136).
  *   - Moeller-Trumbore triangle test, exact FP32 operation sequence:
  *       kernels/geometry/triangle_intersector_moeller.h:62-103 (1-ray form),
  *       :288-330 (k-th ray of a packet, same arithmetic), cross/dot as
@@ -75,6 +79,14 @@ int oracle_moeller(const float org[3], const float dir[3], float tnear, float tf
 int oracle_moeller_verts(const float org[3], const float dir[3], float tnear, float tfar,
                          const float p0[3], const float p1[3], const float p2[3], float out[8]);
 
+/* Pluecker (robust) test of one ray against the triangle p0, p1, p2
+ * (kernels/geometry/triangle_intersector_pluecker.h:55-110, finalize :29-36).
+ * out[0..3] = t, U, V, U+V+W; out[4..6] = Ng (stable_triangle_normal,
+ * common/math/vec3.h:210-222). t = T/den, u = U/UVW, v = V/UVW with IEEE
+ * division (the reference uses rcp). Returns 1 when tnear <= t <= tfar. */
+int oracle_pluecker(const float org[3], const float dir[3], float tnear, float tfar,
+                    const float p0[3], const float p1[3], const float p2[3], float out[8]);
+
 /* 1 if triangle prim of mesh is valid (index range and finite vertices). */
 int oracle_triangle_valid(const oracle_mesh* m, unsigned prim);
 
@@ -87,6 +99,12 @@ void oracle_intersect_brute(const oracle_mesh* meshes, unsigned nmeshes, void* r
 /* Brute-force occlusion; rays are RTCRay records (48 B) spaced `stride`. */
 void oracle_occluded_brute(const oracle_mesh* meshes, unsigned nmeshes, void* rays,
                            unsigned nrays, size_t stride, int nthreads);
+
+/* Same with the Pluecker test (RTC_SCENE_FLAG_ROBUST scenes). */
+void oracle_intersect_brute_robust(const oracle_mesh* meshes, unsigned nmeshes, void* rayhits,
+                                   unsigned nrays, size_t stride, unsigned inst_id, int nthreads);
+void oracle_occluded_brute_robust(const oracle_mesh* meshes, unsigned nmeshes, void* rays,
+                                  unsigned nrays, size_t stride, int nthreads);
 
 typedef struct oracle_bvh oracle_bvh;
 
@@ -110,6 +128,12 @@ void oracle_intersect_bvh(const oracle_bvh* bvh, void* rayhits, unsigned nrays, 
                           unsigned inst_id, int nthreads);
 void oracle_occluded_bvh(const oracle_bvh* bvh, void* rays, unsigned nrays, size_t stride,
                          int nthreads);
+
+/* Same with the Pluecker test. */
+void oracle_intersect_bvh_robust(const oracle_bvh* bvh, void* rayhits, unsigned nrays, size_t stride,
+                                 unsigned inst_id, int nthreads);
+void oracle_occluded_bvh_robust(const oracle_bvh* bvh, void* rays, unsigned nrays, size_t stride,
+                                int nthreads);
 
 /* Per-ray traversal counters of the last oracle_intersect_bvh call pattern:
  * fills nodes[i] and tris[i] (node visits and triangle tests per ray). */

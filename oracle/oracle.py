@@ -32,7 +32,13 @@ def lib():
         L.oracle_moeller.restype = i32
         L.oracle_moeller_verts.argtypes = [fp, fp, ctypes.c_float, ctypes.c_float, fp, fp, fp, fp]
         L.oracle_moeller_verts.restype = i32
+        L.oracle_pluecker.argtypes = [fp, fp, ctypes.c_float, ctypes.c_float, fp, fp, fp, fp]
+        L.oracle_pluecker.restype = i32
         L.oracle_intersect_brute.argtypes = [vp, u32, vp, u32, sz, u32, i32]
+        L.oracle_intersect_brute_robust.argtypes = [vp, u32, vp, u32, sz, u32, i32]
+        L.oracle_occluded_brute_robust.argtypes = [vp, u32, vp, u32, sz, i32]
+        L.oracle_intersect_bvh_robust.argtypes = [vp, vp, u32, sz, u32, i32]
+        L.oracle_occluded_bvh_robust.argtypes = [vp, vp, u32, sz, i32]
         L.oracle_occluded_brute.argtypes = [vp, u32, vp, u32, sz, i32]
         L.oracle_build_sah.argtypes = [vp, u32]
         L.oracle_build_sah.restype = vp
@@ -77,6 +83,15 @@ def moeller_verts(org, dir, tnear, tfar, p0, p1, p2):
     return bool(v), out
 
 
+def pluecker(org, dir, tnear, tfar, p0, p1, p2):
+    """Robust test; returns (valid, [t, U, V, UVW, Ngx, Ngy, Ngz, t or inf])."""
+    o, po = _f3(org); d, pd = _f3(dir); a, pa = _f3(p0); b, pb = _f3(p1); c, pc = _f3(p2)
+    out = np.zeros(8, np.float32)
+    v = lib().oracle_pluecker(po, pd, float(tnear), float(tfar), pa, pb, pc,
+                              out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return bool(v), out
+
+
 class Meshes:
     """Keeps numpy arrays alive for the meshes passed to the oracle."""
 
@@ -95,16 +110,16 @@ class Meshes:
         self.n = len(meshes)
 
 
-def intersect_brute(meshes, rayhits, inst_id=0xFFFFFFFF, nthreads=0):
+def intersect_brute(meshes, rayhits, inst_id=0xFFFFFFFF, nthreads=0, robust=False):
     m = Meshes(meshes)
-    lib().oracle_intersect_brute(ctypes.addressof(m.c), m.n, rayhits.ctypes.data, len(rayhits),
-                                 rayhits.itemsize, inst_id, nthreads)
+    f = lib().oracle_intersect_brute_robust if robust else lib().oracle_intersect_brute
+    f(ctypes.addressof(m.c), m.n, rayhits.ctypes.data, len(rayhits), rayhits.itemsize, inst_id, nthreads)
 
 
-def occluded_brute(meshes, rays, nthreads=0):
+def occluded_brute(meshes, rays, nthreads=0, robust=False):
     m = Meshes(meshes)
-    lib().oracle_occluded_brute(ctypes.addressof(m.c), m.n, rays.ctypes.data, len(rays), rays.itemsize,
-                                nthreads)
+    f = lib().oracle_occluded_brute_robust if robust else lib().oracle_occluded_brute
+    f(ctypes.addressof(m.c), m.n, rays.ctypes.data, len(rays), rays.itemsize, nthreads)
 
 
 class BVH:
@@ -129,12 +144,13 @@ class BVH:
     def validate(self):
         return lib().oracle_bvh_validate(self.h)
 
-    def intersect(self, rayhits, inst_id=0xFFFFFFFF, nthreads=0):
-        lib().oracle_intersect_bvh(self.h, rayhits.ctypes.data, len(rayhits), rayhits.itemsize, inst_id,
-                                   nthreads)
+    def intersect(self, rayhits, inst_id=0xFFFFFFFF, nthreads=0, robust=False):
+        f = lib().oracle_intersect_bvh_robust if robust else lib().oracle_intersect_bvh
+        f(self.h, rayhits.ctypes.data, len(rayhits), rayhits.itemsize, inst_id, nthreads)
 
-    def occluded(self, rays, nthreads=0):
-        lib().oracle_occluded_bvh(self.h, rays.ctypes.data, len(rays), rays.itemsize, nthreads)
+    def occluded(self, rays, nthreads=0, robust=False):
+        f = lib().oracle_occluded_bvh_robust if robust else lib().oracle_occluded_bvh
+        f(self.h, rays.ctypes.data, len(rays), rays.itemsize, nthreads)
 
     def intersect_counted(self, rayhits, nthreads=0):
         n = np.zeros(len(rayhits), np.uint32)

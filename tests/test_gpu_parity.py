@@ -26,14 +26,14 @@ def trace(rtc, scene, rays, occluded=False, stride=None):
     return r
 
 
-def oracle_trace(oracle, meshes, rays, occluded=False, brute=False):
+def oracle_trace(oracle, meshes, rays, occluded=False, brute=False, robust=False):
     r = rays.copy()
     if brute:
-        (oracle.occluded_brute if occluded else oracle.intersect_brute)(meshes, r)
+        (oracle.occluded_brute if occluded else oracle.intersect_brute)(meshes, r, robust=robust)
     else:
         bvh = oracle.BVH(meshes)
         assert bvh.validate() == 0
-        (bvh.occluded if occluded else bvh.intersect)(r)
+        (bvh.occluded if occluded else bvh.intersect)(r, robust=robust)
     return r
 
 
@@ -436,4 +436,62 @@ def test_c2_full_size_parity(rtc, device, oracle):
     bvh.occluded(occ)
     assert_same(o_gpu, occ, "C2 occlusion")
     assert 0.5 < hitrate < 1.0
+    sc.release()
+
+
+# ------------------------------------------ robust scenes (Pluecker test) --
+
+ROBUST = 1 << 2  # RTC_SCENE_FLAG_ROBUST
+
+
+def test_robust_single_triangle(rtc, device, oracle):
+    v = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    ix = np.array([[0, 1, 2]], np.uint32)
+    sc = rtc.Scene(device, [(v, ix)], flags=ROBUST).commit()
+    r = rtc.new_rayhits(2)
+    r["org_x"], r["org_y"], r["org_z"], r["dir_z"] = 0.25, 0.25, -1.0, 1.0
+    r["tnear"][1] = 1.0  # t == tnear is inside for the Pluecker test
+    out = trace(rtc, sc, r)
+    assert (out["geomID"] == 0).all() and (out["tfar"] == 1.0).all()
+    assert (out["u"] == 0.25).all() and (out["v"] == 0.25).all() and (out["Ng_z"] == 1.0).all()
+    assert_same(out, oracle_trace(oracle, [(v, ix)], r, brute=True, robust=True), "robust single triangle")
+    sc.release()
+
+
+@pytest.mark.parametrize("builder", ["morton", "sah"])
+def test_robust_soup_parity(rtc, oracle, builder):
+    d = rtc.Device(f"builder={builder}")
+    v, ix = soup_case(rtc, 20000, 41, h=0.02)
+    sc = rtc.Scene(d, [(v, ix)], flags=ROBUST).commit()
+    rays = rtc.gen_random_rays(100000, seed=42, lo=-0.05, hi=1.05)
+    rays["tfar"][::5] = 0.05
+    rays["tnear"][1::9] = 0.02
+    rays["dir_x"][3::17] = np.nan
+    rays["dir_y"][5::23] = 0.0
+    rays["dir_z"][5::23] = 0.0
+    out = trace(rtc, sc, rays)
+    assert_same(out, oracle_trace(oracle, [(v, ix)], rays, robust=True), f"robust soup {builder}")
+    occ = np.zeros(len(rays), rtc.RTCRay)
+    for k in rtc.RTCRay.names:
+        occ[k] = rays[k]
+    occ["tfar"] = np.minimum(occ["tfar"], 0.1)
+    assert_same(trace(rtc, sc, occ, occluded=True), oracle_trace(oracle, [(v, ix)], occ, occluded=True, robust=True),
+                f"robust occlusion {builder}")
+    sc.release()
+    d.release()
+
+
+def test_robust_watertight_c1_sphere(rtc, device, oracle):
+    """Rays from the centre of the closed C1 sphere through every edge
+    midpoint: none misses, and every record equals the oracle's."""
+    v, ix = rtc.gen_sphere(128)
+    e = np.concatenate([ix[:, [0, 1]], ix[:, [1, 2]], ix[:, [2, 0]]])
+    mids = (0.5 * (v[e[:, 0]] + v[e[:, 1]])).astype(np.float32)
+    r = rtc.new_rayhits(len(mids))
+    r["org_x"] = r["org_y"] = r["org_z"] = 0.0
+    r["dir_x"], r["dir_y"], r["dir_z"] = mids[:, 0], mids[:, 1], mids[:, 2]
+    sc = rtc.Scene(device, [(v, ix)], flags=ROBUST).commit()
+    out = trace(rtc, sc, r)
+    assert (out["geomID"] == 0).all()
+    assert_same(out, oracle_trace(oracle, [(v, ix)], r, robust=True), "robust watertight")
     sc.release()

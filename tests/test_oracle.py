@@ -332,3 +332,117 @@ def test_morton_interleave_textbook(oracle):
     lo = np.array([[0, 0, 0], [1, 1, 1], [0.5, 0, 0]], np.float32)
     codes = oracle.morton_codes(lo, lo)
     assert codes[0] == 0 and codes[1] == oracle.bit_interleave(1013, 1013, 1013)
+
+
+# ---------------------------------------------------- Pluecker (robust mode) --
+
+def test_pluecker_known_answer(oracle):
+    """TriangleHitTest (verify.cpp:2339-2424) with the robust intersector: rays
+    from (0,0,-1) through interior points of the unit triangle hit at t=1 with
+    the point's barycentrics; U+V+W and both doubled terms are exact here."""
+    rng = np.random.default_rng(8)
+    u0 = rng.uniform(0.05, 0.9, 256).astype(np.float32)
+    v0 = (rng.uniform(0.05, 0.9, 256) * (1 - u0) * 0.95).astype(np.float32)
+    d = np.stack([u0, v0, np.ones_like(u0)], -1).astype(np.float32)
+    r = rays([0.0, 0.0, -1.0], d)
+    tri = (np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32), np.array([[0, 1, 2]], np.uint32))
+    oracle.intersect_brute([tri], r, robust=True)
+    ulp = 16 * np.finfo(np.float32).eps
+    assert (r["geomID"] == 0).all() and (r["primID"] == 0).all()
+    assert (r["tfar"] == 1.0).all()
+    assert np.abs(r["u"] - u0).max() <= ulp and np.abs(r["v"] - v0).max() <= ulp
+    assert (r["Ng_x"] == 0).all() and (r["Ng_y"] == 0).all() and (r["Ng_z"] == 1).all()
+    # by hand for the point (0.25, 0.25): U = V = 0.5, W = 1, UVW = 2, den = T = 2
+    ok, h = oracle.pluecker([0.25, 0.25, -1.0], [0, 0, 1], 0.0, np.inf, *tri[0])
+    assert ok and list(h[:4]) == [1.0, 0.5, 0.5, 2.0] and list(h[4:7]) == [0.0, 0.0, 1.0]
+
+
+def test_pluecker_special_cases(oracle):
+    tri = [np.array(p, np.float32) for p in ([0, 0, 1], [1, 0, 1], [0, 1, 1])]
+    # t == tnear and t == tfar are both inside (tnear <= t <= tfar), unlike
+    # Moeller's strict lower bound
+    ok, h = oracle.pluecker([0.2, 0.2, 0.0], [0, 0, 1], 1.0, 1.0, *tri)
+    assert ok and h[0] == 1.0
+    assert not oracle.moeller_verts([0.2, 0.2, 0.0], [0, 0, 1], 1.0, 1.0, *tri)[0]
+    assert not oracle.pluecker([0.2, 0.2, 0.0], [0, 0, 1], 1.0000001, np.inf, *tri)[0]
+    # parallel ray (den == 0) misses; back side hits (no culling)
+    assert not oracle.pluecker([0.2, 0.2, 0.0], [1, 0, 0], 0.0, np.inf, *tri)[0]
+    ok, h = oracle.pluecker([0.2, 0.2, 2.0], [0, 0, -1], 0.0, np.inf, *tri)
+    assert ok and h[0] == 1.0
+    # a ray through a shared edge hits both triangles of a quad; through the
+    # shared vertex of a fan it hits at least one
+    quad = [np.array(p, np.float32) for p in ([0, 0, 1], [1, 0, 1], [1, 1, 1], [0, 1, 1])]
+    for org in ([0.5, 0.5, 0.0], [0.25, 0.25, 0.0], [0.0, 0.0, 0.0], [1.0, 1.0, 0.0]):
+        a = oracle.pluecker(org, [0, 0, 1], 0.0, np.inf, quad[0], quad[1], quad[2])[0]
+        b = oracle.pluecker(org, [0, 0, 1], 0.0, np.inf, quad[0], quad[2], quad[3])[0]
+        assert a and b, org
+
+
+@pytest.mark.parametrize("segments", [50, 200])
+def test_pluecker_watertight_vertices_and_edges(oracle, segments):
+    """Rays from the centre of the closed UV sphere through every edge
+    midpoint never miss with the robust test: the two triangles sharing an
+    edge compute its edge function as exact negatives, so one side always
+    accepts (the watertightness the reference claims for Pluecker,
+    triangle_intersector_pluecker.h:11-16). Through vertices there is no such
+    guarantee; misses stay rare and far below Moeller's."""
+    v, ix = uv_sphere(segments)
+    e = np.concatenate([ix[:, [0, 1]], ix[:, [1, 2]], ix[:, [2, 0]]])
+    mids = (0.5 * (v[e[:, 0]] + v[e[:, 1]])).astype(np.float32)
+    verts = v[np.unique(ix)]
+    bvh = oracle.BVH([(v, ix)])
+    rm = rays([0.0, 0.0, 0.0], mids)
+    bvh.intersect(rm, robust=True)
+    assert (rm["geomID"] == 0).all()
+    rv = rays([0.0, 0.0, 0.0], verts)
+    bvh.intersect(rv, robust=True)
+    robust_vertex_misses = int((rv["geomID"] == INV).sum())
+    assert robust_vertex_misses <= 0.05 * len(verts)
+    mo = rays([0.0, 0.0, 0.0], np.concatenate([mids, verts]))
+    bvh.intersect(mo)
+    assert int((mo["geomID"] == INV).sum()) > 10 * max(1, robust_vertex_misses)
+    rb = rays([0.0, 0.0, 0.0], mids[::97])
+    oracle.intersect_brute([(v, ix)], rb, robust=True)
+    assert records_equal(rm[::97].copy(), rb)
+
+
+def test_pluecker_agrees_with_moeller_off_edges(oracle):
+    """Two different formulas for the same hit: same triangle for rays not
+    grazing an edge, t within 1e-6 relative, u and v within 1e-5."""
+    v, ix = soup(3000, 31, h=0.05)
+    r = random_rays(20000, 32)
+    a, b = r.copy(), r.copy()
+    oracle.intersect_brute([(v, ix)], a)
+    oracle.intersect_brute([(v, ix)], b, robust=True)
+    both = (a["geomID"] == 0) & (b["geomID"] == 0)
+    same = both & (a["primID"] == b["primID"])
+    assert same.sum() >= 0.999 * both.sum() and both.sum() > 1000
+    assert ((a["geomID"] == INV) == (b["geomID"] == INV)).mean() > 0.999
+    # both carry an absolute error of a few coordinate ulps, so t is compared
+    # relatively except where the origin nearly touches the triangle
+    ta, tb = a["tfar"][same], b["tfar"][same]
+    assert (np.abs(ta - tb) <= 1e-6 * ta + 1e-6).all()
+    assert np.percentile(np.abs(ta - tb) / ta, 99) <= 1e-6
+    du, dv = np.abs(a["u"][same] - b["u"][same]), np.abs(a["v"][same] - b["v"][same])
+    assert du.max() <= 1e-4 and dv.max() <= 1e-4
+    assert np.percentile(du, 99) <= 1e-5 and np.percentile(dv, 99) <= 1e-5
+    # Ng: Moeller cross(e2,e1) and the stable normal are the same vector up to rounding
+    na = np.stack([a["Ng_x"], a["Ng_y"], a["Ng_z"]], -1)[same]
+    nb = np.stack([b["Ng_x"], b["Ng_y"], b["Ng_z"]], -1)[same]
+    assert np.abs(na - nb).max() <= 1e-5 * np.abs(na).max()
+
+
+@pytest.mark.parametrize("occluded", [False, True])
+def test_robust_bvh_equals_brute_force(oracle, occluded):
+    v, ix = soup(5000, 33, h=0.03)
+    r = random_rays(20000, 34, tfar=0.2 if occluded else np.inf)
+    bvh = oracle.BVH([(v, ix)])
+    a, b = r.copy(), r.copy()
+    if occluded:
+        bvh.occluded(a, robust=True)
+        oracle.occluded_brute([(v, ix)], b, robust=True)
+        assert 0.05 < np.isneginf(a["tfar"]).mean() < 0.95
+    else:
+        bvh.intersect(a, robust=True)
+        oracle.intersect_brute([(v, ix)], b, robust=True)
+    assert records_equal(a, b)
