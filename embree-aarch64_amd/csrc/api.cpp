@@ -161,6 +161,17 @@ struct Geometry : RefObj {
   }
 };
 
+// ----------------------------------------------------------- split model --
+
+struct SplitModelH : RefObj {
+  Device* dev;
+  rtg::SplitModel m;
+  uint32_t min_count = 64;
+  std::mutex mtx;
+  explicit SplitModelH(Device* d) : dev(d) { ref_inc(d); }
+  ~SplitModelH() override { ref_dec(dev); }
+};
+
 // ----------------------------------------------------------------- scene --
 
 struct Scene : RefObj {
@@ -175,8 +186,10 @@ struct Scene : RefObj {
   rtg::DeviceBVH bvh;
   std::vector<rtg::DevBuf*> gdata;
   std::vector<rtg::GeomDesc> descs;
+  SplitModelH* model = nullptr;  // learned split cost for the next commits (retained)
   explicit Scene(Device* d) : dev(d) { ref_inc(d); }
   ~Scene() override {
+    if (model) ref_dec(model);
     for (Geometry* g : geoms)
       if (g) ref_dec(g);
     for (auto* b : gdata) delete b;
@@ -188,6 +201,7 @@ Device* dev_of(Device* d) { return d; }
 Device* dev_of(Buffer* b) { return b ? b->dev : nullptr; }
 Device* dev_of(Geometry* g) { return g ? g->dev : nullptr; }
 Device* dev_of(Scene* s) { return s ? s->dev : nullptr; }
+Device* dev_of(SplitModelH* m) { return m ? m->dev : nullptr; }
 
 #define API_BEGIN try {
 #define API_END(obj)                                                                                 \
@@ -342,7 +356,17 @@ void commit_scene(Scene* sc) {
   }
   {
     std::lock_guard<std::mutex> bl(d->build_mtx);
-    const rtg::BuildParams p = build_params(d, sc->quality, sc->flags);
+    rtg::BuildParams p = build_params(d, sc->quality, sc->flags);
+    rtg::LearnedCost lc{};
+    if (sc->model) {
+      std::lock_guard<std::mutex> ml(sc->model->mtx);
+      if (!sc->model->m.has_norm) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "split model has no training data");
+      lc.P = sc->model->m.packed.as<rtg::MlpPacked>();
+      lc.sn = sc->model->m.norm;
+      lc.min_count = sc->model->min_count;
+      p.learned = &lc;
+      p.builder = rtg::BUILDER_SAH;
+    }
     d->monitor((ssize_t)prim_offset * (80 + 48));
     rtg::build_bvh(sc->descs, p, d->scratch, sc->bvh, t.stream);
   }
@@ -1148,6 +1172,241 @@ RTC_API void rtcgReadBVH(RTCScene scene, void* nodes, void* tris) {
     RTG_HIP(hipMemcpyAsync(tris, s->bvh.tris.ptr, (size_t)st.num_prims * sizeof(rtg::TriRec), hipMemcpyDeviceToHost,
                            t.stream));
   RTG_HIP(hipStreamSynchronize(t.stream));
+  API_END(s)
+}
+
+// ------------------------------------------------------ learned split cost --
+
+}  // extern "C"
+
+namespace {
+SplitModelH* SM(RTCGSplitModel m) { return (SplitModelH*)m; }
+
+rtg::SceneNorm scene_norm(const rtg::BuildStats& st) {
+  rtg::SceneNorm sn;
+  float e[3];
+  for (int k = 0; k < 3; k++) {
+    sn.lo[k] = st.bounds_lo[k];
+    e[k] = st.bounds_hi[k] - st.bounds_lo[k];
+    sn.inv_ext[k] = e[k] > 0.0f ? 1.0f / e[k] : 0.0f;
+  }
+  const float a = rtg::half_area(e[0], e[1], e[2]);
+  sn.inv_area = a > 0.0f ? 1.0f / a : 0.0f;
+  return sn;
+}
+
+// Host array -> device scratch.
+template <class T>
+T* upload(rtg::DevBuf& b, const T* host, size_t n, hipStream_t s) {
+  b.reserve(n * sizeof(T) + 16);
+  if (n) RTG_HIP(hipMemcpyAsync(b.ptr, host, n * sizeof(T), hipMemcpyHostToDevice, s));
+  return b.as<T>();
+}
+}  // namespace
+
+extern "C" {
+
+RTC_API RTCGSplitModel rtcgNewSplitModel(RTCDevice device, unsigned int seed) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  d->activate();
+  ThreadDev& t = d->td();
+  auto* h = new SplitModelH(d);
+  try {
+    rtg::mlp_init(h->m, seed, t.stream);
+    RTG_HIP(hipStreamSynchronize(t.stream));
+  } catch (...) {
+    ref_dec(h);
+    throw;
+  }
+  return (RTCGSplitModel)h;
+  API_END(d)
+  return nullptr;
+}
+
+RTC_API void rtcgRetainSplitModel(RTCGSplitModel model) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  ref_inc(m);
+  API_END(m)
+}
+
+RTC_API void rtcgReleaseSplitModel(RTCGSplitModel model) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  ref_dec(m);
+  API_END((SplitModelH*)nullptr)
+}
+
+RTC_API unsigned int rtcgSplitModelParamCount(void) { return (unsigned)rtg::MLP_NPARAM; }
+
+RTC_API void rtcgSplitModelSetParams(RTCGSplitModel model, const float* params) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  VERIFY_HANDLE(params);
+  std::lock_guard<std::mutex> lk(m->mtx);
+  m->dev->activate();
+  ThreadDev& t = m->dev->td();
+  rtg::mlp_set_params(m->m, params, t.stream);
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  API_END(m)
+}
+
+RTC_API void rtcgSplitModelGetParams(RTCGSplitModel model, float* params) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  VERIFY_HANDLE(params);
+  std::lock_guard<std::mutex> lk(m->mtx);
+  m->dev->activate();
+  rtg::mlp_get_params(m->m, params, m->dev->td().stream);
+  API_END(m)
+}
+
+RTC_API void rtcgSplitModelPredict(RTCGSplitModel model, const float* features, unsigned int n, float* out) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  if (n && (!features || !out)) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  std::lock_guard<std::mutex> lk(m->mtx);
+  m->dev->activate();
+  ThreadDev& t = m->dev->td();
+  rtg::DevBuf dx, dy;
+  const float* x = upload(dx, features, (size_t)n * rtg::MLP_IN, t.stream);
+  dy.reserve((size_t)n * 4 + 16);
+  rtg::mlp_predict(m->m, x, n, dy.as<float>(), t.stream);
+  if (n) RTG_HIP(hipMemcpyAsync(out, dy.ptr, (size_t)n * 4, hipMemcpyDeviceToHost, t.stream));
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  API_END(m)
+}
+
+RTC_API float rtcgSplitModelGradient(RTCGSplitModel model, const float* features, const float* targets,
+                                     const float* weights, unsigned int n, float* grad) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  VERIFY_HANDLE(grad);
+  if (n && (!features || !targets || !weights)) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  std::lock_guard<std::mutex> lk(m->mtx);
+  m->dev->activate();
+  ThreadDev& t = m->dev->td();
+  rtg::DevBuf dx, dt, dw;
+  const float* x = upload(dx, features, (size_t)n * rtg::MLP_IN, t.stream);
+  const float* tt = upload(dt, targets, n, t.stream);
+  const float* w = upload(dw, weights, n, t.stream);
+  const float loss = rtg::mlp_gradient(m->m, x, tt, w, nullptr, n, t.stream);
+  RTG_HIP(hipMemcpyAsync(grad, m->m.grad.ptr, rtg::MLP_NPARAM * 4, hipMemcpyDeviceToHost, t.stream));
+  RTG_HIP(hipMemsetAsync(m->m.grad.ptr, 0, rtg::MLP_NPARAM * 4, t.stream));
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  return loss;
+  API_END(m)
+  return 0.0f;
+}
+
+RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const void* d_rays, unsigned int M,
+                                   size_t byteStride) {
+  SplitModelH* m = SM(model);
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  VERIFY_HANDLE(scene);
+  if (M && !d_rays) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  if (byteStride < sizeof(RTCRayHit) || byteStride % 4) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid stride");
+  if (M == 0 || s->bvh.stats.num_nodes == 0) return;
+  std::lock_guard<std::mutex> lk(m->mtx);
+  std::lock_guard<std::mutex> slk(s->mtx);
+  Device* d = s->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  const uint32_t nn = s->bvh.stats.num_nodes;
+  rtg::DevBuf copy;
+  copy.reserve((size_t)M * byteStride);
+  RTG_HIP(hipMemcpyAsync(copy.ptr, d_rays, (size_t)M * byteStride, hipMemcpyDeviceToDevice, t.stream));
+  m->m.visits.reserve((size_t)nn * 4);
+  m->m.counts.reserve((size_t)M * 8);
+  RTG_HIP(hipMemsetAsync(m->m.visits.ptr, 0, (size_t)nn * 4, t.stream));
+  uint32_t* cn = m->m.counts.as<uint32_t>();
+  rtg::launch_trace(s->bvh.view(), copy.ptr, M, byteStride, false, RTC_INVALID_GEOMETRY_ID, cn, cn + M,
+                    t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter, m->m.visits.as<uint32_t>());
+  const rtg::SceneNorm sn = m->m.has_norm ? m->m.norm : scene_norm(s->bvh.stats);
+  rtg::mlp_append_samples(m->m, (const float*)s->bvh.info.ptr, m->m.visits.as<uint32_t>(), nn, M, sn, t.stream);
+  m->m.norm = sn;
+  m->m.has_norm = true;
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  check_trace_error(t);
+  API_END(m)
+}
+
+RTC_API unsigned int rtcgSplitModelSampleCount(RTCGSplitModel model) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  return m->m.nsamples;
+  API_END(m)
+  return 0;
+}
+
+RTC_API void rtcgSplitModelGetSamples(RTCGSplitModel model, float* features, float* targets) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  std::lock_guard<std::mutex> lk(m->mtx);
+  m->dev->activate();
+  ThreadDev& t = m->dev->td();
+  const size_t n = m->m.nsamples;
+  if (n && features) RTG_HIP(hipMemcpyAsync(features, m->m.X.ptr, n * rtg::MLP_IN * 4, hipMemcpyDeviceToHost, t.stream));
+  if (n && targets) RTG_HIP(hipMemcpyAsync(targets, m->m.T.ptr, n * 4, hipMemcpyDeviceToHost, t.stream));
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  API_END(m)
+}
+
+RTC_API void rtcgSplitModelClearSamples(RTCGSplitModel model) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  std::lock_guard<std::mutex> lk(m->mtx);
+  m->m.nsamples = 0;
+  m->m.has_norm = false;
+  API_END(m)
+}
+
+RTC_API float rtcgSplitModelTrain(RTCGSplitModel model, unsigned int steps, unsigned int batch, float learning_rate,
+                                  unsigned int seed) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  if (!(learning_rate > 0.0f)) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "learning rate must be positive");
+  std::lock_guard<std::mutex> lk(m->mtx);
+  if (m->m.nsamples == 0) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "split model has no training data");
+  m->dev->activate();
+  return rtg::mlp_train(m->m, steps, batch, learning_rate, seed, m->dev->td().stream);
+  API_END(m)
+  return 0.0f;
+}
+
+RTC_API void rtcgSetSplitModelMinCount(RTCGSplitModel model, unsigned int min_count) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  m->min_count = min_count < 64 ? 64 : min_count;
+  API_END(m)
+}
+
+RTC_API void rtcgSetSceneSplitModel(RTCScene scene, RTCGSplitModel model) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  SplitModelH* m = SM(model);
+  if (m && m->dev != s->dev) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "split model belongs to another device");
+  std::lock_guard<std::mutex> lk(s->mtx);
+  if (m) ref_inc(m);
+  if (s->model) ref_dec(s->model);
+  s->model = m;
   API_END(s)
 }
 

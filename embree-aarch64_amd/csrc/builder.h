@@ -6,6 +6,7 @@
 #include <vector>
 
 #include "builder_sah.h"
+#include "mlp.h"
 #include "bvh_types.h"
 #include "gpu_util.h"
 #include "sort.h"
@@ -21,6 +22,7 @@ struct BuildParams {
   float cost_tri = 0.45f;      // SAH cost of one triangle test, per unit half area
   int quality = 1;             // RTCBuildQuality (LOW: Morton tree, HIGH: binned SAH, MEDIUM: `builder`)
   int robust = 0;              // leaves hold (v0, v1, v2) for the Pluecker test
+  const LearnedCost* learned = nullptr;  // binned-SAH split cost from a trained model (device data)
 };
 
 struct BuildStats {
@@ -38,7 +40,7 @@ struct BuildStats {
 
 // Owns the device arrays of one built scene.
 struct DeviceBVH {
-  DevBuf nodes, tris;
+  DevBuf nodes, tris, info;
   DevScene view() const {
     DevScene s;
     s.nodes = nodes.as<QNode8>();

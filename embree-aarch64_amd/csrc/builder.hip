@@ -404,6 +404,7 @@ struct CollapseArgs {
   const GeomDesc* geoms;
   QNode8* nodes;
   TriRec* tris;
+  NodeInfo* info;
   BuildCounters* cnt;
   uint32_t n;  // capacity of nodes, tris and each queue
   uint32_t robust;
@@ -619,6 +620,12 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
   }
   q.imask = (uint8_t)imask;
   a.nodes[item.node] = q;
+  NodeInfo inf;
+  inf.lo[0] = ul[0]; inf.lo[1] = ul[1]; inf.lo[2] = ul[2];
+  inf.hi[0] = uh[0]; inf.hi[1] = uh[1]; inf.hi[2] = uh[2];
+  inf.count = ninner_prims + nt;
+  inf.depth = item.depth;
+  a.info[item.node] = inf;
 }
 
 // ------------------------------------------------------------------ host --
@@ -696,7 +703,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
       SahTreeOut so;
       build_sah_tree(n, S.vals.as<uint32_t>(), S.slo.as<float4>(), S.shi.as<float4>(), clo, chi,
                      S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
-                     S.range.as<uint2>(), S.sah, so, s);
+                     S.range.as<uint2>(), prm.learned, S.sah, so, s);
       root_bin = so.root;
       // boxes in the final primitive order
       hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(n, BT)), dim3(BT), 0, s, S.vals.as<uint32_t>(),
@@ -725,6 +732,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   // outputs: at most n nodes, exactly n triangle records
   out.nodes.reserve((size_t)n * sizeof(QNode8));
   out.tris.reserve((size_t)n * sizeof(TriRec));
+  out.info.reserve((size_t)n * sizeof(NodeInfo));
   S.queue.reserve((size_t)2 * n * sizeof(QItem));
   QItem* q0 = S.queue.as<QItem>();
   QItem* q1 = q0 + n;
@@ -746,6 +754,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   ca.geoms = S.geoms.as<GeomDesc>();
   ca.nodes = out.nodes.as<QNode8>();
   ca.tris = out.tris.as<TriRec>();
+  ca.info = out.info.as<NodeInfo>();
   ca.cnt = cnt;
   ca.n = n;
   ca.robust = prm.robust ? 1u : 0u;

@@ -6,6 +6,7 @@
 #pragma once
 #include "bvh_types.h"
 #include "gpu_util.h"
+#include "mlp.h"
 #include "sort.h"
 
 namespace rtg {
@@ -26,8 +27,9 @@ struct SahTreeOut {
 // inner node covers a contiguous position range; child/parent_int/
 // parent_leaf/range filled for inner nodes 0..n-2 (inner node ids are split
 // positions); clo/chi are the bounds of lo+hi over all references.
+// `learned` (may be null) replaces the half-area term of the split cost.
 void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi, const float clo[3],
                     const float chi[3], uint32_t* child, uint32_t* parent_int, uint32_t* parent_leaf, uint2* range,
-                    SahScratch& S, SahTreeOut& out, hipStream_t s);
+                    const LearnedCost* learned, SahScratch& S, SahTreeOut& out, hipStream_t s);
 
 }  // namespace rtg

@@ -30,6 +30,7 @@
 #include <string>
 
 #include "builder_sah.h"
+#include "mlp.h"
 
 namespace rtg {
 namespace {
@@ -239,6 +240,109 @@ __device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m
   return cls == 0 ? id : NIL;
 }
 
+// Workgroup-level choice of the split of segment sg from its bins w (LDS or
+// HBM); every thread returns the same Split. Plain SAH: one thread per
+// axis sweeps its bins. Learned cost (lc.P set and the segment larger than
+// lc.min_count): the workgroup evaluates the model on the left and right
+// boxes of all 3*(nb-1) candidates (their boxes and counts from the bins),
+// then one thread per axis keeps the least P(L)*n_L + P(R)*n_R, P = 2^model.
+// Ties keep the lowest axis and the lowest k; without a separating
+// candidate the split is the object median.
+struct ChooseLds {
+  float cost[3];
+  int k[3];
+  uint32_t nl[3];
+  float p[2 * 3 * (NB - 1)];
+  Split sp;
+};
+
+__device__ inline void range_box(const uint32_t* w, int a, int b0, int b1, float lo[3], float hi[3], uint32_t& cnt) {
+  bins_box(w, a, b0, b1, lo, hi);
+  cnt = 0;
+  for (int b = b0; b < b1; b++) cnt += w[wc(a, b)];
+}
+
+template <int BLOCK>
+__device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const Seg& sg, const LearnedCost& lc,
+                                     ChooseLds& L, uint32_t* err) {
+  const int tid = threadIdx.x;
+  const uint32_t depth = sg.meta >> 1;
+  const bool learned = lc.P != nullptr && sg.count > lc.min_count && depth < MEDIAN_DEPTH;
+  const int ncand = m.nb - 1;
+  if (learned) {
+    for (int e = tid; e < 2 * 3 * ncand; e += BLOCK) {
+      const int c = e >> 1, side = e & 1;
+      const int a = c / ncand, k = c % ncand + 1;
+      float lo[3], hi[3];
+      uint32_t cnt;
+      range_box(w, a, side ? k : 0, side ? m.nb : k, lo, hi, cnt);
+      float pv = -1.0f;
+      if (cnt > 0 && m.scale[a] != 0.0f) {
+        float f[MLP_IN];
+        box_features(lo, hi, cnt, lc.sn, f);
+        pv = exp2f(mlp_eval(lc.P, f));
+      }
+      L.p[e] = pv;
+    }
+    __syncthreads();
+  }
+  if (tid < 3) {
+    float c = INFINITY;
+    int k = 0;
+    uint32_t nl = 0;
+    uint32_t total = 0;
+    for (int b = 0; b < m.nb; b++) total += w[wc(tid, b)];
+    if (total != sg.count) atomicOr(err, 2u);
+    if (depth < MEDIAN_DEPTH && total == sg.count) {
+      if (!learned) {
+        if (!best_on_axis(w, m, tid, c, k, nl)) c = INFINITY;
+      } else if (m.scale[tid] != 0.0f) {
+        uint32_t run = 0;
+        for (int kk = 1; kk <= ncand; kk++) {
+          run += w[wc(tid, kk - 1)];
+          const float pl = L.p[2 * (tid * ncand + kk - 1)], pr = L.p[2 * (tid * ncand + kk - 1) + 1];
+          if (run == 0 || run == total || pl < 0.0f || pr < 0.0f) continue;
+          const float v = pl * (float)run + pr * (float)(total - run);
+          if (v < c) {
+            c = v;
+            k = kk;
+            nl = run;
+          }
+        }
+      }
+    }
+    L.cost[tid] = c;
+    L.k[tid] = k;
+    L.nl[tid] = nl;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    Split sp;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      sp.ofs[k] = m.ofs[k];
+      sp.scale[k] = m.scale[k];
+    }
+    sp.nb = m.nb;
+    sp.axis = 3;
+    sp.k = 0;
+    sp.nleft = sg.count / 2;
+    float best = INFINITY;
+    for (int a = 0; a < 3; a++) {
+      if (L.cost[a] < best) {
+        best = L.cost[a];
+        sp.axis = a;
+        sp.k = L.k[a];
+        sp.nleft = L.nl[a];
+      }
+    }
+    sp.lsid = sp.rsid = NIL;
+    L.sp = sp;
+  }
+  __syncthreads();
+  return L.sp;
+}
+
 // ------------------------------------------------------------ big segments --
 
 __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
@@ -319,41 +423,16 @@ __global__ __launch_bounds__(TBS) void k_big_bins(const uint32_t* __restrict__ s
   }
 }
 
-__global__ void k_big_split(const Seg* __restrict__ segs, uint32_t nseg, const uint32_t* __restrict__ gbins,
-                            Split* __restrict__ splits, Lists nx, Tree t) {
-  const uint32_t i = blockIdx.x * 64 + threadIdx.x;
-  if (i >= nseg) return;
+__global__ __launch_bounds__(TBS) void k_big_split(const Seg* __restrict__ segs, const uint32_t* __restrict__ gbins,
+                                                   Split* __restrict__ splits, Lists nx, Tree t, LearnedCost lc) {
+  __shared__ ChooseLds L;
+  const uint32_t i = blockIdx.x;
   const Seg sg = segs[i];
   const uint32_t depth = sg.meta >> 1;
   const Mapping m = make_mapping(sg);
   const uint32_t* w = gbins + (size_t)i * BIN_WORDS;
-  Split sp;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    sp.ofs[k] = m.ofs[k];
-    sp.scale[k] = m.scale[k];
-  }
-  sp.nb = m.nb;
-  sp.axis = 3;
-  sp.k = 0;
-  sp.nleft = sg.count / 2;
-  uint32_t total = 0;
-  for (int b = 0; b < m.nb; b++) total += w[wc(0, b)];
-  if (total != sg.count) atomicOr(&t.cnt->error, 2u);
-  if (depth < MEDIAN_DEPTH && total == sg.count) {
-    float best = INFINITY;
-    for (int a = 0; a < 3; a++) {
-      float c;
-      int k;
-      uint32_t nl;
-      if (best_on_axis(w, m, a, c, k, nl) && c < best) {
-        best = c;
-        sp.axis = a;
-        sp.k = k;
-        sp.nleft = nl;
-      }
-    }
-  }
+  Split sp = choose_split<TBS>(w, m, sg, lc, L, &t.cnt->error);
+  if (threadIdx.x != 0) return;
   // child centroid bounds: lo+hi of a primitive lies in [2 lo, 2 hi] of its
   // side's box, and in the parent's bounds
   float lcl[3], lch[3], rcl[3], rch[3];
@@ -433,14 +512,11 @@ template <int BLOCK, int PER>
 __global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, const uint32_t* __restrict__ idx,
                                                const float4* __restrict__ blo, const float4* __restrict__ bhi,
                                                uint32_t* __restrict__ idx_o, float4* __restrict__ blo_o,
-                                               float4* __restrict__ bhi_o, Lists nx, Tree t) {
+                                               float4* __restrict__ bhi_o, Lists nx, Tree t, LearnedCost lc) {
   static_assert(PER <= 32, "flags are kept in one 32-bit mask");
   constexpr int NW = BLOCK / 64;
   __shared__ uint32_t sb[BIN_WORDS];
-  __shared__ float s_cost[3];
-  __shared__ int s_k[3];
-  __shared__ uint32_t s_nl[3];
-  __shared__ Split s_sp;
+  __shared__ ChooseLds L;
   __shared__ uint32_t s_wsum[NW];
   __shared__ uint32_t s_cb[12];  // ordered bounds of lo+hi: left lo3, left hi3, right lo3, right hi3
   const int tid = threadIdx.x;
@@ -459,44 +535,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, con
     if (p < end) bin_prim(sb, m, blo[p], bhi[p]);
   }
   __syncthreads();
-  if (tid < 3) {
-    float c = INFINITY;
-    int k = 0;
-    uint32_t nl = 0;
-    uint32_t total = 0;
-    for (int b = 0; b < m.nb; b++) total += sb[wc(tid, b)];
-    if (total != sg.count) atomicOr(&t.cnt->error, 2u);
-    if (depth < MEDIAN_DEPTH && total == sg.count && !best_on_axis(sb, m, tid, c, k, nl)) c = INFINITY;
-    s_cost[tid] = c;
-    s_k[tid] = k;
-    s_nl[tid] = nl;
-  }
-  __syncthreads();
-  if (tid == 0) {
-    Split sp;
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      sp.ofs[k] = m.ofs[k];
-      sp.scale[k] = m.scale[k];
-    }
-    sp.nb = m.nb;
-    sp.axis = 3;
-    sp.k = 0;
-    sp.nleft = sg.count / 2;
-    float best = INFINITY;
-    for (int a = 0; a < 3; a++) {
-      if (s_cost[a] < best) {
-        best = s_cost[a];
-        sp.axis = a;
-        sp.k = s_k[a];
-        sp.nleft = s_nl[a];
-      }
-    }
-    sp.lsid = sp.rsid = NIL;
-    s_sp = sp;
-  }
-  __syncthreads();
-  const Split sp = s_sp;
+  const Split sp = choose_split<BLOCK>(sb, m, sg, lc, L, &t.cnt->error);
   // flags of this thread's positions, and the stable destinations
   uint32_t mask = 0u, nl = 0u;
 #pragma unroll 4
@@ -580,7 +619,9 @@ __global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, con
 
 void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi, const float clo[3],
                     const float chi[3], uint32_t* child, uint32_t* parent_int, uint32_t* parent_leaf, uint2* range,
-                    SahScratch& S, SahTreeOut& out, hipStream_t s) {
+                    const LearnedCost* learned, SahScratch& S, SahTreeOut& out, hipStream_t s) {
+  LearnedCost lcost{};
+  if (learned) lcost = *learned;
   if (n < 2) throw hip_error(hipErrorInvalidValue, "SAH tree needs at least two primitives");
   const uint32_t cap[3] = {n / MID_MAX + 2, n / (SMALL_MAX + 1) + 2, n / 2 + 2};
   for (int p = 0; p < 2; p++) {
@@ -640,8 +681,8 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
       hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[o].as<uint32_t>(), (size_t)n, NIL);
       hipLaunchKernelGGL(k_big_bins, dim3(div_up(n, CHUNK)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
                          S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs, S.bins.as<uint32_t>());
-      hipLaunchKernelGGL(k_big_split, dim3(div_up(cur[0], 64)), dim3(64), 0, s, bs, cur[0], S.bins.as<uint32_t>(),
-                         S.splits.as<Split>(), nx, t);
+      hipLaunchKernelGGL(k_big_split, dim3(cur[0]), dim3(TBS), 0, s, bs, S.bins.as<uint32_t>(), S.splits.as<Split>(),
+                         nx, t, lcost);
       hipLaunchKernelGGL(k_big_flags, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
                          S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs, S.splits.as<Split>(),
                          S.flags.as<uint32_t>());
@@ -654,11 +695,11 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
     if (cur[1])
       hipLaunchKernelGGL((k_seg<256, 16>), dim3(cur[1]), dim3(256), 0, s, S.segs[1][in].as<Seg>(),
                          S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
-                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t);
+                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t, lcost);
     if (cur[2])
       hipLaunchKernelGGL((k_seg<64, 1>), dim3(cur[2]), dim3(64), 0, s, S.segs[2][in].as<Seg>(),
                          S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
-                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t);
+                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t, lcost);
     RTG_HIP(hipGetLastError());
     RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
     RTG_HIP(hipStreamSynchronize(s));

@@ -57,6 +57,17 @@ constexpr uint32_t TRI_LAST = 1u;
 // Stack / child reference: bit 31 set = leaf (triangle record index).
 constexpr uint32_t REF_LEAF = 0x80000000u;
 
+// Per-node side record (same index as the QNode8): the float union box of
+// the node's children and the number of primitives below it. Read by the
+// learned split-cost trainer, not by the traversal.
+struct NodeInfo {
+  float lo[3];
+  uint32_t count;
+  float hi[3];
+  uint32_t depth;
+};
+static_assert(sizeof(NodeInfo) == 32, "NodeInfo must be 32 bytes");
+
 // One triangle geometry as the builder reads it (device pointers).
 struct GeomDesc {
   const char* vertices;

@@ -140,3 +140,143 @@ RTC_API void rtcgGenRandomRays(unsigned int numRays, unsigned long long seed, fl
     r.ray.id = i;
   }
 }
+
+namespace {
+
+// Value noise: hashed lattice values in [-1, 1], smoothstep-blended.
+float lattice(uint64_t seed, uint32_t oct, int32_t x, int32_t y) {
+  const uint64_t key = ((uint64_t)(uint32_t)x << 32) | (uint32_t)y;
+  return 2.0f * u01(seed, 16 + oct, key) - 1.0f;
+}
+
+float value_noise(uint64_t seed, uint32_t oct, float x, float y) {
+  const float fx = floorf(x), fy = floorf(y);
+  const int32_t ix = (int32_t)fx, iy = (int32_t)fy;
+  float tx = x - fx, ty = y - fy;
+  tx = tx * tx * (3.0f - 2.0f * tx);
+  ty = ty * ty * (3.0f - 2.0f * ty);
+  const float a = lattice(seed, oct, ix, iy), b = lattice(seed, oct, ix + 1, iy);
+  const float c = lattice(seed, oct, ix, iy + 1), d = lattice(seed, oct, ix + 1, iy + 1);
+  const float ab = a + (b - a) * tx, cd = c + (d - c) * tx;
+  return ab + (cd - ab) * ty;
+}
+
+void normalize3(float v[3]) {
+  const float l = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (l > 0.0f) {
+    v[0] /= l;
+    v[1] /= l;
+    v[2] /= l;
+  }
+}
+
+void cross3(const float a[3], const float b[3], float r[3]) {
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = a[2] * b[0] - a[0] * b[2];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+}  // namespace
+
+// n x n vertex grid over [0,1]^2 (x, y), height z = amplitude * sum over
+// `octaves` of value noise at frequency 4*2^o with weight 2^-o; two
+// triangles per cell, 2*(n-1)^2 in all.
+RTC_API unsigned int rtcgGenHeightfield(unsigned int n, unsigned long long seed, float amplitude, unsigned int octaves,
+                                        float* vertices, unsigned int* indices) {
+  if (n < 2) return 0;
+  const unsigned nt = 2 * (n - 1) * (n - 1);
+  if (!vertices || !indices) return nt;
+  const float inv = 1.0f / (float)(n - 1);
+  for (unsigned j = 0; j < n; j++)
+    for (unsigned i = 0; i < n; i++) {
+      const float x = (float)i * inv, y = (float)j * inv;
+      float h = 0.0f, w = 1.0f, f = 4.0f;
+      for (unsigned o = 0; o < octaves; o++) {
+        h += w * value_noise(seed, o, x * f, y * f);
+        w *= 0.5f;
+        f *= 2.0f;
+      }
+      float* v = vertices + 3 * ((size_t)j * n + i);
+      v[0] = x;
+      v[1] = y;
+      v[2] = amplitude * h;
+    }
+  size_t t = 0;
+  for (unsigned j = 0; j + 1 < n; j++)
+    for (unsigned i = 0; i + 1 < n; i++) {
+      const unsigned a = j * n + i, b = a + 1, c = a + n, d = c + 1;
+      unsigned* ix = indices + 3 * t;
+      ix[0] = a; ix[1] = b; ix[2] = d;
+      ix[3] = a; ix[4] = d; ix[5] = c;
+      t += 2;
+    }
+  return nt;
+}
+
+// Pinhole camera at org looking at target (up vector up, vertical field of
+// view fovDeg), one ray per pixel centre, row-major from the top row.
+RTC_API void rtcgGenCameraRays(unsigned int width, unsigned int height, const float* org, const float* target,
+                               const float* up, float fovDeg, RTCRayHit* rays) {
+  float fw[3] = {target[0] - org[0], target[1] - org[1], target[2] - org[2]};
+  normalize3(fw);
+  float rt[3], u[3];
+  cross3(fw, up, rt);
+  normalize3(rt);
+  cross3(rt, fw, u);
+  const float th = tanf(0.5f * fovDeg * (float)M_PI / 180.0f);
+  const float aspect = (float)width / (float)height;
+  for (unsigned y = 0; y < height; y++)
+    for (unsigned x = 0; x < width; x++) {
+      RTCRayHit& r = rays[(size_t)y * width + x];
+      init_rayhit(r);
+      const float px = (2.0f * ((float)x + 0.5f) / (float)width - 1.0f) * th * aspect;
+      const float py = (1.0f - 2.0f * ((float)y + 0.5f) / (float)height) * th;
+      float d[3] = {fw[0] + px * rt[0] + py * u[0], fw[1] + px * rt[1] + py * u[1], fw[2] + px * rt[2] + py * u[2]};
+      normalize3(d);
+      r.ray.org_x = org[0]; r.ray.org_y = org[1]; r.ray.org_z = org[2];
+      r.ray.dir_x = d[0]; r.ray.dir_y = d[1]; r.ray.dir_z = d[2];
+      r.ray.id = y * width + x;
+    }
+}
+
+// Next-bounce rays of a path tracer: for every hit record in `hits`, a ray
+// from the hit point (tnear = epsilon) in a cosine-weighted direction about
+// the geometric normal turned towards the incoming ray
+// (tutorials/common/math/sampling.h:52-78 cosineSampleHemisphere, frame(N)
+// as common/math/linearspace3.h:113-120). Misses spawn nothing. Returns the
+// number of rays written to out (at most M).
+RTC_API unsigned int rtcgGenBounceRays(const RTCRayHit* hits, unsigned int M, unsigned long long seed, float epsilon,
+                                       RTCRayHit* out) {
+  const float twopi = 2.0f * (float)M_PI;
+  unsigned k = 0;
+  for (unsigned i = 0; i < M; i++) {
+    const RTCRayHit& h = hits[i];
+    if (h.hit.geomID == RTC_INVALID_GEOMETRY_ID) continue;
+    float N[3] = {h.hit.Ng_x, h.hit.Ng_y, h.hit.Ng_z};
+    normalize3(N);
+    if (N[0] * h.ray.dir_x + N[1] * h.ray.dir_y + N[2] * h.ray.dir_z > 0.0f) {
+      N[0] = -N[0]; N[1] = -N[1]; N[2] = -N[2];
+    }
+    const float t = h.ray.tfar;
+    const float p[3] = {h.ray.org_x + t * h.ray.dir_x, h.ray.org_y + t * h.ray.dir_y, h.ray.org_z + t * h.ray.dir_z};
+    // frame(N): dx from the larger of (0, Nz, -Ny) and (-Nz, 0, Nx)
+    float dx0[3] = {0.0f, N[2], -N[1]}, dx1[3] = {-N[2], 0.0f, N[0]};
+    float* dx = (dx0[1] * dx0[1] + dx0[2] * dx0[2] > dx1[0] * dx1[0] + dx1[2] * dx1[2]) ? dx0 : dx1;
+    normalize3(dx);
+    float dy[3];
+    cross3(N, dx, dy);
+    normalize3(dy);
+    const float u = u01(seed, 20, 2ull * i), v = u01(seed, 20, 2ull * i + 1);
+    const float phi = twopi * u, ct = sqrtf(v), st = sqrtf(1.0f - v);
+    const float l[3] = {cosf(phi) * st, sinf(phi) * st, ct};
+    RTCRayHit& r = out[k++];
+    init_rayhit(r);
+    r.ray.org_x = p[0]; r.ray.org_y = p[1]; r.ray.org_z = p[2];
+    r.ray.dir_x = dx[0] * l[0] + dy[0] * l[1] + N[0] * l[2];
+    r.ray.dir_y = dx[1] * l[0] + dy[1] * l[1] + N[1] * l[2];
+    r.ray.dir_z = dx[2] * l[0] + dy[2] * l[1] + N[2] * l[2];
+    r.ray.tnear = epsilon;
+    r.ray.id = i;
+  }
+  return k;
+}

@@ -28,8 +28,10 @@ struct RaySorter {
 // spaced `stride` bytes apart in device memory, in place. cnt_nodes /
 // cnt_tris (optional, M entries each) receive per-ray node visits and
 // triangle tests. err receives bit 0 on a traversal stack overflow.
+// node_visits (optional, with the counters): one atomic increment per node
+// visit, indexed like the QNode8 array.
 void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t inst_id,
                   uint32_t* cnt_nodes, uint32_t* cnt_tris, uint32_t* err, hipStream_t s,
-                  const TraceParams& tp, RaySorter* sorter);
+                  const TraceParams& tp, RaySorter* sorter, uint32_t* node_visits = nullptr);
 
 }  // namespace rtg

@@ -286,7 +286,7 @@ template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool PL>
 __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                               uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                               uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
-                                              const uint32_t* __restrict__ perm) {
+                                              const uint32_t* __restrict__ perm, uint32_t* node_visits) {
   const uint32_t slot = blockIdx.x * TB + threadIdx.x;
   if (slot >= M) return;
   const uint32_t i = perm ? perm[slot] : slot;  // lanes take rays in spatially sorted order when given
@@ -330,7 +330,10 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
       }
       NodeHit h;
       intersect_node(sc.nodes + cur, r, lim, h);
-      if (COUNT) nvis++;
+      if (COUNT) {
+        nvis++;
+        if (node_visits) atomicAdd(&node_visits[cur], 1u);
+      }
       if (h.mask) {
         if (OCCLUDED) {
           const int c0 = __builtin_ctz(h.mask);
@@ -443,7 +446,7 @@ __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene s
                                                 uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                                 uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
                                                 const uint32_t* __restrict__ perm, uint32_t* __restrict__ next,
-                                                int refill_min) {
+                                                int refill_min, uint32_t* node_visits) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   __shared__ uint2 lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
@@ -515,7 +518,10 @@ __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene s
       } else {
         NodeHit h;
         intersect_node(sc.nodes + cur, r, lim, h);
-        if (COUNT) nvis++;
+        if (COUNT) {
+          nvis++;
+          if (node_visits) atomicAdd(&node_visits[cur], 1u);
+        }
         if (h.mask) {
           if (OCCLUDED) {
             const int c0 = __builtin_ctz(h.mask);
@@ -648,6 +654,7 @@ struct LaunchArgs {
   const uint32_t* perm;
   uint32_t* counter;
   int refill;
+  uint32_t* node_visits;
   bool persistent;
   dim3 grid, pgrid;
   int lds_slots;
@@ -658,10 +665,10 @@ template <bool OCC, bool AL, bool CNT, int SL, bool PL>
 void launch_one(const LaunchArgs& a) {
   if (a.persistent)
     hipLaunchKernelGGL((k_trace_p<OCC, AL, CNT, SL, PL>), a.pgrid, dim3(TB), 0, a.s, a.sc, a.rays, a.M, a.stride,
-                       a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm, a.counter, a.refill);
+                       a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm, a.counter, a.refill, a.node_visits);
   else
     hipLaunchKernelGGL((k_trace<OCC, AL, CNT, SL, PL>), a.grid, dim3(TB), 0, a.s, a.sc, a.rays, a.M, a.stride,
-                       a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm);
+                       a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm, a.node_visits);
 }
 
 template <bool OCC, bool AL, bool CNT, bool PL>
@@ -687,7 +694,7 @@ void launch_pl(const LaunchArgs& a, bool count, bool occluded, bool aligned) {
 
 void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t inst_id,
                   uint32_t* cnt_nodes, uint32_t* cnt_tris, uint32_t* err, hipStream_t s, const TraceParams& tp,
-                  RaySorter* sorter) {
+                  RaySorter* sorter, uint32_t* node_visits) {
   if (M == 0 || sc.num_nodes == 0) return;
   const bool aligned = ((uintptr_t)rays % 16 == 0) && (stride % 16 == 0);
   const dim3 grid(div_up(M, TB)), block(TB);
@@ -713,7 +720,8 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
     pgrid = dim3(want < grid.x ? want : grid.x);
   }
   LaunchArgs la{sc, rp, M, stride, inst_id, cnt_nodes, cnt_tris, err, perm,
-                persistent ? sorter->counter.as<uint32_t>() : nullptr, refill, persistent, grid, pgrid, tp.lds_slots, s};
+                persistent ? sorter->counter.as<uint32_t>() : nullptr, refill, count ? node_visits : nullptr,
+                persistent, grid, pgrid, tp.lds_slots, s};
   if (sc.robust) launch_pl<true>(la, count, occluded, aligned);
   else launch_pl<false>(la, count, occluded, aligned);
   RTG_HIP(hipGetLastError());

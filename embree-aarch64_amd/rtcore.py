@@ -141,6 +141,24 @@ SIGNATURES = {
     "rtcgGenPinholeRays": (None, [_u32, _u32, _f32, _f32, _f32, _f32, _vp]),
     "rtcgGenSoup": (None, [_u32, ctypes.c_ulonglong, _f32, _vp, _vp]),
     "rtcgGenRandomRays": (None, [_u32, ctypes.c_ulonglong, _f32, _f32, _f32, _vp]),
+    "rtcgGenHeightfield": (_u32, [_u32, ctypes.c_ulonglong, _f32, _u32, _vp, _vp]),
+    "rtcgGenCameraRays": (None, [_u32, _u32, _vp, _vp, _vp, _f32, _vp]),
+    "rtcgGenBounceRays": (_u32, [_vp, _u32, ctypes.c_ulonglong, _f32, _vp]),
+    "rtcgNewSplitModel": (_vp, [_vp, _u32]),
+    "rtcgRetainSplitModel": (None, [_vp]),
+    "rtcgReleaseSplitModel": (None, [_vp]),
+    "rtcgSplitModelParamCount": (_u32, []),
+    "rtcgSplitModelSetParams": (None, [_vp, _vp]),
+    "rtcgSplitModelGetParams": (None, [_vp, _vp]),
+    "rtcgSplitModelPredict": (None, [_vp, _vp, _u32, _vp]),
+    "rtcgSplitModelGradient": (_f32, [_vp, _vp, _vp, _vp, _u32, _vp]),
+    "rtcgSplitModelCollect": (None, [_vp, _vp, _vp, _u32, _sz]),
+    "rtcgSplitModelSampleCount": (_u32, [_vp]),
+    "rtcgSplitModelGetSamples": (None, [_vp, _vp, _vp]),
+    "rtcgSplitModelClearSamples": (None, [_vp]),
+    "rtcgSplitModelTrain": (_f32, [_vp, _u32, _u32, _f32, _u32]),
+    "rtcgSetSplitModelMinCount": (None, [_vp, _u32]),
+    "rtcgSetSceneSplitModel": (None, [_vp, _vp]),
 }
 
 _lib = None
@@ -228,6 +246,29 @@ def gen_random_rays(n, seed=2, lo=0.0, hi=1.0, tfar=float("inf")):
     r = np.zeros(n, RTCRayHit)
     lib().rtcgGenRandomRays(n, seed, lo, hi, tfar, r.ctypes.data)
     return r
+
+
+def gen_heightfield(n, seed=1, amplitude=0.2, octaves=6):
+    nt = lib().rtcgGenHeightfield(n, seed, amplitude, octaves, None, None)
+    v = np.zeros((n * n, 3), np.float32)
+    ix = np.zeros((nt, 3), np.uint32)
+    lib().rtcgGenHeightfield(n, seed, amplitude, octaves, v.ctypes.data, ix.ctypes.data)
+    return v, ix
+
+
+def gen_camera(width, height, org, target, up=(0.0, 0.0, 1.0), fov=45.0):
+    r = np.zeros(width * height, RTCRayHit)
+    f3 = lambda a: np.ascontiguousarray(a, np.float32)
+    o, t, u = f3(org), f3(target), f3(up)
+    lib().rtcgGenCameraRays(width, height, o.ctypes.data, t.ctypes.data, u.ctypes.data, fov, r.ctypes.data)
+    return r
+
+
+def gen_bounce(hits, seed=3, epsilon=1e-4):
+    out = np.zeros(len(hits), RTCRayHit)
+    hits = np.ascontiguousarray(hits)
+    k = lib().rtcgGenBounceRays(hits.ctypes.data, len(hits), seed, epsilon, out.ctypes.data)
+    return out[:k].copy()
 
 
 class Device:
@@ -336,3 +377,66 @@ class Scene:
         if self.h:
             lib().rtcReleaseScene(self.h)
             self.h = None
+
+
+class SplitModel:
+    """Learned split-cost model (FP16 MLP trained on the GPU)."""
+
+    def __init__(self, device, seed=1):
+        self.device = device
+        self.h = lib().rtcgNewSplitModel(device.h, seed)
+        device.check("rtcgNewSplitModel")
+
+    def release(self):
+        if self.h:
+            lib().rtcgReleaseSplitModel(self.h)
+            self.h = None
+
+    @staticmethod
+    def nparam():
+        return int(lib().rtcgSplitModelParamCount())
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        assert p.size == self.nparam()
+        lib().rtcgSplitModelSetParams(self.h, p.ctypes.data)
+        self.device.check("set_params")
+
+    def get_params(self):
+        p = np.zeros(self.nparam(), np.float32)
+        lib().rtcgSplitModelGetParams(self.h, p.ctypes.data)
+        self.device.check("get_params")
+        return p
+
+    def predict(self, x):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        y = np.zeros(len(x), np.float32)
+        lib().rtcgSplitModelPredict(self.h, x.ctypes.data, len(x), y.ctypes.data)
+        self.device.check("predict")
+        return y
+
+    def gradient(self, x, t, w=None):
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        t = np.ascontiguousarray(t, np.float32)
+        w = np.ones(len(x), np.float32) if w is None else np.ascontiguousarray(w, np.float32)
+        g = np.zeros(self.nparam(), np.float32)
+        loss = lib().rtcgSplitModelGradient(self.h, x.ctypes.data, t.ctypes.data, w.ctypes.data, len(x), g.ctypes.data)
+        self.device.check("gradient")
+        return float(loss), g
+
+    def collect(self, scene, d_rays, n, stride=80):
+        lib().rtcgSplitModelCollect(self.h, scene.h, d_rays, n, stride)
+        self.device.check("collect")
+
+    def samples(self):
+        n = int(lib().rtcgSplitModelSampleCount(self.h))
+        x = np.zeros((n, 8), np.float32)
+        t = np.zeros(n, np.float32)
+        lib().rtcgSplitModelGetSamples(self.h, x.ctypes.data, t.ctypes.data)
+        self.device.check("samples")
+        return x, t
+
+    def train(self, steps, batch=65536, lr=1e-3, seed=1):
+        loss = lib().rtcgSplitModelTrain(self.h, steps, batch, lr, seed)
+        self.device.check("train")
+        return float(loss)

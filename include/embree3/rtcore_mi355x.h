@@ -94,4 +94,70 @@ RTC_API void rtcgGenSoup(unsigned int numTris, unsigned long long seed, float h,
 RTC_API void rtcgGenRandomRays(unsigned int numRays, unsigned long long seed, float lo, float hi, float tfar,
                                struct RTCRayHit* rays);
 
+/* Heightfield: n x n vertices over [0,1]^2, z = amplitude * value-noise fBm
+ * (octaves at frequency 4*2^o, weight 2^-o); returns 2*(n-1)^2 triangles
+ * (NULL buffers: size query). Vertices n*n*3 floats, indices tris*3. */
+RTC_API unsigned int rtcgGenHeightfield(unsigned int n, unsigned long long seed, float amplitude,
+                                        unsigned int octaves, float* vertices, unsigned int* indices);
+
+/* Look-at pinhole camera rays (closest-hit records) for width x height pixels. */
+RTC_API void rtcgGenCameraRays(unsigned int width, unsigned int height, const float* org, const float* target,
+                               const float* up, float fovDeg, struct RTCRayHit* rays);
+
+/* Cosine-weighted next-bounce rays from the hits among M traced records;
+ * returns how many were written (misses spawn none). */
+RTC_API unsigned int rtcgGenBounceRays(const struct RTCRayHit* hits, unsigned int M, unsigned long long seed,
+                                       float epsilon, struct RTCRayHit* out);
+
+/* ---- learned split cost --------------------------------------------- */
+/* No counterpart in the reference: its binned-SAH split score
+ * (kernels/builders/heuristic_binning.h:340-393, BinInfoT::best) is the plain
+ * cost these functions replace. A split model is a small FP16 MLP (8 box
+ * features -> 32 -> 32 -> 1, FP32 accumulate) predicting log2 of the
+ * probability that a ray of a target distribution visits a node with a given
+ * box. Trained on per-node visit counts of the library's own traversal, it
+ * replaces the half-area term of the binned-SAH split cost of segments with
+ * more than min_count (default 64) primitives. Hits do not depend on the
+ * tree, so scenes built with a model give the same records.
+ * Feature vector of a box (lo, hi) holding n primitives, relative to the
+ * scene box (slo, sext) of the collected data: ((lo+hi)/2 - slo)/sext (3),
+ * (hi-lo)/sext (3), log2(halfArea/sceneHalfArea + 2^-30)/16, log2(n)/16. */
+typedef struct RTCGSplitModelTy* RTCGSplitModel;
+
+RTC_API RTCGSplitModel rtcgNewSplitModel(RTCDevice device, unsigned int seed);
+RTC_API void rtcgRetainSplitModel(RTCGSplitModel model);
+RTC_API void rtcgReleaseSplitModel(RTCGSplitModel model);
+
+/* Parameter vector (FP32 master copy): W1[32][8] b1[32] W2[32][32] b2[32] w3[32] b3. */
+RTC_API unsigned int rtcgSplitModelParamCount(void);
+RTC_API void rtcgSplitModelSetParams(RTCGSplitModel model, const float* params);
+RTC_API void rtcgSplitModelGetParams(RTCGSplitModel model, float* params);
+
+/* Inference on n feature vectors (host arrays, n x 8 in, n out). */
+RTC_API void rtcgSplitModelPredict(RTCGSplitModel model, const float* features, unsigned int n, float* out);
+
+/* Loss sum_i w_i (y_i - t_i)^2 / n of the FP16 forward pass and its FP32
+ * gradient with respect to the parameter vector (host arrays). */
+RTC_API float rtcgSplitModelGradient(RTCGSplitModel model, const float* features, const float* targets,
+                                     const float* weights, unsigned int n, float* grad);
+
+/* Traces M closest-hit records (device memory, not modified) through the
+ * committed scene, counts visits per node and appends one training sample
+ * per node: its features and log2((visits + 0.5) / M). */
+RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const void* d_rays, unsigned int M,
+                                   size_t byteStride);
+RTC_API unsigned int rtcgSplitModelSampleCount(RTCGSplitModel model);
+RTC_API void rtcgSplitModelGetSamples(RTCGSplitModel model, float* features, float* targets);
+RTC_API void rtcgSplitModelClearSamples(RTCGSplitModel model);
+
+/* Adam on random minibatches of the collected samples, on the GPU; returns
+ * the last minibatch loss. */
+RTC_API float rtcgSplitModelTrain(RTCGSplitModel model, unsigned int steps, unsigned int batch,
+                                  float learningRate, unsigned int seed);
+RTC_API void rtcgSetSplitModelMinCount(RTCGSplitModel model, unsigned int minCount);
+
+/* Later commits of the scene use the model's split cost (binned-SAH builder);
+ * NULL restores the plain cost. */
+RTC_API void rtcgSetSceneSplitModel(RTCScene scene, RTCGSplitModel model);
+
 RTC_NAMESPACE_END

@@ -1,0 +1,115 @@
+"""Learned split cost on the GPU: inference and gradient kernels against the
+CPU restatement (oracle/mlp.py), training, visit collection, and trees built
+with a trained model giving the same hits as the oracle.
+
+Tolerances: the GPU forward pass rounds weights and activations to FP16 and
+accumulates in FP32 (v_dot2_f32_f16); the oracle rounds the same values to
+FP16 but accumulates in float64, so an activation that lands on an FP16
+rounding boundary may differ by one FP16 ulp. Predictions are compared
+within 2e-3 (1 + |y|) and gradients within 5e-3 of their norm."""
+import numpy as np
+import pytest
+
+from conftest import mismatches
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+INV = 0xFFFFFFFF
+
+
+def rand_params(seed):
+    rng = np.random.default_rng(seed)
+    p = np.zeros(mlp.NPARAM, np.float32)
+    p[mlp.OFF_W1:mlp.OFF_B1] = rng.uniform(-1, 1, mlp.H * mlp.IN) * np.sqrt(6 / mlp.IN)
+    p[mlp.OFF_B1:mlp.OFF_W2] = rng.uniform(-0.2, 0.2, mlp.H)
+    p[mlp.OFF_W2:mlp.OFF_B2] = rng.uniform(-1, 1, mlp.H * mlp.H) * np.sqrt(6 / mlp.H)
+    p[mlp.OFF_B2:mlp.OFF_W3] = rng.uniform(-0.2, 0.2, mlp.H)
+    p[mlp.OFF_W3:mlp.OFF_B3] = rng.uniform(-1, 1, mlp.H) * np.sqrt(3 / mlp.H)
+    p[mlp.OFF_B3] = -3.0
+    return p
+
+
+@pytest.fixture(scope="module")
+def model(rtc, device):
+    m = rtc.SplitModel(device, seed=7)
+    yield m
+    m.release()
+
+
+def test_param_layout(rtc, model):
+    assert rtc.SplitModel.nparam() == mlp.NPARAM
+    p = rand_params(1)
+    model.set_params(p)
+    assert np.array_equal(model.get_params(), p)
+
+
+def test_predict_matches_oracle(model):
+    p = rand_params(2)
+    model.set_params(p)
+    X = np.random.default_rng(3).uniform(-1, 1, (1000, mlp.IN)).astype(np.float32)
+    y = model.predict(X)
+    ref, _ = mlp.forward(p, X)
+    assert np.abs(y - ref).max() <= 2e-3 * (1 + np.abs(ref).max())
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 1000])
+def test_gradient_matches_oracle(model, n):
+    p = rand_params(4 + n)
+    model.set_params(p)
+    rng = np.random.default_rng(n)
+    X = rng.uniform(-1, 1, (n, mlp.IN)).astype(np.float32)
+    T = rng.normal(-4, 2, n).astype(np.float32)
+    W = rng.uniform(0.5, 2, n).astype(np.float32)
+    loss, g = model.gradient(X, T, W)
+    rloss, rg = mlp.loss_and_grad(p, X, T, W)
+    assert abs(loss - rloss) <= 2e-3 * rloss + 1e-6
+    assert np.linalg.norm(g - rg) <= 5e-3 * np.linalg.norm(rg)
+    # every parameter block carries gradient (a dropped term shows here)
+    for a, b in [(mlp.OFF_W1, mlp.OFF_B1), (mlp.OFF_B1, mlp.OFF_W2), (mlp.OFF_W2, mlp.OFF_B2),
+                 (mlp.OFF_B2, mlp.OFF_W3), (mlp.OFF_W3, mlp.OFF_B3), (mlp.OFF_B3, mlp.NPARAM)]:
+        assert np.linalg.norm(g[a:b] - rg[a:b]) <= 1e-2 * np.linalg.norm(rg[a:b]) + 1e-7
+
+
+def heightfield_case(rtc, n=256):
+    v, ix = rtc.gen_heightfield(n, seed=5, amplitude=0.2, octaves=6)
+    cam = rtc.gen_camera(160, 120, (0.5, -0.4, 0.45), (0.5, 0.45, 0.0), up=(0, 0, 1), fov=50.0)
+    return v, ix, cam
+
+
+def test_collect_and_train(rtc, device, oracle):
+    v, ix, cam = heightfield_case(rtc)
+    sc = rtc.Scene(device, [(v, ix)], quality=2).commit()
+    m = rtc.SplitModel(device, seed=11)
+    d = device.malloc(cam.nbytes)
+    try:
+        device.memcpy(d, cam.ctypes.data, cam.nbytes)
+        m.collect(sc, d, len(cam))
+        X, T = m.samples()
+        st = sc.stats()
+        assert len(X) == st["num_nodes"]
+        # the root is visited by every ray; no node more often than that
+        assert abs(T[0] - np.log2((len(cam) + 0.5) / len(cam))) < 1e-5
+        assert T.max() <= T[0] + 1e-6
+        assert (X[:, 0:3] >= -1e-6).all() and (X[:, 0:3] <= 1 + 1e-6).all()
+        l0, _ = m.gradient(X, T)
+        last = m.train(300, batch=4096, lr=3e-3, seed=1)
+        l1, _ = m.gradient(X, T)
+        assert l1 < 0.5 * l0 and last < 0.5 * l0
+        pred = m.predict(X)
+        assert np.corrcoef(pred, T)[0, 1] > 0.8
+        # a scene built with the trained model: same records as the oracle
+        sc2 = rtc.Scene(device, [(v, ix)])
+        rtc.lib().rtcgSetSceneSplitModel(sc2.h, m.h)
+        sc2.commit()
+        out = cam.copy()
+        sc2.intersect1M(out)
+        ref = cam.copy()
+        oracle.BVH([(v, ix)]).intersect(ref)
+        bad = mismatches(out, ref)
+        assert len(bad) == 0, bad[:5]
+        assert (out["geomID"] != INV).mean() > 0.5
+        sc2.release()
+    finally:
+        device.free(d)
+        m.release()
+        sc.release()
