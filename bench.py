@@ -53,6 +53,91 @@ def cpu_baseline(v, ix, rays, sample):
             "build_mtris_per_s": len(ix) / build_s / 1e6, "build_threads": 1}
 
 
+def time_trace(L, dev, sc, d_rays, d_src, n, steps):
+    ms = np.zeros(max(1, steps), np.float32)
+    L.rtcgTimeTrace(sc.h, d_rays, d_src, n, 80, 0, len(ms), ms.ctypes.data)
+    dev.check("rtcgTimeTrace")
+    return float(ms.mean())
+
+
+def counted(L, dev, sc, d_rays, d_src, n, nbytes):
+    d_cn, d_ct = dev.malloc(4 * n), dev.malloc(4 * n)
+    dev.memcpy(d_rays, d_src, nbytes)
+    L.rtcgTraceCounted(sc.h, d_rays, n, 80, 0, d_cn, d_ct)
+    dev.check("rtcgTraceCounted")
+    cn, ct = np.zeros(n, np.uint32), np.zeros(n, np.uint32)
+    dev.memcpy(cn.ctypes.data, d_cn, 4 * n)
+    dev.memcpy(ct.ctypes.data, d_ct, 4 * n)
+    out = np.zeros(n, R.RTCRayHit)
+    dev.memcpy(out.ctypes.data, d_rays, nbytes)
+    dev.free(d_cn)
+    dev.free(d_ct)
+    return float(cn.mean()), float(ct.mean()), out
+
+
+def learned_vs_sah(grid, width, height, bounces, train_steps, steps):
+    """Heightfield terrain (grid x grid vertices, value-noise fBm of 6
+    octaves, amplitude 0.2) seen by a width x height pinhole camera, followed
+    by cosine-hemisphere bounces (depth `bounces` in all). Queries of every
+    depth are traced on the plain binned-SAH tree; the split model is trained
+    on the node visits of the even-indexed queries and the learned tree is
+    timed against the plain tree on all queries."""
+    L = R.lib()
+    t0 = time.perf_counter()
+    v, ix = R.gen_heightfield(grid, seed=11, amplitude=0.2, octaves=6)
+    cam = R.gen_camera(width, height, (0.5, -0.35, 0.5), (0.5, 0.55, 0.0), up=(0.0, 0.0, 1.0), fov=50.0)
+    dev = R.Device("builder=sah")
+    plain = R.Scene(dev, [(v, ix)], quality=2).commit()
+    build_plain = plain.stats()["build_ms"]
+    queries, cur = [cam], cam
+    for b in range(bounces - 1):
+        out = cur.copy()
+        plain.intersect1M(out)
+        cur = R.gen_bounce(out, seed=100 + b)
+        queries.append(cur)
+    Q = np.concatenate(queries)
+    n, nbytes = len(Q), Q.nbytes
+    log(f"learned-vs-SAH: {len(ix)} tris, {n} queries ({[len(q) for q in queries]}), setup {time.perf_counter() - t0:.1f}s")
+    d_src, d_rays = dev.malloc(nbytes), dev.malloc(nbytes)
+    dev.memcpy(d_src, Q.ctypes.data, nbytes)
+    train = np.ascontiguousarray(Q[::2])
+    d_train = dev.malloc(train.nbytes)
+    dev.memcpy(d_train, train.ctypes.data, train.nbytes)
+    model = R.SplitModel(dev, seed=1)
+    t1 = time.perf_counter()
+    model.collect(plain, d_train, len(train))
+    loss = model.train(train_steps, batch=65536, lr=3e-3, seed=1)
+    train_s = time.perf_counter() - t1
+    dev.free(d_train)
+    learned = R.Scene(dev, [(v, ix)])
+    L.rtcgSetSceneSplitModel(learned.h, model.h)
+    learned.commit()
+    build_learned = learned.stats()["build_ms"]
+    res = {}
+    outs = {}
+    for name, sc in (("plain", plain), ("learned", learned)):
+        nb, tb, outs[name] = counted(L, dev, sc, d_rays, d_src, n, nbytes)
+        time_trace(L, dev, sc, d_rays, d_src, n, 2)
+        ms = time_trace(L, dev, sc, d_rays, d_src, n, steps)
+        st = sc.stats()
+        res[name] = {"mrays_per_s": round(n / (ms / 1e3) / 1e6, 2), "ms": round(ms, 3), "nodes_per_ray": round(nb, 2),
+                     "tris_per_ray": round(tb, 2), "bvh_nodes": st["num_nodes"], "sah": round(st["sah"], 3)}
+    same = bool(np.array_equal(outs["plain"].view(np.uint8), outs["learned"].view(np.uint8)))
+    for p_ in (d_src, d_rays):
+        dev.free(p_)
+    model.release()
+    plain.release()
+    learned.release()
+    dev.release()
+    return {"workload": f"heightfield {grid}x{grid} vertices ({len(ix)} tris), {width}x{height} camera + "
+                        f"{bounces - 1} cosine bounces = {n} closest-hit queries",
+            "plain_sah": res["plain"], "learned": res["learned"],
+            "speedup": round(res["learned"]["mrays_per_s"] / res["plain"]["mrays_per_s"], 4),
+            "hits_identical": same,
+            "train_s": round(train_s, 2), "train_loss": round(loss, 4),
+            "build_ms_plain": round(build_plain, 2), "build_ms_learned": round(build_learned, 2)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +148,12 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=1 << 20)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--config", default="")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--hf-grid", type=int, default=4096)
+    ap.add_argument("--hf-width", type=int, default=1920)
+    ap.add_argument("--hf-height", type=int, default=1080)
+    ap.add_argument("--hf-depth", type=int, default=4)
+    ap.add_argument("--train-steps", type=int, default=400)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: run with --gpus 1 (multi-GPU sharding is a design note only)"}))
@@ -146,6 +237,14 @@ def main():
     for p in (d_src, d_rays, d_cn, d_ct, d_osrc):
         dev.free(p)
 
+    sc.release()
+    dev.release()
+
+    learned = None
+    if not args.no_learned:
+        learned = learned_vs_sah(args.hf_grid, args.hf_width, args.hf_height, args.hf_depth, args.train_steps,
+                                 max(3, args.steps // 2))
+
     cpu = None
     if not args.no_cpu:
         log("timing the CPU baseline")
@@ -176,9 +275,8 @@ def main():
         "timed_wall_s_incl_restore": round(wall, 4),
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "learned_vs_sah": learned,
     }
-    sc.release()
-    dev.release()
     print(json.dumps(out), flush=True)
 
 

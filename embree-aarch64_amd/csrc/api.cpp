@@ -10,11 +10,13 @@
 #define RTC_EXPORT_API
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -53,6 +55,7 @@ struct ThreadDev {
   rtg::DevBuf errword;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   rtg::RaySorter sorter;
+  rtg::DevBuf batch, batch_counter;  // batched traversal: stream table, work counter
 };
 
 struct Device : RefObj {
@@ -183,7 +186,8 @@ struct Scene : RefObj {
   void* progress_ptr = nullptr;
   bool committed = false;
   std::mutex mtx;
-  rtg::DeviceBVH bvh;
+  std::shared_ptr<rtg::DeviceBVH> bvh = std::make_shared<rtg::DeviceBVH>();  // shared by the scenes of a batch
+  uint32_t slot = 0;                                                          // this scene's slot in bvh
   std::vector<rtg::DevBuf*> gdata;
   std::vector<rtg::GeomDesc> descs;
   SplitModelH* model = nullptr;  // learned split cost for the next commits (retained)
@@ -315,17 +319,14 @@ void bind_slot(Geometry* g, RTCBufferType type, unsigned slot, RTCFormat format,
   g->committed = false;
 }
 
-// Uploads geometry buffers and builds the device BVH.
-void commit_scene(Scene* sc) {
-  Device* d = sc->dev;
-  d->activate();
-  ThreadDev& t = d->td();
-  std::lock_guard<std::mutex> lk(sc->mtx);
-  if (sc->progress && !sc->progress(sc->progress_ptr, 0.0)) RTC_THROW(RTC_ERROR_CANCELLED, "progress monitor forced termination");
+// Uploads the enabled triangle geometries of sc (scene index `scene` of a
+// build) into device buffers owned by sc, appending their descriptors;
+// prim_offset advances over their triangles.
+void upload_geometry(Scene* sc, Device* d, ThreadDev& t, uint32_t scene, uint32_t& prim_offset,
+                     std::vector<rtg::GeomDesc>& out) {
   for (auto* b : sc->gdata) delete b;
   sc->gdata.clear();
   sc->descs.clear();
-  uint32_t prim_offset = 0;
   for (size_t id = 0; id < sc->geoms.size(); id++) {
     Geometry* g = sc->geoms[id];
     if (!g || !g->enabled || g->type != RTC_GEOMETRY_TYPE_TRIANGLE) continue;
@@ -351,9 +352,24 @@ void commit_scene(Scene* sc) {
     gd.num_tris = (uint32_t)g->index.count;
     gd.prim_offset = prim_offset;
     gd.geomID = (uint32_t)id;
+    gd.scene = scene;
+    gd.pad = 0;
     prim_offset += gd.num_tris;
     sc->descs.push_back(gd);
+    out.push_back(gd);
   }
+}
+
+// Uploads geometry buffers and builds the device BVH.
+void commit_scene(Scene* sc) {
+  Device* d = sc->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  std::lock_guard<std::mutex> lk(sc->mtx);
+  if (sc->progress && !sc->progress(sc->progress_ptr, 0.0)) RTC_THROW(RTC_ERROR_CANCELLED, "progress monitor forced termination");
+  uint32_t prim_offset = 0;
+  std::vector<rtg::GeomDesc> descs;
+  upload_geometry(sc, d, t, 0, prim_offset, descs);
   {
     std::lock_guard<std::mutex> bl(d->build_mtx);
     rtg::BuildParams p = build_params(d, sc->quality, sc->flags);
@@ -368,7 +384,11 @@ void commit_scene(Scene* sc) {
       p.builder = rtg::BUILDER_SAH;
     }
     d->monitor((ssize_t)prim_offset * (80 + 48));
-    rtg::build_bvh(sc->descs, p, d->scratch, sc->bvh, t.stream);
+    if (sc->bvh.use_count() > 1 || sc->slot != 0) {  // leave a batch's shared arrays to the other scenes
+      sc->bvh = std::make_shared<rtg::DeviceBVH>();
+      sc->slot = 0;
+    }
+    rtg::build_bvh(descs, p, d->scratch, *sc->bvh, t.stream);
   }
   RTG_HIP(hipStreamSynchronize(t.stream));
   sc->committed = true;
@@ -408,9 +428,9 @@ void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigne
   d->activate();
   ThreadDev& t = d->td();
   const uint32_t inst = ctx ? ctx->instID[0] : RTC_INVALID_GEOMETRY_ID;
-  if (sc->bvh.stats.num_nodes == 0) return;  // empty scene: every ray misses
+  if (sc->bvh->empty(sc->slot)) return;  // empty scene: every ray misses
   if (is_device_ptr(d, rays)) {
-    rtg::launch_trace(sc->bvh.view(), rays, M, stride, occluded, inst, nullptr, nullptr, t.errword.as<uint32_t>(),
+    rtg::launch_trace(sc->bvh->view(sc->slot), rays, M, stride, occluded, inst, nullptr, nullptr, t.errword.as<uint32_t>(),
                       t.stream, d->trace, &t.sorter);
     check_trace_error(t);
     return;
@@ -425,7 +445,7 @@ void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigne
     RTG_HIP(hipMemcpyAsync(t.staging.ptr, pack.data(), pack.size(), hipMemcpyHostToDevice, t.stream));
     RTG_HIP(hipStreamSynchronize(t.stream));
   }
-  rtg::launch_trace(sc->bvh.view(), t.staging.ptr, M, rec, occluded, inst, nullptr, nullptr,
+  rtg::launch_trace(sc->bvh->view(sc->slot), t.staging.ptr, M, rec, occluded, inst, nullptr, nullptr,
                     t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter);
   if (stride == rec) {
     RTG_HIP(hipMemcpyAsync(rays, t.staging.ptr, (size_t)M * rec, hipMemcpyDeviceToHost, t.stream));
@@ -944,7 +964,7 @@ RTC_API void rtcGetSceneBounds(RTCScene scene, RTCBounds* b) {
   API_BEGIN
   VERIFY_HANDLE(scene);
   if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
-  const rtg::BuildStats& st = s->bvh.stats;
+  const rtg::SceneSlot st = s->slot < s->bvh->slots.size() ? s->bvh->slots[s->slot] : rtg::SceneSlot();
   b->lower_x = st.bounds_lo[0]; b->lower_y = st.bounds_lo[1]; b->lower_z = st.bounds_lo[2];
   b->upper_x = st.bounds_hi[0]; b->upper_y = st.bounds_hi[1]; b->upper_z = st.bounds_hi[2];
   b->align0 = b->align1 = 0.0f;
@@ -1035,7 +1055,7 @@ static void async_trace(Scene* s, const RTCIntersectContext* ctx, void* rays, un
   d->activate();
   ThreadDev& t = d->td();
   const uint32_t inst = ctx ? ctx->instID[0] : RTC_INVALID_GEOMETRY_ID;
-  rtg::launch_trace(s->bvh.view(), rays, M, stride, occluded, inst, nullptr, nullptr, t.errword.as<uint32_t>(),
+  rtg::launch_trace(s->bvh->view(s->slot), rays, M, stride, occluded, inst, nullptr, nullptr, t.errword.as<uint32_t>(),
                     stream ? (hipStream_t)stream : t.stream, d->trace, &t.sorter);
 }
 
@@ -1082,7 +1102,7 @@ RTC_API void rtcgTimeTrace(RTCScene scene, void* d_rays, const void* d_src, unsi
   for (int i = 0; i < steps; i++) {
     if (d_src) RTG_HIP(hipMemcpyAsync(d_rays, d_src, (size_t)M * byteStride, hipMemcpyDeviceToDevice, t.stream));
     RTG_HIP(hipEventRecord(t.ev0, t.stream));
-    rtg::launch_trace(s->bvh.view(), d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, nullptr, nullptr,
+    rtg::launch_trace(s->bvh->view(s->slot), d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, nullptr, nullptr,
                       t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter);
     RTG_HIP(hipEventRecord(t.ev1, t.stream));
     RTG_HIP(hipEventSynchronize(t.ev1));
@@ -1107,8 +1127,10 @@ RTC_API void rtcgTimeBuild(RTCScene scene, int steps, float* ms_out) {
     std::lock_guard<std::mutex> bl(d->build_mtx);
     RTG_HIP(hipStreamSynchronize(t.stream));
     auto t0 = std::chrono::steady_clock::now();
+    if (s->bvh.use_count() > 1 || s->slot != 0)
+      RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene shares its BVH with a batch: commit it on its own first");
     const rtg::BuildParams p = build_params(d, s->quality, s->flags);
-    rtg::build_bvh(s->descs, p, d->scratch, s->bvh, t.stream);
+    rtg::build_bvh(s->descs, p, d->scratch, *s->bvh, t.stream);
     RTG_HIP(hipStreamSynchronize(t.stream));
     if (ms_out) ms_out[i] = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -1125,11 +1147,11 @@ RTC_API void rtcgTraceCounted(RTCScene scene, void* d_rays, unsigned int M, size
   Device* d = s->dev;
   d->activate();
   ThreadDev& t = d->td();
-  if (s->bvh.stats.num_nodes == 0) {
+  if (s->bvh->empty(s->slot)) {
     RTG_HIP(hipMemsetAsync(d_nodes, 0, (size_t)M * 4, t.stream));
     RTG_HIP(hipMemsetAsync(d_tris, 0, (size_t)M * 4, t.stream));
   } else {
-    rtg::launch_trace(s->bvh.view(), d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, d_nodes, d_tris,
+    rtg::launch_trace(s->bvh->view(s->slot), d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, d_nodes, d_tris,
                       t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter);
   }
   RTG_HIP(hipStreamSynchronize(t.stream));
@@ -1142,7 +1164,7 @@ RTC_API void rtcgGetSceneStats(RTCScene scene, RTCGSceneStats* out) {
   API_BEGIN
   VERIFY_HANDLE(scene);
   VERIFY_HANDLE(out);
-  const rtg::BuildStats& st = s->bvh.stats;
+  const rtg::BuildStats& st = s->bvh->stats;
   memset(out, 0, sizeof(*out));
   out->num_prims = st.num_prims;
   out->num_nodes = st.num_nodes;
@@ -1164,12 +1186,12 @@ RTC_API void rtcgReadBVH(RTCScene scene, void* nodes, void* tris) {
   Device* d = s->dev;
   d->activate();
   ThreadDev& t = d->td();
-  const rtg::BuildStats& st = s->bvh.stats;
+  const rtg::BuildStats& st = s->bvh->stats;
   if (nodes && st.num_nodes)
-    RTG_HIP(hipMemcpyAsync(nodes, s->bvh.nodes.ptr, (size_t)st.num_nodes * sizeof(rtg::QNode8), hipMemcpyDeviceToHost,
+    RTG_HIP(hipMemcpyAsync(nodes, s->bvh->nodes.ptr, (size_t)st.num_nodes * sizeof(rtg::QNode8), hipMemcpyDeviceToHost,
                            t.stream));
   if (tris && st.num_prims)
-    RTG_HIP(hipMemcpyAsync(tris, s->bvh.tris.ptr, (size_t)st.num_prims * sizeof(rtg::TriRec), hipMemcpyDeviceToHost,
+    RTG_HIP(hipMemcpyAsync(tris, s->bvh->tris.ptr, (size_t)st.num_prims * sizeof(rtg::TriRec), hipMemcpyDeviceToHost,
                            t.stream));
   RTG_HIP(hipStreamSynchronize(t.stream));
   API_END(s)
@@ -1182,7 +1204,7 @@ RTC_API void rtcgReadBVH(RTCScene scene, void* nodes, void* tris) {
 namespace {
 SplitModelH* SM(RTCGSplitModel m) { return (SplitModelH*)m; }
 
-rtg::SceneNorm scene_norm(const rtg::BuildStats& st) {
+rtg::SceneNorm scene_norm(const rtg::SceneSlot& st) {
   rtg::SceneNorm sn;
   float e[3];
   for (int k = 0; k < 3; k++) {
@@ -1317,13 +1339,13 @@ RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const v
   if (M && !d_rays) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
   if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
   if (byteStride < sizeof(RTCRayHit) || byteStride % 4) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid stride");
-  if (M == 0 || s->bvh.stats.num_nodes == 0) return;
+  if (M == 0 || s->bvh->empty(s->slot)) return;
   std::lock_guard<std::mutex> lk(m->mtx);
   std::lock_guard<std::mutex> slk(s->mtx);
   Device* d = s->dev;
   d->activate();
   ThreadDev& t = d->td();
-  const uint32_t nn = s->bvh.stats.num_nodes;
+  const uint32_t nn = s->bvh->stats.num_nodes;
   rtg::DevBuf copy;
   copy.reserve((size_t)M * byteStride);
   RTG_HIP(hipMemcpyAsync(copy.ptr, d_rays, (size_t)M * byteStride, hipMemcpyDeviceToDevice, t.stream));
@@ -1331,10 +1353,10 @@ RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const v
   m->m.counts.reserve((size_t)M * 8);
   RTG_HIP(hipMemsetAsync(m->m.visits.ptr, 0, (size_t)nn * 4, t.stream));
   uint32_t* cn = m->m.counts.as<uint32_t>();
-  rtg::launch_trace(s->bvh.view(), copy.ptr, M, byteStride, false, RTC_INVALID_GEOMETRY_ID, cn, cn + M,
+  rtg::launch_trace(s->bvh->view(s->slot), copy.ptr, M, byteStride, false, RTC_INVALID_GEOMETRY_ID, cn, cn + M,
                     t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter, m->m.visits.as<uint32_t>());
-  const rtg::SceneNorm sn = m->m.has_norm ? m->m.norm : scene_norm(s->bvh.stats);
-  rtg::mlp_append_samples(m->m, (const float*)s->bvh.info.ptr, m->m.visits.as<uint32_t>(), nn, M, sn, t.stream);
+  const rtg::SceneNorm sn = m->m.has_norm ? m->m.norm : scene_norm(s->bvh->slots[s->slot]);
+  rtg::mlp_append_samples(m->m, (const float*)s->bvh->info.ptr, m->m.visits.as<uint32_t>(), nn, M, sn, t.stream);
   m->m.norm = sn;
   m->m.has_norm = true;
   RTG_HIP(hipStreamSynchronize(t.stream));
@@ -1408,6 +1430,220 @@ RTC_API void rtcgSetSceneSplitModel(RTCScene scene, RTCGSplitModel model) {
   if (s->model) ref_dec(s->model);
   s->model = m;
   API_END(s)
+}
+
+// ------------------------------------------------------------ batches --
+
+RTC_API void rtcgCommitScenes(RTCScene* scenes, unsigned int numScenes) {
+  Scene* s0 = (numScenes && scenes) ? SC(scenes[0]) : nullptr;
+  API_BEGIN
+  if (!scenes || numScenes == 0 || !s0) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  if (numScenes > 65535) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "at most 65535 scenes per batch");
+  Device* d = s0->dev;
+  std::vector<Scene*> sc(numScenes);
+  for (unsigned i = 0; i < numScenes; i++) {
+    sc[i] = SC(scenes[i]);
+    if (!sc[i]) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+    if (sc[i]->dev != d) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "scenes of a batch must share a device");
+    if ((sc[i]->flags & RTC_SCENE_FLAG_ROBUST) != (s0->flags & RTC_SCENE_FLAG_ROBUST))
+      RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "scenes of a batch must agree on RTC_SCENE_FLAG_ROBUST");
+    if (sc[i]->model) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "batched commits use the plain split cost");
+    for (unsigned j = 0; j < i; j++)
+      if (sc[j] == sc[i]) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "a scene appears twice in the batch");
+  }
+  d->activate();
+  ThreadDev& t = d->td();
+  // lock in address order (no deadlock with other batch commits)
+  std::vector<Scene*> order(sc);
+  std::sort(order.begin(), order.end());
+  std::vector<std::unique_lock<std::mutex>> locks;
+  for (Scene* x : order) locks.emplace_back(x->mtx);
+  uint32_t prim_offset = 0;
+  std::vector<rtg::GeomDesc> descs;
+  for (unsigned i = 0; i < numScenes; i++) upload_geometry(sc[i], d, t, i, prim_offset, descs);
+  auto store = std::make_shared<rtg::DeviceBVH>();
+  {
+    std::lock_guard<std::mutex> bl(d->build_mtx);
+    rtg::BuildParams p = build_params(d, s0->quality, s0->flags);
+    p.builder = rtg::BUILDER_SAH;
+    d->monitor((ssize_t)prim_offset * (80 + 48));
+    rtg::build_bvh(descs, p, d->scratch, *store, t.stream, numScenes);
+  }
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  for (unsigned i = 0; i < numScenes; i++) {
+    sc[i]->bvh = store;
+    sc[i]->slot = i;
+    sc[i]->committed = true;
+  }
+  API_END(s0)
+}
+
+}  // extern "C"
+
+namespace {
+
+// Validated device-side table of a batched traversal.
+struct PreparedBatch {
+  Scene* first = nullptr;
+  rtg::DevScene sc{};
+  std::vector<rtg::BatchItem> items;
+  uint32_t total = 0;
+  bool aligned = true;
+};
+
+PreparedBatch prepare_batch(const RTCGBatchItem* items, unsigned n, bool occluded) {
+  if (!items || n == 0) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  PreparedBatch pb;
+  const size_t minstride = occluded ? sizeof(RTCRay) : sizeof(RTCRayHit);
+  const rtg::DeviceBVH* store = nullptr;
+  uint64_t total = 0;
+  for (unsigned i = 0; i < n; i++) {
+    Scene* s = SC(items[i].scene);
+    if (!s) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+    if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+    if (!pb.first) {
+      pb.first = s;
+      store = s->bvh.get();
+    } else if (s->bvh.get() != store) {
+      RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scenes of a batched traversal must come from one rtcgCommitScenes call");
+    }
+    if (items[i].numRays == 0 || s->bvh->empty(s->slot)) continue;  // nothing to hit: records stay as they are
+    if (!items[i].rays) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+    if (items[i].byteStride < minstride || items[i].byteStride % 4)
+      RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid stride");
+    if (!is_device_ptr(s->dev, items[i].rays)) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "batched rays must be in device memory");
+    rtg::BatchItem b;
+    b.rays = (char*)items[i].rays;
+    b.stride = items[i].byteStride;
+    b.first = (uint32_t)total;
+    b.count = items[i].numRays;
+    b.root = s->bvh->view(s->slot).root;
+    b.pad = 0;
+    pb.aligned &= ((uintptr_t)b.rays % 16 == 0) && (b.stride % 16 == 0);
+    total += b.count;
+    if (total > 0xffffffffull) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "too many rays in one batch");
+    pb.items.push_back(b);
+  }
+  pb.total = (uint32_t)total;
+  pb.sc = pb.first->bvh->view(0);
+  return pb;
+}
+
+}  // namespace
+
+struct BatchGraphH : RefObj {
+  Device* dev;
+  std::vector<Scene*> scenes;  // retained: their arrays must outlive the graph
+  rtg::DevBuf table, counter, err;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  explicit BatchGraphH(Device* d) : dev(d) { ref_inc(d); }
+  ~BatchGraphH() override {
+    (void)hipSetDevice(dev->ordinal);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (Scene* s : scenes) ref_dec(s);
+    ref_dec(dev);
+  }
+};
+
+Device* dev_of(BatchGraphH* g) { return g ? g->dev : nullptr; }
+
+extern "C" {
+
+RTC_API void rtcgTraceBatch(const RTCGBatchItem* items, unsigned int numItems, int occluded) {
+  Scene* s0 = (items && numItems) ? SC(items[0].scene) : nullptr;
+  API_BEGIN
+  PreparedBatch pb = prepare_batch(items, numItems, occluded != 0);
+  if (pb.items.empty()) return;
+  Device* d = pb.first->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  t.batch.reserve(pb.items.size() * sizeof(rtg::BatchItem));
+  t.batch_counter.reserve(256);
+  RTG_HIP(hipMemcpyAsync(t.batch.ptr, pb.items.data(), pb.items.size() * sizeof(rtg::BatchItem), hipMemcpyHostToDevice,
+                         t.stream));
+  rtg::launch_trace_batch(pb.sc, t.batch.as<rtg::BatchItem>(), (uint32_t)pb.items.size(), pb.total, occluded != 0,
+                          pb.aligned, t.errword.as<uint32_t>(), t.batch_counter.as<uint32_t>(), t.stream, d->trace);
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  check_trace_error(t);
+  API_END(s0)
+}
+
+RTC_API RTCGBatchGraph rtcgNewBatchGraph(const RTCGBatchItem* items, unsigned int numItems, int occluded) {
+  Scene* s0 = (items && numItems) ? SC(items[0].scene) : nullptr;
+  API_BEGIN
+  PreparedBatch pb = prepare_batch(items, numItems, occluded != 0);
+  Device* d = pb.first->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  auto* g = new BatchGraphH(d);
+  try {
+    for (unsigned i = 0; i < numItems; i++) {
+      Scene* s = SC(items[i].scene);
+      ref_inc(s);
+      g->scenes.push_back(s);
+    }
+    g->table.reserve(pb.items.size() * sizeof(rtg::BatchItem) + 16);
+    g->counter.reserve(256);
+    g->err.reserve(256);
+    if (!pb.items.empty())
+      RTG_HIP(hipMemcpyAsync(g->table.ptr, pb.items.data(), pb.items.size() * sizeof(rtg::BatchItem),
+                             hipMemcpyHostToDevice, t.stream));
+    RTG_HIP(hipMemsetAsync(g->err.ptr, 0, 4, t.stream));
+    RTG_HIP(hipStreamSynchronize(t.stream));
+    // capture: counter clear (a kernel, not a memset node) + the persistent traversal
+    RTG_HIP(hipStreamBeginCapture(t.stream, hipStreamCaptureModeThreadLocal));
+    hipError_t ce = hipSuccess;
+    try {
+      rtg::launch_trace_batch(pb.sc, g->table.as<rtg::BatchItem>(), (uint32_t)pb.items.size(), pb.total,
+                              occluded != 0, pb.aligned, g->err.as<uint32_t>(), g->counter.as<uint32_t>(), t.stream,
+                              d->trace);
+    } catch (...) {
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(t.stream, &tmp);
+      if (tmp) (void)hipGraphDestroy(tmp);
+      throw;
+    }
+    ce = hipStreamEndCapture(t.stream, &g->graph);
+    RTG_HIP(ce);
+    RTG_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+  } catch (...) {
+    ref_dec(g);
+    throw;
+  }
+  return (RTCGBatchGraph)g;
+  API_END(s0)
+  return nullptr;
+}
+
+RTC_API void rtcgLaunchBatchGraph(RTCGBatchGraph graph, void* hipStream) {
+  BatchGraphH* g = (BatchGraphH*)graph;
+  API_BEGIN
+  VERIFY_HANDLE(graph);
+  g->dev->activate();
+  ThreadDev& t = g->dev->td();
+  hipStream_t st = hipStream ? (hipStream_t)hipStream : t.stream;
+  RTG_HIP(hipGraphLaunch(g->exec, st));
+  if (!hipStream) {
+    uint32_t err = 0;
+    RTG_HIP(hipMemcpyAsync(&err, g->err.ptr, 4, hipMemcpyDeviceToHost, st));
+    RTG_HIP(hipStreamSynchronize(st));
+    if (err) {
+      RTG_HIP(hipMemsetAsync(g->err.ptr, 0, 4, st));
+      RTG_HIP(hipStreamSynchronize(st));
+      RTC_THROW(RTC_ERROR_UNKNOWN, (err & 1u) ? "traversal stack overflow" : "corrupt acceleration structure");
+    }
+  }
+  API_END(g)
+}
+
+RTC_API void rtcgReleaseBatchGraph(RTCGBatchGraph graph) {
+  BatchGraphH* g = (BatchGraphH*)graph;
+  API_BEGIN
+  VERIFY_HANDLE(graph);
+  ref_dec(g);
+  API_END((BatchGraphH*)nullptr)
 }
 
 }  // extern "C"

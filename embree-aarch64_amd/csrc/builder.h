@@ -38,38 +38,53 @@ struct BuildStats {
   double ms_phase[8] = {0};    // primrefs, morton, sort, tree, refit, collapse
 };
 
-// Owns the device arrays of one built scene.
+// One scene of a build: its root node and bounds.
+struct SceneSlot {
+  float bounds_lo[3] = {INFINITY, INFINITY, INFINITY};
+  float bounds_hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  uint32_t num_prims = 0;
+  uint32_t root = 0;  // QNode8 index (scene s of a batch has root s)
+};
+
+// Owns the device arrays of one build: one scene, or every scene of a
+// batched build (they share the node and triangle arrays).
 struct DeviceBVH {
   DevBuf nodes, tris, info;
-  DevScene view() const {
+  std::vector<SceneSlot> slots;
+  DevScene view(uint32_t slot = 0) const {
     DevScene s;
     s.nodes = nodes.as<QNode8>();
     s.tris = tris.as<TriRec>();
     s.num_nodes = stats.num_nodes;
     s.num_tris = stats.num_prims;
     s.robust = robust;
+    const SceneSlot& sl = slot < slots.size() ? slots[slot] : SceneSlot();
+    s.root = sl.root;
     for (int k = 0; k < 3; k++) {
-      const float ext = stats.bounds_hi[k] - stats.bounds_lo[k];
-      s.lo[k] = stats.bounds_lo[k];
+      const float ext = sl.bounds_hi[k] - sl.bounds_lo[k];
+      s.lo[k] = sl.bounds_lo[k];
       s.scale[k] = ext > 0.0f ? 1023.99f / ext : 0.0f;
     }
     return s;
   }
-  BuildStats stats;
+  bool empty(uint32_t slot = 0) const { return stats.num_nodes == 0 || slot >= slots.size() || slots[slot].num_prims == 0; }
+  BuildStats stats;  // of the whole build
   uint32_t robust = 0;
 };
 
 // Scratch reused across builds on one device.
 struct BuildScratch {
   DevBuf geoms, lo4, hi4, keys, vals, slo, shi, child, parent_int, parent_leaf, range, nlo, nhi, cost,
-      isleaf, split8, visit, queue, counters;
+      isleaf, split8, visit, queue, counters, sacc;
   RadixSorter sorter;
   SahScratch sah;
 };
 
-// Builds over the given geometries (device pointers in GeomDesc).
+// Builds over the given geometries (device pointers in GeomDesc). With
+// nscenes > 1 every GeomDesc names its scene, the tree is the binned-SAH
+// forest of all scenes in one pass, and scene s gets root node s.
 // Throws hip_error on a HIP failure.
 void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& p, BuildScratch& scratch,
-               DeviceBVH& out, hipStream_t s);
+               DeviceBVH& out, hipStream_t s, uint32_t nscenes = 1);
 
 }  // namespace rtg

@@ -26,7 +26,7 @@ struct BuildCounters {
   uint32_t max_depth;
   uint32_t error;  // bit 0: refit walk too long, bit 1: collapse index out of range, bit 2: bad tree
   uint32_t max_stack;  // most traversal stack entries any root-to-node path can need
-  uint32_t pad;
+  uint32_t roots_seen; // tree roots met by the tree check
   double sah_nodes, sah_leaves, sah_leaves_tri;
 };
 
@@ -58,7 +58,34 @@ __device__ inline uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
-__global__ void k_init_counters(BuildCounters* c) {
+// Per-scene reductions of a batched build.
+struct SceneAcc {
+  uint32_t glo[3], ghi[3], clo[3], chi[3];
+  uint32_t valid;
+  uint32_t pad[3];
+};
+
+__global__ void k_init_sacc(SceneAcc* a, uint32_t n) {
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  if (i >= n) return;
+  for (int k = 0; k < 3; k++) {
+    a[i].glo[k] = a[i].clo[k] = f2o(INFINITY);
+    a[i].ghi[k] = a[i].chi[k] = f2o(-INFINITY);
+  }
+  a[i].valid = 0;
+}
+
+__global__ void k_fill_none(uint32_t* v, uint32_t n) {
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  if (i < n) v[i] = 0xffffffffu;
+}
+
+__global__ void k_iota(uint32_t* v, uint32_t n) {
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  if (i < n) v[i] = i;
+}
+
+__global__ void k_init_counters(BuildCounters* c, uint32_t nscenes) {
   for (int k = 0; k < 3; k++) {
     c->glo[k] = f2o(INFINITY);
     c->ghi[k] = f2o(-INFINITY);
@@ -67,7 +94,8 @@ __global__ void k_init_counters(BuildCounters* c) {
   }
   c->maxabs = f2o(0.0f);
   c->valid = 0;
-  c->node_count = 1;
+  c->node_count = nscenes;
+  c->roots_seen = 0;
   c->tri_count = 0;
   c->qcount[0] = c->qcount[1] = 0;
   c->max_depth = 0;
@@ -83,7 +111,7 @@ __device__ inline bool finite_v(float x) { return (x > -FLT_LARGE_) & (x < FLT_L
 __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ geoms, uint32_t ngeoms,
                                                  uint32_t total, float4* __restrict__ lo4,
                                                  float4* __restrict__ hi4, uint32_t* __restrict__ keys,
-                                                 BuildCounters* __restrict__ cnt) {
+                                                 BuildCounters* __restrict__ cnt, SceneAcc* __restrict__ sacc) {
   const uint32_t i = blockIdx.x * BT + threadIdx.x;
   float b[13];
 #pragma unroll
@@ -94,7 +122,7 @@ __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ ge
     b[9 + k] = -INFINITY;   // cent hi
   }
   b[12] = 0.0f;
-  uint32_t valid = 0;
+  uint32_t valid = 0, scene = 0xffffffffu;
   if (i < total) {
     uint32_t lo = 0, hi = ngeoms - 1;
     while (lo < hi) {  // last geometry with prim_offset <= i
@@ -132,7 +160,8 @@ __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ ge
       }
       lo4[i] = make_float4(l[0], l[1], l[2], __uint_as_float(lo));
       hi4[i] = make_float4(h[0], h[1], h[2], __uint_as_float(prim));
-      keys[i] = 0u;
+      keys[i] = g.scene;
+      scene = g.scene;
       valid = 1;
     } else {
       keys[i] = 0xffffffffu;
@@ -164,6 +193,46 @@ __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ ge
     uint32_t s = 0;
     for (int w = 0; w < BT / 64; w++) s += redc[w];
     if (s) atomicAdd(&cnt->valid, s);
+  }
+  if (!sacc) return;
+  // per-scene bounds: the block's reduction when all its valid primitives
+  // belong to one scene, else one set of atomics per primitive
+  __shared__ uint32_t s_min, s_max;
+  if (threadIdx.x == 0) {
+    s_min = 0xffffffffu;
+    s_max = 0u;
+  }
+  __syncthreads();
+  if (valid) {
+    atomicMin(&s_min, scene);
+    atomicMax(&s_max, scene);
+  }
+  __syncthreads();
+  if (s_min == 0xffffffffu) return;
+  if (s_min == s_max) {
+    SceneAcc* a = sacc + s_min;
+    if (threadIdx.x < 12) {
+      const int k = threadIdx.x;
+      const bool mx = (k >= 3 && k < 6) || (k >= 9);
+      float x = red[0][k];
+      for (int w = 1; w < BT / 64; w++) x = mx ? fmaxf(x, red[w][k]) : fminf(x, red[w][k]);
+      uint32_t* dst = (k < 3) ? &a->glo[k] : (k < 6) ? &a->ghi[k - 3] : (k < 9) ? &a->clo[k - 6] : &a->chi[k - 9];
+      if (mx) atomicMax(dst, f2o(x)); else atomicMin(dst, f2o(x));
+    }
+    if (threadIdx.x == 0) {
+      uint32_t s = 0;
+      for (int w = 0; w < BT / 64; w++) s += redc[w];
+      atomicAdd(&a->valid, s);
+    }
+  } else if (valid) {
+    SceneAcc* a = sacc + scene;
+    for (int k = 0; k < 3; k++) {
+      atomicMin(&a->glo[k], f2o(b[k]));
+      atomicMax(&a->ghi[k], f2o(b[3 + k]));
+      atomicMin(&a->clo[k], f2o(b[6 + k]));
+      atomicMax(&a->chi[k], f2o(b[9 + k]));
+    }
+    atomicAdd(&a->valid, 1u);
   }
 }
 
@@ -356,7 +425,7 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
 // Integrity check of the sorted keys and the radix tree (cheap, one pass):
 // bit 3 keys out of order, bit 4 inner node not finished by the refit,
 // bit 5 parent/child links disagree.
-__global__ __launch_bounds__(BT) void k_check_tree(const uint32_t* __restrict__ keys, uint32_t n, uint32_t root,
+__global__ __launch_bounds__(BT) void k_check_tree(const uint32_t* __restrict__ keys, uint32_t n, uint32_t* roots_seen,
                                                    const uint32_t* __restrict__ child,
                                                    const uint32_t* __restrict__ parent_int,
                                                    const uint32_t* __restrict__ parent_leaf,
@@ -365,9 +434,10 @@ __global__ __launch_bounds__(BT) void k_check_tree(const uint32_t* __restrict__ 
   if (keys && i + 1 < n && keys[i] > keys[i + 1]) atomicOr(err, 8u);
   if (i < n) {
     const uint32_t p = parent_leaf[i];
-    if (p >= n - 1 || (child[2 * p] != (i | REF_LEAF) && child[2 * p + 1] != (i | REF_LEAF))) atomicOr(err, 32u);
+    if (p == NONE) atomicAdd(roots_seen, 1u);  // a scene of one primitive
+    else if (p >= n - 1 || (child[2 * p] != (i | REF_LEAF) && child[2 * p + 1] != (i | REF_LEAF))) atomicOr(err, 32u);
   }
-  if (i + 1 < n) {
+  if (i + 1 < n && parent_int[i] != REF_EMPTY) {
     if (visit[i] != 2u) atomicOr(err, 16u);
 #pragma unroll
     for (int k = 0; k < 2; k++) {
@@ -376,7 +446,9 @@ __global__ __launch_bounds__(BT) void k_check_tree(const uint32_t* __restrict__ 
                                      : (c + 1 < n && parent_int[c] == i);
       if (!ok) atomicOr(err, 32u);
     }
-    if (i != root) {
+    if (parent_int[i] == NONE) {
+      atomicAdd(roots_seen, 1u);
+    } else {
       const uint32_t p = parent_int[i];
       if (p >= n - 1 || (child[2 * p] != i && child[2 * p + 1] != i)) atomicOr(err, 32u);
     }
@@ -406,7 +478,8 @@ struct CollapseArgs {
   TriRec* tris;
   NodeInfo* info;
   BuildCounters* cnt;
-  uint32_t n;  // capacity of nodes, tris and each queue
+  uint32_t n;         // capacity of nodes and of each queue
+  uint32_t tri_cap;   // capacity of tris
   uint32_t robust;
 };
 
@@ -447,6 +520,31 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
   if (t >= a.nin) return;
   const QItem item = a.qin[t];
   const float eps = o2f(a.cnt->maxabs) * BOX_EPS_REL;
+  if (item.bref == REF_EMPTY) {  // root of a scene without primitives: a node with no children
+    if (item.node >= a.n) {
+      atomicOr(&a.cnt->error, 2u);
+      return;
+    }
+    QNode8 q;
+    q.px = q.py = q.pz = 0.0f;
+    q.ex = q.ey = q.ez = 127;
+    q.imask = 0;
+    q.child_base = q.tri_base = 0;
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+      q.meta[k] = 0;
+      q.qlox[k] = q.qloy[k] = q.qloz[k] = 255;
+      q.qhix[k] = q.qhiy[k] = q.qhiz[k] = 0;
+    }
+    a.nodes[item.node] = q;
+    NodeInfo inf;
+    inf.lo[0] = inf.lo[1] = inf.lo[2] = 0.0f;
+    inf.hi[0] = inf.hi[1] = inf.hi[2] = 0.0f;
+    inf.count = 0;
+    inf.depth = item.depth;
+    a.info[item.node] = inf;
+    return;
+  }
 
   // children: the optimal distribution of 8 slots below item.bref
   uint32_t ch[8];
@@ -536,7 +634,7 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
   const uint32_t child_base = ni ? atomicAdd(&a.cnt->node_count, ni) : 0u;
   const uint32_t tri_base = nt ? atomicAdd(&a.cnt->tri_count, nt) : 0u;
   const uint32_t qbase = ni ? atomicAdd(&a.cnt->qcount[a.qout_slot], ni) : 0u;
-  if ((uint64_t)child_base + ni > a.n || (uint64_t)tri_base + nt > a.n || (uint64_t)qbase + ni > a.n ||
+  if ((uint64_t)child_base + ni > a.n || (uint64_t)tri_base + nt > a.tri_cap || (uint64_t)qbase + ni > a.n ||
       item.node >= a.n) {
     atomicOr(&a.cnt->error, 2u);
     return;
@@ -631,12 +729,13 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
 // ------------------------------------------------------------------ host --
 
 void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, BuildScratch& S, DeviceBVH& out,
-               hipStream_t s) {
+               hipStream_t s, uint32_t nscenes) {
   auto t0 = std::chrono::steady_clock::now();
   out.stats = BuildStats();
   out.robust = prm.robust ? 1u : 0u;
   uint32_t total = 0;
   for (const auto& g : geoms) total += g.num_tris;
+  out.slots.assign(nscenes, SceneSlot());
   if (geoms.empty() || total == 0) {
     out.stats.num_prims = 0;
     out.stats.num_nodes = 0;
@@ -653,22 +752,72 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   S.vals.reserve((size_t)total * 4);
   BuildCounters* cnt = S.counters.as<BuildCounters>();
 
-  hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(1), 0, s, cnt);
+  if (nscenes == 0 || nscenes > 65535) throw hip_error(hipErrorInvalidValue, "scene count out of range");
+  const bool batched = nscenes > 1;
+  SceneAcc* sacc = nullptr;
+  if (batched) {
+    S.sacc.reserve((size_t)nscenes * sizeof(SceneAcc));
+    sacc = S.sacc.as<SceneAcc>();
+    hipLaunchKernelGGL(k_init_sacc, dim3(div_up(nscenes, BT)), dim3(BT), 0, s, sacc, nscenes);
+  }
+  hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(1), 0, s, cnt, nscenes);
   hipLaunchKernelGGL(k_primrefs, dim3(div_up(total, BT)), dim3(BT), 0, s, S.geoms.as<GeomDesc>(),
-                     (uint32_t)geoms.size(), total, S.lo4.as<float4>(), S.hi4.as<float4>(), S.keys.as<uint32_t>(), cnt);
-  hipLaunchKernelGGL(k_morton, dim3(div_up(total, BT)), dim3(BT), 0, s, S.lo4.as<float4>(), S.hi4.as<float4>(),
-                     total, S.keys.as<uint32_t>(), S.vals.as<uint32_t>(), cnt);
-  RTG_HIP(hipGetLastError());
-  S.sorter.sort_pairs(S.keys.as<uint32_t>(), S.vals.as<uint32_t>(), total, 0, 32, s);
+                     (uint32_t)geoms.size(), total, S.lo4.as<float4>(), S.hi4.as<float4>(), S.keys.as<uint32_t>(), cnt,
+                     sacc);
+  if (batched) {
+    // group the references by scene (stable, so each scene keeps primref order); invalid ones sort last
+    hipLaunchKernelGGL(k_iota, dim3(div_up(total, BT)), dim3(BT), 0, s, S.vals.as<uint32_t>(), total);
+    RTG_HIP(hipGetLastError());
+    S.sorter.sort_pairs(S.keys.as<uint32_t>(), S.vals.as<uint32_t>(), total, 0, 16, s);
+  } else {
+    hipLaunchKernelGGL(k_morton, dim3(div_up(total, BT)), dim3(BT), 0, s, S.lo4.as<float4>(), S.hi4.as<float4>(),
+                       total, S.keys.as<uint32_t>(), S.vals.as<uint32_t>(), cnt);
+    RTG_HIP(hipGetLastError());
+    S.sorter.sort_pairs(S.keys.as<uint32_t>(), S.vals.as<uint32_t>(), total, 0, 32, s);
+  }
 
   BuildCounters hc;
   RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
+  std::vector<SceneAcc> hs(batched ? nscenes : 0);
+  if (batched) RTG_HIP(hipMemcpyAsync(hs.data(), sacc, nscenes * sizeof(SceneAcc), hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));
   const uint32_t n = hc.valid;
   out.stats.num_prims = n;
   for (int k = 0; k < 3; k++) {
     out.stats.bounds_lo[k] = o2f(hc.glo[k]);
     out.stats.bounds_hi[k] = o2f(hc.ghi[k]);
+  }
+  // scenes: position ranges, bounds, and the binary-tree reference of each root
+  out.slots.assign(nscenes, SceneSlot());
+  std::vector<uint32_t> first(nscenes, 0), count(nscenes, 0);
+  std::vector<SahRoot> trees;
+  std::vector<uint32_t> tree_scene;
+  {
+    uint32_t pos = 0;
+    for (uint32_t sI = 0; sI < nscenes; sI++) {
+      SceneSlot& sl = out.slots[sI];
+      sl.root = sI;
+      count[sI] = batched ? hs[sI].valid : n;
+      first[sI] = pos;
+      pos += count[sI];
+      sl.num_prims = count[sI];
+      for (int k = 0; k < 3; k++) {
+        sl.bounds_lo[k] = o2f(batched ? hs[sI].glo[k] : hc.glo[k]);
+        sl.bounds_hi[k] = o2f(batched ? hs[sI].ghi[k] : hc.ghi[k]);
+      }
+      if (count[sI] >= 2) {
+        SahRoot r;
+        r.first = first[sI];
+        r.count = count[sI];
+        for (int k = 0; k < 3; k++) {
+          r.clo[k] = o2f(batched ? hs[sI].clo[k] : hc.clo[k]);
+          r.chi[k] = o2f(batched ? hs[sI].chi[k] : hc.chi[k]);
+        }
+        trees.push_back(r);
+        tree_scene.push_back(sI);
+      }
+    }
+    if (pos != n) throw hip_error(hipErrorUnknown, "per-scene primitive counts disagree with the total");
   }
   if (n == 0) {
     out.stats.num_nodes = 0;
@@ -691,20 +840,20 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   S.split8.reserve((size_t)ninner * 8);
   S.isleaf.reserve((size_t)ninner);
   S.visit.reserve((size_t)ninner * 4);
-  uint32_t root_bin = 0;
-  if (n > 1) {
+  std::vector<uint32_t> root_ref(nscenes, REF_EMPTY);
+  for (uint32_t sI = 0; sI < nscenes; sI++)
+    if (count[sI] == 1) root_ref[sI] = first[sI] | REF_LEAF;
+  const bool use_sah = batched || prm.builder == BUILDER_SAH || prm.learned;
+  if (!trees.empty()) {
     RTG_HIP(hipMemsetAsync(S.visit.ptr, 0, (size_t)ninner * 4, s));
-    if (prm.builder == BUILDER_SAH) {
-      float clo[3], chi[3];
-      for (int k = 0; k < 3; k++) {
-        clo[k] = o2f(hc.clo[k]);
-        chi[k] = o2f(hc.chi[k]);
-      }
+    if (batched)
+      hipLaunchKernelGGL(k_fill_none, dim3(div_up(n, BT)), dim3(BT), 0, s, S.parent_leaf.as<uint32_t>(), n);
+    if (use_sah) {
       SahTreeOut so;
-      build_sah_tree(n, S.vals.as<uint32_t>(), S.slo.as<float4>(), S.shi.as<float4>(), clo, chi,
-                     S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
-                     S.range.as<uint2>(), prm.learned, S.sah, so, s);
-      root_bin = so.root;
+      build_sah_tree(n, S.vals.as<uint32_t>(), S.slo.as<float4>(), S.shi.as<float4>(), trees, S.child.as<uint32_t>(),
+                     S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(), prm.learned, S.sah,
+                     so, s);
+      for (size_t k = 0; k < trees.size(); k++) root_ref[tree_scene[k]] = so.roots[k];
       // boxes in the final primitive order
       hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(n, BT)), dim3(BT), 0, s, S.vals.as<uint32_t>(),
                          S.lo4.as<float4>(), S.hi4.as<float4>(), n, S.slo.as<float4>(), S.shi.as<float4>());
@@ -712,6 +861,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
       hipLaunchKernelGGL(k_radix_tree, dim3(div_up(n - 1, BT)), dim3(BT), 0, s, S.keys.as<uint32_t>(), (int)n,
                          S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
                          S.range.as<uint2>());
+      root_ref[0] = 0;
     }
     hipLaunchKernelGGL(k_refit, dim3(div_up(n, BT)), dim3(BT), 0, s, n, S.child.as<uint32_t>(),
                        S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(),
@@ -719,25 +869,32 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
                        S.cost.as<float>(), S.split8.as<uint8_t>(), S.isleaf.as<uint8_t>(), S.visit.as<uint32_t>(),
                        prm.cost_node, prm.cost_tri, max_leaf, &cnt->error);
     hipLaunchKernelGGL(k_check_tree, dim3(div_up(n, BT)), dim3(BT), 0, s,
-                       prm.builder == BUILDER_SAH ? nullptr : S.keys.as<uint32_t>(), n, root_bin,
+                       use_sah ? nullptr : S.keys.as<uint32_t>(), n, &cnt->roots_seen,
                        S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
                        S.visit.as<uint32_t>(), &cnt->error);
     RTG_HIP(hipGetLastError());
     // the collapse follows child links: stop here if the tree is inconsistent
-    RTG_HIP(hipMemcpyAsync(&hc.error, &cnt->error, 4, hipMemcpyDeviceToHost, s));
+    RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
     RTG_HIP(hipStreamSynchronize(s));
-    if (hc.error) throw hip_error(hipErrorUnknown, "binary tree check failed: error bits " + std::to_string(hc.error));
+    uint32_t expect = (uint32_t)trees.size();
+    for (uint32_t sI = 0; sI < nscenes; sI++) expect += count[sI] == 1 ? 1u : 0u;
+    if (hc.error || hc.roots_seen != expect)
+      throw hip_error(hipErrorUnknown, "binary tree check failed: error bits " + std::to_string(hc.error) +
+                                           ", roots " + std::to_string(hc.roots_seen) + " of " +
+                                           std::to_string(expect));
   }
 
-  // outputs: at most n nodes, exactly n triangle records
-  out.nodes.reserve((size_t)n * sizeof(QNode8));
+  // outputs: at most n + nscenes nodes, exactly n triangle records
+  const uint32_t cap = n + nscenes;
+  out.nodes.reserve((size_t)cap * sizeof(QNode8));
   out.tris.reserve((size_t)n * sizeof(TriRec));
-  out.info.reserve((size_t)n * sizeof(NodeInfo));
-  S.queue.reserve((size_t)2 * n * sizeof(QItem));
+  out.info.reserve((size_t)cap * sizeof(NodeInfo));
+  S.queue.reserve((size_t)2 * cap * sizeof(QItem));
   QItem* q0 = S.queue.as<QItem>();
-  QItem* q1 = q0 + n;
-  QItem root{n > 1 ? root_bin : (0u | REF_LEAF), 0u, 1u, 0u};
-  RTG_HIP(hipMemcpyAsync(q0, &root, sizeof(root), hipMemcpyHostToDevice, s));
+  QItem* q1 = q0 + cap;
+  std::vector<QItem> roots(nscenes);
+  for (uint32_t sI = 0; sI < nscenes; sI++) roots[sI] = QItem{root_ref[sI], sI, 1u, 0u};
+  RTG_HIP(hipMemcpyAsync(q0, roots.data(), nscenes * sizeof(QItem), hipMemcpyHostToDevice, s));
 
   CollapseArgs ca;
   ca.child = S.child.as<uint32_t>();
@@ -756,9 +913,10 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   ca.tris = out.tris.as<TriRec>();
   ca.info = out.info.as<NodeInfo>();
   ca.cnt = cnt;
-  ca.n = n;
+  ca.n = cap;
+  ca.tri_cap = n;
   ca.robust = prm.robust ? 1u : 0u;
-  uint32_t nin = 1;
+  uint32_t nin = nscenes;
   int levels = 0;
   std::string level_sizes;
   int slot = 0;
@@ -778,7 +936,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     std::swap(qin, qout);
     slot ^= 1;
     level_sizes += " " + std::to_string(nin);
-    if (++levels > 4096 || nin > n) throw hip_error(hipErrorUnknown, "BVH collapse did not terminate");
+    if (++levels > 4096 || nin > cap) throw hip_error(hipErrorUnknown, "BVH collapse did not terminate");
   }
   RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));

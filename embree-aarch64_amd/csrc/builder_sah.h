@@ -4,6 +4,8 @@
 // positions), so the bottom-up SAH refit and the 8-wide collapse run on
 // either tree unchanged.
 #pragma once
+#include <vector>
+
 #include "bvh_types.h"
 #include "gpu_util.h"
 #include "mlp.h"
@@ -12,24 +14,33 @@
 namespace rtg {
 
 struct SahScratch {
-  DevBuf idx[2], blo[2], bhi[2], sid[2], segs[3][2], bins, splits, flags, scan, counters;
+  DevBuf idx[2], blo[2], bhi[2], sid[2], segs[3][2], bins, splits, flags, scan, counters, roots;
   Scanner scanner;
 };
 
-struct SahTreeOut {
-  uint32_t root = 0;       // inner node id of the root (n >= 2)
-  uint32_t levels = 0;     // top-down levels run
-  uint32_t max_depth = 0;  // binary depth
+// One tree of the forest: positions [first, first+count), count >= 2.
+struct SahRoot {
+  uint32_t first, count;
+  float clo[3], chi[3];  // bounds of lo+hi over its references
 };
 
-// Inputs: n >= 2 valid references, `vals` holding their primref indices and
-// slo/shi their boxes in that order. Outputs: vals reordered so that every
-// inner node covers a contiguous position range; child/parent_int/
-// parent_leaf/range filled for inner nodes 0..n-2 (inner node ids are split
-// positions); clo/chi are the bounds of lo+hi over all references.
+struct SahTreeOut {
+  std::vector<uint32_t> roots;  // inner node id of each tree's root
+  uint32_t levels = 0;          // top-down levels run
+  uint32_t max_depth = 0;       // binary depth
+};
+
+// Inputs: n valid references, `vals` holding their primref indices and
+// slo/shi their boxes in that order, and the trees to build (disjoint
+// position ranges of at least two references). Outputs: vals reordered so
+// that every inner node covers a contiguous position range; child/
+// parent_int/parent_leaf/range filled for the inner nodes (an inner node's
+// id is its split position, so ids lie in [0, n-2]; ids no tree uses keep
+// parent_int = REF_EMPTY).
 // `learned` (may be null) replaces the half-area term of the split cost.
-void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi, const float clo[3],
-                    const float chi[3], uint32_t* child, uint32_t* parent_int, uint32_t* parent_leaf, uint2* range,
-                    const LearnedCost* learned, SahScratch& S, SahTreeOut& out, hipStream_t s);
+void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi,
+                    const std::vector<SahRoot>& trees, uint32_t* child, uint32_t* parent_int,
+                    uint32_t* parent_leaf, uint2* range, const LearnedCost* learned, SahScratch& S,
+                    SahTreeOut& out, hipStream_t s);
 
 }  // namespace rtg

@@ -48,6 +48,9 @@ __device__ inline int wc(int a, int b) { return a * NB + b; }
 __device__ inline int wl(int a, int b, int k) { return 3 * NB + (a * NB + b) * 3 + k; }
 __device__ inline int wh(int a, int b, int k) { return 3 * NB + 9 * NB + (a * NB + b) * 3 + k; }
 
+// parent of a tree's root segment: ROOT_TAG | tree index
+constexpr uint32_t ROOT_TAG = 0x80000000u;
+
 struct Seg {
   uint32_t first, count, parent, meta;  // meta = depth << 1 | side of the parent
   float clo[3], chi[3];                 // bounds of lo+hi over the segment
@@ -63,7 +66,7 @@ struct Split {
 
 struct Counters {
   uint32_t nseg[3];  // next-level segment counts: big, mid, small
-  uint32_t root;
+  uint32_t pad;
   uint32_t max_depth;
   uint32_t error;    // bit 0: a segment list overflowed, bit 1: bin counts disagree, bit 2: destination out of range
 };
@@ -80,6 +83,7 @@ struct Tree {
   uint2* range;
   uint32_t* vals;
   Counters* cnt;
+  uint32_t* roots;  // per tree
 };
 
 struct Mapping {
@@ -199,8 +203,8 @@ __device__ inline void bins_box(const uint32_t* w, int a, int b0, int b1, float 
 
 // Records inner node m (the split of segment sg) in the tree.
 __device__ inline void link_inner(const Tree& t, const Seg& sg, uint32_t m) {
-  if (sg.parent == NIL) {
-    t.cnt->root = m;
+  if (sg.parent & ROOT_TAG) {
+    t.roots[sg.parent & ~ROOT_TAG] = m;
     t.parent_int[m] = NIL;
   } else {
     t.child[2 * sg.parent + (sg.meta & 1u)] = m;
@@ -615,14 +619,24 @@ __global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, con
   }
 }
 
+// Writes the big-segment id of every position of the initial big segments.
+__global__ void k_fill_seg_ids(uint32_t* sid, const Seg* segs) {
+  const Seg sg = segs[blockIdx.x];
+  for (uint32_t i = threadIdx.x; i < sg.count; i += TBS) sid[sg.first + i] = blockIdx.x;
+}
+
 }  // namespace
 
-void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi, const float clo[3],
-                    const float chi[3], uint32_t* child, uint32_t* parent_int, uint32_t* parent_leaf, uint2* range,
-                    const LearnedCost* learned, SahScratch& S, SahTreeOut& out, hipStream_t s) {
+void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi,
+                    const std::vector<SahRoot>& trees, uint32_t* child, uint32_t* parent_int,
+                    uint32_t* parent_leaf, uint2* range, const LearnedCost* learned, SahScratch& S,
+                    SahTreeOut& out, hipStream_t s) {
   LearnedCost lcost{};
   if (learned) lcost = *learned;
-  if (n < 2) throw hip_error(hipErrorInvalidValue, "SAH tree needs at least two primitives");
+  if (trees.empty()) throw hip_error(hipErrorInvalidValue, "SAH build without a tree");
+  if (trees.size() >= ROOT_TAG) throw hip_error(hipErrorInvalidValue, "too many trees");
+  for (const SahRoot& r : trees)
+    if (r.count < 2 || (uint64_t)r.first + r.count > n) throw hip_error(hipErrorInvalidValue, "bad SAH tree range");
   const uint32_t cap[3] = {n / MID_MAX + 2, n / (SMALL_MAX + 1) + 2, n / 2 + 2};
   for (int p = 0; p < 2; p++) {
     S.idx[p].reserve((size_t)n * 4);
@@ -636,33 +650,48 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
   S.flags.reserve((size_t)n * 4);
   S.scan.reserve((size_t)n * 4);
   S.counters.reserve(sizeof(Counters));
+  S.roots.reserve(trees.size() * 4);
   Counters* cnt = S.counters.as<Counters>();
 
   RTG_HIP(hipMemcpyAsync(S.idx[0].ptr, vals, (size_t)n * 4, hipMemcpyDeviceToDevice, s));
   RTG_HIP(hipMemcpyAsync(S.blo[0].ptr, slo, (size_t)n * 16, hipMemcpyDeviceToDevice, s));
   RTG_HIP(hipMemcpyAsync(S.bhi[0].ptr, shi, (size_t)n * 16, hipMemcpyDeviceToDevice, s));
+  // inner node ids no tree uses stay marked
+  hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n - 1, TBS)), dim3(TBS), 0, s, parent_int, (size_t)(n - 1), REF_EMPTY);
+  hipLaunchKernelGGL(k_fill_u32, dim3(div_up(trees.size(), TBS)), dim3(TBS), 0, s, S.roots.as<uint32_t>(),
+                     trees.size(), NIL);
 
-  Seg root;
-  root.first = 0;
-  root.count = n;
-  root.parent = NIL;
-  root.meta = 0;
-  for (int k = 0; k < 3; k++) {
-    root.clo[k] = clo[k];
-    root.chi[k] = chi[k];
+  std::vector<Seg> init[3];
+  for (size_t i = 0; i < trees.size(); i++) {
+    const SahRoot& r = trees[i];
+    Seg sg;
+    sg.first = r.first;
+    sg.count = r.count;
+    sg.parent = ROOT_TAG | (uint32_t)i;
+    sg.meta = 0;
+    for (int k = 0; k < 3; k++) {
+      sg.clo[k] = r.clo[k];
+      sg.chi[k] = r.chi[k];
+    }
+    init[r.count > MID_MAX ? 0 : (r.count > SMALL_MAX ? 1 : 2)].push_back(sg);
   }
-  const int rcls = n > MID_MAX ? 0 : (n > SMALL_MAX ? 1 : 2);
-  RTG_HIP(hipMemcpyAsync(S.segs[rcls][0].ptr, &root, sizeof(root), hipMemcpyHostToDevice, s));
-  uint32_t cur[3] = {0, 0, 0};
-  cur[rcls] = 1;
-  if (rcls == 0) hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[0].as<uint32_t>(), (size_t)n, 0u);
+  uint32_t cur[3];
+  for (int c = 0; c < 3; c++) {
+    cur[c] = (uint32_t)init[c].size();
+    if (cur[c])
+      RTG_HIP(hipMemcpyAsync(S.segs[c][0].ptr, init[c].data(), init[c].size() * sizeof(Seg), hipMemcpyHostToDevice, s));
+  }
+  if (cur[0]) {
+    hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[0].as<uint32_t>(), (size_t)n, NIL);
+    hipLaunchKernelGGL(k_fill_seg_ids, dim3(cur[0]), dim3(TBS), 0, s, S.sid[0].as<uint32_t>(), S.segs[0][0].as<Seg>());
+  }
   Counters hc{};
-  hc.root = NIL;
   hc.max_depth = 0;
   hc.error = 0;
   RTG_HIP(hipMemcpyAsync(cnt, &hc, sizeof(hc), hipMemcpyHostToDevice, s));
+  RTG_HIP(hipStreamSynchronize(s));  // the host staging above must outlive its copies
 
-  Tree t{child, parent_int, parent_leaf, range, vals, cnt};
+  Tree t{child, parent_int, parent_leaf, range, vals, cnt, S.roots.as<uint32_t>()};
   int par = 0;
   uint32_t levels = 0;
   while (cur[0] + cur[1] + cur[2] > 0) {
@@ -707,8 +736,11 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
     for (int c = 0; c < 3; c++) cur[c] = hc.nseg[c];
     par = o;
   }
-  if (hc.root >= n - 1) throw hip_error(hipErrorUnknown, "SAH build produced no root");
-  out.root = hc.root;
+  out.roots.assign(trees.size(), NIL);
+  RTG_HIP(hipMemcpyAsync(out.roots.data(), S.roots.ptr, trees.size() * 4, hipMemcpyDeviceToHost, s));
+  RTG_HIP(hipStreamSynchronize(s));
+  for (uint32_t r : out.roots)
+    if (r == NIL || r + 1 >= n) throw hip_error(hipErrorUnknown, "SAH build produced no root");
   out.levels = levels;
   out.max_depth = hc.max_depth;
 }

@@ -56,6 +56,8 @@ constexpr uint32_t TRI_LAST = 1u;
 
 // Stack / child reference: bit 31 set = leaf (triangle record index).
 constexpr uint32_t REF_LEAF = 0x80000000u;
+// Binary-tree reference of a scene with no valid primitive (batched builds).
+constexpr uint32_t REF_EMPTY = 0xfffffffeu;
 
 // Per-node side record (same index as the QNode8): the float union box of
 // the node's children and the number of primitives below it. Read by the
@@ -78,6 +80,8 @@ struct GeomDesc {
   uint32_t num_tris;
   uint32_t prim_offset;  // first global primitive index of this geometry
   uint32_t geomID;
+  uint32_t scene;        // scene of a batched build (0 otherwise)
+  uint32_t pad;
 };
 
 // What the traversal kernels need.
@@ -89,6 +93,7 @@ struct DevScene {
   float lo[3];     // scene box, for ray ordering keys
   float scale[3];  // 1023.99 / extent per axis (0 for a flat axis)
   uint32_t robust; // triangle records hold (v0, v1, v2) for the Pluecker test
+  uint32_t root;   // root node of this scene (scenes of a batch share the arrays)
 };
 
 // Entries of the per-ray traversal stack; the builder rejects a tree whose

@@ -280,3 +280,76 @@ RTC_API unsigned int rtcgGenBounceRays(const RTCRayHit* hits, unsigned int M, un
   }
   return k;
 }
+
+namespace {
+
+float lattice3(uint64_t seed, uint32_t oct, int32_t x, int32_t y, int32_t z) {
+  const uint64_t key = ((uint64_t)(uint32_t)x * 73856093ull) ^ ((uint64_t)(uint32_t)y * 19349663ull << 21) ^
+                       ((uint64_t)(uint32_t)z * 83492791ull << 42);
+  return 2.0f * u01(seed, 32 + oct, key) - 1.0f;
+}
+
+float value_noise3(uint64_t seed, uint32_t oct, float x, float y, float z) {
+  const float fx = floorf(x), fy = floorf(y), fz = floorf(z);
+  const int32_t ix = (int32_t)fx, iy = (int32_t)fy, iz = (int32_t)fz;
+  float tx = x - fx, ty = y - fy, tz = z - fz;
+  tx = tx * tx * (3.0f - 2.0f * tx);
+  ty = ty * ty * (3.0f - 2.0f * ty);
+  tz = tz * tz * (3.0f - 2.0f * tz);
+  float c[2][2];
+  for (int dz = 0; dz < 2; dz++)
+    for (int dy = 0; dy < 2; dy++) {
+      const float a = lattice3(seed, oct, ix, iy + dy, iz + dz), b = lattice3(seed, oct, ix + 1, iy + dy, iz + dz);
+      c[dz][dy] = a + (b - a) * tx;
+    }
+  const float e = c[0][0] + (c[0][1] - c[0][0]) * ty, f = c[1][0] + (c[1][1] - c[1][0]) * ty;
+  return e + (f - e) * tz;
+}
+
+}  // namespace
+
+// Unit UV sphere (reference tutorial topology, numPhi rings) displaced
+// radially by amplitude * 3-octave value noise of the surface point
+// (frequency 2*2^o, weight 2^-o). Returns the triangle count; NULL buffers
+// query the sizes as rtcgGenSphere does.
+RTC_API unsigned int rtcgGenDisplacedSphere(unsigned int numPhi, unsigned long long seed, float amplitude,
+                                            float* vertices, unsigned int* indices, unsigned int* numVertices) {
+  const unsigned nt = rtcgGenSphere(numPhi, 0.0f, 0.0f, 0.0f, 1.0f, vertices, indices, numVertices);
+  if (!vertices || !indices) return nt;
+  const unsigned nv = 2 * numPhi * (numPhi + 1);
+  for (unsigned i = 0; i < nv; i++) {
+    float* v = vertices + 3 * (size_t)i;
+    float h = 0.0f, w = 1.0f, f = 2.0f;
+    for (unsigned o = 0; o < 3; o++) {
+      h += w * value_noise3(seed, o, v[0] * f, v[1] * f, v[2] * f);
+      w *= 0.5f;
+      f *= 2.0f;
+    }
+    const float s = 1.0f + amplitude * h;
+    v[0] *= s;
+    v[1] *= s;
+    v[2] *= s;
+  }
+  return nt;
+}
+
+// Rays from uniform points on the sphere of the given radius towards
+// uniform points inside the unit ball (closest-hit records).
+RTC_API void rtcgGenSphereRays(unsigned int numRays, unsigned long long seed, float radius, RTCRayHit* rays) {
+  const float twopi = 2.0f * (float)M_PI;
+  for (unsigned i = 0; i < numRays; i++) {
+    RTCRayHit& r = rays[i];
+    init_rayhit(r);
+    const float z = 2.0f * u01(seed, 40, 2ull * i) - 1.0f, phi = twopi * u01(seed, 40, 2ull * i + 1);
+    const float rr = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+    const float o[3] = {radius * rr * cosf(phi), radius * rr * sinf(phi), radius * z};
+    // target: direction uniform on the sphere, radius cbrt(u)
+    const float z2 = 2.0f * u01(seed, 41, 3ull * i) - 1.0f, phi2 = twopi * u01(seed, 41, 3ull * i + 1);
+    const float r2 = cbrtf(u01(seed, 41, 3ull * i + 2)), rr2 = sqrtf(fmaxf(0.0f, 1.0f - z2 * z2));
+    float d[3] = {r2 * rr2 * cosf(phi2) - o[0], r2 * rr2 * sinf(phi2) - o[1], r2 * z2 - o[2]};
+    normalize3(d);
+    r.ray.org_x = o[0]; r.ray.org_y = o[1]; r.ray.org_z = o[2];
+    r.ray.dir_x = d[0]; r.ray.dir_y = d[1]; r.ray.dir_z = d[2];
+    r.ray.id = i;
+  }
+}

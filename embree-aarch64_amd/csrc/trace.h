@@ -18,6 +18,16 @@ struct TraceParams {
   int blocks_per_cu = 8;    // resident 256-thread blocks per compute unit for the persistent grid
 };
 
+// One ray stream of a batched traversal (device memory).
+struct BatchItem {
+  char* rays;        // records, `stride` bytes apart
+  uint64_t stride;
+  uint32_t first;    // slot of its first ray in the concatenation of all streams
+  uint32_t count;
+  uint32_t root;     // root node of its scene
+  uint32_t pad;
+};
+
 // Per-thread buffers of the ray reordering.
 struct RaySorter {
   DevBuf keys, vals, counter;
@@ -33,5 +43,12 @@ struct RaySorter {
 void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t inst_id,
                   uint32_t* cnt_nodes, uint32_t* cnt_tris, uint32_t* err, hipStream_t s,
                   const TraceParams& tp, RaySorter* sorter, uint32_t* node_visits = nullptr);
+
+// One persistent launch over several ray streams of scenes that share sc's
+// node and triangle arrays (a batched build). items (device memory) sorted
+// by first; M = total rays. Records traced in place. counter: one device
+// word, cleared by a kernel (the sequence can be captured in a hipGraph).
+void launch_trace_batch(const DevScene& sc, const BatchItem* items, uint32_t nitems, uint32_t M, bool occluded,
+                        bool aligned, uint32_t* err, uint32_t* counter, hipStream_t s, const TraceParams& tp);
 
 }  // namespace rtg

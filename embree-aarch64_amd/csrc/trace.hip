@@ -311,7 +311,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
   __shared__ uint2 lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
   RingStack<LDS_SLOTS> stack;
   stack.init(lds_stack + threadIdx.x);
-  uint32_t cur = 0;  // root node
+  uint32_t cur = sc.root;
   bool occluded = false;
   // every node and triangle record is reached at most once per ray, so the
   // loop runs at most num_nodes + num_tris times; the cap turns a corrupt
@@ -441,12 +441,16 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
 #define RTG_TRACE_MIN_BLOCKS 1
 #endif
 
-template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool PL>
+// BATCH: the ray slots 0..M-1 are the concatenation of the streams in
+// `items` (sorted by `first`); a lane finds its stream by binary search and
+// starts at that stream's scene root.
+template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool PL, bool BATCH = false>
 __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                                 uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                                 uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
                                                 const uint32_t* __restrict__ perm, uint32_t* __restrict__ next,
-                                                int refill_min, uint32_t* node_visits) {
+                                                int refill_min, uint32_t* node_visits,
+                                                const BatchItem* __restrict__ items, uint32_t nitems) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   __shared__ uint2 lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
@@ -457,6 +461,8 @@ __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene s
   Hit best;
   float lim = 0.0f;
   uint32_t cur = 0, ray = 0, nvis = 0, ntri = 0;
+  char* lrp = nullptr;      // BATCH: this lane's record
+  uint32_t lroot = 0;       // BATCH: its scene's root
   bool active = false, occluded = false, pool_empty = false;
   // a ray touches each node and triangle record at most once
   const uint64_t max_iter = (uint64_t)sc.num_nodes + sc.num_tris + 16;
@@ -475,10 +481,25 @@ __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene s
         if (!active) {
           const uint32_t slot = base + (uint32_t)__popcll(idle & lt);
           if (slot < M) {
-            ray = perm ? perm[slot] : slot;
+            char* src;
+            if (BATCH) {
+              uint32_t lo = 0, hi = nitems - 1;
+              while (lo < hi) {
+                const uint32_t mid = (lo + hi + 1) >> 1;
+                if (items[mid].first <= slot) lo = mid; else hi = mid - 1;
+              }
+              const BatchItem& it = items[lo];
+              ray = slot - it.first;
+              lrp = it.rays + (uint64_t)ray * it.stride;
+              lroot = it.root;
+              src = lrp;
+            } else {
+              ray = perm ? perm[slot] : slot;
+              src = rays + (uint64_t)ray * stride;
+            }
             float4 a, b;
             float tfar;
-            load_ray<ALIGNED>(rays + (uint64_t)ray * stride, a, b, tfar);
+            load_ray<ALIGNED>(src, a, b, tfar);
             if (a.w <= tfar) {
               setup_ray(r, a, b, tfar);
               best.found = false;
@@ -486,7 +507,7 @@ __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene s
               best.prim = best.geom = 0xffffffffu;
               best.u = best.v = best.ngx = best.ngy = best.ngz = 0.0f;
               lim = widen_up(tfar);
-              cur = 0;
+              cur = BATCH ? lroot : sc.root;
               stack.init(lds_stack + threadIdx.x);
               occluded = false;
               nvis = ntri = 0;
@@ -600,7 +621,7 @@ __global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene s
       done = !got;
     }
     if (done) {
-      char* rp = rays + (uint64_t)ray * stride;
+      char* rp = BATCH ? lrp : rays + (uint64_t)ray * stride;
       if (COUNT) { cnt_nodes[ray] = nvis; cnt_tris[ray] = ntri; }
       if (OCCLUDED) {
         if (occluded) *reinterpret_cast<float*>(rp + 32) = -INFINITY;
@@ -665,7 +686,8 @@ template <bool OCC, bool AL, bool CNT, int SL, bool PL>
 void launch_one(const LaunchArgs& a) {
   if (a.persistent)
     hipLaunchKernelGGL((k_trace_p<OCC, AL, CNT, SL, PL>), a.pgrid, dim3(TB), 0, a.s, a.sc, a.rays, a.M, a.stride,
-                       a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm, a.counter, a.refill, a.node_visits);
+                       a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm, a.counter, a.refill, a.node_visits,
+                       nullptr, 0u);
   else
     hipLaunchKernelGGL((k_trace<OCC, AL, CNT, SL, PL>), a.grid, dim3(TB), 0, a.s, a.sc, a.rays, a.M, a.stride,
                        a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm, a.node_visits);
@@ -724,6 +746,43 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
                 persistent, grid, pgrid, tp.lds_slots, s};
   if (sc.robust) launch_pl<true>(la, count, occluded, aligned);
   else launch_pl<false>(la, count, occluded, aligned);
+  RTG_HIP(hipGetLastError());
+}
+
+template <bool OCC, bool AL, bool PL>
+void launch_batch_one(const DevScene& sc, const BatchItem* items, uint32_t nitems, uint32_t M, uint32_t* err,
+                      uint32_t* counter, int refill, dim3 pgrid, hipStream_t s) {
+  hipLaunchKernelGGL((k_trace_p<OCC, AL, false, 8, PL, true>), pgrid, dim3(TB), 0, s, sc, (char*)nullptr, M,
+                     (uint64_t)0, 0xffffffffu, (uint32_t*)nullptr, (uint32_t*)nullptr, err,
+                     (const uint32_t*)nullptr, counter, refill, (uint32_t*)nullptr, items, nitems);
+}
+
+void launch_trace_batch(const DevScene& sc, const BatchItem* items, uint32_t nitems, uint32_t M, bool occluded,
+                        bool aligned, uint32_t* err, uint32_t* counter, hipStream_t s, const TraceParams& tp) {
+  if (M == 0 || nitems == 0 || sc.num_nodes == 0) return;
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, counter);
+  const int rmin = occluded ? tp.refill_min_occ : tp.refill_min;
+  const int refill = rmin < 1 ? 1 : (rmin > 64 ? 64 : rmin);
+  const uint32_t want = (uint32_t)tp.cus * (uint32_t)tp.blocks_per_cu;
+  const uint32_t blocks = div_up(M, TB);
+  const dim3 pgrid(want < blocks ? want : blocks);
+  if (sc.robust) {
+    if (occluded) {
+      if (aligned) launch_batch_one<true, true, true>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+      else launch_batch_one<true, false, true>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+    } else {
+      if (aligned) launch_batch_one<false, true, true>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+      else launch_batch_one<false, false, true>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+    }
+  } else {
+    if (occluded) {
+      if (aligned) launch_batch_one<true, true, false>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+      else launch_batch_one<true, false, false>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+    } else {
+      if (aligned) launch_batch_one<false, true, false>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+      else launch_batch_one<false, false, false>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+    }
+  }
   RTG_HIP(hipGetLastError());
 }
 

@@ -144,6 +144,13 @@ SIGNATURES = {
     "rtcgGenHeightfield": (_u32, [_u32, ctypes.c_ulonglong, _f32, _u32, _vp, _vp]),
     "rtcgGenCameraRays": (None, [_u32, _u32, _vp, _vp, _vp, _f32, _vp]),
     "rtcgGenBounceRays": (_u32, [_vp, _u32, ctypes.c_ulonglong, _f32, _vp]),
+    "rtcgGenDisplacedSphere": (_u32, [_u32, ctypes.c_ulonglong, _f32, _vp, _vp, _vp]),
+    "rtcgGenSphereRays": (None, [_u32, ctypes.c_ulonglong, _f32, _vp]),
+    "rtcgCommitScenes": (None, [_vp, _u32]),
+    "rtcgTraceBatch": (None, [_vp, _u32, _i32]),
+    "rtcgNewBatchGraph": (_vp, [_vp, _u32, _i32]),
+    "rtcgLaunchBatchGraph": (None, [_vp, _vp]),
+    "rtcgReleaseBatchGraph": (None, [_vp]),
     "rtcgNewSplitModel": (_vp, [_vp, _u32]),
     "rtcgRetainSplitModel": (None, [_vp]),
     "rtcgReleaseSplitModel": (None, [_vp]),
@@ -269,6 +276,32 @@ def gen_bounce(hits, seed=3, epsilon=1e-4):
     hits = np.ascontiguousarray(hits)
     k = lib().rtcgGenBounceRays(hits.ctypes.data, len(hits), seed, epsilon, out.ctypes.data)
     return out[:k].copy()
+
+
+def gen_displaced_sphere(num_phi, seed=1, amplitude=0.1):
+    nv = ctypes.c_uint(0)
+    nt = lib().rtcgGenDisplacedSphere(num_phi, seed, amplitude, None, None, ctypes.byref(nv))
+    v = np.zeros((nv.value, 3), np.float32)
+    ix = np.zeros((nt, 3), np.uint32)
+    lib().rtcgGenDisplacedSphere(num_phi, seed, amplitude, v.ctypes.data, ix.ctypes.data, None)
+    return v, ix
+
+
+def gen_sphere_rays(n, seed=2, radius=2.0):
+    r = np.zeros(n, RTCRayHit)
+    lib().rtcgGenSphereRays(n, seed, radius, r.ctypes.data)
+    return r
+
+
+def batch_workload(num_scenes=256, seed=7, tri_range=(1e3, 2e6), ray_range=(4e3, 1e6)):
+    """Per-scene (numPhi, triangles, rays) of the batched-scenes workload:
+    triangle and ray counts log-uniform in the given ranges; numPhi is the
+    UV-sphere ring count whose 4*numPhi*(numPhi-1) triangles come closest."""
+    rng = np.random.default_rng(seed)
+    tris = np.exp(rng.uniform(np.log(tri_range[0]), np.log(tri_range[1]), num_scenes))
+    rays = np.round(np.exp(rng.uniform(np.log(ray_range[0]), np.log(ray_range[1]), num_scenes))).astype(np.int64)
+    phis = np.maximum(2, np.round(0.5 + np.sqrt(tris / 4.0))).astype(np.int64)
+    return [(int(p), int(4 * p * (p - 1)), int(r)) for p, r in zip(phis, rays)]
 
 
 class Device:
@@ -440,3 +473,21 @@ class SplitModel:
         loss = lib().rtcgSplitModelTrain(self.h, steps, batch, lr, seed)
         self.device.check("train")
         return float(loss)
+
+
+RTCGBatchItem = np.dtype([("scene", "<u8"), ("rays", "<u8"), ("numRays", "<u4"), ("byteStride", "<u4")])
+
+
+def commit_scenes(scenes):
+    """One batched build of all scenes (rtcgCommitScenes)."""
+    arr = (ctypes.c_void_p * len(scenes))(*[s.h for s in scenes])
+    lib().rtcgCommitScenes(arr, len(scenes))
+    scenes[0].device.check("rtcgCommitScenes")
+
+
+def batch_items(streams):
+    """streams: (scene, device pointer, number of rays, stride) tuples."""
+    it = np.zeros(len(streams), RTCGBatchItem)
+    for i, (sc, ptr, n, stride) in enumerate(streams):
+        it[i] = (sc.h or 0, ptr or 0, n, stride)
+    return it

@@ -109,6 +109,50 @@ RTC_API void rtcgGenCameraRays(unsigned int width, unsigned int height, const fl
 RTC_API unsigned int rtcgGenBounceRays(const struct RTCRayHit* hits, unsigned int M, unsigned long long seed,
                                        float epsilon, struct RTCRayHit* out);
 
+/* Unit UV sphere (numPhi rings) displaced radially by amplitude times a
+ * 3-octave value noise; sizes as rtcgGenSphere. */
+RTC_API unsigned int rtcgGenDisplacedSphere(unsigned int numPhi, unsigned long long seed, float amplitude,
+                                            float* vertices, unsigned int* indices, unsigned int* numVertices);
+
+/* Rays from uniform points on a sphere of the given radius towards uniform
+ * points inside the unit ball. */
+RTC_API void rtcgGenSphereRays(unsigned int numRays, unsigned long long seed, float radius, struct RTCRayHit* rays);
+
+/* ---- batched scenes ------------------------------------------------- */
+/* No counterpart in the reference (each RTCScene commits on its own,
+ * kernels/common/scene.cpp:655-750); for callers with hundreds of scenes.
+ * Builds every scene in one pass (one segmented sort and one binned-SAH
+ * forest over all their triangles, sharing one node and triangle array).
+ * Each scene then answers rtcIntersect1M/rtcOccluded1M as if committed on
+ * its own; a later rtcCommitScene gives it arrays of its own again. All
+ * scenes must belong to one device and agree on RTC_SCENE_FLAG_ROBUST. */
+RTC_API void rtcgCommitScenes(RTCScene* scenes, unsigned int numScenes);
+
+/* One ray stream of a batched traversal: device records (RTCRayHit for
+ * closest hit, RTCRay for occlusion) of a scene of one rtcgCommitScenes. */
+struct RTCGBatchItem
+{
+  RTCScene scene;
+  void* rays;
+  unsigned int numRays;
+  unsigned int byteStride;
+};
+
+/* Traces every stream in one persistent launch (lanes pull rays from one
+ * atomic counter over the concatenated streams); records updated in place
+ * as rtcIntersect1M/rtcOccluded1M would. Synchronous. */
+RTC_API void rtcgTraceBatch(const struct RTCGBatchItem* items, unsigned int numItems, int occluded);
+
+/* The same traversal captured once in a hipGraph (counter clear + launch;
+ * the stream table stays on the device), for callers that replay it. The
+ * graph keeps the scenes alive; the ray buffers must stay valid. A launch on
+ * a NULL stream is synchronous and reports traversal errors; on a caller's
+ * hipStream_t it is asynchronous. Replays of one graph must not overlap. */
+typedef struct RTCGBatchGraphTy* RTCGBatchGraph;
+RTC_API RTCGBatchGraph rtcgNewBatchGraph(const struct RTCGBatchItem* items, unsigned int numItems, int occluded);
+RTC_API void rtcgLaunchBatchGraph(RTCGBatchGraph graph, void* hipStream);
+RTC_API void rtcgReleaseBatchGraph(RTCGBatchGraph graph);
+
 /* ---- learned split cost --------------------------------------------- */
 /* No counterpart in the reference: its binned-SAH split score
  * (kernels/builders/heuristic_binning.h:340-393, BinInfoT::best) is the plain

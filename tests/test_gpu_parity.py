@@ -495,3 +495,95 @@ def test_robust_watertight_c1_sphere(rtc, device, oracle):
     assert (out["geomID"] == 0).all()
     assert_same(out, oracle_trace(oracle, [(v, ix)], r, robust=True), "robust watertight")
     sc.release()
+
+
+# ------------------------------------------------------------ batched scenes --
+
+def _batch_case(rtc):
+    """Displaced spheres of uneven sizes (one empty scene, one of one
+    triangle, one above the 4096-reference chunk size) with rays from a
+    radius-2 sphere towards the unit ball."""
+    phis = [2, 3, 5, 8, 13, 21, 40, 70, 4, 6]
+    scenes = []
+    for i, p in enumerate(phis):
+        scenes.append(rtc.gen_displaced_sphere(p, seed=50 + i, amplitude=0.1))
+    v0, ix0 = scenes[0]
+    scenes.insert(3, (v0, ix0[:0]))       # empty
+    scenes.insert(5, (v0, ix0[:1]))       # one triangle
+    rays = [rtc.gen_sphere_rays(500 + 700 * (i % 4), seed=80 + i) for i in range(len(scenes))]
+    return scenes, rays
+
+
+def test_batched_commit_trace_and_graph(rtc, device, oracle):
+    meshes, rays = _batch_case(rtc)
+    scs = [rtc.Scene(device, [m] if len(m[1]) else []) for m in meshes]
+    rtc.commit_scenes(scs)
+    refs = [oracle_trace(oracle, [m], r, brute=len(m[1]) < 200) if len(m[1]) else r.copy()
+            for m, r in zip(meshes, rays)]
+    # each scene on its own through the standard entry point
+    for i, (sc, r) in enumerate(zip(scs, rays)):
+        assert_same(trace(rtc, sc, r), refs[i], f"batched scene {i}")
+    # one persistent launch over all streams, and the same captured in a graph
+    ptrs = [device.malloc(r.nbytes) for r in rays]
+    try:
+        def upload():
+            for p_, r in zip(ptrs, rays):
+                device.memcpy(p_, r.ctypes.data, r.nbytes)
+
+        def download():
+            outs = []
+            for p_, r in zip(ptrs, rays):
+                o = np.zeros_like(r)
+                device.memcpy(o.ctypes.data, p_, r.nbytes)
+                outs.append(o)
+            return outs
+
+        items = rtc.batch_items([(sc, p_, len(r), 80) for sc, p_, r in zip(scs, ptrs, rays)])
+        upload()
+        rtc.lib().rtcgTraceBatch(items.ctypes.data, len(items), 0)
+        device.check("rtcgTraceBatch")
+        for i, o in enumerate(download()):
+            assert_same(o, refs[i], f"batched launch stream {i}")
+        g = rtc.lib().rtcgNewBatchGraph(items.ctypes.data, len(items), 0)
+        device.check("rtcgNewBatchGraph")
+        try:
+            for rep in range(3):
+                upload()
+                rtc.lib().rtcgLaunchBatchGraph(g, None)
+                device.check("rtcgLaunchBatchGraph")
+                for i, o in enumerate(download()):
+                    assert_same(o, refs[i], f"graph replay {rep} stream {i}")
+        finally:
+            rtc.lib().rtcgReleaseBatchGraph(g)
+        # occlusion streams (RTCRay records)
+        occ = []
+        for r in rays:
+            o = np.zeros(len(r), rtc.RTCRay)
+            for k in rtc.RTCRay.names:
+                o[k] = r[k]
+            o["tfar"] = 2.0
+            occ.append(o)
+        optrs = [device.malloc(o.nbytes) for o in occ]
+        try:
+            for p_, o in zip(optrs, occ):
+                device.memcpy(p_, o.ctypes.data, o.nbytes)
+            oitems = rtc.batch_items([(sc, p_, len(o), 48) for sc, p_, o in zip(scs, optrs, occ)])
+            rtc.lib().rtcgTraceBatch(oitems.ctypes.data, len(oitems), 1)
+            device.check("occluded batch")
+            for i, (p_, o, m) in enumerate(zip(optrs, occ, meshes)):
+                got = np.zeros_like(o)
+                device.memcpy(got.ctypes.data, p_, o.nbytes)
+                ref = oracle_trace(oracle, [m], o, occluded=True, brute=True) if len(m[1]) else o.copy()
+                assert_same(got, ref, f"occluded stream {i}")
+        finally:
+            for p_ in optrs:
+                device.free(p_)
+    finally:
+        for p_ in ptrs:
+            device.free(p_)
+    # a scene committed again on its own leaves the batch
+    scs[7].commit()
+    assert_same(trace(rtc, scs[7], rays[7]), refs[7], "recommitted scene")
+    assert_same(trace(rtc, scs[8], rays[8]), refs[8], "batch after a member left")
+    for sc in scs:
+        sc.release()
