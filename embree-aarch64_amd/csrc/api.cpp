@@ -248,6 +248,7 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "cost_node") d->build.cost_node = (float)atof(v.c_str());
     else if (k == "cost_tri") d->build.cost_tri = (float)atof(v.c_str());
     else if (k == "builder") d->build.builder = (v == "sah") ? rtg::BUILDER_SAH : rtg::BUILDER_MORTON;
+    else if (k == "bvh_width") d->build.width = (uint32_t)atoi(v.c_str());
     else if (k == "trace_stack") d->trace.lds_slots = atoi(v.c_str());
     else if (k == "ray_sort") d->trace.sort_rays = atoi(v.c_str());
     else if (k == "ray_sort_min") d->trace.sort_min = (uint32_t)atol(v.c_str());
@@ -319,6 +320,22 @@ void bind_slot(Geometry* g, RTCBufferType type, unsigned slot, RTCFormat format,
   g->committed = false;
 }
 
+// Build settings of a scene, including its split model (lc receives the
+// model's device view and must outlive the build).
+rtg::BuildParams scene_params(Scene* sc, rtg::LearnedCost& lc) {
+  rtg::BuildParams p = build_params(sc->dev, sc->quality, sc->flags);
+  if (sc->model) {
+    std::lock_guard<std::mutex> ml(sc->model->mtx);
+    if (!sc->model->m.has_norm) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "split model has no training data");
+    lc.P = sc->model->m.packed.as<rtg::MlpPacked>();
+    lc.sn = sc->model->m.norm;
+    lc.min_count = sc->model->min_count;
+    p.learned = &lc;
+    p.builder = rtg::BUILDER_SAH;
+  }
+  return p;
+}
+
 // Uploads the enabled triangle geometries of sc (scene index `scene` of a
 // build) into device buffers owned by sc, appending their descriptors;
 // prim_offset advances over their triangles.
@@ -372,17 +389,8 @@ void commit_scene(Scene* sc) {
   upload_geometry(sc, d, t, 0, prim_offset, descs);
   {
     std::lock_guard<std::mutex> bl(d->build_mtx);
-    rtg::BuildParams p = build_params(d, sc->quality, sc->flags);
     rtg::LearnedCost lc{};
-    if (sc->model) {
-      std::lock_guard<std::mutex> ml(sc->model->mtx);
-      if (!sc->model->m.has_norm) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "split model has no training data");
-      lc.P = sc->model->m.packed.as<rtg::MlpPacked>();
-      lc.sn = sc->model->m.norm;
-      lc.min_count = sc->model->min_count;
-      p.learned = &lc;
-      p.builder = rtg::BUILDER_SAH;
-    }
+    const rtg::BuildParams p = scene_params(sc, lc);
     d->monitor((ssize_t)prim_offset * (80 + 48));
     if (sc->bvh.use_count() > 1 || sc->slot != 0) {  // leave a batch's shared arrays to the other scenes
       sc->bvh = std::make_shared<rtg::DeviceBVH>();
@@ -1129,7 +1137,8 @@ RTC_API void rtcgTimeBuild(RTCScene scene, int steps, float* ms_out) {
     auto t0 = std::chrono::steady_clock::now();
     if (s->bvh.use_count() > 1 || s->slot != 0)
       RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene shares its BVH with a batch: commit it on its own first");
-    const rtg::BuildParams p = build_params(d, s->quality, s->flags);
+    rtg::LearnedCost lc{};
+    const rtg::BuildParams p = scene_params(s, lc);
     rtg::build_bvh(s->descs, p, d->scratch, *s->bvh, t.stream);
     RTG_HIP(hipStreamSynchronize(t.stream));
     if (ms_out) ms_out[i] = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -1644,6 +1653,117 @@ RTC_API void rtcgReleaseBatchGraph(RTCGBatchGraph graph) {
   VERIFY_HANDLE(graph);
   ref_dec(g);
   API_END((BatchGraphH*)nullptr)
+}
+
+// -------------------------------------------------------------- autotune --
+
+}  // extern "C"
+
+namespace {
+
+// Mean kernel time (ms) of `reps` closest-hit or occlusion launches over a
+// scratch copy of the rays (restored before each launch, not timed).
+float time_config(Scene* s, ThreadDev& t, const rtg::TraceParams& tp, const void* src, void* work, unsigned M,
+                  size_t stride, bool occluded, int reps) {
+  float total = 0.0f;
+  for (int i = -1; i < reps; i++) {  // one untimed warm-up
+    RTG_HIP(hipMemcpyAsync(work, src, (size_t)M * stride, hipMemcpyDeviceToDevice, t.stream));
+    RTG_HIP(hipEventRecord(t.ev0, t.stream));
+    rtg::launch_trace(s->bvh->view(s->slot), work, M, stride, occluded, RTC_INVALID_GEOMETRY_ID, nullptr, nullptr,
+                      t.errword.as<uint32_t>(), t.stream, tp, &t.sorter);
+    RTG_HIP(hipEventRecord(t.ev1, t.stream));
+    RTG_HIP(hipEventSynchronize(t.ev1));
+    float ms = 0.0f;
+    RTG_HIP(hipEventElapsedTime(&ms, t.ev0, t.ev1));
+    if (i >= 0) total += ms;
+  }
+  check_trace_error(t);
+  return total / (float)reps;
+}
+
+void rebuild(Scene* s, Device* d, ThreadDev& t) {
+  std::lock_guard<std::mutex> bl(d->build_mtx);
+  rtg::LearnedCost lc{};
+  const rtg::BuildParams p = scene_params(s, lc);
+  rtg::build_bvh(s->descs, p, d->scratch, *s->bvh, t.stream);
+  RTG_HIP(hipStreamSynchronize(t.stream));
+}
+
+}  // namespace
+
+extern "C" {
+
+RTC_API float rtcgAutotune(RTCScene scene, const void* d_rays, unsigned int M, size_t byteStride, int occluded,
+                           int rebuildTree, char* out, size_t outSize) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  if (!d_rays || M == 0) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  if (byteStride < (occluded ? sizeof(RTCRay) : sizeof(RTCRayHit)) || byteStride % 4)
+    RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid stride");
+  if (rebuildTree && (s->bvh.use_count() > 1 || s->slot != 0))
+    RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene shares its BVH with a batch: commit it on its own first");
+  Device* d = s->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  std::lock_guard<std::mutex> lk(s->mtx);
+  if (s->bvh->empty(s->slot)) return 0.0f;
+  rtg::DevBuf work;
+  work.reserve((size_t)M * byteStride);
+  const int reps = 2;
+  rtg::TraceParams tp = d->trace;
+  float best = time_config(s, t, tp, d_rays, work.ptr, M, byteStride, occluded != 0, reps);
+  if (rebuildTree) {
+    // tree shape: node width, leaf size, triangle cost of the collapse
+    const rtg::BuildParams start = d->build;
+    rtg::BuildParams bestb = start;
+    for (uint32_t w : {4u, 8u})
+      for (uint32_t leaf : {4u, 6u, 8u, 12u})
+        for (float ct : {0.3f, 0.45f, 0.6f}) {
+          d->build.width = w;
+          d->build.max_leaf = leaf;
+          d->build.cost_tri = ct;
+          rebuild(s, d, t);
+          const float ms = time_config(s, t, tp, d_rays, work.ptr, M, byteStride, occluded != 0, reps);
+          if (ms < best) {
+            best = ms;
+            bestb = d->build;
+          }
+        }
+    d->build = bestb;
+    rebuild(s, d, t);
+    best = time_config(s, t, tp, d_rays, work.ptr, M, byteStride, occluded != 0, reps);
+  }
+  // traversal: LDS stack entries, refill threshold, resident blocks per CU
+  rtg::TraceParams bestt = tp;
+  for (int slots : {0, 8, 16})
+    for (int refill : {8, 16, 24, 32})
+      for (int bpc : {4, 6, 8}) {
+        rtg::TraceParams c = tp;
+        c.lds_slots = slots;
+        if (occluded) c.refill_min_occ = refill; else c.refill_min = refill;
+        c.blocks_per_cu = bpc;
+        c.persistent = 1;
+        const float ms = time_config(s, t, c, d_rays, work.ptr, M, byteStride, occluded != 0, reps);
+        if (ms < best) {
+          best = ms;
+          bestt = c;
+        }
+      }
+  d->trace = bestt;
+  if (out && outSize) {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "bvh_width=%u,max_leaf=%u,cost_tri=%g,trace_stack=%d,%s=%d,blocks_per_cu=%d",
+             d->build.width, d->build.max_leaf, (double)d->build.cost_tri, d->trace.lds_slots,
+             occluded ? "refill_min_occ" : "refill_min", occluded ? d->trace.refill_min_occ : d->trace.refill_min,
+             d->trace.blocks_per_cu);
+    strncpy(out, buf, outSize - 1);
+    out[outSize - 1] = 0;
+  }
+  return (float)M / (best * 1e-3f) / 1e6f;
+  API_END(s)
+  return 0.0f;
 }
 
 }  // extern "C"

@@ -18,6 +18,7 @@ enum BuilderKind { BUILDER_MORTON = 0, BUILDER_SAH = 1 };
 struct BuildParams {
   int builder = BUILDER_MORTON;  // binary tree: Morton radix tree, or top-down binned SAH
   uint32_t max_leaf = 8;       // max triangles per leaf (<= 15)
+  uint32_t width = 8;          // children per node (2..8; the node record holds 8 slots)
   float cost_node = 1.0f;      // SAH cost of visiting one 8-wide node, per unit half area
   float cost_tri = 0.45f;      // SAH cost of one triangle test, per unit half area
   int quality = 1;             // RTCBuildQuality (LOW: Morton tree, HIGH: binned SAH, MEDIUM: `builder`)

@@ -344,7 +344,8 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
                                               const uint2* __restrict__ range, const float4* __restrict__ slo,
                                               const float4* __restrict__ shi, float4* nlo, float4* nhi,
                                               float* cost8, uint8_t* split8, uint8_t* isleaf, uint32_t* visit,
-                                              float cost_node, float cost_tri, uint32_t max_leaf, uint32_t* err) {
+                                              float cost_node, float cost_tri, uint32_t max_leaf, uint32_t width,
+                                              uint32_t* err) {
   const uint32_t j = blockIdx.x * BT + threadIdx.x;
   if (j >= n) return;
   uint32_t node = parent_leaf[j];
@@ -399,7 +400,7 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
       dj[s] = bj;
     }
     const float leafc = count <= max_leaf ? cost_tri * (float)count * A : INFINITY;
-    const float inner = cost_node * A + d[8];
+    const float inner = cost_node * A + d[width];  // a wide node holds at most `width` children
     float c[8];
     uint32_t sp[8];
     c[0] = fminf(leafc, inner);
@@ -480,6 +481,7 @@ struct CollapseArgs {
   BuildCounters* cnt;
   uint32_t n;         // capacity of nodes and of each queue
   uint32_t tri_cap;   // capacity of tris
+  uint32_t width;     // children per node (2..8)
   uint32_t robust;
 };
 
@@ -556,8 +558,8 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
     uint32_t st_ref[8], st_s[8];
     int sp = 0;
     {
-      const uint32_t j8 = (a.split8[8 * (size_t)item.bref + 7]) & 7u;
-      st_ref[0] = a.child[2 * item.bref + 1]; st_s[0] = 8 - j8;
+      const uint32_t j8 = (a.split8[8 * (size_t)item.bref + a.width - 1]) & 7u;
+      st_ref[0] = a.child[2 * item.bref + 1]; st_s[0] = a.width - j8;
       st_ref[1] = a.child[2 * item.bref];     st_s[1] = j8;
       sp = 2;
     }
@@ -742,6 +744,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     return;
   }
   const uint32_t max_leaf = prm.max_leaf < 1 ? 1 : (prm.max_leaf > 15 ? 15 : prm.max_leaf);
+  const uint32_t width = prm.width < 2 ? 2 : (prm.width > 8 ? 8 : prm.width);
 
   S.geoms.reserve(geoms.size() * sizeof(GeomDesc));
   RTG_HIP(hipMemcpyAsync(S.geoms.ptr, geoms.data(), geoms.size() * sizeof(GeomDesc), hipMemcpyHostToDevice, s));
@@ -867,7 +870,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
                        S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(),
                        S.slo.as<float4>(), S.shi.as<float4>(), S.nlo.as<float4>(), S.nhi.as<float4>(),
                        S.cost.as<float>(), S.split8.as<uint8_t>(), S.isleaf.as<uint8_t>(), S.visit.as<uint32_t>(),
-                       prm.cost_node, prm.cost_tri, max_leaf, &cnt->error);
+                       prm.cost_node, prm.cost_tri, max_leaf, width, &cnt->error);
     hipLaunchKernelGGL(k_check_tree, dim3(div_up(n, BT)), dim3(BT), 0, s,
                        use_sah ? nullptr : S.keys.as<uint32_t>(), n, &cnt->roots_seen,
                        S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
@@ -915,6 +918,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   ca.cnt = cnt;
   ca.n = cap;
   ca.tri_cap = n;
+  ca.width = width;
   ca.robust = prm.robust ? 1u : 0u;
   uint32_t nin = nscenes;
   int levels = 0;

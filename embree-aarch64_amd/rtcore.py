@@ -146,6 +146,7 @@ SIGNATURES = {
     "rtcgGenBounceRays": (_u32, [_vp, _u32, ctypes.c_ulonglong, _f32, _vp]),
     "rtcgGenDisplacedSphere": (_u32, [_u32, ctypes.c_ulonglong, _f32, _vp, _vp, _vp]),
     "rtcgGenSphereRays": (None, [_u32, ctypes.c_ulonglong, _f32, _vp]),
+    "rtcgAutotune": (_f32, [_vp, _vp, _u32, _sz, _i32, _i32, ctypes.c_char_p, _sz]),
     "rtcgCommitScenes": (None, [_vp, _u32]),
     "rtcgTraceBatch": (None, [_vp, _u32, _i32]),
     "rtcgNewBatchGraph": (_vp, [_vp, _u32, _i32]),

@@ -118,6 +118,20 @@ RTC_API unsigned int rtcgGenDisplacedSphere(unsigned int numPhi, unsigned long l
  * points inside the unit ball. */
 RTC_API void rtcgGenSphereRays(unsigned int numRays, unsigned long long seed, float radius, struct RTCRayHit* rays);
 
+/* ---- autotuner ------------------------------------------------------- */
+/* No counterpart in the reference (its BVH and traversal settings are fixed
+ * per ISA). Times traversal configurations on M device-resident sample rays
+ * (closest hit, or occlusion records) of a committed scene and keeps the
+ * fastest as the device's setting: LDS stack entries (0, 8, 16), refill
+ * threshold (8..32 idle lanes) and resident blocks per CU (4, 6, 8); with
+ * rebuildTree != 0 first the tree shape, rebuilding the scene for each node
+ * width (4, 8), leaf size (4..12) and collapse triangle cost, ending with the
+ * best. The sample rays are not modified. Writes the chosen settings as an
+ * rtcNewDevice config string into out (outSize bytes) and returns the best
+ * Mrays/s. */
+RTC_API float rtcgAutotune(RTCScene scene, const void* rays, unsigned int M, size_t byteStride, int occluded,
+                           int rebuildTree, char* out, size_t outSize);
+
 /* ---- batched scenes ------------------------------------------------- */
 /* No counterpart in the reference (each RTCScene commits on its own,
  * kernels/common/scene.cpp:655-750); for callers with hundreds of scenes.

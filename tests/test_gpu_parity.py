@@ -587,3 +587,49 @@ def test_batched_commit_trace_and_graph(rtc, device, oracle):
     assert_same(trace(rtc, scs[8], rays[8]), refs[8], "batch after a member left")
     for sc in scs:
         sc.release()
+
+
+# ------------------------------------------------------------- autotuner --
+
+def test_bvh_width4_structure_and_parity(rtc, oracle):
+    d = rtc.Device("bvh_width=4,builder=sah")
+    v, ix = soup_case(rtc, 20000, 61, h=0.02)
+    sc = rtc.Scene(d, [(v, ix)]).commit()
+    nodes, tris = sc.read_bvh()
+    check_bvh(nodes, tris, [(v, ix)])
+    meta = nodes.reshape(-1, 80)[:, 24:32]
+    assert ((meta != 0).sum(1) <= 4).all()
+    rays = rtc.gen_random_rays(50000, seed=62)
+    assert_same(trace(rtc, sc, rays), oracle_trace(oracle, [(v, ix)], rays), "width 4")
+    sc.release()
+    d.release()
+
+
+def test_autotune_keeps_results(rtc, oracle):
+    d = rtc.Device("")
+    v, ix = soup_case(rtc, 20000, 63, h=0.02)
+    sc = rtc.Scene(d, [(v, ix)]).commit()
+    rays = rtc.gen_random_rays(100000, seed=64)
+    dp = d.malloc(rays.nbytes)
+    try:
+        d.memcpy(dp, rays.ctypes.data, rays.nbytes)
+        buf = ctypes.create_string_buffer(256)
+        mr = rtc.lib().rtcgAutotune(sc.h, dp, len(rays), 80, 0, 1, buf, 256)
+        d.check("rtcgAutotune")
+        cfg = dict(kv.split("=") for kv in buf.value.decode().split(","))
+        assert mr > 0 and set(cfg) >= {"bvh_width", "max_leaf", "cost_tri", "trace_stack", "blocks_per_cu"}
+        back = np.zeros_like(rays)
+        d.memcpy(back.ctypes.data, dp, rays.nbytes)
+        assert np.array_equal(back.view(np.uint8), rays.view(np.uint8))  # sample rays untouched
+        # the tuned device (and a fresh one made from the returned string) give the oracle's records
+        ref = oracle_trace(oracle, [(v, ix)], rays)
+        assert_same(trace(rtc, sc, rays), ref, "after autotune")
+        d2 = rtc.Device(buf.value.decode())
+        sc2 = rtc.Scene(d2, [(v, ix)]).commit()
+        assert_same(trace(rtc, sc2, rays), ref, "tuned config string")
+        sc2.release()
+        d2.release()
+    finally:
+        d.free(dp)
+    sc.release()
+    d.release()
