@@ -138,6 +138,61 @@ def learned_vs_sah(grid, width, height, bounces, train_steps, steps):
             "build_ms_plain": round(build_plain, 2), "build_ms_learned": round(build_learned, 2)}
 
 
+def batched_scenes(num_scenes, steps):
+    """Batched scenes: num_scenes displaced UV spheres (3-octave value noise,
+    amplitude 0.1) with triangle counts log-uniform in [1e3, 2e6] and ray
+    streams log-uniform in [4e3, 1e6] from a radius-2 sphere towards the unit
+    ball. One rtcgCommitScenes builds them all; one persistent launch,
+    captured in a hipGraph, traces every stream."""
+    L = R.lib()
+    work = R.batch_workload(num_scenes, seed=7)
+    dev = R.Device("")
+    t0 = time.perf_counter()
+    scs, rays = [], []
+    for i, (phi, _, nray) in enumerate(work):
+        v, ix = R.gen_displaced_sphere(phi, seed=1000 + i, amplitude=0.1)
+        scs.append(R.Scene(dev, [(v, ix)]))
+        rays.append(R.gen_sphere_rays(nray, seed=2000 + i, radius=2.0))
+    gen_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    R.commit_scenes(scs)
+    commit_s = time.perf_counter() - t1
+    allr = np.concatenate(rays)
+    nbytes, n = allr.nbytes, len(allr)
+    d_src, d_rays = dev.malloc(nbytes), dev.malloc(nbytes)
+    dev.memcpy(d_src, allr.ctypes.data, nbytes)
+    offs = np.concatenate([[0], np.cumsum([len(r) for r in rays])])
+    items = R.batch_items([(sc, d_rays + 80 * int(offs[i]), len(r), 80) for i, (sc, r) in enumerate(zip(scs, rays))])
+    g = L.rtcgNewBatchGraph(items.ctypes.data, len(items), 0)
+    dev.check("rtcgNewBatchGraph")
+    times = []
+    for k in range(steps + 2):
+        dev.memcpy(d_rays, d_src, nbytes)
+        L.rtcgSynchronize(dev.h)
+        ts = time.perf_counter()
+        L.rtcgLaunchBatchGraph(g, None)
+        te = time.perf_counter()
+        dev.check("rtcgLaunchBatchGraph")
+        if k >= 2:
+            times.append(te - ts)
+    out = np.zeros(n, R.RTCRayHit)
+    dev.memcpy(out.ctypes.data, d_rays, nbytes)
+    hit_rate = float((out["geomID"] != R.RTC_INVALID_GEOMETRY_ID).mean())
+    L.rtcgReleaseBatchGraph(g)
+    tri_total = sum(t for _, t, _ in work)
+    for p_ in (d_src, d_rays):
+        dev.free(p_)
+    for sc in scs:
+        sc.release()
+    dev.release()
+    ms = 1e3 * float(np.mean(times))
+    return {"workload": f"{num_scenes} displaced spheres, {tri_total} triangles, {n} rays in {num_scenes} streams",
+            "commit_ms_incl_upload": round(1e3 * commit_s, 2),
+            "commit_mtris_per_s_incl_upload": round(tri_total / commit_s / 1e6, 2),
+            "graph_ms": round(ms, 3), "mrays_per_s": round(n / (ms / 1e3) / 1e6, 2), "hit_rate": round(hit_rate, 4),
+            "generation_s": round(gen_s, 1)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +209,8 @@ def main():
     ap.add_argument("--hf-height", type=int, default=1080)
     ap.add_argument("--hf-depth", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=400)
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-scenes", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: run with --gpus 1 (multi-GPU sharding is a design note only)"}))
@@ -245,6 +302,10 @@ def main():
         learned = learned_vs_sah(args.hf_grid, args.hf_width, args.hf_height, args.hf_depth, args.train_steps,
                                  max(3, args.steps // 2))
 
+    batch = None
+    if not args.no_batch:
+        batch = batched_scenes(args.batch_scenes, max(3, args.steps // 4))
+
     cpu = None
     if not args.no_cpu:
         log("timing the CPU baseline")
@@ -276,6 +337,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "learned_vs_sah": learned,
+        "batched_scenes": batch,
     }
     print(json.dumps(out), flush=True)
 

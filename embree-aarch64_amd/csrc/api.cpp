@@ -1766,4 +1766,189 @@ RTC_API float rtcgAutotune(RTCScene scene, const void* d_rays, unsigned int M, s
   return 0.0f;
 }
 
+// ------------------------------------------------- packet and SoA forms --
+
+}  // extern "C"
+
+namespace {
+
+// A run of rays in SoA form: field f of lane i at r[f] + 4 i (ray fields in
+// RTCRay order) and h[f] + 4 i (hit fields in RTCHit order; unused for
+// occlusion). valid (may be null) masks lanes out with 0.
+struct SoaRun {
+  char* r[12];
+  char* h[8];
+  unsigned n;
+  const int* valid;
+};
+
+SoaRun packet_run(void* base, unsigned K, const int* valid, bool with_hit) {
+  SoaRun run;
+  char* b = (char*)base;
+  for (int f = 0; f < 12; f++) run.r[f] = b + (size_t)f * K * 4;
+  for (int f = 0; f < 8; f++) run.h[f] = with_hit ? b + (size_t)(12 + f) * K * 4 : nullptr;
+  run.n = K;
+  run.valid = valid;
+  return run;
+}
+
+// Gathers the active lanes into AoS records, traces them as one stream and
+// writes back what rtcIntersect1M / rtcOccluded1M would have written.
+void trace_runs(Scene* sc, const RTCIntersectContext* ctx, const std::vector<SoaRun>& runs, bool occluded) {
+  if (!sc->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  std::vector<RTCRayHit> aos;
+  std::vector<std::pair<uint32_t, uint32_t>> where;
+  for (uint32_t ri = 0; ri < runs.size(); ri++) {
+    const SoaRun& run = runs[ri];
+    for (uint32_t i = 0; i < run.n; i++) {
+      if (run.valid && run.valid[i] == 0) continue;
+      RTCRayHit rec;
+      uint32_t* w = reinterpret_cast<uint32_t*>(&rec.ray);
+      for (int f = 0; f < 12; f++) memcpy(&w[f], run.r[f] + 4 * (size_t)i, 4);
+      if (!(rec.ray.tnear <= rec.ray.tfar)) continue;
+      memset(&rec.hit, 0, sizeof(rec.hit));
+      rec.hit.geomID = rec.hit.primID = rec.hit.instID[0] = RTC_INVALID_GEOMETRY_ID;
+      aos.push_back(rec);
+      where.emplace_back(ri, i);
+    }
+  }
+  if (aos.empty()) return;
+  if (occluded) {
+    std::vector<RTCRay> rays(aos.size());
+    for (size_t k = 0; k < aos.size(); k++) rays[k] = aos[k].ray;
+    trace_stream(sc, ctx, rays.data(), (unsigned)rays.size(), sizeof(RTCRay), true);
+    for (size_t k = 0; k < rays.size(); k++)
+      if (rays[k].tfar == -INFINITY) memcpy(runs[where[k].first].r[8] + 4 * (size_t)where[k].second, &rays[k].tfar, 4);
+  } else {
+    trace_stream(sc, ctx, aos.data(), (unsigned)aos.size(), sizeof(RTCRayHit), false);
+    for (size_t k = 0; k < aos.size(); k++) {
+      if (aos[k].hit.geomID == RTC_INVALID_GEOMETRY_ID) continue;
+      const SoaRun& run = runs[where[k].first];
+      const size_t o = 4 * (size_t)where[k].second;
+      memcpy(run.r[8] + o, &aos[k].ray.tfar, 4);
+      const uint32_t* hw = reinterpret_cast<const uint32_t*>(&aos[k].hit);
+      for (int f = 0; f < 8; f++)
+        if (run.h[f]) memcpy(run.h[f] + o, &hw[f], 4);
+    }
+  }
+}
+
+template <unsigned K>
+void packet_call(const int* valid, RTCScene scene, RTCIntersectContext* ctx, void* rays, bool occluded) {
+  Scene* s = SC(scene);
+  VERIFY_HANDLE(scene);
+  VERIFY_HANDLE(rays);
+  VERIFY_HANDLE(valid);
+  trace_runs(s, ctx, {packet_run(rays, K, valid, !occluded)}, occluded);
+}
+
+void np_call(RTCScene scene, RTCIntersectContext* ctx, const void* np, unsigned N, bool occluded) {
+  Scene* s = SC(scene);
+  VERIFY_HANDLE(scene);
+  VERIFY_HANDLE(np);
+  SoaRun run;
+  char* const* p = reinterpret_cast<char* const*>(np);  // RTCRayNp then RTCHitNp: arrays of field pointers
+  for (int f = 0; f < 12; f++) run.r[f] = p[f];
+  for (int f = 0; f < 8; f++) run.h[f] = occluded ? nullptr : p[12 + f];
+  run.n = N;
+  run.valid = nullptr;
+  trace_runs(s, ctx, {run}, occluded);
+}
+
+void nm_call(RTCScene scene, RTCIntersectContext* ctx, void* base, unsigned N, unsigned M, size_t byteStride,
+             bool occluded) {
+  Scene* s = SC(scene);
+  VERIFY_HANDLE(scene);
+  if (M && !base) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  std::vector<SoaRun> runs;
+  for (unsigned j = 0; j < M; j++) runs.push_back(packet_run((char*)base + (size_t)j * byteStride, N, nullptr, !occluded));
+  trace_runs(s, ctx, runs, occluded);
+}
+
+void mp_call(RTCScene scene, RTCIntersectContext* ctx, void** recs, unsigned M, bool occluded) {
+  Scene* s = SC(scene);
+  VERIFY_HANDLE(scene);
+  if (M && !recs) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  // AoS records reached through pointers: each is a run of one lane with field stride 4
+  std::vector<SoaRun> runs;
+  for (unsigned j = 0; j < M; j++) {
+    SoaRun run;
+    char* b = (char*)recs[j];
+    if (!b) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+    for (int f = 0; f < 12; f++) run.r[f] = b + 4 * f;
+    for (int f = 0; f < 8; f++) run.h[f] = occluded ? nullptr : b + 48 + 4 * f;
+    run.n = 1;
+    run.valid = nullptr;
+    runs.push_back(run);
+  }
+  trace_runs(s, ctx, runs, occluded);
+}
+
+}  // namespace
+
+extern "C" {
+
+RTC_API void rtcIntersect4(const int* valid, RTCScene scene, RTCIntersectContext* context, RTCRayHit4* rayhit) {
+  API_BEGIN
+  packet_call<4>(valid, scene, context, rayhit, false);
+  API_END(SC(scene))
+}
+RTC_API void rtcIntersect8(const int* valid, RTCScene scene, RTCIntersectContext* context, RTCRayHit8* rayhit) {
+  API_BEGIN
+  packet_call<8>(valid, scene, context, rayhit, false);
+  API_END(SC(scene))
+}
+RTC_API void rtcIntersect16(const int* valid, RTCScene scene, RTCIntersectContext* context, RTCRayHit16* rayhit) {
+  API_BEGIN
+  packet_call<16>(valid, scene, context, rayhit, false);
+  API_END(SC(scene))
+}
+RTC_API void rtcOccluded4(const int* valid, RTCScene scene, RTCIntersectContext* context, RTCRay4* ray) {
+  API_BEGIN
+  packet_call<4>(valid, scene, context, ray, true);
+  API_END(SC(scene))
+}
+RTC_API void rtcOccluded8(const int* valid, RTCScene scene, RTCIntersectContext* context, RTCRay8* ray) {
+  API_BEGIN
+  packet_call<8>(valid, scene, context, ray, true);
+  API_END(SC(scene))
+}
+RTC_API void rtcOccluded16(const int* valid, RTCScene scene, RTCIntersectContext* context, RTCRay16* ray) {
+  API_BEGIN
+  packet_call<16>(valid, scene, context, ray, true);
+  API_END(SC(scene))
+}
+RTC_API void rtcIntersect1Mp(RTCScene scene, RTCIntersectContext* context, RTCRayHit** rayhit, unsigned int M) {
+  API_BEGIN
+  mp_call(scene, context, (void**)rayhit, M, false);
+  API_END(SC(scene))
+}
+RTC_API void rtcOccluded1Mp(RTCScene scene, RTCIntersectContext* context, RTCRay** ray, unsigned int M) {
+  API_BEGIN
+  mp_call(scene, context, (void**)ray, M, true);
+  API_END(SC(scene))
+}
+RTC_API void rtcIntersectNM(RTCScene scene, RTCIntersectContext* context, RTCRayHitN* rayhit, unsigned int N,
+                            unsigned int M, size_t byteStride) {
+  API_BEGIN
+  nm_call(scene, context, rayhit, N, M, byteStride, false);
+  API_END(SC(scene))
+}
+RTC_API void rtcOccludedNM(RTCScene scene, RTCIntersectContext* context, RTCRayN* ray, unsigned int N, unsigned int M,
+                           size_t byteStride) {
+  API_BEGIN
+  nm_call(scene, context, ray, N, M, byteStride, true);
+  API_END(SC(scene))
+}
+RTC_API void rtcIntersectNp(RTCScene scene, RTCIntersectContext* context, const RTCRayHitNp* rayhit, unsigned int N) {
+  API_BEGIN
+  np_call(scene, context, rayhit, N, false);
+  API_END(SC(scene))
+}
+RTC_API void rtcOccludedNp(RTCScene scene, RTCIntersectContext* context, const RTCRayNp* ray, unsigned int N) {
+  API_BEGIN
+  np_call(scene, context, ray, N, true);
+  API_END(SC(scene))
+}
+
 }  // extern "C"

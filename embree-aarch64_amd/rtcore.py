@@ -147,6 +147,18 @@ SIGNATURES = {
     "rtcgGenDisplacedSphere": (_u32, [_u32, ctypes.c_ulonglong, _f32, _vp, _vp, _vp]),
     "rtcgGenSphereRays": (None, [_u32, ctypes.c_ulonglong, _f32, _vp]),
     "rtcgAutotune": (_f32, [_vp, _vp, _u32, _sz, _i32, _i32, ctypes.c_char_p, _sz]),
+    "rtcIntersect4": (None, [_vp, _vp, _vp, _vp]),
+    "rtcIntersect8": (None, [_vp, _vp, _vp, _vp]),
+    "rtcIntersect16": (None, [_vp, _vp, _vp, _vp]),
+    "rtcOccluded4": (None, [_vp, _vp, _vp, _vp]),
+    "rtcOccluded8": (None, [_vp, _vp, _vp, _vp]),
+    "rtcOccluded16": (None, [_vp, _vp, _vp, _vp]),
+    "rtcIntersect1Mp": (None, [_vp, _vp, _vp, _u32]),
+    "rtcOccluded1Mp": (None, [_vp, _vp, _vp, _u32]),
+    "rtcIntersectNM": (None, [_vp, _vp, _vp, _u32, _u32, _sz]),
+    "rtcOccludedNM": (None, [_vp, _vp, _vp, _u32, _u32, _sz]),
+    "rtcIntersectNp": (None, [_vp, _vp, _vp, _u32]),
+    "rtcOccludedNp": (None, [_vp, _vp, _vp, _u32]),
     "rtcgCommitScenes": (None, [_vp, _u32]),
     "rtcgTraceBatch": (None, [_vp, _u32, _i32]),
     "rtcgNewBatchGraph": (_vp, [_vp, _u32, _i32]),

@@ -57,6 +57,9 @@ int main(void) {
          RTC_DEVICE_PROPERTY_TRIANGLE_GEOMETRY_SUPPORTED, RTC_SCENE_FLAG_ROBUST, RTC_ERROR_CANCELLED);
   struct RTCIntersectContext ctx; rtcInitIntersectContext(&ctx);
   printf("%d %u\n", (int)ctx.flags, ctx.instID[0]);
+  printf("%zu %zu %zu %zu %zu %zu\n", sizeof(struct RTCRayHit4), sizeof(struct RTCRayHit8), sizeof(struct RTCRayHit16),
+         offsetof(struct RTCRayHit4, hit), offsetof(struct RTCRayHit8, hit.u), offsetof(struct RTCRayHit16, hit.instID));
+  printf("%zu %zu\n", sizeof(struct RTCRayHitNp), offsetof(struct RTCRayHitNp, hit.geomID));
   return 0;
 }
 """
@@ -74,6 +77,9 @@ def test_struct_layout_and_enums(tmp_path, compiler):
     assert lines[2].split() == ["8", "16"]
     assert lines[3].split() == [str(0x5003), str(0x9003), "121", "96", "4", "6"]
     assert lines[4].split() == ["0", str(0xFFFFFFFF)]
+    # packets: 12 ray + 8 hit fields of K lanes each; pointer SoA: 20 pointers
+    assert lines[5].split() == ["320", "640", "1280", "192", str(12 * 32 + 3 * 32), str(12 * 64 + 7 * 64)]
+    assert lines[6].split() == ["160", str(8 * 18)]
 
 
 def test_c_program_links_against_library(tmp_path, rtc):

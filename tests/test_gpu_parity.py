@@ -633,3 +633,91 @@ def test_autotune_keeps_results(rtc, oracle):
         d.free(dp)
     sc.release()
     d.release()
+
+
+# ------------------------------------------------ packet and SoA entry points --
+
+def _soa(rays, K, fields):
+    """AoS records -> SoA packets of K lanes (float32/uint32 bit patterns)."""
+    P = len(rays) // K
+    out = np.zeros((P, len(fields), K), np.uint32)
+    for f, name in enumerate(fields):
+        out[:, f, :] = rays[name][:P * K].view(np.uint32).reshape(P, K)
+    return out
+
+
+def _aos_from_soa(soa, K, template, fields):
+    r = template.copy()
+    P = soa.shape[0]
+    for f, name in enumerate(fields):
+        r[name][:P * K] = soa[:, f, :].reshape(-1).view(r.dtype[name])
+    return r
+
+
+def test_packet_and_soa_entry_points(rtc, device):
+    v, ix = soup_case(rtc, 5000, 71, h=0.03)
+    sc = rtc.Scene(device, [(v, ix)]).commit()
+    L = rtc.lib()
+    ctx = rtc.init_context()
+    base = rtc.gen_random_rays(1600, seed=72)
+    base["tnear"][7::29] = 5.0  # inactive by tnear > tfar after a hit? no: tnear > inf is false; make tfar small
+    base["tfar"][7::29] = 1.0
+    ref = trace(rtc, sc, base)
+    fields = list(rtc.RTCRayHit.names)
+    rng = np.random.default_rng(73)
+    for K, fn in ((4, L.rtcIntersect4), (8, L.rtcIntersect8), (16, L.rtcIntersect16)):
+        soa = _soa(base, K, fields)
+        valid = np.where(rng.random(soa.shape[0] * K) < 0.8, -1, 0).astype(np.int32).reshape(-1, K)
+        for p in range(soa.shape[0]):
+            pk = np.ascontiguousarray(soa[p])
+            fn(valid[p].ctypes.data, sc.h, ctypes.byref(ctx), pk.ctypes.data)
+            soa[p] = pk
+        device.check(f"rtcIntersect{K}")
+        got = _aos_from_soa(soa, K, base, fields)
+        on = valid.reshape(-1) != 0
+        n = soa.shape[0] * K
+        assert records_equal(got[:n][on], ref[:n][on]), K
+        assert records_equal(got[:n][~on], base[:n][~on]), K
+    # occlusion packets
+    occ = base.copy()
+    occ["tfar"] = np.minimum(occ["tfar"], 0.2)
+    oref = np.zeros(len(occ), rtc.RTCRay)
+    for k in rtc.RTCRay.names:
+        oref[k] = occ[k]
+    oref = trace(rtc, sc, oref, occluded=True)
+    soa = _soa(occ, 8, list(rtc.RTCRay.names))
+    valid = np.full(8, -1, np.int32)
+    for p in range(soa.shape[0]):
+        pk = np.ascontiguousarray(soa[p])
+        L.rtcOccluded8(valid.ctypes.data, sc.h, ctypes.byref(ctx), pk.ctypes.data)
+        soa[p] = pk
+    got = _aos_from_soa(soa, 8, occ, list(rtc.RTCRay.names))
+    assert np.array_equal(got["tfar"][:len(oref)].view(np.uint32), oref["tfar"].view(np.uint32))
+    # NM: runtime N = 5, packets 100 bytes apart beyond their 20*5*4 = 400 B
+    N, M = 5, 300
+    soa = _soa(base, N, fields)[:M]
+    stride = 20 * N * 4 + 100
+    buf = np.zeros(M * stride, np.uint8)
+    for p in range(M):
+        buf[p * stride:p * stride + 400] = soa[p].reshape(-1).view(np.uint8)
+    L.rtcIntersectNM(sc.h, ctypes.byref(ctx), buf.ctypes.data, N, M, stride)
+    device.check("rtcIntersectNM")
+    for p in range(M):
+        soa[p] = buf[p * stride:p * stride + 400].view(np.uint32).reshape(20, N)
+    assert records_equal(_aos_from_soa(soa, N, base, fields)[:N * M], ref[:N * M])
+    # Np: one array per field
+    arrs = [np.ascontiguousarray(base[name]) for name in fields]
+    ptrs = (ctypes.c_void_p * 20)(*[a.ctypes.data for a in arrs])
+    L.rtcIntersectNp(sc.h, ctypes.byref(ctx), ptrs, len(base))
+    device.check("rtcIntersectNp")
+    got = base.copy()
+    for name, a in zip(fields, arrs):
+        got[name] = a
+    assert records_equal(got, ref)
+    # 1Mp: pointers to records
+    recs = [base[i:i + 1].copy() for i in range(0, 400)]
+    pp = (ctypes.c_void_p * len(recs))(*[r.ctypes.data for r in recs])
+    L.rtcIntersect1Mp(sc.h, ctypes.byref(ctx), pp, len(recs))
+    device.check("rtcIntersect1Mp")
+    assert records_equal(np.concatenate(recs), ref[:400])
+    sc.release()
