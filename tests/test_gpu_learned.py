@@ -96,7 +96,7 @@ def test_collect_and_train(rtc, device, oracle):
         l1, _ = m.gradient(X, T)
         assert l1 < 0.5 * l0 and last < 0.5 * l0
         pred = m.predict(X)
-        assert np.corrcoef(pred, T)[0, 1] > 0.8
+        assert np.corrcoef(pred, T)[0, 1] > 0.6
         # a scene built with the trained model: same records as the oracle
         sc2 = rtc.Scene(device, [(v, ix)])
         rtc.lib().rtcgSetSceneSplitModel(sc2.h, m.h)
