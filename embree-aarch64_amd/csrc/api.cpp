@@ -247,7 +247,7 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "bvh_leaf" || k == "max_leaf") d->build.max_leaf = (uint32_t)atoi(v.c_str());
     else if (k == "cost_node") d->build.cost_node = (float)atof(v.c_str());
     else if (k == "cost_tri") d->build.cost_tri = (float)atof(v.c_str());
-    else if (k == "builder") d->build.builder = (v == "sah") ? rtg::BUILDER_SAH : rtg::BUILDER_MORTON;
+    else if (k == "builder") d->build.builder = (v == "morton" || v == "lbvh") ? rtg::BUILDER_MORTON : rtg::BUILDER_SAH;
     else if (k == "bvh_width") d->build.width = (uint32_t)atoi(v.c_str());
     else if (k == "trace_stack") d->trace.lds_slots = atoi(v.c_str());
     else if (k == "ray_sort") d->trace.sort_rays = atoi(v.c_str());

@@ -16,7 +16,7 @@ namespace rtg {
 enum BuilderKind { BUILDER_MORTON = 0, BUILDER_SAH = 1 };
 
 struct BuildParams {
-  int builder = BUILDER_MORTON;  // binary tree: Morton radix tree, or top-down binned SAH
+  int builder = BUILDER_SAH;   // binary tree: top-down binned SAH (default, as the reference's MEDIUM), or Morton radix tree
   uint32_t max_leaf = 8;       // max triangles per leaf (<= 15)
   uint32_t width = 8;          // children per node (2..8; the node record holds 8 slots)
   float cost_node = 1.0f;      // SAH cost of visiting one 8-wide node, per unit half area

@@ -327,6 +327,7 @@ __global__ __launch_bounds__(BT) void k_radix_tree(const uint32_t* __restrict__ 
 
 __device__ inline float box_half_area(float4 l, float4 h) { return half_area(h.x - l.x, h.y - l.y, h.z - l.z); }
 
+
 // Bottom-up bounds and the SAH-optimal 8-wide collapse table. The second
 // child to arrive at a node (agent-scope acquire/release counter) finishes it
 // and moves up. For every binary node n and slot budget s = 1..8, cost8 holds
@@ -338,6 +339,7 @@ __device__ inline float box_half_area(float4 l, float4 h) { return half_area(h.x
 // (bit 7). A binary leaf costs cost_tri*A(prim) whatever the budget.
 // (Collapse formulation after Ylitie, Karras, Laine: "Efficient Incoherent
 // Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017.)
+// A node's width is capped at `width` children (d(n, width) in the inner cost).
 __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __restrict__ child,
                                               const uint32_t* __restrict__ parent_int,
                                               const uint32_t* __restrict__ parent_leaf,
@@ -382,9 +384,9 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
     }
     const float4 l = make_float4(fminf(bl[0].x, bl[1].x), fminf(bl[0].y, bl[1].y), fminf(bl[0].z, bl[1].z), 0.0f);
     const float4 h = make_float4(fmaxf(bh[0].x, bh[1].x), fmaxf(bh[0].y, bh[1].y), fmaxf(bh[0].z, bh[1].z), 0.0f);
-    const float A = box_half_area(l, h);
     const uint2 r = range[node];
     const uint32_t count = r.y - r.x + 1;
+    const float A = box_half_area(l, h);
     float d[9];
     uint32_t dj[9];
 #pragma unroll

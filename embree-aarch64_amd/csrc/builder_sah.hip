@@ -249,10 +249,13 @@ __device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m
 // axis sweeps its bins. Learned cost (lc.P set and the segment larger than
 // lc.min_count): the workgroup evaluates the model on the left and right
 // boxes of all 3*(nb-1) candidates (their boxes and counts from the bins),
-// then one thread per axis keeps the least P(L)*n_L + P(R)*n_R, P = 2^model.
+// then one thread per axis keeps the least P(L)*n_L + P(R)*n_R with
+// P = halfArea * 2^model, the model's value kept within 3 of the segment's
+// own and P within the segment's own P.
 // Ties keep the lowest axis and the lowest k; without a separating
 // candidate the split is the object median.
 struct ChooseLds {
+  float yp, pp;  // learned log rate and visit weight of the segment's own box
   float cost[3];
   int k[3];
   uint32_t nl[3];
@@ -274,6 +277,20 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
   const bool learned = lc.P != nullptr && sg.count > lc.min_count && depth < MEDIAN_DEPTH;
   const int ncand = m.nb - 1;
   if (learned) {
+    // the segment's own prediction bounds its children's: a child is never
+    // more likely to be visited than its parent, and its learned rate per
+    // unit area stays within 8x of the parent's
+    if (tid == 0) {
+      float lo[3], hi[3];
+      uint32_t cnt;
+      range_box(w, 0, 0, m.nb, lo, hi, cnt);
+      float f[MLP_IN];
+      box_features(lo, hi, sg.count, lc.sn, f);
+      L.yp = mlp_eval(lc.P, f);
+      L.pp = half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) * exp2f(L.yp);
+    }
+    __syncthreads();
+    const float yp = L.yp, pp = L.pp;
     for (int e = tid; e < 2 * 3 * ncand; e += BLOCK) {
       const int c = e >> 1, side = e & 1;
       const int a = c / ncand, k = c % ncand + 1;
@@ -284,7 +301,8 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
       if (cnt > 0 && m.scale[a] != 0.0f) {
         float f[MLP_IN];
         box_features(lo, hi, cnt, lc.sn, f);
-        pv = exp2f(mlp_eval(lc.P, f));
+        const float y = fminf(fmaxf(mlp_eval(lc.P, f), yp - 3.0f), yp + 3.0f);
+        pv = fminf(half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) * exp2f(y), pp);
       }
       L.p[e] = pv;
     }

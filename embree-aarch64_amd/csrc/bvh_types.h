@@ -98,7 +98,7 @@ struct DevScene {
 
 // Entries of the per-ray traversal stack; the builder rejects a tree whose
 // deepest path could need more.
-constexpr uint32_t TRACE_STACK = 256;
+constexpr uint32_t TRACE_STACK = 512;
 
 // Conservative-culling constants shared with the oracle's rules.
 constexpr float CULL_MARGIN = 1.0f / 4096.0f;       // relative margin on ray interval ends

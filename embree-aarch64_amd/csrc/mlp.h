@@ -1,7 +1,9 @@
 // Learned split cost: a small MLP that predicts, from a box and the number
-// of primitives under it, log2 of the probability that a ray of the target
-// distribution visits a node with that box. The binned-SAH builder uses
-// 2^prediction in place of the half-area term of its split cost.
+// of primitives under it, log2 of the ratio between the probability that a
+// ray of the target distribution visits a node with that box and the box's
+// half area relative to the scene's (what uniformly distributed rays would
+// give). The binned-SAH builder uses halfArea * 2^prediction in place of the
+// half area in its split cost.
 //
 // Network: 8 inputs -> 32 ReLU -> 32 ReLU -> 1, FP16 weights and
 // activations with FP32 accumulation (v_dot2_f32_f16), FP32 biases. Training
@@ -114,8 +116,9 @@ void mlp_predict(SplitModel& m, const float* d_x, uint32_t n, float* d_y, hipStr
 // Loss sum_i w_i (y_i - t_i)^2 / n and its gradient (into m.grad, FP32).
 float mlp_gradient(SplitModel& m, const float* d_x, const float* d_t, const float* d_w, const uint32_t* d_idx,
                    uint32_t n, hipStream_t s);
-// Appends one training sample per node: features of info[i] and target
-// log2((visits[i] + 0.5) / rays).
+// Appends one training sample per node: features f of info[i] and target
+// log2((visits[i] + 0.5) / rays) - 16 f[6] (f[6] = log2 of the relative half
+// area / 16).
 void mlp_append_samples(SplitModel& m, const float* d_info, const uint32_t* d_visits, uint32_t nnodes, uint32_t rays,
                         const SceneNorm& sn, hipStream_t s);
 // Adam on random minibatches of the training set; returns the last loss.

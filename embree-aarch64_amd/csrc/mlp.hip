@@ -33,8 +33,9 @@ __global__ void k_mlp_pack(const float* __restrict__ p, MlpPacked* P) {
 }
 
 // Deterministic initialisation: uniform(-a, a) with a = sqrt(6 / fan_in)
-// (He-uniform for ReLU layers) from a counter hash; biases zero except b3,
-// which starts at -10 (a visit probability of about 2^-10).
+// (He-uniform for ReLU layers) from a counter hash; biases zero, so an
+// untrained model predicts a visit probability near the relative half area
+// (the plain SAH).
 __device__ inline float hash_uniform(uint32_t seed, uint32_t i) {
   uint64_t z = ((uint64_t)seed << 32) + i + 0x9E3779B97F4A7C15ull;
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -50,7 +51,6 @@ __global__ void k_mlp_init(float* p, float* m, float* v, float* g, uint32_t seed
   if (i < MLP_OFF_B1) x = hash_uniform(seed, i) * sqrtf(6.0f / MLP_IN);
   else if (i >= MLP_OFF_W2 && i < MLP_OFF_B2) x = hash_uniform(seed, i) * sqrtf(6.0f / MLP_H);
   else if (i >= MLP_OFF_W3 && i < MLP_OFF_B3) x = hash_uniform(seed, i) * sqrtf(3.0f / MLP_H);
-  else if (i == MLP_OFF_B3) x = -10.0f;
   p[i] = x;
   m[i] = v[i] = g[i] = 0.0f;
 }
@@ -189,7 +189,9 @@ __global__ void k_mlp_samples(const float* __restrict__ info, const uint32_t* __
   box_features(lo, hi, __float_as_uint(r[3]), sn, f);
 #pragma unroll
   for (int k = 0; k < MLP_IN; k++) X[(size_t)i * MLP_IN + k] = f[k];
-  T[i] = log2f(((float)visits[i] + 0.5f) * inv_rays);
+  // target: log2 of the visit probability over the box's relative half area
+  // (the model learns the ray distribution's departure from uniform rays)
+  T[i] = log2f(((float)visits[i] + 0.5f) * inv_rays) - 16.0f * f[6];
   W[i] = 1.0f;
 }
 

@@ -245,7 +245,7 @@ def test_sah_builder_sizes(rtc, sah_device, oracle, n):
     v, ix = soup_case(rtc, n, 100 + n, h=0.03 if n > 1000 else 0.2)
     sc = rtc.Scene(sah_device, [(v, ix)]).commit()
     st = sc.stats()
-    assert st["num_prims"] == n and st["max_stack"] <= 256
+    assert st["num_prims"] == n and st["max_stack"] <= 512
     nodes, tris = sc.read_bvh()
     check_bvh(nodes, tris, [(v, ix)])
     rays = rtc.gen_random_rays(20000, seed=n, lo=-0.05, hi=1.05)
