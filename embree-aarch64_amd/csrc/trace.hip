@@ -439,7 +439,8 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
 // compiled for (one wave per SIMD each): the register budget follows from it.
 // 7 (at most 72 VGPRs) measured fastest on the 2^20-triangle soup with 2^24
 // incoherent rays: 1131 Mrays/s closest hit against 1095 at the compiler's
-// own choice (80 VGPRs, 6 waves per SIMD) and 870 at 8 (spills).
+// own choice (80 VGPRs, 6 waves per SIMD) and 870 at 8 (spills). A 16-entry
+// LDS stack (32 KB per block) leaves room for 5 blocks.
 #ifndef RTG_TRACE_MIN_BLOCKS
 #define RTG_TRACE_MIN_BLOCKS 7
 #endif
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
 // `items` (sorted by `first`); a lane finds its stream by binary search and
 // starts at that stream's scene root.
 template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool PL, bool BATCH = false>
-__global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
+__global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                                 uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                                 uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
                                                 const uint32_t* __restrict__ perm, uint32_t* __restrict__ next,
