@@ -53,6 +53,27 @@ def cpu_baseline(v, ix, rays, sample):
             "build_mtris_per_s": len(ix) / build_s / 1e6, "build_threads": 1}
 
 
+PROFILE = os.path.join(ROOT, "profiles", "trace_build_c2_soup1M_16Mrays.txt")
+TRACE_KERNEL = "k_trace_p<false, true, false, 8, false, false>"
+
+
+def profiled_traffic():
+    """HBM-side bytes per launch of the closest-hit kernel (FETCH_SIZE and
+    WRITE_SIZE scaled by a plain copy of known size) and its L2 hit rate,
+    from the committed rocprofv3 summary of this workload."""
+    try:
+        for line in open(PROFILE):
+            if TRACE_KERNEL in line and "GB read" in line:
+                tail = line.split("->")[1]
+                rd = float(tail.split("GB read")[0])
+                wr = float(tail.split("GB read")[1].split("GB written")[0])
+                hit = float(tail.split("L2 hit")[1])
+                return (rd + wr) * 1e9, hit
+    except (OSError, ValueError, IndexError):
+        pass
+    return None, None
+
+
 def time_trace(L, dev, sc, d_rays, d_src, n, steps):
     ms = np.zeros(max(1, steps), np.float32)
     L.rtcgTimeTrace(sc.h, d_rays, d_src, n, 80, 0, len(ms), ms.ctypes.data)
@@ -292,10 +313,15 @@ def main():
     nbar, tbar = float(cn.mean()), float(ct.mean())
     bytes_per_ray = 48 + 36 + nbar * node_b + tbar * tri_b
     achieved = bytes_per_ray * n / (ms_step / 1e3) / 1e9
+    traffic, l2hit = profiled_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": round(traffic) if traffic and n == NUM_RAYS and args.tris == NUM_TRIS else None,
+                "traffic_source": os.path.relpath(PROFILE, ROOT) if traffic else None,
+                "l2_hit_rate": l2hit,
                 "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nbar, 2),
-                "tris_per_ray": round(tbar, 2)}
+                "tris_per_ray": round(tbar, 2),
+                "node_bytes_per_ray": round(nbar * node_b, 1)}
 
     for p in (d_src, d_rays, d_cn, d_ct, d_osrc):
         dev.free(p)
