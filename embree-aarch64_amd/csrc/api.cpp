@@ -249,6 +249,7 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "cost_tri") d->build.cost_tri = (float)atof(v.c_str());
     else if (k == "builder") d->build.builder = (v == "morton" || v == "lbvh") ? rtg::BUILDER_MORTON : rtg::BUILDER_SAH;
     else if (k == "bvh_width") d->build.width = (uint32_t)atoi(v.c_str());
+    else if (k == "sah_median_below") d->build.median_below = (uint32_t)atoi(v.c_str());
     else if (k == "trace_stack") d->trace.lds_slots = atoi(v.c_str());
     else if (k == "ray_sort") d->trace.sort_rays = atoi(v.c_str());
     else if (k == "ray_sort_min") d->trace.sort_min = (uint32_t)atol(v.c_str());

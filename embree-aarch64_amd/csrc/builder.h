@@ -19,6 +19,7 @@ struct BuildParams {
   int builder = BUILDER_SAH;   // binary tree: top-down binned SAH (default, as the reference's MEDIUM), or Morton radix tree
   uint32_t max_leaf = 8;       // max triangles per leaf (<= 15)
   uint32_t width = 8;          // children per node (2..8; the node record holds 8 slots)
+  uint32_t median_below = 0;   // binned-SAH segments of at most this many references (<= 64) split at the median
   float cost_node = 1.0f;      // SAH cost of visiting one 8-wide node, per unit half area
   float cost_tri = 0.45f;      // SAH cost of one triangle test, per unit half area
   int quality = 1;             // RTCBuildQuality (LOW: Morton tree, HIGH: binned SAH, MEDIUM: `builder`)

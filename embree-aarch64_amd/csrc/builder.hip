@@ -856,8 +856,8 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     if (use_sah) {
       SahTreeOut so;
       build_sah_tree(n, S.vals.as<uint32_t>(), S.slo.as<float4>(), S.shi.as<float4>(), trees, S.child.as<uint32_t>(),
-                     S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(), prm.learned, S.sah,
-                     so, s);
+                     S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(), prm.learned,
+                     prm.median_below, S.sah, so, s);
       for (size_t k = 0; k < trees.size(); k++) root_ref[tree_scene[k]] = so.roots[k];
       // boxes in the final primitive order
       hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(n, BT)), dim3(BT), 0, s, S.vals.as<uint32_t>(),

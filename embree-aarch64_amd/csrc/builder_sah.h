@@ -38,9 +38,11 @@ struct SahTreeOut {
 // id is its split position, so ids lie in [0, n-2]; ids no tree uses keep
 // parent_int = REF_EMPTY).
 // `learned` (may be null) replaces the half-area term of the split cost.
+// Segments of at most median_below (<= 64) references are split at their
+// median position all the way down instead of by binned SAH.
 void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi,
                     const std::vector<SahRoot>& trees, uint32_t* child, uint32_t* parent_int,
-                    uint32_t* parent_leaf, uint2* range, const LearnedCost* learned, SahScratch& S,
-                    SahTreeOut& out, hipStream_t s);
+                    uint32_t* parent_leaf, uint2* range, const LearnedCost* learned, uint32_t median_below,
+                    SahScratch& S, SahTreeOut& out, hipStream_t s);
 
 }  // namespace rtg

@@ -68,12 +68,14 @@ struct Counters {
   uint32_t nseg[3];  // next-level segment counts: big, mid, small
   uint32_t pad;
   uint32_t max_depth;
-  uint32_t error;    // bit 0: a segment list overflowed, bit 1: bin counts disagree, bit 2: destination out of range
+  uint32_t error;    // bit 0: a segment list overflowed, bit 1: bin counts disagree, bit 2: destination out of range,
+                     // bit 3: median subtree stack overflow
 };
 
 struct Lists {
   Seg* seg[3];
   uint32_t cap[3];
+  uint32_t fin_max;  // segments up to this size are finished at once by median splits
 };
 
 struct Tree {
@@ -214,14 +216,54 @@ __device__ inline void link_inner(const Tree& t, const Seg& sg, uint32_t m) {
   atomicMax(&t.cnt->max_depth, sg.meta >> 1);
 }
 
+// The whole subtree of a small segment [first, first+count) (child `side`
+// of inner node m at depth+1), split at the median position all the way
+// down. Positions keep the order the binned splits left them in (stable
+// partitions of the Morton-sorted references), so these are spatial-median
+// splits in Morton order; nothing moves, only links are written.
+__device__ inline void finish_median(const Tree& t, uint32_t m, uint32_t side, uint32_t depth, uint32_t first,
+                                     uint32_t count) {
+  uint32_t sf[16], sc[16], sp_[16], sm[16];
+  int sp = 0;
+  sf[0] = first; sc[0] = count; sp_[0] = m; sm[0] = ((depth + 1) << 1) | side;
+  sp = 1;
+  while (sp > 0) {
+    sp--;
+    const uint32_t f = sf[sp], c = sc[sp], p = sp_[sp], meta = sm[sp];
+    const uint32_t nl = c / 2, mid = f + nl - 1, d = meta >> 1;
+    t.child[2 * p + (meta & 1u)] = mid;
+    t.parent_int[mid] = p;
+    t.range[mid] = make_uint2(f, f + c - 1);
+    atomicMax(&t.cnt->max_depth, d);
+    const uint32_t cf[2] = {f, f + nl}, cc[2] = {nl, c - nl};
+#pragma unroll
+    for (int s = 0; s < 2; s++) {
+      if (cc[s] == 1) {
+        t.child[2 * mid + s] = cf[s] | REF_LEAF;
+        t.parent_leaf[cf[s]] = mid;
+      } else if (sp < 16) {
+        sf[sp] = cf[s]; sc[sp] = cc[s]; sp_[sp] = mid; sm[sp] = ((d + 1) << 1) | (uint32_t)s;
+        sp++;
+      } else {
+        atomicOr(&t.cnt->error, 8u);
+      }
+    }
+  }
+}
+
 // Child segment [first, first+count) of inner node m: a leaf link for a
-// single primitive, else a new segment in the list of its size class.
-// Returns the id in the big list, or NIL.
+// single primitive, a finished median subtree up to nx.fin_max primitives,
+// else a new segment in the list of its size class. Returns the id in the
+// big list, or NIL.
 __device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m, uint32_t side, uint32_t depth,
                                       uint32_t first, uint32_t count, const float clo[3], const float chi[3]) {
   if (count == 1) {
     t.child[2 * m + side] = first | REF_LEAF;
     t.parent_leaf[first] = m;
+    return NIL;
+  }
+  if (count <= nx.fin_max) {
+    finish_median(t, m, side, depth, first, count);
     return NIL;
   }
   const int cls = count > MID_MAX ? 0 : (count > SMALL_MAX ? 1 : 2);
@@ -647,8 +689,9 @@ __global__ void k_fill_seg_ids(uint32_t* sid, const Seg* segs) {
 
 void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi,
                     const std::vector<SahRoot>& trees, uint32_t* child, uint32_t* parent_int,
-                    uint32_t* parent_leaf, uint2* range, const LearnedCost* learned, SahScratch& S,
-                    SahTreeOut& out, hipStream_t s) {
+                    uint32_t* parent_leaf, uint2* range, const LearnedCost* learned, uint32_t fin_max,
+                    SahScratch& S, SahTreeOut& out, hipStream_t s) {
+  if (fin_max > SMALL_MAX) fin_max = SMALL_MAX;
   LearnedCost lcost{};
   if (learned) lcost = *learned;
   if (trees.empty()) throw hip_error(hipErrorInvalidValue, "SAH build without a tree");
@@ -720,6 +763,7 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
       nx.seg[c] = S.segs[c][o].as<Seg>();
       nx.cap[c] = cap[c];
     }
+    nx.fin_max = fin_max;
     hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, cnt);
     if (cur[0]) {
       const Seg* bs = S.segs[0][in].as<Seg>();

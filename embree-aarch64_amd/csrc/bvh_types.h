@@ -2,8 +2,9 @@
 // traversal kernels (host and device compile this header).
 //
 // Layout in HBM (one scene):
-//   nodes : QNode8[num_nodes], 80 B each, breadth-first; node 0 is the root;
-//           the internal children of a node are contiguous from child_base.
+//   nodes : QNode8[num_nodes], 80 B each, breadth-first; the root is node 0
+//           (node s for scene s of a batched build); the internal children
+//           of a node are contiguous from child_base.
 //   tris  : TriRec[num_tris], 48 B each; the leaf children of a node are
 //           contiguous from tri_base; a leaf ends at the record whose
 //           TRI_LAST flag is set.
@@ -33,6 +34,7 @@ struct alignas(16) QNode8 {
   uint8_t qloz[8], qhiz[8];
 };
 static_assert(sizeof(QNode8) == 80, "QNode8 must be 80 bytes");
+
 
 // Triangle record in Moeller-ready form, e1 = v0 - v1 and e2 = v2 - v0
 // formed once in FP32 as the reference's leaf fill does

@@ -126,8 +126,8 @@ struct NodeHit {
 // exact expression the builder verified against every child box, then the
 // robust per-plane distance (plane - org) * rdir with rdir rounded down for
 // near and up for far planes.
-__device__ inline void intersect_node(const QNode8* __restrict__ node, const TRay& r, float thi, NodeHit& h) {
-  const uint4* np = reinterpret_cast<const uint4*>(node);
+__device__ inline void intersect_node(const DevScene& sc, uint32_t i, const TRay& r, float thi, NodeHit& h) {
+  const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + i);
   const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
   const float px = __uint_as_float(w0.x), py = __uint_as_float(w0.y), pz = __uint_as_float(w0.z);
   const float sx = __uint_as_float((w0.w & 0xffu) << 23);
@@ -329,7 +329,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
         break;
       }
       NodeHit h;
-      intersect_node(sc.nodes + cur, r, lim, h);
+      intersect_node(sc, cur, r, lim, h);
       if (COUNT) {
         nvis++;
         if (node_visits) atomicAdd(&node_visits[cur], 1u);
@@ -542,7 +542,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
         done = true;
       } else {
         NodeHit h;
-        intersect_node(sc.nodes + cur, r, lim, h);
+        intersect_node(sc, cur, r, lim, h);
         if (COUNT) {
           nvis++;
           if (node_visits) atomicAdd(&node_visits[cur], 1u);
