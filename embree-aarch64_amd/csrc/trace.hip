@@ -579,37 +579,40 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
         }
       }
     } else {
-      uint32_t ti = cur & ~REF_LEAF;
-      while (true) {
-        if (ti >= sc.num_tris) {
-          atomicOr(err, 4u);
-          done = true;
-          break;
-        }
+      // one triangle per step: a lane in a long leaf does not hold the
+      // wave's other lanes in a triangle loop
+      const uint32_t ti = cur & ~REF_LEAF;
+      if (ti >= sc.num_tris) {
+        atomicOr(err, 4u);
+        done = true;
+      } else {
         TriTest h;
         const bool valid = tri_test<PL>(r, sc.tris + ti, h);
         if (COUNT) ntri++;
         if (valid) {
           if (OCCLUDED) {
             occluded = true;
-            break;
-          }
-          const bool better = !best.found || h.t < best.t ||
-                              (h.t == best.t && (h.geom < best.geom || (h.geom == best.geom && h.prim < best.prim)));
-          if (better) {
-            best.found = true;
-            best.t = h.t;
-            best.u = h.un / h.dn;
-            best.v = h.vn / h.dn;
-            best.ngx = h.ngx; best.ngy = h.ngy; best.ngz = h.ngz;
-            best.prim = h.prim; best.geom = h.geom;
+          } else {
+            const bool better = !best.found || h.t < best.t ||
+                                (h.t == best.t && (h.geom < best.geom || (h.geom == best.geom && h.prim < best.prim)));
+            if (better) {
+              best.found = true;
+              best.t = h.t;
+              best.u = h.un / h.dn;
+              best.v = h.vn / h.dn;
+              best.ngx = h.ngx; best.ngy = h.ngy; best.ngz = h.ngz;
+              best.prim = h.prim; best.geom = h.geom;
+              lim = widen_up(fminf(best.t, r.tfar));
+            }
           }
         }
-        if (h.flags & TRI_LAST) break;
-        ti++;
+        if (OCCLUDED && occluded) {
+          done = true;
+        } else if (!(h.flags & TRI_LAST)) {
+          cur = (ti + 1) | REF_LEAF;  // the leaf's next record
+          continue;
+        }
       }
-      if (OCCLUDED && occluded) done = true;
-      if (!OCCLUDED && best.found) lim = widen_up(fminf(best.t, r.tfar));
     }
     if (!done) {
       // pop the next entry that can still hold a closer hit
