@@ -21,7 +21,7 @@ struct BuildParams {
   uint32_t width = 8;          // children per node (2..8; the node record holds 8 slots)
   uint32_t median_below = 0;   // binned-SAH segments of at most this many references (<= 64) split at the median
   float cost_node = 1.0f;      // SAH cost of visiting one 8-wide node, per unit half area
-  float cost_tri = 0.45f;      // SAH cost of one triangle test, per unit half area
+  float cost_tri = 0.6f;       // SAH cost of one triangle test, per unit half area
   int quality = 1;             // RTCBuildQuality (LOW: Morton tree, HIGH: binned SAH, MEDIUM: `builder`)
   int robust = 0;              // leaves hold (v0, v1, v2) for the Pluecker test
   const LearnedCost* learned = nullptr;  // binned-SAH split cost from a trained model (device data)
