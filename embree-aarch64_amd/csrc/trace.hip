@@ -31,13 +31,24 @@ constexpr int STACK = (int)TRACE_STACK;  // entries per ray (the builder checks 
 // Per-lane traversal stack: the top LDS_SLOTS entries live in LDS (slot-major,
 // so lanes touching the same slot hit distinct banks), older entries spill
 // to a private array only when the ring is full. Entry = (ref, entry distance).
+// The spill array is a separate local of the kernel: were it a member here,
+// a dynamically indexed store into it could alias sp and lo, and the
+// compiler would keep the whole struct in scratch memory.
+// The LDS pointer is typed in the local address space so that the ring is
+// reached with ds_* instructions (a generic pointer makes every access a flat
+// access through the vector memory pipeline).
+typedef __attribute__((address_space(3))) uint64_t lds_u64;
+__device__ inline uint64_t pack_entry(uint2 e) { return (uint64_t)e.x | ((uint64_t)e.y << 32); }
+__device__ inline uint2 unpack_entry(uint64_t v) { return make_uint2((uint32_t)v, (uint32_t)(v >> 32)); }
+
 template <int LDS_SLOTS>
 struct RingStack {
-  uint2* lds;  // this lane's slot 0; slot k is lds + k * TB
-  uint2 spill[STACK];
-  int sp, lo;  // [0, lo) spilled, [lo, sp) in the ring
-  __device__ inline void init(uint2* base) {
-    lds = base;
+  lds_u64* lds;    // this lane's slot 0; slot k is lds + k * TB
+  uint64_t* spill; // STACK entries, private
+  int sp, lo;      // [0, lo) spilled, [lo, sp) in the ring
+  __device__ inline void init(uint64_t* base, uint64_t* spill_mem) {
+    lds = (lds_u64*)base;
+    spill = spill_mem;
     sp = lo = 0;
   }
   __device__ inline bool push(uint2 e) {
@@ -46,17 +57,19 @@ struct RingStack {
       spill[lo] = lds[(lo & (LDS_SLOTS - 1)) * TB];
       lo++;
     }
-    lds[(sp & (LDS_SLOTS - 1)) * TB] = e;
+    lds[(sp & (LDS_SLOTS - 1)) * TB] = pack_entry(e);
     sp++;
     return true;
   }
   __device__ inline uint2 pop() {
     sp--;
+    // an entry below the ring comes back through its ring slot, so that the
+    // read below is always an LDS read
     if (sp < lo) {
       lo = sp;
-      return spill[sp];
+      lds[(sp & (LDS_SLOTS - 1)) * TB] = spill[sp];
     }
-    return lds[(sp & (LDS_SLOTS - 1)) * TB];
+    return unpack_entry(lds[(sp & (LDS_SLOTS - 1)) * TB]);
   }
   __device__ inline bool empty() const { return sp == 0; }
 };
@@ -64,15 +77,18 @@ struct RingStack {
 // LDS_SLOTS == 0: the whole stack is a private array.
 template <>
 struct RingStack<0> {
-  uint2 spill[STACK];
+  uint64_t* spill;
   int sp;
-  __device__ inline void init(uint2*) { sp = 0; }
+  __device__ inline void init(uint64_t*, uint64_t* spill_mem) {
+    spill = spill_mem;
+    sp = 0;
+  }
   __device__ inline bool push(uint2 e) {
     if (sp >= STACK) return false;
-    spill[sp++] = e;
+    spill[sp++] = pack_entry(e);
     return true;
   }
-  __device__ inline uint2 pop() { return spill[--sp]; }
+  __device__ inline uint2 pop() { return unpack_entry(spill[--sp]); }
   __device__ inline bool empty() const { return sp == 0; }
 };
 
@@ -308,9 +324,10 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
   best.u = best.v = best.ngx = best.ngy = best.ngz = 0.0f;
   float lim = widen_up(tfar);
 
-  __shared__ uint2 lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
+  __shared__ uint64_t lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
+  uint64_t spill_mem[STACK];
   RingStack<LDS_SLOTS> stack;
-  stack.init(lds_stack + threadIdx.x);
+  stack.init(lds_stack + threadIdx.x, spill_mem);
   uint32_t cur = sc.root;
   bool occluded = false;
   // every node and triangle record is reached at most once per ray, so the
@@ -437,12 +454,12 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
 // M and its lanes are idle.
 // Resident 256-lane blocks per compute unit the persistent kernel is
 // compiled for (one wave per SIMD each): the register budget follows from it.
-// 7 (at most 72 VGPRs) measured fastest on the 2^20-triangle soup with 2^24
-// incoherent rays: 1131 Mrays/s closest hit against 1095 at the compiler's
-// own choice (80 VGPRs, 6 waves per SIMD) and 870 at 8 (spills). A 16-entry
-// LDS stack (32 KB per block) leaves room for 5 blocks.
+// 6 (at most 80 VGPRs, no spills) measured fastest on the 2^20-triangle soup
+// with 2^24 incoherent rays: 1655 Mrays/s closest hit against 1585 at 7 (72
+// VGPRs, five of them spilled to scratch in the node step); 5 compiles to the
+// same code as 6. A 16-entry LDS stack (32 KB per block) leaves room for 5.
 #ifndef RTG_TRACE_MIN_BLOCKS
-#define RTG_TRACE_MIN_BLOCKS 7
+#define RTG_TRACE_MIN_BLOCKS 6
 #endif
 
 // BATCH: the ray slots 0..M-1 are the concatenation of the streams in
@@ -457,9 +474,10 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
                                                 const BatchItem* __restrict__ items, uint32_t nitems) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  __shared__ uint2 lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
+  __shared__ uint64_t lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
+  uint64_t spill_mem[STACK];
   RingStack<LDS_SLOTS> stack;
-  stack.init(lds_stack + threadIdx.x);
+  stack.init(lds_stack + threadIdx.x, spill_mem);
 
   TRay r;
   Hit best;
@@ -512,7 +530,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
               best.u = best.v = best.ngx = best.ngy = best.ngz = 0.0f;
               lim = widen_up(tfar);
               cur = BATCH ? lroot : sc.root;
-              stack.init(lds_stack + threadIdx.x);
+              stack.init(lds_stack + threadIdx.x, spill_mem);
               occluded = false;
               nvis = ntri = 0;
               iter = 0;
