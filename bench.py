@@ -214,6 +214,96 @@ def batched_scenes(num_scenes, steps):
             "generation_s": round(gen_s, 1)}
 
 
+def instanced_scene(num_instances, width, height, steps, check_rays=2048):
+    """Two-level scene: five base meshes (two displaced spheres of about 1M and
+    200k triangles, three heightfield patches of about 1M, 200k and 100k)
+    committed as scenes, and num_instances instances of them with a uniform
+    random rotation, a uniform scale in [0.5, 2] and a translation uniform in
+    [-50, 50]^3. Closest hit: width x height pinhole-camera rays from
+    (0, 20, -160) towards the origin; occlusion: one shadow ray from every
+    camera hit towards the point light (0, 100, 0). A subsample of both is
+    checked against the oracle's instance restatement over per-object BVHs."""
+    from oracle import oracle as O
+    L = R.lib()
+    dev = R.Device("")
+    t0 = time.perf_counter()
+    meshes = R.instance_workload(seed=11)
+    objs = [R.Scene(dev, [m]).commit() for m in meshes]
+    T = R.random_transforms(num_instances, seed=5, scale=(0.5, 2.0), extent=50.0)
+    which = np.random.default_rng(6).integers(0, len(objs), num_instances)
+    top = R.Scene(dev)
+    for k in range(num_instances):
+        top.add_instance(objs[which[k]], T[k])
+    setup_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    top.commit()
+    commit_ms = 1e3 * (time.perf_counter() - t1)
+    rays = R.gen_camera(width, height, org=(0.0, 20.0, -160.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0)
+    n, nbytes = len(rays), rays.nbytes
+    d_src, d_rays = dev.malloc(nbytes), dev.malloc(nbytes)
+    dev.memcpy(d_src, rays.ctypes.data, nbytes)
+    warm = np.zeros(2, np.float32)
+    L.rtcgTimeTrace(top.h, d_rays, d_src, n, 80, 0, 2, warm.ctypes.data)
+    ms = np.zeros(steps, np.float32)
+    L.rtcgTimeTrace(top.h, d_rays, d_src, n, 80, 0, steps, ms.ctypes.data)
+    dev.check("rtcgTimeTrace instanced")
+    hits = np.zeros(n, R.RTCRayHit)
+    dev.memcpy(hits.ctypes.data, d_rays, nbytes)
+    hit = hits["geomID"] != R.RTC_INVALID_GEOMETRY_ID
+    # shadow rays from the camera hits
+    hh = hits[hit]
+    p = np.stack([hh["org_x"] + hh["tfar"] * hh["dir_x"], hh["org_y"] + hh["tfar"] * hh["dir_y"],
+                  hh["org_z"] + hh["tfar"] * hh["dir_z"]], 1).astype(np.float32)
+    sh = np.zeros(len(hh), R.RTCRay)
+    sh["org_x"], sh["org_y"], sh["org_z"] = p.T
+    d = (np.array([0.0, 100.0, 0.0], np.float32) - p).astype(np.float32)
+    sh["dir_x"], sh["dir_y"], sh["dir_z"] = d.T
+    sh["tnear"], sh["tfar"], sh["mask"] = 1e-4, 1.0, 0xFFFFFFFF
+    ns = len(sh)
+    d_ssrc, d_sh = dev.malloc(max(1, sh.nbytes)), dev.malloc(max(1, sh.nbytes))
+    dev.memcpy(d_ssrc, sh.ctypes.data, sh.nbytes)
+    L.rtcgTimeTrace(top.h, d_sh, d_ssrc, ns, 48, 1, 2, warm.ctypes.data)
+    oms = np.zeros(steps, np.float32)
+    L.rtcgTimeTrace(top.h, d_sh, d_ssrc, ns, 48, 1, steps, oms.ctypes.data)
+    dev.check("rtcgTimeTrace instanced occlusion")
+    shadow = np.zeros(ns, R.RTCRay)
+    dev.memcpy(shadow.ctypes.data, d_sh, sh.nbytes)
+    # oracle check on a subsample
+    t2 = time.perf_counter()
+    I = O.Instances()
+    oid = [I.add_object([m]) for m in meshes]
+    for k in range(num_instances):
+        I.add_instance(oid[which[k]], T[k], k)
+    rng = np.random.default_rng(12)
+    pick = rng.choice(n, min(check_rays, n), replace=False)
+    ref = rays[pick].copy()
+    I.intersect_bvh(ref)
+    same_ch = bool(np.array_equal(ref.view(np.uint8).reshape(len(ref), -1),
+                                  hits[pick].view(np.uint8).reshape(len(ref), -1)))
+    spick = rng.choice(ns, min(check_rays, ns), replace=False) if ns else np.zeros(0, np.int64)
+    sref = sh[spick].copy()
+    I.occluded_bvh(sref)
+    same_occ = bool(np.array_equal(sref["tfar"], shadow[spick]["tfar"]))
+    check_s = time.perf_counter() - t2
+    for q in (d_src, d_rays, d_ssrc, d_sh):
+        dev.free(q)
+    top.release()
+    for o in objs:
+        o.release()
+    dev.release()
+    base = [len(m[1]) for m in meshes]
+    c_ms, o_ms = float(ms.mean()), float(oms.mean())
+    return {"workload": f"{num_instances} instances of 5 base meshes ({sum(base)} triangles: {base}), "
+                        f"{n} camera rays ({width}x{height}) closest hit + {ns} shadow rays",
+            "instanced_triangles": int(sum(base[w] for w in which)),
+            "commit_top_ms": round(commit_ms, 2), "setup_s": round(setup_s, 1),
+            "closest_mrays_per_s": round(n / (c_ms / 1e3) / 1e6, 2), "closest_ms": round(c_ms, 3),
+            "occlusion_mrays_per_s": round(ns / (o_ms / 1e3) / 1e6, 2) if ns else None,
+            "hit_rate": round(float(hit.mean()), 4), "shadowed": round(float((shadow["tfar"] < 0).mean()), 4) if ns else None,
+            "oracle_check": {"rays": int(len(pick)), "shadow_rays": int(len(spick)), "closest_identical": same_ch,
+                             "occlusion_identical": same_occ, "seconds": round(check_s, 1)}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -232,6 +322,10 @@ def main():
     ap.add_argument("--train-steps", type=int, default=400)
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-scenes", type=int, default=256)
+    ap.add_argument("--no-instances", action="store_true")
+    ap.add_argument("--instances", type=int, default=10000)
+    ap.add_argument("--inst-width", type=int, default=8192)
+    ap.add_argument("--inst-height", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "single-GPU build: run with --gpus 1 (multi-GPU sharding is a design note only)"}))
@@ -338,6 +432,11 @@ def main():
     if not args.no_batch:
         batch = batched_scenes(args.batch_scenes, max(3, args.steps // 4))
 
+    inst = None
+    if not args.no_instances:
+        log("instanced scene")
+        inst = instanced_scene(args.instances, args.inst_width, args.inst_height, max(3, args.steps // 4))
+
     cpu = None
     if not args.no_cpu:
         log("timing the CPU baseline")
@@ -371,6 +470,7 @@ def main():
         "cpu_baseline": cpu,
         "learned_vs_sah": learned,
         "batched_scenes": batch,
+        "instanced": inst,
     }
     print(json.dumps(out), flush=True)
 

@@ -156,10 +156,15 @@ struct Geometry : RefObj {
   unsigned mask = 0xffffffffu;
   RTCBuildQuality quality = RTC_BUILD_QUALITY_MEDIUM;
   void* user = nullptr;
+  // RTC_GEOMETRY_TYPE_INSTANCE: the instanced scene (a Scene, retained) and
+  // its local-to-world transform, columns vx | vy | vz | p
+  RefObj* inst_scene = nullptr;
+  float l2w[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
   Geometry(Device* d, RTCGeometryType t) : dev(d), type(t) { ref_inc(d); }
   ~Geometry() override {
     vertex.clear();
     index.clear();
+    if (inst_scene) ref_dec(inst_scene);
     ref_dec(dev);
   }
 };
@@ -191,8 +196,23 @@ struct Scene : RefObj {
   std::vector<rtg::DevBuf*> gdata;
   std::vector<rtg::GeomDesc> descs;
   SplitModelH* model = nullptr;  // learned split cost for the next commits (retained)
+  // Two-level scenes (at least one instance at the last commit): the tree
+  // over instance records, the object table it indexes, and the object
+  // trees it was built against with their build generations. Holding the
+  // trees keeps them alive: a later commit of an instanced scene builds into
+  // new arrays, so this scene keeps tracing the trees of its own commit; a
+  // rebuild in place (generation change) is refused at query time.
+  bool two_level = false;
+  rtg::DeviceBVH top;
+  rtg::DevBuf top_objects;
+  rtg::DevTop top_view{};
+  std::vector<std::pair<std::shared_ptr<rtg::DeviceBVH>, uint64_t>> top_deps;
+  std::vector<RefObj*> top_scenes;  // instanced scenes, retained while the top tree refers to them
+  float bounds_lo[3] = {INFINITY, INFINITY, INFINITY};
+  float bounds_hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   explicit Scene(Device* d) : dev(d) { ref_inc(d); }
   ~Scene() override {
+    for (RefObj* o : top_scenes) ref_dec(o);
     if (model) ref_dec(model);
     for (Geometry* g : geoms)
       if (g) ref_dec(g);
@@ -371,11 +391,235 @@ void upload_geometry(Scene* sc, Device* d, ThreadDev& t, uint32_t scene, uint32_
     gd.prim_offset = prim_offset;
     gd.geomID = (uint32_t)id;
     gd.scene = scene;
-    gd.pad = 0;
+    gd.kind = rtg::GEOM_TRIANGLES;
     prim_offset += gd.num_tris;
     sc->descs.push_back(gd);
     out.push_back(gd);
   }
+}
+
+// ------------------------------------------------------------ instances --
+
+// world2local = rcp(local2world) as the reference's Instance::commit forms it
+// (kernels/common/scene_instance.cpp, common/math/affinespace.h:83): the
+// inverse of the linear part is adjoint / det (common/math/linearspace3.h:44-53)
+// with the Vec3fa cross (common/math/vec3fa.h:358-365: fma(a, b, -(c * d)))
+// and the SSE4.1 dpps dot (vec3fa.h:349-351: (x0*y0 + x1*y1) + x2*y2), each
+// element divided by det; the translation is -(il * p) (linearspace3.h:146).
+// Matrices are columns vx | vy | vz | p.
+void affine_inverse(const float* l2w, float* w2l) {
+  auto pd = [](float a, float b, float c, float d) { return fmaf(a, b, -(c * d)); };
+  auto cross = [&](const float* a, const float* b, float* r) {
+    r[0] = pd(a[1], b[2], a[2], b[1]);
+    r[1] = pd(a[2], b[0], a[0], b[2]);
+    r[2] = pd(a[0], b[1], a[1], b[0]);
+  };
+  const float *vx = l2w, *vy = l2w + 3, *vz = l2w + 6, *p = l2w + 9;
+  float c0[3], c1[3], c2[3];
+  cross(vy, vz, c0);
+  cross(vz, vx, c1);
+  cross(vx, vy, c2);
+  const float d01 = vx[0] * c0[0] + vx[1] * c0[1];
+  const float det = d01 + (vx[2] * c0[2] + 0.0f);
+  for (int k = 0; k < 3; k++) {  // column k of the adjoint is (c0[k], c1[k], c2[k])
+    w2l[3 * k + 0] = c0[k] / det;
+    w2l[3 * k + 1] = c1[k] / det;
+    w2l[3 * k + 2] = c2[k] / det;
+  }
+  for (int k = 0; k < 3; k++) w2l[9 + k] = -fmaf(p[0], w2l[k], fmaf(p[1], w2l[3 + k], p[2] * w2l[6 + k]));
+}
+
+// xfmBounds (common/math/affinespace.h:106-118): the box of the 8
+// transformed corners, each transformed as xfmPoint.
+void xfm_bounds(const float* m, const float* lo, const float* hi, float* olo, float* ohi) {
+  for (int k = 0; k < 3; k++) {
+    olo[k] = INFINITY;
+    ohi[k] = -INFINITY;
+  }
+  for (int c = 0; c < 8; c++) {
+    const float x = (c & 4) ? hi[0] : lo[0], y = (c & 2) ? hi[1] : lo[1], z = (c & 1) ? hi[2] : lo[2];
+    for (int k = 0; k < 3; k++) {
+      const float w = fmaf(x, m[k], fmaf(y, m[3 + k], fmaf(z, m[6 + k], m[9 + k])));
+      olo[k] = fminf(olo[k], w);
+      ohi[k] = fmaxf(ohi[k], w);
+    }
+  }
+}
+
+// The box an instance occupies in the top tree: the transformed corners in
+// double precision, widened by 2^-12 of (largest coordinate + extent) and
+// rounded outwards, so that hits found in object space with the rounded
+// inverse transform stay inside it.
+void instance_box(const float* m, const float* lo, const float* hi, float* olo, float* ohi) {
+  double dl[3] = {INFINITY, INFINITY, INFINITY}, dh[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int c = 0; c < 8; c++) {
+    const double x = (c & 4) ? hi[0] : lo[0], y = (c & 2) ? hi[1] : lo[1], z = (c & 1) ? hi[2] : lo[2];
+    for (int k = 0; k < 3; k++) {
+      const double w = x * m[k] + y * m[3 + k] + z * m[6 + k] + m[9 + k];
+      dl[k] = std::min(dl[k], w);
+      dh[k] = std::max(dh[k], w);
+    }
+  }
+  double big = 0.0, ext = 0.0;
+  for (int k = 0; k < 3; k++) {
+    big = std::max(big, std::max(std::fabs(dl[k]), std::fabs(dh[k])));
+    ext = std::max(ext, dh[k] - dl[k]);
+  }
+  const double margin = (big + ext) * (1.0 / 4096.0);
+  for (int k = 0; k < 3; k++) {
+    olo[k] = std::nextafter((float)(dl[k] - margin), -INFINITY);
+    ohi[k] = std::nextafter((float)(dh[k] + margin), INFINITY);
+  }
+}
+
+// Builds the top tree of a scene with instances over its instance records
+// (one per enabled instance of a committed, non-empty scene, plus one
+// identity record for the scene's own triangles). Leaves hold one record,
+// as the reference's instance BVH does (kernels/bvh/bvh_builder_sah.cpp,
+// BVH8InstanceSceneBuilderSAH: leaf size 1). Without instances the scene
+// stays single-level.
+void build_top(Scene* sc, Device* d, ThreadDev& t) {
+  for (RefObj* o : sc->top_scenes) ref_dec(o);
+  sc->top_scenes.clear();
+  sc->top_deps.clear();
+  sc->two_level = false;
+  const rtg::SceneSlot own = sc->slot < sc->bvh->slots.size() ? sc->bvh->slots[sc->slot] : rtg::SceneSlot();
+  for (int k = 0; k < 3; k++) {
+    sc->bounds_lo[k] = own.num_prims ? own.bounds_lo[k] : INFINITY;
+    sc->bounds_hi[k] = own.num_prims ? own.bounds_hi[k] : -INFINITY;
+  }
+  std::vector<std::pair<uint32_t, Geometry*>> ginst;
+  for (size_t id = 0; id < sc->geoms.size(); id++) {
+    Geometry* g = sc->geoms[id];
+    if (g && g->enabled && g->type == RTC_GEOMETRY_TYPE_INSTANCE && g->inst_scene) ginst.push_back({(uint32_t)id, g});
+  }
+  if (ginst.empty()) return;
+
+  std::vector<rtg::DevScene> objects;
+  std::vector<Scene*> object_scene;
+  std::vector<rtg::InstRec> recs;
+  std::vector<float> boxes;
+  uint32_t obj_stack = 0;
+  uint64_t obj_size = 0;
+  auto object_of = [&](Scene* o) -> uint32_t {
+    for (size_t k = 0; k < object_scene.size(); k++)
+      if (object_scene[k] == o) return (uint32_t)k;
+    objects.push_back(o->bvh->view(o->slot));
+    object_scene.push_back(o);
+    obj_stack = std::max(obj_stack, o->bvh->stats.max_stack);
+    obj_size = std::max(obj_size, (uint64_t)o->bvh->stats.num_nodes + o->bvh->stats.num_prims);
+    sc->top_deps.push_back({o->bvh, o->bvh->generation});
+    return (uint32_t)(objects.size() - 1);
+  };
+  auto add_record = [&](const float* w2l, uint32_t inst, uint32_t obj, uint32_t flags, const float* lo, const float* hi) {
+    rtg::InstRec r{};
+    for (int k = 0; k < 12; k++) r.m[k] = w2l[k];
+    r.instID = inst;
+    r.object = obj;
+    r.flags = flags;
+    recs.push_back(r);
+    for (int k = 0; k < 3; k++) boxes.push_back(lo[k]);
+    for (int k = 0; k < 3; k++) boxes.push_back(hi[k]);
+  };
+  if (own.num_prims) {
+    const float id12[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    add_record(id12, RTC_INVALID_GEOMETRY_ID, object_of(sc), rtg::INST_IDENTITY, own.bounds_lo, own.bounds_hi);
+  }
+  for (auto& gi : ginst) {
+    Scene* o = (Scene*)gi.second->inst_scene;
+    if (o == sc) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "a scene cannot instance itself");
+    if (!o->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "instanced scene not committed");
+    const rtg::SceneSlot os = o->slot < o->bvh->slots.size() ? o->bvh->slots[o->slot] : rtg::SceneSlot();
+    if (os.num_prims == 0) continue;  // nothing to hit (instances inside it lie beyond the one instance level)
+    float w2l[12], lo[3], hi[3], blo[3], bhi[3];
+    affine_inverse(gi.second->l2w, w2l);
+    instance_box(gi.second->l2w, os.bounds_lo, os.bounds_hi, lo, hi);
+    xfm_bounds(gi.second->l2w, os.bounds_lo, os.bounds_hi, blo, bhi);
+    for (int k = 0; k < 3; k++) {
+      sc->bounds_lo[k] = std::min(sc->bounds_lo[k], blo[k]);
+      sc->bounds_hi[k] = std::max(sc->bounds_hi[k], bhi[k]);
+    }
+    add_record(w2l, gi.first, object_of(o), 0u, lo, hi);
+  }
+  if (recs.empty() || (recs.size() == 1 && own.num_prims)) {
+    sc->top_deps.clear();
+    return;  // no instance to enter: the scene's own tree serves
+  }
+  const uint32_t n = (uint32_t)recs.size();
+  rtg::DevBuf dboxes, drecs;
+  dboxes.reserve((size_t)n * 24);
+  drecs.reserve((size_t)n * sizeof(rtg::InstRec));
+  RTG_HIP(hipMemcpyAsync(dboxes.ptr, boxes.data(), (size_t)n * 24, hipMemcpyHostToDevice, t.stream));
+  RTG_HIP(hipMemcpyAsync(drecs.ptr, recs.data(), (size_t)n * sizeof(rtg::InstRec), hipMemcpyHostToDevice, t.stream));
+  rtg::GeomDesc gd{};
+  gd.vertices = (const char*)dboxes.ptr;
+  gd.indices = (const char*)drecs.ptr;
+  gd.vstride = 24;
+  gd.istride = sizeof(rtg::InstRec);
+  gd.num_vertices = n;
+  gd.num_tris = n;
+  gd.prim_offset = 0;
+  gd.geomID = 0;
+  gd.scene = 0;
+  gd.kind = rtg::GEOM_INSTANCES;
+  rtg::BuildParams p = build_params(d, sc->quality, sc->flags);
+  p.builder = rtg::BUILDER_SAH;
+  p.max_leaf = 1;
+  p.learned = nullptr;
+  d->monitor((ssize_t)n * (80 + 64));
+  {
+    std::lock_guard<std::mutex> bl(d->build_mtx);
+    rtg::build_bvh({gd}, p, d->scratch, sc->top, t.stream);
+  }
+  sc->top_objects.reserve(objects.size() * sizeof(rtg::DevScene));
+  RTG_HIP(hipMemcpyAsync(sc->top_objects.ptr, objects.data(), objects.size() * sizeof(rtg::DevScene),
+                         hipMemcpyHostToDevice, t.stream));
+  RTG_HIP(hipStreamSynchronize(t.stream));  // the uploads read host vectors and free dboxes/drecs below
+  if (sc->top.stats.num_prims != n)
+    RTC_THROW(RTC_ERROR_INVALID_OPERATION, "invalid instance transform or bounds");
+  if (sc->top.stats.max_stack + obj_stack + 2 > rtg::TRACE_STACK)
+    RTC_THROW(RTC_ERROR_UNKNOWN, "two-level tree too deep for the traversal stack");
+  rtg::DevTop& v = sc->top_view;
+  v.nodes = sc->top.nodes.as<rtg::QNode8>();
+  v.insts = sc->top.insts.as<rtg::InstRec>();
+  v.objects = sc->top_objects.as<rtg::DevScene>();
+  v.num_nodes = sc->top.stats.num_nodes;
+  v.num_insts = n;
+  v.num_objects = (uint32_t)objects.size();
+  v.root = sc->top.slots.empty() ? 0u : sc->top.slots[0].root;
+  v.max_iter = (uint64_t)v.num_nodes + n + (uint64_t)n * (obj_size + 2) + 16;
+  for (Scene* o : object_scene)
+    if (o != sc) {
+      ref_inc(o);
+      sc->top_scenes.push_back(o);
+    }
+  sc->two_level = true;
+}
+
+// Throws when an object tree the top tree refers to was rebuilt since.
+void check_top(Scene* sc) {
+  for (auto& dp : sc->top_deps)
+    if (dp.first->generation != dp.second)
+      RTC_THROW(RTC_ERROR_INVALID_OPERATION, "an instanced scene was committed again: commit this scene again");
+}
+
+bool scene_empty(Scene* sc) { return sc->two_level ? false : sc->bvh->empty(sc->slot); }
+
+// Traverses the scene over device records on `stream`.
+void launch_scene(Scene* sc, ThreadDev& t, void* rays, unsigned M, size_t stride, bool occluded, uint32_t inst,
+                  hipStream_t stream, const rtg::TraceParams& tp) {
+  if (sc->two_level) {
+    check_top(sc);
+    rtg::launch_trace_inst(sc->top_view, rays, M, stride, occluded, inst, t.errword.as<uint32_t>(), stream, tp,
+                           &t.sorter);
+  } else if (!sc->bvh->empty(sc->slot)) {
+    rtg::launch_trace(sc->bvh->view(sc->slot), rays, M, stride, occluded, inst, nullptr, nullptr,
+                      t.errword.as<uint32_t>(), stream, tp, &t.sorter);
+  }
+}
+
+void single_level_only(Scene* sc, const char* what) {
+  if (sc->two_level) RTC_THROW(RTC_ERROR_INVALID_OPERATION, std::string(what) + " is not supported for scenes with instances");
 }
 
 // Uploads geometry buffers and builds the device BVH.
@@ -400,6 +644,7 @@ void commit_scene(Scene* sc) {
     rtg::build_bvh(descs, p, d->scratch, *sc->bvh, t.stream);
   }
   RTG_HIP(hipStreamSynchronize(t.stream));
+  build_top(sc, d, t);
   sc->committed = true;
   if (sc->progress && !sc->progress(sc->progress_ptr, 1.0)) RTC_THROW(RTC_ERROR_CANCELLED, "progress monitor forced termination");
 }
@@ -437,10 +682,9 @@ void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigne
   d->activate();
   ThreadDev& t = d->td();
   const uint32_t inst = ctx ? ctx->instID[0] : RTC_INVALID_GEOMETRY_ID;
-  if (sc->bvh->empty(sc->slot)) return;  // empty scene: every ray misses
+  if (scene_empty(sc)) return;  // empty scene: every ray misses
   if (is_device_ptr(d, rays)) {
-    rtg::launch_trace(sc->bvh->view(sc->slot), rays, M, stride, occluded, inst, nullptr, nullptr, t.errword.as<uint32_t>(),
-                      t.stream, d->trace, &t.sorter);
+    launch_scene(sc, t, rays, M, stride, occluded, inst, t.stream, d->trace);
     check_trace_error(t);
     return;
   }
@@ -454,8 +698,7 @@ void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigne
     RTG_HIP(hipMemcpyAsync(t.staging.ptr, pack.data(), pack.size(), hipMemcpyHostToDevice, t.stream));
     RTG_HIP(hipStreamSynchronize(t.stream));
   }
-  rtg::launch_trace(sc->bvh->view(sc->slot), t.staging.ptr, M, rec, occluded, inst, nullptr, nullptr,
-                    t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter);
+  launch_scene(sc, t, t.staging.ptr, M, rec, occluded, inst, t.stream, d->trace);
   if (stride == rec) {
     RTG_HIP(hipMemcpyAsync(rays, t.staging.ptr, (size_t)M * rec, hipMemcpyDeviceToHost, t.stream));
     RTG_HIP(hipStreamSynchronize(t.stream));
@@ -648,7 +891,8 @@ RTC_API RTCGeometry rtcNewGeometry(RTCDevice device, RTCGeometryType type) {
   Device* d = DE(device);
   API_BEGIN
   VERIFY_HANDLE(device);
-  if (type != RTC_GEOMETRY_TYPE_TRIANGLE) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "geometry type not supported");
+  if (type != RTC_GEOMETRY_TYPE_TRIANGLE && type != RTC_GEOMETRY_TYPE_INSTANCE)
+    RTC_THROW(RTC_ERROR_INVALID_OPERATION, "geometry type not supported");
   return (RTCGeometry) new Geometry(d, type);
   API_END(d)
   return nullptr;
@@ -819,6 +1063,83 @@ RTC_API void* rtcGetGeometryUserData(RTCGeometry geometry) {
   return nullptr;
 }
 
+// Instance geometry (reference rtcore.cpp:1000-1120, kernels/common/scene_instance.cpp).
+RTC_API void rtcSetGeometryInstancedScene(RTCGeometry geometry, RTCScene scene) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  VERIFY_HANDLE(scene);
+  if (g->type != RTC_GEOMETRY_TYPE_INSTANCE) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "operation not supported for this geometry");
+  Scene* s = SC(scene);
+  ref_inc(s);
+  if (g->inst_scene) ref_dec(g->inst_scene);
+  g->inst_scene = s;
+  g->committed = false;
+  API_END(g)
+}
+
+// loadTransform (reference rtcore.cpp:1008-1039): FLOAT3X4 row or column
+// major and FLOAT4X4 column major; other formats are invalid.
+RTC_API void rtcSetGeometryTransform(RTCGeometry geometry, unsigned int timeStep, RTCFormat format, const void* xfm) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  VERIFY_HANDLE(xfm);
+  if (g->type != RTC_GEOMETRY_TYPE_INSTANCE) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "operation not supported for this geometry");
+  if (timeStep != 0) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "invalid timestep");
+  const float* x = (const float*)xfm;
+  float m[12];
+  switch (format) {
+    case RTC_FORMAT_FLOAT3X4_ROW_MAJOR:
+      for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 3; r++) m[3 * c + r] = x[4 * r + c];
+      break;
+    case RTC_FORMAT_FLOAT3X4_COLUMN_MAJOR:
+      for (int k = 0; k < 12; k++) m[k] = x[k];
+      break;
+    case RTC_FORMAT_FLOAT4X4_COLUMN_MAJOR:
+      for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 3; r++) m[3 * c + r] = x[4 * c + r];
+      break;
+    default:
+      RTC_THROW(RTC_ERROR_INVALID_OPERATION, "invalid matrix format");
+  }
+  memcpy(g->l2w, m, sizeof(m));
+  g->committed = false;
+  API_END(g)
+}
+
+// storeTransform (reference rtcore.cpp:1041-1069); one time step, so `time`
+// does not matter.
+RTC_API void rtcGetGeometryTransform(RTCGeometry geometry, float time, RTCFormat format, void* xfm) {
+  Geometry* g = GE(geometry);
+  API_BEGIN
+  VERIFY_HANDLE(geometry);
+  VERIFY_HANDLE(xfm);
+  (void)time;
+  if (g->type != RTC_GEOMETRY_TYPE_INSTANCE) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "operation not supported for this geometry");
+  float* x = (float*)xfm;
+  const float* m = g->l2w;
+  switch (format) {
+    case RTC_FORMAT_FLOAT3X4_ROW_MAJOR:
+      for (int c = 0; c < 4; c++)
+        for (int r = 0; r < 3; r++) x[4 * r + c] = m[3 * c + r];
+      break;
+    case RTC_FORMAT_FLOAT3X4_COLUMN_MAJOR:
+      for (int k = 0; k < 12; k++) x[k] = m[k];
+      break;
+    case RTC_FORMAT_FLOAT4X4_COLUMN_MAJOR:
+      for (int c = 0; c < 4; c++) {
+        for (int r = 0; r < 3; r++) x[4 * c + r] = m[3 * c + r];
+        x[4 * c + 3] = c == 3 ? 1.0f : 0.0f;
+      }
+      break;
+    default:
+      RTC_THROW(RTC_ERROR_INVALID_OPERATION, "invalid matrix format");
+  }
+  API_END(g)
+}
+
 // ------------------------------------------------------------------- scene --
 
 RTC_API RTCScene rtcNewScene(RTCDevice device) {
@@ -977,7 +1298,10 @@ RTC_API void rtcGetSceneBounds(RTCScene scene, RTCBounds* b) {
   b->lower_x = st.bounds_lo[0]; b->lower_y = st.bounds_lo[1]; b->lower_z = st.bounds_lo[2];
   b->upper_x = st.bounds_hi[0]; b->upper_y = st.bounds_hi[1]; b->upper_z = st.bounds_hi[2];
   b->align0 = b->align1 = 0.0f;
-  if (st.num_prims == 0) {
+  if (s->two_level) {  // own triangles and transformed instance bounds
+    b->lower_x = s->bounds_lo[0]; b->lower_y = s->bounds_lo[1]; b->lower_z = s->bounds_lo[2];
+    b->upper_x = s->bounds_hi[0]; b->upper_y = s->bounds_hi[1]; b->upper_z = s->bounds_hi[2];
+  } else if (st.num_prims == 0) {
     b->lower_x = b->lower_y = b->lower_z = INFINITY;
     b->upper_x = b->upper_y = b->upper_z = -INFINITY;
   }
@@ -1064,8 +1388,7 @@ static void async_trace(Scene* s, const RTCIntersectContext* ctx, void* rays, un
   d->activate();
   ThreadDev& t = d->td();
   const uint32_t inst = ctx ? ctx->instID[0] : RTC_INVALID_GEOMETRY_ID;
-  rtg::launch_trace(s->bvh->view(s->slot), rays, M, stride, occluded, inst, nullptr, nullptr, t.errword.as<uint32_t>(),
-                    stream ? (hipStream_t)stream : t.stream, d->trace, &t.sorter);
+  launch_scene(s, t, rays, M, stride, occluded, inst, stream ? (hipStream_t)stream : t.stream, d->trace);
 }
 
 RTC_API void rtcgIntersect1MAsync(RTCScene scene, const RTCIntersectContext* context, RTCRayHit* d_rayhit,
@@ -1111,8 +1434,7 @@ RTC_API void rtcgTimeTrace(RTCScene scene, void* d_rays, const void* d_src, unsi
   for (int i = 0; i < steps; i++) {
     if (d_src) RTG_HIP(hipMemcpyAsync(d_rays, d_src, (size_t)M * byteStride, hipMemcpyDeviceToDevice, t.stream));
     RTG_HIP(hipEventRecord(t.ev0, t.stream));
-    rtg::launch_trace(s->bvh->view(s->slot), d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, nullptr, nullptr,
-                      t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter);
+    launch_scene(s, t, d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, t.stream, d->trace);
     RTG_HIP(hipEventRecord(t.ev1, t.stream));
     RTG_HIP(hipEventSynchronize(t.ev1));
     float ms = 0.0f;
@@ -1128,6 +1450,7 @@ RTC_API void rtcgTimeBuild(RTCScene scene, int steps, float* ms_out) {
   API_BEGIN
   VERIFY_HANDLE(scene);
   if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  single_level_only(s, "rebuild timing");
   Device* d = s->dev;
   d->activate();
   ThreadDev& t = d->td();
@@ -1153,6 +1476,7 @@ RTC_API void rtcgTraceCounted(RTCScene scene, void* d_rays, unsigned int M, size
   API_BEGIN
   VERIFY_HANDLE(scene);
   if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  single_level_only(s, "counted traversal");
   if (!d_nodes || !d_tris) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
   Device* d = s->dev;
   d->activate();
@@ -1348,6 +1672,7 @@ RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const v
   VERIFY_HANDLE(scene);
   if (M && !d_rays) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
   if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  single_level_only(s, "visit collection");
   if (byteStride < sizeof(RTCRayHit) || byteStride % 4) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid stride");
   if (M == 0 || s->bvh->empty(s->slot)) return;
   std::lock_guard<std::mutex> lk(m->mtx);
@@ -1458,6 +1783,9 @@ RTC_API void rtcgCommitScenes(RTCScene* scenes, unsigned int numScenes) {
     if ((sc[i]->flags & RTC_SCENE_FLAG_ROBUST) != (s0->flags & RTC_SCENE_FLAG_ROBUST))
       RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "scenes of a batch must agree on RTC_SCENE_FLAG_ROBUST");
     if (sc[i]->model) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "batched commits use the plain split cost");
+    for (Geometry* g : sc[i]->geoms)
+      if (g && g->enabled && g->type == RTC_GEOMETRY_TYPE_INSTANCE)
+        RTC_THROW(RTC_ERROR_INVALID_OPERATION, "batched commits hold triangle scenes only");
     for (unsigned j = 0; j < i; j++)
       if (sc[j] == sc[i]) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "a scene appears twice in the batch");
   }
@@ -1483,6 +1811,7 @@ RTC_API void rtcgCommitScenes(RTCScene* scenes, unsigned int numScenes) {
   for (unsigned i = 0; i < numScenes; i++) {
     sc[i]->bvh = store;
     sc[i]->slot = i;
+    build_top(sc[i], d, t);  // no instances: resets any earlier top tree
     sc[i]->committed = true;
   }
   API_END(s0)
@@ -1511,6 +1840,7 @@ PreparedBatch prepare_batch(const RTCGBatchItem* items, unsigned n, bool occlude
     Scene* s = SC(items[i].scene);
     if (!s) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
     if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+    single_level_only(s, "batched traversal");
     if (!pb.first) {
       pb.first = s;
       store = s->bvh.get();
@@ -1670,8 +2000,7 @@ float time_config(Scene* s, ThreadDev& t, const rtg::TraceParams& tp, const void
   for (int i = -1; i < reps; i++) {  // one untimed warm-up
     RTG_HIP(hipMemcpyAsync(work, src, (size_t)M * stride, hipMemcpyDeviceToDevice, t.stream));
     RTG_HIP(hipEventRecord(t.ev0, t.stream));
-    rtg::launch_trace(s->bvh->view(s->slot), work, M, stride, occluded, RTC_INVALID_GEOMETRY_ID, nullptr, nullptr,
-                      t.errword.as<uint32_t>(), t.stream, tp, &t.sorter);
+    launch_scene(s, t, work, M, stride, occluded, RTC_INVALID_GEOMETRY_ID, t.stream, tp);
     RTG_HIP(hipEventRecord(t.ev1, t.stream));
     RTG_HIP(hipEventSynchronize(t.ev1));
     float ms = 0.0f;
@@ -1700,6 +2029,7 @@ RTC_API float rtcgAutotune(RTCScene scene, const void* d_rays, unsigned int M, s
   API_BEGIN
   VERIFY_HANDLE(scene);
   if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  single_level_only(s, "autotuning");
   if (!d_rays || M == 0) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
   if (byteStride < (occluded ? sizeof(RTCRay) : sizeof(RTCRayHit)) || byteStride % 4)
     RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid stride");

@@ -52,6 +52,9 @@ struct SceneSlot {
 // batched build (they share the node and triangle arrays).
 struct DeviceBVH {
   DevBuf nodes, tris, info;
+  DevBuf insts;            // leaf records of an instance build (then tris is empty)
+  bool instances = false;
+  uint64_t generation = 0; // counts builds into these arrays (a build may move them)
   std::vector<SceneSlot> slots;
   DevScene view(uint32_t slot = 0) const {
     DevScene s;

@@ -131,26 +131,41 @@ __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ ge
     }
     const GeomDesc g = geoms[lo];
     const uint32_t prim = i - g.prim_offset;
-    const uint32_t* ix = (const uint32_t*)(g.indices + (uint64_t)prim * g.istride);
-    const uint32_t i0 = ix[0], i1 = ix[1], i2 = ix[2];
-    bool ok = (i0 < g.num_vertices) & (i1 < g.num_vertices) & (i2 < g.num_vertices);
-    float v[9];
-    if (ok) {
-      const float* p0 = (const float*)(g.vertices + (uint64_t)i0 * g.vstride);
-      const float* p1 = (const float*)(g.vertices + (uint64_t)i1 * g.vstride);
-      const float* p2 = (const float*)(g.vertices + (uint64_t)i2 * g.vstride);
-      v[0] = p0[0]; v[1] = p0[1]; v[2] = p0[2];
-      v[3] = p1[0]; v[4] = p1[1]; v[5] = p1[2];
-      v[6] = p2[0]; v[7] = p2[1]; v[8] = p2[2];
-#pragma unroll
-      for (int k = 0; k < 9; k++) ok &= finite_v(v[k]);
-    }
-    if (ok) {
-      float l[3], h[3];
+    bool ok;
+    float l[3], h[3];
+    if (g.kind == GEOM_INSTANCES) {  // a precomputed world box per instance record
+      const float* bx = (const float*)(g.vertices + (uint64_t)prim * g.vstride);
+      ok = true;
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        l[k] = fminf(v[k], fminf(v[3 + k], v[6 + k]));
-        h[k] = fmaxf(v[k], fmaxf(v[3 + k], v[6 + k]));
+        l[k] = bx[k];
+        h[k] = bx[3 + k];
+        ok &= finite_v(l[k]) & finite_v(h[k]) & (l[k] <= h[k]);
+      }
+    } else {
+      const uint32_t* ix = (const uint32_t*)(g.indices + (uint64_t)prim * g.istride);
+      const uint32_t i0 = ix[0], i1 = ix[1], i2 = ix[2];
+      ok = (i0 < g.num_vertices) & (i1 < g.num_vertices) & (i2 < g.num_vertices);
+      float v[9];
+      if (ok) {
+        const float* p0 = (const float*)(g.vertices + (uint64_t)i0 * g.vstride);
+        const float* p1 = (const float*)(g.vertices + (uint64_t)i1 * g.vstride);
+        const float* p2 = (const float*)(g.vertices + (uint64_t)i2 * g.vstride);
+        v[0] = p0[0]; v[1] = p0[1]; v[2] = p0[2];
+        v[3] = p1[0]; v[4] = p1[1]; v[5] = p1[2];
+        v[6] = p2[0]; v[7] = p2[1]; v[8] = p2[2];
+#pragma unroll
+        for (int k = 0; k < 9; k++) ok &= finite_v(v[k]);
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        l[k] = ok ? fminf(v[k], fminf(v[3 + k], v[6 + k])) : 0.0f;
+        h[k] = ok ? fmaxf(v[k], fmaxf(v[3 + k], v[6 + k])) : 0.0f;
+      }
+    }
+    if (ok) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
         const float c2 = l[k] + h[k];
         b[k] = l[k];
         b[3 + k] = h[k];
@@ -480,6 +495,7 @@ struct CollapseArgs {
   QNode8* nodes;
   TriRec* tris;
   NodeInfo* info;
+  InstRec* insts;     // leaf records of an instance build (tris unused)
   BuildCounters* cnt;
   uint32_t n;         // capacity of nodes and of each queue
   uint32_t tri_cap;   // capacity of tris
@@ -694,6 +710,12 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
           const uint32_t gi = __float_as_uint(a.lo4[o].w);
           const uint32_t prim = __float_as_uint(a.hi4[o].w);
           const GeomDesc g = a.geoms[gi];
+          if (g.kind == GEOM_INSTANCES) {
+            InstRec ir = *(const InstRec*)(g.indices + (uint64_t)prim * g.istride);
+            ir.flags = (ir.flags & ~TRI_LAST) | ((j + 1 == cntk) ? TRI_LAST : 0u);
+            a.insts[tri_base + off_t + j] = ir;
+            continue;
+          }
           const uint32_t* ix = (const uint32_t*)(g.indices + (uint64_t)prim * g.istride);
           const float* p0 = (const float*)(g.vertices + (uint64_t)ix[0] * g.vstride);
           const float* p1 = (const float*)(g.vertices + (uint64_t)ix[1] * g.vstride);
@@ -735,6 +757,7 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
 void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, BuildScratch& S, DeviceBVH& out,
                hipStream_t s, uint32_t nscenes) {
   auto t0 = std::chrono::steady_clock::now();
+  out.generation++;
   out.stats = BuildStats();
   out.robust = prm.robust ? 1u : 0u;
   uint32_t total = 0;
@@ -892,7 +915,13 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   // outputs: at most n + nscenes nodes, exactly n triangle records
   const uint32_t cap = n + nscenes;
   out.nodes.reserve((size_t)cap * sizeof(QNode8));
-  out.tris.reserve((size_t)n * sizeof(TriRec));
+  const bool inst_build = geoms[0].kind == GEOM_INSTANCES;
+  for (const auto& g : geoms)
+    if ((g.kind == GEOM_INSTANCES) != inst_build)
+      throw hip_error(hipErrorInvalidValue, "a build holds either triangles or instance records");
+  out.instances = inst_build;
+  if (inst_build) out.insts.reserve((size_t)n * sizeof(InstRec));
+  else out.tris.reserve((size_t)n * sizeof(TriRec));
   out.info.reserve((size_t)cap * sizeof(NodeInfo));
   S.queue.reserve((size_t)2 * cap * sizeof(QItem));
   QItem* q0 = S.queue.as<QItem>();
@@ -915,7 +944,8 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   ca.hi4 = S.hi4.as<float4>();
   ca.geoms = S.geoms.as<GeomDesc>();
   ca.nodes = out.nodes.as<QNode8>();
-  ca.tris = out.tris.as<TriRec>();
+  ca.tris = inst_build ? nullptr : out.tris.as<TriRec>();
+  ca.insts = inst_build ? out.insts.as<InstRec>() : nullptr;
   ca.info = out.info.as<NodeInfo>();
   ca.cnt = cnt;
   ca.n = cap;

@@ -56,10 +56,34 @@ static_assert(sizeof(TriRec) == 48, "TriRec must be 48 bytes");
 
 constexpr uint32_t TRI_LAST = 1u;
 
+// Leaf record of a two-level scene's top tree: one instance of an object
+// (a committed scene's own triangle tree). The traversal moves the ray into
+// object space with m, as the reference's instance intersector does with
+// the instance's world2local (kernels/geometry/instance_intersector.cpp:59-66,
+// xfmPoint / xfmVector common/math/affinespace.h:102-103,
+// common/math/linearspace3.h:156):
+//   org' = fma(org.x, vx, fma(org.y, vy, fma(org.z, vz, p)))
+//   dir' = fma(dir.x, vx, fma(dir.y, vy, dir.z * vz))
+// per component; t keeps its meaning, so hits compare across instances.
+struct alignas(16) InstRec {
+  float m[12];      // world-to-object affine, columns vx | vy | vz | p
+  uint32_t instID;  // instID[0] of hits below this record
+  uint32_t object;  // index into the scene's object table
+  uint32_t flags;   // TRI_LAST, INST_IDENTITY
+  uint32_t pad;
+};
+static_assert(sizeof(InstRec) == 64, "InstRec must be 64 bytes");
+
+// The record stands for the scene's own triangles: no transform, and hits
+// report the query context's instID[0].
+constexpr uint32_t INST_IDENTITY = 2u;
+
 // Stack / child reference: bit 31 set = leaf (triangle record index).
 constexpr uint32_t REF_LEAF = 0x80000000u;
 // Binary-tree reference of a scene with no valid primitive (batched builds).
 constexpr uint32_t REF_EMPTY = 0xfffffffeu;
+// Stack entry that returns a two-level traversal from an object to the top tree.
+constexpr uint32_t REF_INST_EXIT = 0xffffffffu;
 
 // Per-node side record (same index as the QNode8): the float union box of
 // the node's children and the number of primitives below it. Read by the
@@ -83,8 +107,11 @@ struct GeomDesc {
   uint32_t prim_offset;  // first global primitive index of this geometry
   uint32_t geomID;
   uint32_t scene;        // scene of a batched build (0 otherwise)
-  uint32_t pad;
+  uint32_t kind;         // GEOM_TRIANGLES, or GEOM_INSTANCES: `vertices` holds one
+                         // box (lo xyz, hi xyz) per primitive, `indices` its InstRec
 };
+constexpr uint32_t GEOM_TRIANGLES = 0u;
+constexpr uint32_t GEOM_INSTANCES = 1u;
 
 // What the traversal kernels need.
 struct DevScene {
@@ -96,6 +123,19 @@ struct DevScene {
   float scale[3];  // 1023.99 / extent per axis (0 for a flat axis)
   uint32_t robust; // triangle records hold (v0, v1, v2) for the Pluecker test
   uint32_t root;   // root node of this scene (scenes of a batch share the arrays)
+};
+
+// What the two-level traversal needs: the top tree over instance records
+// and the table of object trees they refer to.
+struct DevTop {
+  const QNode8* nodes;
+  const InstRec* insts;
+  const DevScene* objects;
+  uint32_t num_nodes;
+  uint32_t num_insts;
+  uint32_t num_objects;
+  uint32_t root;
+  uint64_t max_iter;  // bound on the steps of one ray (corrupt-structure guard)
 };
 
 // Entries of the per-ray traversal stack; the builder rejects a tree whose

@@ -55,4 +55,10 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
 void launch_trace_batch(const DevScene& sc, const BatchItem* items, uint32_t nitems, uint32_t M, bool occluded,
                         bool aligned, uint32_t* err, uint32_t* counter, hipStream_t s, const TraceParams& tp);
 
+// Two-level traversal of a scene with instances (top tree over InstRec
+// leaves, object trees in top.objects); otherwise as launch_trace. ctx_inst
+// is the query context's instID[0]; sorter supplies the ray counter.
+void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t ctx_inst,
+                       uint32_t* err, hipStream_t s, const TraceParams& tp, RaySorter* sorter);
+
 }  // namespace rtg

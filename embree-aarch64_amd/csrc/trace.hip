@@ -142,8 +142,9 @@ struct NodeHit {
 // exact expression the builder verified against every child box, then the
 // robust per-plane distance (plane - org) * rdir with rdir rounded down for
 // near and up for far planes.
-__device__ inline void intersect_node(const DevScene& sc, uint32_t i, const TRay& r, float thi, NodeHit& h) {
-  const uint4* np = reinterpret_cast<const uint4*>(sc.nodes + i);
+__device__ inline void intersect_node(const QNode8* __restrict__ nodes, uint32_t i, const TRay& r, float thi,
+                                      NodeHit& h) {
+  const uint4* np = reinterpret_cast<const uint4*>(nodes + i);
   const uint4 w0 = np[0], w1 = np[1], w2 = np[2], w3 = np[3], w4 = np[4];
   const float px = __uint_as_float(w0.x), py = __uint_as_float(w0.y), pz = __uint_as_float(w0.z);
   const float sx = __uint_as_float((w0.w & 0xffu) << 23);
@@ -346,7 +347,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
         break;
       }
       NodeHit h;
-      intersect_node(sc, cur, r, lim, h);
+      intersect_node(sc.nodes, cur, r, lim, h);
       if (COUNT) {
         nvis++;
         if (node_visits) atomicAdd(&node_visits[cur], 1u);
@@ -560,7 +561,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
         done = true;
       } else {
         NodeHit h;
-        intersect_node(sc, cur, r, lim, h);
+        intersect_node(sc.nodes, cur, r, lim, h);
         if (COUNT) {
           nvis++;
           if (node_visits) atomicAdd(&node_visits[cur], 1u);
@@ -662,6 +663,261 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
         u[17] = best.prim;
         u[18] = best.geom;
         u[19] = inst_id;
+      }
+      active = false;
+    }
+  }
+}
+
+// ------------------------------------------------------------- two-level --
+
+// Two-level traversal of a scene with instances. The top tree's leaves are
+// InstRec records; reaching one pushes a REF_INST_EXIT entry, moves the ray
+// into the object's space with the record's world-to-object transform
+// (kernels/geometry/instance_intersector.cpp:52-71 for closest hit, :73-98
+// for occlusion) and continues at the object tree's root; popping the exit
+// entry restores the world-space ray from its record. Both trees share one
+// stack, so the current hit distance culls object entries and top entries
+// alike (t has the same meaning in both spaces). Persistent and refilled as
+// k_trace_p. Hits are ordered by (t, instID, geomID, primID).
+// ctx_inst != RTC_INVALID_GEOMETRY_ID: the query context's one-level
+// instance stack is already in use, so every instance push fails and only
+// the scene's own triangles are tested (kernels/common/instance_stack.h:21-38).
+template <bool OCCLUDED, bool ALIGNED>
+__global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restrict__ rays, uint32_t M,
+                                                      uint64_t stride, uint32_t ctx_inst, uint32_t* __restrict__ err,
+                                                      uint32_t* __restrict__ next, int refill_min) {
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  __shared__ uint64_t lds_stack[8 * TB];
+  uint64_t spill_mem[STACK];
+  RingStack<8> stack;
+  stack.init(lds_stack + threadIdx.x, spill_mem);
+
+  TRay r;
+  Hit best;
+  uint32_t best_inst = 0;
+  float lim = 0.0f;
+  uint32_t cur = 0, ray = 0, inst = 0;
+  bool in_obj = false;  // in an object tree (else the top tree)
+  bool moved = false;   // r is in an instance's object space
+  const QNode8* onodes = nullptr;
+  const TriRec* otris = nullptr;
+  uint32_t onn = 0, otn = 0;
+  bool orobust = false;
+  bool active = false, occluded = false, pool_empty = false;
+  uint64_t iter = 0;
+  const float neg_inf = -INFINITY;
+
+  while (true) {
+    // ---- refill idle lanes (wave-uniform decision)
+    if (!pool_empty) {
+      const uint64_t idle = __ballot(!active);
+      const int nidle = __popcll(idle);
+      if (nidle >= refill_min) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(next, (uint32_t)nidle);
+        base = __shfl(base, 0, 64);
+        if ((uint64_t)base + nidle >= M) pool_empty = true;
+        if (!active) {
+          const uint32_t slot = base + (uint32_t)__popcll(idle & lt);
+          if (slot < M) {
+            ray = slot;
+            float4 a, b;
+            float tfar;
+            load_ray<ALIGNED>(rays + (uint64_t)ray * stride, a, b, tfar);
+            if (a.w <= tfar) {
+              setup_ray(r, a, b, tfar);
+              best.found = false;
+              best.t = INFINITY;
+              best.prim = best.geom = 0xffffffffu;
+              best.u = best.v = best.ngx = best.ngy = best.ngz = 0.0f;
+              best_inst = 0xffffffffu;
+              lim = widen_up(tfar);
+              cur = top.root;
+              in_obj = false;
+              moved = false;
+              stack.init(lds_stack + threadIdx.x, spill_mem);
+              occluded = false;
+              iter = 0;
+              active = true;
+            }
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (pool_empty) break;
+      continue;
+    }
+    if (!active) continue;
+
+    // ---- one traversal step
+    bool done = false;
+    if (++iter > top.max_iter) {
+      atomicOr(err, 2u);
+      done = true;
+    } else if (!(cur & REF_LEAF)) {
+      const QNode8* nodes = in_obj ? onodes : top.nodes;
+      const uint32_t nn = in_obj ? onn : top.num_nodes;
+      if (cur >= nn) {
+        atomicOr(err, 4u);
+        done = true;
+      } else {
+        NodeHit h;
+        intersect_node(nodes, cur, r, lim, h);
+        if (h.mask) {
+          if (OCCLUDED) {
+            const int c0 = __builtin_ctz(h.mask);
+            uint32_t rem = h.mask & (h.mask - 1u);
+            while (rem) {
+              const int c = __builtin_ctz(rem);
+              rem &= rem - 1u;
+              if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
+            }
+            cur = child_ref(h, c0);
+          } else {
+            int bi = -1;
+            float bv = 0.0f;
+#pragma unroll
+            for (int c = 0; c < 8; c++)
+              if (((h.mask >> c) & 1u) && (bi < 0 || h.tn[c] < bv)) { bi = c; bv = h.tn[c]; }
+            uint32_t rem = h.mask & ~(1u << bi);
+            while (rem) {
+              int mc = -1;
+              float mv = 0.0f;
+#pragma unroll
+              for (int c = 0; c < 8; c++)
+                if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
+              rem &= ~(1u << mc);
+              if (!stack.push(make_uint2(child_ref(h, mc), __float_as_uint(mv)))) atomicOr(err, 1u);
+            }
+            cur = child_ref(h, bi);
+          }
+          continue;
+        }
+      }
+    } else if (in_obj) {
+      // one triangle of the current object
+      const uint32_t ti = cur & ~REF_LEAF;
+      if (ti >= otn) {
+        atomicOr(err, 4u);
+        done = true;
+      } else {
+        TriTest h;
+        const bool valid = orobust ? tri_test<true>(r, otris + ti, h) : tri_test<false>(r, otris + ti, h);
+        if (valid) {
+          if (OCCLUDED) {
+            occluded = true;
+          } else {
+            const bool better =
+                !best.found || h.t < best.t ||
+                (h.t == best.t &&
+                 (inst < best_inst ||
+                  (inst == best_inst &&
+                   (h.geom < best.geom || (h.geom == best.geom && h.prim < best.prim)))));
+            if (better) {
+              best.found = true;
+              best.t = h.t;
+              best.u = h.un / h.dn;
+              best.v = h.vn / h.dn;
+              best.ngx = h.ngx; best.ngy = h.ngy; best.ngz = h.ngz;
+              best.prim = h.prim; best.geom = h.geom;
+              best_inst = inst;
+              lim = widen_up(fminf(best.t, r.tfar));
+            }
+          }
+        }
+        if (OCCLUDED && occluded) {
+          done = true;
+        } else if (!(h.flags & TRI_LAST)) {
+          cur = (ti + 1) | REF_LEAF;
+          continue;
+        }
+      }
+    } else {
+      // an instance record of the top tree
+      const uint32_t ii = cur & ~REF_LEAF;
+      if (ii >= top.num_insts) {
+        atomicOr(err, 4u);
+        done = true;
+      } else {
+        const float4* q = reinterpret_cast<const float4*>(top.insts + ii);
+        const float4 m0 = q[0], m1 = q[1], m2 = q[2];
+        const uint4 w = reinterpret_cast<const uint4*>(top.insts + ii)[3];
+        const bool identity = (w.z & INST_IDENTITY) != 0u;
+        // the rest of a multi-record leaf resumes after this instance
+        if (!(w.z & TRI_LAST) && !stack.push(make_uint2((ii + 1u) | REF_LEAF, __float_as_uint(neg_inf))))
+          atomicOr(err, 1u);
+        if ((identity || ctx_inst == 0xffffffffu) && w.y < top.num_objects) {
+          const DevScene o = top.objects[w.y];
+          if (o.num_nodes > 0) {
+            if (!stack.push(make_uint2(REF_INST_EXIT, __float_as_uint(neg_inf)))) atomicOr(err, 1u);
+            if (!identity) {
+              // columns vx = m[0..2], vy = m[3..5], vz = m[6..8], p = m[9..11]
+              const float ox = __builtin_fmaf(r.ox, m0.x, __builtin_fmaf(r.oy, m0.w, __builtin_fmaf(r.oz, m1.z, m2.y)));
+              const float oy = __builtin_fmaf(r.ox, m0.y, __builtin_fmaf(r.oy, m1.x, __builtin_fmaf(r.oz, m1.w, m2.z)));
+              const float oz = __builtin_fmaf(r.ox, m0.z, __builtin_fmaf(r.oy, m1.y, __builtin_fmaf(r.oz, m2.x, m2.w)));
+              const float dx = __builtin_fmaf(r.dx, m0.x, __builtin_fmaf(r.dy, m0.w, r.dz * m1.z));
+              const float dy = __builtin_fmaf(r.dx, m0.y, __builtin_fmaf(r.dy, m1.x, r.dz * m1.w));
+              const float dz = __builtin_fmaf(r.dx, m0.z, __builtin_fmaf(r.dy, m1.y, r.dz * m2.x));
+              setup_ray(r, make_float4(ox, oy, oz, r.tnear), make_float4(dx, dy, dz, 0.0f), r.tfar);
+              moved = true;
+            }
+            in_obj = true;
+            inst = identity ? ctx_inst : w.x;
+            onodes = o.nodes;
+            otris = o.tris;
+            onn = o.num_nodes;
+            otn = o.num_tris;
+            orobust = o.robust != 0u;
+            cur = o.root;
+            continue;
+          }
+        }
+      }
+    }
+    if (!done) {
+      // pop the next entry that can still hold a closer hit; an exit entry
+      // returns the ray to world space
+      bool got = false;
+      while (!stack.empty()) {
+        const uint2 e = stack.pop();
+        if (e.x == REF_INST_EXIT) {
+          in_obj = false;
+          if (moved) {
+            float4 a, b;
+            float tf;
+            load_ray<ALIGNED>(rays + (uint64_t)ray * stride, a, b, tf);
+            setup_ray(r, a, b, r.tfar);
+            moved = false;
+          }
+          continue;
+        }
+        if (OCCLUDED || __uint_as_float(e.y) <= lim) {
+          cur = e.x;
+          got = true;
+          break;
+        }
+      }
+      done = !got;
+    }
+    if (done) {
+      char* rp = rays + (uint64_t)ray * stride;
+      if (OCCLUDED) {
+        if (occluded) *reinterpret_cast<float*>(rp + 32) = -INFINITY;
+      } else if (best.found) {
+        float* f = reinterpret_cast<float*>(rp);
+        uint32_t* u = reinterpret_cast<uint32_t*>(rp);
+        f[8] = best.t;
+        f[12] = best.ngx;
+        f[13] = best.ngy;
+        f[14] = best.ngz;
+        f[15] = best.u;
+        f[16] = best.v;
+        u[17] = best.prim;
+        u[18] = best.geom;
+        u[19] = best_inst;
       }
       active = false;
     }
@@ -811,4 +1067,28 @@ void launch_trace_batch(const DevScene& sc, const BatchItem* items, uint32_t nit
   RTG_HIP(hipGetLastError());
 }
 
+void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t ctx_inst,
+                       uint32_t* err, hipStream_t s, const TraceParams& tp, RaySorter* sorter) {
+  if (M == 0 || top.num_nodes == 0) return;
+  const bool aligned = ((uintptr_t)rays % 16 == 0) && (stride % 16 == 0);
+  const uint32_t blocks = div_up(M, TB);
+  const uint32_t want = (uint32_t)tp.cus * (uint32_t)tp.blocks_per_cu;
+  const dim3 grid(want < blocks ? want : blocks), block(TB);
+  const int rmin = occluded ? tp.refill_min_occ : tp.refill_min;
+  const int refill = rmin < 1 ? 1 : (rmin > 64 ? 64 : rmin);
+  sorter->counter.reserve(256);
+  uint32_t* counter = sorter->counter.as<uint32_t>();
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, counter);
+  char* rp = (char*)rays;
+  if (occluded) {
+    if (aligned) hipLaunchKernelGGL((k_trace_inst<true, true>), grid, block, 0, s, top, rp, M, stride, ctx_inst, err, counter, refill);
+    else hipLaunchKernelGGL((k_trace_inst<true, false>), grid, block, 0, s, top, rp, M, stride, ctx_inst, err, counter, refill);
+  } else {
+    if (aligned) hipLaunchKernelGGL((k_trace_inst<false, true>), grid, block, 0, s, top, rp, M, stride, ctx_inst, err, counter, refill);
+    else hipLaunchKernelGGL((k_trace_inst<false, false>), grid, block, 0, s, top, rp, M, stride, ctx_inst, err, counter, refill);
+  }
+  RTG_HIP(hipGetLastError());
+}
+
 }  // namespace rtg
+

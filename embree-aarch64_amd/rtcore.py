@@ -24,6 +24,10 @@ RTC_BUFFER_TYPE_VERTEX = 1
 RTC_FORMAT_UINT3 = 0x5003
 RTC_FORMAT_FLOAT3 = 0x9003
 RTC_FORMAT_FLOAT4 = 0x9004
+RTC_FORMAT_FLOAT3X4_ROW_MAJOR = 0x9134
+RTC_FORMAT_FLOAT4X4_ROW_MAJOR = 0x9144
+RTC_FORMAT_FLOAT3X4_COLUMN_MAJOR = 0x9234
+RTC_FORMAT_FLOAT4X4_COLUMN_MAJOR = 0x9244
 RTC_BUILD_QUALITY_LOW = 0
 RTC_BUILD_QUALITY_MEDIUM = 1
 RTC_BUILD_QUALITY_HIGH = 2
@@ -105,6 +109,9 @@ SIGNATURES = {
     "rtcSetGeometryOccludedFilterFunction": (None, [_vp, _vp]),
     "rtcSetGeometryUserData": (None, [_vp, _vp]),
     "rtcGetGeometryUserData": (_vp, [_vp]),
+    "rtcSetGeometryInstancedScene": (None, [_vp, _vp]),
+    "rtcSetGeometryTransform": (None, [_vp, _u32, _i32, _vp]),
+    "rtcGetGeometryTransform": (None, [_vp, _f32, _i32, _vp]),
     "rtcNewScene": (_vp, [_vp]),
     "rtcGetSceneDevice": (_vp, [_vp]),
     "rtcRetainScene": (None, [_vp]),
@@ -317,6 +324,40 @@ def batch_workload(num_scenes=256, seed=7, tri_range=(1e3, 2e6), ray_range=(4e3,
     return [(int(p), int(4 * p * (p - 1)), int(r)) for p, r in zip(phis, rays)]
 
 
+def random_transforms(n, seed=5, scale=(0.5, 2.0), extent=50.0):
+    """n local-to-world transforms (n x 12 floats, columns vx | vy | vz | p):
+    a uniform random rotation (Shoemake's quaternion), a uniform scale in
+    `scale` and a translation uniform in [-extent, extent]^3."""
+    rng = np.random.default_rng(seed)
+    u1, u2, u3 = rng.random((3, n))
+    a, b = np.sqrt(1.0 - u1), np.sqrt(u1)
+    qx, qy = a * np.sin(2 * np.pi * u2), a * np.cos(2 * np.pi * u2)
+    qz, qw = b * np.sin(2 * np.pi * u3), b * np.cos(2 * np.pi * u3)
+    R = np.empty((n, 3, 3))
+    R[:, 0, 0] = 1 - 2 * (qy * qy + qz * qz); R[:, 0, 1] = 2 * (qx * qy - qz * qw); R[:, 0, 2] = 2 * (qx * qz + qy * qw)
+    R[:, 1, 0] = 2 * (qx * qy + qz * qw); R[:, 1, 1] = 1 - 2 * (qx * qx + qz * qz); R[:, 1, 2] = 2 * (qy * qz - qx * qw)
+    R[:, 2, 0] = 2 * (qx * qz - qy * qw); R[:, 2, 1] = 2 * (qy * qz + qx * qw); R[:, 2, 2] = 1 - 2 * (qx * qx + qy * qy)
+    sc = rng.uniform(scale[0], scale[1], n)
+    t = rng.uniform(-extent, extent, (n, 3))
+    out = np.empty((n, 12), np.float32)
+    for c in range(3):  # column c = R[:, :, c] * scale
+        out[:, 3 * c:3 * c + 3] = R[:, :, c] * sc[:, None]
+    out[:, 9:12] = t
+    return out
+
+
+def instance_workload(seed=11, sizes=None):
+    """Base meshes of the instanced-scene workload: two displaced spheres and
+    three heightfield terrain patches. sizes = (sphere numPhi, sphere numPhi,
+    terrain n, terrain n, terrain n); default about 1M, 200k, 1M, 200k and
+    100k triangles."""
+    sizes = sizes or (500, 224, 708, 317, 225)
+    meshes = [gen_displaced_sphere(sizes[0], seed=seed), gen_displaced_sphere(sizes[1], seed=seed + 1)]
+    for k, n in enumerate(sizes[2:]):
+        meshes.append(gen_heightfield(n, seed=seed + 2 + k))
+    return meshes
+
+
 class Device:
     def __init__(self, config=""):
         self.h = lib().rtcNewDevice(config.encode())
@@ -384,6 +425,25 @@ class Scene:
         gid = L.rtcAttachGeometry(self.h, g)
         L.rtcReleaseGeometry(g)
         self.device.check("add_mesh")
+        return gid
+
+    def add_instance(self, scene, l2w, enabled=True):
+        """Attaches an instance of the committed `scene` with the local-to-world
+        transform l2w (3x4: columns vx, vy, vz, p as the rows of a 4x3 array, or
+        a 3x4 row-major matrix given as 12 floats in rows); returns its geomID."""
+        L = lib()
+        m = np.ascontiguousarray(l2w, np.float32).reshape(-1)
+        if m.size != 12:
+            raise ValueError("l2w needs 12 floats (FLOAT3X4 column major)")
+        g = L.rtcNewGeometry(self.device.h, RTC_GEOMETRY_TYPE_INSTANCE)
+        L.rtcSetGeometryInstancedScene(g, scene.h)
+        L.rtcSetGeometryTransform(g, 0, RTC_FORMAT_FLOAT3X4_COLUMN_MAJOR, m.ctypes.data)
+        if not enabled:
+            L.rtcDisableGeometry(g)
+        L.rtcCommitGeometry(g)
+        gid = L.rtcAttachGeometry(self.h, g)
+        L.rtcReleaseGeometry(g)
+        self.device.check("add_instance")
         return gid
 
     def commit(self):

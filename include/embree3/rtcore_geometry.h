@@ -104,6 +104,28 @@ RTC_API void rtcUpdateGeometryBuffer(RTCGeometry geometry, enum RTCBufferType ty
 RTC_API void rtcSetGeometryIntersectFilterFunction(RTCGeometry geometry, RTCFilterFunctionN filter);
 RTC_API void rtcSetGeometryOccludedFilterFunction(RTCGeometry geometry, RTCFilterFunctionN filter);
 
+/* Instances (reference: rtcore_geometry.h:224-233; kernels/common/scene_instance.cpp,
+ * kernels/geometry/instance_intersector.cpp:52-98). An RTC_GEOMETRY_TYPE_INSTANCE
+ * geometry places a committed scene into its parent with an affine
+ * local-to-world transform; the parent's commit builds a top tree over its
+ * instances (and its own triangles) and the traversal moves each ray into
+ * the instance's object space. One instance level (RTC_MAX_INSTANCE_LEVEL_COUNT
+ * 1): instances inside an instanced scene are not entered. Hits report
+ * instID[0] = the instance's geomID in the parent and Ng in object space.
+ * An instanced scene committed again after its parent's commit stays, for
+ * the parent, as it was at the parent's commit (the parent holds that tree)
+ * until the parent is committed again. One time step only. */
+RTC_API void rtcSetGeometryInstancedScene(RTCGeometry geometry, RTCScene scene);
+
+/* Transform of time step 0 (reference: rtcore_geometry.h:227, rtcore.cpp:1008-1039):
+ * RTC_FORMAT_FLOAT3X4_ROW_MAJOR, RTC_FORMAT_FLOAT3X4_COLUMN_MAJOR or
+ * RTC_FORMAT_FLOAT4X4_COLUMN_MAJOR; other formats record RTC_ERROR_INVALID_OPERATION. */
+RTC_API void rtcSetGeometryTransform(RTCGeometry geometry, unsigned int timeStep, enum RTCFormat format, const void* xfm);
+
+/* Reads the transform back in the given format (reference: rtcore_geometry.h:233,
+ * rtcore.cpp:1041-1069). */
+RTC_API void rtcGetGeometryTransform(RTCGeometry geometry, float time, enum RTCFormat format, void* xfm);
+
 /* User data pointer (reference: rtcore_geometry.h:196-199). */
 RTC_API void rtcSetGeometryUserData(RTCGeometry geometry, void* ptr);
 RTC_API void* rtcGetGeometryUserData(RTCGeometry geometry);

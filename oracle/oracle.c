@@ -329,6 +329,157 @@ void oracle_occluded_brute_robust(const oracle_mesh* meshes, unsigned nmeshes, v
   o_occluded_brute(meshes, nmeshes, rays, nrays, stride, nthreads, 1);
 }
 
+/* ------------------------------------------------------------ instances -- */
+
+/* Vec3fa cross: shuffle<1,2,0,3>(prod_diff(a, b.yzx, a.yzx, b)) gives
+ * (a.y*b.z - a.z*b.y, ...) with the same fma(a,b,-(c*d)) as cross3. */
+static inline float dot_dpps(const float a[3], const float b[3]) {
+  const float p0 = a[0] * b[0], p1 = a[1] * b[1], p2 = a[2] * b[2];
+  return (p0 + p1) + (p2 + 0.0f);
+}
+
+void oracle_affine_inverse(const float l2w[12], float w2l[12]) {
+  const float* vx = l2w;
+  const float* vy = l2w + 3;
+  const float* vz = l2w + 6;
+  const float* p = l2w + 9;
+  float c0[3], c1[3], c2[3];
+  cross3(vy, vz, c0);
+  cross3(vz, vx, c1);
+  cross3(vx, vy, c2);
+  const float det = dot_dpps(vx, c0);
+  /* adjoint = LinearSpace3(c0, c1, c2).transposed(): column k = (c0[k], c1[k], c2[k]) */
+  float il[9];
+  for (int k = 0; k < 3; k++) {
+    il[3 * k + 0] = c0[k] / det;
+    il[3 * k + 1] = c1[k] / det;
+    il[3 * k + 2] = c2[k] / det;
+  }
+  for (int k = 0; k < 9; k++) w2l[k] = il[k];
+  for (int k = 0; k < 3; k++)
+    w2l[9 + k] = -fmaf(p[0], il[k], fmaf(p[1], il[3 + k], p[2] * il[6 + k]));
+}
+
+void oracle_xfm_point(const float m[12], const float p[3], float out[3]) {
+  for (int k = 0; k < 3; k++) out[k] = fmaf(p[0], m[k], fmaf(p[1], m[3 + k], fmaf(p[2], m[6 + k], m[9 + k])));
+}
+
+void oracle_xfm_vector(const float m[12], const float v[3], float out[3]) {
+  for (int k = 0; k < 3; k++) out[k] = fmaf(v[0], m[k], fmaf(v[1], m[3 + k], v[2] * m[6 + k]));
+}
+
+/* Lexicographic (t, instID, geomID, primID) order across instances. */
+static inline int o_better_inst(const o_best* a, unsigned ia, const o_best* b, unsigned ib) {
+  if (!b->found) return 1;
+  if (a->t < b->t) return 1;
+  if (a->t > b->t) return 0;
+  if (ia != ib) return ia < ib;
+  if (a->geomID != b->geomID) return a->geomID < b->geomID;
+  return a->primID < b->primID;
+}
+
+/* Closest hit in one object for a ray already in its space. */
+static void o_object_best(const oracle_object* ob, const float org[3], const float dir[3], float tnear, float tfar,
+                          o_best* b) {
+  memset(b, 0, sizeof(*b));
+  for (unsigned g = 0; g < ob->nmeshes; g++) {
+    const oracle_mesh* m = &ob->meshes[g];
+    for (unsigned p = 0; p < m->num_triangles; p++) {
+      if (!oracle_triangle_valid(m, p)) continue;
+      if (ob->robust) {
+        const float *a, *bb, *c;
+        o_tri_verts(m, p, &a, &bb, &c);
+        o_consider_pl(org, dir, tnear, tfar, a, bb, c, g, p, b);
+      } else {
+        float p0[3], e1[3], e2[3];
+        o_tri_edges(m, p, p0, e1, e2);
+        o_consider(org, dir, tnear, tfar, p0, e1, e2, g, p, b);
+      }
+    }
+  }
+}
+
+static int o_object_any(const oracle_object* ob, const float org[3], const float dir[3], float tnear, float tfar) {
+  for (unsigned g = 0; g < ob->nmeshes; g++) {
+    const oracle_mesh* m = &ob->meshes[g];
+    for (unsigned p = 0; p < m->num_triangles; p++) {
+      if (!oracle_triangle_valid(m, p)) continue;
+      float h[8];
+      if (ob->robust) {
+        const float *a, *bb, *c;
+        o_tri_verts(m, p, &a, &bb, &c);
+        if (oracle_pluecker(org, dir, tnear, tfar, a, bb, c, h)) return 1;
+      } else {
+        float p0[3], e1[3], e2[3];
+        o_tri_edges(m, p, p0, e1, e2);
+        if (oracle_moeller(org, dir, tnear, tfar, p0, e1, e2, h)) return 1;
+      }
+    }
+  }
+  return 0;
+}
+
+/* The ray in an instance's object space (instance_intersector.cpp:59-61). */
+static inline void o_inst_ray(const oracle_instance* in, const float org[3], const float dir[3], float o[3],
+                              float d[3]) {
+  if (in->identity) {
+    memcpy(o, org, 12);
+    memcpy(d, dir, 12);
+  } else {
+    oracle_xfm_point(in->w2l, org, o);
+    oracle_xfm_vector(in->w2l, dir, d);
+  }
+}
+
+void oracle_intersect_instances(const oracle_object* objects, const oracle_instance* insts, unsigned ninsts,
+                                void* rayhits, unsigned nrays, size_t stride, unsigned ctx_inst, int nthreads) {
+  int nt = o_threads(nthreads);
+  (void)nt;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+  for (long i = 0; i < (long)nrays; i++) {
+    o_rayhit* r = (o_rayhit*)((char*)rayhits + (size_t)i * stride);
+    if (!(r->tnear <= r->tfar)) continue;
+    o_best best;
+    memset(&best, 0, sizeof(best));
+    unsigned best_inst = 0xffffffffu;
+    for (unsigned k = 0; k < ninsts; k++) {
+      const oracle_instance* in = &insts[k];
+      if (!in->identity && ctx_inst != 0xffffffffu) continue;
+      const unsigned id = in->identity ? ctx_inst : in->instID;
+      float o[3], d[3];
+      o_inst_ray(in, r->org, r->dir, o, d);
+      o_best b;
+      o_object_best(&objects[in->object], o, d, r->tnear, r->tfar, &b);
+      if (b.found && o_better_inst(&b, id, &best, best_inst)) {
+        best = b;
+        best_inst = id;
+      }
+    }
+    if (best.found) o_write_hit(r, &best, best_inst);
+  }
+}
+
+void oracle_occluded_instances(const oracle_object* objects, const oracle_instance* insts, unsigned ninsts,
+                               void* rays, unsigned nrays, size_t stride, unsigned ctx_inst, int nthreads) {
+  int nt = o_threads(nthreads);
+  (void)nt;
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nt)
+  for (long i = 0; i < (long)nrays; i++) {
+    o_rayhit* r = (o_rayhit*)((char*)rays + (size_t)i * stride);
+    if (!(r->tnear <= r->tfar)) continue;
+    for (unsigned k = 0; k < ninsts; k++) {
+      const oracle_instance* in = &insts[k];
+      if (!in->identity && ctx_inst != 0xffffffffu) continue;
+      float o[3], d[3];
+      o_inst_ray(in, r->org, r->dir, o, d);
+      if (o_object_any(&objects[in->object], o, d, r->tnear, r->tfar)) {
+        r->tfar = -INFINITY;
+        break;
+      }
+    }
+  }
+}
+
 /* ------------------------------------------------------ binned SAH build -- */
 
 typedef struct {
@@ -1006,5 +1157,70 @@ void oracle_morton_codes(const float* lo, const float* hi, unsigned n, unsigned*
       q[k] = (unsigned)(int)((c2 - cb.lo[k]) * scale[k]);
     }
     codes[i] = oracle_bit_interleave(q[0], q[1], q[2]);
+  }
+}
+
+/* Instances over per-object BVHs: same result as the brute-force instance
+ * functions (each object BVH's result equals its brute force), for scenes too
+ * large to brute-force. objects[k] is the BVH of object k and robust[k]
+ * selects its triangle test. */
+void oracle_intersect_instances_bvh(const oracle_bvh* const* objects, const int* robust,
+                                    const oracle_instance* insts, unsigned ninsts, void* rayhits, unsigned nrays,
+                                    size_t stride, unsigned ctx_inst, int nthreads) {
+  int nt = o_threads(nthreads);
+  (void)nt;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
+  for (long i = 0; i < (long)nrays; i++) {
+    o_rayhit* r = (o_rayhit*)((char*)rayhits + (size_t)i * stride);
+    if (!(r->tnear <= r->tfar)) continue;
+    o_best best;
+    memset(&best, 0, sizeof(best));
+    unsigned best_inst = 0xffffffffu;
+    for (unsigned k = 0; k < ninsts; k++) {
+      const oracle_instance* in = &insts[k];
+      if (!in->identity && ctx_inst != 0xffffffffu) continue;
+      const unsigned id = in->identity ? ctx_inst : in->instID;
+      o_rayhit q = *r;
+      o_inst_ray(in, r->org, r->dir, q.org, q.dir);
+      q.geomID = 0xffffffffu;
+      o_trace_one(objects[in->object], &q, 0, id, NULL, NULL, robust[in->object]);
+      if (q.geomID == 0xffffffffu) continue;
+      o_best b;
+      b.found = 1;
+      b.t = q.tfar;
+      b.u = q.u;
+      b.v = q.v;
+      b.Ng[0] = q.Ng[0]; b.Ng[1] = q.Ng[1]; b.Ng[2] = q.Ng[2];
+      b.geomID = q.geomID;
+      b.primID = q.primID;
+      if (o_better_inst(&b, id, &best, best_inst)) {
+        best = b;
+        best_inst = id;
+      }
+    }
+    if (best.found) o_write_hit(r, &best, best_inst);
+  }
+}
+
+void oracle_occluded_instances_bvh(const oracle_bvh* const* objects, const int* robust,
+                                   const oracle_instance* insts, unsigned ninsts, void* rays, unsigned nrays,
+                                   size_t stride, unsigned ctx_inst, int nthreads) {
+  int nt = o_threads(nthreads);
+  (void)nt;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
+  for (long i = 0; i < (long)nrays; i++) {
+    o_rayhit* r = (o_rayhit*)((char*)rays + (size_t)i * stride);
+    if (!(r->tnear <= r->tfar)) continue;
+    for (unsigned k = 0; k < ninsts; k++) {
+      const oracle_instance* in = &insts[k];
+      if (!in->identity && ctx_inst != 0xffffffffu) continue;
+      o_rayhit q = *r;
+      o_inst_ray(in, r->org, r->dir, q.org, q.dir);
+      o_trace_one(objects[in->object], &q, 1, 0, NULL, NULL, robust[in->object]);
+      if (q.tfar == -INFINITY) {
+        r->tfar = -INFINITY;
+        break;
+      }
+    }
   }
 }

@@ -106,6 +106,50 @@ void oracle_intersect_brute_robust(const oracle_mesh* meshes, unsigned nmeshes, 
 void oracle_occluded_brute_robust(const oracle_mesh* meshes, unsigned nmeshes, void* rays,
                                   unsigned nrays, size_t stride, int nthreads);
 
+/* ---- instances (two-level scenes) ----
+ * world2local = rcp(local2world) as Instance::commit computes it
+ * (kernels/common/scene_instance.cpp commit, common/math/affinespace.h:83):
+ * il = adjoint/det (common/math/linearspace3.h:44-53 with the column-vector
+ * cross of common/math/vec3fa.h:358-365 and the SSE4.1 dpps dot of
+ * vec3fa.h:349-351, i.e. (x0*y0 + x1*y1) + x2*y2), p' = -(il * p)
+ * (linearspace3.h:146). Matrices are 12 floats, columns vx | vy | vz | p. */
+void oracle_affine_inverse(const float l2w[12], float w2l[12]);
+
+/* xfmPoint / xfmVector (common/math/affinespace.h:102-103,
+ * common/math/linearspace3.h:156): fma(x, vx, fma(y, vy, fma(z, vz, p))) and
+ * fma(x, vx, fma(y, vy, z * vz)) per component. */
+void oracle_xfm_point(const float m[12], const float p[3], float out[3]);
+void oracle_xfm_vector(const float m[12], const float v[3], float out[3]);
+
+/* An instanced object: the triangle meshes of a committed scene (geomID =
+ * index in meshes) and whether that scene is robust (Pluecker test). */
+typedef struct {
+  const oracle_mesh* meshes;
+  unsigned nmeshes;
+  int robust;
+} oracle_object;
+
+/* One instance: its world2local transform, object and instID. identity = 1
+ * marks the parent scene's own triangles (no transform; hits report the
+ * context's instID[0]). */
+typedef struct {
+  float w2l[12];
+  unsigned object;
+  unsigned instID;
+  int identity;
+} oracle_instance;
+
+/* Brute-force closest hit over every instance: the ray moves into each
+ * instance's object space (kernels/geometry/instance_intersector.cpp:52-71),
+ * keeps [tnear, tfar], and the hit is the lexicographic minimum of
+ * (t, instID, geomID, primID); Ng stays in object space. When ctx_inst is not
+ * RTC_INVALID_GEOMETRY_ID (0xffffffff) the one-level instance stack is full
+ * and only identity instances are tested (kernels/common/instance_stack.h:21-38). */
+void oracle_intersect_instances(const oracle_object* objects, const oracle_instance* insts, unsigned ninsts,
+                                void* rayhits, unsigned nrays, size_t stride, unsigned ctx_inst, int nthreads);
+void oracle_occluded_instances(const oracle_object* objects, const oracle_instance* insts, unsigned ninsts,
+                               void* rays, unsigned nrays, size_t stride, unsigned ctx_inst, int nthreads);
+
 typedef struct oracle_bvh oracle_bvh;
 
 /* Builds the reference-style binned-SAH BVH8 over all valid triangles. The
@@ -154,6 +198,16 @@ float oracle_half_area(const float lo[3], const float hi[3]);
  * tests as the plain-SAH judge on single nodes. */
 int oracle_binned_split(const float* lo, const float* hi, unsigned n, unsigned* pos, float* sah,
                         unsigned* num_bins);
+
+/* Instances over per-object oracle BVHs (objects[k] built over object k's
+ * meshes): the same results as oracle_intersect_instances /
+ * oracle_occluded_instances, fast enough for large scenes. */
+void oracle_intersect_instances_bvh(const oracle_bvh* const* objects, const int* robust,
+                                    const oracle_instance* insts, unsigned ninsts, void* rayhits, unsigned nrays,
+                                    size_t stride, unsigned ctx_inst, int nthreads);
+void oracle_occluded_instances_bvh(const oracle_bvh* const* objects, const int* robust,
+                                   const oracle_instance* insts, unsigned ninsts, void* rays, unsigned nrays,
+                                   size_t stride, unsigned ctx_inst, int nthreads);
 
 #ifdef __cplusplus
 }

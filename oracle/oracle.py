@@ -17,6 +17,15 @@ class OracleMesh(ctypes.Structure):
                 ("indices", ctypes.c_void_p), ("num_triangles", ctypes.c_uint)]
 
 
+class OracleObject(ctypes.Structure):
+    _fields_ = [("meshes", ctypes.c_void_p), ("nmeshes", ctypes.c_uint), ("robust", ctypes.c_int)]
+
+
+class OracleInstance(ctypes.Structure):
+    _fields_ = [("w2l", ctypes.c_float * 12), ("object", ctypes.c_uint), ("instID", ctypes.c_uint),
+                ("identity", ctypes.c_int)]
+
+
 _lib = None
 
 
@@ -57,6 +66,13 @@ def lib():
         L.oracle_binned_split.argtypes = [vp, vp, u32, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_float),
                                           ctypes.POINTER(u32)]
         L.oracle_binned_split.restype = i32
+        L.oracle_affine_inverse.argtypes = [fp, fp]
+        L.oracle_xfm_point.argtypes = [fp, fp, fp]
+        L.oracle_xfm_vector.argtypes = [fp, fp, fp]
+        L.oracle_intersect_instances.argtypes = [vp, vp, u32, vp, u32, sz, u32, i32]
+        L.oracle_occluded_instances.argtypes = [vp, vp, u32, vp, u32, sz, u32, i32]
+        L.oracle_intersect_instances_bvh.argtypes = [vp, vp, vp, u32, vp, u32, sz, u32, i32]
+        L.oracle_occluded_instances_bvh.argtypes = [vp, vp, vp, u32, vp, u32, sz, u32, i32]
         _lib = L
     return _lib
 
@@ -184,3 +200,103 @@ def binned_split(lo, hi):
     dim = lib().oracle_binned_split(lo.ctypes.data, hi.ctypes.data, lo.shape[0], ctypes.byref(pos),
                                     ctypes.byref(sah), ctypes.byref(nb))
     return dim, pos.value, sah.value, nb.value
+
+
+def affine_inverse(l2w):
+    """world2local of a local-to-world transform (12 floats, columns vx|vy|vz|p)."""
+    a, pa = _f3(np.asarray(l2w, np.float32).reshape(12))
+    out = np.zeros(12, np.float32)
+    lib().oracle_affine_inverse(pa, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return out
+
+
+def xfm_point(m, p):
+    a, pa = _f3(np.asarray(m, np.float32).reshape(12)); b, pb = _f3(p)
+    out = np.zeros(3, np.float32)
+    lib().oracle_xfm_point(pa, pb, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return out
+
+
+def xfm_vector(m, v):
+    a, pa = _f3(np.asarray(m, np.float32).reshape(12)); b, pb = _f3(v)
+    out = np.zeros(3, np.float32)
+    lib().oracle_xfm_vector(pa, pb, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)))
+    return out
+
+
+class Instances:
+    """Two-level scene for the instance restatement: objects are lists of
+    (vertices, indices) meshes (geomID = list index) with a robust flag;
+    instances are (object, l2w, instID) or the scene's own triangles as an
+    identity instance."""
+
+    def __init__(self):
+        self.objects, self.insts, self._keep = [], [], []
+        self._bvhs = None
+
+    def __del__(self):
+        if self._bvhs is not None and lib is not None:
+            for b in self._bvhs:
+                lib().oracle_bvh_free(b)
+
+    def add_object(self, meshes, robust=False):
+        ms = (OracleMesh * max(1, len(meshes)))()
+        for k, (v, ix) in enumerate(meshes):
+            v = np.ascontiguousarray(v, np.float32)
+            ix = np.ascontiguousarray(ix, np.uint32)
+            self._keep += [v, ix]
+            ms[k] = OracleMesh(v.ctypes.data, len(v), ix.ctypes.data, len(ix))
+        self._keep.append(ms)
+        self.objects.append(OracleObject(ctypes.cast(ms, ctypes.c_void_p), len(meshes), int(robust)))
+        return len(self.objects) - 1
+
+    def add_instance(self, obj, l2w, inst_id):
+        i = OracleInstance()
+        i.w2l[:] = list(affine_inverse(l2w))
+        i.object, i.instID, i.identity = obj, inst_id, 0
+        self.insts.append(i)
+
+    def add_identity(self, obj):
+        i = OracleInstance()
+        i.w2l[:] = [1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0]
+        i.object, i.instID, i.identity = obj, 0xFFFFFFFF, 1
+        self.insts.append(i)
+
+    def _arrays(self):
+        objs = (OracleObject * len(self.objects))(*self.objects)
+        insts = (OracleInstance * max(1, len(self.insts)))(*self.insts)
+        return objs, insts
+
+    def intersect(self, rays, ctx_inst=0xFFFFFFFF, nthreads=0):
+        objs, insts = self._arrays()
+        lib().oracle_intersect_instances(ctypes.cast(objs, ctypes.c_void_p), ctypes.cast(insts, ctypes.c_void_p),
+                                         len(self.insts), rays.ctypes.data, len(rays), rays.itemsize, ctx_inst,
+                                         nthreads)
+
+    def occluded(self, rays, ctx_inst=0xFFFFFFFF, nthreads=0):
+        objs, insts = self._arrays()
+        lib().oracle_occluded_instances(ctypes.cast(objs, ctypes.c_void_p), ctypes.cast(insts, ctypes.c_void_p),
+                                        len(self.insts), rays.ctypes.data, len(rays), rays.itemsize, ctx_inst,
+                                        nthreads)
+
+    def _bvh_arrays(self):
+        if self._bvhs is None:
+            self._bvhs = [lib().oracle_build_sah(o.meshes, o.nmeshes) for o in self.objects]
+        bv = (ctypes.c_void_p * len(self._bvhs))(*self._bvhs)
+        rb = (ctypes.c_int * len(self.objects))(*[o.robust for o in self.objects])
+        return bv, rb
+
+    def intersect_bvh(self, rays, ctx_inst=0xFFFFFFFF, nthreads=0):
+        """As intersect(), over per-object oracle BVHs (large scenes)."""
+        bv, rb = self._bvh_arrays()
+        _, insts = self._arrays()
+        lib().oracle_intersect_instances_bvh(ctypes.cast(bv, ctypes.c_void_p), ctypes.cast(rb, ctypes.c_void_p),
+                                             ctypes.cast(insts, ctypes.c_void_p), len(self.insts), rays.ctypes.data,
+                                             len(rays), rays.itemsize, ctx_inst, nthreads)
+
+    def occluded_bvh(self, rays, ctx_inst=0xFFFFFFFF, nthreads=0):
+        bv, rb = self._bvh_arrays()
+        _, insts = self._arrays()
+        lib().oracle_occluded_instances_bvh(ctypes.cast(bv, ctypes.c_void_p), ctypes.cast(rb, ctypes.c_void_p),
+                                            ctypes.cast(insts, ctypes.c_void_p), len(self.insts), rays.ctypes.data,
+                                            len(rays), rays.itemsize, ctx_inst, nthreads)

@@ -446,3 +446,130 @@ def test_robust_bvh_equals_brute_force(oracle, occluded):
         bvh.intersect(a, robust=True)
         oracle.intersect_brute([(v, ix)], b, robust=True)
     assert records_equal(a, b)
+
+
+# -------------------------------------------------------------- instances --
+# The instance restatement is pinned by closed forms of the affine inverse,
+# by W*L = I on random transforms (a transposed adjoint, a dropped cofactor
+# or a sign slip breaks it), by translation invariance on an exact grid
+# (an instance translated by t must hit exactly what the mesh moved by t
+# hits), and by the per-object BVH path agreeing with brute force.
+
+def random_l2w(n, seed):
+    rng = np.random.default_rng(seed)
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([np.stack([1 - 2 * (y * y + z * z), 2 * (x * y + z * w), 2 * (x * z - y * w)], -1),
+                  np.stack([2 * (x * y - z * w), 1 - 2 * (x * x + z * z), 2 * (y * z + x * w)], -1),
+                  np.stack([2 * (x * z + y * w), 2 * (y * z - x * w), 1 - 2 * (x * x + y * y)], -1)], 1)
+    s = rng.uniform(0.5, 2.0, (n, 3))  # non-uniform scale: the inverse is not a transpose
+    out = np.empty((n, 12), np.float32)
+    out[:, :9] = (R * s[:, :, None]).reshape(n, 9)  # columns vx, vy, vz
+    out[:, 9:] = rng.uniform(-50, 50, (n, 3))
+    return out
+
+
+def test_affine_inverse_closed_forms(oracle):
+    m = np.array([2, 0, 0, 0, 0.25, 0, 0, 0, 8, 1.5, -3, 64], np.float32)
+    assert oracle.affine_inverse(m).tolist() == [0.5, 0, 0, 0, 4, 0, 0, 0, 0.125, -0.75, 12, -8]
+    # a cyclic axis permutation: x -> y, y -> z, z -> x, then translate
+    p = np.array([0, 1, 0, 0, 0, 1, 1, 0, 0, 5, 6, 7], np.float32)
+    w = oracle.affine_inverse(p)
+    assert w.tolist() == [0, 0, 1, 1, 0, 0, 0, 1, 0, -6, -7, -5]
+    for pt in ([1, 2, 3], [-0.5, 0.25, 10]):
+        assert oracle.xfm_point(w, oracle.xfm_point(p, pt)).tolist() == pt
+
+
+def test_affine_inverse_random_transforms(oracle):
+    for m in random_l2w(64, 1):
+        w = oracle.affine_inverse(m)
+        L = m[:9].reshape(3, 3).T.astype(np.float64)   # columns -> matrix
+        W = w[:9].reshape(3, 3).T.astype(np.float64)
+        assert np.abs(W @ L - np.eye(3)).max() < 2e-6
+        assert np.abs(W @ m[9:] + w[9:]).max() < 2e-4
+        p = np.array([0.3, -7.0, 11.0], np.float32)
+        back = oracle.xfm_point(m, oracle.xfm_point(w, p))
+        assert np.abs(back - p).max() < 1e-4
+        v = np.array([1.0, 2.0, -3.0], np.float32)
+        assert np.allclose(oracle.xfm_vector(m, v), L @ v, rtol=1e-6, atol=1e-5)  # no translation
+
+
+def test_instance_translation_invariance(oracle):
+    # vertices and ray origins on a 2^-8 grid in [-4, 4), translation by
+    # powers of two: every subtraction in the Moeller test is exact, so the
+    # instance must report bit-identical t, u, v, Ng to the moved mesh
+    rng = np.random.default_rng(3)
+    n = 400
+    v = np.round(rng.uniform(-2, 2, (3 * n, 3)) * 256) / 256
+    v = v.astype(np.float32)
+    ix = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    t = np.array([8.0, -16.0, 4.0], np.float32)
+    org = (np.round(rng.uniform(-4, 4, (3000, 3)) * 256) / 256).astype(np.float32)
+    tgt = v[rng.integers(0, 3 * n, 3000)] + rng.uniform(-0.05, 0.05, (3000, 3)).astype(np.float32)
+    r = rays(org + t, tgt - org)
+    I = oracle.Instances()
+    I.add_instance(I.add_object([(v, ix)]), np.concatenate([np.eye(3, dtype=np.float32).reshape(-1), t]), 5)
+    a = r.copy()
+    I.intersect(a)
+    b = r.copy()
+    oracle.intersect_brute([(v + t, ix)], b)
+    hit = b["geomID"] != INV
+    assert hit.mean() > 0.3
+    assert (a["geomID"] != INV).tolist() == hit.tolist()
+    for k in ("tfar", "u", "v", "Ng_x", "Ng_y", "Ng_z", "primID"):
+        assert a[k].tobytes() == b[k].tobytes(), k
+    assert (a["instID"][hit] == 5).all()
+
+
+def test_instance_order_and_identity(oracle):
+    v, ix = soup(300, 21, h=0.05)
+    I = oracle.Instances()
+    ob = I.add_object([(v, ix)])
+    own = I.add_object([soup(200, 22, h=0.05)])
+    I.add_identity(own)
+    m = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], np.float32)
+    # two instances of the same object at the same place: equal t everywhere,
+    # the smaller instID wins
+    I.add_instance(ob, m, 9)
+    I.add_instance(ob, m, 4)
+    r = random_rays(3000, 23, -0.1, 1.1)
+    a = r.copy()
+    I.intersect(a)
+    b = r.copy()
+    oracle.intersect_brute([(v, ix)], b, inst_id=4)
+    c = r.copy()
+    oracle.intersect_brute([soup(200, 22, h=0.05)], c)
+    for i in range(len(r)):
+        cands = [(x["tfar"], x["instID"]) for x in (b[i], c[i]) if x["geomID"] != INV]
+        if not cands:
+            assert a["geomID"][i] == INV
+            continue
+        assert (a["tfar"][i], a["instID"][i]) == min(cands)
+    # a full instance stack keeps only the identity record
+    d = r.copy()
+    I.intersect(d, ctx_inst=3)
+    c3 = r.copy()
+    oracle.intersect_brute([soup(200, 22, h=0.05)], c3, inst_id=3)
+    assert records_equal(d, c3)
+
+
+def test_instance_bvh_equals_brute_force(oracle):
+    I = oracle.Instances()
+    objs = [I.add_object([soup(400, 30 + k, h=0.05)], robust=(k == 1)) for k in range(3)]
+    for k, m in enumerate(random_l2w(25, 7)):
+        m = m.copy()
+        m[9:] *= 0.02
+        I.add_instance(objs[k % 3], m, k)
+    r = random_rays(2000, 31, -1.5, 1.5)
+    a, b = r.copy(), r.copy()
+    I.intersect(a)
+    I.intersect_bvh(b)
+    assert records_equal(a, b)
+    assert (a["geomID"] != INV).mean() > 0.2
+    r["tfar"] = 0.5
+    a, b = r.copy(), r.copy()
+    I.occluded(a)
+    I.occluded_bvh(b)
+    assert records_equal(a, b)
+    assert 0 < np.isneginf(a["tfar"]).sum() < len(r)
