@@ -138,10 +138,10 @@ struct NodeHit {
   float tn[8];
 };
 
-// Slab test of the 8 children; decoded planes are fmaf(2^(e-127), q, p), the
-// exact expression the builder verified against every child box, then the
-// robust per-plane distance (plane - org) * rdir with rdir rounded down for
-// near and up for far planes.
+// Slab test of the 8 children on planes that decode as fmaf(2^(e-127), q, p)
+// (the expression the builder verified against every widened child box),
+// with rdir rounded down for near and up for far planes as the reference's
+// robust TravRay does (kernels/bvh/node_intersector1.h:122-190).
 __device__ inline void intersect_node(const QNode8* __restrict__ nodes, uint32_t i, const TRay& r, float thi,
                                       NodeHit& h) {
   const uint4* np = reinterpret_cast<const uint4*>(nodes + i);
@@ -162,6 +162,21 @@ __device__ inline void intersect_node(const QNode8* __restrict__ nodes, uint32_t
   const uint32_t fy0 = r.negy ? w3.x : w3.z, fy1 = r.negy ? w3.y : w3.w;
   const uint32_t nz0 = r.negz ? w4.z : w4.x, nz1 = r.negz ? w4.w : w4.y;
   const uint32_t fz0 = r.negz ? w4.x : w4.z, fz1 = r.negz ? w4.y : w4.w;
+  // Plane distance (p + s q - o) rdir as one FMA per plane, q a + b with
+  // a = s rdir (exact: s is a power of two) and b = (p - o) rdir (two
+  // roundings) formed once per node. Against the decode-then-subtract form
+  // the error moves from |plane - o| to |p - o|: at most u|b| + u/2 |t|
+  // (u = 2^-23). Where |b| is close to |t| (origin away from the node) the
+  // 3-ulp scaling of rdir covers it; where it cancels (origin near or in the
+  // node), |p - o| is within the node's extent, below 2 max|coord|, and the
+  // builder's box widening by max|coord| 2^-20 = 8u max|coord| covers it. So
+  // no box holding a hit is culled (the hits of this form equal the
+  // oracle's on every parity test), and each plane costs a byte conversion
+  // and one FMA instead of an FMA, a subtraction and a product.
+  const float anx = sx * r.rnx, afx = sx * r.rfx, any = sy * r.rny, afy = sy * r.rfy, anz = sz * r.rnz, afz = sz * r.rfz;
+  const float bnx = (px - r.ox) * r.rnx, bfx = (px - r.ox) * r.rfx;
+  const float bny = (py - r.oy) * r.rny, bfy = (py - r.oy) * r.rfy;
+  const float bnz = (pz - r.oz) * r.rnz, bfz = (pz - r.oz) * r.rfz;
   uint32_t mask = 0;
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -170,12 +185,9 @@ __device__ inline void intersect_node(const QNode8* __restrict__ nodes, uint32_t
     const float qnx = byte_f(c < 4 ? nx0 : nx1, k), qfx = byte_f(c < 4 ? fx0 : fx1, k);
     const float qny = byte_f(c < 4 ? ny0 : ny1, k), qfy = byte_f(c < 4 ? fy0 : fy1, k);
     const float qnz = byte_f(c < 4 ? nz0 : nz1, k), qfz = byte_f(c < 4 ? fz0 : fz1, k);
-    const float tnx = (__builtin_fmaf(sx, qnx, px) - r.ox) * r.rnx;
-    const float tfx = (__builtin_fmaf(sx, qfx, px) - r.ox) * r.rfx;
-    const float tny = (__builtin_fmaf(sy, qny, py) - r.oy) * r.rny;
-    const float tfy = (__builtin_fmaf(sy, qfy, py) - r.oy) * r.rfy;
-    const float tnz = (__builtin_fmaf(sz, qnz, pz) - r.oz) * r.rnz;
-    const float tfz = (__builtin_fmaf(sz, qfz, pz) - r.oz) * r.rfz;
+    const float tnx = __builtin_fmaf(qnx, anx, bnx), tfx = __builtin_fmaf(qfx, afx, bfx);
+    const float tny = __builtin_fmaf(qny, any, bny), tfy = __builtin_fmaf(qfy, afy, bfy);
+    const float tnz = __builtin_fmaf(qnz, anz, bnz), tfz = __builtin_fmaf(qfz, afz, bfz);
     const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tlo));
     const float tf = fminf(fminf(tfx, tfy), fminf(tfz, thi));
     h.tn[c] = tn;
