@@ -371,8 +371,20 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
       atomicOr(err, 1u);
       return;
     }
-    const uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    // Hand-off to the sibling's thread: this thread's stores of the child it
+    // finished in the previous iteration (box, cost table, split bytes, leaf
+    // flag) must be visible, across XCDs, before the counter moves. The
+    // compiler's release sequence for an acq_rel atomic here lacked the
+    // vmcnt(0) waits on the loop back-edge (present in straight-line code),
+    // so the sibling's thread could read those stores stale and identical
+    // builds now and then gave different trees. Spelled out: release fence
+    // (L2 write-back), wait for the stores and the write-back, relaxed
+    // increment, acquire fence (wait, then L2 invalidate) before reading.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_s_waitcnt(0);
+    const uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     float4 bl[2], bh[2];
     float cc[2][8];
 #pragma unroll

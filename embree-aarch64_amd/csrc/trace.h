@@ -12,12 +12,13 @@ struct TraceParams {
   int sort_rays = 0;        // trace in origin-Morton order (a permutation; records stay in place)
   uint32_t sort_min = 65536;  // smallest stream that is reordered
   int persistent = 1;       // persistent waves pulling rays from a counter
-  // Idle lanes of a wave that trigger a refill. A refill stalls the wave for
-  // one ray-record load, so it pays to wait for a larger group; on the
-  // 2^20-triangle soup with 2^24 incoherent rays 24 gave 1655 Mrays/s closest
-  // hit (16: 1303, 32: 1587) and 56 gave 3372 Mrays/s occlusion (32: 2521).
-  int refill_min = 24;      // closest hit
-  int refill_min_occ = 56;  // occlusion rays, which finish sooner
+  // Idle lanes of a wave that trigger a refill. A refill stalls the wave
+  // for one ray-record load (and, once per 256 rays, the global counter
+  // atomic); on the 2^20-triangle soup with 2^24 incoherent rays 10 gave
+  // 1936 Mrays/s closest hit (8-14 within 1 %, 24: 1854, 32: 1818) and 24
+  // gave 3795 Mrays/s occlusion (48: 3339).
+  int refill_min = 10;      // closest hit
+  int refill_min_occ = 24;  // occlusion rays, which finish sooner
   int cus = 256;            // compute units (set from the device)
   int blocks_per_cu = 8;    // resident 256-thread blocks per compute unit for the persistent grid
 };

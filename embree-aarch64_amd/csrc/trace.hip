@@ -92,6 +92,45 @@ struct RingStack<0> {
   __device__ inline bool empty() const { return sp == 0; }
 };
 
+// Rays of a persistent wave come from a private run of consecutive slots
+// that the wave takes from the global counter CHUNK at a time, so a refill
+// costs a global atomic only once every CHUNK rays (the atomic's latency
+// stalls the whole wave). Every lane computes the same values (all inputs
+// are wave-uniform), so the state needs no broadcast after the one atomic.
+struct WavePool {
+  static constexpr uint32_t CHUNK = 256;
+  uint32_t lb = 0, le = 0;  // unassigned slots [lb, le) of the current run
+  bool gdone = false;       // the global counter has passed M
+  // Hands out nidle slots (idle lane `rank` gets the rank-th); returns this
+  // lane's slot or 0xffffffff when the rays have run out.
+  __device__ inline uint32_t take(uint32_t* next, uint32_t M, int nidle, uint32_t rank, int lane) {
+    const uint32_t have = le - lb;
+    uint32_t nb = 0, ne = 0;
+    if (have < (uint32_t)nidle && !gdone) {
+      uint32_t g = 0;
+      if (lane == 0) g = atomicAdd(next, CHUNK);
+      g = __shfl(g, 0, 64);
+      if ((uint64_t)g + CHUNK >= M) gdone = true;
+      if (g < M) {
+        nb = g;
+        ne = (uint64_t)g + CHUNK < M ? g + CHUNK : M;
+      }
+    }
+    uint32_t slot = 0xffffffffu;
+    if (rank < have) slot = lb + rank;
+    else if (nb + (rank - have) < ne) slot = nb + (rank - have);
+    if (have >= (uint32_t)nidle) {
+      lb += (uint32_t)nidle;
+    } else {
+      const uint32_t need = (uint32_t)nidle - have;
+      lb = nb + (need < ne - nb ? need : ne - nb);
+      le = ne;
+    }
+    return slot;
+  }
+  __device__ inline bool empty() const { return gdone && lb >= le; }
+};
+
 __device__ inline float widen_up(float x) { return __builtin_isinf(x) ? x : x + fabsf(x) * CULL_MARGIN; }
 __device__ inline float widen_down(float x) { return __builtin_isinf(x) ? x : x - fabsf(x) * CULL_MARGIN; }
 
@@ -199,6 +238,68 @@ __device__ inline void intersect_node(const QNode8* __restrict__ nodes, uint32_t
 __device__ inline uint32_t child_ref(const NodeHit& h, int c) {
   const uint32_t mb = ((c < 4 ? h.meta_lo : h.meta_hi) >> (8 * (c & 3))) & 0x7fu;
   return ((h.imask >> c) & 1u) ? (h.child_base + mb) : ((h.tri_base + mb) | REF_LEAF);
+}
+
+// Closest hit: pushes the hit children except the nearest, farthest first,
+// each with its entry distance, and returns the nearest child.
+// Keys are the entry distances mapped to unsigned order with the slot in the
+// low 3 bits (a distance perturbed by at most 7 ulps, which changes only the
+// order among nearly equal children); the stored distance is the key with
+// those bits cleared, mapped back, which is never above the true entry
+// distance, so the pop test stays conservative. A 19-comparator network
+// (Batcher's odd-even merge sort) orders the keys without branches; hit-free
+// slots carry key 0 and sort last.
+template <class Stack>
+__device__ inline uint32_t push_near_to_far(const NodeHit& h, Stack& stack, uint32_t* err) {
+#ifdef RTG_SELECT_SORT
+  int bi = -1;
+  float bv = 0.0f;
+#pragma unroll
+  for (int c = 0; c < 8; c++)
+    if (((h.mask >> c) & 1u) && (bi < 0 || h.tn[c] < bv)) { bi = c; bv = h.tn[c]; }
+  uint32_t rem = h.mask & ~(1u << bi);
+  while (rem) {
+    int mc = -1;
+    float mv = 0.0f;
+#pragma unroll
+    for (int c = 0; c < 8; c++)
+      if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
+    rem &= ~(1u << mc);
+    if (!stack.push(make_uint2(child_ref(h, mc), __float_as_uint(mv)))) atomicOr(err, 1u);
+  }
+  return child_ref(h, bi);
+#else
+  uint32_t k[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) {
+    const uint32_t b = __float_as_uint(h.tn[c]);
+    const uint32_t o = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    k[c] = ((h.mask >> c) & 1u) ? ((o & ~7u) | (uint32_t)c) : 0u;
+  }
+  auto ce = [&](int i, int j) {
+    const uint32_t hi = max(k[i], k[j]), lo = min(k[i], k[j]);
+    k[i] = hi;
+    k[j] = lo;
+  };
+  ce(0, 1); ce(2, 3); ce(4, 5); ce(6, 7);
+  ce(0, 2); ce(1, 3); ce(4, 6); ce(5, 7);
+  ce(1, 2); ce(5, 6);
+  ce(0, 4); ce(1, 5); ce(2, 6); ce(3, 7);
+  ce(2, 4); ce(3, 5);
+  ce(1, 2); ce(3, 4); ce(5, 6);
+  const int n = __popc(h.mask);
+  uint32_t near = k[0];
+#pragma unroll
+  for (int j = 0; j < 7; j++) {
+    if (j + 1 < n) {
+      const uint32_t o = k[j] & ~7u;
+      const uint32_t d = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+      if (!stack.push(make_uint2(child_ref(h, (int)(k[j] & 7u)), d))) atomicOr(err, 1u);
+      near = k[j + 1];
+    }
+  }
+  return child_ref(h, (int)(near & 7u));
+#endif
 }
 
 struct Hit {
@@ -376,22 +477,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
           cur = child_ref(h, c0);
           continue;
         } else {
-          int bi = -1;
-          float bv = 0.0f;
-#pragma unroll
-          for (int c = 0; c < 8; c++)
-            if (((h.mask >> c) & 1u) && (bi < 0 || h.tn[c] < bv)) { bi = c; bv = h.tn[c]; }
-          uint32_t rem = h.mask & ~(1u << bi);
-          while (rem) {
-            int mc = -1;
-            float mv = 0.0f;
-#pragma unroll
-            for (int c = 0; c < 8; c++)
-              if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
-            rem &= ~(1u << mc);
-            if (!stack.push(make_uint2(child_ref(h, mc), __float_as_uint(mv)))) atomicOr(err, 1u);
-          }
-          cur = child_ref(h, bi);
+          cur = push_near_to_far(h, stack, err);
           continue;
         }
       }
@@ -499,6 +585,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
   char* lrp = nullptr;      // BATCH: this lane's record
   uint32_t lroot = 0;       // BATCH: its scene's root
   bool active = false, occluded = false, pool_empty = false;
+  WavePool pool;
   // a ray touches each node and triangle record at most once
   const uint64_t max_iter = (uint64_t)sc.num_nodes + sc.num_tris + 16;
   uint64_t iter = 0;
@@ -509,13 +596,10 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
       const uint64_t idle = __ballot(!active);
       const int nidle = __popcll(idle);
       if (nidle >= refill_min) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(next, (uint32_t)nidle);
-        base = __shfl(base, 0, 64);
-        if ((uint64_t)base + nidle >= M) pool_empty = true;
+        const uint32_t slot = pool.take(next, M, nidle, (uint32_t)__popcll(idle & lt), lane);
+        pool_empty = pool.empty();
         if (!active) {
-          const uint32_t slot = base + (uint32_t)__popcll(idle & lt);
-          if (slot < M) {
+          if (slot != 0xffffffffu) {
             char* src;
             if (BATCH) {
               uint32_t lo = 0, hi = nitems - 1;
@@ -589,22 +673,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
             }
             cur = child_ref(h, c0);
           } else {
-            int bi = -1;
-            float bv = 0.0f;
-#pragma unroll
-            for (int c = 0; c < 8; c++)
-              if (((h.mask >> c) & 1u) && (bi < 0 || h.tn[c] < bv)) { bi = c; bv = h.tn[c]; }
-            uint32_t rem = h.mask & ~(1u << bi);
-            while (rem) {
-              int mc = -1;
-              float mv = 0.0f;
-#pragma unroll
-              for (int c = 0; c < 8; c++)
-                if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
-              rem &= ~(1u << mc);
-              if (!stack.push(make_uint2(child_ref(h, mc), __float_as_uint(mv)))) atomicOr(err, 1u);
-            }
-            cur = child_ref(h, bi);
+            cur = push_near_to_far(h, stack, err);
           }
           continue;  // next step descends into cur
         }
@@ -718,6 +787,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
   uint32_t onn = 0, otn = 0;
   bool orobust = false;
   bool active = false, occluded = false, pool_empty = false;
+  WavePool pool;
   uint64_t iter = 0;
   const float neg_inf = -INFINITY;
 
@@ -727,13 +797,10 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
       const uint64_t idle = __ballot(!active);
       const int nidle = __popcll(idle);
       if (nidle >= refill_min) {
-        uint32_t base = 0;
-        if (lane == 0) base = atomicAdd(next, (uint32_t)nidle);
-        base = __shfl(base, 0, 64);
-        if ((uint64_t)base + nidle >= M) pool_empty = true;
+        const uint32_t slot = pool.take(next, M, nidle, (uint32_t)__popcll(idle & lt), lane);
+        pool_empty = pool.empty();
         if (!active) {
-          const uint32_t slot = base + (uint32_t)__popcll(idle & lt);
-          if (slot < M) {
+          if (slot != 0xffffffffu) {
             ray = slot;
             float4 a, b;
             float tfar;
@@ -789,22 +856,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
             }
             cur = child_ref(h, c0);
           } else {
-            int bi = -1;
-            float bv = 0.0f;
-#pragma unroll
-            for (int c = 0; c < 8; c++)
-              if (((h.mask >> c) & 1u) && (bi < 0 || h.tn[c] < bv)) { bi = c; bv = h.tn[c]; }
-            uint32_t rem = h.mask & ~(1u << bi);
-            while (rem) {
-              int mc = -1;
-              float mv = 0.0f;
-#pragma unroll
-              for (int c = 0; c < 8; c++)
-                if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
-              rem &= ~(1u << mc);
-              if (!stack.push(make_uint2(child_ref(h, mc), __float_as_uint(mv)))) atomicOr(err, 1u);
-            }
-            cur = child_ref(h, bi);
+            cur = push_near_to_far(h, stack, err);
           }
           continue;
         }
