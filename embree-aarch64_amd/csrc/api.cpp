@@ -169,6 +169,43 @@ struct Geometry : RefObj {
   }
 };
 
+// ------------------------------------------------------------ user BVH --
+
+// Memory handed to the caller's node and leaf callbacks: bump allocation in
+// blocks owned by the RTCBVH, released when it is rebuilt or released.
+struct Arena {
+  std::vector<std::unique_ptr<char[]>> blocks;
+  size_t used = 0, cap = 0;
+  void reset() {
+    blocks.clear();
+    used = cap = 0;
+  }
+  void* alloc(size_t bytes, size_t align) {
+    if (align == 0 || (align & (align - 1))) align = 16;
+    size_t off = (used + align - 1) & ~(align - 1);
+    if (blocks.empty() || off + bytes > cap) {
+      const size_t want = std::max<size_t>(bytes + align, 1 << 20);
+      blocks.emplace_back(new char[want]);
+      cap = want;
+      used = 0;
+      const uintptr_t base = (uintptr_t)blocks.back().get();
+      off = (size_t)(((base + align - 1) & ~(uintptr_t)(align - 1)) - base);
+    }
+    char* p = blocks.back().get() + off;
+    used = off + bytes;
+    memset(p, 0, bytes);
+    return p;
+  }
+};
+
+struct BVHUser : RefObj {
+  Device* dev;
+  Arena arena;
+  std::mutex mtx;
+  explicit BVHUser(Device* d) : dev(d) { ref_inc(d); }
+  ~BVHUser() override { ref_dec(dev); }
+};
+
 // ----------------------------------------------------------- split model --
 
 struct SplitModelH : RefObj {
@@ -226,6 +263,7 @@ Device* dev_of(Buffer* b) { return b ? b->dev : nullptr; }
 Device* dev_of(Geometry* g) { return g ? g->dev : nullptr; }
 Device* dev_of(Scene* s) { return s ? s->dev : nullptr; }
 Device* dev_of(SplitModelH* m) { return m ? m->dev : nullptr; }
+Device* dev_of(BVHUser* b) { return b ? b->dev : nullptr; }
 
 #define API_BEGIN try {
 #define API_END(obj)                                                                                 \
@@ -2280,6 +2318,220 @@ RTC_API void rtcOccludedNp(RTCScene scene, RTCIntersectContext* context, const R
   API_BEGIN
   np_call(scene, context, ray, N, true);
   API_END(SC(scene))
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------ user BVH --
+
+namespace {
+
+// Walks a collapsed user tree and calls the caller's callbacks, in the
+// reference binned-SAH builder's order per node: createNode, setNodeBounds,
+// the children, setNodeChildren (kernels/common/rtcore_builder.cpp:169-188).
+struct UserWalk {
+  const rtg::UserTree& T;
+  const RTCBuildArguments& a;
+  RTCBuildPrimitive* prims;  // in tree position order
+  RTCThreadLocalAllocator alloc;
+  size_t done = 0;
+
+  bool inner(uint32_t c) const { return !(c & rtg::REF_LEAF) && !T.isleaf[c]; }
+
+  void box(uint32_t c, RTCBounds& b) const {
+    const float* f = (c & rtg::REF_LEAF) ? &T.pbox[8 * (size_t)(c & ~rtg::REF_LEAF)] : &T.nbox[8 * (size_t)c];
+    b.lower_x = f[0]; b.lower_y = f[1]; b.lower_z = f[2]; b.align0 = 0.0f;
+    b.upper_x = f[4]; b.upper_y = f[5]; b.upper_z = f[6]; b.align1 = 0.0f;
+  }
+
+  void progress(size_t k) {
+    done += k;
+    if (a.buildProgress && !a.buildProgress(a.userPtr, std::min(1.0, (double)done / (double)T.n)))
+      RTC_THROW(RTC_ERROR_CANCELLED, "progress monitor forced termination");
+  }
+
+  void* leaf(uint32_t c) {
+    uint32_t first, count;
+    if (c & rtg::REF_LEAF) {
+      first = c & ~rtg::REF_LEAF;
+      count = 1;
+    } else {
+      first = T.range[2 * (size_t)c];
+      count = T.range[2 * (size_t)c + 1] - first + 1;
+    }
+    void* l = a.createLeaf(alloc, prims + first, count, a.userPtr);
+    progress(count);
+    return l;
+  }
+
+  // the children the collapse chose for inner node r (k_collapse's walk)
+  int children(uint32_t r, uint32_t ch[8]) const {
+    const uint32_t W = T.width;
+    uint32_t st_ref[8], st_s[8];
+    int sp = 0, n = 0;
+    const uint32_t j8 = T.split8[8 * (size_t)r + W - 1] & 7u;
+    st_ref[sp] = T.child[2 * (size_t)r + 1]; st_s[sp++] = W - j8;
+    st_ref[sp] = T.child[2 * (size_t)r];     st_s[sp++] = j8;
+    while (sp > 0 && n < 8) {
+      sp--;
+      const uint32_t c = st_ref[sp];
+      uint32_t slots = st_s[sp];
+      if (!(c & rtg::REF_LEAF)) {
+        while (slots > 1 && (T.split8[8 * (size_t)c + slots - 1] & 0x80u)) slots--;
+      } else {
+        slots = 1;
+      }
+      if (slots <= 1) {
+        ch[n++] = c;
+      } else {
+        if (sp + 2 > 8) RTC_THROW(RTC_ERROR_UNKNOWN, "inconsistent user tree");
+        const uint32_t jj = T.split8[8 * (size_t)c + slots - 1] & 7u;
+        st_ref[sp] = T.child[2 * (size_t)c + 1]; st_s[sp++] = slots - jj;
+        st_ref[sp] = T.child[2 * (size_t)c];     st_s[sp++] = jj;
+      }
+    }
+    return n;
+  }
+
+  void* node(uint32_t r, int depth) {
+    if (!inner(r)) return leaf(r);
+    if (depth > 4096) RTC_THROW(RTC_ERROR_UNKNOWN, "user tree too deep");
+    uint32_t ch[8];
+    const int n = children(r, ch);
+    void* nd = a.createNode(alloc, (unsigned)n, a.userPtr);
+    RTCBounds bb[8];
+    const RTCBounds* bp[8];
+    for (int k = 0; k < n; k++) {
+      box(ch[k], bb[k]);
+      bp[k] = &bb[k];
+    }
+    a.setNodeBounds(nd, bp, (unsigned)n, a.userPtr);
+    void* cp[8];
+    for (int k = 0; k < n; k++) cp[k] = node(ch[k], depth + 1);
+    a.setNodeChildren(nd, cp, (unsigned)n, a.userPtr);
+    return nd;
+  }
+};
+
+}  // namespace
+
+extern "C" {
+
+RTC_API RTCBVH rtcNewBVH(RTCDevice device) {
+  Device* d = DE(device);
+  API_BEGIN
+  VERIFY_HANDLE(device);
+  return (RTCBVH) new BVHUser(d);
+  API_END(d)
+  return nullptr;
+}
+
+RTC_API void rtcRetainBVH(RTCBVH bvh) {
+  BVHUser* b = (BVHUser*)bvh;
+  API_BEGIN
+  VERIFY_HANDLE(bvh);
+  ref_inc(b);
+  API_END(b)
+}
+
+RTC_API void rtcReleaseBVH(RTCBVH bvh) {
+  BVHUser* b = (BVHUser*)bvh;
+  API_BEGIN
+  VERIFY_HANDLE(bvh);
+  ref_dec(b);
+  API_END((BVHUser*)nullptr)
+}
+
+RTC_API void* rtcThreadLocalAlloc(RTCThreadLocalAllocator allocator, size_t bytes, size_t align) {
+  API_BEGIN
+  VERIFY_HANDLE(allocator);
+  return ((Arena*)allocator)->alloc(bytes, align);
+  API_END((BVHUser*)nullptr)
+  return nullptr;
+}
+
+// Reference kernels/common/rtcore_builder.cpp:355-395.
+RTC_API void* rtcBuildBVH(const RTCBuildArguments* args) {
+  BVHUser* b = args ? (BVHUser*)args->bvh : nullptr;
+  API_BEGIN
+  VERIFY_HANDLE(args);
+  VERIFY_HANDLE(b);
+  VERIFY_HANDLE(args->createNode);
+  VERIFY_HANDLE(args->setNodeChildren);
+  VERIFY_HANDLE(args->setNodeBounds);
+  VERIFY_HANDLE(args->createLeaf);
+  if (args->primitiveArrayCapacity < args->primitiveCount)
+    RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "primitiveArrayCapacity must be greater or equal to primitiveCount");
+  if (args->buildQuality != RTC_BUILD_QUALITY_LOW && args->buildQuality != RTC_BUILD_QUALITY_MEDIUM &&
+      args->buildQuality != RTC_BUILD_QUALITY_HIGH)
+    RTC_THROW(RTC_ERROR_INVALID_OPERATION, "invalid build quality");
+  if (args->primitiveCount && !args->primitives) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  if (args->primitiveCount > 0x7fffffffull) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "too many primitives");
+  std::lock_guard<std::mutex> lk(b->mtx);
+  b->arena.reset();
+  RTCThreadLocalAllocator alloc = (RTCThreadLocalAllocator)&b->arena;
+  const uint32_t n = (uint32_t)args->primitiveCount;
+  if (n == 0) return args->createLeaf(alloc, args->primitives, 0, args->userPtr);
+  Device* d = b->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  // boxes to the device as GEOM_INSTANCES primitives (lo xyz, hi xyz)
+  std::vector<float> boxes((size_t)n * 6);
+  for (uint32_t i = 0; i < n; i++) {
+    const RTCBuildPrimitive& p = args->primitives[i];
+    float* f = &boxes[6 * (size_t)i];
+    f[0] = p.lower_x; f[1] = p.lower_y; f[2] = p.lower_z;
+    f[3] = p.upper_x; f[4] = p.upper_y; f[5] = p.upper_z;
+  }
+  rtg::DevBuf dboxes;
+  dboxes.reserve(boxes.size() * 4);
+  RTG_HIP(hipMemcpyAsync(dboxes.ptr, boxes.data(), boxes.size() * 4, hipMemcpyHostToDevice, t.stream));
+  rtg::GeomDesc gd{};
+  gd.vertices = (const char*)dboxes.ptr;
+  gd.indices = nullptr;
+  gd.vstride = 24;
+  gd.istride = 0;
+  gd.num_vertices = n;
+  gd.num_tris = n;
+  gd.kind = rtg::GEOM_INSTANCES;
+  rtg::BuildParams p = d->build;
+  p.builder = args->buildQuality == RTC_BUILD_QUALITY_LOW ? rtg::BUILDER_MORTON : rtg::BUILDER_SAH;
+  p.width = std::min(8u, std::max(2u, args->maxBranchingFactor));
+  p.max_leaf = std::min<unsigned>(RTC_BUILD_MAX_PRIMITIVES_PER_LEAF, std::max(1u, args->maxLeafSize));
+  p.min_leaf = std::max(1u, std::min(args->minLeafSize, p.max_leaf));
+  p.block = std::max(1u, args->sahBlockSize);
+  p.cost_node = args->traversalCost;
+  p.cost_tri = args->intersectionCost;
+  p.learned = nullptr;
+  p.robust = 0;
+  rtg::UserTree tree;
+  rtg::DeviceBVH scratch_out;
+  {
+    std::lock_guard<std::mutex> bl(d->build_mtx);
+    rtg::build_bvh({gd}, p, d->scratch, scratch_out, t.stream, 1, &tree);
+  }
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  // the primitive array in tree order (left-out primitives after the placed ones)
+  std::vector<RTCBuildPrimitive> sorted((size_t)n);
+  std::vector<uint8_t> placed(n, 0);
+  for (uint32_t q = 0; q < tree.n; q++) {
+    uint32_t src;
+    memcpy(&src, &tree.pbox[8 * (size_t)q + 7], 4);
+    if (src >= n || placed[src]) RTC_THROW(RTC_ERROR_UNKNOWN, "inconsistent user tree");
+    placed[src] = 1;
+    sorted[q] = args->primitives[src];
+  }
+  uint32_t q = tree.n;
+  for (uint32_t i = 0; i < n; i++)
+    if (!placed[i]) sorted[q++] = args->primitives[i];
+  memcpy(args->primitives, sorted.data(), sorted.size() * sizeof(RTCBuildPrimitive));
+  if (tree.n == 0) return args->createLeaf(alloc, args->primitives, 0, args->userPtr);
+  UserWalk w{tree, *args, args->primitives, alloc};
+  if (args->buildProgress && !args->buildProgress(args->userPtr, 0.0))
+    RTC_THROW(RTC_ERROR_CANCELLED, "progress monitor forced termination");
+  return w.node(tree.root, 0);
+  API_END(b)
+  return nullptr;
 }
 
 }  // extern "C"

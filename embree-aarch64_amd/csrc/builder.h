@@ -24,6 +24,8 @@ struct BuildParams {
   float cost_tri = 0.6f;       // SAH cost of one triangle test, per unit half area
   int quality = 1;             // RTCBuildQuality (LOW: Morton tree, HIGH: binned SAH, MEDIUM: `builder`)
   int robust = 0;              // leaves hold (v0, v1, v2) for the Pluecker test
+  uint32_t min_leaf = 1;       // subtrees of at most this many primitives become leaves
+  uint32_t block = 1;          // a leaf's cost counts blocks of this many primitives
   const LearnedCost* learned = nullptr;  // binned-SAH split cost from a trained model (device data)
 };
 
@@ -85,11 +87,28 @@ struct BuildScratch {
   SahScratch sah;
 };
 
+// A binary tree over caller boxes with the SAH-optimal collapse decisions
+// (for rtcBuildBVH): inner node i has children child[2i], child[2i+1]
+// (REF_LEAF | position for one primitive), covers positions range[i],
+// split8/isleaf as the collapse reads them; boxes are 8 floats (lo xyz, w,
+// hi xyz, w); for a primitive box pbox[8p + 7] holds, as float bits, the
+// input index of the primitive at position p.
+struct UserTree {
+  uint32_t n = 0;        // primitives placed (valid boxes)
+  uint32_t root = 0;     // binary root reference
+  uint32_t width = 8;
+  std::vector<uint32_t> child, range;
+  std::vector<uint8_t> split8, isleaf;
+  std::vector<float> nbox, pbox;
+};
+
 // Builds over the given geometries (device pointers in GeomDesc). With
 // nscenes > 1 every GeomDesc names its scene, the tree is the binned-SAH
 // forest of all scenes in one pass, and scene s gets root node s.
 // Throws hip_error on a HIP failure.
+// With `user` set the build stops after the binary tree and hands it over
+// (one scene of GEOM_INSTANCES boxes; no nodes or records are written).
 void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& p, BuildScratch& scratch,
-               DeviceBVH& out, hipStream_t s, uint32_t nscenes = 1);
+               DeviceBVH& out, hipStream_t s, uint32_t nscenes = 1, UserTree* user = nullptr);
 
 }  // namespace rtg

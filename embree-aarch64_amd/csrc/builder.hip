@@ -362,7 +362,7 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
                                               const float4* __restrict__ shi, float4* nlo, float4* nhi,
                                               float* cost8, uint8_t* split8, uint8_t* isleaf, uint32_t* visit,
                                               float cost_node, float cost_tri, uint32_t max_leaf, uint32_t width,
-                                              uint32_t* err) {
+                                              uint32_t min_leaf, uint32_t block, uint32_t* err) {
   const uint32_t j = blockIdx.x * BT + threadIdx.x;
   if (j >= n) return;
   uint32_t node = parent_leaf[j];
@@ -428,8 +428,10 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
       d[s] = best;
       dj[s] = bj;
     }
-    const float leafc = count <= max_leaf ? cost_tri * (float)count * A : INFINITY;
-    const float inner = cost_node * A + d[width];  // a wide node holds at most `width` children
+    // a leaf pays per block of `block` primitives (the reference's
+    // sahBlockSize); subtrees of at most min_leaf primitives are leaves
+    const float leafc = count <= max_leaf ? cost_tri * (float)((count + block - 1) / block) * A : INFINITY;
+    const float inner = count <= min_leaf ? INFINITY : cost_node * A + d[width];  // at most `width` children
     float c[8];
     uint32_t sp[8];
     c[0] = fminf(leafc, inner);
@@ -767,7 +769,7 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
 // ------------------------------------------------------------------ host --
 
 void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, BuildScratch& S, DeviceBVH& out,
-               hipStream_t s, uint32_t nscenes) {
+               hipStream_t s, uint32_t nscenes, UserTree* user) {
   auto t0 = std::chrono::steady_clock::now();
   out.generation++;
   out.stats = BuildStats();
@@ -775,13 +777,20 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   uint32_t total = 0;
   for (const auto& g : geoms) total += g.num_tris;
   out.slots.assign(nscenes, SceneSlot());
+  if (user) *user = UserTree();
   if (geoms.empty() || total == 0) {
     out.stats.num_prims = 0;
     out.stats.num_nodes = 0;
     return;
   }
-  const uint32_t max_leaf = prm.max_leaf < 1 ? 1 : (prm.max_leaf > 15 ? 15 : prm.max_leaf);
+  if (user && nscenes != 1) throw hip_error(hipErrorInvalidValue, "a user tree is one scene");
+  // a QNode8 leaf slot addresses at most 15 records; a user tree's leaves
+  // are handed to the caller and may hold up to 32
+  const uint32_t leaf_cap = user ? 32u : 15u;
+  const uint32_t max_leaf = prm.max_leaf < 1 ? 1 : (prm.max_leaf > leaf_cap ? leaf_cap : prm.max_leaf);
   const uint32_t width = prm.width < 2 ? 2 : (prm.width > 8 ? 8 : prm.width);
+  const uint32_t min_leaf = prm.min_leaf < 1 ? 1 : (prm.min_leaf > max_leaf ? max_leaf : prm.min_leaf);
+  const uint32_t block = prm.block < 1 ? 1 : prm.block;
 
   S.geoms.reserve(geoms.size() * sizeof(GeomDesc));
   RTG_HIP(hipMemcpyAsync(S.geoms.ptr, geoms.data(), geoms.size() * sizeof(GeomDesc), hipMemcpyHostToDevice, s));
@@ -863,6 +872,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     out.stats.num_nodes = 0;
     return;
   }
+  if (user) user->n = n;
 
   S.slo.reserve((size_t)n * 16);
   S.shi.reserve((size_t)n * 16);
@@ -907,7 +917,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
                        S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(),
                        S.slo.as<float4>(), S.shi.as<float4>(), S.nlo.as<float4>(), S.nhi.as<float4>(),
                        S.cost.as<float>(), S.split8.as<uint8_t>(), S.isleaf.as<uint8_t>(), S.visit.as<uint32_t>(),
-                       prm.cost_node, prm.cost_tri, max_leaf, width, &cnt->error);
+                       prm.cost_node, prm.cost_tri, max_leaf, width, min_leaf, block, &cnt->error);
     hipLaunchKernelGGL(k_check_tree, dim3(div_up(n, BT)), dim3(BT), 0, s,
                        use_sah ? nullptr : S.keys.as<uint32_t>(), n, &cnt->roots_seen,
                        S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
@@ -922,6 +932,32 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
       throw hip_error(hipErrorUnknown, "binary tree check failed: error bits " + std::to_string(hc.error) +
                                            ", roots " + std::to_string(hc.roots_seen) + " of " +
                                            std::to_string(expect));
+  }
+
+  if (user) {  // the binary tree with its collapse decisions, for the caller's own nodes
+    const uint32_t ni = n > 1 ? n - 1 : 0;
+    user->root = root_ref[0];
+    user->child.resize(2 * (size_t)ni);
+    user->range.resize(2 * (size_t)ni);
+    user->split8.resize(8 * (size_t)ni);
+    user->isleaf.resize(ni);
+    user->nbox.resize(8 * (size_t)ni);
+    user->pbox.resize(8 * (size_t)n);
+    if (ni) {
+      RTG_HIP(hipMemcpyAsync(user->child.data(), S.child.ptr, 8 * (size_t)ni, hipMemcpyDeviceToHost, s));
+      RTG_HIP(hipMemcpyAsync(user->range.data(), S.range.ptr, 8 * (size_t)ni, hipMemcpyDeviceToHost, s));
+      RTG_HIP(hipMemcpyAsync(user->split8.data(), S.split8.ptr, 8 * (size_t)ni, hipMemcpyDeviceToHost, s));
+      RTG_HIP(hipMemcpyAsync(user->isleaf.data(), S.isleaf.ptr, ni, hipMemcpyDeviceToHost, s));
+      RTG_HIP(hipMemcpy2DAsync(user->nbox.data(), 32, S.nlo.ptr, 16, 16, ni, hipMemcpyDeviceToHost, s));
+      RTG_HIP(hipMemcpy2DAsync(user->nbox.data() + 4, 32, S.nhi.ptr, 16, 16, ni, hipMemcpyDeviceToHost, s));
+    }
+    RTG_HIP(hipMemcpy2DAsync(user->pbox.data(), 32, S.slo.ptr, 16, 16, n, hipMemcpyDeviceToHost, s));
+    RTG_HIP(hipMemcpy2DAsync(user->pbox.data() + 4, 32, S.shi.ptr, 16, 16, n, hipMemcpyDeviceToHost, s));
+    RTG_HIP(hipStreamSynchronize(s));
+    user->width = width;
+    out.stats.num_prims = n;
+    out.stats.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return;
   }
 
   // outputs: at most n + nscenes nodes, exactly n triangle records

@@ -59,6 +59,43 @@ class RTCIntersectContext(ctypes.Structure):
     _fields_ = [("flags", ctypes.c_int), ("filter", ctypes.c_void_p), ("instID", ctypes.c_uint * 1)]
 
 
+RTCBuildPrimitive = np.dtype([("lower_x", "<f4"), ("lower_y", "<f4"), ("lower_z", "<f4"), ("geomID", "<u4"),
+                              ("upper_x", "<f4"), ("upper_y", "<f4"), ("upper_z", "<f4"), ("primID", "<u4")])
+assert RTCBuildPrimitive.itemsize == 32
+
+CREATE_NODE_FUNC = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint, ctypes.c_void_p)
+SET_NODE_CHILDREN_FUNC = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint,
+                                          ctypes.c_void_p)
+SET_NODE_BOUNDS_FUNC = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ctypes.c_void_p), ctypes.c_uint,
+                                        ctypes.c_void_p)
+CREATE_LEAF_FUNC = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
+                                    ctypes.c_void_p)
+BUILD_PROGRESS_FUNC = ctypes.CFUNCTYPE(ctypes.c_bool, ctypes.c_void_p, ctypes.c_double)
+
+
+class RTCBuildArguments(ctypes.Structure):
+    _fields_ = [("byteSize", ctypes.c_size_t), ("buildQuality", ctypes.c_int), ("buildFlags", ctypes.c_int),
+                ("maxBranchingFactor", ctypes.c_uint), ("maxDepth", ctypes.c_uint), ("sahBlockSize", ctypes.c_uint),
+                ("minLeafSize", ctypes.c_uint), ("maxLeafSize", ctypes.c_uint), ("traversalCost", ctypes.c_float),
+                ("intersectionCost", ctypes.c_float), ("bvh", ctypes.c_void_p), ("primitives", ctypes.c_void_p),
+                ("primitiveCount", ctypes.c_size_t), ("primitiveArrayCapacity", ctypes.c_size_t),
+                ("createNode", CREATE_NODE_FUNC), ("setNodeChildren", SET_NODE_CHILDREN_FUNC),
+                ("setNodeBounds", SET_NODE_BOUNDS_FUNC), ("createLeaf", CREATE_LEAF_FUNC),
+                ("splitPrimitive", ctypes.c_void_p), ("buildProgress", BUILD_PROGRESS_FUNC),
+                ("userPtr", ctypes.c_void_p)]
+
+
+def default_build_arguments():
+    """rtcDefaultBuildArguments (reference rtcore_builder.h:82-108)."""
+    a = RTCBuildArguments()
+    a.byteSize = ctypes.sizeof(a)
+    a.buildQuality = RTC_BUILD_QUALITY_MEDIUM
+    a.maxBranchingFactor, a.maxDepth, a.sahBlockSize = 2, 32, 1
+    a.minLeafSize, a.maxLeafSize = 1, 32
+    a.traversalCost = a.intersectionCost = 1.0
+    return a
+
+
 class RTCBounds(ctypes.Structure):
     _fields_ = [("lower_x", ctypes.c_float), ("lower_y", ctypes.c_float), ("lower_z", ctypes.c_float),
                 ("align0", ctypes.c_float), ("upper_x", ctypes.c_float), ("upper_y", ctypes.c_float),
@@ -110,6 +147,11 @@ SIGNATURES = {
     "rtcSetGeometryUserData": (None, [_vp, _vp]),
     "rtcGetGeometryUserData": (_vp, [_vp]),
     "rtcSetGeometryInstancedScene": (None, [_vp, _vp]),
+    "rtcNewBVH": (_vp, [_vp]),
+    "rtcBuildBVH": (_vp, [_vp]),
+    "rtcThreadLocalAlloc": (_vp, [_vp, _sz, _sz]),
+    "rtcRetainBVH": (None, [_vp]),
+    "rtcReleaseBVH": (None, [_vp]),
     "rtcSetGeometryTransform": (None, [_vp, _u32, _i32, _vp]),
     "rtcGetGeometryTransform": (None, [_vp, _f32, _i32, _vp]),
     "rtcNewScene": (_vp, [_vp]),
@@ -564,3 +606,60 @@ def batch_items(streams):
     for i, (sc, ptr, n, stride) in enumerate(streams):
         it[i] = (sc.h or 0, ptr or 0, n, stride)
     return it
+
+
+def build_user_bvh(device, prims, quality=RTC_BUILD_QUALITY_MEDIUM, branching=2, min_leaf=1, max_leaf=32,
+                   block=1, traversal_cost=1.0, intersection_cost=1.0, progress=None, capacity=None):
+    """rtcBuildBVH with Python callbacks that record the tree the library
+    builds over `prims` (an RTCBuildPrimitive array, reordered in place).
+    Returns (root, nodes, leaves): nodes[id] = {"bounds": [(lo, hi)...],
+    "children": [...]}, leaves[id] = (offset into prims, count); ids are the
+    void* values the callbacks returned (nodes odd, leaves even)."""
+    L = lib()
+    bvh = L.rtcNewBVH(device.h)
+    device.check("rtcNewBVH")
+    nodes, leaves = {}, {}
+    base = prims.ctypes.data
+    counter = [0]
+
+    def create_node(alloc, n, user):
+        mem = L.rtcThreadLocalAlloc(alloc, 64, 16)
+        counter[0] += 1
+        nid = 2 * counter[0] + 1
+        nodes[nid] = {"n": n, "bounds": None, "children": None, "mem": mem}
+        return nid
+
+    def set_bounds(node, bptr, n, user):
+        out = []
+        for k in range(n):
+            b = ctypes.cast(bptr[k], ctypes.POINTER(RTCBounds)).contents
+            out.append(((b.lower_x, b.lower_y, b.lower_z), (b.upper_x, b.upper_y, b.upper_z)))
+        nodes[node]["bounds"] = out
+
+    def set_children(node, cptr, n, user):
+        nodes[node]["children"] = [cptr[k] for k in range(n)]
+
+    def create_leaf(alloc, pptr, n, user):
+        counter[0] += 1
+        lid = 2 * counter[0]
+        off = ((pptr or base) - base) // 32
+        leaves[lid] = (off, n)
+        return lid
+
+    cbs = [CREATE_NODE_FUNC(create_node), SET_NODE_CHILDREN_FUNC(set_children), SET_NODE_BOUNDS_FUNC(set_bounds),
+           CREATE_LEAF_FUNC(create_leaf), BUILD_PROGRESS_FUNC(progress or (lambda u, f: True))]
+    a = default_build_arguments()
+    a.buildQuality = quality
+    a.maxBranchingFactor, a.minLeafSize, a.maxLeafSize, a.sahBlockSize = branching, min_leaf, max_leaf, block
+    a.traversalCost, a.intersectionCost = traversal_cost, intersection_cost
+    a.bvh = bvh
+    a.primitives = base
+    a.primitiveCount = len(prims)
+    a.primitiveArrayCapacity = len(prims) if capacity is None else capacity
+    a.createNode, a.setNodeChildren, a.setNodeBounds, a.createLeaf, a.buildProgress = cbs
+    root = L.rtcBuildBVH(ctypes.byref(a))
+    err = L.rtcGetDeviceError(device.h)
+    L.rtcReleaseBVH(bvh)
+    if err:
+        raise RTCError(err, "rtcBuildBVH")
+    return root, nodes, leaves

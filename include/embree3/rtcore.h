@@ -9,3 +9,4 @@
 #include "rtcore_ray.h"
 #include "rtcore_geometry.h"
 #include "rtcore_scene.h"
+#include "rtcore_builder.h"

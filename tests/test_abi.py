@@ -60,6 +60,16 @@ int main(void) {
   printf("%zu %zu %zu %zu %zu %zu\n", sizeof(struct RTCRayHit4), sizeof(struct RTCRayHit8), sizeof(struct RTCRayHit16),
          offsetof(struct RTCRayHit4, hit), offsetof(struct RTCRayHit8, hit.u), offsetof(struct RTCRayHit16, hit.instID));
   printf("%zu %zu\n", sizeof(struct RTCRayHitNp), offsetof(struct RTCRayHitNp, hit.geomID));
+  printf("%zu %zu %zu %zu\n", sizeof(struct RTCBuildPrimitive), offsetof(struct RTCBuildPrimitive, geomID),
+         offsetof(struct RTCBuildPrimitive, upper_x), offsetof(struct RTCBuildPrimitive, primID));
+  printf("%zu %zu %zu %zu %zu %zu %zu %zu\n", sizeof(struct RTCBuildArguments),
+         offsetof(struct RTCBuildArguments, buildQuality), offsetof(struct RTCBuildArguments, maxLeafSize),
+         offsetof(struct RTCBuildArguments, intersectionCost), offsetof(struct RTCBuildArguments, bvh),
+         offsetof(struct RTCBuildArguments, primitiveArrayCapacity), offsetof(struct RTCBuildArguments, createNode),
+         offsetof(struct RTCBuildArguments, userPtr));
+  struct RTCBuildArguments a = rtcDefaultBuildArguments();
+  printf("%zu %d %u %u %u %u %u %g %g\n", a.byteSize, (int)a.buildQuality, a.maxBranchingFactor, a.maxDepth,
+         a.sahBlockSize, a.minLeafSize, a.maxLeafSize, a.traversalCost, a.intersectionCost);
   return 0;
 }
 """
@@ -80,6 +90,10 @@ def test_struct_layout_and_enums(tmp_path, compiler):
     # packets: 12 ray + 8 hit fields of K lanes each; pointer SoA: 20 pointers
     assert lines[5].split() == ["320", "640", "1280", "192", str(12 * 32 + 3 * 32), str(12 * 64 + 7 * 64)]
     assert lines[6].split() == ["160", str(8 * 18)]
+    # user builder (reference rtcore_builder.h:14-108)
+    assert lines[7].split() == ["32", "12", "16", "28"]
+    assert lines[8].split() == ["136", "8", "32", "40", "48", "72", "80", "128"]
+    assert lines[9].split() == ["136", "1", "2", "32", "1", "1", "32", "1", "1"]
 
 
 def test_c_program_links_against_library(tmp_path, rtc):
