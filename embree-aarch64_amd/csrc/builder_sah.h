@@ -14,7 +14,7 @@
 namespace rtg {
 
 struct SahScratch {
-  DevBuf idx[2], blo[2], bhi[2], sid[2], segs[3][2], bins, splits, flags, scan, counters, roots;
+  DevBuf idx[2], blo[2], bhi[2], sid[2], segs[4][2], bins, splits, flags, scan, counters, roots;
   Scanner scanner;
 };
 

@@ -23,7 +23,10 @@
 //                   left flags gives every position its stable destination;
 //   mid   (65..4096): one 256-lane workgroup per segment bins, picks and
 //                   partitions in LDS;
-//   small (2..64): one 64-lane workgroup per segment, same code.
+//   small (9..64): one 64-lane workgroup per segment, same code;
+//   tiny  (2..8): one thread finishes the whole subtree with exact SAH
+//                 splits (k_tiny), launched right after the level that made
+//                 the segment; a root segment of 2..64 starts as small.
 // Every inner node gets the id of its split position (first + n_left - 1),
 // which is unique, so the output does not depend on the order in which
 // segments are processed.
@@ -39,6 +42,7 @@ constexpr uint32_t NIL = 0xffffffffu;
 constexpr int NB = 32;                  // bins per axis
 constexpr int BIN_WORDS = 3 * NB * 7;   // counts, then lo xyz, then hi xyz, per (axis, bin)
 constexpr uint32_t SMALL_MAX = 64;
+constexpr uint32_t TINY_MAX = 8;        // segments up to this size are finished by one thread (exact SAH)
 constexpr uint32_t MID_MAX = 4096;
 constexpr uint32_t CHUNK = 4096;        // positions per block of the big-segment binning
 constexpr uint32_t MEDIAN_DEPTH = 48;   // from this binary depth on, splits are object medians
@@ -65,16 +69,15 @@ struct Split {
 };
 
 struct Counters {
-  uint32_t nseg[3];  // next-level segment counts: big, mid, small
-  uint32_t pad;
+  uint32_t nseg[4];  // next-level segment counts: big, mid, small, tiny
   uint32_t max_depth;
   uint32_t error;    // bit 0: a segment list overflowed, bit 1: bin counts disagree, bit 2: destination out of range,
-                     // bit 3: median subtree stack overflow
+                     // bit 3: median subtree stack overflow, bit 4: a tiny segment out of range
 };
 
 struct Lists {
-  Seg* seg[3];
-  uint32_t cap[3];
+  Seg* seg[4];
+  uint32_t cap[4];
   uint32_t fin_max;  // segments up to this size are finished at once by median splits
 };
 
@@ -266,7 +269,7 @@ __device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m
     finish_median(t, m, side, depth, first, count);
     return NIL;
   }
-  const int cls = count > MID_MAX ? 0 : (count > SMALL_MAX ? 1 : 2);
+  const int cls = count > MID_MAX ? 0 : (count > SMALL_MAX ? 1 : (count > TINY_MAX ? 2 : 3));
   const uint32_t id = atomicAdd(&t.cnt->nseg[cls], 1u);
   if (id >= nx.cap[cls]) {
     atomicOr(&t.cnt->error, 1u);
@@ -422,7 +425,7 @@ __global__ void k_init_gbins(uint32_t* g, uint32_t nseg) {
 }
 
 __global__ void k_zero_counts(Counters* c) {
-  if (threadIdx.x < 3) c->nseg[threadIdx.x] = 0u;
+  if (threadIdx.x < 4) c->nseg[threadIdx.x] = 0u;
 }
 
 // A chunk of CHUNK positions meets at most two big segments (each is longer
@@ -679,6 +682,119 @@ __global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, con
   }
 }
 
+// ------------------------------------------------------------ tiny segments --
+
+constexpr int TT = 64;  // threads per block of k_tiny, one segment each
+
+// Stable insertion sort of entries [f, f+c) by centroid (lo + hi) on axis a.
+__device__ inline void tiny_sort(float4* lo, float4* hi, uint32_t* id, uint32_t f, uint32_t c, int a) {
+  for (uint32_t j = f + 1; j < f + c; j++) {
+    const float4 l = lo[j], h = hi[j];
+    const uint32_t v = id[j];
+    const float key = comp(l, a) + comp(h, a);
+    uint32_t k = j;
+    while (k > f && comp(lo[k - 1], a) + comp(hi[k - 1], a) > key) {
+      lo[k] = lo[k - 1];
+      hi[k] = hi[k - 1];
+      id[k] = id[k - 1];
+      k--;
+    }
+    lo[k] = l;
+    hi[k] = h;
+    id[k] = v;
+  }
+}
+
+// One thread finishes the whole subtree of a segment of 2..TINY_MAX
+// references: every split is the exact SAH optimum over the references
+// sorted by centroid on each axis (least A_L*n_L + A_R*n_R, first minimum
+// over axes then positions), inner node ids are split positions as for the
+// binned levels, and vals receives the final order. The references are
+// staged in this thread's rows of LDS.
+__global__ __launch_bounds__(TT) void k_tiny(const Seg* __restrict__ segs, uint32_t nseg,
+                                             const uint32_t* __restrict__ idx, const float4* __restrict__ blo,
+                                             const float4* __restrict__ bhi, Tree t) {
+  __shared__ float4 s_lo[TT][TINY_MAX], s_hi[TT][TINY_MAX];
+  __shared__ uint32_t s_id[TT][TINY_MAX];
+  const uint32_t i = blockIdx.x * TT + threadIdx.x;
+  if (i >= nseg) return;
+  const Seg sg = segs[i];
+  if (sg.count < 2 || sg.count > TINY_MAX) {
+    atomicOr(&t.cnt->error, 16u);
+    return;
+  }
+  float4* lo = s_lo[threadIdx.x];
+  float4* hi = s_hi[threadIdx.x];
+  uint32_t* id = s_id[threadIdx.x];
+  const uint32_t m = sg.count;
+  for (uint32_t j = 0; j < m; j++) {
+    lo[j] = blo[sg.first + j];
+    hi[j] = bhi[sg.first + j];
+    id[j] = idx[sg.first + j];
+  }
+  uint32_t sf[TINY_MAX], sc[TINY_MAX], spar[TINY_MAX], smeta[TINY_MAX];
+  int sp = 0;
+  sf[0] = 0; sc[0] = m; spar[0] = sg.parent; smeta[0] = sg.meta;
+  sp = 1;
+  while (sp > 0) {
+    sp--;
+    const uint32_t f = sf[sp], c = sc[sp], p = spar[sp], meta = smeta[sp];
+    const uint32_t side = meta & 1u, depth = meta >> 1;
+    if (c == 1) {
+      t.child[2 * p + side] = (sg.first + f) | REF_LEAF;
+      t.parent_leaf[sg.first + f] = p;
+      continue;
+    }
+    float best = INFINITY;
+    int ba = 0;
+    uint32_t bk = 1;
+    for (int a = 0; a < 3; a++) {
+      tiny_sort(lo, hi, id, f, c, a);
+      float ra[TINY_MAX];
+      float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+#pragma unroll
+      for (int k = TINY_MAX - 1; k >= 1; k--) {
+        if ((uint32_t)k < c) {
+          const float4 l = lo[f + k], h = hi[f + k];
+          bl[0] = fminf(bl[0], l.x); bl[1] = fminf(bl[1], l.y); bl[2] = fminf(bl[2], l.z);
+          bh[0] = fmaxf(bh[0], h.x); bh[1] = fmaxf(bh[1], h.y); bh[2] = fmaxf(bh[2], h.z);
+          ra[k] = half_area(bh[0] - bl[0], bh[1] - bl[1], bh[2] - bl[2]);
+        }
+      }
+      bl[0] = bl[1] = bl[2] = INFINITY;
+      bh[0] = bh[1] = bh[2] = -INFINITY;
+#pragma unroll
+      for (int k = 1; k < (int)TINY_MAX; k++) {
+        if ((uint32_t)k < c) {
+          const float4 l = lo[f + k - 1], h = hi[f + k - 1];
+          bl[0] = fminf(bl[0], l.x); bl[1] = fminf(bl[1], l.y); bl[2] = fminf(bl[2], l.z);
+          bh[0] = fmaxf(bh[0], h.x); bh[1] = fmaxf(bh[1], h.y); bh[2] = fmaxf(bh[2], h.z);
+          const float cost = half_area(bh[0] - bl[0], bh[1] - bl[1], bh[2] - bl[2]) * (float)k +
+                             ra[k] * (float)(c - (uint32_t)k);
+          if (cost < best) {
+            best = cost;
+            ba = a;
+            bk = (uint32_t)k;
+          }
+        }
+      }
+    }
+    if (ba != 2) tiny_sort(lo, hi, id, f, c, ba);  // the last sort left axis 2's order
+    const uint32_t mid = sg.first + f + bk - 1;
+    t.child[2 * p + side] = mid;
+    t.parent_int[mid] = p;
+    t.range[mid] = make_uint2(sg.first + f, sg.first + f + c - 1);
+    atomicMax(&t.cnt->max_depth, depth);
+    if (sp + 2 > (int)TINY_MAX) {
+      atomicOr(&t.cnt->error, 16u);
+      return;
+    }
+    sf[sp] = f + bk; sc[sp] = c - bk; spar[sp] = mid; smeta[sp] = ((depth + 1) << 1) | 1u; sp++;
+    sf[sp] = f;      sc[sp] = bk;     spar[sp] = mid; smeta[sp] = ((depth + 1) << 1);      sp++;
+  }
+  for (uint32_t j = 0; j < m; j++) t.vals[sg.first + j] = id[j];
+}
+
 // Writes the big-segment id of every position of the initial big segments.
 __global__ void k_fill_seg_ids(uint32_t* sid, const Seg* segs) {
   const Seg sg = segs[blockIdx.x];
@@ -698,13 +814,13 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
   if (trees.size() >= ROOT_TAG) throw hip_error(hipErrorInvalidValue, "too many trees");
   for (const SahRoot& r : trees)
     if (r.count < 2 || (uint64_t)r.first + r.count > n) throw hip_error(hipErrorInvalidValue, "bad SAH tree range");
-  const uint32_t cap[3] = {n / MID_MAX + 2, n / (SMALL_MAX + 1) + 2, n / 2 + 2};
+  const uint32_t cap[4] = {n / MID_MAX + 2, n / (SMALL_MAX + 1) + 2, n / (TINY_MAX + 1) + 2, n / 2 + 2};
   for (int p = 0; p < 2; p++) {
     S.idx[p].reserve((size_t)n * 4);
     S.blo[p].reserve((size_t)n * 16);
     S.bhi[p].reserve((size_t)n * 16);
     S.sid[p].reserve((size_t)n * 4);
-    for (int c = 0; c < 3; c++) S.segs[c][p].reserve((size_t)cap[c] * sizeof(Seg));
+    for (int c = 0; c < 4; c++) S.segs[c][p].reserve((size_t)cap[c] * sizeof(Seg));
   }
   S.bins.reserve((size_t)cap[0] * BIN_WORDS * 4);
   S.splits.reserve((size_t)cap[0] * sizeof(Split));
@@ -759,7 +875,7 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
     if (++levels > 4096) throw hip_error(hipErrorUnknown, "SAH build did not terminate");
     const int in = par, o = par ^ 1;
     Lists nx;
-    for (int c = 0; c < 3; c++) {
+    for (int c = 0; c < 4; c++) {
       nx.seg[c] = S.segs[c][o].as<Seg>();
       nx.cap[c] = cap[c];
     }
@@ -796,11 +912,18 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
     RTG_HIP(hipStreamSynchronize(s));
     if (hc.error) throw hip_error(hipErrorUnknown, "SAH build failed: error bits " + std::to_string(hc.error));
     for (int c = 0; c < 3; c++) cur[c] = hc.nseg[c];
+    // this level's tiny children, finished in place (their positions are
+    // outside every segment the next levels touch)
+    if (hc.nseg[3])
+      hipLaunchKernelGGL(k_tiny, dim3(div_up(hc.nseg[3], TT)), dim3(TT), 0, s, S.segs[3][o].as<Seg>(), hc.nseg[3],
+                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), t);
     par = o;
   }
   out.roots.assign(trees.size(), NIL);
   RTG_HIP(hipMemcpyAsync(out.roots.data(), S.roots.ptr, trees.size() * 4, hipMemcpyDeviceToHost, s));
+  RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));
+  if (hc.error) throw hip_error(hipErrorUnknown, "SAH build failed: error bits " + std::to_string(hc.error));
   for (uint32_t r : out.roots)
     if (r == NIL || r + 1 >= n) throw hip_error(hipErrorUnknown, "SAH build produced no root");
   out.levels = levels;
