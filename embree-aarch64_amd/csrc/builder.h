@@ -82,7 +82,7 @@ struct DeviceBVH {
 // Scratch reused across builds on one device.
 struct BuildScratch {
   DevBuf geoms, lo4, hi4, keys, vals, slo, shi, child, parent_int, parent_leaf, range, nlo, nhi, cost,
-      isleaf, split8, visit, queue, counters, sacc;
+      isleaf, split8, visit, queue, counters, sacc, depth, lvl_hist, lvl_order;
   RadixSorter sorter;
   SahScratch sah;
 };

@@ -355,20 +355,120 @@ __device__ inline float box_half_area(float4 l, float4 h) { return half_area(h.x
 // (Collapse formulation after Ylitie, Karras, Laine: "Efficient Incoherent
 // Ray Traversal on GPUs Through Compressed Wide BVHs", HPG 2017.)
 // A node's width is capped at `width` children (d(n, width) in the inner cost).
-__global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __restrict__ child,
-                                              const uint32_t* __restrict__ parent_int,
-                                              const uint32_t* __restrict__ parent_leaf,
-                                              const uint2* __restrict__ range, const float4* __restrict__ slo,
-                                              const float4* __restrict__ shi, float4* nlo, float4* nhi,
-                                              float* cost8, uint8_t* split8, uint8_t* isleaf, uint32_t* visit,
-                                              float cost_node, float cost_tri, uint32_t max_leaf, uint32_t width,
-                                              uint32_t min_leaf, uint32_t block, uint32_t* err) {
+// Bottom-up step of one inner node: box of its two children, and the
+// SAH-optimal way to spend 1..8 child slots below it (the 8-wide collapse
+// DP): cost8[s-1] = least cost of covering the subtree with s slots,
+// split8[s-1] = how many of them go left (0x80: fewer slots do as well),
+// isleaf = a leaf beats an inner node. Returns false on a broken link.
+struct RefitArgs {
+  uint32_t n;
+  const uint32_t* child;
+  const uint2* range;
+  const float4* slo;
+  const float4* shi;
+  float4* nlo;
+  float4* nhi;
+  float* cost8;
+  uint8_t* split8;
+  uint8_t* isleaf;
+  float cost_node, cost_tri;
+  uint32_t max_leaf, width, min_leaf, block;
+  uint32_t* err;
+};
+
+__device__ inline bool refit_node(const RefitArgs& a, uint32_t node) {
+  const uint32_t n = a.n;
+  const uint32_t* child = a.child;
+  const uint2* range = a.range;
+  const float4* slo = a.slo;
+  const float4* shi = a.shi;
+  float4* nlo = a.nlo;
+  float4* nhi = a.nhi;
+  float* cost8 = a.cost8;
+  uint8_t* split8 = a.split8;
+  uint8_t* isleaf = a.isleaf;
+  const float cost_node = a.cost_node, cost_tri = a.cost_tri;
+  const uint32_t max_leaf = a.max_leaf, width = a.width, min_leaf = a.min_leaf, block = a.block;
+  uint32_t* err = a.err;
+  float4 bl[2], bh[2];
+  float cc[2][8];
+#pragma unroll
+  for (int k = 0; k < 2; k++) {
+    const uint32_t c = child[2 * node + k];
+    if ((c & REF_LEAF) ? (c & ~REF_LEAF) >= n : c + 1 >= n) {
+      atomicOr(err, 1u);
+      return false;
+    }
+    if (c & REF_LEAF) {
+      bl[k] = slo[c & ~REF_LEAF];
+      bh[k] = shi[c & ~REF_LEAF];
+      const float ct = cost_tri * box_half_area(bl[k], bh[k]);
+#pragma unroll
+      for (int s = 0; s < 8; s++) cc[k][s] = ct;
+    } else {
+      bl[k] = nlo[c];
+      bh[k] = nhi[c];
+      const float4* t = reinterpret_cast<const float4*>(cost8 + 8 * (size_t)c);
+      const float4 t0 = t[0], t1 = t[1];
+      cc[k][0] = t0.x; cc[k][1] = t0.y; cc[k][2] = t0.z; cc[k][3] = t0.w;
+      cc[k][4] = t1.x; cc[k][5] = t1.y; cc[k][6] = t1.z; cc[k][7] = t1.w;
+    }
+  }
+  const float4 l = make_float4(fminf(bl[0].x, bl[1].x), fminf(bl[0].y, bl[1].y), fminf(bl[0].z, bl[1].z), 0.0f);
+  const float4 h = make_float4(fmaxf(bh[0].x, bh[1].x), fmaxf(bh[0].y, bh[1].y), fmaxf(bh[0].z, bh[1].z), 0.0f);
+  const uint2 r = range[node];
+  const uint32_t count = r.y - r.x + 1;
+  const float A = box_half_area(l, h);
+  float d[9];
+  uint32_t dj[9];
+#pragma unroll
+  for (int s = 2; s <= 8; s++) {
+    float best = INFINITY;
+    uint32_t bj = 1;
+#pragma unroll
+    for (int jj = 1; jj < s; jj++) {
+      const float v = cc[0][jj - 1] + cc[1][s - jj - 1];
+      if (v < best) { best = v; bj = (uint32_t)jj; }
+    }
+    d[s] = best;
+    dj[s] = bj;
+  }
+  // a leaf pays per block of `block` primitives (the reference's
+  // sahBlockSize); subtrees of at most min_leaf primitives are leaves
+  const float leafc = count <= max_leaf ? cost_tri * (float)((count + block - 1) / block) * A : INFINITY;
+  const float inner = count <= min_leaf ? INFINITY : cost_node * A + d[width];  // at most `width` children
+  float c[8];
+  uint32_t sp[8];
+  c[0] = fminf(leafc, inner);
+  sp[0] = 0;
+#pragma unroll
+  for (int s = 2; s <= 8; s++) {
+    if (d[s] < c[s - 2]) { c[s - 1] = d[s]; sp[s - 1] = dj[s]; }
+    else { c[s - 1] = c[s - 2]; sp[s - 1] = 0x80u | dj[s]; }
+  }
+  nlo[node] = l;
+  nhi[node] = h;
+  float4* t = reinterpret_cast<float4*>(cost8 + 8 * (size_t)node);
+  t[0] = make_float4(c[0], c[1], c[2], c[3]);
+  t[1] = make_float4(c[4], c[5], c[6], c[7]);
+  uint2* sp8 = reinterpret_cast<uint2*>(split8 + 8 * (size_t)node);
+  *sp8 = make_uint2(sp[0] | (sp[1] << 8) | (sp[2] << 16) | (sp[3] << 24),
+                    sp[4] | (sp[5] << 8) | (sp[6] << 16) | (sp[7] << 24));
+  isleaf[node] = leafc <= inner ? 1 : 0;
+  return true;
+}
+
+// Refit by walking up from every primitive: the second thread to reach a
+// node (a counter per node) processes it.
+__global__ __launch_bounds__(BT) void k_refit(RefitArgs a, const uint32_t* __restrict__ parent_int,
+                                              const uint32_t* __restrict__ parent_leaf, uint32_t* visit) {
+  const uint32_t n = a.n;
   const uint32_t j = blockIdx.x * BT + threadIdx.x;
   if (j >= n) return;
   uint32_t node = parent_leaf[j];
   for (int steps = 0; node != NONE; steps++) {
     if (node >= n - 1 || steps > 512) {  // a radix tree over n keys is at most n-1 levels deep
-      atomicOr(err, 1u);
+      atomicOr(a.err, 1u);
       return;
     }
     // Hand-off to the sibling's thread: this thread's stores of the child it
@@ -385,73 +485,54 @@ __global__ __launch_bounds__(BT) void k_refit(uint32_t n, const uint32_t* __rest
     const uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     if (old == 0) return;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    float4 bl[2], bh[2];
-    float cc[2][8];
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-      const uint32_t c = child[2 * node + k];
-      if ((c & REF_LEAF) ? (c & ~REF_LEAF) >= n : c + 1 >= n) {
-        atomicOr(err, 1u);
-        return;
-      }
-      if (c & REF_LEAF) {
-        bl[k] = slo[c & ~REF_LEAF];
-        bh[k] = shi[c & ~REF_LEAF];
-        const float ct = cost_tri * box_half_area(bl[k], bh[k]);
-#pragma unroll
-        for (int s = 0; s < 8; s++) cc[k][s] = ct;
-      } else {
-        bl[k] = nlo[c];
-        bh[k] = nhi[c];
-        const float4* t = reinterpret_cast<const float4*>(cost8 + 8 * (size_t)c);
-        const float4 t0 = t[0], t1 = t[1];
-        cc[k][0] = t0.x; cc[k][1] = t0.y; cc[k][2] = t0.z; cc[k][3] = t0.w;
-        cc[k][4] = t1.x; cc[k][5] = t1.y; cc[k][6] = t1.z; cc[k][7] = t1.w;
-      }
-    }
-    const float4 l = make_float4(fminf(bl[0].x, bl[1].x), fminf(bl[0].y, bl[1].y), fminf(bl[0].z, bl[1].z), 0.0f);
-    const float4 h = make_float4(fmaxf(bh[0].x, bh[1].x), fmaxf(bh[0].y, bh[1].y), fmaxf(bh[0].z, bh[1].z), 0.0f);
-    const uint2 r = range[node];
-    const uint32_t count = r.y - r.x + 1;
-    const float A = box_half_area(l, h);
-    float d[9];
-    uint32_t dj[9];
-#pragma unroll
-    for (int s = 2; s <= 8; s++) {
-      float best = INFINITY;
-      uint32_t bj = 1;
-#pragma unroll
-      for (int jj = 1; jj < s; jj++) {
-        const float v = cc[0][jj - 1] + cc[1][s - jj - 1];
-        if (v < best) { best = v; bj = (uint32_t)jj; }
-      }
-      d[s] = best;
-      dj[s] = bj;
-    }
-    // a leaf pays per block of `block` primitives (the reference's
-    // sahBlockSize); subtrees of at most min_leaf primitives are leaves
-    const float leafc = count <= max_leaf ? cost_tri * (float)((count + block - 1) / block) * A : INFINITY;
-    const float inner = count <= min_leaf ? INFINITY : cost_node * A + d[width];  // at most `width` children
-    float c[8];
-    uint32_t sp[8];
-    c[0] = fminf(leafc, inner);
-    sp[0] = 0;
-#pragma unroll
-    for (int s = 2; s <= 8; s++) {
-      if (d[s] < c[s - 2]) { c[s - 1] = d[s]; sp[s - 1] = dj[s]; }
-      else { c[s - 1] = c[s - 2]; sp[s - 1] = 0x80u | dj[s]; }
-    }
-    nlo[node] = l;
-    nhi[node] = h;
-    float4* t = reinterpret_cast<float4*>(cost8 + 8 * (size_t)node);
-    t[0] = make_float4(c[0], c[1], c[2], c[3]);
-    t[1] = make_float4(c[4], c[5], c[6], c[7]);
-    uint2* sp8 = reinterpret_cast<uint2*>(split8 + 8 * (size_t)node);
-    *sp8 = make_uint2(sp[0] | (sp[1] << 8) | (sp[2] << 16) | (sp[3] << 24),
-                      sp[4] | (sp[5] << 8) | (sp[6] << 16) | (sp[7] << 24));
-    isleaf[node] = leafc <= inner ? 1 : 0;
+    if (!refit_node(a, node)) return;
     node = parent_int[node];
   }
+}
+
+// Refit of one depth level of a tree whose inner nodes know their depth:
+// the deeper levels are complete (earlier launches), so no thread waits on
+// another and no cross-XCD hand-off is needed.
+__global__ __launch_bounds__(BT) void k_refit_level(RefitArgs a, const uint32_t* __restrict__ order, uint32_t count,
+                                                    uint32_t* visit) {
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  if (i >= count) return;
+  const uint32_t node = order[i];
+  if (node + 1 >= a.n) {
+    atomicOr(a.err, 1u);
+    return;
+  }
+  if (refit_node(a, node)) visit[node] = 2u;  // what the tree check expects of a refitted node
+}
+
+// Inner nodes grouped by depth: a histogram, then a scatter into the
+// host-computed level offsets (order within a level does not matter).
+__global__ __launch_bounds__(BT) void k_depth_hist(const uint8_t* __restrict__ depth, uint32_t m,
+                                                   uint32_t* __restrict__ hist) {
+  __shared__ uint32_t h[256];
+  h[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  if (i < m) atomicAdd(&h[depth[i]], 1u);
+  __syncthreads();
+  if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// One global atomic per (block, depth): local ranks from LDS, then a
+// block-wide reservation per depth (all threads of a level would otherwise
+// contend on one address).
+__global__ __launch_bounds__(BT) void k_depth_scatter(const uint8_t* __restrict__ depth, uint32_t m,
+                                                      uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
+  __shared__ uint32_t h[256], base[256];
+  h[threadIdx.x] = 0u;
+  __syncthreads();
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  const uint32_t d = i < m ? depth[i] : 255u;  // 255: no node, or an id no tree uses
+  const uint32_t local = d != 255u ? atomicAdd(&h[d], 1u) : 0u;
+  __syncthreads();
+  if (h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);
+  __syncthreads();
+  if (d != 255u) order[base[d] + local] = i;
 }
 
 // Integrity check of the sorted keys and the radix tree (cheap, one pass):
@@ -890,6 +971,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   S.split8.reserve((size_t)ninner * 8);
   S.isleaf.reserve((size_t)ninner);
   S.visit.reserve((size_t)ninner * 4);
+  S.depth.reserve((size_t)ninner);
   std::vector<uint32_t> root_ref(nscenes, REF_EMPTY);
   for (uint32_t sI = 0; sI < nscenes; sI++)
     if (count[sI] == 1) root_ref[sI] = first[sI] | REF_LEAF;
@@ -901,8 +983,8 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     if (use_sah) {
       SahTreeOut so;
       build_sah_tree(n, S.vals.as<uint32_t>(), S.slo.as<float4>(), S.shi.as<float4>(), trees, S.child.as<uint32_t>(),
-                     S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(), prm.learned,
-                     prm.median_below, S.sah, so, s);
+                     S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(),
+                     S.depth.as<uint8_t>(), prm.learned, prm.median_below, S.sah, so, s);
       for (size_t k = 0; k < trees.size(); k++) root_ref[tree_scene[k]] = so.roots[k];
       // boxes in the final primitive order
       hipLaunchKernelGGL(k_gather_sorted, dim3(div_up(n, BT)), dim3(BT), 0, s, S.vals.as<uint32_t>(),
@@ -913,11 +995,39 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
                          S.range.as<uint2>());
       root_ref[0] = 0;
     }
-    hipLaunchKernelGGL(k_refit, dim3(div_up(n, BT)), dim3(BT), 0, s, n, S.child.as<uint32_t>(),
-                       S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(), S.range.as<uint2>(),
-                       S.slo.as<float4>(), S.shi.as<float4>(), S.nlo.as<float4>(), S.nhi.as<float4>(),
-                       S.cost.as<float>(), S.split8.as<uint8_t>(), S.isleaf.as<uint8_t>(), S.visit.as<uint32_t>(),
-                       prm.cost_node, prm.cost_tri, max_leaf, width, min_leaf, block, &cnt->error);
+    RefitArgs ra{n, S.child.as<uint32_t>(), S.range.as<uint2>(), S.slo.as<float4>(), S.shi.as<float4>(),
+                 S.nlo.as<float4>(), S.nhi.as<float4>(), S.cost.as<float>(), S.split8.as<uint8_t>(),
+                 S.isleaf.as<uint8_t>(), prm.cost_node, prm.cost_tri, max_leaf, width, min_leaf, block, &cnt->error};
+    if (use_sah) {
+      // the binned-SAH builder recorded every inner node's depth: refit one
+      // level per launch, deepest first
+      const uint32_t m = n - 1;
+      S.lvl_hist.reserve(2 * 256 * 4);
+      S.lvl_order.reserve((size_t)m * 4);
+      uint32_t* hist = S.lvl_hist.as<uint32_t>();
+      RTG_HIP(hipMemsetAsync(hist, 0, 2 * 256 * 4, s));
+      hipLaunchKernelGGL(k_depth_hist, dim3(div_up(m, BT)), dim3(BT), 0, s, S.depth.as<uint8_t>(), m, hist);
+      std::vector<uint32_t> h(256);
+      RTG_HIP(hipMemcpyAsync(h.data(), hist, 256 * 4, hipMemcpyDeviceToHost, s));
+      RTG_HIP(hipStreamSynchronize(s));
+      std::vector<uint32_t> off(256, 0);
+      uint32_t total = 0;
+      for (int d = 0; d < 255; d++) {
+        off[d] = total;
+        total += h[d];
+      }
+      RTG_HIP(hipMemcpyAsync(hist + 256, off.data(), 256 * 4, hipMemcpyHostToDevice, s));
+      hipLaunchKernelGGL(k_depth_scatter, dim3(div_up(m, BT)), dim3(BT), 0, s, S.depth.as<uint8_t>(), m, hist + 256,
+                         S.lvl_order.as<uint32_t>());
+      for (int d = 254; d >= 0; d--)
+        if (h[d])
+          hipLaunchKernelGGL(k_refit_level, dim3(div_up(h[d], BT)), dim3(BT), 0, s, ra,
+                             S.lvl_order.as<uint32_t>() + off[d], h[d], S.visit.as<uint32_t>());
+      RTG_HIP(hipStreamSynchronize(s));  // `off` is read by the copy above
+    } else {
+      hipLaunchKernelGGL(k_refit, dim3(div_up(n, BT)), dim3(BT), 0, s, ra, S.parent_int.as<uint32_t>(),
+                         S.parent_leaf.as<uint32_t>(), S.visit.as<uint32_t>());
+    }
     hipLaunchKernelGGL(k_check_tree, dim3(div_up(n, BT)), dim3(BT), 0, s,
                        use_sah ? nullptr : S.keys.as<uint32_t>(), n, &cnt->roots_seen,
                        S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),

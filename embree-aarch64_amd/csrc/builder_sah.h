@@ -42,7 +42,7 @@ struct SahTreeOut {
 // median position all the way down instead of by binned SAH.
 void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi,
                     const std::vector<SahRoot>& trees, uint32_t* child, uint32_t* parent_int,
-                    uint32_t* parent_leaf, uint2* range, const LearnedCost* learned, uint32_t median_below,
-                    SahScratch& S, SahTreeOut& out, hipStream_t s);
+                    uint32_t* parent_leaf, uint2* range, uint8_t* depth, const LearnedCost* learned,
+                    uint32_t median_below, SahScratch& S, SahTreeOut& out, hipStream_t s);
 
 }  // namespace rtg

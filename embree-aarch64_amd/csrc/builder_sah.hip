@@ -42,7 +42,10 @@ constexpr uint32_t NIL = 0xffffffffu;
 constexpr int NB = 32;                  // bins per axis
 constexpr int BIN_WORDS = 3 * NB * 7;   // counts, then lo xyz, then hi xyz, per (axis, bin)
 constexpr uint32_t SMALL_MAX = 64;
-constexpr uint32_t TINY_MAX = 8;        // segments up to this size are finished by one thread (exact SAH)
+#ifndef RTG_TINY_MAX
+#define RTG_TINY_MAX 16
+#endif
+constexpr uint32_t TINY_MAX = RTG_TINY_MAX;        // segments up to this size are finished by one thread (exact SAH)
 constexpr uint32_t MID_MAX = 4096;
 constexpr uint32_t CHUNK = 4096;        // positions per block of the big-segment binning
 constexpr uint32_t MEDIAN_DEPTH = 48;   // from this binary depth on, splits are object medians
@@ -72,7 +75,8 @@ struct Counters {
   uint32_t nseg[4];  // next-level segment counts: big, mid, small, tiny
   uint32_t max_depth;
   uint32_t error;    // bit 0: a segment list overflowed, bit 1: bin counts disagree, bit 2: destination out of range,
-                     // bit 3: median subtree stack overflow, bit 4: a tiny segment out of range
+                     // bit 3: median subtree stack overflow, bit 4: a tiny segment out of range,
+                     // bit 5: a depth beyond 254
 };
 
 struct Lists {
@@ -89,7 +93,13 @@ struct Tree {
   uint32_t* vals;
   Counters* cnt;
   uint32_t* roots;  // per tree
+  uint8_t* depth;   // binary depth of each inner node (0xff: unused id)
 };
+
+__device__ inline void set_depth(const Tree& t, uint32_t m, uint32_t d) {
+  if (d > 254u) atomicOr(&t.cnt->error, 32u);
+  t.depth[m] = (uint8_t)(d > 254u ? 254u : d);
+}
 
 struct Mapping {
   float ofs[3], scale[3];
@@ -217,6 +227,7 @@ __device__ inline void link_inner(const Tree& t, const Seg& sg, uint32_t m) {
   }
   t.range[m] = make_uint2(sg.first, sg.first + sg.count - 1);
   atomicMax(&t.cnt->max_depth, sg.meta >> 1);
+  set_depth(t, m, sg.meta >> 1);
 }
 
 // The whole subtree of a small segment [first, first+count) (child `side`
@@ -238,6 +249,7 @@ __device__ inline void finish_median(const Tree& t, uint32_t m, uint32_t side, u
     t.parent_int[mid] = p;
     t.range[mid] = make_uint2(f, f + c - 1);
     atomicMax(&t.cnt->max_depth, d);
+    set_depth(t, mid, d);
     const uint32_t cf[2] = {f, f + nl}, cc[2] = {nl, c - nl};
 #pragma unroll
     for (int s = 0; s < 2; s++) {
@@ -684,7 +696,10 @@ __global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, con
 
 // ------------------------------------------------------------ tiny segments --
 
-constexpr int TT = 64;  // threads per block of k_tiny, one segment each
+#ifndef RTG_TINY_THREADS
+#define RTG_TINY_THREADS 32
+#endif
+constexpr int TT = RTG_TINY_THREADS;  // threads per block of k_tiny, one segment each
 
 // Stable insertion sort of entries [f, f+c) by centroid (lo + hi) on axis a.
 __device__ inline void tiny_sort(float4* lo, float4* hi, uint32_t* id, uint32_t f, uint32_t c, int a) {
@@ -785,6 +800,7 @@ __global__ __launch_bounds__(TT) void k_tiny(const Seg* __restrict__ segs, uint3
     t.parent_int[mid] = p;
     t.range[mid] = make_uint2(sg.first + f, sg.first + f + c - 1);
     atomicMax(&t.cnt->max_depth, depth);
+    set_depth(t, mid, depth);
     if (sp + 2 > (int)TINY_MAX) {
       atomicOr(&t.cnt->error, 16u);
       return;
@@ -805,8 +821,8 @@ __global__ void k_fill_seg_ids(uint32_t* sid, const Seg* segs) {
 
 void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4* shi,
                     const std::vector<SahRoot>& trees, uint32_t* child, uint32_t* parent_int,
-                    uint32_t* parent_leaf, uint2* range, const LearnedCost* learned, uint32_t fin_max,
-                    SahScratch& S, SahTreeOut& out, hipStream_t s) {
+                    uint32_t* parent_leaf, uint2* range, uint8_t* depth, const LearnedCost* learned,
+                    uint32_t fin_max, SahScratch& S, SahTreeOut& out, hipStream_t s) {
   if (fin_max > SMALL_MAX) fin_max = SMALL_MAX;
   LearnedCost lcost{};
   if (learned) lcost = *learned;
@@ -837,6 +853,7 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
   hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n - 1, TBS)), dim3(TBS), 0, s, parent_int, (size_t)(n - 1), REF_EMPTY);
   hipLaunchKernelGGL(k_fill_u32, dim3(div_up(trees.size(), TBS)), dim3(TBS), 0, s, S.roots.as<uint32_t>(),
                      trees.size(), NIL);
+  RTG_HIP(hipMemsetAsync(depth, 0xff, (size_t)(n - 1), s));
 
   std::vector<Seg> init[3];
   for (size_t i = 0; i < trees.size(); i++) {
@@ -868,7 +885,7 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
   RTG_HIP(hipMemcpyAsync(cnt, &hc, sizeof(hc), hipMemcpyHostToDevice, s));
   RTG_HIP(hipStreamSynchronize(s));  // the host staging above must outlive its copies
 
-  Tree t{child, parent_int, parent_leaf, range, vals, cnt, S.roots.as<uint32_t>()};
+  Tree t{child, parent_int, parent_leaf, range, vals, cnt, S.roots.as<uint32_t>(), depth};
   int par = 0;
   uint32_t levels = 0;
   while (cur[0] + cur[1] + cur[2] > 0) {
