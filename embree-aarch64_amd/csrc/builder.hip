@@ -27,6 +27,7 @@ struct BuildCounters {
   uint32_t error;  // bit 0: refit walk too long, bit 1: collapse index out of range, bit 2: bad tree
   uint32_t max_stack;  // most traversal stack entries any root-to-node path can need
   uint32_t roots_seen; // tree roots met by the tree check
+  uint32_t deep;       // a radix-tree node deeper than the level refit handles
   double sah_nodes, sah_leaves, sah_leaves_tri;
 };
 
@@ -96,6 +97,7 @@ __global__ void k_init_counters(BuildCounters* c, uint32_t nscenes) {
   c->valid = 0;
   c->node_count = nscenes;
   c->roots_seen = 0;
+  c->deep = 0;
   c->tri_count = 0;
   c->qcount[0] = c->qcount[1] = 0;
   c->max_depth = 0;
@@ -503,6 +505,26 @@ __global__ __launch_bounds__(BT) void k_refit_level(RefitArgs a, const uint32_t*
     return;
   }
   if (refit_node(a, node)) visit[node] = 2u;  // what the tree check expects of a refitted node
+}
+
+// Depth of every inner node of a radix tree (its ancestors counted by
+// walking the parent links); depths past 254 raise *deep and the caller
+// falls back to the bottom-up refit.
+__global__ __launch_bounds__(BT) void k_node_depth(const uint32_t* __restrict__ parent_int, uint32_t m,
+                                                   uint8_t* __restrict__ depth, uint32_t* deep) {
+  const uint32_t i = blockIdx.x * BT + threadIdx.x;
+  if (i >= m) return;
+  uint32_t d = 0, p = parent_int[i];
+  while (p != NONE && d < 255u) {
+    if (p >= m) {  // a broken link: the tree check reports it
+      d = 255u;
+      break;
+    }
+    d++;
+    p = parent_int[p];
+  }
+  if (d > 254u) atomicOr(deep, 1u);
+  depth[i] = (uint8_t)(d > 254u ? 254u : d);
 }
 
 // Inner nodes grouped by depth: a histogram, then a scatter into the
@@ -998,9 +1020,19 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     RefitArgs ra{n, S.child.as<uint32_t>(), S.range.as<uint2>(), S.slo.as<float4>(), S.shi.as<float4>(),
                  S.nlo.as<float4>(), S.nhi.as<float4>(), S.cost.as<float>(), S.split8.as<uint8_t>(),
                  S.isleaf.as<uint8_t>(), prm.cost_node, prm.cost_tri, max_leaf, width, min_leaf, block, &cnt->error};
-    if (use_sah) {
-      // the binned-SAH builder recorded every inner node's depth: refit one
-      // level per launch, deepest first
+    // every inner node's depth (recorded by the binned-SAH builder, counted
+    // for the radix tree): refit one level per launch, deepest first
+    bool by_levels = true;
+    if (!use_sah) {
+      uint32_t* deep = &cnt->deep;
+      hipLaunchKernelGGL(k_node_depth, dim3(div_up(n - 1, BT)), dim3(BT), 0, s, S.parent_int.as<uint32_t>(), n - 1,
+                         S.depth.as<uint8_t>(), deep);
+      uint32_t hdeep = 0;
+      RTG_HIP(hipMemcpyAsync(&hdeep, deep, 4, hipMemcpyDeviceToHost, s));
+      RTG_HIP(hipStreamSynchronize(s));
+      by_levels = hdeep == 0;
+    }
+    if (by_levels) {
       const uint32_t m = n - 1;
       S.lvl_hist.reserve(2 * 256 * 4);
       S.lvl_order.reserve((size_t)m * 4);
