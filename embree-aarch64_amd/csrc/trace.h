@@ -19,6 +19,11 @@ struct TraceParams {
   // gave 3795 Mrays/s occlusion (48: 3339).
   int refill_min = 10;      // closest hit
   int refill_min_occ = 24;  // occlusion rays, which finish sooner
+  // Persistent kernel with a separate triangle cursor per lane: 0 never,
+  // 1 always, 2 for occlusion only. On the 2^20-triangle soup with 2^24
+  // incoherent rays it gave 3934 vs 3789 Mrays/s occlusion but 1724 vs 1939
+  // closest hit (speculative node visits before a pending leaf's hits).
+  int dual = 2;
   int cus = 256;            // compute units (set from the device)
   int blocks_per_cu = 8;    // resident 256-thread blocks per compute unit for the persistent grid
 };

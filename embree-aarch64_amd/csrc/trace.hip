@@ -750,6 +750,183 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
   }
 }
 
+// Persistent traversal with a second cursor for triangles: a lane keeps
+// one leaf in progress (`tri`) next to its node cursor (`cur`), and each
+// loop iteration does one node step and one triangle step, so the wave's
+// triangle branch no longer runs while those lanes' node work waits (and the
+// other way round). A leaf met while a leaf is still in progress waits in
+// `cur`. Node entries are popped after the triangle step, so the cull test
+// sees the newest hit distance; nodes tested while a leaf is pending use the
+// older one (speculative; hits do not depend on it).
+template <bool OCCLUDED, bool ALIGNED, bool COUNT, bool PL>
+__global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, char* __restrict__ rays, uint32_t M,
+                                                                    uint64_t stride, uint32_t inst_id,
+                                                                    uint32_t* __restrict__ cnt_nodes,
+                                                                    uint32_t* __restrict__ cnt_tris,
+                                                                    uint32_t* __restrict__ err,
+                                                                    const uint32_t* __restrict__ perm,
+                                                                    uint32_t* __restrict__ next, int refill_min,
+                                                                    uint32_t* node_visits) {
+  constexpr uint32_t NONE_REF = 0xffffffffu;
+  const int lane = threadIdx.x & 63;
+  const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  __shared__ uint64_t lds_stack[8 * TB];
+  uint64_t spill_mem[STACK];
+  RingStack<8> stack;
+  stack.init(lds_stack + threadIdx.x, spill_mem);
+
+  TRay r;
+  Hit best;
+  float lim = 0.0f;
+  uint32_t cur = NONE_REF, tri = NONE_REF, ray = 0, nvis = 0, ntri = 0;
+  bool active = false, occluded = false, pool_empty = false;
+  WavePool pool;
+  const uint64_t max_iter = (uint64_t)sc.num_nodes + sc.num_tris + 16;
+  uint64_t iter = 0;
+
+  while (true) {
+    if (!pool_empty) {
+      const uint64_t idle = __ballot(!active);
+      const int nidle = __popcll(idle);
+      if (nidle >= refill_min) {
+        const uint32_t slot = pool.take(next, M, nidle, (uint32_t)__popcll(idle & lt), lane);
+        pool_empty = pool.empty();
+        if (!active && slot != 0xffffffffu) {
+          ray = perm ? perm[slot] : slot;
+          float4 a, b;
+          float tfar;
+          load_ray<ALIGNED>(rays + (uint64_t)ray * stride, a, b, tfar);
+          if (a.w <= tfar) {
+            setup_ray(r, a, b, tfar);
+            best.found = false;
+            best.t = INFINITY;
+            best.prim = best.geom = 0xffffffffu;
+            best.u = best.v = best.ngx = best.ngy = best.ngz = 0.0f;
+            lim = widen_up(tfar);
+            cur = sc.root;
+            tri = NONE_REF;
+            stack.init(lds_stack + threadIdx.x, spill_mem);
+            occluded = false;
+            nvis = ntri = 0;
+            iter = 0;
+            active = true;
+          } else if (COUNT) {
+            cnt_nodes[ray] = 0;
+            cnt_tris[ray] = 0;
+          }
+        }
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (pool_empty) break;
+      continue;
+    }
+    if (!active) continue;
+
+    bool done = false;
+    if (++iter > max_iter) {
+      atomicOr(err, 2u);
+      done = true;
+    }
+    // a waiting leaf starts once the triangle cursor is free
+    if (!done && tri == NONE_REF && cur != NONE_REF && (cur & REF_LEAF)) {
+      tri = cur & ~REF_LEAF;
+      cur = NONE_REF;
+    }
+    // node step
+    if (!done && cur != NONE_REF && !(cur & REF_LEAF)) {
+      if (cur >= sc.num_nodes) {
+        atomicOr(err, 4u);
+        done = true;
+      } else {
+        NodeHit h;
+        intersect_node(sc.nodes, cur, r, lim, h);
+        if (COUNT) {
+          nvis++;
+          if (node_visits) atomicAdd(&node_visits[cur], 1u);
+        }
+        if (h.mask) {
+          if (OCCLUDED) {
+            const int c0 = __builtin_ctz(h.mask);
+            uint32_t rem = h.mask & (h.mask - 1u);
+            while (rem) {
+              const int c = __builtin_ctz(rem);
+              rem &= rem - 1u;
+              if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
+            }
+            cur = child_ref(h, c0);
+          } else {
+            cur = push_near_to_far(h, stack, err);
+          }
+        } else {
+          cur = NONE_REF;
+        }
+      }
+    }
+    // triangle step
+    if (!done && tri != NONE_REF) {
+      if (tri >= sc.num_tris) {
+        atomicOr(err, 4u);
+        done = true;
+      } else {
+        TriTest h;
+        const bool valid = tri_test<PL>(r, sc.tris + tri, h);
+        if (COUNT) ntri++;
+        if (valid) {
+          if (OCCLUDED) {
+            occluded = true;
+          } else {
+            const bool better = !best.found || h.t < best.t ||
+                                (h.t == best.t && (h.geom < best.geom || (h.geom == best.geom && h.prim < best.prim)));
+            if (better) {
+              best.found = true;
+              best.t = h.t;
+              best.u = h.un / h.dn;
+              best.v = h.vn / h.dn;
+              best.ngx = h.ngx; best.ngy = h.ngy; best.ngz = h.ngz;
+              best.prim = h.prim; best.geom = h.geom;
+              lim = widen_up(fminf(best.t, r.tfar));
+            }
+          }
+        }
+        if (OCCLUDED && occluded) done = true;
+        else tri = (h.flags & TRI_LAST) ? NONE_REF : tri + 1u;
+      }
+    }
+    // the node cursor takes the next entry that can still hold a closer hit
+    if (!done && cur == NONE_REF) {
+      while (!stack.empty()) {
+        const uint2 e = stack.pop();
+        if (OCCLUDED || __uint_as_float(e.y) <= lim) {
+          cur = e.x;
+          break;
+        }
+      }
+      if (cur == NONE_REF && tri == NONE_REF) done = true;
+    }
+    if (done) {
+      char* rp = rays + (uint64_t)ray * stride;
+      if (COUNT) { cnt_nodes[ray] = nvis; cnt_tris[ray] = ntri; }
+      if (OCCLUDED) {
+        if (occluded) *reinterpret_cast<float*>(rp + 32) = -INFINITY;
+      } else if (best.found) {
+        float* f = reinterpret_cast<float*>(rp);
+        uint32_t* u = reinterpret_cast<uint32_t*>(rp);
+        f[8] = best.t;
+        f[12] = best.ngx;
+        f[13] = best.ngy;
+        f[14] = best.ngz;
+        f[15] = best.u;
+        f[16] = best.v;
+        u[17] = best.prim;
+        u[18] = best.geom;
+        u[19] = inst_id;
+      }
+      active = false;
+    }
+  }
+}
+
 // ------------------------------------------------------------- two-level --
 
 // Two-level traversal of a scene with instances. The top tree's leaves are
@@ -1025,11 +1202,15 @@ struct LaunchArgs {
   dim3 grid, pgrid;
   int lds_slots;
   hipStream_t s;
+  bool dual;
 };
 
 template <bool OCC, bool AL, bool CNT, int SL, bool PL>
 void launch_one(const LaunchArgs& a) {
-  if (a.persistent)
+  if (a.persistent && a.dual && SL == 8)
+    hipLaunchKernelGGL((k_trace_d<OCC, AL, CNT, PL>), a.pgrid, dim3(TB), 0, a.s, a.sc, a.rays, a.M, a.stride,
+                       a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm, a.counter, a.refill, a.node_visits);
+  else if (a.persistent)
     hipLaunchKernelGGL((k_trace_p<OCC, AL, CNT, SL, PL>), a.pgrid, dim3(TB), 0, a.s, a.sc, a.rays, a.M, a.stride,
                        a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm, a.counter, a.refill, a.node_visits,
                        nullptr, 0u);
@@ -1088,7 +1269,7 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
   }
   LaunchArgs la{sc, rp, M, stride, inst_id, cnt_nodes, cnt_tris, err, perm,
                 persistent ? sorter->counter.as<uint32_t>() : nullptr, refill, count ? node_visits : nullptr,
-                persistent, grid, pgrid, tp.lds_slots, s};
+                persistent, grid, pgrid, tp.lds_slots, s, tp.dual == 1 || (tp.dual == 2 && occluded)};
   if (sc.robust) launch_pl<true>(la, count, occluded, aligned);
   else launch_pl<false>(la, count, occluded, aligned);
   RTG_HIP(hipGetLastError());
