@@ -42,10 +42,7 @@ constexpr uint32_t NIL = 0xffffffffu;
 constexpr int NB = 32;                  // bins per axis
 constexpr int BIN_WORDS = 3 * NB * 7;   // counts, then lo xyz, then hi xyz, per (axis, bin)
 constexpr uint32_t SMALL_MAX = 64;
-#ifndef RTG_TINY_MAX
-#define RTG_TINY_MAX 16
-#endif
-constexpr uint32_t TINY_MAX = RTG_TINY_MAX;        // segments up to this size are finished by one thread (exact SAH)
+constexpr uint32_t TINY_MAX = 16;        // segments up to this size are finished by one thread (exact SAH)
 constexpr uint32_t MID_MAX = 4096;
 constexpr uint32_t CHUNK = 4096;        // positions per block of the big-segment binning
 constexpr uint32_t MEDIAN_DEPTH = 48;   // from this binary depth on, splits are object medians
@@ -696,10 +693,7 @@ __global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, con
 
 // ------------------------------------------------------------ tiny segments --
 
-#ifndef RTG_TINY_THREADS
-#define RTG_TINY_THREADS 32
-#endif
-constexpr int TT = RTG_TINY_THREADS;  // threads per block of k_tiny, one segment each
+constexpr int TT = 32;  // threads per block of k_tiny, one segment each
 
 // Stable insertion sort of entries [f, f+c) by centroid (lo + hi) on axis a.
 __device__ inline void tiny_sort(float4* lo, float4* hi, uint32_t* id, uint32_t f, uint32_t c, int a) {

@@ -251,24 +251,6 @@ __device__ inline uint32_t child_ref(const NodeHit& h, int c) {
 // slots carry key 0 and sort last.
 template <class Stack>
 __device__ inline uint32_t push_near_to_far(const NodeHit& h, Stack& stack, uint32_t* err) {
-#ifdef RTG_SELECT_SORT
-  int bi = -1;
-  float bv = 0.0f;
-#pragma unroll
-  for (int c = 0; c < 8; c++)
-    if (((h.mask >> c) & 1u) && (bi < 0 || h.tn[c] < bv)) { bi = c; bv = h.tn[c]; }
-  uint32_t rem = h.mask & ~(1u << bi);
-  while (rem) {
-    int mc = -1;
-    float mv = 0.0f;
-#pragma unroll
-    for (int c = 0; c < 8; c++)
-      if (((rem >> c) & 1u) && (mc < 0 || h.tn[c] > mv)) { mc = c; mv = h.tn[c]; }
-    rem &= ~(1u << mc);
-    if (!stack.push(make_uint2(child_ref(h, mc), __float_as_uint(mv)))) atomicOr(err, 1u);
-  }
-  return child_ref(h, bi);
-#else
   uint32_t k[8];
 #pragma unroll
   for (int c = 0; c < 8; c++) {
@@ -299,7 +281,6 @@ __device__ inline uint32_t push_near_to_far(const NodeHit& h, Stack& stack, uint
     }
   }
   return child_ref(h, (int)(near & 7u));
-#endif
 }
 
 struct Hit {
@@ -557,15 +538,13 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
 // with 2^24 incoherent rays: 1655 Mrays/s closest hit against 1585 at 7 (72
 // VGPRs, five of them spilled to scratch in the node step); 5 compiles to the
 // same code as 6. A 16-entry LDS stack (32 KB per block) leaves room for 5.
-#ifndef RTG_TRACE_MIN_BLOCKS
-#define RTG_TRACE_MIN_BLOCKS 6
-#endif
+constexpr int TRACE_MIN_BLOCKS = 6;
 
 // BATCH: the ray slots 0..M-1 are the concatenation of the streams in
 // `items` (sorted by `first`); a lane finds its stream by binary search and
 // starts at that stream's scene root.
 template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool PL, bool BATCH = false>
-__global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
+__global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                                 uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                                 uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
                                                 const uint32_t* __restrict__ perm, uint32_t* __restrict__ next,
@@ -759,7 +738,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : RTG_TRACE_MIN_BLOCKS) void 
 // sees the newest hit distance; nodes tested while a leaf is pending use the
 // older one (speculative; hits do not depend on it).
 template <bool OCCLUDED, bool ALIGNED, bool COUNT, bool PL>
-__global__ __launch_bounds__(TB, RTG_TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, char* __restrict__ rays, uint32_t M,
+__global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, char* __restrict__ rays, uint32_t M,
                                                                     uint64_t stride, uint32_t inst_id,
                                                                     uint32_t* __restrict__ cnt_nodes,
                                                                     uint32_t* __restrict__ cnt_tris,
