@@ -192,7 +192,7 @@ __global__ void k_mlp_samples(const float* __restrict__ info, const uint32_t* __
   // target: log2 of the visit probability over the box's relative half area
   // (the model learns the ray distribution's departure from uniform rays)
   T[i] = log2f(((float)visits[i] + 0.5f) * inv_rays) - 16.0f * f[6];
-  W[i] = 1.0f;
+  W[i] = 1.0f;  // visit-weighted samples (log2 or sqrt of the count) measured worse
 }
 
 // minibatch indices: a multiplicative permutation of a random window

@@ -289,6 +289,41 @@ def test_soup_occlusion_parity(rtc, device, oracle):
     sc.release()
 
 
+@pytest.mark.parametrize("config", ["dual=0", "dual=1", "refill_min=64,refill_min_occ=1,blocks_per_cu=2"])
+def test_traversal_variants_parity(rtc, oracle, config):
+    # every persistent-kernel form (with and without the triangle cursor,
+    # extreme refill thresholds, a small grid) returns the oracle's records
+    dev = rtc.Device(config)
+    v, ix = soup_case(rtc, 20000, 17, h=0.02)
+    sc = rtc.Scene(dev, [(v, ix)]).commit()
+    rh = rtc.gen_random_rays(60000, seed=18)
+    assert_same(trace(rtc, sc, rh), oracle_trace(oracle, [(v, ix)], rh), f"closest hit {config}")
+    rays = np.zeros(len(rh), rtc.RTCRay)
+    for k in rtc.RTCRay.names:
+        rays[k] = rh[k]
+    rays["tfar"] = 0.1
+    assert_same(trace(rtc, sc, rays, occluded=True), oracle_trace(oracle, [(v, ix)], rays, occluded=True),
+                f"occlusion {config}")
+    sc.release()
+    dev.release()
+
+
+def test_builds_are_deterministic(rtc):
+    # the same scene built on fresh devices gives the same tree (node count
+    # and SAH statistic), for both builders
+    v, ix = soup_case(rtc, 200000, 19, h=0.01)
+    for cfg in ("builder=sah", "builder=morton"):
+        stats = set()
+        for _ in range(4):
+            dev = rtc.Device(cfg)
+            sc = rtc.Scene(dev, [(v, ix)]).commit()
+            st = sc.stats()
+            stats.add((st["num_nodes"], st["sah"], st["max_depth"]))
+            sc.release()
+            dev.release()
+        assert len(stats) == 1, (cfg, stats)
+
+
 def test_brute_force_small(rtc, device, oracle):
     v, ix = soup_case(rtc, 3000, 9, h=0.05)
     sc = rtc.Scene(device, [(v, ix)]).commit()
