@@ -136,10 +136,17 @@ def learned_vs_sah(grid, width, height, bounces, train_steps, steps):
     build_learned = learned.stats()["build_ms"]
     res = {}
     outs = {}
+    # the two trees are timed alternately, three times each, so that clock
+    # drift over the run reaches both alike; each keeps its median
+    times = {"plain": [], "learned": []}
+    for name, sc in (("plain", plain), ("learned", learned)):
+        time_trace(L, dev, sc, d_rays, d_src, n, 2)
+    for _ in range(3):
+        for name, sc in (("plain", plain), ("learned", learned)):
+            times[name].append(time_trace(L, dev, sc, d_rays, d_src, n, steps))
     for name, sc in (("plain", plain), ("learned", learned)):
         nb, tb, outs[name] = counted(L, dev, sc, d_rays, d_src, n, nbytes)
-        time_trace(L, dev, sc, d_rays, d_src, n, 2)
-        ms = time_trace(L, dev, sc, d_rays, d_src, n, steps)
+        ms = float(np.median(times[name]))
         st = sc.stats()
         res[name] = {"mrays_per_s": round(n / (ms / 1e3) / 1e6, 2), "ms": round(ms, 3), "nodes_per_ray": round(nb, 2),
                      "tris_per_ray": round(tb, 2), "bvh_nodes": st["num_nodes"], "sah": round(st["sah"], 3)}

@@ -212,6 +212,7 @@ struct SplitModelH : RefObj {
   Device* dev;
   rtg::SplitModel m;
   uint32_t min_count = 64;
+  float sah_slack = 0.05f;
   std::mutex mtx;
   explicit SplitModelH(Device* d) : dev(d) { ref_inc(d); }
   ~SplitModelH() override { ref_dec(dev); }
@@ -390,6 +391,7 @@ rtg::BuildParams scene_params(Scene* sc, rtg::LearnedCost& lc) {
     lc.P = sc->model->m.packed.as<rtg::MlpPacked>();
     lc.sn = sc->model->m.norm;
     lc.min_count = sc->model->min_count;
+    lc.sah_slack = sc->model->sah_slack;
     p.learned = &lc;
     p.builder = rtg::BUILDER_SAH;
   }
@@ -1790,6 +1792,15 @@ RTC_API void rtcgSetSplitModelMinCount(RTCGSplitModel model, unsigned int min_co
   API_BEGIN
   VERIFY_HANDLE(model);
   m->min_count = min_count < 64 ? 64 : min_count;
+  API_END(m)
+}
+
+RTC_API void rtcgSetSplitModelSahSlack(RTCGSplitModel model, float slack) {
+  SplitModelH* m = SM(model);
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  if (!(slack >= 0.0f)) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "SAH slack must be >= 0");
+  m->sah_slack = slack;
   API_END(m)
 }
 

@@ -304,16 +304,19 @@ __device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m
 // lc.min_count): the workgroup evaluates the model on the left and right
 // boxes of all 3*(nb-1) candidates (their boxes and counts from the bins),
 // then one thread per axis keeps the least P(L)*n_L + P(R)*n_R with
-// P = halfArea * 2^model, the model's value kept within 3 of the segment's
-// own and P within the segment's own P.
+// P = halfArea * 2^model (bounds below) among the candidates whose plain
+// cost A(L)*n_L + A(R)*n_R is within (1 + lc.sah_slack) of the least over
+// all axes.
 // Ties keep the lowest axis and the lowest k; without a separating
 // candidate the split is the object median.
 struct ChooseLds {
   float yp, pp;  // learned log rate and visit weight of the segment's own box
   float cost[3];
+  float amin[3];  // learned: least plain cost per axis
   int k[3];
   uint32_t nl[3];
-  float p[2 * 3 * (NB - 1)];
+  float p[2 * 3 * (NB - 1)];  // learned: visit weight of each candidate's left and right box
+  float a[2 * 3 * (NB - 1)];  // learned: their half areas
   Split sp;
 };
 
@@ -321,6 +324,11 @@ __device__ inline void range_box(const uint32_t* w, int a, int b0, int b1, float
   bins_box(w, a, b0, b1, lo, hi);
   cnt = 0;
   for (int b = b0; b < b1; b++) cnt += w[wc(a, b)];
+}
+
+// Plain SAH cost of a candidate from its two half areas and counts.
+__device__ inline float split_area_cost(float al, float ar, uint32_t nl, uint32_t nr) {
+  return al * (float)nl + ar * (float)nr;
 }
 
 template <int BLOCK>
@@ -355,14 +363,29 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
       float lo[3], hi[3];
       uint32_t cnt;
       range_box(w, a, side ? k : 0, side ? m.nb : k, lo, hi, cnt);
-      float pv = -1.0f;
+      float pv = -1.0f, av = -1.0f;
       if (cnt > 0 && m.scale[a] != 0.0f) {
         float f[MLP_IN];
         box_features(lo, hi, cnt, lc.sn, f);
         const float y = fminf(fmaxf(mlp_eval(lc.P, f), yp - 1.0f), yp + 1.0f);
-        pv = fminf(half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) * exp2f(y), pp);
+        av = half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
+        pv = fminf(av * exp2f(y), pp);
       }
       L.p[e] = pv;
+      L.a[e] = av;
+    }
+    __syncthreads();
+    if (tid < 3) {
+      float c = INFINITY;
+      uint32_t run = 0, total = 0;
+      for (int b = 0; b < m.nb; b++) total += w[wc(tid, b)];
+      for (int kk = 1; kk <= ncand; kk++) {
+        run += w[wc(tid, kk - 1)];
+        const float al = L.a[2 * (tid * ncand + kk - 1)], ar = L.a[2 * (tid * ncand + kk - 1) + 1];
+        if (run == 0 || run == total || al < 0.0f || ar < 0.0f) continue;
+        c = fminf(c, split_area_cost(al, ar, run, total - run));
+      }
+      L.amin[tid] = c;
     }
     __syncthreads();
   }
@@ -377,11 +400,14 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
       if (!learned) {
         if (!best_on_axis(w, m, tid, c, k, nl)) c = INFINITY;
       } else if (m.scale[tid] != 0.0f) {
+        const float guard = fminf(fminf(L.amin[0], L.amin[1]), L.amin[2]) * (1.0f + lc.sah_slack);
         uint32_t run = 0;
         for (int kk = 1; kk <= ncand; kk++) {
           run += w[wc(tid, kk - 1)];
-          const float pl = L.p[2 * (tid * ncand + kk - 1)], pr = L.p[2 * (tid * ncand + kk - 1) + 1];
+          const int e = 2 * (tid * ncand + kk - 1);
+          const float pl = L.p[e], pr = L.p[e + 1];
           if (run == 0 || run == total || pl < 0.0f || pr < 0.0f) continue;
+          if (!(split_area_cost(L.a[e], L.a[e + 1], run, total - run) <= guard)) continue;
           const float v = pl * (float)run + pr * (float)(total - run);
           if (v < c) {
             c = v;

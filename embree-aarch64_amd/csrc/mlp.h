@@ -49,6 +49,7 @@ struct LearnedCost {
   const MlpPacked* P;
   SceneNorm sn;
   uint32_t min_count;  // segments with at most this many primitives use plain SAH
+  float sah_slack;     // candidates whose plain SAH cost exceeds (1 + slack) x the best are not considered
 };
 
 // Features of a box holding `count` primitives.

@@ -227,6 +227,7 @@ SIGNATURES = {
     "rtcgSplitModelClearSamples": (None, [_vp]),
     "rtcgSplitModelTrain": (_f32, [_vp, _u32, _u32, _f32, _u32]),
     "rtcgSetSplitModelMinCount": (None, [_vp, _u32]),
+    "rtcgSetSplitModelSahSlack": (None, [_vp, ctypes.c_float]),
     "rtcgSetSceneSplitModel": (None, [_vp, _vp]),
 }
 

@@ -213,6 +213,10 @@ RTC_API void rtcgSplitModelClearSamples(RTCGSplitModel model);
 RTC_API float rtcgSplitModelTrain(RTCGSplitModel model, unsigned int steps, unsigned int batch,
                                   float learningRate, unsigned int seed);
 RTC_API void rtcgSetSplitModelMinCount(RTCGSplitModel model, unsigned int minCount);
+/* The model only chooses among split candidates whose plain binned-SAH cost
+ * is within (1 + slack) of the segment's best (default 0.05; INFINITY lets it
+ * choose freely). Negative or NaN slack is RTC_ERROR_INVALID_ARGUMENT. */
+RTC_API void rtcgSetSplitModelSahSlack(RTCGSplitModel model, float slack);
 
 /* Later commits of the scene use the model's split cost (binned-SAH builder);
  * NULL restores the plain cost. */

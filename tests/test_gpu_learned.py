@@ -115,3 +115,41 @@ def test_collect_and_train(rtc, device, oracle):
         device.free(d)
         m.release()
         sc.release()
+
+
+def test_sah_slack_bounds_the_tree(rtc, device):
+    """The model chooses only among split candidates whose plain binned-SAH
+    cost is within (1 + slack) of the segment's least: slack 0 gives the plain
+    tree back, a small slack keeps the tree's SAH cost near the plain tree's,
+    and a negative or NaN slack is an invalid argument."""
+    import ctypes
+    v, ix, cam = heightfield_case(rtc)
+    sc = rtc.Scene(device, [(v, ix)], quality=2).commit()
+    m = rtc.SplitModel(device, seed=11)
+    d = device.malloc(cam.nbytes)
+    try:
+        device.memcpy(d, cam.ctypes.data, cam.nbytes)
+        m.collect(sc, d, len(cam))
+        m.train(300, batch=4096, lr=3e-3, seed=1)
+        plain = sc.stats()
+        got = {}
+        for slack in (0.0, 0.05, float("inf")):
+            rtc.lib().rtcgSetSplitModelSahSlack(m.h, ctypes.c_float(slack))
+            device.check("slack")
+            s2 = rtc.Scene(device, [(v, ix)])
+            rtc.lib().rtcgSetSceneSplitModel(s2.h, m.h)
+            s2.commit()
+            got[slack] = s2.stats()
+            s2.release()
+        assert got[0.0]["num_nodes"] == plain["num_nodes"]
+        assert abs(got[0.0]["sah"] - plain["sah"]) <= 1e-4 * plain["sah"], (got[0.0]["sah"], plain["sah"])
+        assert got[0.05]["sah"] <= 1.05 * plain["sah"], (got[0.05]["sah"], plain["sah"])
+        for bad in (-0.5, float("nan")):
+            with pytest.raises(rtc.RTCError) as e:
+                rtc.lib().rtcgSetSplitModelSahSlack(m.h, ctypes.c_float(bad))
+                device.check("slack")
+            assert e.value.code == rtc.RTC_ERROR_INVALID_ARGUMENT
+    finally:
+        device.free(d)
+        m.release()
+        sc.release()
