@@ -175,6 +175,7 @@ struct NodeHit {
   uint32_t tri_base;
   uint32_t meta_lo, meta_hi;
   float tn[8];
+  bool hit[8];
 };
 
 // Slab test of the 8 children on planes that decode as fmaf(2^(e-127), q, p)
@@ -230,7 +231,8 @@ __device__ inline void intersect_node(const QNode8* __restrict__ nodes, uint32_t
     const float tn = fmaxf(fmaxf(tnx, tny), fmaxf(tnz, r.tlo));
     const float tf = fminf(fminf(tfx, tfy), fminf(tfz, thi));
     h.tn[c] = tn;
-    mask |= ((tn <= tf) && (mb != 0u)) ? (1u << c) : 0u;
+    h.hit[c] = (tn <= tf) && (mb != 0u);
+    mask |= h.hit[c] ? (1u << c) : 0u;
   }
   h.mask = mask;
 }
@@ -241,23 +243,25 @@ __device__ inline uint32_t child_ref(const NodeHit& h, int c) {
 }
 
 // Closest hit: pushes the hit children except the nearest, farthest first,
-// each with its entry distance, and returns the nearest child.
-// Keys are the entry distances mapped to unsigned order with the slot in the
-// low 3 bits (a distance perturbed by at most 7 ulps, which changes only the
-// order among nearly equal children); the stored distance is the key with
-// those bits cleared, mapped back, which is never above the true entry
-// distance, so the pop test stays conservative. A 19-comparator network
-// (Batcher's odd-even merge sort) orders the keys without branches; hit-free
-// slots carry key 0 and sort last.
+// each with its entry distance, sets `next` to the nearest child and returns
+// whether any child was hit.
+// A key is the entry distance's bit pattern with the slot in the low 3 bits,
+// or 0xffffffff for a child that is not hit. Entry distances are never below
+// the ray's widened tnear, so for tnear >= 0 the bit patterns order like the
+// distances (a negative distance, possible only with tnear < 0, orders as
+// farther than every positive one: a different visiting order, the same
+// result). The perturbation of at most 7 ulps changes only the order among
+// nearly equal children. A 19-comparator network (Batcher's odd-even merge
+// sort) orders the keys decreasingly without branches: children not hit
+// first, the nearest hit child last. The stored distance is the key with its
+// low 3 bits cleared (set for a negative distance), never above the true
+// entry distance, so the pop test stays conservative.
 template <class Stack>
-__device__ inline uint32_t push_near_to_far(const NodeHit& h, Stack& stack, uint32_t* err) {
+__device__ inline bool push_near_to_far(const NodeHit& h, Stack& stack, uint32_t* err, uint32_t& next) {
+  constexpr uint32_t NO_HIT = 0xffffffffu;
   uint32_t k[8];
 #pragma unroll
-  for (int c = 0; c < 8; c++) {
-    const uint32_t b = __float_as_uint(h.tn[c]);
-    const uint32_t o = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
-    k[c] = ((h.mask >> c) & 1u) ? ((o & ~7u) | (uint32_t)c) : 0u;
-  }
+  for (int c = 0; c < 8; c++) k[c] = h.hit[c] ? ((__float_as_uint(h.tn[c]) & ~7u) | (uint32_t)c) : NO_HIT;
   auto ce = [&](int i, int j) {
     const uint32_t hi = max(k[i], k[j]), lo = min(k[i], k[j]);
     k[i] = hi;
@@ -269,18 +273,16 @@ __device__ inline uint32_t push_near_to_far(const NodeHit& h, Stack& stack, uint
   ce(0, 4); ce(1, 5); ce(2, 6); ce(3, 7);
   ce(2, 4); ce(3, 5);
   ce(1, 2); ce(3, 4); ce(5, 6);
-  const int n = __popc(h.mask);
-  uint32_t near = k[0];
+  if (k[7] == NO_HIT) return false;
 #pragma unroll
   for (int j = 0; j < 7; j++) {
-    if (j + 1 < n) {
-      const uint32_t o = k[j] & ~7u;
-      const uint32_t d = (o & 0x80000000u) ? (o & 0x7fffffffu) : ~o;
+    if (k[j] != NO_HIT) {
+      const uint32_t d = (k[j] & 0x80000000u) ? (k[j] | 7u) : (k[j] & ~7u);
       if (!stack.push(make_uint2(child_ref(h, (int)(k[j] & 7u)), d))) atomicOr(err, 1u);
-      near = k[j + 1];
     }
   }
-  return child_ref(h, (int)(near & 7u));
+  next = child_ref(h, (int)(k[7] & 7u));
+  return true;
 }
 
 struct Hit {
@@ -446,8 +448,8 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
         nvis++;
         if (node_visits) atomicAdd(&node_visits[cur], 1u);
       }
-      if (h.mask) {
-        if (OCCLUDED) {
+      if (OCCLUDED) {
+        if (h.mask) {
           const int c0 = __builtin_ctz(h.mask);
           uint32_t rem = h.mask & (h.mask - 1u);
           while (rem) {
@@ -457,10 +459,9 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
           }
           cur = child_ref(h, c0);
           continue;
-        } else {
-          cur = push_near_to_far(h, stack, err);
-          continue;
         }
+      } else if (push_near_to_far(h, stack, err, cur)) {
+        continue;
       }
     } else {
       uint32_t ti = cur & ~REF_LEAF;
@@ -641,8 +642,8 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_tr
           nvis++;
           if (node_visits) atomicAdd(&node_visits[cur], 1u);
         }
-        if (h.mask) {
-          if (OCCLUDED) {
+        if (OCCLUDED) {
+          if (h.mask) {
             const int c0 = __builtin_ctz(h.mask);
             uint32_t rem = h.mask & (h.mask - 1u);
             while (rem) {
@@ -651,9 +652,9 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_tr
               if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
             }
             cur = child_ref(h, c0);
-          } else {
-            cur = push_near_to_far(h, stack, err);
+            continue;  // next step descends into cur
           }
+        } else if (push_near_to_far(h, stack, err, cur)) {
           continue;  // next step descends into cur
         }
       }
@@ -824,8 +825,8 @@ __global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, c
           nvis++;
           if (node_visits) atomicAdd(&node_visits[cur], 1u);
         }
-        if (h.mask) {
-          if (OCCLUDED) {
+        if (OCCLUDED) {
+          if (h.mask) {
             const int c0 = __builtin_ctz(h.mask);
             uint32_t rem = h.mask & (h.mask - 1u);
             while (rem) {
@@ -835,9 +836,9 @@ __global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, c
             }
             cur = child_ref(h, c0);
           } else {
-            cur = push_near_to_far(h, stack, err);
+            cur = NONE_REF;
           }
-        } else {
+        } else if (!push_near_to_far(h, stack, err, cur)) {
           cur = NONE_REF;
         }
       }
@@ -1001,8 +1002,8 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
       } else {
         NodeHit h;
         intersect_node(nodes, cur, r, lim, h);
-        if (h.mask) {
-          if (OCCLUDED) {
+        if (OCCLUDED) {
+          if (h.mask) {
             const int c0 = __builtin_ctz(h.mask);
             uint32_t rem = h.mask & (h.mask - 1u);
             while (rem) {
@@ -1011,9 +1012,9 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
               if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
             }
             cur = child_ref(h, c0);
-          } else {
-            cur = push_near_to_far(h, stack, err);
+            continue;
           }
+        } else if (push_near_to_far(h, stack, err, cur)) {
           continue;
         }
       }
