@@ -41,6 +41,18 @@ typedef __attribute__((address_space(3))) uint64_t lds_u64;
 __device__ inline uint64_t pack_entry(uint2 e) { return (uint64_t)e.x | ((uint64_t)e.y << 32); }
 __device__ inline uint2 unpack_entry(uint64_t v) { return make_uint2((uint32_t)v, (uint32_t)(v >> 32)); }
 
+// Second word of a closest-hit stack entry: the entry distance rounded down
+// to a bfloat16 pattern (high half; -inf for a negative distance) and the
+// least distance of the entries below it (low half), so that a pop can tell
+// when every entry left is farther than the closest hit.
+__device__ inline uint32_t pack_dist(uint32_t d16, float& smin) {
+  const uint32_t y = (d16 << 16) | (__float_as_uint(smin) >> 16);
+  smin = fminf(smin, __uint_as_float(d16 << 16));
+  return y;
+}
+__device__ inline float entry_dist(uint32_t y) { return __uint_as_float(y & 0xffff0000u); }
+__device__ inline float below_dist(uint32_t y) { return __uint_as_float(y << 16); }
+
 template <int LDS_SLOTS>
 struct RingStack {
   lds_u64* lds;    // this lane's slot 0; slot k is lds + k * TB
@@ -72,6 +84,7 @@ struct RingStack {
     return unpack_entry(lds[(sp & (LDS_SLOTS - 1)) * TB]);
   }
   __device__ inline bool empty() const { return sp == 0; }
+  __device__ inline void clear() { sp = lo = 0; }
 };
 
 // LDS_SLOTS == 0: the whole stack is a private array.
@@ -90,6 +103,7 @@ struct RingStack<0> {
   }
   __device__ inline uint2 pop() { return unpack_entry(spill[--sp]); }
   __device__ inline bool empty() const { return sp == 0; }
+  __device__ inline void clear() { sp = 0; }
 };
 
 // Rays of a persistent wave come from a private run of consecutive slots
@@ -253,11 +267,12 @@ __device__ inline uint32_t child_ref(const NodeHit& h, int c) {
 // result). The perturbation of at most 7 ulps changes only the order among
 // nearly equal children. A 19-comparator network (Batcher's odd-even merge
 // sort) orders the keys decreasingly without branches: children not hit
-// first, the nearest hit child last. The stored distance is the key with its
-// low 3 bits cleared (set for a negative distance), never above the true
+// first, the nearest hit child last. The stored distance is the key's high
+// half (rounded down; -inf for a negative distance), never above the true
 // entry distance, so the pop test stays conservative.
 template <class Stack>
-__device__ inline bool push_near_to_far(const NodeHit& h, Stack& stack, uint32_t* err, uint32_t& next) {
+__device__ inline bool push_near_to_far(const NodeHit& h, Stack& stack, float& smin, uint32_t* err,
+                                        uint32_t& next) {
   constexpr uint32_t NO_HIT = 0xffffffffu;
   uint32_t k[8];
 #pragma unroll
@@ -277,8 +292,8 @@ __device__ inline bool push_near_to_far(const NodeHit& h, Stack& stack, uint32_t
 #pragma unroll
   for (int j = 0; j < 7; j++) {
     if (k[j] != NO_HIT) {
-      const uint32_t d = (k[j] & 0x80000000u) ? (k[j] | 7u) : (k[j] & ~7u);
-      if (!stack.push(make_uint2(child_ref(h, (int)(k[j] & 7u)), d))) atomicOr(err, 1u);
+      const uint32_t d16 = (k[j] & 0x80000000u) ? 0xff80u : (k[j] >> 16);
+      if (!stack.push(make_uint2(child_ref(h, (int)(k[j] & 7u)), pack_dist(d16, smin)))) atomicOr(err, 1u);
     }
   }
   next = child_ref(h, (int)(k[7] & 7u));
@@ -425,13 +440,15 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
   uint64_t spill_mem[STACK];
   RingStack<LDS_SLOTS> stack;
   stack.init(lds_stack + threadIdx.x, spill_mem);
+  float smin = INFINITY;  // least entry distance on the stack (closest hit)
   uint32_t cur = sc.root;
   bool occluded = false;
   // every node and triangle record is reached at most once per ray, so the
   // loop runs at most num_nodes + num_tris times; the cap turns a corrupt
   // structure into an error instead of a hang
-  const uint64_t max_iter = (uint64_t)sc.num_nodes + sc.num_tris + 16;
-  uint64_t iter = 0;
+  // (node and triangle counts are below 2^31 each, so the bound fits 32 bits)
+  const uint32_t max_iter = sc.num_nodes + sc.num_tris + 16u;
+  uint32_t iter = 0;
   while (true) {
     if (++iter > max_iter) {
       atomicOr(err, 2u);
@@ -460,7 +477,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
           cur = child_ref(h, c0);
           continue;
         }
-      } else if (push_near_to_far(h, stack, err, cur)) {
+      } else if (push_near_to_far(h, stack, smin, err, cur)) {
         continue;
       }
     } else {
@@ -499,9 +516,16 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
     bool got = false;
     while (!stack.empty()) {
       const uint2 e = stack.pop();
-      if (OCCLUDED || __uint_as_float(e.y) <= lim) {
+      if (!OCCLUDED) smin = below_dist(e.y);
+      if (OCCLUDED || entry_dist(e.y) <= lim) {
         cur = e.x;
         got = true;
+        break;
+      }
+      if (smin > lim) {
+        // every entry left is farther than the closest hit so far
+        stack.clear();
+        smin = INFINITY;
         break;
       }
     }
@@ -557,6 +581,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_tr
   uint64_t spill_mem[STACK];
   RingStack<LDS_SLOTS> stack;
   stack.init(lds_stack + threadIdx.x, spill_mem);
+  float smin = INFINITY;  // least entry distance on the stack (closest hit)
 
   TRay r;
   Hit best;
@@ -567,8 +592,9 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_tr
   bool active = false, occluded = false, pool_empty = false;
   WavePool pool;
   // a ray touches each node and triangle record at most once
-  const uint64_t max_iter = (uint64_t)sc.num_nodes + sc.num_tris + 16;
-  uint64_t iter = 0;
+  // (node and triangle counts are below 2^31 each, so the bound fits 32 bits)
+  const uint32_t max_iter = sc.num_nodes + sc.num_tris + 16u;
+  uint32_t iter = 0;
 
   while (true) {
     // ---- refill idle lanes (wave-uniform decision)
@@ -608,6 +634,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_tr
               lim = widen_up(tfar);
               cur = BATCH ? lroot : sc.root;
               stack.init(lds_stack + threadIdx.x, spill_mem);
+              smin = INFINITY;
               occluded = false;
               nvis = ntri = 0;
               iter = 0;
@@ -654,7 +681,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_tr
             cur = child_ref(h, c0);
             continue;  // next step descends into cur
           }
-        } else if (push_near_to_far(h, stack, err, cur)) {
+        } else if (push_near_to_far(h, stack, smin, err, cur)) {
           continue;  // next step descends into cur
         }
       }
@@ -699,9 +726,16 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_tr
       bool got = false;
       while (!stack.empty()) {
         const uint2 e = stack.pop();
-        if (OCCLUDED || __uint_as_float(e.y) <= lim) {
+        if (!OCCLUDED) smin = below_dist(e.y);
+        if (OCCLUDED || entry_dist(e.y) <= lim) {
           cur = e.x;
           got = true;
+          break;
+        }
+        if (smin > lim) {
+          // every entry left is farther than the closest hit so far
+          stack.clear();
+          smin = INFINITY;
           break;
         }
       }
@@ -754,6 +788,7 @@ __global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, c
   uint64_t spill_mem[STACK];
   RingStack<8> stack;
   stack.init(lds_stack + threadIdx.x, spill_mem);
+  float smin = INFINITY;  // least entry distance on the stack (closest hit)
 
   TRay r;
   Hit best;
@@ -761,8 +796,9 @@ __global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, c
   uint32_t cur = NONE_REF, tri = NONE_REF, ray = 0, nvis = 0, ntri = 0;
   bool active = false, occluded = false, pool_empty = false;
   WavePool pool;
-  const uint64_t max_iter = (uint64_t)sc.num_nodes + sc.num_tris + 16;
-  uint64_t iter = 0;
+  // (node and triangle counts are below 2^31 each, so the bound fits 32 bits)
+  const uint32_t max_iter = sc.num_nodes + sc.num_tris + 16u;
+  uint32_t iter = 0;
 
   while (true) {
     if (!pool_empty) {
@@ -786,6 +822,7 @@ __global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, c
             cur = sc.root;
             tri = NONE_REF;
             stack.init(lds_stack + threadIdx.x, spill_mem);
+            smin = INFINITY;
             occluded = false;
             nvis = ntri = 0;
             iter = 0;
@@ -838,7 +875,7 @@ __global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, c
           } else {
             cur = NONE_REF;
           }
-        } else if (!push_near_to_far(h, stack, err, cur)) {
+        } else if (!push_near_to_far(h, stack, smin, err, cur)) {
           cur = NONE_REF;
         }
       }
@@ -877,8 +914,15 @@ __global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, c
     if (!done && cur == NONE_REF) {
       while (!stack.empty()) {
         const uint2 e = stack.pop();
-        if (OCCLUDED || __uint_as_float(e.y) <= lim) {
+        if (!OCCLUDED) smin = below_dist(e.y);
+        if (OCCLUDED || entry_dist(e.y) <= lim) {
           cur = e.x;
+          break;
+        }
+        if (smin > lim) {
+          // every entry left is farther than the closest hit so far
+          stack.clear();
+          smin = INFINITY;
           break;
         }
       }
@@ -931,6 +975,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
   uint64_t spill_mem[STACK];
   RingStack<8> stack;
   stack.init(lds_stack + threadIdx.x, spill_mem);
+  float smin = INFINITY;  // least entry distance on the stack (closest hit)
 
   TRay r;
   Hit best;
@@ -974,6 +1019,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
               in_obj = false;
               moved = false;
               stack.init(lds_stack + threadIdx.x, spill_mem);
+              smin = INFINITY;
               occluded = false;
               iter = 0;
               active = true;
@@ -1014,7 +1060,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
             cur = child_ref(h, c0);
             continue;
           }
-        } else if (push_near_to_far(h, stack, err, cur)) {
+        } else if (push_near_to_far(h, stack, smin, err, cur)) {
           continue;
         }
       }
@@ -1068,12 +1114,12 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
         const uint4 w = reinterpret_cast<const uint4*>(top.insts + ii)[3];
         const bool identity = (w.z & INST_IDENTITY) != 0u;
         // the rest of a multi-record leaf resumes after this instance
-        if (!(w.z & TRI_LAST) && !stack.push(make_uint2((ii + 1u) | REF_LEAF, __float_as_uint(neg_inf))))
+        if (!(w.z & TRI_LAST) && !stack.push(make_uint2((ii + 1u) | REF_LEAF, pack_dist(0xff80u, smin))))
           atomicOr(err, 1u);
         if ((identity || ctx_inst == 0xffffffffu) && w.y < top.num_objects) {
           const DevScene o = top.objects[w.y];
           if (o.num_nodes > 0) {
-            if (!stack.push(make_uint2(REF_INST_EXIT, __float_as_uint(neg_inf)))) atomicOr(err, 1u);
+            if (!stack.push(make_uint2(REF_INST_EXIT, pack_dist(0xff80u, smin)))) atomicOr(err, 1u);
             if (!identity) {
               // columns vx = m[0..2], vy = m[3..5], vz = m[6..8], p = m[9..11]
               const float ox = __builtin_fmaf(r.ox, m0.x, __builtin_fmaf(r.oy, m0.w, __builtin_fmaf(r.oz, m1.z, m2.y)));
@@ -1104,6 +1150,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
       bool got = false;
       while (!stack.empty()) {
         const uint2 e = stack.pop();
+        if (!OCCLUDED) smin = below_dist(e.y);
         if (e.x == REF_INST_EXIT) {
           in_obj = false;
           if (moved) {
@@ -1115,9 +1162,15 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
           }
           continue;
         }
-        if (OCCLUDED || __uint_as_float(e.y) <= lim) {
+        if (OCCLUDED || entry_dist(e.y) <= lim) {
           cur = e.x;
           got = true;
+          break;
+        }
+        if (smin > lim) {
+          // every entry left is farther than the closest hit so far
+          stack.clear();
+          smin = INFINITY;
           break;
         }
       }
