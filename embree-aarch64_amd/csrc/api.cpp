@@ -315,6 +315,7 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "persistent") d->trace.persistent = atoi(v.c_str());
     else if (k == "refill_min") d->trace.refill_min = atoi(v.c_str());
     else if (k == "refill_min_occ") d->trace.refill_min_occ = atoi(v.c_str());
+    else if (k == "refill_min_occ_inst") d->trace.refill_min_occ_inst = atoi(v.c_str());
     else if (k == "blocks_per_cu") d->trace.blocks_per_cu = atoi(v.c_str());
     else if (k == "dual") d->trace.dual = atoi(v.c_str());
     // threads, isa, tri_accel, tri_builder, set_affinity, ... are CPU settings: accepted, ignored

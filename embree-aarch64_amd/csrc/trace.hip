@@ -972,6 +972,9 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   __shared__ uint64_t lds_stack[8 * TB];
+  // the world-space ray and its reciprocals while the lane is in a moved
+  // object (restored from here on the way out, not reloaded and divided again)
+  __shared__ float4 lds_world[3 * TB];
   uint64_t spill_mem[STACK];
   RingStack<8> stack;
   stack.init(lds_stack + threadIdx.x, spill_mem);
@@ -1121,6 +1124,9 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
           if (o.num_nodes > 0) {
             if (!stack.push(make_uint2(REF_INST_EXIT, pack_dist(0xff80u, smin)))) atomicOr(err, 1u);
             if (!identity) {
+              lds_world[threadIdx.x] = make_float4(r.ox, r.oy, r.oz, r.dx);
+              lds_world[TB + threadIdx.x] = make_float4(r.dy, r.dz, r.rnx, r.rny);
+              lds_world[2 * TB + threadIdx.x] = make_float4(r.rnz, r.rfx, r.rfy, r.rfz);
               // columns vx = m[0..2], vy = m[3..5], vz = m[6..8], p = m[9..11]
               const float ox = __builtin_fmaf(r.ox, m0.x, __builtin_fmaf(r.oy, m0.w, __builtin_fmaf(r.oz, m1.z, m2.y)));
               const float oy = __builtin_fmaf(r.ox, m0.y, __builtin_fmaf(r.oy, m1.x, __builtin_fmaf(r.oz, m1.w, m2.z)));
@@ -1154,10 +1160,14 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
         if (e.x == REF_INST_EXIT) {
           in_obj = false;
           if (moved) {
-            float4 a, b;
-            float tf;
-            load_ray<ALIGNED>(rays + (uint64_t)ray * stride, a, b, tf);
-            setup_ray(r, a, b, r.tfar);
+            const float4 a = lds_world[threadIdx.x], b = lds_world[TB + threadIdx.x];
+            const float4 c = lds_world[2 * TB + threadIdx.x];
+            r.ox = a.x; r.oy = a.y; r.oz = a.z; r.dx = a.w;
+            r.dy = b.x; r.dz = b.y; r.rnx = b.z; r.rny = b.w;
+            r.rnz = c.x; r.rfx = c.y; r.rfy = c.z; r.rfz = c.w;
+            r.negx = !(r.rnx >= 0.0f);
+            r.negy = !(r.rny >= 0.0f);
+            r.negz = !(r.rnz >= 0.0f);
             moved = false;
           }
           continue;
@@ -1352,7 +1362,7 @@ void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t strid
   const uint32_t blocks = div_up(M, TB);
   const uint32_t want = (uint32_t)tp.cus * (uint32_t)tp.blocks_per_cu;
   const dim3 grid(want < blocks ? want : blocks), block(TB);
-  const int rmin = occluded ? tp.refill_min_occ : tp.refill_min;
+  const int rmin = occluded ? tp.refill_min_occ_inst : tp.refill_min;
   const int refill = rmin < 1 ? 1 : (rmin > 64 ? 64 : rmin);
   sorter->counter.reserve(256);
   uint32_t* counter = sorter->counter.as<uint32_t>();
