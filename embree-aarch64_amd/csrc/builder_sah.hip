@@ -400,7 +400,11 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
       if (!learned) {
         if (!best_on_axis(w, m, tid, c, k, nl)) c = INFINITY;
       } else if (m.scale[tid] != 0.0f) {
-        const float guard = fminf(fminf(L.amin[0], L.amin[1]), L.amin[2]) * (1.0f + lc.sah_slack);
+        // an unbounded slack admits every candidate (also where the least
+        // plain cost is 0, which would make 0 * inf a NaN)
+        const float guard = __builtin_isinf(lc.sah_slack)
+                                ? INFINITY
+                                : fminf(fminf(L.amin[0], L.amin[1]), L.amin[2]) * (1.0f + lc.sah_slack);
         uint32_t run = 0;
         for (int kk = 1; kk <= ncand; kk++) {
           run += w[wc(tid, kk - 1)];
