@@ -27,7 +27,6 @@ struct SahRoot {
 struct SahTreeOut {
   std::vector<uint32_t> roots;  // inner node id of each tree's root
   uint32_t levels = 0;          // top-down levels run
-  uint32_t max_depth = 0;       // binary depth
 };
 
 // Inputs: n valid references, `vals` holding their primref indices and

@@ -70,7 +70,6 @@ struct Split {
 
 struct Counters {
   uint32_t nseg[4];  // next-level segment counts: big, mid, small, tiny
-  uint32_t max_depth;
   uint32_t error;    // bit 0: a segment list overflowed, bit 1: bin counts disagree, bit 2: destination out of range,
                      // bit 3: median subtree stack overflow, bit 4: a tiny segment out of range,
                      // bit 5: a depth beyond 254
@@ -223,7 +222,6 @@ __device__ inline void link_inner(const Tree& t, const Seg& sg, uint32_t m) {
     t.parent_int[m] = sg.parent;
   }
   t.range[m] = make_uint2(sg.first, sg.first + sg.count - 1);
-  atomicMax(&t.cnt->max_depth, sg.meta >> 1);
   set_depth(t, m, sg.meta >> 1);
 }
 
@@ -245,7 +243,6 @@ __device__ inline void finish_median(const Tree& t, uint32_t m, uint32_t side, u
     t.child[2 * p + (meta & 1u)] = mid;
     t.parent_int[mid] = p;
     t.range[mid] = make_uint2(f, f + c - 1);
-    atomicMax(&t.cnt->max_depth, d);
     set_depth(t, mid, d);
     const uint32_t cf[2] = {f, f + nl}, cc[2] = {nl, c - nl};
 #pragma unroll
@@ -729,22 +726,34 @@ __global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, con
 
 constexpr int TT = 32;  // threads per block of k_tiny, one segment each
 
+// A thread's entries in LDS, entry j at j * TT (the rows of a block's
+// threads interleave, so that the lanes of a wave touch consecutive words
+// instead of one bank).
+struct TinyRows {
+  float4* lo;
+  float4* hi;
+  uint32_t* id;
+  __device__ inline float4& L(uint32_t j) const { return lo[j * TT]; }
+  __device__ inline float4& H(uint32_t j) const { return hi[j * TT]; }
+  __device__ inline uint32_t& I(uint32_t j) const { return id[j * TT]; }
+};
+
 // Stable insertion sort of entries [f, f+c) by centroid (lo + hi) on axis a.
-__device__ inline void tiny_sort(float4* lo, float4* hi, uint32_t* id, uint32_t f, uint32_t c, int a) {
+__device__ inline void tiny_sort(const TinyRows& e, uint32_t f, uint32_t c, int a) {
   for (uint32_t j = f + 1; j < f + c; j++) {
-    const float4 l = lo[j], h = hi[j];
-    const uint32_t v = id[j];
+    const float4 l = e.L(j), h = e.H(j);
+    const uint32_t v = e.I(j);
     const float key = comp(l, a) + comp(h, a);
     uint32_t k = j;
-    while (k > f && comp(lo[k - 1], a) + comp(hi[k - 1], a) > key) {
-      lo[k] = lo[k - 1];
-      hi[k] = hi[k - 1];
-      id[k] = id[k - 1];
+    while (k > f && comp(e.L(k - 1), a) + comp(e.H(k - 1), a) > key) {
+      e.L(k) = e.L(k - 1);
+      e.H(k) = e.H(k - 1);
+      e.I(k) = e.I(k - 1);
       k--;
     }
-    lo[k] = l;
-    hi[k] = h;
-    id[k] = v;
+    e.L(k) = l;
+    e.H(k) = h;
+    e.I(k) = v;
   }
 }
 
@@ -757,8 +766,8 @@ __device__ inline void tiny_sort(float4* lo, float4* hi, uint32_t* id, uint32_t 
 __global__ __launch_bounds__(TT) void k_tiny(const Seg* __restrict__ segs, uint32_t nseg,
                                              const uint32_t* __restrict__ idx, const float4* __restrict__ blo,
                                              const float4* __restrict__ bhi, Tree t) {
-  __shared__ float4 s_lo[TT][TINY_MAX], s_hi[TT][TINY_MAX];
-  __shared__ uint32_t s_id[TT][TINY_MAX];
+  __shared__ float4 s_lo[TINY_MAX][TT], s_hi[TINY_MAX][TT];
+  __shared__ uint32_t s_id[TINY_MAX][TT];
   const uint32_t i = blockIdx.x * TT + threadIdx.x;
   if (i >= nseg) return;
   const Seg sg = segs[i];
@@ -766,14 +775,12 @@ __global__ __launch_bounds__(TT) void k_tiny(const Seg* __restrict__ segs, uint3
     atomicOr(&t.cnt->error, 16u);
     return;
   }
-  float4* lo = s_lo[threadIdx.x];
-  float4* hi = s_hi[threadIdx.x];
-  uint32_t* id = s_id[threadIdx.x];
+  const TinyRows e{&s_lo[0][threadIdx.x], &s_hi[0][threadIdx.x], &s_id[0][threadIdx.x]};
   const uint32_t m = sg.count;
   for (uint32_t j = 0; j < m; j++) {
-    lo[j] = blo[sg.first + j];
-    hi[j] = bhi[sg.first + j];
-    id[j] = idx[sg.first + j];
+    e.L(j) = blo[sg.first + j];
+    e.H(j) = bhi[sg.first + j];
+    e.I(j) = idx[sg.first + j];
   }
   uint32_t sf[TINY_MAX], sc[TINY_MAX], spar[TINY_MAX], smeta[TINY_MAX];
   int sp = 0;
@@ -792,13 +799,13 @@ __global__ __launch_bounds__(TT) void k_tiny(const Seg* __restrict__ segs, uint3
     int ba = 0;
     uint32_t bk = 1;
     for (int a = 0; a < 3; a++) {
-      tiny_sort(lo, hi, id, f, c, a);
+      tiny_sort(e, f, c, a);
       float ra[TINY_MAX];
       float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
 #pragma unroll
       for (int k = TINY_MAX - 1; k >= 1; k--) {
         if ((uint32_t)k < c) {
-          const float4 l = lo[f + k], h = hi[f + k];
+          const float4 l = e.L(f + k), h = e.H(f + k);
           bl[0] = fminf(bl[0], l.x); bl[1] = fminf(bl[1], l.y); bl[2] = fminf(bl[2], l.z);
           bh[0] = fmaxf(bh[0], h.x); bh[1] = fmaxf(bh[1], h.y); bh[2] = fmaxf(bh[2], h.z);
           ra[k] = half_area(bh[0] - bl[0], bh[1] - bl[1], bh[2] - bl[2]);
@@ -809,7 +816,7 @@ __global__ __launch_bounds__(TT) void k_tiny(const Seg* __restrict__ segs, uint3
 #pragma unroll
       for (int k = 1; k < (int)TINY_MAX; k++) {
         if ((uint32_t)k < c) {
-          const float4 l = lo[f + k - 1], h = hi[f + k - 1];
+          const float4 l = e.L(f + k - 1), h = e.H(f + k - 1);
           bl[0] = fminf(bl[0], l.x); bl[1] = fminf(bl[1], l.y); bl[2] = fminf(bl[2], l.z);
           bh[0] = fmaxf(bh[0], h.x); bh[1] = fmaxf(bh[1], h.y); bh[2] = fmaxf(bh[2], h.z);
           const float cost = half_area(bh[0] - bl[0], bh[1] - bl[1], bh[2] - bl[2]) * (float)k +
@@ -822,12 +829,11 @@ __global__ __launch_bounds__(TT) void k_tiny(const Seg* __restrict__ segs, uint3
         }
       }
     }
-    if (ba != 2) tiny_sort(lo, hi, id, f, c, ba);  // the last sort left axis 2's order
+    if (ba != 2) tiny_sort(e, f, c, ba);  // the last sort left axis 2's order
     const uint32_t mid = sg.first + f + bk - 1;
     t.child[2 * p + side] = mid;
     t.parent_int[mid] = p;
     t.range[mid] = make_uint2(sg.first + f, sg.first + f + c - 1);
-    atomicMax(&t.cnt->max_depth, depth);
     set_depth(t, mid, depth);
     if (sp + 2 > (int)TINY_MAX) {
       atomicOr(&t.cnt->error, 16u);
@@ -836,7 +842,7 @@ __global__ __launch_bounds__(TT) void k_tiny(const Seg* __restrict__ segs, uint3
     sf[sp] = f + bk; sc[sp] = c - bk; spar[sp] = mid; smeta[sp] = ((depth + 1) << 1) | 1u; sp++;
     sf[sp] = f;      sc[sp] = bk;     spar[sp] = mid; smeta[sp] = ((depth + 1) << 1);      sp++;
   }
-  for (uint32_t j = 0; j < m; j++) t.vals[sg.first + j] = id[j];
+  for (uint32_t j = 0; j < m; j++) t.vals[sg.first + j] = e.I(j);
 }
 
 // Writes the big-segment id of every position of the initial big segments.
@@ -908,7 +914,6 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
     hipLaunchKernelGGL(k_fill_seg_ids, dim3(cur[0]), dim3(TBS), 0, s, S.sid[0].as<uint32_t>(), S.segs[0][0].as<Seg>());
   }
   Counters hc{};
-  hc.max_depth = 0;
   hc.error = 0;
   RTG_HIP(hipMemcpyAsync(cnt, &hc, sizeof(hc), hipMemcpyHostToDevice, s));
   RTG_HIP(hipStreamSynchronize(s));  // the host staging above must outlive its copies
@@ -972,7 +977,6 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
   for (uint32_t r : out.roots)
     if (r == NIL || r + 1 >= n) throw hip_error(hipErrorUnknown, "SAH build produced no root");
   out.levels = levels;
-  out.max_depth = hc.max_depth;
 }
 
 }  // namespace rtg
