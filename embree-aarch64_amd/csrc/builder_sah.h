@@ -13,9 +13,30 @@
 
 namespace rtg {
 
+// A second stream (default priority) for the tiny-segment finisher, which
+// runs beside the following levels, with one event per segment-list parity.
+struct SideStream {
+  hipStream_t s = nullptr;
+  hipEvent_t done[2] = {nullptr, nullptr};
+  void ensure() {
+    if (s) return;
+    RTG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    for (hipEvent_t& e : done) RTG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
+  ~SideStream() {
+    for (hipEvent_t e : done)
+      if (e) (void)hipEventDestroy(e);
+    if (s) (void)hipStreamDestroy(s);
+  }
+  SideStream() = default;
+  SideStream(const SideStream&) = delete;
+  SideStream& operator=(const SideStream&) = delete;
+};
+
 struct SahScratch {
   DevBuf idx[2], blo[2], bhi[2], sid[2], segs[4][2], bins, splits, flags, scan, counters, roots;
   Scanner scanner;
+  SideStream side;
 };
 
 // One tree of the forest: positions [first, first+count), count >= 2.

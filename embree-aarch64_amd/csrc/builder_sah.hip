@@ -919,11 +919,24 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
   RTG_HIP(hipStreamSynchronize(s));  // the host staging above must outlive its copies
 
   Tree t{child, parent_int, parent_leaf, range, vals, cnt, S.roots.as<uint32_t>(), depth};
+  S.side.ensure();
+  bool tiny_pending[2] = {false, false};
   int par = 0;
   uint32_t levels = 0;
+  // a failed build leaves no finisher running on the scratch it reuses
+  auto fail = [&](const std::string& msg) {
+    (void)hipStreamSynchronize(S.side.s);
+    throw hip_error(hipErrorUnknown, msg);
+  };
   while (cur[0] + cur[1] + cur[2] > 0) {
-    if (++levels > 4096) throw hip_error(hipErrorUnknown, "SAH build did not terminate");
+    if (++levels > 4096) fail("SAH build did not terminate");
     const int in = par, o = par ^ 1;
+    // this level rewrites the segment lists of parity o, which the finisher
+    // of two levels back may still be reading
+    if (tiny_pending[o]) {
+      RTG_HIP(hipStreamWaitEvent(s, S.side.done[o], 0));
+      tiny_pending[o] = false;
+    }
     Lists nx;
     for (int c = 0; c < 4; c++) {
       nx.seg[c] = S.segs[c][o].as<Seg>();
@@ -960,15 +973,23 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
     RTG_HIP(hipGetLastError());
     RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
     RTG_HIP(hipStreamSynchronize(s));
-    if (hc.error) throw hip_error(hipErrorUnknown, "SAH build failed: error bits " + std::to_string(hc.error));
+    if (hc.error) fail("SAH build failed: error bits " + std::to_string(hc.error));
     for (int c = 0; c < 3; c++) cur[c] = hc.nseg[c];
-    // this level's tiny children, finished in place (their positions are
-    // outside every segment the next levels touch)
-    if (hc.nseg[3])
-      hipLaunchKernelGGL(k_tiny, dim3(div_up(hc.nseg[3], TT)), dim3(TT), 0, s, S.segs[3][o].as<Seg>(), hc.nseg[3],
-                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), t);
+    // this level's tiny children, finished in place on the side stream while
+    // the next levels run (their positions are outside every segment the
+    // next levels touch; this level's work is complete after the
+    // synchronisation above)
+    if (hc.nseg[3]) {
+      hipLaunchKernelGGL(k_tiny, dim3(div_up(hc.nseg[3], TT)), dim3(TT), 0, S.side.s, S.segs[3][o].as<Seg>(),
+                         hc.nseg[3], S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), t);
+      RTG_HIP(hipGetLastError());
+      RTG_HIP(hipEventRecord(S.side.done[o], S.side.s));
+      tiny_pending[o] = true;
+    }
     par = o;
   }
+  for (int p = 0; p < 2; p++)
+    if (tiny_pending[p]) RTG_HIP(hipStreamWaitEvent(s, S.side.done[p], 0));
   out.roots.assign(trees.size(), NIL);
   RTG_HIP(hipMemcpyAsync(out.roots.data(), S.roots.ptr, trees.size() * 4, hipMemcpyDeviceToHost, s));
   RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
