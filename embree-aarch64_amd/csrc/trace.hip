@@ -1087,13 +1087,24 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
                   (inst == best_inst &&
                    (h.geom < best.geom || (h.geom == best.geom && h.prim < best.prim)))));
             if (better) {
+              // the hit goes to the record at once, so that only the fields
+              // the comparison needs stay in registers
               best.found = true;
               best.t = h.t;
-              best.u = h.un / h.dn;
-              best.v = h.vn / h.dn;
-              best.ngx = h.ngx; best.ngy = h.ngy; best.ngz = h.ngz;
-              best.prim = h.prim; best.geom = h.geom;
+              best.prim = h.prim;
+              best.geom = h.geom;
               best_inst = inst;
+              float* f = reinterpret_cast<float*>(rays + (uint64_t)ray * stride);
+              uint32_t* u = reinterpret_cast<uint32_t*>(f);
+              f[8] = h.t;
+              f[12] = h.ngx;
+              f[13] = h.ngy;
+              f[14] = h.ngz;
+              f[15] = h.un / h.dn;
+              f[16] = h.vn / h.dn;
+              u[17] = h.prim;
+              u[18] = h.geom;
+              u[19] = inst;
               lim = widen_up(fminf(best.t, r.tfar));
             }
           }
@@ -1190,18 +1201,6 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
       char* rp = rays + (uint64_t)ray * stride;
       if (OCCLUDED) {
         if (occluded) *reinterpret_cast<float*>(rp + 32) = -INFINITY;
-      } else if (best.found) {
-        float* f = reinterpret_cast<float*>(rp);
-        uint32_t* u = reinterpret_cast<uint32_t*>(rp);
-        f[8] = best.t;
-        f[12] = best.ngx;
-        f[13] = best.ngy;
-        f[14] = best.ngz;
-        f[15] = best.u;
-        f[16] = best.v;
-        u[17] = best.prim;
-        u[18] = best.geom;
-        u[19] = best_inst;
       }
       active = false;
     }
