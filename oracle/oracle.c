@@ -657,12 +657,13 @@ static void o_split_set(o_primref* prims, const o_rec* r, const o_split* s, o_re
   } else {
     /* partition by bin(center2)[dim] < pos; stable (order does not change the sets). */
     size_t nl = 0, nr = 0, n = r->end - r->begin;
+    o_primref* t = tmp + r->begin; /* range-local: subtrees may be built concurrently */
     for (size_t i = r->begin; i < r->end; i++) {
       const float c2 = prims[i].lo[s->dim] + prims[i].hi[s->dim];
       if (o_bin_of(s, c2, s->dim) < (int)s->pos) prims[r->begin + nl++] = prims[i];
-      else tmp[nr++] = prims[i];
+      else t[nr++] = prims[i];
     }
-    memcpy(prims + r->begin + nl, tmp, nr * sizeof(o_primref));
+    memcpy(prims + r->begin + nl, t, nr * sizeof(o_primref));
     (void)n;
     mid = r->begin + nl;
   }
@@ -701,7 +702,24 @@ struct oracle_bvh {
   float* tri;
 };
 
-static int o_new_node(oracle_bvh* b) {
+/* Nodes and leaves of one subtree under construction. Subtrees of large
+ * ranges are built concurrently, each into an arena of its own, and appended
+ * to their parent's arena in child order, so the tree (and its numbering in
+ * pre-order) does not depend on the thread count or scheduling. */
+typedef struct {
+  o_node* nodes;
+  size_t nnodes, cap_nodes;
+  o_leaf* leaves;
+  size_t nleaves, cap_leaves;
+  unsigned max_depth;
+} o_arena;
+
+typedef struct {
+  o_primref* prims; /* reordered in place; ranges of concurrent subtrees are disjoint */
+  o_primref* tmp;   /* partition scratch, indexed like prims */
+} o_build;
+
+static int o_new_node(o_arena* b) {
   if (b->nnodes == b->cap_nodes) {
     b->cap_nodes = b->cap_nodes ? 2 * b->cap_nodes : 1024;
     b->nodes = (o_node*)realloc(b->nodes, b->cap_nodes * sizeof(o_node));
@@ -710,7 +728,7 @@ static int o_new_node(oracle_bvh* b) {
   return (int)b->nnodes++;
 }
 
-static int o_new_leaf(oracle_bvh* b, size_t begin, size_t count) {
+static int o_new_leaf(o_arena* b, size_t begin, size_t count) {
   if (b->nleaves == b->cap_leaves) {
     b->cap_leaves = b->cap_leaves ? 2 * b->cap_leaves : 1024;
     b->leaves = (o_leaf*)realloc(b->leaves, b->cap_leaves * sizeof(o_leaf));
@@ -719,6 +737,28 @@ static int o_new_leaf(oracle_bvh* b, size_t begin, size_t count) {
   b->leaves[b->nleaves].count = count;
   return ~(int)b->nleaves++;
 }
+
+static int o_shift_ref(int ref, size_t node_ofs, size_t leaf_ofs) {
+  return ref >= 0 ? ref + (int)node_ofs : ~(~ref + (int)leaf_ofs);
+}
+
+/* Appends subtree arena s to a; returns s's root reference renumbered. */
+static int o_append(o_arena* a, o_arena* s, int root) {
+  const size_t no = a->nnodes, lo = a->nleaves;
+  for (size_t i = 0; i < s->nnodes; i++) {
+    int id = o_new_node(a);
+    a->nodes[id] = s->nodes[i];
+    for (int c = 0; c < a->nodes[id].n; c++) a->nodes[id].ref[c] = o_shift_ref(a->nodes[id].ref[c], no, lo);
+  }
+  for (size_t i = 0; i < s->nleaves; i++) o_new_leaf(a, s->leaves[i].begin, s->leaves[i].count);
+  if (s->max_depth > a->max_depth) a->max_depth = s->max_depth;
+  free(s->nodes);
+  free(s->leaves);
+  return o_shift_ref(root, no, lo);
+}
+
+/* ranges of at least this many references build their children as tasks */
+#define O_TASK_MIN 16384
 
 #define O_MIN_LEAF 4
 #define O_MAX_LEAF 28
@@ -731,7 +771,7 @@ static int o_children_cmp(const void* a, const void* b) {
   return na > nb ? -1 : na < nb ? 1 : 0; /* std::greater: larger first */
 }
 
-static int o_make_node(oracle_bvh* b, o_rec* ch, int n) {
+static int o_make_node(o_arena* b, o_rec* ch, int n) {
   int id = o_new_node(b);
   b->nodes[id].n = n;
   for (int i = 0; i < n; i++) {
@@ -744,7 +784,7 @@ static int o_make_node(oracle_bvh* b, o_rec* ch, int n) {
 }
 
 /* createLargeLeaf (kernels/builders/bvh_builder_sah.h:155-220). */
-static int o_large_leaf(oracle_bvh* b, o_rec* cur, o_primref* tmp) {
+static int o_large_leaf(const o_build* bd, o_arena* b, o_rec* cur) {
   size_t N = cur->end - cur->begin;
   if (cur->depth > b->max_depth) b->max_depth = cur->depth;
   if (N <= O_MAX_LEAF) return o_new_leaf(b, cur->begin, N);
@@ -764,7 +804,7 @@ static int o_large_leaf(oracle_bvh* b, o_rec* cur, o_primref* tmp) {
     fb.dim = -1;
     o_rec L, R;
     L.depth = R.depth = cur->depth + 1;
-    o_split_set(b->prims, &ch[best], &fb, &L, &R, tmp);
+    o_split_set(bd->prims, &ch[best], &fb, &L, &R, bd->tmp);
     L.depth = R.depth = cur->depth + 1;
     ch[best] = ch[n - 1];
     ch[n - 1] = L;
@@ -773,27 +813,29 @@ static int o_large_leaf(oracle_bvh* b, o_rec* cur, o_primref* tmp) {
   } while (n < O_BRANCH);
   int id = o_make_node(b, ch, n);
   for (int i = 0; i < n; i++) {
-    int r = o_large_leaf(b, &ch[i], tmp);
+    int r = o_large_leaf(bd, b, &ch[i]);
     b->nodes[id].ref[i] = r;
   }
   return id;
 }
 
-/* BuilderT::recurse (kernels/builders/bvh_builder_sah.h:222-330). */
-static int o_recurse(oracle_bvh* b, o_rec* cur, o_primref* tmp) {
+/* BuilderT::recurse (kernels/builders/bvh_builder_sah.h:222-330); the
+ * reference also recurses into the children in parallel above its
+ * singleThreadThreshold (:320-327). */
+static int o_recurse(const o_build* bd, o_arena* b, o_rec* cur) {
   const size_t N = cur->end - cur->begin;
   if (cur->depth > b->max_depth) b->max_depth = cur->depth;
   o_split s;
-  o_find(b->prims, cur, &s);
+  o_find(bd->prims, cur, &s);
   const float ha = box_half_area(&cur->geom);
   const float leafSAH = 1.0f * (ha * (float)((N + 3) >> 2));
   const float splitSAH = 1.0f * ha + 1.0f * s.sah;
   if (N <= O_MIN_LEAF || cur->depth + 8 >= O_MAX_DEPTH || (N <= O_MAX_LEAF && leafSAH <= splitSAH)) {
-    qsort(b->prims + cur->begin, N, sizeof(o_primref), o_primref_cmp);
-    return o_large_leaf(b, cur, tmp);
+    qsort(bd->prims + cur->begin, N, sizeof(o_primref), o_primref_cmp);
+    return o_large_leaf(bd, b, cur);
   }
   o_rec ch[O_BRANCH];
-  o_split_set(b->prims, cur, &s, &ch[0], &ch[1], tmp);
+  o_split_set(bd->prims, cur, &s, &ch[0], &ch[1], bd->tmp);
   int n = 2;
   while (n < O_BRANCH) {
     float bestArea = -INFINITY;
@@ -805,23 +847,42 @@ static int o_recurse(oracle_bvh* b, o_rec* cur, o_primref* tmp) {
     }
     if (best < 0) break;
     o_split cs;
-    o_find(b->prims, &ch[best], &cs);
+    o_find(bd->prims, &ch[best], &cs);
     o_rec L, R;
-    o_split_set(b->prims, &ch[best], &cs, &L, &R, tmp);
+    o_split_set(bd->prims, &ch[best], &cs, &L, &R, bd->tmp);
     L.depth = R.depth = cur->depth + 1;
     ch[best] = L;
     ch[n++] = R;
   }
   qsort(ch, n, sizeof(o_rec), o_children_cmp);
   int id = o_make_node(b, ch, n);
+  if (N < O_TASK_MIN) {
+    for (int i = 0; i < n; i++) {
+      int r = o_recurse(bd, b, &ch[i]);
+      b->nodes[id].ref[i] = r;
+    }
+    return id;
+  }
+  o_arena sub[O_BRANCH];
+  int root[O_BRANCH];
+  memset(sub, 0, sizeof(sub));
   for (int i = 0; i < n; i++) {
-    int r = o_recurse(b, &ch[i], tmp);
+#pragma omp task default(none) firstprivate(i) shared(bd, sub, root, ch)
+    root[i] = o_recurse(bd, &sub[i], &ch[i]);
+  }
+#pragma omp taskwait
+  for (int i = 0; i < n; i++) {
+    int r = o_append(b, &sub[i], root[i]);
     b->nodes[id].ref[i] = r;
   }
   return id;
 }
 
 oracle_bvh* oracle_build_sah(const oracle_mesh* meshes, unsigned nmeshes) {
+  return oracle_build_sah_threads(meshes, nmeshes, 0);
+}
+
+oracle_bvh* oracle_build_sah_threads(const oracle_mesh* meshes, unsigned nmeshes, int nthreads) {
   oracle_bvh* b = (oracle_bvh*)calloc(1, sizeof(oracle_bvh));
   b->meshes = meshes;
   b->nmeshes = nmeshes;
@@ -857,9 +918,24 @@ oracle_bvh* oracle_build_sah(const oracle_mesh* meshes, unsigned nmeshes) {
   root.begin = 0; root.end = b->nprims; root.depth = 1;
   rec_compute(b->prims, &root);
   b->bounds = root.geom;
-  o_primref* tmp = (o_primref*)malloc(sizeof(o_primref) * b->nprims);
-  b->root = o_recurse(b, &root, tmp);
-  free(tmp);
+  o_build bd;
+  bd.prims = b->prims;
+  bd.tmp = (o_primref*)malloc(sizeof(o_primref) * b->nprims);
+  o_arena a;
+  memset(&a, 0, sizeof(a));
+  int top = 0;
+#pragma omp parallel num_threads(o_threads(nthreads)) default(none) shared(bd, a, root, top)
+#pragma omp single
+  top = o_recurse(&bd, &a, &root);
+  free(bd.tmp);
+  b->root = top;
+  b->nodes = a.nodes;
+  b->nnodes = a.nnodes;
+  b->cap_nodes = a.cap_nodes;
+  b->leaves = a.leaves;
+  b->nleaves = a.nleaves;
+  b->cap_leaves = a.cap_leaves;
+  b->max_depth = a.max_depth;
   /* widen every stored child box by eps (conservative culling). */
   for (size_t i = 0; i < b->nnodes; i++)
     for (int c = 0; c < b->nodes[i].n; c++)

@@ -155,6 +155,9 @@ typedef struct oracle_bvh oracle_bvh;
 /* Builds the reference-style binned-SAH BVH8 over all valid triangles. The
  * meshes array and the data it points to must outlive the BVH. */
 oracle_bvh* oracle_build_sah(const oracle_mesh* meshes, unsigned nmeshes);
+/* The same tree built on nthreads threads (0: all); subtrees are numbered in
+ * pre-order whatever the thread count. */
+oracle_bvh* oracle_build_sah_threads(const oracle_mesh* meshes, unsigned nmeshes, int nthreads);
 void oracle_bvh_free(oracle_bvh* bvh);
 
 /* Statistics: [0] inner nodes, [1] leaves, [2] primitives, [3] max depth,

@@ -51,6 +51,8 @@ def lib():
         L.oracle_occluded_brute.argtypes = [vp, u32, vp, u32, sz, i32]
         L.oracle_build_sah.argtypes = [vp, u32]
         L.oracle_build_sah.restype = vp
+        L.oracle_build_sah_threads.argtypes = [vp, u32, i32]
+        L.oracle_build_sah_threads.restype = vp
         L.oracle_bvh_free.argtypes = [vp]
         L.oracle_bvh_stats.argtypes = [vp, vp]
         L.oracle_bvh_validate.argtypes = [vp]
@@ -141,9 +143,10 @@ def occluded_brute(meshes, rays, nthreads=0, robust=False):
 class BVH:
     """Reference-style binned-SAH BVH8 built by the CPU restatement."""
 
-    def __init__(self, meshes):
+    def __init__(self, meshes, nthreads=0):
+        """nthreads: build threads (0: all); the tree does not depend on it."""
         self.m = Meshes(meshes)
-        self.h = lib().oracle_build_sah(ctypes.addressof(self.m.c), self.m.n)
+        self.h = lib().oracle_build_sah_threads(ctypes.addressof(self.m.c), self.m.n, nthreads)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -289,6 +289,54 @@ def test_sah_builder_structure(oracle):
     assert bvh.validate() == 0 and bvh.stats()["prims"] == 300
 
 
+def test_parallel_build_is_the_serial_tree(oracle):
+    """Subtrees built concurrently (per-subtree arenas appended in child
+    order) give the serial build's tree: the same node, leaf and depth
+    counts, the same SAH statistic to the last bit, and every ray visits
+    the same number of nodes and triangles in both."""
+    v, ix = soup(200000, 11)
+    trees = [oracle.BVH([(v, ix)], nthreads=t) for t in (1, 3, 8)]
+    st = [b.stats() for b in trees]
+    assert st[0] == st[1] == st[2]
+    assert all(b.validate() == 0 for b in trees)
+    rays = random_rays(20000, 12)
+    counts = [b.intersect_counted(rays.copy()) for b in trees]
+    for n, t in counts[1:]:
+        assert np.array_equal(n, counts[0][0]) and np.array_equal(t, counts[0][1])
+
+
+def test_sah_statistic_closed_form(oracle):
+    """The SAH statistic (reference kernels/bvh/bvh_statistics.cpp:40-56,
+    bvh_statistics.h:36-38,99-101: half area of every inner node, plus half
+    area times primitive blocks of every leaf, over the root's half area) on
+    a tree whose shape the build cannot miss: 8 tight clusters of 4
+    triangles at the corners of a cube make one inner node with 8 leaves of
+    one block each, so SAH = 1 + sum(A_cluster) / A_root exactly."""
+    rng = np.random.default_rng(21)
+    verts, boxes = [], []
+    for c in range(8):
+        corner = np.array([(c >> 0) & 1, (c >> 1) & 1, (c >> 2) & 1], np.float32) * 10.0
+        tv = (corner + rng.uniform(0.0, 0.5, (12, 3))).astype(np.float32)
+        verts.append(tv)
+        boxes.append((tv.min(0).astype(np.float64), tv.max(0).astype(np.float64)))
+    v = np.concatenate(verts)
+    ix = np.arange(len(v), dtype=np.uint32).reshape(-1, 3)
+    bvh = oracle.BVH([(v, ix)])
+    s = bvh.stats()
+    assert (s["inner_nodes"], s["leaves"], s["prims"]) == (1, 8, 32)
+
+    def ha(lo, hi):
+        d = hi - lo
+        return d[0] * d[1] + d[1] * d[2] + d[2] * d[0]
+
+    root = ha(v.min(0).astype(np.float64), v.max(0).astype(np.float64))
+    expect = 1.0 + sum(ha(lo, hi) for lo, hi in boxes) / root
+    assert abs(s["sah"] - expect) <= 1e-6 * expect, (s["sah"], expect)
+    # leaf cost per triangle counts 4 per leaf instead of 1 block
+    expect_tri = 1.0 + 4.0 * sum(ha(lo, hi) for lo, hi in boxes) / root
+    assert abs(s["sah_per_tri"] - expect_tri) <= 1e-6 * expect_tri
+
+
 def test_binned_split_cost_matches_partition(oracle):
     """The split the binner returns costs exactly what its partition costs
     under halfArea * ceil(n/4), and separates two clusters."""
