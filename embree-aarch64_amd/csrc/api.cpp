@@ -69,6 +69,10 @@ struct Device : RefObj {
   void* memptr = nullptr;
   std::mutex build_mtx;
   rtg::BuildScratch scratch;
+  // tri_builder / builder setting: "default" lets the build quality choose
+  // (reference bvh8_factory.cpp:865-870); an unknown name fails the commit.
+  enum { TB_DEFAULT, TB_SAH, TB_MORTON, TB_UNKNOWN } tri_builder = TB_DEFAULT;
+  std::string tri_builder_name = "default";
   std::mutex tls_mtx;
   std::unordered_map<std::thread::id, ThreadDev*> threads;
   ~Device() override {
@@ -94,6 +98,17 @@ struct Device : RefObj {
     RTG_HIP(hipMemsetAsync(t->errword.ptr, 0, 4, t->stream));
     threads[std::this_thread::get_id()] = t;
     return *t;
+  }
+  // build and trace settings: written by rtcNewDevice and rtcgAutotune,
+  // read by commits and queries on any thread (copies taken under the lock)
+  mutable std::mutex settings_mtx;
+  rtg::TraceParams trace_params() const {
+    std::lock_guard<std::mutex> g(settings_mtx);
+    return trace;
+  }
+  rtg::BuildParams build_settings() const {
+    std::lock_guard<std::mutex> g(settings_mtx);
+    return build;
   }
   void monitor(ssize_t bytes) {
     if (memfn && !memfn(memptr, bytes, false)) RTC_THROW(RTC_ERROR_OUT_OF_MEMORY, "memory monitor forced termination");
@@ -306,7 +321,16 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "bvh_leaf" || k == "max_leaf") d->build.max_leaf = (uint32_t)atoi(v.c_str());
     else if (k == "cost_node") d->build.cost_node = (float)atof(v.c_str());
     else if (k == "cost_tri") d->build.cost_tri = (float)atof(v.c_str());
-    else if (k == "builder") d->build.builder = (v == "morton" || v == "lbvh") ? rtg::BUILDER_MORTON : rtg::BUILDER_SAH;
+    else if (k == "builder" || k == "tri_builder") {
+      // reference state.cpp:309-310 (aliases) and bvh8_factory.cpp:871-878
+      // (names): sah and the two-level "dynamic" SAH build the binned-SAH
+      // tree here, morton (and lbvh) the Morton radix tree
+      d->tri_builder_name = v;
+      if (v == "default") d->tri_builder = Device::TB_DEFAULT;
+      else if (v == "sah" || v == "dynamic") d->tri_builder = Device::TB_SAH;
+      else if (v == "morton" || v == "lbvh") d->tri_builder = Device::TB_MORTON;
+      else d->tri_builder = Device::TB_UNKNOWN;
+    }
     else if (k == "bvh_width") d->build.width = (uint32_t)atoi(v.c_str());
     else if (k == "sah_median_below") d->build.median_below = (uint32_t)atoi(v.c_str());
     else if (k == "trace_stack") d->trace.lds_slots = atoi(v.c_str());
@@ -318,18 +342,26 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "refill_min_occ_inst") d->trace.refill_min_occ_inst = atoi(v.c_str());
     else if (k == "blocks_per_cu") d->trace.blocks_per_cu = atoi(v.c_str());
     else if (k == "dual") d->trace.dual = atoi(v.c_str());
-    // threads, isa, tri_accel, tri_builder, set_affinity, ... are CPU settings: accepted, ignored
+    // threads, isa, tri_accel, set_affinity, ... are CPU settings: accepted, ignored
   }
 }
 
-// Build settings for a scene: LOW asks for the Morton radix tree, HIGH for
-// the binned-SAH tree, MEDIUM for the device's configured builder.
+// Build settings for a scene. With the default builder the quality chooses
+// (reference scene.cpp:93-116, bvh8_factory.cpp:865-870): LOW the Morton
+// radix tree, MEDIUM and HIGH the binned-SAH tree; a builder named in the
+// device config applies at every quality; an unknown name is an error here,
+// at commit, as in the reference's factory (bvh8_factory.cpp:878).
 rtg::BuildParams build_params(const Device* d, RTCBuildQuality q, RTCSceneFlags flags) {
-  rtg::BuildParams p = d->build;
+  rtg::BuildParams p = d->build_settings();
   p.quality = q;
   p.robust = (flags & RTC_SCENE_FLAG_ROBUST) ? 1 : 0;
-  if (q == RTC_BUILD_QUALITY_LOW) p.builder = rtg::BUILDER_MORTON;
-  else if (q == RTC_BUILD_QUALITY_HIGH) p.builder = rtg::BUILDER_SAH;
+  switch (d->tri_builder) {
+    case Device::TB_DEFAULT: p.builder = q == RTC_BUILD_QUALITY_LOW ? rtg::BUILDER_MORTON : rtg::BUILDER_SAH; break;
+    case Device::TB_SAH: p.builder = rtg::BUILDER_SAH; break;
+    case Device::TB_MORTON: p.builder = rtg::BUILDER_MORTON; break;
+    default:
+      RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "unknown builder " + d->tri_builder_name + " for BVH8<Triangle4>");
+  }
   return p;
 }
 
@@ -384,8 +416,13 @@ void bind_slot(Geometry* g, RTCBufferType type, unsigned slot, RTCFormat format,
 
 // Build settings of a scene, including its split model (lc receives the
 // model's device view and must outlive the build).
-rtg::BuildParams scene_params(Scene* sc, rtg::LearnedCost& lc) {
+rtg::BuildParams scene_params(Scene* sc, rtg::LearnedCost& lc, const rtg::BuildParams* shape = nullptr) {
   rtg::BuildParams p = build_params(sc->dev, sc->quality, sc->flags);
+  if (shape) {  // a tree shape under trial (autotuner)
+    p.width = shape->width;
+    p.max_leaf = shape->max_leaf;
+    p.cost_tri = shape->cost_tri;
+  }
   if (sc->model) {
     std::lock_guard<std::mutex> ml(sc->model->mtx);
     if (!sc->model->m.has_norm) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "split model has no training data");
@@ -726,7 +763,7 @@ void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigne
   const uint32_t inst = ctx ? ctx->instID[0] : RTC_INVALID_GEOMETRY_ID;
   if (scene_empty(sc)) return;  // empty scene: every ray misses
   if (is_device_ptr(d, rays)) {
-    launch_scene(sc, t, rays, M, stride, occluded, inst, t.stream, d->trace);
+    launch_scene(sc, t, rays, M, stride, occluded, inst, t.stream, d->trace_params());
     check_trace_error(t);
     return;
   }
@@ -740,7 +777,7 @@ void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigne
     RTG_HIP(hipMemcpyAsync(t.staging.ptr, pack.data(), pack.size(), hipMemcpyHostToDevice, t.stream));
     RTG_HIP(hipStreamSynchronize(t.stream));
   }
-  launch_scene(sc, t, t.staging.ptr, M, rec, occluded, inst, t.stream, d->trace);
+  launch_scene(sc, t, t.staging.ptr, M, rec, occluded, inst, t.stream, d->trace_params());
   if (stride == rec) {
     RTG_HIP(hipMemcpyAsync(rays, t.staging.ptr, (size_t)M * rec, hipMemcpyDeviceToHost, t.stream));
     RTG_HIP(hipStreamSynchronize(t.stream));
@@ -1430,7 +1467,7 @@ static void async_trace(Scene* s, const RTCIntersectContext* ctx, void* rays, un
   d->activate();
   ThreadDev& t = d->td();
   const uint32_t inst = ctx ? ctx->instID[0] : RTC_INVALID_GEOMETRY_ID;
-  launch_scene(s, t, rays, M, stride, occluded, inst, stream ? (hipStream_t)stream : t.stream, d->trace);
+  launch_scene(s, t, rays, M, stride, occluded, inst, stream ? (hipStream_t)stream : t.stream, d->trace_params());
 }
 
 RTC_API void rtcgIntersect1MAsync(RTCScene scene, const RTCIntersectContext* context, RTCRayHit* d_rayhit,
@@ -1476,7 +1513,7 @@ RTC_API void rtcgTimeTrace(RTCScene scene, void* d_rays, const void* d_src, unsi
   for (int i = 0; i < steps; i++) {
     if (d_src) RTG_HIP(hipMemcpyAsync(d_rays, d_src, (size_t)M * byteStride, hipMemcpyDeviceToDevice, t.stream));
     RTG_HIP(hipEventRecord(t.ev0, t.stream));
-    launch_scene(s, t, d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, t.stream, d->trace);
+    launch_scene(s, t, d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, t.stream, d->trace_params());
     RTG_HIP(hipEventRecord(t.ev1, t.stream));
     RTG_HIP(hipEventSynchronize(t.ev1));
     float ms = 0.0f;
@@ -1528,7 +1565,7 @@ RTC_API void rtcgTraceCounted(RTCScene scene, void* d_rays, unsigned int M, size
     RTG_HIP(hipMemsetAsync(d_tris, 0, (size_t)M * 4, t.stream));
   } else {
     rtg::launch_trace(s->bvh->view(s->slot), d_rays, M, byteStride, occluded != 0, RTC_INVALID_GEOMETRY_ID, d_nodes, d_tris,
-                      t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter);
+                      t.errword.as<uint32_t>(), t.stream, d->trace_params(), &t.sorter);
   }
   RTG_HIP(hipStreamSynchronize(t.stream));
   check_trace_error(t);
@@ -1731,7 +1768,7 @@ RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const v
   RTG_HIP(hipMemsetAsync(m->m.visits.ptr, 0, (size_t)nn * 4, t.stream));
   uint32_t* cn = m->m.counts.as<uint32_t>();
   rtg::launch_trace(s->bvh->view(s->slot), copy.ptr, M, byteStride, false, RTC_INVALID_GEOMETRY_ID, cn, cn + M,
-                    t.errword.as<uint32_t>(), t.stream, d->trace, &t.sorter, m->m.visits.as<uint32_t>());
+                    t.errword.as<uint32_t>(), t.stream, d->trace_params(), &t.sorter, m->m.visits.as<uint32_t>());
   const rtg::SceneNorm sn = m->m.has_norm ? m->m.norm : scene_norm(s->bvh->slots[s->slot]);
   rtg::mlp_append_samples(m->m, (const float*)s->bvh->info.ptr, m->m.visits.as<uint32_t>(), nn, M, sn, t.stream);
   m->m.norm = sn;
@@ -1924,7 +1961,14 @@ PreparedBatch prepare_batch(const RTCGBatchItem* items, unsigned n, bool occlude
 
 struct BatchGraphH : RefObj {
   Device* dev;
-  std::vector<Scene*> scenes;  // retained: their arrays must outlive the graph
+  std::vector<Scene*> scenes;  // retained
+  // The captured kernel arguments hold the shared node and triangle arrays'
+  // addresses: the graph keeps the arrays alive and remembers the build
+  // generation and each scene's slot, and a launch after any scene was
+  // committed again (new arrays, or a rebuild in place) is refused.
+  std::shared_ptr<rtg::DeviceBVH> store;
+  uint64_t generation = 0;
+  std::vector<uint32_t> slots;  // per retained scene
   rtg::DevBuf table, counter, err;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1955,7 +1999,7 @@ RTC_API void rtcgTraceBatch(const RTCGBatchItem* items, unsigned int numItems, i
   RTG_HIP(hipMemcpyAsync(t.batch.ptr, pb.items.data(), pb.items.size() * sizeof(rtg::BatchItem), hipMemcpyHostToDevice,
                          t.stream));
   rtg::launch_trace_batch(pb.sc, t.batch.as<rtg::BatchItem>(), (uint32_t)pb.items.size(), pb.total, occluded != 0,
-                          pb.aligned, t.errword.as<uint32_t>(), t.batch_counter.as<uint32_t>(), t.stream, d->trace);
+                          pb.aligned, t.errword.as<uint32_t>(), t.batch_counter.as<uint32_t>(), t.stream, d->trace_params());
   RTG_HIP(hipStreamSynchronize(t.stream));
   check_trace_error(t);
   API_END(s0)
@@ -1970,10 +2014,13 @@ RTC_API RTCGBatchGraph rtcgNewBatchGraph(const RTCGBatchItem* items, unsigned in
   ThreadDev& t = d->td();
   auto* g = new BatchGraphH(d);
   try {
+    g->store = pb.first->bvh;
+    g->generation = g->store->generation;
     for (unsigned i = 0; i < numItems; i++) {
       Scene* s = SC(items[i].scene);
       ref_inc(s);
       g->scenes.push_back(s);
+      g->slots.push_back(s->slot);
     }
     g->table.reserve(pb.items.size() * sizeof(rtg::BatchItem) + 16);
     g->counter.reserve(256);
@@ -1989,7 +2036,7 @@ RTC_API RTCGBatchGraph rtcgNewBatchGraph(const RTCGBatchItem* items, unsigned in
     try {
       rtg::launch_trace_batch(pb.sc, g->table.as<rtg::BatchItem>(), (uint32_t)pb.items.size(), pb.total,
                               occluded != 0, pb.aligned, g->err.as<uint32_t>(), g->counter.as<uint32_t>(), t.stream,
-                              d->trace);
+                              d->trace_params());
     } catch (...) {
       hipGraph_t tmp = nullptr;
       (void)hipStreamEndCapture(t.stream, &tmp);
@@ -2012,6 +2059,12 @@ RTC_API void rtcgLaunchBatchGraph(RTCGBatchGraph graph, void* hipStream) {
   BatchGraphH* g = (BatchGraphH*)graph;
   API_BEGIN
   VERIFY_HANDLE(graph);
+  if (g->store->generation != g->generation)
+    RTC_THROW(RTC_ERROR_INVALID_OPERATION, "the batch's arrays were rebuilt since the graph was made: make a new graph");
+  for (size_t i = 0; i < g->scenes.size(); i++)
+    if (g->scenes[i]->bvh != g->store || g->scenes[i]->slot != g->slots[i])
+      RTC_THROW(RTC_ERROR_INVALID_OPERATION,
+                "a scene of the batch graph was committed again since the graph was made: make a new graph");
   g->dev->activate();
   ThreadDev& t = g->dev->td();
   hipStream_t st = hipStream ? (hipStream_t)hipStream : t.stream;
@@ -2062,10 +2115,12 @@ float time_config(Scene* s, ThreadDev& t, const rtg::TraceParams& tp, const void
   return total / (float)reps;
 }
 
-void rebuild(Scene* s, Device* d, ThreadDev& t) {
+// Rebuilds the scene's tree with the given node width, leaf size and
+// collapse triangle cost (the device's settings stay untouched).
+void rebuild(Scene* s, Device* d, ThreadDev& t, const rtg::BuildParams& shape) {
   std::lock_guard<std::mutex> bl(d->build_mtx);
   rtg::LearnedCost lc{};
-  const rtg::BuildParams p = scene_params(s, lc);
+  const rtg::BuildParams p = scene_params(s, lc, &shape);
   rtg::build_bvh(s->descs, p, d->scratch, *s->bvh, t.stream);
   RTG_HIP(hipStreamSynchronize(t.stream));
 }
@@ -2094,27 +2149,35 @@ RTC_API float rtcgAutotune(RTCScene scene, const void* d_rays, unsigned int M, s
   rtg::DevBuf work;
   work.reserve((size_t)M * byteStride);
   const int reps = 2;
-  rtg::TraceParams tp = d->trace;
+  // The sweep works on local copies; the device's settings change once, at
+  // the end, under its settings lock, so that commits and queries on other
+  // threads never see a trial value.
+  const rtg::TraceParams tp = d->trace_params();
+  const rtg::BuildParams start = d->build_settings();
+  rtg::BuildParams bestb = start;
   float best = time_config(s, t, tp, d_rays, work.ptr, M, byteStride, occluded != 0, reps);
   if (rebuildTree) {
     // tree shape: node width, leaf size, triangle cost of the collapse
-    const rtg::BuildParams start = d->build;
-    rtg::BuildParams bestb = start;
-    for (uint32_t w : {4u, 8u})
-      for (uint32_t leaf : {4u, 6u, 8u, 12u})
-        for (float ct : {0.3f, 0.45f, 0.6f}) {
-          d->build.width = w;
-          d->build.max_leaf = leaf;
-          d->build.cost_tri = ct;
-          rebuild(s, d, t);
-          const float ms = time_config(s, t, tp, d_rays, work.ptr, M, byteStride, occluded != 0, reps);
-          if (ms < best) {
-            best = ms;
-            bestb = d->build;
+    try {
+      for (uint32_t w : {4u, 8u})
+        for (uint32_t leaf : {4u, 6u, 8u, 12u})
+          for (float ct : {0.3f, 0.45f, 0.6f}) {
+            rtg::BuildParams c = start;
+            c.width = w;
+            c.max_leaf = leaf;
+            c.cost_tri = ct;
+            rebuild(s, d, t, c);
+            const float ms = time_config(s, t, tp, d_rays, work.ptr, M, byteStride, occluded != 0, reps);
+            if (ms < best) {
+              best = ms;
+              bestb = c;
+            }
           }
-        }
-    d->build = bestb;
-    rebuild(s, d, t);
+      rebuild(s, d, t, bestb);
+    } catch (...) {
+      rebuild(s, d, t, start);  // the scene keeps the tree of the device's settings
+      throw;
+    }
     best = time_config(s, t, tp, d_rays, work.ptr, M, byteStride, occluded != 0, reps);
   }
   // traversal: LDS stack entries, refill threshold, resident blocks per CU
@@ -2133,13 +2196,19 @@ RTC_API float rtcgAutotune(RTCScene scene, const void* d_rays, unsigned int M, s
           bestt = c;
         }
       }
-  d->trace = bestt;
+  {
+    std::lock_guard<std::mutex> g(d->settings_mtx);
+    d->build.width = bestb.width;
+    d->build.max_leaf = bestb.max_leaf;
+    d->build.cost_tri = bestb.cost_tri;
+    d->trace = bestt;
+  }
   if (out && outSize) {
     char buf[256];
     snprintf(buf, sizeof(buf), "bvh_width=%u,max_leaf=%u,cost_tri=%g,trace_stack=%d,%s=%d,blocks_per_cu=%d",
-             d->build.width, d->build.max_leaf, (double)d->build.cost_tri, d->trace.lds_slots,
-             occluded ? "refill_min_occ" : "refill_min", occluded ? d->trace.refill_min_occ : d->trace.refill_min,
-             d->trace.blocks_per_cu);
+             bestb.width, bestb.max_leaf, (double)bestb.cost_tri, bestt.lds_slots,
+             occluded ? "refill_min_occ" : "refill_min", occluded ? bestt.refill_min_occ : bestt.refill_min,
+             bestt.blocks_per_cu);
     strncpy(out, buf, outSize - 1);
     out[outSize - 1] = 0;
   }
@@ -2156,7 +2225,8 @@ namespace {
 
 // A run of rays in SoA form: field f of lane i at r[f] + 4 i (ray fields in
 // RTCRay order) and h[f] + 4 i (hit fields in RTCHit order; unused for
-// occlusion). valid (may be null) masks lanes out with 0.
+// occlusion). valid (may be null): a lane is active only where it holds -1,
+// as the reference reads the mask (kernels/bvh/bvh_intersector_hybrid.cpp:131,386).
 struct SoaRun {
   char* r[12];
   char* h[8];
@@ -2183,7 +2253,7 @@ void trace_runs(Scene* sc, const RTCIntersectContext* ctx, const std::vector<Soa
   for (uint32_t ri = 0; ri < runs.size(); ri++) {
     const SoaRun& run = runs[ri];
     for (uint32_t i = 0; i < run.n; i++) {
-      if (run.valid && run.valid[i] == 0) continue;
+      if (run.valid && run.valid[i] != -1) continue;
       RTCRayHit rec;
       uint32_t* w = reinterpret_cast<uint32_t*>(&rec.ray);
       for (int f = 0; f < 12; f++) memcpy(&w[f], run.r[f] + 4 * (size_t)i, 4);
@@ -2507,7 +2577,7 @@ RTC_API void* rtcBuildBVH(const RTCBuildArguments* args) {
   gd.num_vertices = n;
   gd.num_tris = n;
   gd.kind = rtg::GEOM_INSTANCES;
-  rtg::BuildParams p = d->build;
+  rtg::BuildParams p = d->build_settings();
   p.builder = args->buildQuality == RTC_BUILD_QUALITY_LOW ? rtg::BUILDER_MORTON : rtg::BUILDER_SAH;
   p.width = std::min(8u, std::max(2u, args->maxBranchingFactor));
   p.max_leaf = std::min<unsigned>(RTC_BUILD_MAX_PRIMITIVES_PER_LEAF, std::max(1u, args->maxLeafSize));

@@ -142,7 +142,7 @@ __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ ge
       for (int k = 0; k < 3; k++) {
         l[k] = bx[k];
         h[k] = bx[3 + k];
-        ok &= finite_v(l[k]) & finite_v(h[k]) & (l[k] <= h[k]);
+        ok = ok && finite_v(l[k]) && finite_v(h[k]) && (l[k] <= h[k]);
       }
     } else {
       const uint32_t* ix = (const uint32_t*)(g.indices + (uint64_t)prim * g.istride);

@@ -53,9 +53,12 @@ struct RaySorter {
 // triangle tests. err receives bit 0 on a traversal stack overflow.
 // node_visits (optional, with the counters): one atomic increment per node
 // visit, indexed like the QNode8 array.
+// counter (optional): the persistent kernel's ray counter (one device word);
+// without it the sorter's is used, and with neither the grid covers every ray.
 void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t inst_id,
                   uint32_t* cnt_nodes, uint32_t* cnt_tris, uint32_t* err, hipStream_t s,
-                  const TraceParams& tp, RaySorter* sorter, uint32_t* node_visits = nullptr);
+                  const TraceParams& tp, RaySorter* sorter, uint32_t* node_visits = nullptr,
+                  uint32_t* counter = nullptr);
 
 // One persistent launch over several ray streams of scenes that share sc's
 // node and triangle arrays (a batched build). items (device memory) sorted
@@ -66,8 +69,10 @@ void launch_trace_batch(const DevScene& sc, const BatchItem* items, uint32_t nit
 
 // Two-level traversal of a scene with instances (top tree over InstRec
 // leaves, object trees in top.objects); otherwise as launch_trace. ctx_inst
-// is the query context's instID[0]; sorter supplies the ray counter.
+// is the query context's instID[0]; counter (one device word) or, if null,
+// the sorter's supplies the ray counter.
 void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t ctx_inst,
-                       uint32_t* err, hipStream_t s, const TraceParams& tp, RaySorter* sorter);
+                       uint32_t* err, hipStream_t s, const TraceParams& tp, RaySorter* sorter,
+                       uint32_t* counter = nullptr);
 
 }  // namespace rtg

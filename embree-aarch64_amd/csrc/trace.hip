@@ -994,7 +994,6 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
   bool active = false, occluded = false, pool_empty = false;
   WavePool pool;
   uint64_t iter = 0;
-  const float neg_inf = -INFINITY;
 
   while (true) {
     // ---- refill idle lanes (wave-uniform decision)
@@ -1284,7 +1283,7 @@ void launch_pl(const LaunchArgs& a, bool count, bool occluded, bool aligned) {
 
 void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t inst_id,
                   uint32_t* cnt_nodes, uint32_t* cnt_tris, uint32_t* err, hipStream_t s, const TraceParams& tp,
-                  RaySorter* sorter, uint32_t* node_visits) {
+                  RaySorter* sorter, uint32_t* node_visits, uint32_t* counter) {
   if (M == 0 || sc.num_nodes == 0) return;
   const bool aligned = ((uintptr_t)rays % 16 == 0) && (stride % 16 == 0);
   const dim3 grid(div_up(M, TB)), block(TB);
@@ -1299,18 +1298,21 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
     sorter->sort.sort_pairs(sorter->keys.as<uint32_t>(), sorter->vals.as<uint32_t>(), M, 0, 32, s);
     perm = sorter->vals.as<uint32_t>();
   }
-  const bool persistent = tp.persistent && sorter;
+  const bool persistent = tp.persistent && (sorter || counter);
   const int rmin = occluded ? tp.refill_min_occ : tp.refill_min;
   const int refill = rmin < 1 ? 1 : (rmin > 64 ? 64 : rmin);  // 64 idle lanes always refill
   dim3 pgrid(1);
   if (persistent) {
-    sorter->counter.reserve(256);
-    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, sorter->counter.as<uint32_t>());
+    if (!counter) {
+      sorter->counter.reserve(256);
+      counter = sorter->counter.as<uint32_t>();
+    }
+    hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, counter);
     const uint32_t want = (uint32_t)tp.cus * (uint32_t)tp.blocks_per_cu;
     pgrid = dim3(want < grid.x ? want : grid.x);
   }
   LaunchArgs la{sc, rp, M, stride, inst_id, cnt_nodes, cnt_tris, err, perm,
-                persistent ? sorter->counter.as<uint32_t>() : nullptr, refill, count ? node_visits : nullptr,
+                persistent ? counter : nullptr, refill, count ? node_visits : nullptr,
                 persistent, grid, pgrid, tp.lds_slots, s, tp.dual == 1 || (tp.dual == 2 && occluded)};
   if (sc.robust) launch_pl<true>(la, count, occluded, aligned);
   else launch_pl<false>(la, count, occluded, aligned);
@@ -1355,7 +1357,7 @@ void launch_trace_batch(const DevScene& sc, const BatchItem* items, uint32_t nit
 }
 
 void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t stride, bool occluded, uint32_t ctx_inst,
-                       uint32_t* err, hipStream_t s, const TraceParams& tp, RaySorter* sorter) {
+                       uint32_t* err, hipStream_t s, const TraceParams& tp, RaySorter* sorter, uint32_t* counter) {
   if (M == 0 || top.num_nodes == 0) return;
   const bool aligned = ((uintptr_t)rays % 16 == 0) && (stride % 16 == 0);
   const uint32_t blocks = div_up(M, TB);
@@ -1363,8 +1365,10 @@ void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t strid
   const dim3 grid(want < blocks ? want : blocks), block(TB);
   const int rmin = occluded ? tp.refill_min_occ_inst : tp.refill_min;
   const int refill = rmin < 1 ? 1 : (rmin > 64 ? 64 : rmin);
-  sorter->counter.reserve(256);
-  uint32_t* counter = sorter->counter.as<uint32_t>();
+  if (!counter) {
+    sorter->counter.reserve(256);
+    counter = sorter->counter.as<uint32_t>();
+  }
   hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, counter);
   char* rp = (char*)rays;
   if (occluded) {

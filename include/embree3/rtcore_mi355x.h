@@ -94,8 +94,9 @@ RTC_API void rtcgGenSoup(unsigned int numTris, unsigned long long seed, float h,
 RTC_API void rtcgGenRandomRays(unsigned int numRays, unsigned long long seed, float lo, float hi, float tfar,
                                struct RTCRayHit* rays);
 
-/* Heightfield: n x n vertices over [0,1]^2, z = amplitude * value-noise fBm
- * (octaves at frequency 4*2^o, weight 2^-o); returns 2*(n-1)^2 triangles
+/* Heightfield: n x n vertices over [0,1]^2, z = amplitude * Perlin
+ * gradient-noise fBm (improved noise with a quintic fade; octaves at
+ * frequency 4*2^o, weight 2^-o); returns 2*(n-1)^2 triangles
  * (NULL buffers: size query). Vertices n*n*3 floats, indices tris*3. */
 RTC_API unsigned int rtcgGenHeightfield(unsigned int n, unsigned long long seed, float amplitude,
                                         unsigned int octaves, float* vertices, unsigned int* indices);
@@ -159,7 +160,10 @@ RTC_API void rtcgTraceBatch(const struct RTCGBatchItem* items, unsigned int numI
 
 /* The same traversal captured once in a hipGraph (counter clear + launch;
  * the stream table stays on the device), for callers that replay it. The
- * graph keeps the scenes alive; the ray buffers must stay valid. A launch on
+ * graph keeps the scenes and their device arrays alive; a launch after any
+ * of its scenes was committed again (rtcCommitScene, another
+ * rtcgCommitScenes, rtcgAutotune's rebuilds) is RTC_ERROR_INVALID_OPERATION:
+ * make a new graph. The ray buffers must stay valid. A launch on
  * a NULL stream is synchronous and reports traversal errors; on a caller's
  * hipStream_t it is asynchronous. Replays of one graph must not overlap. */
 typedef struct RTCGBatchGraphTy* RTCGBatchGraph;

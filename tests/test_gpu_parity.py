@@ -689,27 +689,33 @@ def _aos_from_soa(soa, K, template, fields):
     return r
 
 
-def test_packet_and_soa_entry_points(rtc, device):
+def test_packet_and_soa_entry_points(rtc, device, oracle):
+    """Packet (4/8/16), NM, Np and 1Mp forms against the oracle. A packet
+    lane is active only where its valid word is -1, as the reference reads
+    the mask (kernels/bvh/bvh_intersector_hybrid.cpp:131): lanes holding 0 or
+    1 are left untouched."""
     v, ix = soup_case(rtc, 5000, 71, h=0.03)
     sc = rtc.Scene(device, [(v, ix)]).commit()
     L = rtc.lib()
     ctx = rtc.init_context()
     base = rtc.gen_random_rays(1600, seed=72)
-    base["tnear"][7::29] = 5.0  # inactive by tnear > tfar after a hit? no: tnear > inf is false; make tfar small
+    base["tnear"][7::29] = 5.0  # tnear > tfar: skipped
     base["tfar"][7::29] = 1.0
-    ref = trace(rtc, sc, base)
+    ref = oracle_trace(oracle, [(v, ix)], base)
     fields = list(rtc.RTCRayHit.names)
     rng = np.random.default_rng(73)
     for K, fn in ((4, L.rtcIntersect4), (8, L.rtcIntersect8), (16, L.rtcIntersect16)):
         soa = _soa(base, K, fields)
-        valid = np.where(rng.random(soa.shape[0] * K) < 0.8, -1, 0).astype(np.int32).reshape(-1, K)
+        u = rng.random(soa.shape[0] * K)
+        valid = np.where(u < 0.7, -1, np.where(u < 0.85, 0, 1)).astype(np.int32).reshape(-1, K)
         for p in range(soa.shape[0]):
             pk = np.ascontiguousarray(soa[p])
             fn(valid[p].ctypes.data, sc.h, ctypes.byref(ctx), pk.ctypes.data)
             soa[p] = pk
         device.check(f"rtcIntersect{K}")
         got = _aos_from_soa(soa, K, base, fields)
-        on = valid.reshape(-1) != 0
+        on = valid.reshape(-1) == -1
+        assert (valid == 1).any()
         n = soa.shape[0] * K
         assert records_equal(got[:n][on], ref[:n][on]), K
         assert records_equal(got[:n][~on], base[:n][~on]), K
@@ -719,7 +725,7 @@ def test_packet_and_soa_entry_points(rtc, device):
     oref = np.zeros(len(occ), rtc.RTCRay)
     for k in rtc.RTCRay.names:
         oref[k] = occ[k]
-    oref = trace(rtc, sc, oref, occluded=True)
+    oref = oracle_trace(oracle, [(v, ix)], oref, occluded=True)
     soa = _soa(occ, 8, list(rtc.RTCRay.names))
     valid = np.full(8, -1, np.int32)
     for p in range(soa.shape[0]):
