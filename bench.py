@@ -35,22 +35,58 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+CALIBRATION = os.path.join(ROOT, "tests", "golden", "cpu_calibration.json")
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_threads():
+    """Host threads this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS where the host sets one (a GPU box's per-GPU CPU share)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(n, int(omp))) if omp.isdigit() and int(omp) > 0 else max(1, n)
+
+
 def cpu_baseline(v, ix, rays, sample):
     """The CPU restatement (oracle/) on the host cores: BVH build over the full
-    soup and closest-hit traversal of a bounded ray sample."""
+    soup (parallel subtree builds) and closest-hit traversal of a bounded ray
+    sample on every thread this process may use. The restatement is scalar C,
+    not the reference's SIMD code: `r` (from tests/golden/cpu_calibration.json)
+    is its single-thread rate over the compiled reference's on the same
+    workload and host model, so value / r estimates the reference here."""
     from oracle import oracle as O
-    cores = min(16, os.cpu_count() or 1)
+    threads = cpu_threads()
     t0 = time.perf_counter()
-    bvh = O.BVH([(v, ix)])
+    bvh = O.BVH([(v, ix)], nthreads=threads)
     build_s = time.perf_counter() - t0
     r = rays[:sample].copy()
     t0 = time.perf_counter()
-    bvh.intersect(r, nthreads=cores)
+    bvh.intersect(r, nthreads=threads)
     trace_s = time.perf_counter() - t0
-    return {"value": sample / trace_s / 1e6, "unit": "Mrays/s", "cores": cores, "kind": "port",
-            "sample": f"{sample} of the {len(rays)} rays, closest hit, oracle BVH8 built single-threaded over "
-                      f"all {len(ix)} triangles",
-            "build_mtris_per_s": len(ix) / build_s / 1e6, "build_threads": 1}
+    value = sample / trace_s / 1e6
+    out = {"value": round(value, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": f"{sample} of the {len(rays)} rays, closest hit, oracle BVH8 built over all {len(ix)} triangles",
+           "cpu_model": cpu_model(), "host_cpus": os.cpu_count(),
+           "build_mtris_per_s": round(len(ix) / build_s / 1e6, 3), "build_threads": threads}
+    try:
+        cal = json.load(open(CALIBRATION))
+        out["calibration"] = {"r": cal["r"], "measured_on": cal["host_cpu"],
+                              "oracle_single_thread_mrays_per_s": cal["oracle_mrays_per_s_single_thread"],
+                              "reference_single_thread_mrays_per_s": cal["reference_mrays_per_s_single_thread"],
+                              "workload": cal["workload"]}
+        out["reference_estimate_mrays_per_s"] = round(value / cal["r"], 3)
+    except (OSError, ValueError, KeyError):
+        out["calibration"] = None
+    return out
 
 
 PROFILE = os.path.join(ROOT, "profiles", "trace_build_c2_soup1M_16Mrays.txt")
@@ -97,12 +133,13 @@ def counted(L, dev, sc, d_rays, d_src, n, nbytes):
 
 
 def learned_vs_sah(grid, width, height, bounces, train_steps, steps):
-    """Heightfield terrain (grid x grid vertices, value-noise fBm of 6
+    """Heightfield terrain (grid x grid vertices, Perlin-noise fBm of 6
     octaves, amplitude 0.2) seen by a width x height pinhole camera, followed
     by cosine-hemisphere bounces (depth `bounces` in all). Queries of every
     depth are traced on the plain binned-SAH tree; the split model is trained
-    on the node visits of the even-indexed queries and the learned tree is
-    timed against the plain tree on all queries."""
+    on the node visits of the even-indexed queries, and the learned tree is
+    timed against the plain tree on the odd-indexed queries only (held out:
+    never seen in training)."""
     L = R.lib()
     t0 = time.perf_counter()
     v, ix = R.gen_heightfield(grid, seed=11, amplitude=0.2, octaves=6)
@@ -117,11 +154,13 @@ def learned_vs_sah(grid, width, height, bounces, train_steps, steps):
         cur = R.gen_bounce(out, seed=100 + b)
         queries.append(cur)
     Q = np.concatenate(queries)
-    n, nbytes = len(Q), Q.nbytes
-    log(f"learned-vs-SAH: {len(ix)} tris, {n} queries ({[len(q) for q in queries]}), setup {time.perf_counter() - t0:.1f}s")
-    d_src, d_rays = dev.malloc(nbytes), dev.malloc(nbytes)
-    dev.memcpy(d_src, Q.ctypes.data, nbytes)
     train = np.ascontiguousarray(Q[::2])
+    held = np.ascontiguousarray(Q[1::2])
+    n, nbytes = len(held), held.nbytes
+    log(f"learned-vs-SAH: {len(ix)} tris, {len(Q)} queries ({[len(q) for q in queries]}), {len(train)} for "
+        f"training, {n} held out and timed, setup {time.perf_counter() - t0:.1f}s")
+    d_src, d_rays = dev.malloc(nbytes), dev.malloc(nbytes)
+    dev.memcpy(d_src, held.ctypes.data, nbytes)
     d_train = dev.malloc(train.nbytes)
     dev.memcpy(d_train, train.ctypes.data, train.nbytes)
     model = R.SplitModel(dev, seed=1)
@@ -158,7 +197,8 @@ def learned_vs_sah(grid, width, height, bounces, train_steps, steps):
     learned.release()
     dev.release()
     return {"workload": f"heightfield {grid}x{grid} vertices ({len(ix)} tris), {width}x{height} camera + "
-                        f"{bounces - 1} cosine bounces = {n} closest-hit queries",
+                        f"{bounces - 1} cosine bounces = {len(Q)} closest-hit queries; model trained on the "
+                        f"{len(train)} even-indexed, both trees timed on the {n} odd-indexed (held out)",
             "plain_sah": res["plain"], "learned": res["learned"],
             "speedup": round(res["learned"]["mrays_per_s"] / res["plain"]["mrays_per_s"], 4),
             "hits_identical": same,
@@ -170,8 +210,12 @@ def batched_scenes(num_scenes, steps):
     """Batched scenes: num_scenes displaced UV spheres (3-octave value noise,
     amplitude 0.1) with triangle counts log-uniform in [1e3, 2e6] and ray
     streams log-uniform in [4e3, 1e6] from a radius-2 sphere towards the unit
-    ball. One rtcgCommitScenes builds them all; one persistent launch,
-    captured in a hipGraph, traces every stream."""
+    ball. The tree shape (node width, leaf size, collapse triangle cost) and
+    the traversal launch (LDS stack entries, refill threshold, resident
+    blocks per CU) are autotuned first on the scene of median size with its
+    own ray stream; then one rtcgCommitScenes builds them all with the tuned
+    settings and one persistent launch, captured in a hipGraph, traces every
+    stream."""
     L = R.lib()
     work = R.batch_workload(num_scenes, seed=7)
     dev = R.Device("")
@@ -182,6 +226,17 @@ def batched_scenes(num_scenes, steps):
         scs.append(R.Scene(dev, [(v, ix)]))
         rays.append(R.gen_sphere_rays(nray, seed=2000 + i, radius=2.0))
     gen_s = time.perf_counter() - t0
+    # autotune on the median-size scene
+    ta = time.perf_counter()
+    med = int(np.argsort([t for _, t, _ in work])[len(work) // 2])
+    scs[med].commit()
+    d_sample = dev.malloc(rays[med].nbytes)
+    dev.memcpy(d_sample, rays[med].ctypes.data, rays[med].nbytes)
+    tuned = ctypes.create_string_buffer(256)
+    tuned_mrays = L.rtcgAutotune(scs[med].h, d_sample, len(rays[med]), 80, 0, 1, tuned, 256)
+    dev.check("rtcgAutotune")
+    dev.free(d_sample)
+    tune_s = time.perf_counter() - ta
     t1 = time.perf_counter()
     R.commit_scenes(scs)
     commit_s = time.perf_counter() - t1
@@ -215,6 +270,9 @@ def batched_scenes(num_scenes, steps):
     dev.release()
     ms = 1e3 * float(np.mean(times))
     return {"workload": f"{num_scenes} displaced spheres, {tri_total} triangles, {n} rays in {num_scenes} streams",
+            "autotuned": {"config": tuned.value.decode(), "on_scene": med, "scene_tris": int(work[med][1]),
+                          "scene_rays": int(work[med][2]), "mrays_per_s_on_scene": round(float(tuned_mrays), 2),
+                          "seconds": round(tune_s, 2)},
             "commit_ms_incl_upload": round(1e3 * commit_s, 2),
             "commit_mtris_per_s_incl_upload": round(tri_total / commit_s / 1e6, 2),
             "graph_ms": round(ms, 3), "mrays_per_s": round(n / (ms / 1e3) / 1e6, 2), "hit_rate": round(hit_rate, 4),
@@ -227,9 +285,12 @@ def instanced_scene(num_instances, width, height, steps, check_rays=2048):
     committed as scenes, and num_instances instances of them with a uniform
     random rotation, a uniform scale in [0.5, 2] and a translation uniform in
     [-50, 50]^3. Closest hit: width x height pinhole-camera rays from
-    (0, 20, -160) towards the origin; occlusion: one shadow ray from every
-    camera hit towards the point light (0, 100, 0). A subsample of both is
-    checked against the oracle's instance restatement over per-object BVHs."""
+    (0, 20, -160) towards the origin; occlusion: one shadow ray per camera
+    ray (as many as the camera rays: 2^25 at 8192 x 4096) towards the point
+    light (0, 100, 0), from the hit point, or for a miss from the ray's point
+    nearest the origin. A subsample of both is checked against the oracle's
+    instance restatement over per-object BVHs (the full-size parity test
+    checks 64k of each)."""
     from oracle import oracle as O
     L = R.lib()
     dev = R.Device("")
@@ -257,15 +318,9 @@ def instanced_scene(num_instances, width, height, steps, check_rays=2048):
     hits = np.zeros(n, R.RTCRayHit)
     dev.memcpy(hits.ctypes.data, d_rays, nbytes)
     hit = hits["geomID"] != R.RTC_INVALID_GEOMETRY_ID
-    # shadow rays from the camera hits
-    hh = hits[hit]
-    p = np.stack([hh["org_x"] + hh["tfar"] * hh["dir_x"], hh["org_y"] + hh["tfar"] * hh["dir_y"],
-                  hh["org_z"] + hh["tfar"] * hh["dir_z"]], 1).astype(np.float32)
-    sh = np.zeros(len(hh), R.RTCRay)
-    sh["org_x"], sh["org_y"], sh["org_z"] = p.T
-    d = (np.array([0.0, 100.0, 0.0], np.float32) - p).astype(np.float32)
-    sh["dir_x"], sh["dir_y"], sh["dir_z"] = d.T
-    sh["tnear"], sh["tfar"], sh["mask"] = 1e-4, 1.0, 0xFFFFFFFF
+    # one shadow ray per camera ray towards the light (from the hit point, or
+    # for a miss from the ray's point nearest the instance cloud's centre)
+    sh = R.gen_shadow_rays(hits, light=(0.0, 100.0, 0.0), centre=(0.0, 0.0, 0.0))
     ns = len(sh)
     d_ssrc, d_sh = dev.malloc(max(1, sh.nbytes)), dev.malloc(max(1, sh.nbytes))
     dev.memcpy(d_ssrc, sh.ctypes.data, sh.nbytes)

@@ -6,6 +6,8 @@
 #include <stdint.h>
 #include <string.h>
 
+#include <vector>
+
 #include "embree3/rtcore.h"
 #include "embree3/rtcore_mi355x.h"
 
@@ -143,23 +145,49 @@ RTC_API void rtcgGenRandomRays(unsigned int numRays, unsigned long long seed, fl
 
 namespace {
 
-// Value noise: hashed lattice values in [-1, 1], smoothstep-blended.
-float lattice(uint64_t seed, uint32_t oct, int32_t x, int32_t y) {
-  const uint64_t key = ((uint64_t)(uint32_t)x << 32) | (uint32_t)y;
-  return 2.0f * u01(seed, 16 + oct, key) - 1.0f;
-}
-
-float value_noise(uint64_t seed, uint32_t oct, float x, float y) {
-  const float fx = floorf(x), fy = floorf(y);
-  const int32_t ix = (int32_t)fx, iy = (int32_t)fy;
-  float tx = x - fx, ty = y - fy;
-  tx = tx * tx * (3.0f - 2.0f * tx);
-  ty = ty * ty * (3.0f - 2.0f * ty);
-  const float a = lattice(seed, oct, ix, iy), b = lattice(seed, oct, ix + 1, iy);
-  const float c = lattice(seed, oct, ix, iy + 1), d = lattice(seed, oct, ix + 1, iy + 1);
-  const float ab = a + (b - a) * tx, cd = c + (d - c) * tx;
-  return ab + (cd - ab) * ty;
-}
+// Gradient noise after Perlin's "Improved Noise" (SIGGRAPH 2002): a
+// seeded permutation of 0..255 hashes each lattice corner to one of eight
+// gradient directions (+-1,+-1), (+-1,0), (0,+-1); the corner's value is the
+// dot product of its gradient with the offset to the point, blended with the
+// quintic fade 6t^5 - 15t^4 + 10t^3. Zero at every lattice point; |n| <= 1.
+struct Perlin {
+  uint8_t perm[512];
+  Perlin(uint64_t seed, uint32_t oct) {
+    for (int i = 0; i < 256; i++) perm[i] = (uint8_t)i;
+    for (int i = 255; i > 0; i--) {  // Fisher-Yates with the counter-based generator
+      const int j = (int)(u01(seed, 64 + oct, (uint64_t)i) * (float)(i + 1));
+      const uint8_t t = perm[i];
+      perm[i] = perm[j < i ? j : i];
+      perm[j < i ? j : i] = t;
+    }
+    for (int i = 0; i < 256; i++) perm[256 + i] = perm[i];
+  }
+  static float fade(float t) { return t * t * t * (t * (t * 6.0f - 15.0f) + 10.0f); }
+  static float grad(uint32_t h, float x, float y) {
+    switch (h & 7u) {
+      case 0: return x + y;
+      case 1: return -x + y;
+      case 2: return x - y;
+      case 3: return -x - y;
+      case 4: return x;
+      case 5: return -x;
+      case 6: return y;
+      default: return -y;
+    }
+  }
+  float operator()(float x, float y) const {
+    const float fx = floorf(x), fy = floorf(y);
+    const uint32_t X = (uint32_t)(int32_t)fx & 255u, Y = (uint32_t)(int32_t)fy & 255u;
+    x -= fx;
+    y -= fy;
+    const float u = fade(x), v = fade(y);
+    const uint32_t a = perm[X] + Y, b = perm[X + 1] + Y;
+    const float n00 = grad(perm[a], x, y), n10 = grad(perm[b], x - 1.0f, y);
+    const float n01 = grad(perm[a + 1], x, y - 1.0f), n11 = grad(perm[b + 1], x - 1.0f, y - 1.0f);
+    const float nx0 = n00 + u * (n10 - n00), nx1 = n01 + u * (n11 - n01);
+    return 0.5f * (nx0 + v * (nx1 - nx0));
+  }
+};
 
 void normalize3(float v[3]) {
   const float l = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
@@ -179,20 +207,22 @@ void cross3(const float a[3], const float b[3], float r[3]) {
 }  // namespace
 
 // n x n vertex grid over [0,1]^2 (x, y), height z = amplitude * sum over
-// `octaves` of value noise at frequency 4*2^o with weight 2^-o; two
-// triangles per cell, 2*(n-1)^2 in all.
+// `octaves` of Perlin gradient noise at frequency 4*2^o with weight 2^-o
+// (one permutation per octave); two triangles per cell, 2*(n-1)^2 in all.
 RTC_API unsigned int rtcgGenHeightfield(unsigned int n, unsigned long long seed, float amplitude, unsigned int octaves,
                                         float* vertices, unsigned int* indices) {
   if (n < 2) return 0;
   const unsigned nt = 2 * (n - 1) * (n - 1);
   if (!vertices || !indices) return nt;
   const float inv = 1.0f / (float)(n - 1);
+  std::vector<Perlin> noise;
+  for (unsigned o = 0; o < octaves; o++) noise.emplace_back(seed, o);
   for (unsigned j = 0; j < n; j++)
     for (unsigned i = 0; i < n; i++) {
       const float x = (float)i * inv, y = (float)j * inv;
       float h = 0.0f, w = 1.0f, f = 4.0f;
       for (unsigned o = 0; o < octaves; o++) {
-        h += w * value_noise(seed, o, x * f, y * f);
+        h += w * noise[o](x * f, y * f);
         w *= 0.5f;
         f *= 2.0f;
       }

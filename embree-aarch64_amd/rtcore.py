@@ -389,6 +389,28 @@ def random_transforms(n, seed=5, scale=(0.5, 2.0), extent=50.0):
     return out
 
 
+def gen_shadow_rays(hits, light=(0.0, 100.0, 0.0), centre=(0.0, 0.0, 0.0), tnear=1e-4):
+    """One occlusion record (RTCRay) per traced closest-hit record, towards
+    the point light: from the hit point (org + tfar dir, float32) where the
+    ray hit; where it missed, from the point of the ray nearest `centre` (a
+    light sample inside the scene's volume, at t >= 0). Direction = light -
+    origin, unnormalised, so the light sits at t = 1: tfar = 1."""
+    hit = hits["geomID"] != RTC_INVALID_GEOMETRY_ID
+    o = np.stack([hits["org_x"], hits["org_y"], hits["org_z"]], 1).astype(np.float32)
+    d = np.stack([hits["dir_x"], hits["dir_y"], hits["dir_z"]], 1).astype(np.float32)
+    c = np.asarray(centre, np.float32)
+    t_near_c = np.maximum(np.float32(0), ((c - o) * d).sum(1, dtype=np.float32) / (d * d).sum(1, dtype=np.float32))
+    t = np.where(hit, hits["tfar"], t_near_c).astype(np.float32)
+    p = (o + t[:, None] * d).astype(np.float32)
+    sh = np.zeros(len(hits), RTCRay)
+    sh["org_x"], sh["org_y"], sh["org_z"] = p.T
+    dl = (np.asarray(light, np.float32) - p).astype(np.float32)
+    sh["dir_x"], sh["dir_y"], sh["dir_z"] = dl.T
+    sh["tnear"], sh["tfar"], sh["mask"] = tnear, 1.0, 0xFFFFFFFF
+    sh["id"] = np.arange(len(hits), dtype=np.uint32)
+    return sh
+
+
 def instance_workload(seed=11, sizes=None):
     """Base meshes of the instanced-scene workload: two displaced spheres and
     three heightfield terrain patches. sizes = (sphere numPhi, sphere numPhi,
