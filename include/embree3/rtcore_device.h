@@ -12,10 +12,19 @@ typedef struct RTCDeviceTy* RTCDevice;
 
 /* Creates a device from a config string (reference: rtcore_device.h:14,
  * kernels/common/rtcore.cpp:27, kernels/common/state.cpp:249-420 for the
- * key=value syntax). Recognised keys: verbose, threads (ignored), isa
- * (ignored), tri_builder (sah|morton), bvh_leaf (max triangles per leaf),
- * split_cost (sah|mlp), gpu (HIP device ordinal). Unknown keys are ignored
- * as the reference does. */
+ * key=value syntax). Recognised keys: verbose; gpu (HIP device ordinal);
+ * tri_builder or its alias builder (state.cpp:309-310): default (the build
+ * quality chooses: LOW the Morton tree, MEDIUM and HIGH the binned-SAH tree),
+ * sah and dynamic (binned SAH), morton or lbvh (Morton radix tree), at every
+ * quality; any other name fails rtcCommitScene with
+ * RTC_ERROR_INVALID_ARGUMENT, as the reference's factory does
+ * (bvh8_factory.cpp:878); bvh_leaf / max_leaf, bvh_width, cost_node,
+ * cost_tri (tree shape); trace_stack, refill_min, refill_min_occ,
+ * refill_min_occ_inst, blocks_per_cu, persistent, dual, ray_sort
+ * (traversal launch settings, as rtcgAutotune writes them). The reference's
+ * CPU keys (threads, isa, set_affinity, ...) and unknown keys are ignored as
+ * the reference ignores unknown keys. The learned split cost is chosen per
+ * scene (rtcgSetSceneSplitModel), not by a device key. */
 RTC_API RTCDevice rtcNewDevice(const char* config);
 
 /* Reference counting (reference: rtcore_device.h:17-20, rtcore.cpp:37-68). */
