@@ -106,6 +106,20 @@ def test_c_program_links_against_library(tmp_path, rtc):
                    check=True)
 
 
+def test_integration_example_compiles_as_c(tmp_path, rtc):
+    """The caller program INTEGRATION.md shows is strict C (gcc -Wall -Werror)
+    against these headers; the GPU test runs it."""
+    import re
+    text = open(os.path.join(os.path.dirname(INCLUDE), "INTEGRATION.md")).read()
+    m = re.search(r"```c example-app\n(.*?)```", text, re.S)
+    assert m, "INTEGRATION.md has no example-app block"
+    src = tmp_path / "app.c"
+    src.write_text(m.group(1))
+    libdir = os.path.dirname(rtc.LIB_PATH)
+    subprocess.run(["gcc", "-std=c11", "-O2", "-Wall", "-Werror", "-I", INCLUDE, str(src), "-L", libdir,
+                    "-l:libembree3.so", "-o", str(tmp_path / "app")], check=True)
+
+
 def _gpu_visible():
     return os.path.exists("/dev/kfd") and any(n.startswith("renderD") for n in os.listdir("/dev/dri")) \
         if os.path.isdir("/dev/dri") else False
