@@ -810,6 +810,7 @@ RTC_API RTCDevice rtcNewDevice(const char* config) {
     RTC_THROW(RTC_ERROR_UNSUPPORTED_CPU, "no usable GPU");
   }
   d->activate();
+  rtg::noalloc_setup(d->ordinal);
   {
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, d->ordinal) == hipSuccess && prop.multiProcessorCount > 0)
@@ -1485,6 +1486,51 @@ RTC_API void rtcgOccluded1MAsync(RTCScene scene, const RTCIntersectContext* cont
   API_BEGIN
   VERIFY_HANDLE(scene);
   async_trace(s, context, d_ray, M, byteStride, true, hipStream);
+  API_END(s)
+}
+
+// No reference counterpart: rtcgIntersect1MAsync / rtcgOccluded1MAsync for
+// callers that cannot allocate. The kernels' ray counter and error bits are
+// words 0 and 1 of the GPU's fixed scratch array; nothing here allocates
+// device memory or creates a stream.
+RTC_API void rtcgQuery1MNoAlloc(RTCScene scene, const RTCIntersectContext* context, void* d_rays, unsigned int M,
+                                size_t byteStride, int occluded, void* hipStream) {
+  Scene* s = SC(scene);
+  API_BEGIN
+  VERIFY_HANDLE(scene);
+  if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  if (context && context->filter) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "filter functions are not supported");
+  if (M == 0) return;
+  if (M > rtg::NOALLOC_MAX_RAYS)
+    RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "rtcgQuery1MNoAlloc: at most 2^31 rays per call");
+  const size_t rec = occluded ? sizeof(RTCRay) : sizeof(RTCRayHit);
+  if (!d_rays) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  if (byteStride < rec && M > 1) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "stride smaller than the ray record");
+  Device* d = s->dev;
+  d->activate();
+  if (!is_device_ptr(d, d_rays))
+    RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "rtcgQuery1MNoAlloc: the records must be in this device's memory");
+  uint32_t* scr = rtg::noalloc_scratch(d->ordinal);
+  if (!scr) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "rtcgQuery1MNoAlloc: no scratch on this GPU");
+  const hipStream_t st = (hipStream_t)hipStream;  // NULL: the null stream, synchronous
+  const uint32_t inst = context ? context->instID[0] : RTC_INVALID_GEOMETRY_ID;
+  const rtg::TraceParams tp = d->trace_params();
+  if (s->two_level) {
+    check_top(s);
+    rtg::launch_trace_inst(s->top_view, d_rays, M, byteStride, occluded != 0, inst, scr + 1, st, tp, nullptr, scr);
+  } else if (!s->bvh->empty(s->slot)) {
+    rtg::launch_trace(s->bvh->view(s->slot), d_rays, M, byteStride, occluded != 0, inst, nullptr, nullptr, scr + 1,
+                      st, tp, nullptr, nullptr, scr);
+  }
+  if (hipStream) return;
+  RTG_HIP(hipStreamSynchronize(st));
+  uint32_t err = 0;
+  RTG_HIP(hipMemcpy(&err, scr + 1, 4, hipMemcpyDeviceToHost));
+  if (err) {
+    rtg::fill_words(scr + 1, 1, 0u, st);
+    RTG_HIP(hipStreamSynchronize(st));
+    RTC_THROW(RTC_ERROR_UNKNOWN, (err & 1u) ? "traversal stack overflow" : "corrupt acceleration structure");
+  }
   API_END(s)
 }
 

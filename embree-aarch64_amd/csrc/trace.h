@@ -75,4 +75,18 @@ void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t strid
                        uint32_t* err, hipStream_t s, const TraceParams& tp, RaySorter* sorter,
                        uint32_t* counter = nullptr);
 
+// Scratch of the no-allocation queries: word 0 is the persistent kernel's
+// ray counter, word 1 the traversal error bits; one fixed array per GPU
+// (NOALLOC_SCRATCH_WORDS words), held until the process ends.
+constexpr uint32_t NOALLOC_SCRATCH_WORDS = 64;
+constexpr uint32_t NOALLOC_MAX_RAYS = 1u << 31;  // the counter word's range, with room for every wave's overshoot
+// Looks up the device address of this GPU's copy of the scratch array; the
+// GPU `ordinal` must be current. Once per GPU (later calls return at once);
+// throws hip_error when the lookup fails.
+void noalloc_setup(int ordinal);
+// Device address of GPU `ordinal`'s scratch array, or nullptr before noalloc_setup.
+uint32_t* noalloc_scratch(int ordinal);
+// Writes v to n words at p on stream s (a kernel, so the call can be captured).
+void fill_words(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s);
+
 }  // namespace rtg

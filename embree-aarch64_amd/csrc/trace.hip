@@ -23,6 +23,13 @@
 // 2^-12 relative).
 #include "trace.h"
 
+#include <mutex>
+
+// Scratch of the no-allocation queries (word 0: ray counter, word 1: error
+// bits), one copy per GPU. In host code this name is a host-side stand-in,
+// not a device address: kernels get the address noalloc_setup() looked up.
+static __device__ __align__(256) uint32_t scratch[rtg::NOALLOC_SCRATCH_WORDS];
+
 namespace rtg {
 
 constexpr int TB = 256;
@@ -1378,6 +1385,41 @@ void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t strid
     if (aligned) hipLaunchKernelGGL((k_trace_inst<false, true>), grid, block, 0, s, top, rp, M, stride, ctx_inst, err, counter, refill);
     else hipLaunchKernelGGL((k_trace_inst<false, false>), grid, block, 0, s, top, rp, M, stride, ctx_inst, err, counter, refill);
   }
+  RTG_HIP(hipGetLastError());
+}
+
+namespace {
+constexpr int NOALLOC_MAX_GPUS = 64;
+std::mutex g_noalloc_mtx;
+uint32_t* g_noalloc_ptr[NOALLOC_MAX_GPUS] = {};
+
+__global__ void k_fill_words(uint32_t* p, uint32_t n, uint32_t v) {
+  const uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+}  // namespace
+
+void noalloc_setup(int ordinal) {
+  if (ordinal < 0 || ordinal >= NOALLOC_MAX_GPUS) throw hip_error(hipErrorInvalidDevice, "GPU ordinal out of range");
+  std::lock_guard<std::mutex> g(g_noalloc_mtx);
+  if (g_noalloc_ptr[ordinal]) return;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || p == nullptr)
+    throw hip_error(e != hipSuccess ? e : hipErrorInvalidSymbol,
+                    std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_noalloc_ptr[ordinal] = static_cast<uint32_t*>(p);
+}
+
+uint32_t* noalloc_scratch(int ordinal) {
+  if (ordinal < 0 || ordinal >= NOALLOC_MAX_GPUS) return nullptr;
+  std::lock_guard<std::mutex> g(g_noalloc_mtx);
+  return g_noalloc_ptr[ordinal];
+}
+
+void fill_words(uint32_t* p, uint32_t n, uint32_t v, hipStream_t s) {
+  if (n == 0) return;
+  hipLaunchKernelGGL(k_fill_words, dim3(div_up(n, TB)), dim3(TB), 0, s, p, n, v);
   RTG_HIP(hipGetLastError());
 }
 

@@ -179,6 +179,7 @@ SIGNATURES = {
     "rtcgMemcpy": (None, [_vp, _vp, _vp, _sz]),
     "rtcgIntersect1MAsync": (None, [_vp, _vp, _vp, _u32, _sz, _vp]),
     "rtcgOccluded1MAsync": (None, [_vp, _vp, _vp, _u32, _sz, _vp]),
+    "rtcgQuery1MNoAlloc": (None, [_vp, _vp, _vp, _u32, _sz, _i32, _vp]),
     "rtcgSynchronize": (None, [_vp]),
     "rtcgGetErrorString": (ctypes.c_char_p, []),
     "rtcgTimeTrace": (None, [_vp, _vp, _vp, _u32, _sz, _i32, _i32, _vp]),

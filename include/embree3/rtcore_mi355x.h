@@ -26,6 +26,23 @@ RTC_API void rtcgIntersect1MAsync(RTCScene scene, const struct RTCIntersectConte
 RTC_API void rtcgOccluded1MAsync(RTCScene scene, const struct RTCIntersectContext* context, struct RTCRay* d_ray,
                                  unsigned int M, size_t byteStride, void* hipStream);
 
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the query of rtcgIntersect1MAsync (occluded = 0,
+ * RTCRayHit records) or rtcgOccluded1MAsync (occluded = 1, RTCRay records)
+ * over records in this device's memory, with the same results, allocating
+ * nothing. Its kernels keep their ray counter and error bits in a fixed
+ * 256-byte array per GPU that the library holds until the process ends.
+ * At most 2^31 rays per call (more is RTC_ERROR_INVALID_ARGUMENT); host
+ * records are RTC_ERROR_INVALID_ARGUMENT; the device's ray-reordering setting
+ * is not applied. Every call on one GPU shares that GPU's array, so calls that
+ * overlap on one GPU (other streams, other threads) race: the caller orders
+ * them. hipStream NULL: synchronous on the null stream, and reports traversal
+ * errors, including those left by earlier asynchronous calls on this GPU;
+ * a caller's hipStream_t: asynchronous. */
+RTC_API void rtcgQuery1MNoAlloc(RTCScene scene, const struct RTCIntersectContext* context, void* d_rays,
+                                unsigned int M, size_t byteStride, int occluded, void* hipStream);
+
 /* Message of the calling thread's most recent error (any device); the
  * pointer stays valid until the thread's next error. */
 RTC_API const char* rtcgGetErrorString(void);
