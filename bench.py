@@ -110,6 +110,28 @@ def profiled_traffic():
     return None, None
 
 
+def build_bytes_per_tri(n):
+    """Algorithmic HBM bytes per triangle of the binned-SAH build: every
+    stage reads its inputs and writes its outputs once.
+      primitive references: 3 indices + 3 vertices in, box + key out   84
+      Morton codes: box in, key + id out                                40
+      radix sort: 4 passes of 8 B key + id in and out                   64
+      boxes gathered in sorted order: id + box in, box out              68
+      copies into the level buffers: id + box in and out                72
+      each big level (segments above 4096): id + box read to bin and
+        again to flag, flags written and scanned, id + box + flag +
+        scan read and id + box + value + segment id written            176
+      each subtree level: id + box read to bin and to partition,
+        id + box + value written                                       112
+      boxes in final order                                              68
+      bottom-up boxes: box read and written per inner node              64
+      collapse: vertices read, 48 B triangle record written, 80 B node
+        per ~2.9 triangles                                             110
+    Big levels: ceil(log2(n / 4096)); subtree levels: log2(4096 / 16) = 8."""
+    big = max(0, int(np.ceil(np.log2(max(1.0, n / 4096.0)))))
+    return 84 + 40 + 64 + 68 + 72 + 176 * big + 112 * 8 + 68 + 64 + 110, big
+
+
 def time_trace(L, dev, sc, d_rays, d_src, n, steps):
     ms = np.zeros(max(1, steps), np.float32)
     L.rtcgTimeTrace(sc.h, d_rays, d_src, n, 80, 0, len(ms), ms.ctypes.data)
@@ -145,7 +167,7 @@ def learned_vs_sah(grid, width, height, bounces, train_steps, steps):
     v, ix = R.gen_heightfield(grid, seed=11, amplitude=0.2, octaves=6)
     cam = R.gen_camera(width, height, (0.5, -0.35, 0.5), (0.5, 0.55, 0.0), up=(0.0, 0.0, 1.0), fov=50.0)
     dev = R.Device("builder=sah")
-    plain = R.Scene(dev, [(v, ix)], quality=2).commit()
+    plain = R.Scene(dev, [(v, ix)]).commit()
     build_plain = plain.stats()["build_ms"]
     queries, cur = [cam], cam
     for b in range(bounces - 1):
@@ -449,6 +471,32 @@ def main():
     ms_step = float(ms.mean())
     mrays = n / (ms_step / 1e3) / 1e6
 
+    # HIGH quality (spatial splits) on the same triangles and rays
+    hsc = R.Scene(dev, [(v, ix)], quality=R.RTC_BUILD_QUALITY_HIGH).commit()
+    hst = hsc.stats()
+    hbms = np.zeros(max(3, args.steps // 4), np.float32)
+    L.rtcgTimeBuild(hsc.h, len(hbms), hbms.ctypes.data)
+    dev.check("rtcgTimeBuild high")
+    hms = time_trace(L, dev, hsc, d_rays, d_src, n, max(3, args.steps // 2))
+    high = {"triangle_records": int(hst["num_prims"]), "sah": round(hst["sah"], 3),
+            "build_ms": round(float(np.median(hbms)), 3),
+            "build_mtris_per_s": round(st["num_prims"] / (float(np.median(hbms)) / 1e3) / 1e6, 2),
+            "mrays_per_s": round(n / (hms / 1e3) / 1e6, 2)}
+    hsc.release()
+
+    # the drop-in caller's path: host records through rtcIntersect1M
+    # (upload, trace, download; PCIe-inclusive, never the headline value)
+    host_rays = rays.copy()
+    ctx = R.init_context()
+    L.rtcIntersect1M(sc.h, ctypes.byref(ctx), host_rays.ctypes.data, n, 80)
+    dev.check("rtcIntersect1M host records")
+    host_rays[:] = rays
+    t0 = time.perf_counter()
+    L.rtcIntersect1M(sc.h, ctypes.byref(ctx), host_rays.ctypes.data, n, 80)
+    host_s = time.perf_counter() - t0
+    dev.check("rtcIntersect1M host records")
+    del host_rays
+
     # occlusion pass, tfar = 0.1 (RTCRay records)
     occ = np.zeros(n, R.RTCRay)
     for k in R.RTCRay.names:
@@ -478,6 +526,12 @@ def main():
                 "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nbar, 2),
                 "tris_per_ray": round(tbar, 2),
                 "node_bytes_per_ray": round(nbar * node_b, 1)}
+
+    b_tri, big_levels = build_bytes_per_tri(st["num_prims"])
+    b_ach = b_tri * st["num_prims"] / (build_ms / 1e3) / 1e9
+    build_roofline = {"bound": "hbm", "achieved": round(b_ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                      "frac": round(b_ach / HBM_PEAK_GBS, 4), "bytes_per_tri": b_tri, "big_levels": big_levels,
+                      "build_ms": round(build_ms, 3)}
 
     for p in (d_src, d_rays, d_cn, d_ct, d_osrc):
         dev.free(p)
@@ -524,6 +578,10 @@ def main():
         "build_mtris_per_s": round(st["num_prims"] / (build_ms / 1e3) / 1e6, 2),
         "build_ms": round(build_ms, 3),
         "build_mtris_per_s_morton": round(st["num_prims"] / (build_ms_morton / 1e3) / 1e6, 2),
+        "build_roofline": build_roofline,
+        "high_quality_spatial_splits": high,
+        "host_records_mrays_per_s_incl_pcie": round(n / host_s / 1e6, 2),
+        "host_records_ms": round(1e3 * host_s, 2),
         "occlusion_mrays_per_s": round(occ_mrays, 2),
         "hit_rate": round(hit_rate, 4),
         "sah": round(st["sah"], 3),

@@ -73,6 +73,9 @@ struct Device : RefObj {
   // (reference bvh8_factory.cpp:865-870); an unknown name fails the commit.
   enum { TB_DEFAULT, TB_SAH, TB_MORTON, TB_UNKNOWN } tri_builder = TB_DEFAULT;
   std::string tri_builder_name = "default";
+  // HIGH quality: spatial splits add up to this fraction of extra triangle
+  // references (the reference's spatial-split SAH at HIGH, bvh8_factory.cpp:867-870)
+  float spatial_budget = 0.3f;
   std::mutex tls_mtx;
   std::unordered_map<std::thread::id, ThreadDev*> threads;
   ~Device() override {
@@ -261,6 +264,9 @@ struct Scene : RefObj {
   rtg::DevTop top_view{};
   std::vector<std::pair<std::shared_ptr<rtg::DeviceBVH>, uint64_t>> top_deps;
   std::vector<RefObj*> top_scenes;  // instanced scenes, retained while the top tree refers to them
+  std::vector<rtg::InstRec> top_recs;  // the top tree's records and their world boxes (lo xyz, hi xyz), as built
+  std::vector<float> top_boxes;
+  uint32_t top_obj_stack = 0;          // deepest object tree's traversal stack
   float bounds_lo[3] = {INFINITY, INFINITY, INFINITY};
   float bounds_hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   explicit Scene(Device* d) : dev(d) { ref_inc(d); }
@@ -321,6 +327,8 @@ void parse_config(Device* d, const char* cfg) {
     else if (k == "bvh_leaf" || k == "max_leaf") d->build.max_leaf = (uint32_t)atoi(v.c_str());
     else if (k == "cost_node") d->build.cost_node = (float)atof(v.c_str());
     else if (k == "cost_tri") d->build.cost_tri = (float)atof(v.c_str());
+    else if (k == "sah_block") d->build.block = (uint32_t)std::max(1, atoi(v.c_str()));
+    else if (k == "spatial_budget") d->spatial_budget = std::max(0.0f, (float)atof(v.c_str()));
     else if (k == "builder" || k == "tri_builder") {
       // reference state.cpp:309-310 (aliases) and bvh8_factory.cpp:871-878
       // (names): sah and the two-level "dynamic" SAH build the binned-SAH
@@ -355,6 +363,7 @@ rtg::BuildParams build_params(const Device* d, RTCBuildQuality q, RTCSceneFlags 
   rtg::BuildParams p = d->build_settings();
   p.quality = q;
   p.robust = (flags & RTC_SCENE_FLAG_ROBUST) ? 1 : 0;
+  p.split_budget = q == RTC_BUILD_QUALITY_HIGH ? d->spatial_budget : 0.0f;
   switch (d->tri_builder) {
     case Device::TB_DEFAULT: p.builder = q == RTC_BUILD_QUALITY_LOW ? rtg::BUILDER_MORTON : rtg::BUILDER_SAH; break;
     case Device::TB_SAH: p.builder = rtg::BUILDER_SAH; break;
@@ -641,10 +650,12 @@ void build_top(Scene* sc, Device* d, ThreadDev& t) {
   gd.geomID = 0;
   gd.scene = 0;
   gd.kind = rtg::GEOM_INSTANCES;
-  rtg::BuildParams p = build_params(d, sc->quality, sc->flags);
+  // the scene's split model, if any, prices the top tree's splits too
+  rtg::LearnedCost lc{};
+  rtg::BuildParams p = scene_params(sc, lc);
   p.builder = rtg::BUILDER_SAH;
   p.max_leaf = 1;
-  p.learned = nullptr;
+  p.split_budget = 0.0f;
   d->monitor((ssize_t)n * (80 + 64));
   {
     std::lock_guard<std::mutex> bl(d->build_mtx);
@@ -667,6 +678,9 @@ void build_top(Scene* sc, Device* d, ThreadDev& t) {
   v.num_objects = (uint32_t)objects.size();
   v.root = sc->top.slots.empty() ? 0u : sc->top.slots[0].root;
   v.max_iter = (uint64_t)v.num_nodes + n + (uint64_t)n * (obj_size + 2) + 16;
+  sc->top_recs = std::move(recs);
+  sc->top_boxes = std::move(boxes);
+  sc->top_obj_stack = obj_stack;
   for (Scene* o : object_scene)
     if (o != sc) {
       ref_inc(o);
@@ -1676,6 +1690,91 @@ rtg::SceneNorm scene_norm(const rtg::SceneSlot& st) {
   return sn;
 }
 
+// Spatial splits of a user build at HIGH quality (the reference passes
+// splitPrimitive to its spatial-split builder there,
+// kernels/common/rtcore_builder.cpp:379-380): while the caller's array has
+// room (primitiveArrayCapacity), the primitive whose cut most reduces the
+// sum of box half areas is cut by splitPrimitive across the largest axis of
+// its box, at the coarsest plane of the dyadic grid over all primitives'
+// bounds that lies inside it (the midpoint when rounding puts that plane
+// outside); the two halves replace it and may be cut again. Cuts that do
+// not reduce the half-area sum are not made.
+void user_spatial_splits(const RTCBuildArguments& a, std::vector<RTCBuildPrimitive>& prims) {
+  const size_t room = a.primitiveArrayCapacity > prims.size() ? a.primitiveArrayCapacity - prims.size() : 0;
+  if (room == 0 || prims.empty()) return;
+  float glo[3] = {INFINITY, INFINITY, INFINITY}, ghi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (const RTCBuildPrimitive& p : prims) {
+    glo[0] = std::min(glo[0], p.lower_x); glo[1] = std::min(glo[1], p.lower_y); glo[2] = std::min(glo[2], p.lower_z);
+    ghi[0] = std::max(ghi[0], p.upper_x); ghi[1] = std::max(ghi[1], p.upper_y); ghi[2] = std::max(ghi[2], p.upper_z);
+  }
+  auto lo = [](const RTCBuildPrimitive& p, int k) { return k == 0 ? p.lower_x : (k == 1 ? p.lower_y : p.lower_z); };
+  auto hi = [](const RTCBuildPrimitive& p, int k) { return k == 0 ? p.upper_x : (k == 1 ? p.upper_y : p.upper_z); };
+  auto area = [&](const RTCBuildPrimitive& p) {
+    return rtg::half_area(std::max(0.0f, p.upper_x - p.lower_x), std::max(0.0f, p.upper_y - p.lower_y),
+                          std::max(0.0f, p.upper_z - p.lower_z));
+  };
+  struct Cut {
+    float gain;
+    uint32_t i;
+    RTCBuildPrimitive l, r;
+  };
+  auto cut = [&](uint32_t i, Cut& c) {
+    const RTCBuildPrimitive& p = prims[i];
+    int ax = 0;
+    for (int k = 1; k < 3; k++)
+      if (hi(p, k) - lo(p, k) > hi(p, ax) - lo(p, ax)) ax = k;
+    const float l = lo(p, ax), h = hi(p, ax), ext = ghi[ax] - glo[ax];
+    if (!(h > l) || !(ext > 0.0f)) return false;
+    // coarsest dyadic plane in (l, h) of the grid over [glo, ghi]
+    const double ul = (l - glo[ax]) / (double)ext, uh = (h - glo[ax]) / (double)ext;
+    float pos = 0.5f * (l + h);
+    for (int level = 1; level <= 24; level++) {
+      const double step = std::ldexp(1.0, -level), m = std::floor(uh / step) * step;
+      if (m > ul && m < uh) {
+        const float x = (float)(glo[ax] + m * ext);
+        if (x > l && x < h) pos = x;
+        break;
+      }
+    }
+    RTCBounds lb{}, rb{};
+    a.splitPrimitive(&p, (unsigned)ax, pos, &lb, &rb, a.userPtr);
+    c.i = i;
+    c.l = p;
+    c.r = p;
+    c.l.lower_x = lb.lower_x; c.l.lower_y = lb.lower_y; c.l.lower_z = lb.lower_z;
+    c.l.upper_x = lb.upper_x; c.l.upper_y = lb.upper_y; c.l.upper_z = lb.upper_z;
+    c.r.lower_x = rb.lower_x; c.r.lower_y = rb.lower_y; c.r.lower_z = rb.lower_z;
+    c.r.upper_x = rb.upper_x; c.r.upper_y = rb.upper_y; c.r.upper_z = rb.upper_z;
+    c.gain = area(p) - area(c.l) - area(c.r);
+    return c.gain > 0.0f;
+  };
+  auto cmp = [](const Cut& x, const Cut& y) { return x.gain < y.gain; };
+  std::vector<Cut> heap;
+  for (uint32_t i = 0; i < prims.size(); i++) {
+    Cut c;
+    if (cut(i, c)) heap.push_back(c);
+  }
+  std::make_heap(heap.begin(), heap.end(), cmp);
+  size_t made = 0;
+  while (made < room && !heap.empty()) {
+    std::pop_heap(heap.begin(), heap.end(), cmp);
+    const Cut c = heap.back();
+    heap.pop_back();
+    prims[c.i] = c.l;
+    prims.push_back(c.r);
+    made++;
+    Cut x;
+    if (cut(c.i, x)) {
+      heap.push_back(x);
+      std::push_heap(heap.begin(), heap.end(), cmp);
+    }
+    if (cut((uint32_t)prims.size() - 1, x)) {
+      heap.push_back(x);
+      std::push_heap(heap.begin(), heap.end(), cmp);
+    }
+  }
+}
+
 // Host array -> device scratch.
 template <class T>
 T* upload(rtg::DevBuf& b, const T* host, size_t n, hipStream_t s) {
@@ -1805,7 +1904,28 @@ RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const v
   Device* d = s->dev;
   d->activate();
   ThreadDev& t = d->td();
-  const uint32_t nn = s->bvh->stats.num_nodes;
+  // a binary tree of the scene's triangles: its parent/child pairs are the
+  // binary splits the builder prices
+  std::vector<rtg::GeomDesc> descs = s->descs;
+  uint32_t off = 0;
+  for (rtg::GeomDesc& g : descs) {
+    g.scene = 0;
+    g.prim_offset = off;
+    off += g.num_tris;
+  }
+  rtg::LearnedCost none{};
+  rtg::BuildParams p = scene_params(s, none);
+  p.learned = nullptr;
+  p.builder = rtg::BUILDER_SAH;
+  p.width = 2;
+  p.split_budget = 0.0f;
+  rtg::DeviceBVH bin;
+  {
+    std::lock_guard<std::mutex> bl(d->build_mtx);
+    rtg::build_bvh(descs, p, d->scratch, bin, t.stream);
+  }
+  const uint32_t nn = bin.stats.num_nodes;
+  if (nn == 0) return;
   rtg::DevBuf copy;
   copy.reserve((size_t)M * byteStride);
   RTG_HIP(hipMemcpyAsync(copy.ptr, d_rays, (size_t)M * byteStride, hipMemcpyDeviceToDevice, t.stream));
@@ -1813,10 +1933,11 @@ RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const v
   m->m.counts.reserve((size_t)M * 8);
   RTG_HIP(hipMemsetAsync(m->m.visits.ptr, 0, (size_t)nn * 4, t.stream));
   uint32_t* cn = m->m.counts.as<uint32_t>();
-  rtg::launch_trace(s->bvh->view(s->slot), copy.ptr, M, byteStride, false, RTC_INVALID_GEOMETRY_ID, cn, cn + M,
+  rtg::launch_trace(bin.view(0), copy.ptr, M, byteStride, false, RTC_INVALID_GEOMETRY_ID, cn, cn + M,
                     t.errword.as<uint32_t>(), t.stream, d->trace_params(), &t.sorter, m->m.visits.as<uint32_t>());
   const rtg::SceneNorm sn = m->m.has_norm ? m->m.norm : scene_norm(s->bvh->slots[s->slot]);
-  rtg::mlp_append_samples(m->m, (const float*)s->bvh->info.ptr, m->m.visits.as<uint32_t>(), nn, M, sn, t.stream);
+  rtg::mlp_append_samples(m->m, bin.nodes.as<rtg::QNode8>(), bin.info.as<rtg::NodeInfo>(), m->m.visits.as<uint32_t>(),
+                          nn, sn, t.stream);
   m->m.norm = sn;
   m->m.has_norm = true;
   RTG_HIP(hipStreamSynchronize(t.stream));
@@ -2599,15 +2720,18 @@ RTC_API void* rtcBuildBVH(const RTCBuildArguments* args) {
   std::lock_guard<std::mutex> lk(b->mtx);
   b->arena.reset();
   RTCThreadLocalAllocator alloc = (RTCThreadLocalAllocator)&b->arena;
-  const uint32_t n = (uint32_t)args->primitiveCount;
-  if (n == 0) return args->createLeaf(alloc, args->primitives, 0, args->userPtr);
+  if (args->primitiveCount == 0) return args->createLeaf(alloc, args->primitives, 0, args->userPtr);
+  std::vector<RTCBuildPrimitive> prims(args->primitives, args->primitives + args->primitiveCount);
+  if (args->buildQuality == RTC_BUILD_QUALITY_HIGH && args->splitPrimitive)
+    user_spatial_splits(*args, prims);
+  const uint32_t n = (uint32_t)prims.size();
   Device* d = b->dev;
   d->activate();
   ThreadDev& t = d->td();
   // boxes to the device as GEOM_INSTANCES primitives (lo xyz, hi xyz)
   std::vector<float> boxes((size_t)n * 6);
   for (uint32_t i = 0; i < n; i++) {
-    const RTCBuildPrimitive& p = args->primitives[i];
+    const RTCBuildPrimitive& p = prims[i];
     float* f = &boxes[6 * (size_t)i];
     f[0] = p.lower_x; f[1] = p.lower_y; f[2] = p.lower_z;
     f[3] = p.upper_x; f[4] = p.upper_y; f[5] = p.upper_z;
@@ -2648,11 +2772,11 @@ RTC_API void* rtcBuildBVH(const RTCBuildArguments* args) {
     memcpy(&src, &tree.pbox[8 * (size_t)q + 7], 4);
     if (src >= n || placed[src]) RTC_THROW(RTC_ERROR_UNKNOWN, "inconsistent user tree");
     placed[src] = 1;
-    sorted[q] = args->primitives[src];
+    sorted[q] = prims[src];
   }
   uint32_t q = tree.n;
   for (uint32_t i = 0; i < n; i++)
-    if (!placed[i]) sorted[q++] = args->primitives[i];
+    if (!placed[i]) sorted[q++] = prims[i];
   memcpy(args->primitives, sorted.data(), sorted.size() * sizeof(RTCBuildPrimitive));
   if (tree.n == 0) return args->createLeaf(alloc, args->primitives, 0, args->userPtr);
   UserWalk w{tree, *args, args->primitives, alloc};

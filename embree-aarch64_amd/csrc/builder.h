@@ -10,6 +10,7 @@
 #include "bvh_types.h"
 #include "gpu_util.h"
 #include "sort.h"
+#include "spatial.h"
 
 namespace rtg {
 
@@ -26,11 +27,12 @@ struct BuildParams {
   int robust = 0;              // leaves hold (v0, v1, v2) for the Pluecker test
   uint32_t min_leaf = 1;       // subtrees of at most this many primitives become leaves
   uint32_t block = 1;          // a leaf's cost counts blocks of this many primitives
+  float split_budget = 0.0f;   // spatial splits: extra references up to this fraction of the triangles (0: none)
   const LearnedCost* learned = nullptr;  // binned-SAH split cost from a trained model (device data)
 };
 
 struct BuildStats {
-  uint32_t num_prims = 0;      // valid triangles
+  uint32_t num_prims = 0;      // triangle records: the valid triangles, more when spatial splits cut some
   uint32_t num_nodes = 0;
   uint32_t max_depth = 0;
   uint32_t max_stack = 0;      // traversal stack entries the deepest path can need
@@ -85,6 +87,9 @@ struct BuildScratch {
       isleaf, split8, visit, queue, counters, sacc, depth, lvl_hist, lvl_order;
   RadixSorter sorter;
   SahScratch sah;
+  SpatialScratch spatial;
+  LevelReadback rb;  // collapse level counters
+  PinnedBuf lvl_host;  // refit level histogram and offsets
 };
 
 // A binary tree over caller boxes with the SAH-optimal collapse decisions

@@ -9,6 +9,7 @@
 //     opens the child with the largest half area: kernels/builders/bvh_builder_sah.h:222-330;
 //   leaf fill e1 = v0-v1, e2 = v2-v0: kernels/geometry/triangle.h:40-41,98-120.
 #include <chrono>
+#include <cstring>
 #include <string>
 
 #include "builder.h"
@@ -27,7 +28,7 @@ struct BuildCounters {
   uint32_t error;  // bit 0: refit walk too long, bit 1: collapse index out of range, bit 2: bad tree
   uint32_t max_stack;  // most traversal stack entries any root-to-node path can need
   uint32_t roots_seen; // tree roots met by the tree check
-  uint32_t deep;       // a radix-tree node deeper than the level refit handles
+  uint32_t deep;       // a radix-tree node deeper than 254 levels (a broken parent link)
   double sah_nodes, sah_leaves, sah_leaves_tri;
 };
 
@@ -99,7 +100,8 @@ __global__ void k_init_counters(BuildCounters* c, uint32_t nscenes) {
   c->roots_seen = 0;
   c->deep = 0;
   c->tri_count = 0;
-  c->qcount[0] = c->qcount[1] = 0;
+  c->qcount[0] = 0;
+  c->qcount[1] = nscenes;  // the collapse's first level reads slot 1: one root item per scene
   c->max_depth = 0;
   c->max_stack = 0;
   c->error = 0;
@@ -460,38 +462,6 @@ __device__ inline bool refit_node(const RefitArgs& a, uint32_t node) {
   return true;
 }
 
-// Refit by walking up from every primitive: the second thread to reach a
-// node (a counter per node) processes it.
-__global__ __launch_bounds__(BT) void k_refit(RefitArgs a, const uint32_t* __restrict__ parent_int,
-                                              const uint32_t* __restrict__ parent_leaf, uint32_t* visit) {
-  const uint32_t n = a.n;
-  const uint32_t j = blockIdx.x * BT + threadIdx.x;
-  if (j >= n) return;
-  uint32_t node = parent_leaf[j];
-  for (int steps = 0; node != NONE; steps++) {
-    if (node >= n - 1 || steps > 512) {  // a radix tree over n keys is at most n-1 levels deep
-      atomicOr(a.err, 1u);
-      return;
-    }
-    // Hand-off to the sibling's thread: this thread's stores of the child it
-    // finished in the previous iteration (box, cost table, split bytes, leaf
-    // flag) must be visible, across XCDs, before the counter moves. The
-    // compiler's release sequence for an acq_rel atomic here lacked the
-    // vmcnt(0) waits on the loop back-edge (present in straight-line code),
-    // so the sibling's thread could read those stores stale and identical
-    // builds now and then gave different trees. Spelled out: release fence
-    // (L2 write-back), wait for the stores and the write-back, relaxed
-    // increment, acquire fence (wait, then L2 invalidate) before reading.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_s_waitcnt(0);
-    const uint32_t old = __hip_atomic_fetch_add(&visit[node], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    if (!refit_node(a, node)) return;
-    node = parent_int[node];
-  }
-}
-
 // Refit of one depth level of a tree whose inner nodes know their depth:
 // the deeper levels are complete (earlier launches), so no thread waits on
 // another and no cross-XCD hand-off is needed.
@@ -508,8 +478,8 @@ __global__ __launch_bounds__(BT) void k_refit_level(RefitArgs a, const uint32_t*
 }
 
 // Depth of every inner node of a radix tree (its ancestors counted by
-// walking the parent links); depths past 254 raise *deep and the caller
-// falls back to the bottom-up refit.
+// walking the parent links); a count past 254 (only a broken link makes one)
+// raises *deep and the build fails.
 __global__ __launch_bounds__(BT) void k_node_depth(const uint32_t* __restrict__ parent_int, uint32_t m,
                                                    uint8_t* __restrict__ depth, uint32_t* deep) {
   const uint32_t i = blockIdx.x * BT + threadIdx.x;
@@ -595,7 +565,7 @@ __global__ __launch_bounds__(BT) void k_check_tree(const uint32_t* __restrict__ 
 struct CollapseArgs {
   const QItem* qin;
   QItem* qout;
-  uint32_t nin;
+  const uint32_t* nin;  // items in qin (a device counter)
   uint32_t qout_slot;
   const uint32_t* child;
   const uint2* range;
@@ -652,9 +622,27 @@ __device__ inline uint32_t quant_hi(float v, float p, float s) {
   return qi;
 }
 
-__global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
-  const uint32_t t = blockIdx.x * BT + threadIdx.x;
-  if (t >= a.nin) return;
+// Slots from a shared counter for the wave's active lanes in one atomic:
+// lane values (below 2^bits) are summed bit by bit over ballots, so lanes
+// that have left the code path add nothing; returns this lane's first slot.
+__device__ inline uint32_t wave_alloc(uint32_t* counter, uint32_t v, int bits) {
+  const int lane = (int)(threadIdx.x & 63u);
+  const uint64_t active = __ballot(1);
+  const int leader = __ffsll((unsigned long long)active) - 1;
+  const uint64_t below = active & ((1ull << lane) - 1ull);
+  uint32_t prefix = 0, total = 0;
+  for (int b = 0; b < bits; b++) {
+    const uint64_t m = __ballot((v >> b) & 1u);
+    prefix += (uint32_t)__popcll(m & below) << b;
+    total += (uint32_t)__popcll(m) << b;
+  }
+  uint32_t base = 0;
+  if (lane == leader && total) base = atomicAdd(counter, total);
+  base = __shfl(base, leader, 64);
+  return base + prefix;
+}
+
+__device__ inline void collapse_item(const CollapseArgs& a, uint32_t t) {
   const QItem item = a.qin[t];
   const float eps = o2f(a.cnt->maxabs) * BOX_EPS_REL;
   if (item.bref == REF_EMPTY) {  // root of a scene without primitives: a node with no children
@@ -768,9 +756,10 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
     const uint32_t total_prims = (b & REF_LEAF) ? 1u : (rng32[2 * b + 1] - rng32[2 * b] + 1u);
     if (total_prims - ninner_prims != nt) atomicOr(&a.cnt->error, 64u);
   }
-  const uint32_t child_base = ni ? atomicAdd(&a.cnt->node_count, ni) : 0u;
-  const uint32_t tri_base = nt ? atomicAdd(&a.cnt->tri_count, nt) : 0u;
-  const uint32_t qbase = ni ? atomicAdd(&a.cnt->qcount[a.qout_slot], ni) : 0u;
+  // node, record and queue slots: one atomic per wave and counter
+  const uint32_t child_base = wave_alloc(&a.cnt->node_count, ni, 4);
+  const uint32_t tri_base = wave_alloc(&a.cnt->tri_count, nt, 9);
+  const uint32_t qbase = wave_alloc(&a.cnt->qcount[a.qout_slot], ni, 4);
   if ((uint64_t)child_base + ni > a.n || (uint64_t)tri_base + nt > a.tri_cap || (uint64_t)qbase + ni > a.n ||
       item.node >= a.n) {
     atomicOr(&a.cnt->error, 2u);
@@ -869,6 +858,17 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
   a.info[item.node] = inf;
 }
 
+// One level of the collapse: the grid strides over the level's queue, whose
+// length it reads on the device. A tree that failed its check (error bits
+// set) is not followed.
+__global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
+  if (a.cnt->error) return;
+  const uint32_t nin = *a.nin;
+  for (uint32_t t = blockIdx.x * BT + threadIdx.x; t < nin; t += gridDim.x * BT) collapse_item(a, t);
+}
+
+__global__ void k_zero_word(uint32_t* p) { *p = 0u; }
+
 // ------------------------------------------------------------------ host --
 
 void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, BuildScratch& S, DeviceBVH& out,
@@ -933,7 +933,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   std::vector<SceneAcc> hs(batched ? nscenes : 0);
   if (batched) RTG_HIP(hipMemcpyAsync(hs.data(), sacc, nscenes * sizeof(SceneAcc), hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));
-  const uint32_t n = hc.valid;
+  uint32_t n = hc.valid;
   out.stats.num_prims = n;
   for (int k = 0; k < 3; k++) {
     out.stats.bounds_lo[k] = o2f(hc.glo[k]);
@@ -974,6 +974,24 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   if (n == 0) {
     out.stats.num_nodes = 0;
     return;
+  }
+  // spatial splits (one scene of triangles, binned-SAH tree): the tree is
+  // built over the pieces, whose centroids may lie anywhere in the scene box
+  if (prm.split_budget > 0.0f && !batched && !user && geoms[0].kind == GEOM_TRIANGLES && trees.size() == 1 &&
+      (prm.builder == BUILDER_SAH || prm.learned)) {
+    const uint32_t nr = spatial_presplit(n, prm.split_budget, S.geoms.as<GeomDesc>(), out.stats.bounds_lo,
+                                         out.stats.bounds_hi, S.vals, S.lo4, S.hi4, S.spatial, s);
+    if (nr != n) {
+      n = nr;
+      out.stats.num_prims = n;
+      out.slots[0].num_prims = n;
+      count[0] = n;
+      trees[0].count = n;
+      for (int k = 0; k < 3; k++) {
+        trees[0].clo[k] = 2.0f * out.stats.bounds_lo[k];
+        trees[0].chi[k] = 2.0f * out.stats.bounds_hi[k];
+      }
+    }
   }
   if (user) user->n = n;
 
@@ -1021,59 +1039,56 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
                  S.nlo.as<float4>(), S.nhi.as<float4>(), S.cost.as<float>(), S.split8.as<uint8_t>(),
                  S.isleaf.as<uint8_t>(), prm.cost_node, prm.cost_tri, max_leaf, width, min_leaf, block, &cnt->error};
     // every inner node's depth (recorded by the binned-SAH builder, counted
-    // for the radix tree): refit one level per launch, deepest first
-    bool by_levels = true;
-    if (!use_sah) {
-      uint32_t* deep = &cnt->deep;
-      hipLaunchKernelGGL(k_node_depth, dim3(div_up(n - 1, BT)), dim3(BT), 0, s, S.parent_int.as<uint32_t>(), n - 1,
+    // for the radix tree): refit one level per launch, deepest first. A
+    // radix tree over 32-bit codes with index tie-breaks splits each range
+    // at a longer common prefix than its parent's, so it is at most 65 deep;
+    // a count beyond 254 means a broken parent link.
+    const uint32_t m = n - 1;
+    uint32_t* deep = &cnt->deep;
+    if (!use_sah)
+      hipLaunchKernelGGL(k_node_depth, dim3(div_up(m, BT)), dim3(BT), 0, s, S.parent_int.as<uint32_t>(), m,
                          S.depth.as<uint8_t>(), deep);
-      uint32_t hdeep = 0;
-      RTG_HIP(hipMemcpyAsync(&hdeep, deep, 4, hipMemcpyDeviceToHost, s));
-      RTG_HIP(hipStreamSynchronize(s));
-      by_levels = hdeep == 0;
+    S.lvl_hist.reserve(2 * 256 * 4);
+    S.lvl_order.reserve((size_t)m * 4);
+    S.lvl_host.reserve(2 * 257 * 4);
+    uint32_t* hist = S.lvl_hist.as<uint32_t>();
+    uint32_t* hh = S.lvl_host.as<uint32_t>();  // [0, 256): histogram, 256: deep flag, [257, 513): offsets
+    RTG_HIP(hipMemsetAsync(hist, 0, 2 * 256 * 4, s));
+    hipLaunchKernelGGL(k_depth_hist, dim3(div_up(m, BT)), dim3(BT), 0, s, S.depth.as<uint8_t>(), m, hist);
+    RTG_HIP(hipMemcpyAsync(hh, hist, 256 * 4, hipMemcpyDeviceToHost, s));
+    RTG_HIP(hipMemcpyAsync(hh + 256, deep, 4, hipMemcpyDeviceToHost, s));
+    RTG_HIP(hipStreamSynchronize(s));
+    if (hh[256]) throw hip_error(hipErrorUnknown, "radix tree deeper than 254 levels: broken parent links");
+    uint32_t* off = hh + 257;
+    uint32_t total = 0;
+    for (int d = 0; d < 256; d++) {
+      off[d] = total;
+      total += d < 255 ? hh[d] : 0u;
     }
-    if (by_levels) {
-      const uint32_t m = n - 1;
-      S.lvl_hist.reserve(2 * 256 * 4);
-      S.lvl_order.reserve((size_t)m * 4);
-      uint32_t* hist = S.lvl_hist.as<uint32_t>();
-      RTG_HIP(hipMemsetAsync(hist, 0, 2 * 256 * 4, s));
-      hipLaunchKernelGGL(k_depth_hist, dim3(div_up(m, BT)), dim3(BT), 0, s, S.depth.as<uint8_t>(), m, hist);
-      std::vector<uint32_t> h(256);
-      RTG_HIP(hipMemcpyAsync(h.data(), hist, 256 * 4, hipMemcpyDeviceToHost, s));
-      RTG_HIP(hipStreamSynchronize(s));
-      std::vector<uint32_t> off(256, 0);
-      uint32_t total = 0;
-      for (int d = 0; d < 255; d++) {
-        off[d] = total;
-        total += h[d];
-      }
-      RTG_HIP(hipMemcpyAsync(hist + 256, off.data(), 256 * 4, hipMemcpyHostToDevice, s));
-      hipLaunchKernelGGL(k_depth_scatter, dim3(div_up(m, BT)), dim3(BT), 0, s, S.depth.as<uint8_t>(), m, hist + 256,
-                         S.lvl_order.as<uint32_t>());
-      for (int d = 254; d >= 0; d--)
-        if (h[d])
-          hipLaunchKernelGGL(k_refit_level, dim3(div_up(h[d], BT)), dim3(BT), 0, s, ra,
-                             S.lvl_order.as<uint32_t>() + off[d], h[d], S.visit.as<uint32_t>());
-      RTG_HIP(hipStreamSynchronize(s));  // `off` is read by the copy above
-    } else {
-      hipLaunchKernelGGL(k_refit, dim3(div_up(n, BT)), dim3(BT), 0, s, ra, S.parent_int.as<uint32_t>(),
-                         S.parent_leaf.as<uint32_t>(), S.visit.as<uint32_t>());
-    }
+    RTG_HIP(hipMemcpyAsync(hist + 256, off, 256 * 4, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_depth_scatter, dim3(div_up(m, BT)), dim3(BT), 0, s, S.depth.as<uint8_t>(), m, hist + 256,
+                       S.lvl_order.as<uint32_t>());
+    for (int d = 254; d >= 0; d--)
+      if (hh[d])
+        hipLaunchKernelGGL(k_refit_level, dim3(div_up(hh[d], BT)), dim3(BT), 0, s, ra,
+                           S.lvl_order.as<uint32_t>() + off[d], hh[d], S.visit.as<uint32_t>());
     hipLaunchKernelGGL(k_check_tree, dim3(div_up(n, BT)), dim3(BT), 0, s,
                        use_sah ? nullptr : S.keys.as<uint32_t>(), n, &cnt->roots_seen,
                        S.child.as<uint32_t>(), S.parent_int.as<uint32_t>(), S.parent_leaf.as<uint32_t>(),
                        S.visit.as<uint32_t>(), &cnt->error);
     RTG_HIP(hipGetLastError());
-    // the collapse follows child links: stop here if the tree is inconsistent
+  }
+  // tree roots the check must meet (its error bits stop the collapse on the
+  // device; both are read back with the collapse's results)
+  uint32_t roots_expected = (uint32_t)trees.size();
+  for (uint32_t sI = 0; sI < nscenes; sI++) roots_expected += count[sI] == 1 ? 1u : 0u;
+  if (user) {
     RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
     RTG_HIP(hipStreamSynchronize(s));
-    uint32_t expect = (uint32_t)trees.size();
-    for (uint32_t sI = 0; sI < nscenes; sI++) expect += count[sI] == 1 ? 1u : 0u;
-    if (hc.error || hc.roots_seen != expect)
+    if (!trees.empty() && (hc.error || hc.roots_seen != roots_expected))
       throw hip_error(hipErrorUnknown, "binary tree check failed: error bits " + std::to_string(hc.error) +
                                            ", roots " + std::to_string(hc.roots_seen) + " of " +
-                                           std::to_string(expect));
+                                           std::to_string(roots_expected));
   }
 
   if (user) {  // the binary tree with its collapse decisions, for the caller's own nodes
@@ -1142,30 +1157,46 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   ca.tri_cap = n;
   ca.width = width;
   ca.robust = prm.robust ? 1u : 0u;
-  uint32_t nin = nscenes;
-  int levels = 0;
-  std::string level_sizes;
-  int slot = 0;
-  QItem* qin = q0;
-  QItem* qout = q1;
-  while (nin > 0) {
-    uint32_t zero = 0;
-    RTG_HIP(hipMemcpyAsync(&cnt->qcount[slot], &zero, 4, hipMemcpyHostToDevice, s));
-    ca.qin = qin;
-    ca.qout = qout;
-    ca.nin = nin;
-    ca.qout_slot = (uint32_t)slot;
-    hipLaunchKernelGGL(k_collapse, dim3(div_up(nin, BT)), dim3(BT), 0, s, ca);
+  // one launch per level of the 8-wide tree; the kernels read their queue
+  // lengths on the device, so the host queues level k + 1 before it learns
+  // (through a pinned slot) whether level k produced any items
+  const uint32_t cgrid = std::min<uint32_t>(div_up(cap, BT), 2048u);
+  auto enqueue_level = [&](uint32_t k) {
+    const uint32_t o = k & 1u;
+    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, &cnt->qcount[o]);
+    ca.qin = o ? q1 : q0;
+    ca.qout = o ? q0 : q1;
+    ca.nin = &cnt->qcount[o ^ 1u];
+    ca.qout_slot = o;
+    hipLaunchKernelGGL(k_collapse, dim3(cgrid), dim3(BT), 0, s, ca);
     RTG_HIP(hipGetLastError());
-    RTG_HIP(hipMemcpyAsync(&nin, &cnt->qcount[slot], 4, hipMemcpyDeviceToHost, s));
-    RTG_HIP(hipStreamSynchronize(s));
-    std::swap(qin, qout);
-    slot ^= 1;
-    level_sizes += " " + std::to_string(nin);
-    if (++levels > 4096 || nin > cap) throw hip_error(hipErrorUnknown, "BVH collapse did not terminate");
+    RTG_HIP(hipMemcpyAsync(S.rb.slot((int)k), cnt, sizeof(BuildCounters), hipMemcpyDeviceToHost, s));
+    RTG_HIP(hipEventRecord(S.rb.ev[k % LevelReadback::K], s));
+  };
+  S.rb.ensure(sizeof(BuildCounters));
+  std::string level_sizes;
+  enqueue_level(0);
+  for (uint32_t k = 1;; k++) {
+    if (k > 4096) {
+      (void)hipStreamSynchronize(s);
+      throw hip_error(hipErrorUnknown, "BVH collapse did not terminate");
+    }
+    enqueue_level(k);  // speculative: without input it does nothing
+    RTG_HIP(hipEventSynchronize(S.rb.ev[(k - 1) % LevelReadback::K]));
+    memcpy(&hc, S.rb.slot((int)(k - 1)), sizeof(hc));
+    const uint32_t next = hc.qcount[(k - 1) & 1u];
+    level_sizes += " " + std::to_string(next);
+    if (next == 0 || hc.error) break;
+    if (next > cap) {
+      (void)hipStreamSynchronize(s);
+      throw hip_error(hipErrorUnknown, "BVH collapse queue overflow");
+    }
   }
   RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));
+  if (!trees.empty() && (hc.roots_seen != roots_expected || (hc.error & 5u)))
+    throw hip_error(hipErrorUnknown, "binary tree check failed: error bits " + std::to_string(hc.error) + ", roots " +
+                                         std::to_string(hc.roots_seen) + " of " + std::to_string(roots_expected));
   if (hc.error || hc.tri_count != n)
     throw hip_error(hipErrorUnknown, "BVH build consistency check failed: error bits " + std::to_string(hc.error) +
                                          ", primitives " + std::to_string(n) + ", triangle records " +

@@ -13,30 +13,38 @@
 
 namespace rtg {
 
-// A second stream (default priority) for the tiny-segment finisher, which
-// runs beside the following levels, with one event per segment-list parity.
-struct SideStream {
-  hipStream_t s = nullptr;
-  hipEvent_t done[2] = {nullptr, nullptr};
-  void ensure() {
-    if (s) return;
-    RTG_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    for (hipEvent_t& e : done) RTG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+// Pinned host slots and events through which the host reads a level's
+// counters while the next level already runs.
+struct LevelReadback {
+  static constexpr int K = 2;
+  void* host = nullptr;
+  size_t bytes = 0;  // per slot
+  hipEvent_t ev[K] = {};
+  void ensure(size_t b) {
+    if (!ev[0])
+      for (hipEvent_t& e : ev) RTG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (b <= bytes) return;
+    if (host) (void)hipHostFree(host);
+    host = nullptr;
+    bytes = 0;
+    RTG_HIP(hipHostMalloc(&host, K * b, hipHostMallocDefault));
+    bytes = b;
   }
-  ~SideStream() {
-    for (hipEvent_t e : done)
+  void* slot(int k) const { return (char*)host + (size_t)(k % K) * bytes; }
+  ~LevelReadback() {
+    for (hipEvent_t e : ev)
       if (e) (void)hipEventDestroy(e);
-    if (s) (void)hipStreamDestroy(s);
+    if (host) (void)hipHostFree(host);
   }
-  SideStream() = default;
-  SideStream(const SideStream&) = delete;
-  SideStream& operator=(const SideStream&) = delete;
+  LevelReadback() = default;
+  LevelReadback(const LevelReadback&) = delete;
+  LevelReadback& operator=(const LevelReadback&) = delete;
 };
 
 struct SahScratch {
-  DevBuf idx[2], blo[2], bhi[2], sid[2], segs[4][2], bins, splits, flags, scan, counters, roots;
+  DevBuf idx[2], blo[2], bhi[2], sid[2], segs[2][2], subroots, bins, splits, flags, scan, counters, roots;
   Scanner scanner;
-  SideStream side;
+  LevelReadback rb;
 };
 
 // One tree of the forest: positions [first, first+count), count >= 2.

@@ -15,21 +15,28 @@
 // single primitives; below depth 48 every split is an object median, which
 // bounds the depth.
 //
-// Segments (contiguous position ranges still to split) are processed in three
-// size classes:
+// Segments (contiguous position ranges still to split) are split by size
+// class; the class fixes how a segment is split, whatever kernel does it:
 //   big   (> 4096): positions in chunks of 4096 bin into LDS, the chunk bins
 //                   merge into per-segment bins in HBM with atomics, one
-//                   thread per segment picks the split, and a global scan of
-//                   left flags gives every position its stable destination;
-//   mid   (65..4096): one 256-lane workgroup per segment bins, picks and
-//                   partitions in LDS;
-//   small (9..64): one 64-lane workgroup per segment, same code;
-//   tiny  (2..8): one thread finishes the whole subtree with exact SAH
-//                 splits (k_tiny), launched right after the level that made
-//                 the segment; a root segment of 2..64 starts as small.
+//                   workgroup per segment picks the split, and a global scan
+//                   of left flags gives every position its stable destination;
+//                   one level of big segments per step of the level loop;
+//   mid   (65..4096): a 256-lane group bins, picks and partitions in LDS;
+//   small (17..64): a 64-lane group (one wave), same code;
+//   tiny  (2..16): exact SAH splits over the references sorted by centroid
+//                  on each axis, the whole subtree at once; a tree's own root
+//                  of 2..64 references is split as small.
+// Every segment of at most SUB_MAX references is finished by one workgroup
+// of k_subtree (mid segments by the whole workgroup, small and tiny ones by
+// single waves); the level loop runs only while larger segments remain, and
+// its kernels read their segment counts on the device, so the host queues a
+// level before it learns whether the previous one left work.
 // Every inner node gets the id of its split position (first + n_left - 1),
 // which is unique, so the output does not depend on the order in which
 // segments are processed.
+#include <algorithm>
+#include <cstring>
 #include <string>
 
 #include "builder_sah.h"
@@ -44,6 +51,7 @@ constexpr int BIN_WORDS = 3 * NB * 7;   // counts, then lo xyz, then hi xyz, per
 constexpr uint32_t SMALL_MAX = 64;
 constexpr uint32_t TINY_MAX = 16;        // segments up to this size are finished by one thread (exact SAH)
 constexpr uint32_t MID_MAX = 4096;
+constexpr uint32_t SUB_MAX = 64;        // segments up to this size are finished by one wave (k_subtree)
 constexpr uint32_t CHUNK = 4096;        // positions per block of the big-segment binning
 constexpr uint32_t MEDIAN_DEPTH = 48;   // from this binary depth on, splits are object medians
 constexpr int TBS = 256;
@@ -68,16 +76,30 @@ struct Split {
   uint32_t lsid, rsid;  // ids of the children in the next big list (NIL if not big)
 };
 
+// A segment of at most SUB_MAX references whose whole subtree one
+// workgroup finishes, with the parity of the buffers holding its references.
+struct WSeg {
+  Seg s;
+  uint32_t par;
+  uint32_t flags;  // WSEG_BINNED: split by bins even when tiny (a tree's own root)
+};
+constexpr uint32_t WSEG_BINNED = 1u;
+
 struct Counters {
-  uint32_t nseg[4];  // next-level segment counts: big, mid, small, tiny
+  uint32_t nseg[2][2];  // segment counts by list parity: big, mid (a level reads one parity and fills the other)
+  uint32_t nroot;       // subtree segments so far (all levels)
   uint32_t error;    // bit 0: a segment list overflowed, bit 1: bin counts disagree, bit 2: destination out of range,
                      // bit 3: median subtree stack overflow, bit 4: a tiny segment out of range,
                      // bit 5: a depth beyond 254
 };
 
 struct Lists {
-  Seg* seg[4];
-  uint32_t cap[4];
+  Seg* seg[2];       // next level's big and mid segments
+  uint32_t* nseg;    // and their counts
+  uint32_t cap[2];
+  WSeg* roots;       // subtree segments
+  uint32_t root_cap;
+  uint32_t par;      // parity of the buffers this level writes
   uint32_t fin_max;  // segments up to this size are finished at once by median splits
 };
 
@@ -145,57 +167,6 @@ __device__ inline void bin_prim(uint32_t* w, const Mapping& m, float4 l, float4 
       atomicMax(&w[wh(a, b, k)], oh[k]);
     }
   }
-}
-
-// Best "bin < k" split along axis a from bin words w (LDS or HBM): least
-// A_L*n_L + A_R*n_R, first minimum in k. Returns false when no k separates
-// the primitives.
-__device__ inline bool best_on_axis(const uint32_t* w, const Mapping& m, int a, float& cost, int& kbest,
-                                    uint32_t& nleft) {
-  if (m.scale[a] == 0.0f) return false;
-  float ra[NB];
-  uint32_t rn[NB];
-  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
-  uint32_t nn = 0;
-  for (int b = m.nb - 1; b >= 1; b--) {
-    const uint32_t cb = w[wc(a, b)];
-    if (cb) {
-      nn += cb;
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        lo[k] = fminf(lo[k], o2f(w[wl(a, b, k)]));
-        hi[k] = fmaxf(hi[k], o2f(w[wh(a, b, k)]));
-      }
-    }
-    rn[b] = nn;
-    ra[b] = nn ? half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) : 0.0f;
-  }
-  bool found = false;
-  cost = INFINITY;
-  nn = 0;
-#pragma unroll
-  for (int k = 0; k < 3; k++) { lo[k] = INFINITY; hi[k] = -INFINITY; }
-  for (int b = 0; b + 1 < m.nb; b++) {
-    const uint32_t cb = w[wc(a, b)];
-    if (cb) {
-      nn += cb;
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        lo[k] = fminf(lo[k], o2f(w[wl(a, b, k)]));
-        hi[k] = fmaxf(hi[k], o2f(w[wh(a, b, k)]));
-      }
-    }
-    const uint32_t nr = rn[b + 1];
-    if (nn == 0 || nr == 0) continue;
-    const float c = half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) * (float)nn + ra[b + 1] * (float)nr;
-    if (c < cost) {
-      cost = c;
-      kbest = b + 1;
-      nleft = nn;
-      found = true;
-    }
-  }
-  return found;
 }
 
 // Box of bins [b0, b1) on axis a.
@@ -275,12 +246,6 @@ __device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m
     finish_median(t, m, side, depth, first, count);
     return NIL;
   }
-  const int cls = count > MID_MAX ? 0 : (count > SMALL_MAX ? 1 : (count > TINY_MAX ? 2 : 3));
-  const uint32_t id = atomicAdd(&t.cnt->nseg[cls], 1u);
-  if (id >= nx.cap[cls]) {
-    atomicOr(&t.cnt->error, 1u);
-    return NIL;
-  }
   Seg c;
   c.first = first;
   c.count = count;
@@ -291,23 +256,135 @@ __device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m
     c.clo[k] = clo[k];
     c.chi[k] = chi[k];
   }
+  if (count <= SUB_MAX) {
+    const uint32_t id = atomicAdd(&t.cnt->nroot, 1u);
+    if (id >= nx.root_cap) {
+      atomicOr(&t.cnt->error, 1u);
+      return NIL;
+    }
+    nx.roots[id] = WSeg{c, nx.par, 0u};
+    return NIL;
+  }
+  const int cls = count > MID_MAX ? 0 : 1;
+  const uint32_t id = atomicAdd(&nx.nseg[cls], 1u);
+  if (id >= nx.cap[cls]) {
+    atomicOr(&t.cnt->error, 1u);
+    return NIL;
+  }
   nx.seg[cls][id] = c;
   return cls == 0 ? id : NIL;
 }
 
+// Synchronisation of the threads that share a segment's LDS: the whole
+// workgroup, or one wave. A wave's LDS instructions execute in order, so
+// between its lanes it takes only a wait for its LDS operations
+// (lgkmcnt(0); vmcnt and expcnt left at their maxima) and a compiler fence
+// that keeps loads and stores on their side; it does not wait for the
+// wave's global stores.
+template <bool WAVE>
+__device__ inline void group_sync() {
+  if constexpr (WAVE) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0xC07F);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  } else {
+    __syncthreads();
+  }
+}
+
+// One wave's global stores visible to its own later loads (the wave's
+// references written by a split, read back by the finisher).
+__device__ inline void wave_sync_global() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
+
+// Plain binned-SAH sweep of axis a by one half-wave (32 lanes; lane b holds
+// bin b): the least A_L*n_L + A_R*n_R over the "bin < k" candidates, first
+// minimum in k, with the float operations of a left-to-right scan (prefix
+// and suffix boxes are exact minima and maxima, so the costs are the scan's
+// bit for bit). cost is INFINITY when no k separates the references; every
+// lane of the half returns the result and the axis's reference count.
+__device__ inline void sweep_axis_half(const uint32_t* w, const Mapping& m, int a, int b, float& cost, int& kbest,
+                                       uint32_t& nleft, uint32_t& total) {
+  const uint32_t c = b < m.nb ? w[wc(a, b)] : 0u;
+  float llo[3], lhi[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    llo[k] = c ? o2f(w[wl(a, b, k)]) : INFINITY;
+    lhi[k] = c ? o2f(w[wh(a, b, k)]) : -INFINITY;
+  }
+  float rlo[3] = {llo[0], llo[1], llo[2]}, rhi[3] = {lhi[0], lhi[1], lhi[2]};
+  uint32_t nl = c, nr = c;
+#pragma unroll
+  for (int d = 1; d < 32; d <<= 1) {
+    const uint32_t un = __shfl_up(nl, d, 32), dn = __shfl_down(nr, d, 32);
+    float ul[3], uh[3], dl[3], dh[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      ul[k] = __shfl_up(llo[k], d, 32);
+      uh[k] = __shfl_up(lhi[k], d, 32);
+      dl[k] = __shfl_down(rlo[k], d, 32);
+      dh[k] = __shfl_down(rhi[k], d, 32);
+    }
+    if (b >= d) {
+      nl += un;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        llo[k] = fminf(llo[k], ul[k]);
+        lhi[k] = fmaxf(lhi[k], uh[k]);
+      }
+    }
+    if (b + d < 32) {
+      nr += dn;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        rlo[k] = fminf(rlo[k], dl[k]);
+        rhi[k] = fmaxf(rhi[k], dh[k]);
+      }
+    }
+  }
+  // candidate k = b + 1: bins [0, b] left, the suffix from bin b + 1 right
+  const uint32_t nr1 = __shfl_down(nr, 1, 32);
+  float r1l[3], r1h[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    r1l[k] = __shfl_down(rlo[k], 1, 32);
+    r1h[k] = __shfl_down(rhi[k], 1, 32);
+  }
+  float cst = INFINITY;
+  if (m.scale[a] != 0.0f && b + 1 < m.nb && nl != 0u && nr1 != 0u)
+    cst = half_area(lhi[0] - llo[0], lhi[1] - llo[1], lhi[2] - llo[2]) * (float)nl +
+          half_area(r1h[0] - r1l[0], r1h[1] - r1l[1], r1h[2] - r1l[2]) * (float)nr1;
+  int kk = b + 1;
+  uint32_t nlv = nl;
+  total = __shfl(nl, 31, 32);
+#pragma unroll
+  for (int d = 16; d >= 1; d >>= 1) {
+    const float oc = __shfl_xor(cst, d, 32);
+    const int ok = __shfl_xor(kk, d, 32);
+    const uint32_t on = __shfl_xor(nlv, d, 32);
+    if (oc < cst || (oc == cst && ok < kk)) {
+      cst = oc;
+      kk = ok;
+      nlv = on;
+    }
+  }
+  cost = cst;
+  kbest = kk;
+  nleft = nlv;
+}
+
 // Workgroup-level choice of the split of segment sg from its bins w (LDS or
-// HBM); every thread returns the same Split. Plain SAH: one thread per
+// HBM); every thread returns the same Split. Plain SAH: one half-wave per
 // axis sweeps its bins. Learned cost (lc.P set and the segment larger than
 // lc.min_count): the workgroup evaluates the model on the left and right
-// boxes of all 3*(nb-1) candidates (their boxes and counts from the bins),
-// then one thread per axis keeps the least P(L)*n_L + P(R)*n_R with
-// P = halfArea * 2^model (bounds below) among the candidates whose plain
-// cost A(L)*n_L + A(R)*n_R is within (1 + lc.sah_slack) of the least over
-// all axes.
+// sides of all 3*(nb-1) candidates (each side's box and count from the
+// bins, against the segment's), then one thread per axis keeps the least
+// P(L)*n_L + P(R)*n_R with P = halfArea * 2^model (bounds below) among the
+// candidates whose plain cost A(L)*n_L + A(R)*n_R is within
+// (1 + lc.sah_slack) of the least over all axes.
 // Ties keep the lowest axis and the lowest k; without a separating
 // candidate the split is the object median.
 struct ChooseLds {
-  float yp, pp;  // learned log rate and visit weight of the segment's own box
+  float slo[3], shi[3];  // learned: the segment's box
   float cost[3];
   float amin[3];  // learned: least plain cost per axis
   int k[3];
@@ -328,32 +405,25 @@ __device__ inline float split_area_cost(float al, float ar, uint32_t nl, uint32_
   return al * (float)nl + ar * (float)nr;
 }
 
-template <int BLOCK>
+template <int BLOCK, bool WAVE, bool LEARNED>
 __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const Seg& sg, const LearnedCost& lc,
-                                     ChooseLds& L, uint32_t* err) {
-  const int tid = threadIdx.x;
+                                     ChooseLds& L, uint32_t* err, int tid) {
   const uint32_t depth = sg.meta >> 1;
-  const bool learned = lc.P != nullptr && sg.count > lc.min_count && depth < MEDIAN_DEPTH;
+  const bool learned = LEARNED && lc.P != nullptr && sg.count > lc.min_count && depth < MEDIAN_DEPTH;
   const int ncand = m.nb - 1;
   if (learned) {
-    // the segment's own prediction bounds its children's: a child is never
-    // more likely to be visited than its parent, and its learned rate per
-    // unit area stays within 2x of the parent's, which stays within 4x of
-    // the uniform-ray rate. A model queried far from its training boxes
-    // could otherwise price a split at almost nothing and the tree would
-    // degenerate (measured: 6 Mrays/s after 3000 training steps with an
-    // 8x band).
+    // the segment's box and count (the union of one axis's bins)
     if (tid == 0) {
-      float lo[3], hi[3];
       uint32_t cnt;
-      range_box(w, 0, 0, m.nb, lo, hi, cnt);
-      float f[MLP_IN];
-      box_features(lo, hi, sg.count, lc.sn, f);
-      L.yp = fminf(fmaxf(mlp_eval(lc.P, f), -2.0f), 2.0f);
-      L.pp = half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) * exp2f(L.yp);
+      range_box(w, 0, 0, m.nb, L.slo, L.shi, cnt);
     }
-    __syncthreads();
-    const float yp = L.yp, pp = L.pp;
+    group_sync<WAVE>();
+    const float slo[3] = {L.slo[0], L.slo[1], L.slo[2]}, shi[3] = {L.shi[0], L.shi[1], L.shi[2]};
+    // every candidate's two sides: the per-bin statistics (box and count of
+    // the side's bins, against the segment's) go through the model, which
+    // corrects the SAH's entry-probability estimate A_C / A_S by 2^y; a
+    // prediction is held within [MLP_Y_MIN, MLP_Y_MAX] and never makes a
+    // child more likely to be entered than its parent
     for (int e = tid; e < 2 * 3 * ncand; e += BLOCK) {
       const int c = e >> 1, side = e & 1;
       const int a = c / ncand, k = c % ncand + 1;
@@ -363,15 +433,16 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
       float pv = -1.0f, av = -1.0f;
       if (cnt > 0 && m.scale[a] != 0.0f) {
         float f[MLP_IN];
-        box_features(lo, hi, cnt, lc.sn, f);
-        const float y = fminf(fmaxf(mlp_eval(lc.P, f), yp - 1.0f), yp + 1.0f);
+        split_features(lo, hi, cnt, slo, shi, sg.count, lc.sn, f);
+        const float ymax = fminf(MLP_Y_MAX, -16.0f * f[0]);
+        const float y = fminf(fmaxf(mlp_eval(lc.P, f), MLP_Y_MIN), ymax);
         av = half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]);
-        pv = fminf(av * exp2f(y), pp);
+        pv = av * exp2f(y);
       }
       L.p[e] = pv;
       L.a[e] = av;
     }
-    __syncthreads();
+    group_sync<WAVE>();
     if (tid < 3) {
       float c = INFINITY;
       uint32_t run = 0, total = 0;
@@ -384,9 +455,27 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
       }
       L.amin[tid] = c;
     }
-    __syncthreads();
+    group_sync<WAVE>();
   }
-  if (tid < 3) {
+  if (!learned && tid < 64) {
+    // one half-wave per axis: x and y, then z
+    const int half = tid >> 5, b = tid & 31;
+    for (int pass = 0; pass < 2; pass++) {
+      const int a = pass == 0 ? half : 2;
+      if (pass == 1 && half == 1) break;
+      float c;
+      int k;
+      uint32_t nl, total;
+      sweep_axis_half(w, m, a, b, c, k, nl, total);
+      if (b == 0) {
+        if (total != sg.count) atomicOr(err, 2u);
+        L.cost[a] = depth < MEDIAN_DEPTH && total == sg.count ? c : INFINITY;
+        L.k[a] = k;
+        L.nl[a] = nl;
+      }
+    }
+  }
+  if (learned && tid < 3) {
     float c = INFINITY;
     int k = 0;
     uint32_t nl = 0;
@@ -394,9 +483,7 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
     for (int b = 0; b < m.nb; b++) total += w[wc(tid, b)];
     if (total != sg.count) atomicOr(err, 2u);
     if (depth < MEDIAN_DEPTH && total == sg.count) {
-      if (!learned) {
-        if (!best_on_axis(w, m, tid, c, k, nl)) c = INFINITY;
-      } else if (m.scale[tid] != 0.0f) {
+      if (m.scale[tid] != 0.0f) {
         // an unbounded slack admits every candidate (also where the least
         // plain cost is 0, which would make 0 * inf a NaN)
         const float guard = __builtin_isinf(lc.sah_slack)
@@ -422,7 +509,7 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
     L.k[tid] = k;
     L.nl[tid] = nl;
   }
-  __syncthreads();
+  group_sync<WAVE>();
   if (tid == 0) {
     Split sp;
 #pragma unroll
@@ -446,7 +533,7 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
     sp.lsid = sp.rsid = NIL;
     L.sp = sp;
   }
-  __syncthreads();
+  group_sync<WAVE>();
   return L.sp;
 }
 
@@ -457,15 +544,17 @@ __global__ void k_fill_u32(uint32_t* p, size_t n, uint32_t v) {
   if (i < n) p[i] = v;
 }
 
-__global__ void k_init_gbins(uint32_t* g, uint32_t nseg) {
+// Bins of the level's big segments; the grid covers the list's capacity
+// and the count is read on the device.
+__global__ void k_init_gbins(uint32_t* g, const uint32_t* nseg) {
   const size_t i = (size_t)blockIdx.x * TBS + threadIdx.x;
-  if (i >= (size_t)nseg * BIN_WORDS) return;
+  if (i >= (size_t)*nseg * BIN_WORDS) return;
   const int w = (int)(i % BIN_WORDS);
   g[i] = w < 3 * NB ? 0u : (w < 3 * NB + 9 * NB ? NIL : 0u);
 }
 
-__global__ void k_zero_counts(Counters* c) {
-  if (threadIdx.x < 4) c->nseg[threadIdx.x] = 0u;
+__global__ void k_zero_counts(uint32_t* nseg) {
+  if (threadIdx.x < 2) nseg[threadIdx.x] = 0u;
 }
 
 // A chunk of CHUNK positions meets at most two big segments (each is longer
@@ -474,18 +563,23 @@ __global__ void k_zero_counts(Counters* c) {
 __global__ __launch_bounds__(TBS) void k_big_bins(const uint32_t* __restrict__ sid, const float4* __restrict__ blo,
                                                   const float4* __restrict__ bhi, uint32_t n,
                                                   const Seg* __restrict__ segs, uint32_t* gbins) {
-  __shared__ uint32_t sb[2][BIN_WORDS];
+  // one set of bins per wave and segment slot (fewer lanes per LDS atomic),
+  // merged before the merge into HBM
+  constexpr int NW = TBS / 64;
+  __shared__ uint32_t sb[2][NW][BIN_WORDS];
   __shared__ uint32_t s_lo, s_hi;
   __shared__ Mapping sm[2];
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x, wave = tid >> 6;
   const uint32_t c0 = blockIdx.x * CHUNK;
   const uint32_t c1 = c0 + CHUNK < n ? c0 + CHUNK : n;
   if (tid == 0) {
     s_lo = NIL;
     s_hi = 0u;
   }
-  init_bins(sb[0], tid, TBS);
-  init_bins(sb[1], tid, TBS);
+  for (int w = 0; w < NW; w++) {
+    init_bins(sb[0][w], tid, TBS);
+    init_bins(sb[1][w], tid, TBS);
+  }
   __syncthreads();
   uint32_t mn = NIL, mx = 0u;
   for (uint32_t i = c0 + tid; i < c1; i += TBS) {
@@ -509,36 +603,45 @@ __global__ __launch_bounds__(TBS) void k_big_bins(const uint32_t* __restrict__ s
     const uint32_t s = sid[i];
     if (s == NIL) continue;
     const float4 l = blo[i], h = bhi[i];
-    if (s == slo_id) bin_prim(sb[0], sm[0], l, h);
-    else if (s == shi_id) bin_prim(sb[1], sm[1], l, h);
+    if (s == slo_id) bin_prim(sb[0][wave], sm[0], l, h);
+    else if (s == shi_id) bin_prim(sb[1][wave], sm[1], l, h);
     else bin_prim(gbins + (size_t)s * BIN_WORDS, make_mapping(segs[s]), l, h);
   }
   __syncthreads();
   for (int slot = 0; slot < 2; slot++) {
     if (slot == 1 && shi_id == slo_id) break;
     uint32_t* g = gbins + (size_t)(slot ? shi_id : slo_id) * BIN_WORDS;
-    const uint32_t* w = sb[slot];
     for (int i = tid; i < BIN_WORDS; i += TBS) {
-      if (i < 3 * NB) {
-        if (w[i]) atomicAdd(&g[i], w[i]);
-      } else if (i < 3 * NB + 9 * NB) {
-        if (w[(i - 3 * NB) / 3]) atomicMin(&g[i], w[i]);
-      } else {
-        if (w[(i - 3 * NB - 9 * NB) / 3]) atomicMax(&g[i], w[i]);
+      // the waves' words: counts add, ordered bounds take min / max
+      const int kind = i < 3 * NB ? 0 : (i < 3 * NB + 9 * NB ? 1 : 2);
+      const int cw = kind == 0 ? i : (kind == 1 ? (i - 3 * NB) / 3 : (i - 3 * NB - 9 * NB) / 3);
+      uint32_t v = sb[slot][0][i], c = sb[slot][0][cw];
+      for (int w = 1; w < NW; w++) {
+        const uint32_t x = sb[slot][w][i];
+        c += sb[slot][w][cw];
+        v = kind == 0 ? v + x : (kind == 1 ? (x < v ? x : v) : (x > v ? x : v));
+      }
+      if (kind == 0) {
+        if (v) atomicAdd(&g[i], v);
+      } else if (c) {
+        if (kind == 1) atomicMin(&g[i], v); else atomicMax(&g[i], v);
       }
     }
   }
 }
 
-__global__ __launch_bounds__(TBS) void k_big_split(const Seg* __restrict__ segs, const uint32_t* __restrict__ gbins,
-                                                   Split* __restrict__ splits, Lists nx, Tree t, LearnedCost lc) {
+template <bool LEARNED>
+__global__ __launch_bounds__(TBS) void k_big_split(const Seg* __restrict__ segs, const uint32_t* __restrict__ nseg,
+                                                   const uint32_t* __restrict__ gbins, Split* __restrict__ splits,
+                                                   Lists nx, Tree t, LearnedCost lc) {
   __shared__ ChooseLds L;
   const uint32_t i = blockIdx.x;
+  if (i >= *nseg) return;
   const Seg sg = segs[i];
   const uint32_t depth = sg.meta >> 1;
   const Mapping m = make_mapping(sg);
   const uint32_t* w = gbins + (size_t)i * BIN_WORDS;
-  Split sp = choose_split<TBS>(w, m, sg, lc, L, &t.cnt->error);
+  Split sp = choose_split<TBS, false, LEARNED>(w, m, sg, lc, L, &t.cnt->error, threadIdx.x);
   if (threadIdx.x != 0) return;
   // child centroid bounds: lo+hi of a primitive lies in [2 lo, 2 hi] of its
   // side's box, and in the parent's bounds
@@ -615,34 +718,42 @@ __global__ __launch_bounds__(TBS) void k_big_scatter(const uint32_t* __restrict_
 
 // ------------------------------------------------- mid and small segments --
 
-template <int BLOCK, int PER>
-__global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, const uint32_t* __restrict__ idx,
-                                               const float4* __restrict__ blo, const float4* __restrict__ bhi,
-                                               uint32_t* __restrict__ idx_o, float4* __restrict__ blo_o,
-                                               float4* __restrict__ bhi_o, Lists nx, Tree t, LearnedCost lc) {
+// LDS of one segment split by a group (a workgroup, or one wave).
+template <int BLOCK>
+struct SegLds {
+  uint32_t sb[BIN_WORDS];
+  ChooseLds L;
+  uint32_t wsum[BLOCK / 64];
+  uint32_t cb[12];  // ordered bounds of lo+hi: left lo3, left hi3, right lo3, right hi3
+};
+
+// Bins, splits and stably partitions segment sg with the BLOCK threads of a
+// group (tid 0..BLOCK-1, PER positions each): references read from
+// (idx, blo, bhi), written to (idx_o, blo_o, bhi_o) and their ids to
+// t.vals at the same positions; links the inner node (the split position)
+// into the tree. Every thread returns the split and, in lcl..rch, the
+// centroid bounds of the two children.
+template <int BLOCK, int PER, bool WAVE, bool LEARNED>
+__device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict__ idx, const float4* __restrict__ blo,
+                                      const float4* __restrict__ bhi, uint32_t* __restrict__ idx_o,
+                                      float4* __restrict__ blo_o, float4* __restrict__ bhi_o, const Tree& t,
+                                      const LearnedCost& lc, SegLds<BLOCK>& S, int tid, float lcl[3], float lch[3],
+                                      float rcl[3], float rch[3]) {
   static_assert(PER <= 32, "flags are kept in one 32-bit mask");
-  constexpr int NW = BLOCK / 64;
-  __shared__ uint32_t sb[BIN_WORDS];
-  __shared__ ChooseLds L;
-  __shared__ uint32_t s_wsum[NW];
-  __shared__ uint32_t s_cb[12];  // ordered bounds of lo+hi: left lo3, left hi3, right lo3, right hi3
-  const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
-  const Seg sg = segs[blockIdx.x];
-  const uint32_t depth = sg.meta >> 1;
   const Mapping m = make_mapping(sg);
-  init_bins(sb, tid, BLOCK);
-  if (tid < 12) s_cb[tid] = (tid % 6) < 3 ? NIL : 0u;
-  __syncthreads();
+  init_bins(S.sb, tid, BLOCK);
+  if (tid < 12) S.cb[tid] = (tid % 6) < 3 ? NIL : 0u;
+  group_sync<WAVE>();
   const uint32_t end = sg.first + sg.count;
   const uint32_t p0 = sg.first + (uint32_t)tid * PER;
 #pragma unroll 4
   for (int j = 0; j < PER; j++) {
     const uint32_t p = p0 + j;
-    if (p < end) bin_prim(sb, m, blo[p], bhi[p]);
+    if (p < end) bin_prim(S.sb, m, blo[p], bhi[p]);
   }
-  __syncthreads();
-  const Split sp = choose_split<BLOCK>(sb, m, sg, lc, L, &t.cnt->error);
+  group_sync<WAVE>();
+  const Split sp = choose_split<BLOCK, WAVE, LEARNED>(S.sb, m, sg, lc, S.L, &t.cnt->error, tid);
   // flags of this thread's positions, and the stable destinations
   uint32_t mask = 0u, nl = 0u;
 #pragma unroll 4
@@ -659,10 +770,10 @@ __global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, con
     const uint32_t y = __shfl_up(x, d, 64);
     if (lane >= d) x += y;
   }
-  if (lane == 63) s_wsum[wave] = x;
-  __syncthreads();
+  if (lane == 63) S.wsum[wave] = x;
+  group_sync<WAVE>();
   uint32_t lbase = x - nl;
-  for (int w = 0; w < wave; w++) lbase += s_wsum[w];
+  for (int w = 0; w < wave; w++) lbase += S.wsum[w];
   float bnd[12];
 #pragma unroll
   for (int k = 0; k < 12; k++) bnd[k] = (k % 6) < 3 ? INFINITY : -INFINITY;
@@ -702,153 +813,322 @@ __global__ __launch_bounds__(BLOCK) void k_seg(const Seg* __restrict__ segs, con
       v = mx ? fmaxf(v, y) : fminf(v, y);
     }
     if (lane == 0 && !__builtin_isinf(v)) {
-      if (mx) atomicMax(&s_cb[k], f2o(v)); else atomicMin(&s_cb[k], f2o(v));
+      if (mx) atomicMax(&S.cb[k], f2o(v)); else atomicMin(&S.cb[k], f2o(v));
     }
   }
-  __syncthreads();
-  if (tid == 0) {
-    const uint32_t mid = sg.first + sp.nleft - 1;
-    link_inner(t, sg, mid);
-    float lcl[3], lch[3], rcl[3], rch[3];
+  group_sync<WAVE>();
+  if (tid == 0) link_inner(t, sg, sg.first + sp.nleft - 1);
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      lcl[k] = o2f(s_cb[k]);
-      lch[k] = o2f(s_cb[3 + k]);
-      rcl[k] = o2f(s_cb[6 + k]);
-      rch[k] = o2f(s_cb[9 + k]);
-    }
+  for (int k = 0; k < 3; k++) {
+    lcl[k] = o2f(S.cb[k]);
+    lch[k] = o2f(S.cb[3 + k]);
+    rcl[k] = o2f(S.cb[6 + k]);
+    rch[k] = o2f(S.cb[9 + k]);
+  }
+  return sp;
+}
+
+// One workgroup per mid segment of a level (between SUB_MAX and MID_MAX
+// references).
+template <bool LEARNED>
+__global__ __launch_bounds__(256) void k_seg(const Seg* __restrict__ segs, const uint32_t* __restrict__ nseg,
+                                             const uint32_t* __restrict__ idx, const float4* __restrict__ blo,
+                                             const float4* __restrict__ bhi, uint32_t* __restrict__ idx_o,
+                                             float4* __restrict__ blo_o, float4* __restrict__ bhi_o, Lists nx,
+                                             Tree t, LearnedCost lc) {
+  __shared__ SegLds<256> S;
+  if (blockIdx.x >= *nseg) return;
+  const Seg sg = segs[blockIdx.x];
+  float lcl[3], lch[3], rcl[3], rch[3];
+  const Split sp = split_segment<256, 16, false, LEARNED>(sg, idx, blo, bhi, idx_o, blo_o, bhi_o, t, lc, S, threadIdx.x, lcl,
+                                                 lch, rcl, rch);
+  if (threadIdx.x == 0) {
+    const uint32_t mid = sg.first + sp.nleft - 1, depth = sg.meta >> 1;
     emit_child(t, nx, mid, 0, depth, sg.first, sp.nleft, lcl, lch);
     emit_child(t, nx, mid, 1, depth, sg.first + sp.nleft, sg.count - sp.nleft, rcl, rch);
   }
 }
 
-// ------------------------------------------------------------ tiny segments --
+// ---------------------------------------------------------------- subtrees --
 
-constexpr int TT = 32;  // threads per block of k_tiny, one segment each
+constexpr int SUB_TB = 256;
+constexpr int SUB_WAVES = SUB_TB / 64;
+constexpr int SMALL_STACK = 4;   // pending small segments of a wave: disjoint, > TINY_MAX each, <= SMALL_MAX in all
+static_assert(SUB_MAX <= SMALL_MAX, "a subtree segment is finished by one wave");
 
-// A thread's entries in LDS, entry j at j * TT (the rows of a block's
-// threads interleave, so that the lanes of a wave touch consecutive words
-// instead of one bank).
-struct TinyRows {
-  float4* lo;
-  float4* hi;
-  uint32_t* id;
-  __device__ inline float4& L(uint32_t j) const { return lo[j * TT]; }
-  __device__ inline float4& H(uint32_t j) const { return hi[j * TT]; }
-  __device__ inline uint32_t& I(uint32_t j) const { return id[j * TT]; }
+struct Bufs {
+  uint32_t* idx[2];
+  float4* blo[2];
+  float4* bhi[2];
 };
 
-// Stable insertion sort of entries [f, f+c) by centroid (lo + hi) on axis a.
-__device__ inline void tiny_sort(const TinyRows& e, uint32_t f, uint32_t c, int a) {
-  for (uint32_t j = f + 1; j < f + c; j++) {
-    const float4 l = e.L(j), h = e.H(j);
-    const uint32_t v = e.I(j);
-    const float key = comp(l, a) + comp(h, a);
-    uint32_t k = j;
-    while (k > f && comp(e.L(k - 1), a) + comp(e.H(k - 1), a) > key) {
-      e.L(k) = e.L(k - 1);
-      e.H(k) = e.H(k - 1);
-      e.I(k) = e.I(k - 1);
-      k--;
-    }
-    e.L(k) = l;
-    e.H(k) = h;
-    e.I(k) = v;
-  }
+struct TinyLds {
+  float k[3][TINY_MAX];
+  float4 lo[TINY_MAX], hi[TINY_MAX];
+  uint32_t id[TINY_MAX];
+};
+
+struct WaveLds {
+  SegLds<64> seg;
+  TinyLds tiny;
+  WSeg stack[SMALL_STACK];
+};
+
+
+__device__ inline bool tiny_better(float c, int a, uint32_t k, float bc, int ba, uint32_t bk) {
+  return c < bc || (c == bc && (a < ba || (a == ba && k < bk)));
 }
 
-// One thread finishes the whole subtree of a segment of 2..TINY_MAX
-// references: every split is the exact SAH optimum over the references
-// sorted by centroid on each axis (least A_L*n_L + A_R*n_R, first minimum
-// over axes then positions), inner node ids are split positions as for the
-// binned levels, and vals receives the final order. The references are
-// staged in this thread's rows of LDS.
-__global__ __launch_bounds__(TT) void k_tiny(const Seg* __restrict__ segs, uint32_t nseg,
-                                             const uint32_t* __restrict__ idx, const float4* __restrict__ blo,
-                                             const float4* __restrict__ bhi, Tree t) {
-  __shared__ float4 s_lo[TINY_MAX][TT], s_hi[TINY_MAX][TT];
-  __shared__ uint32_t s_id[TINY_MAX][TT];
-  const uint32_t i = blockIdx.x * TT + threadIdx.x;
-  if (i >= nseg) return;
-  const Seg sg = segs[i];
-  if (sg.count < 2 || sg.count > TINY_MAX) {
-    atomicOr(&t.cnt->error, 16u);
+// Whole subtree of a tiny segment sg (2..TINY_MAX references in buffer
+// parity p) with exact SAH splits, by the lanes 0..count-1 of one wave (lane
+// j holds the reference at position j of the segment). The decisions are
+// those of a sequential finisher: at each node the references are stably
+// sorted by centroid (lo + hi) on x, then on y, then on z, each sort
+// starting from the order the previous one left; every split position of
+// each of the three orders is priced A_L*n_L + A_R*n_R; the first minimum
+// over (axis, position) wins; and the node's references take the order of a
+// stable sort on the winning axis applied to the z order (for z, the z order
+// itself). A stable sort here is a rank: the number of the node's
+// references with a smaller key tuple, (k_x, position) for the x order,
+// (k_y, k_x, position) for y, (k_z, k_y, k_x, position) for z. All nodes of
+// one depth are split together. vals receives the final order.
+__device__ inline void tiny_wave(const Seg& sg, uint32_t p, const Bufs& B, const Tree& t, TinyLds& T, int lane) {
+  const uint32_t c = sg.count;
+  if (c < 2 || c > TINY_MAX) {
+    if (lane == 0) atomicOr(&t.cnt->error, 16u);
     return;
   }
-  const TinyRows e{&s_lo[0][threadIdx.x], &s_hi[0][threadIdx.x], &s_id[0][threadIdx.x]};
-  const uint32_t m = sg.count;
-  for (uint32_t j = 0; j < m; j++) {
-    e.L(j) = blo[sg.first + j];
-    e.H(j) = bhi[sg.first + j];
-    e.I(j) = idx[sg.first + j];
+  const uint32_t pos = (uint32_t)lane;
+  const bool act = pos < c;
+  float4 l = make_float4(0.0f, 0.0f, 0.0f, 0.0f), h = l;
+  uint32_t id = 0u;
+  if (act) {
+    l = B.blo[p][sg.first + pos];
+    h = B.bhi[p][sg.first + pos];
+    id = B.idx[p][sg.first + pos];
   }
-  uint32_t sf[TINY_MAX], sc[TINY_MAX], spar[TINY_MAX], smeta[TINY_MAX];
-  int sp = 0;
-  sf[0] = 0; sc[0] = m; spar[0] = sg.parent; smeta[0] = sg.meta;
-  sp = 1;
-  while (sp > 0) {
-    sp--;
-    const uint32_t f = sf[sp], c = sc[sp], p = spar[sp], meta = smeta[sp];
-    const uint32_t side = meta & 1u, depth = meta >> 1;
-    if (c == 1) {
-      t.child[2 * p + side] = (sg.first + f) | REF_LEAF;
-      t.parent_leaf[sg.first + f] = p;
-      continue;
+  // the node holding this lane's position: [nf, nf + nc) of the segment
+  uint32_t nf = 0u, nc = act ? c : 0u, np = sg.parent, nm = sg.meta;
+  for (uint32_t it = 0;; it++) {
+    if (nc == 1u) {
+      t.child[2 * np + (nm & 1u)] = (sg.first + nf) | REF_LEAF;
+      t.parent_leaf[sg.first + nf] = np;
+      nc = 0u;
     }
-    float best = INFINITY;
-    int ba = 0;
-    uint32_t bk = 1;
-    for (int a = 0; a < 3; a++) {
-      tiny_sort(e, f, c, a);
-      float ra[TINY_MAX];
-      float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
-#pragma unroll
-      for (int k = TINY_MAX - 1; k >= 1; k--) {
-        if ((uint32_t)k < c) {
-          const float4 l = e.L(f + k), h = e.H(f + k);
-          bl[0] = fminf(bl[0], l.x); bl[1] = fminf(bl[1], l.y); bl[2] = fminf(bl[2], l.z);
-          bh[0] = fmaxf(bh[0], h.x); bh[1] = fmaxf(bh[1], h.y); bh[2] = fmaxf(bh[2], h.z);
-          ra[k] = half_area(bh[0] - bl[0], bh[1] - bl[1], bh[2] - bl[2]);
-        }
+    const bool busy = nc >= 2u;
+    if (__ballot(busy) == 0) break;
+    if (it >= TINY_MAX) {
+      if (lane == 0) atomicOr(&t.cnt->error, 16u);
+      break;
+    }
+    const float k0 = l.x + h.x, k1 = l.y + h.y, k2 = l.z + h.z;
+    if (pos < TINY_MAX) {
+      T.k[0][pos] = k0;
+      T.k[1][pos] = k1;
+      T.k[2][pos] = k2;
+    }
+    group_sync<true>();
+    uint32_t r0 = 0u, r1 = 0u, r2 = 0u, f0 = 0u, f1 = 0u;
+    if (busy) {
+      for (uint32_t i = nf; i < nf + nc; i++) {
+        const float a0 = T.k[0][i], a1 = T.k[1][i], a2 = T.k[2][i];
+        const bool lt0 = a0 < k0 || (a0 == k0 && i < pos);
+        const bool lt1 = a1 < k1 || (a1 == k1 && lt0);
+        const bool lt2 = a2 < k2 || (a2 == k2 && lt1);
+        r0 += lt0 ? 1u : 0u;
+        r1 += lt1 ? 1u : 0u;
+        r2 += lt2 ? 1u : 0u;
+        f0 += (a0 < k0 || (a0 == k0 && lt2)) ? 1u : 0u;
+        f1 += (a1 < k1 || (a1 == k1 && lt2)) ? 1u : 0u;
       }
-      bl[0] = bl[1] = bl[2] = INFINITY;
-      bh[0] = bh[1] = bh[2] = -INFINITY;
+    }
+    float bc = INFINITY;
+    int ba = 0;
+    uint32_t bk = 1u;
+    for (int a = 0; a < 3; a++) {
+      const uint32_t r = a == 0 ? r0 : (a == 1 ? r1 : r2);
+      group_sync<true>();
+      if (busy) {
+        T.lo[nf + r] = l;
+        T.hi[nf + r] = h;
+      }
+      group_sync<true>();
+      float4 sl = l, sh = h;
+      if (busy) {
+        sl = T.lo[pos];
+        sh = T.hi[pos];
+      }
+      // prefix box over the node's sorted positions [nf, pos], suffix over [pos, nf + nc)
+      float pl[3] = {sl.x, sl.y, sl.z}, ph[3] = {sh.x, sh.y, sh.z};
+      float ql[3] = {sl.x, sl.y, sl.z}, qh[3] = {sh.x, sh.y, sh.z};
 #pragma unroll
-      for (int k = 1; k < (int)TINY_MAX; k++) {
-        if ((uint32_t)k < c) {
-          const float4 l = e.L(f + k - 1), h = e.H(f + k - 1);
-          bl[0] = fminf(bl[0], l.x); bl[1] = fminf(bl[1], l.y); bl[2] = fminf(bl[2], l.z);
-          bh[0] = fmaxf(bh[0], h.x); bh[1] = fmaxf(bh[1], h.y); bh[2] = fmaxf(bh[2], h.z);
-          const float cost = half_area(bh[0] - bl[0], bh[1] - bl[1], bh[2] - bl[2]) * (float)k +
-                             ra[k] * (float)(c - (uint32_t)k);
-          if (cost < best) {
-            best = cost;
-            ba = a;
-            bk = (uint32_t)k;
+      for (int d = 1; d < (int)TINY_MAX; d <<= 1) {
+        float ul[3], uh[3], dl[3], dh[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          ul[k] = __shfl_up(pl[k], d, TINY_MAX);
+          uh[k] = __shfl_up(ph[k], d, TINY_MAX);
+          dl[k] = __shfl_down(ql[k], d, TINY_MAX);
+          dh[k] = __shfl_down(qh[k], d, TINY_MAX);
+        }
+        if (busy && pos >= nf + (uint32_t)d) {
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            pl[k] = fminf(pl[k], ul[k]);
+            ph[k] = fmaxf(ph[k], uh[k]);
+          }
+        }
+        if (busy && pos + (uint32_t)d < nf + nc) {
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            ql[k] = fminf(ql[k], dl[k]);
+            qh[k] = fmaxf(qh[k], dh[k]);
           }
         }
       }
+      float rl[3], rh[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        rl[k] = __shfl_down(ql[k], 1, TINY_MAX);
+        rh[k] = __shfl_down(qh[k], 1, TINY_MAX);
+      }
+      // candidate: the first pos - nf + 1 sorted references left
+      if (busy && pos + 1u < nf + nc) {
+        const uint32_t k = pos - nf + 1u;
+        const float cost = half_area(ph[0] - pl[0], ph[1] - pl[1], ph[2] - pl[2]) * (float)k +
+                           half_area(rh[0] - rl[0], rh[1] - rl[1], rh[2] - rl[2]) * (float)(nc - k);
+        if (cost < bc) {
+          bc = cost;
+          ba = a;
+          bk = k;
+        }
+      }
     }
-    if (ba != 2) tiny_sort(e, f, c, ba);  // the last sort left axis 2's order
-    const uint32_t mid = sg.first + f + bk - 1;
-    t.child[2 * p + side] = mid;
-    t.parent_int[mid] = p;
-    t.range[mid] = make_uint2(sg.first + f, sg.first + f + c - 1);
-    set_depth(t, mid, depth);
-    if (sp + 2 > (int)TINY_MAX) {
-      atomicOr(&t.cnt->error, 16u);
-      return;
+    // least (cost, axis, position) of each node reaches the node's first lane
+#pragma unroll
+    for (int d = 1; d < (int)TINY_MAX; d <<= 1) {
+      const float oc = __shfl_down(bc, d, TINY_MAX);
+      const int oa = __shfl_down(ba, d, TINY_MAX);
+      const uint32_t ok = __shfl_down(bk, d, TINY_MAX);
+      if (busy && pos + (uint32_t)d < nf + nc && tiny_better(oc, oa, ok, bc, ba, bk)) {
+        bc = oc;
+        ba = oa;
+        bk = ok;
+      }
     }
-    sf[sp] = f + bk; sc[sp] = c - bk; spar[sp] = mid; smeta[sp] = ((depth + 1) << 1) | 1u; sp++;
-    sf[sp] = f;      sc[sp] = bk;     spar[sp] = mid; smeta[sp] = ((depth + 1) << 1);      sp++;
+    ba = __shfl(ba, (int)(nf & (TINY_MAX - 1u)), TINY_MAX);
+    bk = __shfl(bk, (int)(nf & (TINY_MAX - 1u)), TINY_MAX);
+    const uint32_t rf = ba == 0 ? f0 : (ba == 1 ? f1 : r2);
+    const uint32_t mid = sg.first + nf + bk - 1u;
+    if (busy && pos == nf) {
+      t.child[2 * np + (nm & 1u)] = mid;
+      t.parent_int[mid] = np;
+      t.range[mid] = make_uint2(sg.first + nf, sg.first + nf + nc - 1u);
+      set_depth(t, mid, nm >> 1);
+    }
+    group_sync<true>();
+    if (busy) {
+      T.lo[nf + rf] = l;
+      T.hi[nf + rf] = h;
+      T.id[nf + rf] = id;
+    }
+    group_sync<true>();
+    if (busy) {
+      l = T.lo[pos];
+      h = T.hi[pos];
+      id = T.id[pos];
+      const uint32_t dm = ((nm >> 1) + 1u) << 1;
+      if (pos - nf < bk) {
+        nc = bk;
+        nm = dm;
+      } else {
+        nf += bk;
+        nc -= bk;
+        nm = dm | 1u;
+      }
+      np = mid;
+    }
   }
-  for (uint32_t j = 0; j < m; j++) t.vals[sg.first + j] = e.I(j);
+  if (act) t.vals[sg.first + pos] = id;
+  group_sync<true>();
 }
 
-// Writes the big-segment id of every position of the initial big segments.
+// Whole subtree of a segment of at most SMALL_MAX references by one wave:
+// binned splits (as one 64-lane workgroup per segment would make them) down
+// to tiny segments, which tiny_wave finishes.
+template <bool LEARNED>
+__device__ inline void small_tree(const WSeg& item, const Bufs& B, const Tree& t, const LearnedCost& lc,
+                                  uint32_t fin_max, WaveLds& W, int lane) {
+  if (item.s.count <= TINY_MAX && !(item.flags & WSEG_BINNED)) {
+    tiny_wave(item.s, item.par, B, t, W.tiny, lane);
+    return;
+  }
+  int sp = 0;
+  WSeg cur = item;
+  for (;;) {
+    const uint32_t p = cur.par, q = p ^ 1u;
+    float bl[2][3], bh[2][3];
+    const Split spl = split_segment<64, 1, true, LEARNED>(cur.s, B.idx[p], B.blo[p], B.bhi[p], B.idx[q], B.blo[q], B.bhi[q], t,
+                                                 lc, W.seg, lane, bl[0], bh[0], bl[1], bh[1]);
+    wave_sync_global();
+    const uint32_t mid = cur.s.first + spl.nleft - 1u, depth = cur.s.meta >> 1;
+    for (int side = 0; side < 2; side++) {
+      const uint32_t first = side ? cur.s.first + spl.nleft : cur.s.first;
+      const uint32_t count = side ? cur.s.count - spl.nleft : spl.nleft;
+      if (count == 1u) {
+        if (lane == 0) {
+          t.child[2 * mid + side] = first | REF_LEAF;
+          t.parent_leaf[first] = mid;
+        }
+        continue;
+      }
+      if (count <= fin_max) {
+        if (lane == 0) finish_median(t, mid, (uint32_t)side, depth, first, count);
+        continue;
+      }
+      Seg c;
+      c.first = first;
+      c.count = count;
+      c.parent = mid;
+      c.meta = ((depth + 1u) << 1) | (uint32_t)side;
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        c.clo[k] = bl[side][k];
+        c.chi[k] = bh[side][k];
+      }
+      if (count <= TINY_MAX) {
+        tiny_wave(c, q, B, t, W.tiny, lane);
+      } else if (sp < SMALL_STACK) {
+        if (lane == 0) W.stack[sp] = WSeg{c, q, 0u};
+        sp++;
+      } else if (lane == 0) {
+        atomicOr(&t.cnt->error, 1u);
+      }
+    }
+    group_sync<true>();
+    if (sp == 0) break;
+    cur = W.stack[--sp];
+  }
+}
+
+// One wave per subtree segment (at most SUB_MAX references), finished by
+// small_tree; four segments per workgroup.
+template <bool LEARNED>
+__global__ __launch_bounds__(SUB_TB) void k_subtree(const WSeg* __restrict__ roots, uint32_t nroots, Bufs B, Tree t,
+                                                    LearnedCost lc, uint32_t fin_max) {
+  __shared__ WaveLds W[SUB_WAVES];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t r = blockIdx.x * SUB_WAVES + (uint32_t)wave;
+  if (r >= nroots) return;
+  small_tree<LEARNED>(roots[r], B, t, lc, fin_max, W[wave], lane);
+}
+
+// Writes the big-segment id of every position of the initial big segments
+// (segment blockIdx.y, positions in blocks of TBS along x).
 __global__ void k_fill_seg_ids(uint32_t* sid, const Seg* segs) {
-  const Seg sg = segs[blockIdx.x];
-  for (uint32_t i = threadIdx.x; i < sg.count; i += TBS) sid[sg.first + i] = blockIdx.x;
+  const Seg sg = segs[blockIdx.y];
+  const uint32_t i = blockIdx.x * TBS + threadIdx.x;
+  if (i < sg.count) sid[sg.first + i] = blockIdx.y;
 }
 
 }  // namespace
@@ -864,14 +1144,16 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
   if (trees.size() >= ROOT_TAG) throw hip_error(hipErrorInvalidValue, "too many trees");
   for (const SahRoot& r : trees)
     if (r.count < 2 || (uint64_t)r.first + r.count > n) throw hip_error(hipErrorInvalidValue, "bad SAH tree range");
-  const uint32_t cap[4] = {n / MID_MAX + 2, n / (SMALL_MAX + 1) + 2, n / (TINY_MAX + 1) + 2, n / 2 + 2};
+  const uint32_t cap[2] = {n / MID_MAX + 2, n / (SUB_MAX + 1) + 2};
+  const uint32_t root_cap = n / 2 + 2;  // subtree segments are disjoint and hold two or more references
   for (int p = 0; p < 2; p++) {
     S.idx[p].reserve((size_t)n * 4);
     S.blo[p].reserve((size_t)n * 16);
     S.bhi[p].reserve((size_t)n * 16);
     S.sid[p].reserve((size_t)n * 4);
-    for (int c = 0; c < 4; c++) S.segs[c][p].reserve((size_t)cap[c] * sizeof(Seg));
+    for (int c = 0; c < 2; c++) S.segs[c][p].reserve((size_t)cap[c] * sizeof(Seg));
   }
+  S.subroots.reserve((size_t)root_cap * sizeof(WSeg));
   S.bins.reserve((size_t)cap[0] * BIN_WORDS * 4);
   S.splits.reserve((size_t)cap[0] * sizeof(Split));
   S.flags.reserve((size_t)n * 4);
@@ -889,7 +1171,11 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
                      trees.size(), NIL);
   RTG_HIP(hipMemsetAsync(depth, 0xff, (size_t)(n - 1), s));
 
-  std::vector<Seg> init[3];
+  // trees above SUB_MAX references start as big or mid segments of the
+  // level loop, the others as subtree segments (split by bins even when
+  // tiny, as a tree's root always is)
+  std::vector<Seg> init[2];
+  std::vector<WSeg> init_sub;
   for (size_t i = 0; i < trees.size(); i++) {
     const SahRoot& r = trees[i];
     Seg sg;
@@ -901,58 +1187,66 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
       sg.clo[k] = r.clo[k];
       sg.chi[k] = r.chi[k];
     }
-    init[r.count > MID_MAX ? 0 : (r.count > SMALL_MAX ? 1 : 2)].push_back(sg);
+    if (r.count > SUB_MAX) init[r.count > MID_MAX ? 0 : 1].push_back(sg);
+    else init_sub.push_back(WSeg{sg, 0u, WSEG_BINNED});
   }
-  uint32_t cur[3];
-  for (int c = 0; c < 3; c++) {
+  uint32_t cur[2];
+  for (int c = 0; c < 2; c++) {
     cur[c] = (uint32_t)init[c].size();
     if (cur[c])
       RTG_HIP(hipMemcpyAsync(S.segs[c][0].ptr, init[c].data(), init[c].size() * sizeof(Seg), hipMemcpyHostToDevice, s));
   }
-  if (cur[0]) {
+  if (!init_sub.empty())
+    RTG_HIP(hipMemcpyAsync(S.subroots.ptr, init_sub.data(), init_sub.size() * sizeof(WSeg), hipMemcpyHostToDevice, s));
+  if (cur[0]) {  // the big-segment kernels read every position's big-segment id
     hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[0].as<uint32_t>(), (size_t)n, NIL);
-    hipLaunchKernelGGL(k_fill_seg_ids, dim3(cur[0]), dim3(TBS), 0, s, S.sid[0].as<uint32_t>(), S.segs[0][0].as<Seg>());
+    uint32_t longest = 0;
+    for (const Seg& sg : init[0]) longest = std::max(longest, sg.count);
+    hipLaunchKernelGGL(k_fill_seg_ids, dim3(div_up(longest, TBS), cur[0]), dim3(TBS), 0, s, S.sid[0].as<uint32_t>(),
+                       S.segs[0][0].as<Seg>());
   }
   Counters hc{};
+  hc.nseg[0][0] = cur[0];
+  hc.nseg[0][1] = cur[1];
+  hc.nroot = (uint32_t)init_sub.size();
   hc.error = 0;
   RTG_HIP(hipMemcpyAsync(cnt, &hc, sizeof(hc), hipMemcpyHostToDevice, s));
   RTG_HIP(hipStreamSynchronize(s));  // the host staging above must outlive its copies
 
   Tree t{child, parent_int, parent_leaf, range, vals, cnt, S.roots.as<uint32_t>(), depth};
-  S.side.ensure();
-  bool tiny_pending[2] = {false, false};
-  int par = 0;
-  uint32_t levels = 0;
-  // a failed build leaves no finisher running on the scratch it reuses
-  auto fail = [&](const std::string& msg) {
-    (void)hipStreamSynchronize(S.side.s);
-    throw hip_error(hipErrorUnknown, msg);
-  };
-  while (cur[0] + cur[1] + cur[2] > 0) {
-    if (++levels > 4096) fail("SAH build did not terminate");
-    const int in = par, o = par ^ 1;
-    // this level rewrites the segment lists of parity o, which the finisher
-    // of two levels back may still be reading
-    if (tiny_pending[o]) {
-      RTG_HIP(hipStreamWaitEvent(s, S.side.done[o], 0));
-      tiny_pending[o] = false;
-    }
+  // Level l reads the segment lists of parity l & 1 and fills those of the
+  // other parity; its kernels take their segment counts from the device, so
+  // the host queues level l + 1 before it knows whether level l left any
+  // work, and reads level l's counters (through a pinned slot) meanwhile.
+  // big-segment kernels run until a level is known to have left no big
+  // segment (children are smaller than their parents, so none comes back)
+  bool big_live = cur[0] > 0;
+  auto enqueue_level = [&](int in) {
+    const int o = in ^ 1;
     Lists nx;
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < 2; c++) {
       nx.seg[c] = S.segs[c][o].as<Seg>();
       nx.cap[c] = cap[c];
     }
+    nx.nseg = cnt->nseg[o];
+    nx.roots = S.subroots.as<WSeg>();
+    nx.root_cap = root_cap;
+    nx.par = (uint32_t)o;
     nx.fin_max = fin_max;
-    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, cnt);
-    if (cur[0]) {
-      const Seg* bs = S.segs[0][in].as<Seg>();
-      hipLaunchKernelGGL(k_init_gbins, dim3(div_up((size_t)cur[0] * BIN_WORDS, TBS)), dim3(TBS), 0, s,
-                         S.bins.as<uint32_t>(), cur[0]);
+    const Seg* bs = S.segs[0][in].as<Seg>();
+    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, cnt->nseg[o]);
+    if (big_live) {
+      hipLaunchKernelGGL(k_init_gbins, dim3(div_up((size_t)cap[0] * BIN_WORDS, TBS)), dim3(TBS), 0, s,
+                         S.bins.as<uint32_t>(), &cnt->nseg[in][0]);
       hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[o].as<uint32_t>(), (size_t)n, NIL);
       hipLaunchKernelGGL(k_big_bins, dim3(div_up(n, CHUNK)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
                          S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs, S.bins.as<uint32_t>());
-      hipLaunchKernelGGL(k_big_split, dim3(cur[0]), dim3(TBS), 0, s, bs, S.bins.as<uint32_t>(), S.splits.as<Split>(),
-                         nx, t, lcost);
+      if (lcost.P)
+        hipLaunchKernelGGL(k_big_split<true>, dim3(cap[0]), dim3(TBS), 0, s, bs, &cnt->nseg[in][0],
+                           S.bins.as<uint32_t>(), S.splits.as<Split>(), nx, t, lcost);
+      else
+        hipLaunchKernelGGL(k_big_split<false>, dim3(cap[0]), dim3(TBS), 0, s, bs, &cnt->nseg[in][0],
+                           S.bins.as<uint32_t>(), S.splits.as<Split>(), nx, t, lcost);
       hipLaunchKernelGGL(k_big_flags, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
                          S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs, S.splits.as<Split>(),
                          S.flags.as<uint32_t>());
@@ -962,34 +1256,51 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
                          S.splits.as<Split>(), S.flags.as<uint32_t>(), S.scan.as<uint32_t>(), S.sid[o].as<uint32_t>(),
                          S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), vals, cnt);
     }
-    if (cur[1])
-      hipLaunchKernelGGL((k_seg<256, 16>), dim3(cur[1]), dim3(256), 0, s, S.segs[1][in].as<Seg>(),
-                         S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
-                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t, lcost);
-    if (cur[2])
-      hipLaunchKernelGGL((k_seg<64, 1>), dim3(cur[2]), dim3(64), 0, s, S.segs[2][in].as<Seg>(),
-                         S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
+    if (SUB_MAX < MID_MAX)
+      hipLaunchKernelGGL(lcost.P ? k_seg<true> : k_seg<false>, dim3(cap[1]), dim3(256), 0, s, S.segs[1][in].as<Seg>(),
+                         &cnt->nseg[in][1], S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
                          S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t, lcost);
     RTG_HIP(hipGetLastError());
-    RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
-    RTG_HIP(hipStreamSynchronize(s));
-    if (hc.error) fail("SAH build failed: error bits " + std::to_string(hc.error));
-    for (int c = 0; c < 3; c++) cur[c] = hc.nseg[c];
-    // this level's tiny children, finished in place on the side stream while
-    // the next levels run (their positions are outside every segment the
-    // next levels touch; this level's work is complete after the
-    // synchronisation above)
-    if (hc.nseg[3]) {
-      hipLaunchKernelGGL(k_tiny, dim3(div_up(hc.nseg[3], TT)), dim3(TT), 0, S.side.s, S.segs[3][o].as<Seg>(),
-                         hc.nseg[3], S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), t);
-      RTG_HIP(hipGetLastError());
-      RTG_HIP(hipEventRecord(S.side.done[o], S.side.s));
-      tiny_pending[o] = true;
+  };
+  S.rb.ensure(sizeof(Counters));
+  auto readback = [&](uint32_t level) {
+    RTG_HIP(hipMemcpyAsync(S.rb.slot((int)level), cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
+    RTG_HIP(hipEventRecord(S.rb.ev[level % LevelReadback::K], s));
+  };
+  uint32_t levels = 0;
+  if (cur[0] + cur[1] > 0) {
+    enqueue_level(0);
+    readback(0);
+    for (levels = 1;; levels++) {
+      if (levels > 4096) {
+        (void)hipStreamSynchronize(s);
+        throw hip_error(hipErrorUnknown, "SAH build did not terminate");
+      }
+      enqueue_level((int)(levels & 1u));  // speculative: without input it does nothing
+      readback(levels);
+      RTG_HIP(hipEventSynchronize(S.rb.ev[(levels - 1) % LevelReadback::K]));
+      memcpy(&hc, S.rb.slot((int)(levels - 1)), sizeof(hc));
+      if (hc.error) {
+        (void)hipStreamSynchronize(s);
+        throw hip_error(hipErrorUnknown, "SAH build failed: error bits " + std::to_string(hc.error));
+      }
+      const uint32_t* next = hc.nseg[levels & 1u];  // input of the level just queued
+      if (next[0] + next[1] == 0) break;
+      if (next[0] == 0) big_live = false;
     }
-    par = o;
   }
-  for (int p = 0; p < 2; p++)
-    if (tiny_pending[p]) RTG_HIP(hipStreamWaitEvent(s, S.side.done[p], 0));
+  // every segment of at most SUB_MAX references, finished by one workgroup each
+  if (hc.nroot) {
+    Bufs B;
+    for (int p = 0; p < 2; p++) {
+      B.idx[p] = S.idx[p].as<uint32_t>();
+      B.blo[p] = S.blo[p].as<float4>();
+      B.bhi[p] = S.bhi[p].as<float4>();
+    }
+    hipLaunchKernelGGL(lcost.P ? k_subtree<true> : k_subtree<false>, dim3(div_up(hc.nroot, SUB_WAVES)), dim3(SUB_TB),
+                       0, s, S.subroots.as<WSeg>(), hc.nroot, B, t, lcost, fin_max);
+    RTG_HIP(hipGetLastError());
+  }
   out.roots.assign(trees.size(), NIL);
   RTG_HIP(hipMemcpyAsync(out.roots.data(), S.roots.ptr, trees.size() * 4, hipMemcpyDeviceToHost, s));
   RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));

@@ -70,4 +70,26 @@ struct DevBuf {
   DevBuf& operator=(const DevBuf&) = delete;
 };
 
+// Grow-only pinned host buffer (the source or target of an asynchronous
+// copy that must not wait for the host).
+struct PinnedBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  void reserve(size_t n) {
+    if (n <= bytes) return;
+    if (ptr) (void)hipHostFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    RTG_HIP(hipHostMalloc(&ptr, n, hipHostMallocDefault));
+    bytes = n;
+  }
+  template <class T> T* as() const { return (T*)ptr; }
+  ~PinnedBuf() {
+    if (ptr) (void)hipHostFree(ptr);
+  }
+  PinnedBuf() = default;
+  PinnedBuf(const PinnedBuf&) = delete;
+  PinnedBuf& operator=(const PinnedBuf&) = delete;
+};
+
 }  // namespace rtg

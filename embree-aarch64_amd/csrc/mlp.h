@@ -1,20 +1,24 @@
-// Learned split cost: a small MLP that predicts, from a box and the number
-// of primitives under it, log2 of the ratio between the probability that a
-// ray of the target distribution visits a node with that box and the box's
-// half area relative to the scene's (what uniformly distributed rays would
-// give). The binned-SAH builder uses halfArea * 2^prediction in place of the
-// half area in its split cost.
+// Learned split cost: a small MLP that corrects the SAH's estimate of how
+// often a child box is entered when its parent box is. For a candidate child
+// C of a segment S -- the per-bin statistics a binned split has for each
+// candidate: C's box and primitive count from its bins, S's box and count --
+// it predicts y = log2(P(C | S)) - log2(A_C / A_S), where P(C | S) is the
+// fraction of the rays of the target distribution that enter S and then C,
+// and A is the half area (the SAH assumes y = 0). The binned-SAH builder
+// prices the split A_L 2^y_L n_L + A_R 2^y_R n_R in place of
+// A_L n_L + A_R n_R.
 //
-// Network: 8 inputs -> 32 ReLU -> 32 ReLU -> 1, FP16 weights and
+// Network: 12 inputs -> 32 ReLU -> 32 ReLU -> 1, FP16 weights and
 // activations with FP32 accumulation (v_dot2_f32_f16), FP32 biases. Training
 // keeps FP32 master weights and runs the forward pass in the same FP16 form
 // as inference.
 #pragma once
+#include "bvh_types.h"
 #include "gpu_util.h"
 
 namespace rtg {
 
-constexpr int MLP_IN = 8;
+constexpr int MLP_IN = 12;
 constexpr int MLP_H = 32;
 // FP32 parameter vector: W1[H][IN] | b1[H] | W2[H][H] | b2[H] | w3[H] | b3
 constexpr int MLP_OFF_W1 = 0;
@@ -52,16 +56,32 @@ struct LearnedCost {
   float sah_slack;     // candidates whose plain SAH cost exceeds (1 + slack) x the best are not considered
 };
 
-// Features of a box holding `count` primitives.
-__host__ __device__ inline void box_features(const float lo[3], const float hi[3], uint32_t count, const SceneNorm& sn,
-                                             float f[MLP_IN]) {
+constexpr float MLP_AREA_FLOOR = 9.3132257e-10f;  // 2^-30
+constexpr float MLP_Y_MIN = -3.0f, MLP_Y_MAX = 3.0f;  // bounds of a prediction in use
+
+// Features of child box C (clo..chi, cn primitives) of parent box S
+// (plo..phi, pn primitives):
+//   f0     log2(A_C / A_S) / 16             the SAH's estimate of P(C | S)
+//   f1     n_C / n_S
+//   f2-f4  (centre_C - centre_S) / extent_S  per axis (0 on a flat axis)
+//   f5-f7  extent_C / extent_S              per axis (1 on a flat axis)
+//   f8-f10 (centre_S - scene lo) / scene extent
+//   f11    log2(A_S / A_scene) / 16
+__host__ __device__ inline void split_features(const float clo[3], const float chi[3], uint32_t cn, const float plo[3],
+                                               const float phi[3], uint32_t pn, const SceneNorm& sn,
+                                               float f[MLP_IN]) {
+  const float ac = half_area(chi[0] - clo[0], chi[1] - clo[1], chi[2] - clo[2]);
+  const float ap = half_area(phi[0] - plo[0], phi[1] - plo[1], phi[2] - plo[2]);
+  f[0] = ap > 0.0f ? log2f(ac / ap + MLP_AREA_FLOOR) * 0.0625f : 0.0f;
+  f[1] = pn > 0 ? (float)cn / (float)pn : 1.0f;
   for (int k = 0; k < 3; k++) {
-    f[k] = ((lo[k] + hi[k]) * 0.5f - sn.lo[k]) * sn.inv_ext[k];
-    f[3 + k] = (hi[k] - lo[k]) * sn.inv_ext[k];
+    const float ep = phi[k] - plo[k];
+    const float inv = ep > 0.0f ? 1.0f / ep : 0.0f;
+    f[2 + k] = ((chi[k] + clo[k]) - (phi[k] + plo[k])) * 0.5f * inv;
+    f[5 + k] = ep > 0.0f ? (chi[k] - clo[k]) * inv : 1.0f;
+    f[8 + k] = ((phi[k] + plo[k]) * 0.5f - sn.lo[k]) * sn.inv_ext[k];
   }
-  const float a = half_area(hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]) * sn.inv_area;
-  f[6] = log2f(a + 9.3132257e-10f) * 0.0625f;  // 2^-30 floor
-  f[7] = log2f((float)(count > 0 ? count : 1)) * 0.0625f;
+  f[11] = log2f(ap * sn.inv_area + MLP_AREA_FLOOR) * 0.0625f;
 }
 
 // Forward pass: z1 = W1 x + b1, h1 = half(relu(z1)), z2 = W2 h1 + b2,
@@ -102,7 +122,7 @@ __device__ inline float mlp_eval(const MlpPacked* __restrict__ P, const float x[
 struct SplitModel {
   DevBuf params, m, v, grad, packed, loss;  // FP32 master, Adam moments, gradient, inference form
   DevBuf X, T, W;                           // training set: features, targets, sample weights
-  DevBuf idx, visits, counts;               // minibatch indices, visit logging scratch
+  DevBuf idx, visits, counts, scount;       // minibatch indices, visit logging scratch, appended-sample count
   uint32_t nsamples = 0;
   uint32_t adam_t = 0;
   SceneNorm norm{};                         // of the scene the data came from
@@ -117,11 +137,15 @@ void mlp_predict(SplitModel& m, const float* d_x, uint32_t n, float* d_y, hipStr
 // Loss sum_i w_i (y_i - t_i)^2 / n and its gradient (into m.grad, FP32).
 float mlp_gradient(SplitModel& m, const float* d_x, const float* d_t, const float* d_w, const uint32_t* d_idx,
                    uint32_t n, hipStream_t s);
-// Appends one training sample per node: features f of info[i] and target
-// log2((visits[i] + 0.5) / rays) - 16 f[6] (f[6] = log2 of the relative half
-// area / 16).
-void mlp_append_samples(SplitModel& m, const float* d_info, const uint32_t* d_visits, uint32_t nnodes, uint32_t rays,
-                        const SceneNorm& sn, hipStream_t s);
+// Appends one training sample per (parent, internal child) pair of the
+// nodes (a binary tree: width 2) whose parent was entered by at least
+// MLP_MIN_VISITS rays: split_features of the child's box and count against
+// the parent's (info: NodeInfo per node) and the target
+// log2((v_C + 0.5) / (v_S + 0.5)) - log2(A_C / A_S + 2^-30), where v counts
+// the rays that entered a node.
+constexpr uint32_t MLP_MIN_VISITS = 8;
+void mlp_append_samples(SplitModel& m, const QNode8* d_nodes, const NodeInfo* d_info, const uint32_t* d_visits,
+                        uint32_t nnodes, const SceneNorm& sn, hipStream_t s);
 // Adam on random minibatches of the training set; returns the last loss.
 float mlp_train(SplitModel& m, uint32_t steps, uint32_t batch, float lr, uint32_t seed, hipStream_t s);
 

@@ -178,21 +178,36 @@ __global__ void k_mlp_adam(float* p, float* m, float* v, float* g, float lr, flo
   pack_one(P, i, pi);
 }
 
-__global__ void k_mlp_samples(const float* __restrict__ info, const uint32_t* __restrict__ visits, uint32_t nnodes,
-                              float inv_rays, SceneNorm sn, float* __restrict__ X, float* __restrict__ T,
-                              float* __restrict__ W) {
+// One thread per parent node: a sample per internal child, appended at a
+// slot reserved with one atomic per node.
+__global__ void k_mlp_samples(const QNode8* __restrict__ nodes, const NodeInfo* __restrict__ info,
+                              const uint32_t* __restrict__ visits, uint32_t nnodes, SceneNorm sn,
+                              float* __restrict__ X, float* __restrict__ T, float* __restrict__ W,
+                              uint32_t* __restrict__ count, uint32_t cap) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= nnodes) return;
-  const float* r = info + (size_t)i * 8;
-  const float lo[3] = {r[0], r[1], r[2]}, hi[3] = {r[4], r[5], r[6]};
-  float f[MLP_IN];
-  box_features(lo, hi, __float_as_uint(r[3]), sn, f);
+  const uint32_t vs = visits[i];
+  if (vs < MLP_MIN_VISITS) return;
+  const QNode8 q = nodes[i];
+  uint32_t kids[8];
+  int nk = 0;
+  for (int c = 0; c < 8; c++)
+    if (q.meta[c] != 0 && ((q.imask >> c) & 1u)) kids[nk++] = q.child_base + (q.meta[c] & 0x7fu);
+  if (nk == 0) return;
+  const uint32_t base = atomicAdd(count, (uint32_t)nk);
+  const NodeInfo ps = info[i];
+  for (int j = 0; j < nk; j++) {
+    const uint32_t slot = base + (uint32_t)j;
+    if (slot >= cap || kids[j] >= nnodes) break;
+    const NodeInfo pc = info[kids[j]];
+    float f[MLP_IN];
+    split_features(pc.lo, pc.hi, pc.count, ps.lo, ps.hi, ps.count, sn, f);
 #pragma unroll
-  for (int k = 0; k < MLP_IN; k++) X[(size_t)i * MLP_IN + k] = f[k];
-  // target: log2 of the visit probability over the box's relative half area
-  // (the model learns the ray distribution's departure from uniform rays)
-  T[i] = log2f(((float)visits[i] + 0.5f) * inv_rays) - 16.0f * f[6];
-  W[i] = 1.0f;  // visit-weighted samples (log2 or sqrt of the count) measured worse
+    for (int k = 0; k < MLP_IN; k++) X[(size_t)slot * MLP_IN + k] = f[k];
+    // departure of the measured P(C | S) from the SAH's area ratio
+    T[slot] = log2f(((float)visits[kids[j]] + 0.5f) / ((float)vs + 0.5f)) - 16.0f * f[0];
+    W[slot] = 1.0f;
+  }
 }
 
 // minibatch indices: a multiplicative permutation of a random window
@@ -263,10 +278,11 @@ float mlp_gradient(SplitModel& m, const float* d_x, const float* d_t, const floa
   return l / (float)n;
 }
 
-void mlp_append_samples(SplitModel& m, const float* d_info, const uint32_t* d_visits, uint32_t nnodes, uint32_t rays,
-                        const SceneNorm& sn, hipStream_t s) {
-  if (nnodes == 0 || rays == 0) return;
-  const size_t total = (size_t)m.nsamples + nnodes;
+void mlp_append_samples(SplitModel& m, const QNode8* d_nodes, const NodeInfo* d_info, const uint32_t* d_visits,
+                        uint32_t nnodes, const SceneNorm& sn, hipStream_t s) {
+  if (nnodes == 0) return;
+  const size_t room = (size_t)nnodes * 8;  // at most one sample per child slot
+  const size_t total = (size_t)m.nsamples + room;
   if (total > 0xffffffffull) throw hip_error(hipErrorInvalidValue, "split-model training set too large");
   // grow (keeping the samples already there)
   auto grow = [&](DevBuf& b, size_t bytes_per) {
@@ -281,11 +297,17 @@ void mlp_append_samples(SplitModel& m, const float* d_info, const uint32_t* d_vi
   grow(m.X, MLP_IN * 4);
   grow(m.T, 4);
   grow(m.W, 4);
-  hipLaunchKernelGGL(k_mlp_samples, dim3(div_up(nnodes, 256)), dim3(256), 0, s, d_info, d_visits, nnodes,
-                     1.0f / (float)rays, sn, m.X.as<float>() + (size_t)m.nsamples * MLP_IN,
-                     m.T.as<float>() + m.nsamples, m.W.as<float>() + m.nsamples);
+  m.scount.reserve(256);
+  uint32_t* cnt = m.scount.as<uint32_t>();
+  RTG_HIP(hipMemsetAsync(cnt, 0, 4, s));
+  hipLaunchKernelGGL(k_mlp_samples, dim3(div_up(nnodes, 256)), dim3(256), 0, s, d_nodes, d_info, d_visits, nnodes, sn,
+                     m.X.as<float>() + (size_t)m.nsamples * MLP_IN, m.T.as<float>() + m.nsamples,
+                     m.W.as<float>() + m.nsamples, cnt, (uint32_t)room);
   RTG_HIP(hipGetLastError());
-  m.nsamples = (uint32_t)total;
+  uint32_t added = 0;
+  RTG_HIP(hipMemcpyAsync(&added, cnt, 4, hipMemcpyDeviceToHost, s));
+  RTG_HIP(hipStreamSynchronize(s));
+  m.nsamples += added < room ? added : (uint32_t)room;
 }
 
 float mlp_train(SplitModel& m, uint32_t steps, uint32_t batch, float lr, uint32_t seed, hipStream_t s) {

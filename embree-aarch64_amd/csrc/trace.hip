@@ -972,10 +972,11 @@ __global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, c
 // ctx_inst != RTC_INVALID_GEOMETRY_ID: the query context's one-level
 // instance stack is already in use, so every instance push fails and only
 // the scene's own triangles are tested (kernels/common/instance_stack.h:21-38).
-template <bool OCCLUDED, bool ALIGNED>
+template <bool OCCLUDED, bool ALIGNED, bool CNT = false>
 __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restrict__ rays, uint32_t M,
                                                       uint64_t stride, uint32_t ctx_inst, uint32_t* __restrict__ err,
-                                                      uint32_t* __restrict__ next, int refill_min) {
+                                                      uint32_t* __restrict__ next, int refill_min,
+                                                      uint32_t* __restrict__ top_visits = nullptr) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   __shared__ uint64_t lds_stack[8 * TB];
@@ -1055,6 +1056,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
         atomicOr(err, 4u);
         done = true;
       } else {
+        if (CNT && !in_obj) atomicAdd(top_visits + cur, 1u);
         NodeHit h;
         intersect_node(nodes, cur, r, lim, h);
         if (OCCLUDED) {
@@ -1385,6 +1387,25 @@ void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t strid
     if (aligned) hipLaunchKernelGGL((k_trace_inst<false, true>), grid, block, 0, s, top, rp, M, stride, ctx_inst, err, counter, refill);
     else hipLaunchKernelGGL((k_trace_inst<false, false>), grid, block, 0, s, top, rp, M, stride, ctx_inst, err, counter, refill);
   }
+  RTG_HIP(hipGetLastError());
+}
+
+void launch_trace_inst_counted(const DevTop& top, void* rays, uint32_t M, uint64_t stride, uint32_t* err,
+                               uint32_t* counter, uint32_t* top_visits, hipStream_t s, const TraceParams& tp) {
+  if (M == 0 || top.num_nodes == 0) return;
+  const bool aligned = ((uintptr_t)rays % 16 == 0) && (stride % 16 == 0);
+  const uint32_t blocks = div_up(M, TB);
+  const uint32_t want = (uint32_t)tp.cus * (uint32_t)tp.blocks_per_cu;
+  const dim3 grid(want < blocks ? want : blocks), block(TB);
+  const int refill = tp.refill_min < 1 ? 1 : (tp.refill_min > 64 ? 64 : tp.refill_min);
+  hipLaunchKernelGGL(k_zero_u32, dim3(1), dim3(1), 0, s, counter);
+  char* rp = (char*)rays;
+  if (aligned)
+    hipLaunchKernelGGL((k_trace_inst<false, true, true>), grid, block, 0, s, top, rp, M, stride, 0xffffffffu, err,
+                       counter, refill, top_visits);
+  else
+    hipLaunchKernelGGL((k_trace_inst<false, false, true>), grid, block, 0, s, top, rp, M, stride, 0xffffffffu, err,
+                       counter, refill, top_visits);
   RTG_HIP(hipGetLastError());
 }
 

@@ -71,6 +71,8 @@ SET_NODE_BOUNDS_FUNC = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.POINTER(ct
 CREATE_LEAF_FUNC = ctypes.CFUNCTYPE(ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t,
                                     ctypes.c_void_p)
 BUILD_PROGRESS_FUNC = ctypes.CFUNCTYPE(ctypes.c_bool, ctypes.c_void_p, ctypes.c_double)
+SPLIT_PRIMITIVE_FUNC = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_uint, ctypes.c_float, ctypes.c_void_p,
+                                        ctypes.c_void_p, ctypes.c_void_p)
 
 
 class RTCBuildArguments(ctypes.Structure):
@@ -552,7 +554,10 @@ class Scene:
 
 
 class SplitModel:
-    """Learned split-cost model (FP16 MLP trained on the GPU)."""
+    """Learned split-cost model (FP16 MLP trained on the GPU) on the per-bin
+    statistics of split candidates: FEATURES inputs per sample."""
+
+    FEATURES = 12
 
     def __init__(self, device, seed=1):
         self.device = device
@@ -581,14 +586,14 @@ class SplitModel:
         return p
 
     def predict(self, x):
-        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.FEATURES)
         y = np.zeros(len(x), np.float32)
         lib().rtcgSplitModelPredict(self.h, x.ctypes.data, len(x), y.ctypes.data)
         self.device.check("predict")
         return y
 
     def gradient(self, x, t, w=None):
-        x = np.ascontiguousarray(x, np.float32).reshape(-1, 8)
+        x = np.ascontiguousarray(x, np.float32).reshape(-1, self.FEATURES)
         t = np.ascontiguousarray(t, np.float32)
         w = np.ones(len(x), np.float32) if w is None else np.ascontiguousarray(w, np.float32)
         g = np.zeros(self.nparam(), np.float32)
@@ -602,7 +607,7 @@ class SplitModel:
 
     def samples(self):
         n = int(lib().rtcgSplitModelSampleCount(self.h))
-        x = np.zeros((n, 8), np.float32)
+        x = np.zeros((n, self.FEATURES), np.float32)
         t = np.zeros(n, np.float32)
         lib().rtcgSplitModelGetSamples(self.h, x.ctypes.data, t.ctypes.data)
         self.device.check("samples")
@@ -633,12 +638,17 @@ def batch_items(streams):
 
 
 def build_user_bvh(device, prims, quality=RTC_BUILD_QUALITY_MEDIUM, branching=2, min_leaf=1, max_leaf=32,
-                   block=1, traversal_cost=1.0, intersection_cost=1.0, progress=None, capacity=None):
+                   block=1, traversal_cost=1.0, intersection_cost=1.0, progress=None, capacity=None, count=None,
+                   split=None):
     """rtcBuildBVH with Python callbacks that record the tree the library
     builds over `prims` (an RTCBuildPrimitive array, reordered in place).
     Returns (root, nodes, leaves): nodes[id] = {"bounds": [(lo, hi)...],
     "children": [...]}, leaves[id] = (offset into prims, count); ids are the
-    void* values the callbacks returned (nodes odd, leaves even)."""
+    void* values the callbacks returned (nodes odd, leaves even). With
+    `count`, only the first `count` entries are primitives and the whole
+    array is the capacity; `split(prim, dim, pos)` (prim an RTCBuildPrimitive
+    record) returns the bounds of the two sides ((lo, hi), (lo, hi)) as the
+    splitPrimitive callback."""
     L = lib()
     bvh = L.rtcNewBVH(device.h)
     device.check("rtcNewBVH")
@@ -670,17 +680,28 @@ def build_user_bvh(device, prims, quality=RTC_BUILD_QUALITY_MEDIUM, branching=2,
         leaves[lid] = (off, n)
         return lid
 
+    def split_prim(pp, dim, pos, lb, rb, user):
+        rec = np.frombuffer(ctypes.string_at(pp, prims.itemsize), dtype=prims.dtype)[0]
+        (l0, h0), (l1, h1) = split(rec, dim, pos)
+        for ptr, (lo, hi) in ((lb, (l0, h0)), (rb, (l1, h1))):
+            b = ctypes.cast(ptr, ctypes.POINTER(RTCBounds)).contents
+            b.lower_x, b.lower_y, b.lower_z = lo
+            b.upper_x, b.upper_y, b.upper_z = hi
+
     cbs = [CREATE_NODE_FUNC(create_node), SET_NODE_CHILDREN_FUNC(set_children), SET_NODE_BOUNDS_FUNC(set_bounds),
            CREATE_LEAF_FUNC(create_leaf), BUILD_PROGRESS_FUNC(progress or (lambda u, f: True))]
+    split_cb = SPLIT_PRIMITIVE_FUNC(split_prim) if split else None
     a = default_build_arguments()
     a.buildQuality = quality
     a.maxBranchingFactor, a.minLeafSize, a.maxLeafSize, a.sahBlockSize = branching, min_leaf, max_leaf, block
     a.traversalCost, a.intersectionCost = traversal_cost, intersection_cost
     a.bvh = bvh
     a.primitives = base
-    a.primitiveCount = len(prims)
-    a.primitiveArrayCapacity = len(prims) if capacity is None else capacity
+    a.primitiveCount = len(prims) if count is None else count
+    a.primitiveArrayCapacity = (len(prims) if count is None else len(prims)) if capacity is None else capacity
     a.createNode, a.setNodeChildren, a.setNodeBounds, a.createLeaf, a.buildProgress = cbs
+    if split_cb:
+        a.splitPrimitive = ctypes.cast(split_cb, ctypes.c_void_p)
     root = L.rtcBuildBVH(ctypes.byref(a))
     err = L.rtcGetDeviceError(device.h)
     L.rtcReleaseBVH(bvh)

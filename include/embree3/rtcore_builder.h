@@ -112,8 +112,13 @@ RTC_API RTCBVH rtcNewBVH(RTCDevice device);
  * reordered in place, and leaves receive pointers into it, as the
  * reference's binned-SAH builder does. Differences: maxBranchingFactor is
  * capped at 8 (the GPU collapse's node width); maxDepth is not enforced
- * (leaf sizes are); RTC_BUILD_QUALITY_HIGH builds the binned-SAH tree
- * without spatial splits (splitPrimitive is not called); primitives with
+ * (leaf sizes are); at RTC_BUILD_QUALITY_HIGH with a splitPrimitive
+ * callback, primitives are first cut by that callback while the array has
+ * room (primitiveArrayCapacity), the cut that most reduces the summed box
+ * half areas first, at the coarsest plane of a dyadic grid over the
+ * primitives' bounds across each box's largest axis, and the tree is built
+ * over the pieces (a primitive may then sit in several leaves, as with the
+ * reference's spatial splits, rtcore_builder.cpp:379-380); primitives with
  * non-finite or inverted bounds are left out of the tree; the callbacks run
  * on the calling thread. Memory from rtcThreadLocalAlloc stays valid until
  * the next build on the same RTCBVH or its release. */

@@ -191,28 +191,33 @@ RTC_API void rtcgReleaseBatchGraph(RTCGBatchGraph graph);
 /* ---- learned split cost --------------------------------------------- */
 /* No counterpart in the reference: its binned-SAH split score
  * (kernels/builders/heuristic_binning.h:340-393, BinInfoT::best) is the plain
- * cost these functions replace. A split model is a small FP16 MLP (8 box
- * features -> 32 -> 32 -> 1, FP32 accumulate) predicting log2 of the
- * probability that a ray of a target distribution visits a node with a given
- * box. Trained on per-node visit counts of the library's own traversal, it
- * replaces the half-area term of the binned-SAH split cost of segments with
- * more than min_count (default 64) primitives. Hits do not depend on the
- * tree, so scenes built with a model give the same records.
- * Feature vector of a box (lo, hi) holding n primitives, relative to the
- * scene box (slo, sext) of the collected data: ((lo+hi)/2 - slo)/sext (3),
- * (hi-lo)/sext (3), log2(halfArea/sceneHalfArea + 2^-30)/16, log2(n)/16. */
+ * cost these functions correct. A split model is a small FP16 MLP (12
+ * features -> 32 -> 32 -> 1, FP32 accumulate) on the per-bin statistics of a
+ * split candidate: for a child C (one side's bins: box and primitive count)
+ * of a segment S (box and count) it predicts y = log2 P(C | S) -
+ * log2(A_C / A_S), how far the fraction of the target distribution's rays
+ * that enter S and then C departs from the SAH's area ratio. The binned-SAH
+ * builder then prices a split A_L 2^y_L n_L + A_R 2^y_R n_R (y held within
+ * [-3, 3] and to P <= 1) for segments with more than min_count (default 64)
+ * primitives. The model is trained on the GPU from the node-entry counts of
+ * the library's own traversal of a binary (width-2) tree of the scene. Hits
+ * do not depend on the tree, so scenes built with a model give the same
+ * records.
+ * Features of C in S, scene box (slo, sext): log2(A_C/A_S + 2^-30)/16,
+ * n_C/n_S, (centre_C - centre_S)/extent_S (3), extent_C/extent_S (3),
+ * (centre_S - slo)/sext (3), log2(A_S/A_scene + 2^-30)/16. */
 typedef struct RTCGSplitModelTy* RTCGSplitModel;
 
 RTC_API RTCGSplitModel rtcgNewSplitModel(RTCDevice device, unsigned int seed);
 RTC_API void rtcgRetainSplitModel(RTCGSplitModel model);
 RTC_API void rtcgReleaseSplitModel(RTCGSplitModel model);
 
-/* Parameter vector (FP32 master copy): W1[32][8] b1[32] W2[32][32] b2[32] w3[32] b3. */
+/* Parameter vector (FP32 master copy): W1[32][12] b1[32] W2[32][32] b2[32] w3[32] b3. */
 RTC_API unsigned int rtcgSplitModelParamCount(void);
 RTC_API void rtcgSplitModelSetParams(RTCGSplitModel model, const float* params);
 RTC_API void rtcgSplitModelGetParams(RTCGSplitModel model, float* params);
 
-/* Inference on n feature vectors (host arrays, n x 8 in, n out). */
+/* Inference on n feature vectors (host arrays, n x 12 in, n out). */
 RTC_API void rtcgSplitModelPredict(RTCGSplitModel model, const float* features, unsigned int n, float* out);
 
 /* Loss sum_i w_i (y_i - t_i)^2 / n of the FP16 forward pass and its FP32
@@ -220,9 +225,12 @@ RTC_API void rtcgSplitModelPredict(RTCGSplitModel model, const float* features, 
 RTC_API float rtcgSplitModelGradient(RTCGSplitModel model, const float* features, const float* targets,
                                      const float* weights, unsigned int n, float* grad);
 
-/* Traces M closest-hit records (device memory, not modified) through the
- * committed scene, counts visits per node and appends one training sample
- * per node: its features and log2((visits + 0.5) / M). */
+/* Builds a binary (width-2) binned-SAH tree of the committed scene's
+ * triangles, traces M closest-hit records (device memory, not modified)
+ * through it counting the rays that enter each node, and appends one
+ * training sample per (parent, internal child) pair whose parent at least 8
+ * rays entered: the child's features in the parent and the target
+ * log2((v_C + 0.5) / (v_S + 0.5)) - log2(A_C / A_S). */
 RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const void* d_rays, unsigned int M,
                                    size_t byteStride);
 RTC_API unsigned int rtcgSplitModelSampleCount(RTCGSplitModel model);

@@ -6,17 +6,19 @@ product imports this module.
 
 The reference has no learned component: its split score is the binned-SAH
 `BinInfoT::best` (kernels/builders/heuristic_binning.h:340-393). This module
-restates the network the GPU library substitutes for the half-area term:
-8 inputs -> 32 ReLU -> 32 ReLU -> 1, weights and activations rounded to
+restates the network with which the GPU library corrects the half-area
+term, and the features it is fed (the per-bin statistics of a split
+candidate): 12 inputs -> 32 ReLU -> 32 ReLU -> 1, weights and activations
+rounded to
 FP16, products accumulated in higher precision, FP32 biases; loss
 sum_i w_i (y_i - t_i)^2 / n. With quantize=False every value stays float64,
 which is the form the finite-difference test pins.
 
-Parameter vector: W1[32][8] | b1[32] | W2[32][32] | b2[32] | w3[32] | b3.
+Parameter vector: W1[32][12] | b1[32] | W2[32][32] | b2[32] | w3[32] | b3.
 """
 import numpy as np
 
-IN, H = 8, 32
+IN, H = 12, 32
 OFF_W1, OFF_B1 = 0, H * IN
 OFF_W2 = OFF_B1 + H
 OFF_B2 = OFF_W2 + H * H
@@ -79,20 +81,34 @@ def loss_and_grad(p, X, T, Wt=None, quantize=True):
     return loss, g
 
 
-def box_features(lo, hi, count, scene_lo, scene_hi):
-    """Features of boxes (n x 3 lo/hi) holding `count` primitives, relative to
-    the scene box, as the builder and the trainer form them."""
-    lo = np.asarray(lo, np.float32).reshape(-1, 3)
-    hi = np.asarray(hi, np.float32).reshape(-1, 3)
-    slo = np.asarray(scene_lo, np.float32)
-    ext = np.asarray(scene_hi, np.float32) - slo
-    inv = np.where(ext > 0, np.float32(1) / np.where(ext > 0, ext, 1), 0).astype(np.float32)
-    d = hi - lo
-    ha = lambda v: v[:, 0] * (v[:, 1] + v[:, 2]) + v[:, 1] * v[:, 2]
-    sa = ext[0] * (ext[1] + ext[2]) + ext[1] * ext[2]
-    f = np.zeros((len(lo), IN), np.float32)
-    f[:, 0:3] = ((lo + hi) * np.float32(0.5) - slo) * inv
-    f[:, 3:6] = d * inv
-    f[:, 6] = np.log2(ha(d) / sa + 2.0 ** -30) / 16
-    f[:, 7] = np.log2(np.maximum(np.asarray(count, np.float64), 1)) / 16
+def split_features(clo, chi, ccount, plo, phi, pcount, scene_lo, scene_hi):
+    """Features of child boxes C (n x 3 lo/hi, ccount primitives) of parent
+    boxes S (plo/phi, pcount), as the builder forms them for each side of a
+    split candidate from its bins and the trainer for each (parent, child)
+    pair of a binary tree: log2(A_C/A_S + 2^-30)/16, n_C/n_S, centre offset
+    over the parent's extent (3), extent ratio (3, 1 on a flat axis), the
+    parent's centre in the scene box (3), log2(A_S/A_scene + 2^-30)/16;
+    float32 arithmetic in the builder's order."""
+    f32 = np.float32
+    clo, chi = (np.asarray(a, f32).reshape(-1, 3) for a in (clo, chi))
+    plo, phi = (np.asarray(a, f32).reshape(-1, 3) for a in (plo, phi))
+    slo = np.asarray(scene_lo, f32)
+    sext = np.asarray(scene_hi, f32) - slo
+    sinv = np.where(sext > 0, f32(1) / np.where(sext > 0, sext, 1), 0).astype(f32)
+    ha = lambda d: d[:, 0] * (d[:, 1] + d[:, 2]) + d[:, 1] * d[:, 2]
+    sa = f32(sext[0] * (sext[1] + sext[2]) + sext[1] * sext[2])
+    s_inv_area = f32(1) / sa if sa > 0 else f32(0)
+    ac, ap = ha(chi - clo), ha(phi - plo)
+    n = len(clo)
+    f = np.zeros((n, IN), f32)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        f[:, 0] = np.where(ap > 0, np.log2(ac / np.where(ap > 0, ap, 1) + f32(2.0 ** -30)) * f32(0.0625), 0)
+        pc = np.asarray(pcount, np.float64) * np.ones(n)
+        f[:, 1] = np.where(pc > 0, np.asarray(ccount, np.float64) / np.where(pc > 0, pc, 1), 1)
+        ep = phi - plo
+        inv = np.where(ep > 0, f32(1) / np.where(ep > 0, ep, 1), 0).astype(f32)
+        f[:, 2:5] = ((chi + clo) - (phi + plo)) * f32(0.5) * inv
+        f[:, 5:8] = np.where(ep > 0, (chi - clo) * inv, 1)
+        f[:, 8:11] = ((phi + plo) * f32(0.5) - slo) * sinv
+        f[:, 11] = np.log2(ap * s_inv_area + f32(2.0 ** -30)) * f32(0.0625)
     return f

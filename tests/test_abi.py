@@ -104,6 +104,10 @@ def test_c_program_links_against_library(tmp_path, rtc):
     libdir = os.path.dirname(rtc.LIB_PATH)
     subprocess.run(["gcc", "-I", INCLUDE, str(src), "-L", libdir, "-l:libembree3.so", "-o", str(tmp_path / "app")],
                    check=True)
+    # the program needs the library by its soname, libembree3.so.3, as a
+    # program linked against the reference does
+    env = dict(os.environ, LD_LIBRARY_PATH=libdir)
+    subprocess.run([str(tmp_path / "app")], check=True, env=env, timeout=120)
 
 
 def test_integration_example_compiles_as_c(tmp_path, rtc):

@@ -78,27 +78,31 @@ def heightfield_case(rtc, n=256):
 
 def test_collect_and_train(rtc, device, oracle):
     v, ix, cam = heightfield_case(rtc)
-    sc = rtc.Scene(device, [(v, ix)], quality=2).commit()
+    sc = rtc.Scene(device, [(v, ix)]).commit()
     m = rtc.SplitModel(device, seed=11)
     d = device.malloc(cam.nbytes)
     try:
         device.memcpy(d, cam.ctypes.data, cam.nbytes)
         m.collect(sc, d, len(cam))
         X, T = m.samples()
-        st = sc.stats()
-        assert len(X) == st["num_nodes"]
-        # targets are log2(visit probability) - log2(relative half area): the
-        # root (the scene box) is visited by every ray, no node more often
-        logp = T + 16 * X[:, 6]
-        assert abs(T[0] - np.log2((len(cam) + 0.5) / len(cam))) < 1e-5
-        assert logp.max() <= logp[0] + 1e-4
-        assert (X[:, 0:3] >= -1e-6).all() and (X[:, 0:3] <= 1 + 1e-6).all()
+        assert X.shape[1] == mlp.IN and len(X) == len(T) > 100
+        # one sample per (parent, child) pair of a binary tree: the target is
+        # log2 of the measured P(C | S) minus log2 of the area ratio, and a
+        # child is entered only after its parent, so their sum is at most 0
+        logp = T + 16 * X[:, 0]
+        assert logp.max() <= 1e-4 and X[:, 0].max() <= 1e-6
+        assert (X[:, 1] > 0).all() and (X[:, 1] < 1).all()              # a child holds part of the parent
+        assert (X[:, 5:8] >= -1e-6).all() and (X[:, 5:8] <= 1 + 1e-6).all()  # and lies inside it
+        assert (np.abs(X[:, 2:5]) <= 0.5 + 1e-6).all()
+        assert (X[:, 8:11] >= -1e-6).all() and (X[:, 8:11] <= 1 + 1e-6).all()
         l0, _ = m.gradient(X, T)
         last = m.train(300, batch=4096, lr=3e-3, seed=1)
         l1, _ = m.gradient(X, T)
         assert l1 < 0.5 * l0 and last < 0.5 * l0
         pred = m.predict(X)
-        assert np.corrcoef(pred, T)[0, 1] > 0.6
+        # conditional entry rates are noisy (a few rays per small node); the
+        # fit explains a good part of their spread
+        assert np.corrcoef(pred, T)[0, 1] > 0.35
         # a scene built with the trained model: same records as the oracle
         sc2 = rtc.Scene(device, [(v, ix)])
         rtc.lib().rtcgSetSceneSplitModel(sc2.h, m.h)
@@ -124,7 +128,7 @@ def test_sah_slack_bounds_the_tree(rtc, device):
     and a negative or NaN slack is an invalid argument."""
     import ctypes
     v, ix, cam = heightfield_case(rtc)
-    sc = rtc.Scene(device, [(v, ix)], quality=2).commit()
+    sc = rtc.Scene(device, [(v, ix)]).commit()
     m = rtc.SplitModel(device, seed=11)
     d = device.malloc(cam.nbytes)
     try:

@@ -1,0 +1,117 @@
+"""Tree quality of the GPU binned-SAH builder against the oracle's BVH8 (the
+reference's builder restated: kernels/builders/bvh_builder_sah.h:155-330 with
+the triangle-scene settings of kernels/bvh/bvh_builder_sah.cpp:560-565:
+leaves of 4..28 triangles priced per block of 4, node and block cost 1), by
+the reference's SAH statistic (kernels/bvh/bvh_statistics.cpp:52-56,139-146):
+node half areas plus leaf half areas times ceil(count/4), over the root's
+half area.
+
+The statistic prices leaves the way the reference's builder does, so the GPU
+binned-SAH tree (MEDIUM quality, no spatial splits) is built with the same
+pricing for the comparison (device keys
+sah_block=4, cost_node=1, cost_tri=1; its leaves hold at most 15 triangles,
+the reference's up to 28): it must be at least as good. The GPU's default
+tree prices every triangle (0.6 per triangle, leaves of at most 8), which
+suits one-triangle-per-lane traversal; its per-triangle statistic is
+printed beside the others. The plain GPU
+binned-SAH tree is the judge of the learned split cost. The Morton tree (LOW
+quality) is allowed to be worse by a stated margin."""
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SAH_BOUND = 1.005      # GPU tree / oracle tree, reference leaf pricing
+MORTON_BOUND = 1.25    # GPU Morton tree / oracle tree
+REFERENCE_PRICING = "sah_block=4,cost_node=1,cost_tri=1,max_leaf=15"
+
+
+def _build(rtc, meshes, config, quality):
+    dev = rtc.Device(config)
+    try:
+        sc = rtc.Scene(dev, meshes, quality=quality).commit()
+        st = sc.stats()
+        sc.release()
+        return st
+    finally:
+        dev.release()
+
+
+def _meshes(rtc, workload):
+    if workload == "soup_2^20":
+        return [rtc.gen_soup(1 << 20, seed=1, h=0.005)]
+    if workload == "heightfield_1024":
+        return [rtc.gen_heightfield(1024, seed=11, amplitude=0.2, octaves=6)]
+    if workload == "sphere_c1":
+        return [rtc.gen_sphere(128)]
+    return [rtc.gen_displaced_sphere(400, seed=1001, amplitude=0.1)]
+
+
+@pytest.mark.parametrize("workload", ["soup_2^20", "heightfield_1024", "sphere_c1", "displaced_sphere"])
+def test_sah_statistic_against_oracle(rtc, oracle, workload):
+    meshes = _meshes(rtc, workload)
+    ref = oracle.BVH(meshes).stats()
+    g = _build(rtc, meshes, REFERENCE_PRICING, rtc.RTC_BUILD_QUALITY_MEDIUM)
+    assert g["num_prims"] == ref["prims"]
+    print(f"{workload}: reference pricing: GPU {g['sah']:.4f}, oracle {ref['sah']:.4f}, "
+          f"ratio {g['sah'] / ref['sah']:.4f}")
+    assert g["sah"] <= SAH_BOUND * ref["sah"], (workload, g["sah"], ref["sah"])
+    d = _build(rtc, meshes, "", rtc.RTC_BUILD_QUALITY_MEDIUM)
+    print(f"{workload}: per-triangle statistic: GPU default {d['sah_tri']:.4f}, GPU reference-priced "
+          f"{g['sah_tri']:.4f}, oracle {ref['sah_per_tri']:.4f}")
+    m = _build(rtc, meshes, REFERENCE_PRICING, rtc.RTC_BUILD_QUALITY_LOW)
+    print(f"{workload}: Morton tree {m['sah']:.4f}, ratio {m['sah'] / ref['sah']:.4f}")
+    assert m["sah"] <= MORTON_BOUND * ref["sah"], (workload, m["sah"], ref["sah"])
+
+
+def _slivers(n=20000, seed=3):
+    """Long thin triangles across a unit cube: boxes far larger than the
+    triangles, where spatial splits pay."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    p = rng.random((n, 3), dtype=np.float32)
+    d = rng.normal(size=(n, 3)).astype(np.float32)
+    d *= np.float32(0.4) / np.linalg.norm(d, axis=1, keepdims=True)
+    w = rng.normal(size=(n, 3)).astype(np.float32) * np.float32(0.004)
+    v = np.stack([p, p + d, p + d + w], 1).reshape(-1, 3).astype(np.float32)
+    ix = np.arange(3 * n, dtype=np.uint32).reshape(-1, 3)
+    return v, ix
+
+
+@pytest.mark.parametrize("workload", ["slivers", "soup_2^20", "heightfield_1024"])
+def test_spatial_splits_at_high_quality(rtc, oracle, workload):
+    """RTC_BUILD_QUALITY_HIGH cuts triangles into pieces before the
+    binned-SAH build (the reference's spatial-split SAH at HIGH,
+    kernels/bvh/bvh8_factory.cpp:867-870): the tree holds more triangle
+    records than triangles, its SAH statistic (references counted in their
+    leaves) is no worse than the MEDIUM tree's, and every closest hit and
+    occlusion record equals the oracle's."""
+    meshes = [_slivers()] if workload == "slivers" else _meshes(rtc, workload)
+    ntri = sum(len(ix) for _, ix in meshes)
+    med = _build(rtc, meshes, "", rtc.RTC_BUILD_QUALITY_MEDIUM)
+    dev = rtc.Device("")
+    try:
+        sc = rtc.Scene(dev, meshes, quality=rtc.RTC_BUILD_QUALITY_HIGH).commit()
+        hi = sc.stats()
+        print(f"{workload}: HIGH {hi['num_prims']} records for {ntri} triangles, SAH {hi['sah']:.4f} "
+              f"(MEDIUM {med['sah']:.4f}), per-triangle {hi['sah_tri']:.4f} (MEDIUM {med['sah_tri']:.4f})")
+        assert ntri < hi["num_prims"] <= ntri + int(0.3 * ntri) + 1
+        assert hi["sah"] <= med["sah"] * 1.0001, (hi["sah"], med["sah"])
+        rays = rtc.gen_random_rays(1 << 18, seed=41)
+        rays["tfar"][::5] = 0.05
+        got = rays.copy()
+        sc.intersect1M(got)
+        ref = rays.copy()
+        oracle.BVH(meshes).intersect(ref)
+        from conftest import mismatches
+        bad = mismatches(got, ref)
+        assert len(bad) == 0, f"{len(bad)} records differ, first {bad[:5]}"
+        occ = rays.copy()
+        occ["tfar"] = 0.2
+        g2 = occ.copy()
+        sc.occluded1M(g2)
+        r2 = occ.copy()
+        oracle.BVH(meshes).occluded(r2)
+        assert (g2["tfar"] == r2["tfar"]).all()
+        sc.release()
+    finally:
+        dev.release()

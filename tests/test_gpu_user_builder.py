@@ -8,8 +8,9 @@ builders guarantee and that a caller relies on: every primitive in exactly
 one leaf, leaves pointing into the caller's reordered primitive array, the
 bounds given to setNodeBounds equal to the union of the boxes below, child
 counts within [2, maxBranchingFactor], leaf sizes within maxLeafSize, no
-inner node over at most minLeafSize primitives, and the SAH cost no worse
-than a plain object-median tree over the same boxes."""
+inner node over at most minLeafSize primitives, the SAH cost no worse than a
+plain object-median tree over the same boxes, and, with the reference's
+triangle-scene settings, within 2 % of the oracle's binned-SAH tree."""
 import numpy as np
 import pytest
 
@@ -39,8 +40,9 @@ def half_area(lo, hi):
     return d[..., 0] * (d[..., 1] + d[..., 2]) + d[..., 1] * d[..., 2]
 
 
-def check_tree(prims, root, nodes, leaves, branching, min_leaf, max_leaf):
-    """Walks the recorded tree; returns its SAH (nodes cost 1, primitives 1)."""
+def check_tree(prims, root, nodes, leaves, branching, min_leaf, max_leaf, block=1):
+    """Walks the recorded tree; returns its SAH (nodes cost 1, leaves their
+    count in blocks of `block` primitives)."""
     lo, hi = boxes_of(prims)
     seen = np.zeros(len(prims), np.int32)
     sah = [0.0]
@@ -51,7 +53,7 @@ def check_tree(prims, root, nodes, leaves, branching, min_leaf, max_leaf):
             assert 1 <= cnt <= max_leaf
             seen[off:off + cnt] += 1
             l, h = lo[off:off + cnt].min(0), hi[off:off + cnt].max(0)
-            sah[0] += half_area(l, h) * cnt
+            sah[0] += half_area(l, h) * ((cnt + block - 1) // block)
             return l, h, cnt
         nd = nodes[ref]
         n = nd["n"]
@@ -110,6 +112,26 @@ def test_user_builder_tree(rtc, device, quality, branching, min_leaf, max_leaf):
     assert sah <= ref * (1.3 if quality == 0 else 1.05), (sah, ref)
 
 
+def test_user_builder_against_oracle_binned_sah(rtc, device, oracle):
+    """With the reference's triangle-scene settings (branching 8, leaves of
+    4..28 primitives priced per block of 4; kernels/bvh/bvh_builder_sah.cpp:560-565)
+    the user tree's SAH statistic is within 2 % of the oracle's binned-SAH
+    tree over the same boxes; the oracle gets each box as the degenerate
+    triangle (lo, hi, lo), whose bounds the box is."""
+    prims = random_prims(rtc, 20000, seed=77)
+    orig = prims.copy()
+    root, nodes, leaves = rtc.build_user_bvh(device, prims, quality=1, branching=8, min_leaf=4, max_leaf=28,
+                                             block=4)
+    sah = check_tree(prims, root, nodes, leaves, 8, 4, 28, block=4)
+    lo, hi = boxes_of(orig)
+    v = np.ascontiguousarray(np.stack([lo, hi, lo], 1).reshape(-1, 3), np.float32)
+    ix = np.arange(len(v), dtype=np.uint32).reshape(-1, 3)
+    ref = oracle.BVH([(v, ix)]).stats()
+    assert ref["prims"] == len(prims)
+    print(f"user tree SAH {sah:.4f}, oracle binned SAH {ref['sah']:.4f}")
+    assert sah <= 1.02 * ref["sah"], (sah, ref["sah"])
+
+
 def test_user_builder_small_and_empty(rtc, device):
     for n in (1, 2, 3, 33):
         prims = random_prims(rtc, n, seed=n)
@@ -143,3 +165,84 @@ def test_user_builder_sah_block_and_costs(rtc, device):
     _, _, leaves_c = rtc.build_user_bvh(device, prims.copy(), branching=4, max_leaf=16, block=4)
     _, _, leaves_d = rtc.build_user_bvh(device, prims.copy(), branching=4, max_leaf=16, block=1)
     assert mean(leaves_c) >= mean(leaves_d)
+
+
+def test_user_builder_high_quality_spatial_splits(rtc, device):
+    """At HIGH quality with a splitPrimitive callback (reference
+    kernels/common/rtcore_builder.cpp:379-380) the builder cuts primitives
+    while the array has room: here thin diagonal segments, whose callback
+    returns the bounds of the segment's part on each side of the plane.
+    The callback is called, the pieces (primID kept) stay inside their
+    primitive's box and together cover the segment, the tree passes the
+    structural checks, and its SAH is below the MEDIUM tree's over the uncut
+    boxes."""
+    rng = np.random.default_rng(17)
+    n, cap = 4000, 8000
+    # endpoints exact in float32, so the float32 boxes hold the segments
+    a = rng.random((n, 3)).astype(np.float32).astype(np.float64)
+    b = (a + rng.normal(size=(n, 3)) * 0.15).astype(np.float32).astype(np.float64)
+    prims = np.zeros(cap, rtc.RTCBuildPrimitive)
+    lo, hi = np.minimum(a, b).astype(np.float32), np.maximum(a, b).astype(np.float32)
+    prims["lower_x"][:n], prims["lower_y"][:n], prims["lower_z"][:n] = lo.T
+    prims["upper_x"][:n], prims["upper_y"][:n], prims["upper_z"][:n] = hi.T
+    prims["geomID"][:n] = 3
+    prims["primID"][:n] = np.arange(n, dtype=np.uint32)
+    orig = prims[:n].copy()
+    calls = [0]
+
+    def split(rec, dim, pos):
+        # the part of segment a->b inside the record's box, cut at x_dim = pos
+        calls[0] += 1
+        i = int(rec["primID"])
+        p0, p1 = a[i], b[i]
+        blo = np.array([rec["lower_x"], rec["lower_y"], rec["lower_z"]], np.float64)
+        bhi = np.array([rec["upper_x"], rec["upper_y"], rec["upper_z"]], np.float64)
+        t0, t1 = 0.0, 1.0
+        d = p1 - p0
+        for k in range(3):
+            if abs(d[k]) > 1e-12:
+                ta, tb = sorted(((blo[k] - p0[k]) / d[k], (bhi[k] - p0[k]) / d[k]))
+                t0, t1 = max(t0, ta), min(t1, tb)
+        tc = (pos - p0[dim]) / d[dim] if abs(d[dim]) > 1e-12 else 0.5
+        tc = min(max(tc, t0), t1)
+        out = []
+        for u, v in ((t0, tc), (tc, t1)):
+            q0, q1 = p0 + u * d, p0 + v * d
+            plo = np.maximum(np.minimum(q0, q1) - 1e-6, blo)
+            phi = np.minimum(np.maximum(q0, q1) + 1e-6, bhi)
+            out.append((tuple(float(x) for x in plo), tuple(float(x) for x in phi)))
+        # the side below the plane first
+        if p0[dim] > p1[dim]:
+            out = out[::-1]
+        return out[0], out[1]
+
+    root, nodes, leaves = rtc.build_user_bvh(device, prims, quality=rtc.RTC_BUILD_QUALITY_HIGH, branching=4,
+                                             max_leaf=8, count=n, split=split)
+    assert calls[0] > 0
+    used = sum(c for _, c in leaves.values())
+    assert n < used <= cap
+    refs = prims[:used]
+    pid = refs["primID"].astype(np.int64)
+    assert set(pid.tolist()) == set(range(n))
+    # pieces inside their primitive's box
+    for f in ("lower_x", "lower_y", "lower_z"):
+        assert (refs[f] >= orig[f][pid] - 1e-6).all()
+    for f in ("upper_x", "upper_y", "upper_z"):
+        assert (refs[f] <= orig[f][pid] + 1e-6).all()
+    # every point of every segment lies in one of its pieces
+    rlo = np.stack([refs["lower_x"], refs["lower_y"], refs["lower_z"]], 1)
+    rhi = np.stack([refs["upper_x"], refs["upper_y"], refs["upper_z"]], 1)
+    order = np.argsort(pid, kind="stable")
+    starts = np.searchsorted(pid[order], np.arange(n))
+    ends = np.searchsorted(pid[order], np.arange(n), side="right")
+    for i in range(0, n, 7):
+        sel = order[starts[i]:ends[i]]
+        pts = a[i] + np.linspace(0, 1, 33)[:, None] * (b[i] - a[i])
+        inside = ((pts[:, None, :] >= rlo[sel][None] - 1e-5) & (pts[:, None, :] <= rhi[sel][None] + 1e-5)).all(2)
+        assert inside.any(1).all(), i
+    sah_high = check_tree(refs, root, nodes, leaves, 4, 1, 8)
+    mp = orig.copy()
+    r2, n2, l2 = rtc.build_user_bvh(device, mp, quality=rtc.RTC_BUILD_QUALITY_MEDIUM, branching=4, max_leaf=8)
+    sah_med = check_tree(mp, r2, n2, l2, 4, 1, 8)
+    print(f"HIGH with splits: {used} references, SAH {sah_high:.3f}; MEDIUM: SAH {sah_med:.3f}")
+    assert sah_high < sah_med

@@ -57,9 +57,23 @@ def test_gradient_descent_on_a_known_function():
     assert l1 < 0.1 * l0
 
 
-def test_box_features_closed_form():
-    f = mlp.box_features([[0.25, 0, 0]], [[0.75, 1, 0.5]], [256], [0, 0, 0], [1, 1, 1])[0]
-    # centre, extent, log2(half area ratio)/16 with ha = 0.5*(1+0.5)+1*0.5 = 1.25 of 3, log2(256)/16
-    assert np.allclose(f[:6], [0.5, 0.5, 0.25, 0.5, 1.0, 0.5])
-    assert np.isclose(f[6], np.log2(1.25 / 3 + 2.0 ** -30) / 16, rtol=1e-6)
-    assert f[7] == 0.5
+def test_split_features_closed_form():
+    # child [0.25,0,0]-[0.75,1,0.5] with 256 of the 1024 primitives of the
+    # parent [0,0,0]-[1,1,1], in the scene box [0,0,0]-[2,2,2]
+    f = mlp.split_features([[0.25, 0, 0]], [[0.75, 1, 0.5]], [256], [[0, 0, 0]], [[1, 1, 1]], [1024],
+                           [0, 0, 0], [2, 2, 2])[0]
+    # half areas: child 0.5*(1+0.5)+1*0.5 = 1.25, parent 3, scene 12
+    assert np.isclose(f[0], np.log2(1.25 / 3 + 2.0 ** -30) / 16, rtol=1e-6)
+    assert f[1] == 0.25
+    assert np.allclose(f[2:5], [0.0, 0.0, -0.25])   # centre offsets over the parent's extent
+    assert np.allclose(f[5:8], [0.5, 1.0, 0.5])     # extent ratios
+    assert np.allclose(f[8:11], [0.25, 0.25, 0.25])  # parent centre in the scene box
+    assert np.isclose(f[11], np.log2(3 / 12 + 2.0 ** -30) / 16, rtol=1e-6)
+    # a child equal to its parent: SAH estimate 1 (feature 0 at the floor's
+    # distance from 0), no offset, full extent
+    g = mlp.split_features([[0, 0, 0]], [[1, 1, 1]], [8], [[0, 0, 0]], [[1, 1, 1]], [8], [0, 0, 0], [1, 1, 1])[0]
+    assert abs(g[0]) < 1e-9 and g[1] == 1 and np.allclose(g[2:5], 0) and np.allclose(g[5:8], 1)
+    # a flat parent axis: offset 0 and extent ratio 1 on it
+    h = mlp.split_features([[0, 0, 0.5]], [[0.5, 1, 0.5]], [1], [[0, 0, 0.5]], [[1, 1, 0.5]], [2], [0, 0, 0],
+                           [1, 1, 1])[0]
+    assert h[4] == 0 and h[7] == 1 and np.isclose(h[5], 0.5)
