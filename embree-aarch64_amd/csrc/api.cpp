@@ -445,6 +445,25 @@ rtg::BuildParams scene_params(Scene* sc, rtg::LearnedCost& lc, const rtg::BuildP
   return p;
 }
 
+// Builds the scene's own tree into sc->bvh. Where spatial splits cut
+// triangles (HIGH quality), the plain binned-SAH tree over the same
+// triangles is built as well and the tree with the lower SAH statistic is
+// kept: the splits are chosen by a per-triangle estimate before the tree
+// exists, and this keeps them from making a tree worse than it would be
+// without them (the reference's spatial-split builder compares both kinds
+// of split at every node, kernels/builders/heuristic_spatial_array.h).
+void build_scene_tree(Scene* sc, Device* d, const std::vector<rtg::GeomDesc>& descs, const rtg::BuildParams& p,
+                      hipStream_t st) {
+  rtg::build_bvh(descs, p, d->scratch, *sc->bvh, st);
+  if (sc->bvh->stats.num_split == 0) return;
+  rtg::BuildParams q = p;
+  q.split_budget = 0.0f;
+  rtg::DeviceBVH plain;
+  rtg::build_bvh(descs, q, d->scratch, plain, st);
+  RTG_HIP(hipStreamSynchronize(st));
+  if (plain.stats.sah <= sc->bvh->stats.sah) sc->bvh->swap_tree(plain);
+}
+
 // Uploads the enabled triangle geometries of sc (scene index `scene` of a
 // build) into device buffers owned by sc, appending their descriptors;
 // prim_offset advances over their triangles.
@@ -734,7 +753,7 @@ void commit_scene(Scene* sc) {
       sc->bvh = std::make_shared<rtg::DeviceBVH>();
       sc->slot = 0;
     }
-    rtg::build_bvh(descs, p, d->scratch, *sc->bvh, t.stream);
+    build_scene_tree(sc, d, descs, p, t.stream);
   }
   RTG_HIP(hipStreamSynchronize(t.stream));
   build_top(sc, d, t);
@@ -1602,7 +1621,7 @@ RTC_API void rtcgTimeBuild(RTCScene scene, int steps, float* ms_out) {
       RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene shares its BVH with a batch: commit it on its own first");
     rtg::LearnedCost lc{};
     const rtg::BuildParams p = scene_params(s, lc);
-    rtg::build_bvh(s->descs, p, d->scratch, *s->bvh, t.stream);
+    build_scene_tree(s, d, s->descs, p, t.stream);
     RTG_HIP(hipStreamSynchronize(t.stream));
     if (ms_out) ms_out[i] = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   }
@@ -2288,7 +2307,7 @@ void rebuild(Scene* s, Device* d, ThreadDev& t, const rtg::BuildParams& shape) {
   std::lock_guard<std::mutex> bl(d->build_mtx);
   rtg::LearnedCost lc{};
   const rtg::BuildParams p = scene_params(s, lc, &shape);
-  rtg::build_bvh(s->descs, p, d->scratch, *s->bvh, t.stream);
+  build_scene_tree(s, d, s->descs, p, t.stream);
   RTG_HIP(hipStreamSynchronize(t.stream));
 }
 

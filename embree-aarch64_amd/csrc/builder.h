@@ -33,6 +33,7 @@ struct BuildParams {
 
 struct BuildStats {
   uint32_t num_prims = 0;      // triangle records: the valid triangles, more when spatial splits cut some
+  uint32_t num_split = 0;      // records added by spatial splits
   uint32_t num_nodes = 0;
   uint32_t max_depth = 0;
   uint32_t max_stack = 0;      // traversal stack entries the deepest path can need
@@ -79,12 +80,25 @@ struct DeviceBVH {
   bool empty(uint32_t slot = 0) const { return stats.num_nodes == 0 || slot >= slots.size() || slots[slot].num_prims == 0; }
   BuildStats stats;  // of the whole build
   uint32_t robust = 0;
+  // Exchanges the trees of two builds (a new generation for both).
+  void swap_tree(DeviceBVH& o) {
+    nodes.swap(o.nodes);
+    tris.swap(o.tris);
+    info.swap(o.info);
+    insts.swap(o.insts);
+    std::swap(instances, o.instances);
+    std::swap(slots, o.slots);
+    std::swap(stats, o.stats);
+    std::swap(robust, o.robust);
+    generation++;
+    o.generation++;
+  }
 };
 
 // Scratch reused across builds on one device.
 struct BuildScratch {
   DevBuf geoms, lo4, hi4, keys, vals, slo, shi, child, parent_int, parent_leaf, range, nlo, nhi, cost,
-      isleaf, split8, visit, queue, counters, sacc, depth, lvl_hist, lvl_order;
+      isleaf, split8, visit, queue, counters, sacc, depth, lvl_hist, lvl_order, leaf_src;
   RadixSorter sorter;
   SahScratch sah;
   SpatialScratch spatial;

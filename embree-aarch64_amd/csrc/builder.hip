@@ -575,20 +575,16 @@ struct CollapseArgs {
   const uint8_t* split8;
   const float4* slo;
   const float4* shi;
-  const uint32_t* vals;
-  const float4* lo4;
-  const float4* hi4;
-  const GeomDesc* geoms;
   QNode8* nodes;
-  TriRec* tris;
+  uint32_t* leaf_src; // position of each leaf record's primitive, bit 31: last of its leaf
   NodeInfo* info;
-  InstRec* insts;     // leaf records of an instance build (tris unused)
   BuildCounters* cnt;
   uint32_t n;         // capacity of nodes and of each queue
   uint32_t tri_cap;   // capacity of tris
   uint32_t width;     // children per node (2..8)
-  uint32_t robust;
 };
+
+constexpr uint32_t LEAF_SRC_LAST = 0x80000000u;  // positions are below 2^31 (REF_LEAF marks them likewise)
 
 __device__ inline bool is_inner(uint32_t ref, const uint8_t* isleaf) {
   return !(ref & REF_LEAF) && !isleaf[ref];
@@ -807,39 +803,11 @@ __device__ inline void collapse_item(const CollapseArgs& a, uint32_t t) {
         rank_i++;
       } else {
         q.meta[k] = (uint8_t)(0x80u | off_t);
-        // leaf triangles
+        // leaf records are written afterwards, one thread each (k_write_leaves)
         const uint32_t cntk = s_count[threadIdx.x][k];
         const uint32_t firstk = s_first[threadIdx.x][k];
-        for (uint32_t j = 0; j < cntk; j++) {
-          const uint32_t sj = firstk + j;
-          const uint32_t o = a.vals[sj];
-          const uint32_t gi = __float_as_uint(a.lo4[o].w);
-          const uint32_t prim = __float_as_uint(a.hi4[o].w);
-          const GeomDesc g = a.geoms[gi];
-          if (g.kind == GEOM_INSTANCES) {
-            InstRec ir = *(const InstRec*)(g.indices + (uint64_t)prim * g.istride);
-            ir.flags = (ir.flags & ~TRI_LAST) | ((j + 1 == cntk) ? TRI_LAST : 0u);
-            a.insts[tri_base + off_t + j] = ir;
-            continue;
-          }
-          const uint32_t* ix = (const uint32_t*)(g.indices + (uint64_t)prim * g.istride);
-          const float* p0 = (const float*)(g.vertices + (uint64_t)ix[0] * g.vstride);
-          const float* p1 = (const float*)(g.vertices + (uint64_t)ix[1] * g.vstride);
-          const float* p2 = (const float*)(g.vertices + (uint64_t)ix[2] * g.vstride);
-          TriRec tr;
-          tr.v0x = p0[0]; tr.v0y = p0[1]; tr.v0z = p0[2];
-          if (a.robust) {
-            tr.e1x = p1[0]; tr.e1y = p1[1]; tr.e1z = p1[2];
-            tr.e2x = p2[0]; tr.e2y = p2[1]; tr.e2z = p2[2];
-          } else {
-            tr.e1x = p0[0] - p1[0]; tr.e1y = p0[1] - p1[1]; tr.e1z = p0[2] - p1[2];
-            tr.e2x = p2[0] - p0[0]; tr.e2y = p2[1] - p0[1]; tr.e2z = p2[2] - p0[2];
-          }
-          tr.primID = prim;
-          tr.geomID = g.geomID;
-          tr.flags = (j + 1 == cntk) ? TRI_LAST : 0u;
-          a.tris[tri_base + off_t + j] = tr;
-        }
+        for (uint32_t j = 0; j < cntk; j++)
+          a.leaf_src[tri_base + off_t + j] = (firstk + j) | ((j + 1 == cntk) ? LEAF_SRC_LAST : 0u);
         off_t += cntk;
       }
     } else {
@@ -856,6 +824,48 @@ __device__ inline void collapse_item(const CollapseArgs& a, uint32_t t) {
   inf.count = ninner_prims + nt;
   inf.depth = item.depth;
   a.info[item.node] = inf;
+}
+
+// Leaf record r: the primitive at position leaf_src[r] (primref vals[pos],
+// whose .w fields hold its geometry and primitive index) as a TriRec
+// (v0, v0 - v1, v2 - v0; the three vertices for the robust test) or as the
+// instance's InstRec, with TRI_LAST on the last record of each leaf.
+__global__ __launch_bounds__(BT) void k_write_leaves(const uint32_t* __restrict__ leaf_src,
+                                                     const uint32_t* __restrict__ vals, const float4* __restrict__ lo4,
+                                                     const float4* __restrict__ hi4, const GeomDesc* __restrict__ geoms,
+                                                     const BuildCounters* cnt, TriRec* __restrict__ tris,
+                                                     InstRec* __restrict__ insts, uint32_t robust) {
+  const uint32_t r = blockIdx.x * BT + threadIdx.x;
+  if (cnt->error || r >= cnt->tri_count) return;
+  const uint32_t e = leaf_src[r];
+  const uint32_t o = vals[e & ~LEAF_SRC_LAST];
+  const uint32_t last = (e & LEAF_SRC_LAST) ? TRI_LAST : 0u;
+  const uint32_t gi = __float_as_uint(lo4[o].w);
+  const uint32_t prim = __float_as_uint(hi4[o].w);
+  const GeomDesc g = geoms[gi];
+  if (g.kind == GEOM_INSTANCES) {
+    InstRec ir = *(const InstRec*)(g.indices + (uint64_t)prim * g.istride);
+    ir.flags = (ir.flags & ~TRI_LAST) | last;
+    insts[r] = ir;
+    return;
+  }
+  const uint32_t* ix = (const uint32_t*)(g.indices + (uint64_t)prim * g.istride);
+  const float* p0 = (const float*)(g.vertices + (uint64_t)ix[0] * g.vstride);
+  const float* p1 = (const float*)(g.vertices + (uint64_t)ix[1] * g.vstride);
+  const float* p2 = (const float*)(g.vertices + (uint64_t)ix[2] * g.vstride);
+  TriRec tr;
+  tr.v0x = p0[0]; tr.v0y = p0[1]; tr.v0z = p0[2];
+  if (robust) {
+    tr.e1x = p1[0]; tr.e1y = p1[1]; tr.e1z = p1[2];
+    tr.e2x = p2[0]; tr.e2y = p2[1]; tr.e2z = p2[2];
+  } else {
+    tr.e1x = p0[0] - p1[0]; tr.e1y = p0[1] - p1[1]; tr.e1z = p0[2] - p1[2];
+    tr.e2x = p2[0] - p0[0]; tr.e2y = p2[1] - p0[1]; tr.e2z = p2[2] - p0[2];
+  }
+  tr.primID = prim;
+  tr.geomID = g.geomID;
+  tr.flags = last;
+  tris[r] = tr;
 }
 
 // One level of the collapse: the grid strides over the level's queue, whose
@@ -877,8 +887,11 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   out.generation++;
   out.stats = BuildStats();
   out.robust = prm.robust ? 1u : 0u;
-  uint32_t total = 0;
-  for (const auto& g : geoms) total += g.num_tris;
+  uint64_t total64 = 0;
+  for (const auto& g : geoms) total64 += g.num_tris;
+  // positions and primitive references carry a flag in bit 31
+  if (total64 >= REF_LEAF) throw hip_error(hipErrorInvalidValue, "a build holds fewer than 2^31 primitives");
+  const uint32_t total = (uint32_t)total64;
   out.slots.assign(nscenes, SceneSlot());
   if (user) *user = UserTree();
   if (geoms.empty() || total == 0) {
@@ -982,6 +995,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     const uint32_t nr = spatial_presplit(n, prm.split_budget, S.geoms.as<GeomDesc>(), out.stats.bounds_lo,
                                          out.stats.bounds_hi, S.vals, S.lo4, S.hi4, S.spatial, s);
     if (nr != n) {
+      out.stats.num_split = nr - n;
       n = nr;
       out.stats.num_prims = n;
       out.slots[0].num_prims = n;
@@ -1144,19 +1158,14 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   ca.split8 = S.split8.as<uint8_t>();
   ca.slo = S.slo.as<float4>();
   ca.shi = S.shi.as<float4>();
-  ca.vals = S.vals.as<uint32_t>();
-  ca.lo4 = S.lo4.as<float4>();
-  ca.hi4 = S.hi4.as<float4>();
-  ca.geoms = S.geoms.as<GeomDesc>();
   ca.nodes = out.nodes.as<QNode8>();
-  ca.tris = inst_build ? nullptr : out.tris.as<TriRec>();
-  ca.insts = inst_build ? out.insts.as<InstRec>() : nullptr;
+  S.leaf_src.reserve((size_t)n * 4);
+  ca.leaf_src = S.leaf_src.as<uint32_t>();
   ca.info = out.info.as<NodeInfo>();
   ca.cnt = cnt;
   ca.n = cap;
   ca.tri_cap = n;
   ca.width = width;
-  ca.robust = prm.robust ? 1u : 0u;
   // one launch per level of the 8-wide tree; the kernels read their queue
   // lengths on the device, so the host queues level k + 1 before it learns
   // (through a pinned slot) whether level k produced any items
@@ -1192,6 +1201,11 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
       throw hip_error(hipErrorUnknown, "BVH collapse queue overflow");
     }
   }
+  hipLaunchKernelGGL(k_write_leaves, dim3(div_up(n, BT)), dim3(BT), 0, s, S.leaf_src.as<uint32_t>(),
+                     S.vals.as<uint32_t>(), S.lo4.as<float4>(), S.hi4.as<float4>(), S.geoms.as<GeomDesc>(), cnt,
+                     inst_build ? nullptr : out.tris.as<TriRec>(), inst_build ? out.insts.as<InstRec>() : nullptr,
+                     prm.robust ? 1u : 0u);
+  RTG_HIP(hipGetLastError());
   RTG_HIP(hipMemcpyAsync(&hc, cnt, sizeof(hc), hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));
   if (!trees.empty() && (hc.roots_seen != roots_expected || (hc.error & 5u)))

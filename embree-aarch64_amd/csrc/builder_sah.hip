@@ -17,21 +17,20 @@
 //
 // Segments (contiguous position ranges still to split) are split by size
 // class; the class fixes how a segment is split, whatever kernel does it:
-//   big   (> 4096): positions in chunks of 4096 bin into LDS, the chunk bins
+//   big   (> 4096): positions in chunks of 2048 bin into LDS, the chunk bins
 //                   merge into per-segment bins in HBM with atomics, one
 //                   workgroup per segment picks the split, and a global scan
 //                   of left flags gives every position its stable destination;
-//                   one level of big segments per step of the level loop;
-//   mid   (65..4096): a 256-lane group bins, picks and partitions in LDS;
-//   small (17..64): a 64-lane group (one wave), same code;
+//   mid   (513..4096): a 256-lane workgroup bins, picks and partitions in LDS;
+//   small (17..512): one wave, same code;
 //   tiny  (2..16): exact SAH splits over the references sorted by centroid
 //                  on each axis, the whole subtree at once; a tree's own root
-//                  of 2..64 references is split as small.
-// Every segment of at most SUB_MAX references is finished by one workgroup
-// of k_subtree (mid segments by the whole workgroup, small and tiny ones by
-// single waves); the level loop runs only while larger segments remain, and
-// its kernels read their segment counts on the device, so the host queues a
-// level before it learns whether the previous one left work.
+//                  of 2..512 references is split as small.
+// Big, mid and small segments are split one level per step of the level
+// loop, whose kernels read their segment counts on the device, so the host
+// queues a level before it learns whether the previous one left work; tiny
+// segments collect in a list that k_tiny finishes after the loop, four per
+// wave.
 // Every inner node gets the id of its split position (first + n_left - 1),
 // which is unique, so the output does not depend on the order in which
 // segments are processed.
@@ -48,11 +47,11 @@ namespace {
 constexpr uint32_t NIL = 0xffffffffu;
 constexpr int NB = 32;                  // bins per axis
 constexpr int BIN_WORDS = 3 * NB * 7;   // counts, then lo xyz, then hi xyz, per (axis, bin)
-constexpr uint32_t SMALL_MAX = 64;
+constexpr uint32_t MEDIAN_MAX = 64;      // largest median_below honoured
 constexpr uint32_t TINY_MAX = 16;        // segments up to this size are finished by one thread (exact SAH)
 constexpr uint32_t MID_MAX = 4096;
-constexpr uint32_t SUB_MAX = 64;        // segments up to this size are finished by one wave (k_subtree)
-constexpr uint32_t CHUNK = 4096;        // positions per block of the big-segment binning
+constexpr uint32_t SMALL_MAX = 512;     // segments up to this size are split by one wave
+constexpr uint32_t CHUNK = 2048;        // positions per block of the big-segment binning
 constexpr uint32_t MEDIAN_DEPTH = 48;   // from this binary depth on, splits are object medians
 constexpr int TBS = 256;
 
@@ -76,29 +75,27 @@ struct Split {
   uint32_t lsid, rsid;  // ids of the children in the next big list (NIL if not big)
 };
 
-// A segment of at most SUB_MAX references whose whole subtree one
-// workgroup finishes, with the parity of the buffers holding its references.
-struct WSeg {
+// A tiny segment waiting for k_tiny, with the parity of the buffers
+// holding its references.
+struct TinyItem {
   Seg s;
   uint32_t par;
-  uint32_t flags;  // WSEG_BINNED: split by bins even when tiny (a tree's own root)
 };
-constexpr uint32_t WSEG_BINNED = 1u;
 
 struct Counters {
-  uint32_t nseg[2][2];  // segment counts by list parity: big, mid (a level reads one parity and fills the other)
-  uint32_t nroot;       // subtree segments so far (all levels)
+  uint32_t nseg[2][3];  // segment counts by list parity: big, mid, small (a level reads one parity and fills the other)
+  uint32_t ntiny;       // tiny segments so far (all levels)
   uint32_t error;    // bit 0: a segment list overflowed, bit 1: bin counts disagree, bit 2: destination out of range,
                      // bit 3: median subtree stack overflow, bit 4: a tiny segment out of range,
                      // bit 5: a depth beyond 254
 };
 
 struct Lists {
-  Seg* seg[2];       // next level's big and mid segments
+  Seg* seg[3];       // next level's big, mid and small segments
   uint32_t* nseg;    // and their counts
-  uint32_t cap[2];
-  WSeg* roots;       // subtree segments
-  uint32_t root_cap;
+  uint32_t cap[3];
+  TinyItem* tiny;    // tiny segments
+  uint32_t tiny_cap;
   uint32_t par;      // parity of the buffers this level writes
   uint32_t fin_max;  // segments up to this size are finished at once by median splits
 };
@@ -231,21 +228,35 @@ __device__ inline void finish_median(const Tree& t, uint32_t m, uint32_t side, u
   }
 }
 
-// Child segment [first, first+count) of inner node m: a leaf link for a
-// single primitive, a finished median subtree up to nx.fin_max primitives,
-// else a new segment in the list of its size class. Returns the id in the
-// big list, or NIL.
-__device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m, uint32_t side, uint32_t depth,
-                                      uint32_t first, uint32_t count, const float clo[3], const float chi[3]) {
+// Child segment [first, first+count) of inner node m at depth+1: a leaf
+// link for a single primitive and a finished median subtree up to
+// nx.fin_max primitives are written here (result -1); any other child
+// goes to the list of its size class, which is returned (0 big, 1 mid,
+// 2 small, 3 tiny).
+constexpr int LIST_TINY = 3;
+
+__device__ inline int child_list(const Tree& t, const Lists& nx, uint32_t m, uint32_t side, uint32_t depth,
+                                 uint32_t first, uint32_t count) {
   if (count == 1) {
     t.child[2 * m + side] = first | REF_LEAF;
     t.parent_leaf[first] = m;
-    return NIL;
+    return -1;
   }
   if (count <= nx.fin_max) {
     finish_median(t, m, side, depth, first, count);
-    return NIL;
+    return -1;
   }
+  return count <= TINY_MAX ? LIST_TINY : (count > MID_MAX ? 0 : (count > SMALL_MAX ? 1 : 2));
+}
+
+__device__ inline uint32_t* list_counter(const Tree& t, const Lists& nx, int cls) {
+  return cls == LIST_TINY ? &t.cnt->ntiny : &nx.nseg[cls];
+}
+
+// Writes the child into slot id of list cls (error bit 0 past its capacity).
+__device__ inline void put_child(const Tree& t, const Lists& nx, int cls, uint32_t id, uint32_t m, uint32_t side,
+                                 uint32_t depth, uint32_t first, uint32_t count, const float clo[3],
+                                 const float chi[3]) {
   Seg c;
   c.first = first;
   c.count = count;
@@ -256,23 +267,23 @@ __device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m
     c.clo[k] = clo[k];
     c.chi[k] = chi[k];
   }
-  if (count <= SUB_MAX) {
-    const uint32_t id = atomicAdd(&t.cnt->nroot, 1u);
-    if (id >= nx.root_cap) {
-      atomicOr(&t.cnt->error, 1u);
-      return NIL;
-    }
-    nx.roots[id] = WSeg{c, nx.par, 0u};
-    return NIL;
-  }
-  const int cls = count > MID_MAX ? 0 : 1;
-  const uint32_t id = atomicAdd(&nx.nseg[cls], 1u);
-  if (id >= nx.cap[cls]) {
+  if (id >= (cls == LIST_TINY ? nx.tiny_cap : nx.cap[cls])) {
     atomicOr(&t.cnt->error, 1u);
-    return NIL;
+    return;
   }
-  nx.seg[cls][id] = c;
-  return cls == 0 ? id : NIL;
+  if (cls == LIST_TINY) nx.tiny[id] = TinyItem{c, nx.par};
+  else nx.seg[cls][id] = c;
+}
+
+// child_list and put_child with a slot of its own; returns the id in the
+// big list, or NIL.
+__device__ inline uint32_t emit_child(const Tree& t, const Lists& nx, uint32_t m, uint32_t side, uint32_t depth,
+                                      uint32_t first, uint32_t count, const float clo[3], const float chi[3]) {
+  const int cls = child_list(t, nx, m, side, depth, first, count);
+  if (cls < 0) return NIL;
+  const uint32_t id = atomicAdd(list_counter(t, nx, cls), 1u);
+  put_child(t, nx, cls, id, m, side, depth, first, count, clo, chi);
+  return cls == 0 && id < nx.cap[0] ? id : NIL;
 }
 
 // Synchronisation of the threads that share a segment's LDS: the whole
@@ -292,9 +303,6 @@ __device__ inline void group_sync() {
   }
 }
 
-// One wave's global stores visible to its own later loads (the wave's
-// references written by a split, read back by the finisher).
-__device__ inline void wave_sync_global() { __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup"); }
 
 // Plain binned-SAH sweep of axis a by one half-wave (32 lanes; lane b holds
 // bin b): the least A_L*n_L + A_R*n_R over the "bin < k" candidates, first
@@ -554,7 +562,7 @@ __global__ void k_init_gbins(uint32_t* g, const uint32_t* nseg) {
 }
 
 __global__ void k_zero_counts(uint32_t* nseg) {
-  if (threadIdx.x < 2) nseg[threadIdx.x] = 0u;
+  if (threadIdx.x < 3) nseg[threadIdx.x] = 0u;
 }
 
 // A chunk of CHUNK positions meets at most two big segments (each is longer
@@ -599,13 +607,80 @@ __global__ __launch_bounds__(TBS) void k_big_bins(const uint32_t* __restrict__ s
   if (tid == 0) sm[0] = make_mapping(segs[slo_id]);
   if (tid == 1) sm[1] = make_mapping(segs[shi_id]);
   __syncthreads();
-  for (uint32_t i = c0 + tid; i < c1; i += TBS) {
-    const uint32_t s = sid[i];
+  // each thread bins a run of RUN consecutive positions; neighbours in the
+  // Morton order mostly share their bins, so a thread sums each axis's run
+  // of equal bins in registers and adds the sum to the bins when the bin
+  // (or the segment) changes: few LDS atomics, on few shared addresses
+  constexpr int RUN = (int)(CHUNK / TBS);
+  const uint32_t r0 = c0 + (uint32_t)tid * RUN;
+  uint32_t rs[RUN];
+  float4 rl[RUN], rh[RUN];
+#pragma unroll
+  for (int j = 0; j < RUN; j++) {
+    const uint32_t i = r0 + j;
+    rs[j] = i < c1 ? sid[i] : NIL;
+    if (rs[j] != NIL) {
+      rl[j] = blo[i];
+      rh[j] = bhi[i];
+    }
+  }
+  int cslot = -1;
+  int cb[3] = {-1, -1, -1};
+  uint32_t cn[3] = {0u, 0u, 0u}, ql[3][3], qh[3][3];
+  auto flush = [&](int a) {
+    if (cn[a] == 0u) return;
+    uint32_t* w = sb[cslot][wave];
+    atomicAdd(&w[wc(a, cb[a])], cn[a]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      atomicMin(&w[wl(a, cb[a], k)], ql[a][k]);
+      atomicMax(&w[wh(a, cb[a], k)], qh[a][k]);
+    }
+    cn[a] = 0u;
+  };
+#pragma unroll
+  for (int j = 0; j < RUN; j++) {
+    const uint32_t s = rs[j];
     if (s == NIL) continue;
-    const float4 l = blo[i], h = bhi[i];
-    if (s == slo_id) bin_prim(sb[0][wave], sm[0], l, h);
-    else if (s == shi_id) bin_prim(sb[1][wave], sm[1], l, h);
-    else bin_prim(gbins + (size_t)s * BIN_WORDS, make_mapping(segs[s]), l, h);
+    const float4 l = rl[j], h = rh[j];
+    const int slot = s == slo_id ? 0 : (s == shi_id ? 1 : 2);
+    if (slot == 2) {
+      bin_prim(gbins + (size_t)s * BIN_WORDS, make_mapping(segs[s]), l, h);
+      continue;
+    }
+    if (slot != cslot) {
+#pragma unroll
+      for (int a = 0; a < 3; a++) flush(a);
+      cslot = slot;
+    }
+    const Mapping& m = sm[slot];
+    const float c[3] = {l.x + h.x, l.y + h.y, l.z + h.z};
+    const uint32_t ol[3] = {f2o(l.x), f2o(l.y), f2o(l.z)};
+    const uint32_t oh[3] = {f2o(h.x), f2o(h.y), f2o(h.z)};
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+      const int b = bin_of(c[a], m.ofs[a], m.scale[a], m.nb);
+      if (b != cb[a] || cn[a] == 0u) {
+        flush(a);
+        cb[a] = b;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          ql[a][k] = ol[k];
+          qh[a][k] = oh[k];
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          ql[a][k] = ol[k] < ql[a][k] ? ol[k] : ql[a][k];
+          qh[a][k] = oh[k] > qh[a][k] ? oh[k] : qh[a][k];
+        }
+      }
+      cn[a]++;
+    }
+  }
+  if (cslot >= 0) {
+#pragma unroll
+    for (int a = 0; a < 3; a++) flush(a);
   }
   __syncthreads();
   for (int slot = 0; slot < 2; slot++) {
@@ -728,7 +803,9 @@ struct SegLds {
 };
 
 // Bins, splits and stably partitions segment sg with the BLOCK threads of a
-// group (tid 0..BLOCK-1, PER positions each): references read from
+// group (tid 0..BLOCK-1, each a run of per = ceil(count / BLOCK) <= PER
+// consecutive positions, so that small segments spread over the group's
+// lanes): references read from
 // (idx, blo, bhi), written to (idx_o, blo_o, bhi_o) and their ids to
 // t.vals at the same positions; links the inner node (the split position)
 // into the tree. Every thread returns the split and, in lcl..rch, the
@@ -746,9 +823,10 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
   if (tid < 12) S.cb[tid] = (tid % 6) < 3 ? NIL : 0u;
   group_sync<WAVE>();
   const uint32_t end = sg.first + sg.count;
-  const uint32_t p0 = sg.first + (uint32_t)tid * PER;
+  const int per = PER == 1 ? 1 : (int)((sg.count + BLOCK - 1) / BLOCK);
+  const uint32_t p0 = sg.first + (uint32_t)tid * (uint32_t)per;
 #pragma unroll 4
-  for (int j = 0; j < PER; j++) {
+  for (int j = 0; j < per; j++) {
     const uint32_t p = p0 + j;
     if (p < end) bin_prim(S.sb, m, blo[p], bhi[p]);
   }
@@ -757,7 +835,7 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
   // flags of this thread's positions, and the stable destinations
   uint32_t mask = 0u, nl = 0u;
 #pragma unroll 4
-  for (int j = 0; j < PER; j++) {
+  for (int j = 0; j < per; j++) {
     const uint32_t p = p0 + j;
     if (p < end && goes_left(sp, blo[p], bhi[p], p - sg.first)) {
       mask |= 1u << j;
@@ -774,11 +852,12 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
   group_sync<WAVE>();
   uint32_t lbase = x - nl;
   for (int w = 0; w < wave; w++) lbase += S.wsum[w];
-  float bnd[12];
+  // centroid bounds of the left (lb) and right (rb) references: lo3, hi3
+  float lb[6], rb[6];
 #pragma unroll
-  for (int k = 0; k < 12; k++) bnd[k] = (k % 6) < 3 ? INFINITY : -INFINITY;
+  for (int k = 0; k < 6; k++) lb[k] = rb[k] = k < 3 ? INFINITY : -INFINITY;
   uint32_t lr = 0u;
-  for (int j = 0; j < PER; j++) {
+  for (int j = 0; j < per; j++) {
     const uint32_t p = p0 + j;
     if (p >= end) break;
     const bool left = (mask >> j) & 1u;
@@ -796,16 +875,23 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
     bhi_o[dst] = h;
     t.vals[dst] = v;
     const float c[3] = {l.x + h.x, l.y + h.y, l.z + h.z};
-    const int o = left ? 0 : 6;
+    if (left) {
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      bnd[o + k] = fminf(bnd[o + k], c[k]);
-      bnd[o + 3 + k] = fmaxf(bnd[o + 3 + k], c[k]);
+      for (int k = 0; k < 3; k++) {
+        lb[k] = fminf(lb[k], c[k]);
+        lb[3 + k] = fmaxf(lb[3 + k], c[k]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        rb[k] = fminf(rb[k], c[k]);
+        rb[3 + k] = fmaxf(rb[3 + k], c[k]);
+      }
     }
   }
 #pragma unroll
   for (int k = 0; k < 12; k++) {
-    float v = bnd[k];
+    float v = k < 6 ? lb[k] : rb[k - 6];
     const bool mx = (k % 6) >= 3;
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) {
@@ -828,7 +914,7 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
   return sp;
 }
 
-// One workgroup per mid segment of a level (between SUB_MAX and MID_MAX
+// One workgroup per mid segment of a level (between SMALL_MAX and MID_MAX
 // references).
 template <bool LEARNED>
 __global__ __launch_bounds__(256) void k_seg(const Seg* __restrict__ segs, const uint32_t* __restrict__ nseg,
@@ -849,12 +935,66 @@ __global__ __launch_bounds__(256) void k_seg(const Seg* __restrict__ segs, const
   }
 }
 
-// ---------------------------------------------------------------- subtrees --
+// One wave per small segment of a level (up to SMALL_MAX references, or a
+// tree's root of any size up to that), SEGW_WAVES per workgroup; the grid
+// covers the list's capacity and the count is read on the device. The
+// workgroup reserves its children's list slots with one global atomic per
+// list: the level's segments number in the tens of thousands, and an atomic
+// per child on one address would serialise them. (A grid-stride loop over
+// the count doubles the kernel's registers.)
+constexpr int SEGW_TB = 512;
+constexpr int SEGW_WAVES = SEGW_TB / 64;
+constexpr int SEGW_PER = (int)(SMALL_MAX / 64);
 
-constexpr int SUB_TB = 256;
-constexpr int SUB_WAVES = SUB_TB / 64;
-constexpr int SMALL_STACK = 4;   // pending small segments of a wave: disjoint, > TINY_MAX each, <= SMALL_MAX in all
-static_assert(SUB_MAX <= SMALL_MAX, "a subtree segment is finished by one wave");
+template <bool LEARNED>
+__global__ __launch_bounds__(SEGW_TB) void k_seg_wave(const Seg* __restrict__ segs, const uint32_t* __restrict__ nseg,
+                                                      const uint32_t* __restrict__ idx, const float4* __restrict__ blo,
+                                                      const float4* __restrict__ bhi, uint32_t* __restrict__ idx_o,
+                                                      float4* __restrict__ blo_o, float4* __restrict__ bhi_o, Lists nx,
+                                                      Tree t, LearnedCost lc) {
+  __shared__ SegLds<64> S[SEGW_WAVES];
+  __shared__ uint32_t lcount[4], lbase[4];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t i = blockIdx.x * SEGW_WAVES + (uint32_t)wave;
+  if (threadIdx.x < 4) lcount[threadIdx.x] = 0u;
+  __syncthreads();
+  const bool active = i < *nseg && i < nx.cap[2];  // both parities' lists have this capacity
+  Seg sg;
+  Split sp;
+  float cb[2][6];  // children's centroid bounds: lo3, hi3
+  int cls[2] = {-1, -1};
+  uint32_t slot[2] = {0u, 0u};
+  if (active) {
+    sg = segs[i];
+    sp = split_segment<64, SEGW_PER, true, LEARNED>(sg, idx, blo, bhi, idx_o, blo_o, bhi_o, t, lc, S[wave], lane,
+                                                    &cb[0][0], &cb[0][3], &cb[1][0], &cb[1][3]);
+    if (lane == 0) {
+      const uint32_t mid = sg.first + sp.nleft - 1, depth = sg.meta >> 1;
+      cls[0] = child_list(t, nx, mid, 0, depth, sg.first, sp.nleft);
+      cls[1] = child_list(t, nx, mid, 1, depth, sg.first + sp.nleft, sg.count - sp.nleft);
+#pragma unroll
+      for (int c = 0; c < 2; c++)
+        if (cls[c] >= 0) slot[c] = atomicAdd(&lcount[cls[c]], 1u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 && lcount[threadIdx.x])
+    lbase[threadIdx.x] = atomicAdd(list_counter(t, nx, (int)threadIdx.x), lcount[threadIdx.x]);
+  __syncthreads();
+  if (active && lane == 0) {
+    const uint32_t mid = sg.first + sp.nleft - 1, depth = sg.meta >> 1;
+#pragma unroll
+    for (int c = 0; c < 2; c++)
+      if (cls[c] >= 0)
+        put_child(t, nx, cls[c], lbase[cls[c]] + slot[c], mid, (uint32_t)c, depth, c ? sg.first + sp.nleft : sg.first,
+                  c ? sg.count - sp.nleft : sp.nleft, &cb[c][0], &cb[c][3]);
+  }
+}
+
+// ------------------------------------------------------------ tiny segments --
+
+static_assert(SMALL_MAX <= 64 * 32, "a small segment is split by one wave of at most 32 positions per lane");
+static_assert(MID_MAX <= 256 * 16, "a mid segment is split by 256 threads of at most 16 positions each");
 
 struct Bufs {
   uint32_t* idx[2];
@@ -868,51 +1008,57 @@ struct TinyLds {
   uint32_t id[TINY_MAX];
 };
 
-struct WaveLds {
-  SegLds<64> seg;
-  TinyLds tiny;
-  WSeg stack[SMALL_STACK];
-};
-
+constexpr int TINY_GROUPS = 64 / (int)TINY_MAX;
+static_assert(TINY_GROUPS * TINY_MAX == 64, "tiny segments take 16-lane groups of a wave");
 
 __device__ inline bool tiny_better(float c, int a, uint32_t k, float bc, int ba, uint32_t bk) {
   return c < bc || (c == bc && (a < ba || (a == ba && k < bk)));
 }
 
-// Whole subtree of a tiny segment sg (2..TINY_MAX references in buffer
-// parity p) with exact SAH splits, by the lanes 0..count-1 of one wave (lane
-// j holds the reference at position j of the segment). The decisions are
-// those of a sequential finisher: at each node the references are stably
-// sorted by centroid (lo + hi) on x, then on y, then on z, each sort
-// starting from the order the previous one left; every split position of
-// each of the three orders is priced A_L*n_L + A_R*n_R; the first minimum
-// over (axis, position) wins; and the node's references take the order of a
-// stable sort on the winning axis applied to the z order (for z, the z order
+// Whole subtrees of up to TINY_GROUPS tiny segments (2..TINY_MAX
+// references each), one per 16-lane group of a wave: group g finishes
+// q[g] (g < nq) with exact SAH splits, lane j of the group holding the
+// reference at position j of its segment. The decisions are those of a
+// sequential finisher: at each node the references are stably sorted by
+// centroid (lo + hi) on x, then on y, then on z, each sort starting from
+// the order the previous one left; every split position of each of the
+// three orders is priced A_L*n_L + A_R*n_R; the first minimum over (axis,
+// position) wins; and the node's references take the order of a stable
+// sort on the winning axis applied to the z order (for z, the z order
 // itself). A stable sort here is a rank: the number of the node's
 // references with a smaller key tuple, (k_x, position) for the x order,
 // (k_y, k_x, position) for y, (k_z, k_y, k_x, position) for z. All nodes of
 // one depth are split together. vals receives the final order.
-__device__ inline void tiny_wave(const Seg& sg, uint32_t p, const Bufs& B, const Tree& t, TinyLds& T, int lane) {
-  const uint32_t c = sg.count;
-  if (c < 2 || c > TINY_MAX) {
-    if (lane == 0) atomicOr(&t.cnt->error, 16u);
-    return;
+__device__ inline void tiny_groups(const TinyItem* q, int nq, const Bufs& B, const Tree& t, TinyLds* TG, int lane) {
+  const int g = lane >> 4;
+  const uint32_t pos = (uint32_t)lane & (TINY_MAX - 1u);
+  TinyLds& T = TG[g];
+  uint32_t first = 0u, c = 0u, np = 0u, nm = 0u, par = 0u;
+  if (g < nq) {
+    first = q[g].s.first;
+    c = q[g].s.count;
+    np = q[g].s.parent;
+    nm = q[g].s.meta;
+    par = q[g].par;
+    if (c < 2u || c > TINY_MAX) {
+      if (pos == 0u) atomicOr(&t.cnt->error, 16u);
+      c = 0u;
+    }
   }
-  const uint32_t pos = (uint32_t)lane;
   const bool act = pos < c;
   float4 l = make_float4(0.0f, 0.0f, 0.0f, 0.0f), h = l;
   uint32_t id = 0u;
   if (act) {
-    l = B.blo[p][sg.first + pos];
-    h = B.bhi[p][sg.first + pos];
-    id = B.idx[p][sg.first + pos];
+    l = B.blo[par][first + pos];
+    h = B.bhi[par][first + pos];
+    id = B.idx[par][first + pos];
   }
   // the node holding this lane's position: [nf, nf + nc) of the segment
-  uint32_t nf = 0u, nc = act ? c : 0u, np = sg.parent, nm = sg.meta;
+  uint32_t nf = 0u, nc = act ? c : 0u;
   for (uint32_t it = 0;; it++) {
     if (nc == 1u) {
-      t.child[2 * np + (nm & 1u)] = (sg.first + nf) | REF_LEAF;
-      t.parent_leaf[sg.first + nf] = np;
+      t.child[2 * np + (nm & 1u)] = (first + nf) | REF_LEAF;
+      t.parent_leaf[first + nf] = np;
       nc = 0u;
     }
     const bool busy = nc >= 2u;
@@ -922,11 +1068,9 @@ __device__ inline void tiny_wave(const Seg& sg, uint32_t p, const Bufs& B, const
       break;
     }
     const float k0 = l.x + h.x, k1 = l.y + h.y, k2 = l.z + h.z;
-    if (pos < TINY_MAX) {
-      T.k[0][pos] = k0;
-      T.k[1][pos] = k1;
-      T.k[2][pos] = k2;
-    }
+    T.k[0][pos] = k0;
+    T.k[1][pos] = k1;
+    T.k[2][pos] = k2;
     group_sync<true>();
     uint32_t r0 = 0u, r1 = 0u, r2 = 0u, f0 = 0u, f1 = 0u;
     if (busy) {
@@ -1016,14 +1160,14 @@ __device__ inline void tiny_wave(const Seg& sg, uint32_t p, const Bufs& B, const
         bk = ok;
       }
     }
-    ba = __shfl(ba, (int)(nf & (TINY_MAX - 1u)), TINY_MAX);
-    bk = __shfl(bk, (int)(nf & (TINY_MAX - 1u)), TINY_MAX);
+    ba = __shfl(ba, (int)nf, TINY_MAX);
+    bk = __shfl(bk, (int)nf, TINY_MAX);
     const uint32_t rf = ba == 0 ? f0 : (ba == 1 ? f1 : r2);
-    const uint32_t mid = sg.first + nf + bk - 1u;
+    const uint32_t mid = first + nf + bk - 1u;
     if (busy && pos == nf) {
       t.child[2 * np + (nm & 1u)] = mid;
       t.parent_int[mid] = np;
-      t.range[mid] = make_uint2(sg.first + nf, sg.first + nf + nc - 1u);
+      t.range[mid] = make_uint2(first + nf, first + nf + nc - 1u);
       set_depth(t, mid, nm >> 1);
     }
     group_sync<true>();
@@ -1049,78 +1193,25 @@ __device__ inline void tiny_wave(const Seg& sg, uint32_t p, const Bufs& B, const
       np = mid;
     }
   }
-  if (act) t.vals[sg.first + pos] = id;
+  if (act) t.vals[first + pos] = id;
   group_sync<true>();
 }
 
-// Whole subtree of a segment of at most SMALL_MAX references by one wave:
-// binned splits (as one 64-lane workgroup per segment would make them) down
-// to tiny segments, which tiny_wave finishes.
-template <bool LEARNED>
-__device__ inline void small_tree(const WSeg& item, const Bufs& B, const Tree& t, const LearnedCost& lc,
-                                  uint32_t fin_max, WaveLds& W, int lane) {
-  if (item.s.count <= TINY_MAX && !(item.flags & WSEG_BINNED)) {
-    tiny_wave(item.s, item.par, B, t, W.tiny, lane);
-    return;
-  }
-  int sp = 0;
-  WSeg cur = item;
-  for (;;) {
-    const uint32_t p = cur.par, q = p ^ 1u;
-    float bl[2][3], bh[2][3];
-    const Split spl = split_segment<64, 1, true, LEARNED>(cur.s, B.idx[p], B.blo[p], B.bhi[p], B.idx[q], B.blo[q], B.bhi[q], t,
-                                                 lc, W.seg, lane, bl[0], bh[0], bl[1], bh[1]);
-    wave_sync_global();
-    const uint32_t mid = cur.s.first + spl.nleft - 1u, depth = cur.s.meta >> 1;
-    for (int side = 0; side < 2; side++) {
-      const uint32_t first = side ? cur.s.first + spl.nleft : cur.s.first;
-      const uint32_t count = side ? cur.s.count - spl.nleft : spl.nleft;
-      if (count == 1u) {
-        if (lane == 0) {
-          t.child[2 * mid + side] = first | REF_LEAF;
-          t.parent_leaf[first] = mid;
-        }
-        continue;
-      }
-      if (count <= fin_max) {
-        if (lane == 0) finish_median(t, mid, (uint32_t)side, depth, first, count);
-        continue;
-      }
-      Seg c;
-      c.first = first;
-      c.count = count;
-      c.parent = mid;
-      c.meta = ((depth + 1u) << 1) | (uint32_t)side;
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        c.clo[k] = bl[side][k];
-        c.chi[k] = bh[side][k];
-      }
-      if (count <= TINY_MAX) {
-        tiny_wave(c, q, B, t, W.tiny, lane);
-      } else if (sp < SMALL_STACK) {
-        if (lane == 0) W.stack[sp] = WSeg{c, q, 0u};
-        sp++;
-      } else if (lane == 0) {
-        atomicOr(&t.cnt->error, 1u);
-      }
-    }
-    group_sync<true>();
-    if (sp == 0) break;
-    cur = W.stack[--sp];
-  }
-}
+// Whole subtrees of the tiny segments, TINY_GROUPS per wave.
+constexpr int TINY_TB = 256;
+constexpr int TINY_PER_BLOCK = TINY_GROUPS * (TINY_TB / 64);
 
-// One wave per subtree segment (at most SUB_MAX references), finished by
-// small_tree; four segments per workgroup.
-template <bool LEARNED>
-__global__ __launch_bounds__(SUB_TB) void k_subtree(const WSeg* __restrict__ roots, uint32_t nroots, Bufs B, Tree t,
-                                                    LearnedCost lc, uint32_t fin_max) {
-  __shared__ WaveLds W[SUB_WAVES];
+__global__ __launch_bounds__(TINY_TB) void k_tiny(const TinyItem* __restrict__ items, uint32_t nitems, Bufs B,
+                                                  Tree t) {
+  __shared__ TinyLds T[TINY_TB / 64][TINY_GROUPS];
+  __shared__ TinyItem Q[TINY_TB / 64][TINY_GROUPS];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const uint32_t r = blockIdx.x * SUB_WAVES + (uint32_t)wave;
-  if (r >= nroots) return;
-  small_tree<LEARNED>(roots[r], B, t, lc, fin_max, W[wave], lane);
+  const uint32_t first = blockIdx.x * TINY_PER_BLOCK + (uint32_t)wave * TINY_GROUPS;
+  if (first >= nitems) return;
+  const int nq = (int)min((uint32_t)TINY_GROUPS, nitems - first);
+  if (lane < nq) Q[wave][lane] = items[first + lane];
+  group_sync<true>();
+  tiny_groups(Q[wave], nq, B, t, T[wave], lane);
 }
 
 // Writes the big-segment id of every position of the initial big segments
@@ -1137,23 +1228,26 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
                     const std::vector<SahRoot>& trees, uint32_t* child, uint32_t* parent_int,
                     uint32_t* parent_leaf, uint2* range, uint8_t* depth, const LearnedCost* learned,
                     uint32_t fin_max, SahScratch& S, SahTreeOut& out, hipStream_t s) {
-  if (fin_max > SMALL_MAX) fin_max = SMALL_MAX;
+  if (fin_max > MEDIAN_MAX) fin_max = MEDIAN_MAX;
   LearnedCost lcost{};
   if (learned) lcost = *learned;
   if (trees.empty()) throw hip_error(hipErrorInvalidValue, "SAH build without a tree");
   if (trees.size() >= ROOT_TAG) throw hip_error(hipErrorInvalidValue, "too many trees");
   for (const SahRoot& r : trees)
     if (r.count < 2 || (uint64_t)r.first + r.count > n) throw hip_error(hipErrorInvalidValue, "bad SAH tree range");
-  const uint32_t cap[2] = {n / MID_MAX + 2, n / (SUB_MAX + 1) + 2};
-  const uint32_t root_cap = n / 2 + 2;  // subtree segments are disjoint and hold two or more references
+  // segments of one list are disjoint and larger than the class below,
+  // except trees' roots, which may be small at any size
+  const uint32_t ntrees = (uint32_t)trees.size();
+  const uint32_t cap[3] = {n / MID_MAX + 2, n / (SMALL_MAX + 1) + 2, n / (TINY_MAX + 1) + 2 + ntrees};
+  const uint32_t tiny_cap = n / 2 + 2;  // tiny segments are disjoint and hold two or more references
   for (int p = 0; p < 2; p++) {
     S.idx[p].reserve((size_t)n * 4);
     S.blo[p].reserve((size_t)n * 16);
     S.bhi[p].reserve((size_t)n * 16);
     S.sid[p].reserve((size_t)n * 4);
-    for (int c = 0; c < 2; c++) S.segs[c][p].reserve((size_t)cap[c] * sizeof(Seg));
+    for (int c = 0; c < 3; c++) S.segs[c][p].reserve((size_t)cap[c] * sizeof(Seg));
   }
-  S.subroots.reserve((size_t)root_cap * sizeof(WSeg));
+  S.tiny.reserve((size_t)tiny_cap * sizeof(TinyItem));
   S.bins.reserve((size_t)cap[0] * BIN_WORDS * 4);
   S.splits.reserve((size_t)cap[0] * sizeof(Split));
   S.flags.reserve((size_t)n * 4);
@@ -1171,11 +1265,9 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
                      trees.size(), NIL);
   RTG_HIP(hipMemsetAsync(depth, 0xff, (size_t)(n - 1), s));
 
-  // trees above SUB_MAX references start as big or mid segments of the
-  // level loop, the others as subtree segments (split by bins even when
-  // tiny, as a tree's root always is)
-  std::vector<Seg> init[2];
-  std::vector<WSeg> init_sub;
+  // trees start as big, mid or small segments of the level loop (a tree's
+  // root is split by bins even when tiny)
+  std::vector<Seg> init[3];
   for (size_t i = 0; i < trees.size(); i++) {
     const SahRoot& r = trees[i];
     Seg sg;
@@ -1187,17 +1279,14 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
       sg.clo[k] = r.clo[k];
       sg.chi[k] = r.chi[k];
     }
-    if (r.count > SUB_MAX) init[r.count > MID_MAX ? 0 : 1].push_back(sg);
-    else init_sub.push_back(WSeg{sg, 0u, WSEG_BINNED});
+    init[r.count > MID_MAX ? 0 : (r.count > SMALL_MAX ? 1 : 2)].push_back(sg);
   }
-  uint32_t cur[2];
-  for (int c = 0; c < 2; c++) {
+  uint32_t cur[3];
+  for (int c = 0; c < 3; c++) {
     cur[c] = (uint32_t)init[c].size();
     if (cur[c])
       RTG_HIP(hipMemcpyAsync(S.segs[c][0].ptr, init[c].data(), init[c].size() * sizeof(Seg), hipMemcpyHostToDevice, s));
   }
-  if (!init_sub.empty())
-    RTG_HIP(hipMemcpyAsync(S.subroots.ptr, init_sub.data(), init_sub.size() * sizeof(WSeg), hipMemcpyHostToDevice, s));
   if (cur[0]) {  // the big-segment kernels read every position's big-segment id
     hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[0].as<uint32_t>(), (size_t)n, NIL);
     uint32_t longest = 0;
@@ -1206,9 +1295,8 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
                        S.segs[0][0].as<Seg>());
   }
   Counters hc{};
-  hc.nseg[0][0] = cur[0];
-  hc.nseg[0][1] = cur[1];
-  hc.nroot = (uint32_t)init_sub.size();
+  for (int c = 0; c < 3; c++) hc.nseg[0][c] = cur[c];
+  hc.ntiny = 0;
   hc.error = 0;
   RTG_HIP(hipMemcpyAsync(cnt, &hc, sizeof(hc), hipMemcpyHostToDevice, s));
   RTG_HIP(hipStreamSynchronize(s));  // the host staging above must outlive its copies
@@ -1221,16 +1309,17 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
   // big-segment kernels run until a level is known to have left no big
   // segment (children are smaller than their parents, so none comes back)
   bool big_live = cur[0] > 0;
+  const uint32_t small_grid = div_up(cap[2], SEGW_WAVES);
   auto enqueue_level = [&](int in) {
     const int o = in ^ 1;
     Lists nx;
-    for (int c = 0; c < 2; c++) {
+    for (int c = 0; c < 3; c++) {
       nx.seg[c] = S.segs[c][o].as<Seg>();
       nx.cap[c] = cap[c];
     }
     nx.nseg = cnt->nseg[o];
-    nx.roots = S.subroots.as<WSeg>();
-    nx.root_cap = root_cap;
+    nx.tiny = S.tiny.as<TinyItem>();
+    nx.tiny_cap = tiny_cap;
     nx.par = (uint32_t)o;
     nx.fin_max = fin_max;
     const Seg* bs = S.segs[0][in].as<Seg>();
@@ -1256,10 +1345,13 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
                          S.splits.as<Split>(), S.flags.as<uint32_t>(), S.scan.as<uint32_t>(), S.sid[o].as<uint32_t>(),
                          S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), vals, cnt);
     }
-    if (SUB_MAX < MID_MAX)
-      hipLaunchKernelGGL(lcost.P ? k_seg<true> : k_seg<false>, dim3(cap[1]), dim3(256), 0, s, S.segs[1][in].as<Seg>(),
-                         &cnt->nseg[in][1], S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
-                         S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t, lcost);
+    hipLaunchKernelGGL(lcost.P ? k_seg<true> : k_seg<false>, dim3(cap[1]), dim3(256), 0, s, S.segs[1][in].as<Seg>(),
+                       &cnt->nseg[in][1], S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
+                       S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t, lcost);
+    hipLaunchKernelGGL(lcost.P ? k_seg_wave<true> : k_seg_wave<false>, dim3(small_grid), dim3(SEGW_TB), 0, s,
+                       S.segs[2][in].as<Seg>(), &cnt->nseg[in][2], S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(),
+                       S.bhi[in].as<float4>(), S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(),
+                       nx, t, lcost);
     RTG_HIP(hipGetLastError());
   };
   S.rb.ensure(sizeof(Counters));
@@ -1268,7 +1360,7 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
     RTG_HIP(hipEventRecord(S.rb.ev[level % LevelReadback::K], s));
   };
   uint32_t levels = 0;
-  if (cur[0] + cur[1] > 0) {
+  if (cur[0] + cur[1] + cur[2] > 0) {
     enqueue_level(0);
     readback(0);
     for (levels = 1;; levels++) {
@@ -1285,20 +1377,21 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
         throw hip_error(hipErrorUnknown, "SAH build failed: error bits " + std::to_string(hc.error));
       }
       const uint32_t* next = hc.nseg[levels & 1u];  // input of the level just queued
-      if (next[0] + next[1] == 0) break;
+      if (next[0] + next[1] + next[2] == 0) break;
       if (next[0] == 0) big_live = false;
     }
   }
-  // every segment of at most SUB_MAX references, finished by one workgroup each
-  if (hc.nroot) {
+  // the tiny segments of all levels
+  if (hc.ntiny > tiny_cap) throw hip_error(hipErrorUnknown, "SAH build: tiny segment list overflowed");
+  if (hc.ntiny) {
     Bufs B;
     for (int p = 0; p < 2; p++) {
       B.idx[p] = S.idx[p].as<uint32_t>();
       B.blo[p] = S.blo[p].as<float4>();
       B.bhi[p] = S.bhi[p].as<float4>();
     }
-    hipLaunchKernelGGL(lcost.P ? k_subtree<true> : k_subtree<false>, dim3(div_up(hc.nroot, SUB_WAVES)), dim3(SUB_TB),
-                       0, s, S.subroots.as<WSeg>(), hc.nroot, B, t, lcost, fin_max);
+    hipLaunchKernelGGL(k_tiny, dim3(div_up(hc.ntiny, TINY_PER_BLOCK)), dim3(TINY_TB), 0, s, S.tiny.as<TinyItem>(),
+                       hc.ntiny, B, t);
     RTG_HIP(hipGetLastError());
   }
   out.roots.assign(trees.size(), NIL);

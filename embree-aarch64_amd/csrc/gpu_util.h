@@ -4,6 +4,7 @@
 #include <stdint.h>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 namespace rtg {
 
@@ -64,6 +65,10 @@ struct DevBuf {
     bytes = 0;
   }
   template <class T> T* as() const { return (T*)ptr; }
+  void swap(DevBuf& o) {
+    std::swap(ptr, o.ptr);
+    std::swap(bytes, o.bytes);
+  }
   ~DevBuf() { release(); }
   DevBuf() = default;
   DevBuf(const DevBuf&) = delete;

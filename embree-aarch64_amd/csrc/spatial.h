@@ -20,7 +20,8 @@ namespace rtg {
 constexpr uint32_t SPLIT_MAX_PIECES = 16;  // references one triangle may become
 
 struct SpatialScratch {
-  DevBuf prio, count, offset, lo4, hi4, vals;
+  DevBuf prio, count, offset, lo4, hi4, vals, sample;
+  PinnedBuf sample_host;
   Scanner scanner;
 };
 

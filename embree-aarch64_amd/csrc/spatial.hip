@@ -281,6 +281,11 @@ __global__ __launch_bounds__(TB) void k_split_emit(const uint32_t* __restrict__ 
   }
 }
 
+__global__ void k_gather_strided(const float* __restrict__ src, uint32_t stride, uint32_t m, float* __restrict__ dst) {
+  const uint32_t i = blockIdx.x * TB + threadIdx.x;
+  if (i < m) dst[i] = src[(size_t)i * stride];
+}
+
 }  // namespace
 
 uint32_t spatial_presplit(uint32_t n, float budget, const GeomDesc* geoms, const float glo[3], const float ghi[3],
@@ -305,9 +310,14 @@ uint32_t spatial_presplit(uint32_t n, float budget, const GeomDesc* geoms, const
   // priorities: the largest with (n / m) sum min(15, floor(p scale)) <= extra
   const uint32_t m = n < 65536u ? n : 65536u;
   const uint32_t stride = n / m;
-  std::vector<float> smp(m);
-  RTG_HIP(hipMemcpy2DAsync(smp.data(), 4, S.prio.ptr, (size_t)stride * 4, 4, m, hipMemcpyDeviceToHost, s));
+  S.sample.reserve((size_t)m * 4);
+  S.sample_host.reserve((size_t)m * 4);
+  hipLaunchKernelGGL(k_gather_strided, dim3(div_up(m, TB)), dim3(TB), 0, s, S.prio.as<float>(), stride, m,
+                     S.sample.as<float>());
+  RTG_HIP(hipMemcpyAsync(S.sample_host.ptr, S.sample.ptr, (size_t)m * 4, hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));
+  const float* smp_p = S.sample_host.as<float>();
+  const std::vector<float> smp(smp_p, smp_p + m);
   const double extra = std::floor((double)budget * (double)n);
   float pmin = INFINITY;
   for (float p : smp)
@@ -334,7 +344,7 @@ uint32_t spatial_presplit(uint32_t n, float budget, const GeomDesc* geoms, const
   RTG_HIP(hipStreamSynchronize(s));
   const uint64_t total = (uint64_t)last[0] + last[1];
   if (total <= n) return n;  // nothing to split
-  if (total > 0xfffffffeull) throw hip_error(hipErrorInvalidValue, "too many spatial-split references");
+  if (total >= REF_LEAF) throw hip_error(hipErrorInvalidValue, "too many spatial-split references (2^31 at most)");
   const uint32_t nr = (uint32_t)total;
   S.lo4.reserve((size_t)nr * 16);
   S.hi4.reserve((size_t)nr * 16);
