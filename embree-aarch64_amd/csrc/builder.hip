@@ -23,7 +23,7 @@ struct BuildCounters {
   uint32_t valid;
   uint32_t node_count;
   uint32_t tri_count;
-  uint32_t qcount[2];
+  uint32_t qcount[3];  // collapse level k reads slot (k + 2) % 3, fills k % 3, clears (k + 1) % 3
   uint32_t max_depth;
   uint32_t error;  // bit 0: refit walk too long, bit 1: collapse index out of range, bit 2: bad tree
   uint32_t max_stack;  // most traversal stack entries any root-to-node path can need
@@ -55,6 +55,16 @@ __device__ inline float wave_max(float v) {
   return v;
 }
 __device__ inline uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  return v;
+}
+__device__ inline uint32_t wave_umax(uint32_t v) {
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v = max(v, (uint32_t)__shfl_xor(v, d, 64));
+  return v;
+}
+__device__ inline double wave_sum(double v) {
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
   return v;
@@ -101,7 +111,8 @@ __global__ void k_init_counters(BuildCounters* c, uint32_t nscenes) {
   c->deep = 0;
   c->tri_count = 0;
   c->qcount[0] = 0;
-  c->qcount[1] = nscenes;  // the collapse's first level reads slot 1: one root item per scene
+  c->qcount[1] = 0;
+  c->qcount[2] = nscenes;  // the collapse's first level reads slot 2: one root item per scene
   c->max_depth = 0;
   c->max_stack = 0;
   c->error = 0;
@@ -112,11 +123,16 @@ __global__ void k_init_counters(BuildCounters* c, uint32_t nscenes) {
 
 __device__ inline bool finite_v(float x) { return (x > -FLT_LARGE_) & (x < FLT_LARGE_); }
 
+// Primitive references of PR_PER * BT consecutive primitives per block
+// (PR_PER per thread, BT apart), with the scene bounds and valid count
+// reduced over the block before one set of global atomics: one set per
+// BT primitives would serialise on those few addresses.
+constexpr int PR_PER = 4;
+
 __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ geoms, uint32_t ngeoms,
                                                  uint32_t total, float4* __restrict__ lo4,
                                                  float4* __restrict__ hi4, uint32_t* __restrict__ keys,
                                                  BuildCounters* __restrict__ cnt, SceneAcc* __restrict__ sacc) {
-  const uint32_t i = blockIdx.x * BT + threadIdx.x;
   float b[13];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
@@ -126,8 +142,12 @@ __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ ge
     b[9 + k] = -INFINITY;   // cent hi
   }
   b[12] = 0.0f;
-  uint32_t valid = 0, scene = 0xffffffffu;
-  if (i < total) {
+  uint32_t valid = 0, smin = 0xffffffffu, smax = 0u;
+  const uint32_t base = blockIdx.x * (BT * PR_PER) + threadIdx.x;
+#pragma unroll
+  for (int j = 0; j < PR_PER; j++) {
+    const uint32_t i = base + (uint32_t)j * BT;
+    if (i >= total) break;
     uint32_t lo = 0, hi = ngeoms - 1;
     while (lo < hi) {  // last geometry with prim_offset <= i
       uint32_t mid = (lo + hi + 1) >> 1;
@@ -171,17 +191,18 @@ __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ ge
 #pragma unroll
       for (int k = 0; k < 3; k++) {
         const float c2 = l[k] + h[k];
-        b[k] = l[k];
-        b[3 + k] = h[k];
-        b[6 + k] = c2;
-        b[9 + k] = c2;
+        b[k] = fminf(b[k], l[k]);
+        b[3 + k] = fmaxf(b[3 + k], h[k]);
+        b[6 + k] = fminf(b[6 + k], c2);
+        b[9 + k] = fmaxf(b[9 + k], c2);
         b[12] = fmaxf(b[12], fmaxf(fabsf(l[k]), fabsf(h[k])));
       }
       lo4[i] = make_float4(l[0], l[1], l[2], __uint_as_float(lo));
       hi4[i] = make_float4(h[0], h[1], h[2], __uint_as_float(prim));
       keys[i] = g.scene;
-      scene = g.scene;
-      valid = 1;
+      smin = min(smin, g.scene);
+      smax = max(smax, g.scene);
+      valid++;
     } else {
       keys[i] = 0xffffffffu;
     }
@@ -198,7 +219,16 @@ __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ ge
   }
   uint32_t vc = wave_sum(valid);
   if (lane == 0) redc[wave] = vc;
+  __shared__ uint32_t s_min, s_max;
+  if (threadIdx.x == 0) {
+    s_min = 0xffffffffu;
+    s_max = 0u;
+  }
   __syncthreads();
+  if (sacc && valid) {
+    atomicMin(&s_min, smin);
+    atomicMax(&s_max, smax);
+  }
   if (threadIdx.x < 13) {
     const int k = threadIdx.x;
     const bool mx = (k >= 3 && k < 6) || (k >= 9);
@@ -206,26 +236,18 @@ __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ ge
     for (int w = 1; w < BT / 64; w++) x = mx ? fmaxf(x, red[w][k]) : fminf(x, red[w][k]);
     uint32_t* dst = (k < 3) ? &cnt->glo[k] : (k < 6) ? &cnt->ghi[k - 3] : (k < 9) ? &cnt->clo[k - 6]
                     : (k < 12) ? &cnt->chi[k - 9] : &cnt->maxabs;
-    if (mx) atomicMax(dst, f2o(x)); else atomicMin(dst, f2o(x));
+    if (!__builtin_isinf(x) && !(k == 12 && x == 0.0f)) {
+      if (mx) atomicMax(dst, f2o(x)); else atomicMin(dst, f2o(x));
+    }
   }
   if (threadIdx.x == 0) {
-    uint32_t s = 0;
-    for (int w = 0; w < BT / 64; w++) s += redc[w];
-    if (s) atomicAdd(&cnt->valid, s);
+    uint32_t sum = 0;
+    for (int w = 0; w < BT / 64; w++) sum += redc[w];
+    if (sum) atomicAdd(&cnt->valid, sum);
   }
   if (!sacc) return;
   // per-scene bounds: the block's reduction when all its valid primitives
   // belong to one scene, else one set of atomics per primitive
-  __shared__ uint32_t s_min, s_max;
-  if (threadIdx.x == 0) {
-    s_min = 0xffffffffu;
-    s_max = 0u;
-  }
-  __syncthreads();
-  if (valid) {
-    atomicMin(&s_min, scene);
-    atomicMax(&s_max, scene);
-  }
   __syncthreads();
   if (s_min == 0xffffffffu) return;
   if (s_min == s_max) {
@@ -239,19 +261,27 @@ __global__ __launch_bounds__(BT) void k_primrefs(const GeomDesc* __restrict__ ge
       if (mx) atomicMax(dst, f2o(x)); else atomicMin(dst, f2o(x));
     }
     if (threadIdx.x == 0) {
-      uint32_t s = 0;
-      for (int w = 0; w < BT / 64; w++) s += redc[w];
-      atomicAdd(&a->valid, s);
+      uint32_t sum = 0;
+      for (int w = 0; w < BT / 64; w++) sum += redc[w];
+      atomicAdd(&a->valid, sum);
     }
-  } else if (valid) {
-    SceneAcc* a = sacc + scene;
-    for (int k = 0; k < 3; k++) {
-      atomicMin(&a->glo[k], f2o(b[k]));
-      atomicMax(&a->ghi[k], f2o(b[3 + k]));
-      atomicMin(&a->clo[k], f2o(b[6 + k]));
-      atomicMax(&a->chi[k], f2o(b[9 + k]));
+  } else {
+    for (int j = 0; j < PR_PER; j++) {
+      const uint32_t i = base + (uint32_t)j * BT;
+      if (i >= total) break;
+      const uint32_t sc = keys[i];
+      if (sc == 0xffffffffu) continue;
+      SceneAcc* a = sacc + sc;
+      const float4 l = lo4[i], h = hi4[i];
+      const float lv[3] = {l.x, l.y, l.z}, hv[3] = {h.x, h.y, h.z};
+      for (int k = 0; k < 3; k++) {
+        atomicMin(&a->glo[k], f2o(lv[k]));
+        atomicMax(&a->ghi[k], f2o(hv[k]));
+        atomicMin(&a->clo[k], f2o(lv[k] + hv[k]));
+        atomicMax(&a->chi[k], f2o(lv[k] + hv[k]));
+      }
+      atomicAdd(&a->valid, 1u);
     }
-    atomicAdd(&a->valid, 1u);
   }
 }
 
@@ -498,33 +528,64 @@ __global__ __launch_bounds__(BT) void k_node_depth(const uint32_t* __restrict__ 
 }
 
 // Inner nodes grouped by depth: a histogram, then a scatter into the
-// host-computed level offsets (order within a level does not matter).
+// host-computed level offsets (order within a level does not matter). A
+// block covers DEPTH_PER * BT nodes and adds to each global counter once:
+// a level's nodes are many and their depths few, and one global atomic per
+// (small block, depth) would serialise on those few addresses. Within a
+// wave, one LDS atomic serves every lane of one depth.
+constexpr int DEPTH_PER = 16;
+
+__device__ inline uint32_t depth_rank(uint32_t* h, uint32_t d, uint32_t none) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+  uint32_t local = 0u;
+  uint64_t todo = __ballot(d != none);
+  while (todo) {
+    const int leader = __ffsll((unsigned long long)todo) - 1;
+    const uint32_t dl = (uint32_t)__shfl((int)d, leader, 64);
+    const uint64_t same = __ballot(d == dl);
+    uint32_t b = 0u;
+    if ((int)lane == leader) b = atomicAdd(&h[dl], (uint32_t)__popcll(same));
+    b = (uint32_t)__shfl((int)b, leader, 64);
+    if (d == dl) local = b + (uint32_t)__popcll(same & below);
+    todo &= ~same;
+  }
+  return local;
+}
+
 __global__ __launch_bounds__(BT) void k_depth_hist(const uint8_t* __restrict__ depth, uint32_t m,
                                                    uint32_t* __restrict__ hist) {
   __shared__ uint32_t h[256];
   h[threadIdx.x] = 0u;
   __syncthreads();
-  const uint32_t i = blockIdx.x * BT + threadIdx.x;
-  if (i < m) atomicAdd(&h[depth[i]], 1u);
+  const uint32_t base = blockIdx.x * (BT * DEPTH_PER) + threadIdx.x;
+  for (int j = 0; j < DEPTH_PER; j++) {
+    const uint32_t i = base + (uint32_t)j * BT;
+    (void)depth_rank(h, i < m ? depth[i] : 256u, 256u);
+  }
   __syncthreads();
   if (h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], h[threadIdx.x]);
 }
 
-// One global atomic per (block, depth): local ranks from LDS, then a
-// block-wide reservation per depth (all threads of a level would otherwise
-// contend on one address).
 __global__ __launch_bounds__(BT) void k_depth_scatter(const uint8_t* __restrict__ depth, uint32_t m,
                                                       uint32_t* __restrict__ cursor, uint32_t* __restrict__ order) {
   __shared__ uint32_t h[256], base[256];
   h[threadIdx.x] = 0u;
   __syncthreads();
-  const uint32_t i = blockIdx.x * BT + threadIdx.x;
-  const uint32_t d = i < m ? depth[i] : 255u;  // 255: no node, or an id no tree uses
-  const uint32_t local = d != 255u ? atomicAdd(&h[d], 1u) : 0u;
+  const uint32_t first = blockIdx.x * (BT * DEPTH_PER) + threadIdx.x;
+  uint32_t d[DEPTH_PER], local[DEPTH_PER];
+#pragma unroll
+  for (int j = 0; j < DEPTH_PER; j++) {
+    const uint32_t i = first + (uint32_t)j * BT;
+    d[j] = i < m ? depth[i] : 255u;  // 255: no node, or an id no tree uses
+    local[j] = depth_rank(h, d[j], 255u);
+  }
   __syncthreads();
   if (h[threadIdx.x]) base[threadIdx.x] = atomicAdd(&cursor[threadIdx.x], h[threadIdx.x]);
   __syncthreads();
-  if (d != 255u) order[base[d] + local] = i;
+#pragma unroll
+  for (int j = 0; j < DEPTH_PER; j++)
+    if (d[j] != 255u) order[base[d[j]] + local[j]] = first + (uint32_t)j * BT;
 }
 
 // Integrity check of the sorted keys and the radix tree (cheap, one pass):
@@ -567,6 +628,7 @@ struct CollapseArgs {
   QItem* qout;
   const uint32_t* nin;  // items in qin (a device counter)
   uint32_t qout_slot;
+  uint32_t* qclear;     // the count the next level fills, cleared here (no kernel of this level uses it)
   const uint32_t* child;
   const uint2* range;
   const float4* nlo;
@@ -618,34 +680,76 @@ __device__ inline uint32_t quant_hi(float v, float p, float s) {
   return qi;
 }
 
-// Slots from a shared counter for the wave's active lanes in one atomic:
-// lane values (below 2^bits) are summed bit by bit over ballots, so lanes
-// that have left the code path add nothing; returns this lane's first slot.
-__device__ inline uint32_t wave_alloc(uint32_t* counter, uint32_t v, int bits) {
-  const int lane = (int)(threadIdx.x & 63u);
-  const uint64_t active = __ballot(1);
-  const int leader = __ffsll((unsigned long long)active) - 1;
-  const uint64_t below = active & ((1ull << lane) - 1ull);
-  uint32_t prefix = 0, total = 0;
-  for (int b = 0; b < bits; b++) {
-    const uint64_t m = __ballot((v >> b) & 1u);
-    prefix += (uint32_t)__popcll(m & below) << b;
-    total += (uint32_t)__popcll(m) << b;
+// Statistics of the nodes a thread wrote, reduced over the wave before
+// the global atomics (one per item would serialise on their addresses).
+struct CollapseStats {
+  uint32_t max_depth = 0, max_stack = 0;
+  double sah_nodes = 0.0, sah_leaves = 0.0, sah_leaves_tri = 0.0;
+};
+
+// Node, record and queue slots for the workgroup's items in one atomic per
+// counter (atomics on one address from every wave would serialise): the
+// bit-sliced ballot sums give each lane its offset within the wave, LDS
+// the waves' offsets within the workgroup. Called by every thread of the
+// workgroup; ni (children that are inner nodes) is below 16 and nt (leaf
+// records) below 512.
+__device__ inline void block_alloc(BuildCounters* cnt, uint32_t qslot, uint32_t ni, uint32_t nt, uint32_t& child_base,
+                                   uint32_t& tri_base, uint32_t& qbase) {
+  __shared__ uint32_t wtot[BT / 64][2], bbase[3];
+  const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+  const uint64_t below = lane == 0u ? 0ull : (~0ull >> (64u - lane));
+  uint32_t pi = 0u, ti = 0u, pt = 0u, tt = 0u;
+#pragma unroll
+  for (int b = 0; b < 4; b++) {
+    const uint64_t m = __ballot((ni >> b) & 1u);
+    pi += (uint32_t)__popcll(m & below) << b;
+    ti += (uint32_t)__popcll(m) << b;
   }
-  uint32_t base = 0;
-  if (lane == leader && total) base = atomicAdd(counter, total);
-  base = __shfl(base, leader, 64);
-  return base + prefix;
+#pragma unroll
+  for (int b = 0; b < 9; b++) {
+    const uint64_t m = __ballot((nt >> b) & 1u);
+    pt += (uint32_t)__popcll(m & below) << b;
+    tt += (uint32_t)__popcll(m) << b;
+  }
+  if (lane == 0u) {
+    wtot[wave][0] = ti;
+    wtot[wave][1] = tt;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t si = 0u, st = 0u;
+    for (int w = 0; w < BT / 64; w++) {
+      si += wtot[w][0];
+      st += wtot[w][1];
+    }
+    bbase[0] = si ? atomicAdd(&cnt->node_count, si) : 0u;
+    bbase[1] = st ? atomicAdd(&cnt->tri_count, st) : 0u;
+    bbase[2] = si ? atomicAdd(&cnt->qcount[qslot], si) : 0u;
+  }
+  __syncthreads();
+  uint32_t oi = 0u, ot = 0u;
+  for (uint32_t w = 0; w < wave; w++) {
+    oi += wtot[w][0];
+    ot += wtot[w][1];
+  }
+  child_base = bbase[0] + oi + pi;
+  tri_base = bbase[1] + ot + pt;
+  qbase = bbase[2] + oi + pi;
+  __syncthreads();  // wtot and bbase are reused by the next item
 }
 
-__device__ inline void collapse_item(const CollapseArgs& a, uint32_t t) {
-  const QItem item = a.qin[t];
+// One item (an 8-wide node to write) per thread; every thread of the
+// workgroup calls it (active: the thread has an item), for block_alloc.
+__device__ inline void collapse_item(const CollapseArgs& a, uint32_t t, bool active, CollapseStats& st) {
+  bool go = active;
+  QItem item{};
+  if (go) item = a.qin[t];
   const float eps = o2f(a.cnt->maxabs) * BOX_EPS_REL;
-  if (item.bref == REF_EMPTY) {  // root of a scene without primitives: a node with no children
+  if (go && item.bref == REF_EMPTY) {  // root of a scene without primitives: a node with no children
+    go = false;
     if (item.node >= a.n) {
       atomicOr(&a.cnt->error, 2u);
-      return;
-    }
+    } else {
     QNode8 q;
     q.px = q.py = q.pz = 0.0f;
     q.ex = q.ey = q.ez = 127;
@@ -664,13 +768,14 @@ __device__ inline void collapse_item(const CollapseArgs& a, uint32_t t) {
     inf.count = 0;
     inf.depth = item.depth;
     a.info[item.node] = inf;
-    return;
+    }
   }
 
   // children: the optimal distribution of 8 slots below item.bref
   uint32_t ch[8];
   int n = 0;
-  if (!is_inner(item.bref, a.isleaf)) {
+  if (!go) {
+  } else if (!is_inner(item.bref, a.isleaf)) {
     ch[0] = item.bref;  // whole (small) scene as one leaf
     n = 1;
   } else {
@@ -694,7 +799,12 @@ __device__ inline void collapse_item(const CollapseArgs& a, uint32_t t) {
         ch[n++] = c;
       } else {
         const uint32_t jj = a.split8[8 * (size_t)c + slots - 1] & 7u;
-        if (sp + 2 > 8) { atomicOr(&a.cnt->error, 128u); return; }
+        if (sp + 2 > 8) {
+          atomicOr(&a.cnt->error, 128u);
+          go = false;
+          n = 0;
+          break;
+        }
         st_ref[sp] = a.child[2 * c + 1]; st_s[sp] = slots - jj; sp++;
         st_ref[sp] = a.child[2 * c];     st_s[sp] = jj;          sp++;
       }
@@ -747,28 +857,28 @@ __device__ inline void collapse_item(const CollapseArgs& a, uint32_t t) {
     }
   }
   // the children partition the parent's range: a second count of the leaf primitives
-  {
+  if (go) {
     const uint32_t b = item.bref;
     const uint32_t total_prims = (b & REF_LEAF) ? 1u : (rng32[2 * b + 1] - rng32[2 * b] + 1u);
     if (total_prims - ninner_prims != nt) atomicOr(&a.cnt->error, 64u);
   }
-  // node, record and queue slots: one atomic per wave and counter
-  const uint32_t child_base = wave_alloc(&a.cnt->node_count, ni, 4);
-  const uint32_t tri_base = wave_alloc(&a.cnt->tri_count, nt, 9);
-  const uint32_t qbase = wave_alloc(&a.cnt->qcount[a.qout_slot], ni, 4);
+  // node, record and queue slots: one atomic per workgroup and counter
+  uint32_t child_base, tri_base, qbase;
+  block_alloc(a.cnt, a.qout_slot, go ? ni : 0u, go ? nt : 0u, child_base, tri_base, qbase);
+  if (!go) return;
   if ((uint64_t)child_base + ni > a.n || (uint64_t)tri_base + nt > a.tri_cap || (uint64_t)qbase + ni > a.n ||
       item.node >= a.n) {
     atomicOr(&a.cnt->error, 2u);
     return;
   }
-  atomicMax(&a.cnt->max_depth, item.depth);
+  st.max_depth = max(st.max_depth, item.depth);
   // a node pushes at most n-1 of its n children
   const uint32_t need = item.need + (uint32_t)n - 1u;
-  atomicMax(&a.cnt->max_stack, need);
-  atomicAdd(&a.cnt->sah_nodes, (double)half_area(uh[0] - ul[0], uh[1] - ul[1], uh[2] - ul[2]));
+  st.max_stack = max(st.max_stack, need);
+  st.sah_nodes += (double)half_area(uh[0] - ul[0], uh[1] - ul[1], uh[2] - ul[2]);
   if (nt) {
-    atomicAdd(&a.cnt->sah_leaves, sah_leaf);
-    atomicAdd(&a.cnt->sah_leaves_tri, sah_leaf_tri);
+    st.sah_leaves += sah_leaf;
+    st.sah_leaves_tri += sah_leaf_tri;
   }
 
   // quantisation grid over the widened union box
@@ -872,12 +982,29 @@ __global__ __launch_bounds__(BT) void k_write_leaves(const uint32_t* __restrict_
 // length it reads on the device. A tree that failed its check (error bits
 // set) is not followed.
 __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
-  if (a.cnt->error) return;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *a.qclear = 0u;
+  // the workgroup decides together (block_alloc synchronises it)
+  __shared__ uint32_t s_run;
+  if (threadIdx.x == 0) s_run = a.cnt->error ? 0u : 1u;
+  __syncthreads();
   const uint32_t nin = *a.nin;
-  for (uint32_t t = blockIdx.x * BT + threadIdx.x; t < nin; t += gridDim.x * BT) collapse_item(a, t);
+  if (s_run && blockIdx.x * BT < nin) {
+    CollapseStats st;
+    // block-uniform loop: every thread takes part in each item's block_alloc
+    for (uint32_t base = blockIdx.x * BT; base < nin; base += gridDim.x * BT)
+      collapse_item(a, base + threadIdx.x, base + threadIdx.x < nin, st);
+    const uint32_t md = wave_umax(st.max_depth), ms = wave_umax(st.max_stack);
+    const double sn = wave_sum(st.sah_nodes), sl = wave_sum(st.sah_leaves), slt = wave_sum(st.sah_leaves_tri);
+    if ((threadIdx.x & 63u) == 0u) {
+      if (md) atomicMax(&a.cnt->max_depth, md);
+      if (ms) atomicMax(&a.cnt->max_stack, ms);
+      if (sn != 0.0) atomicAdd(&a.cnt->sah_nodes, sn);
+      if (sl != 0.0) atomicAdd(&a.cnt->sah_leaves, sl);
+      if (slt != 0.0) atomicAdd(&a.cnt->sah_leaves_tri, slt);
+    }
+  }
 }
 
-__global__ void k_zero_word(uint32_t* p) { *p = 0u; }
 
 // ------------------------------------------------------------------ host --
 
@@ -926,7 +1053,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     hipLaunchKernelGGL(k_init_sacc, dim3(div_up(nscenes, BT)), dim3(BT), 0, s, sacc, nscenes);
   }
   hipLaunchKernelGGL(k_init_counters, dim3(1), dim3(1), 0, s, cnt, nscenes);
-  hipLaunchKernelGGL(k_primrefs, dim3(div_up(total, BT)), dim3(BT), 0, s, S.geoms.as<GeomDesc>(),
+  hipLaunchKernelGGL(k_primrefs, dim3(div_up(total, (size_t)BT * PR_PER)), dim3(BT), 0, s, S.geoms.as<GeomDesc>(),
                      (uint32_t)geoms.size(), total, S.lo4.as<float4>(), S.hi4.as<float4>(), S.keys.as<uint32_t>(), cnt,
                      sacc);
   if (batched) {
@@ -1068,7 +1195,8 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     uint32_t* hist = S.lvl_hist.as<uint32_t>();
     uint32_t* hh = S.lvl_host.as<uint32_t>();  // [0, 256): histogram, 256: deep flag, [257, 513): offsets
     RTG_HIP(hipMemsetAsync(hist, 0, 2 * 256 * 4, s));
-    hipLaunchKernelGGL(k_depth_hist, dim3(div_up(m, BT)), dim3(BT), 0, s, S.depth.as<uint8_t>(), m, hist);
+    hipLaunchKernelGGL(k_depth_hist, dim3(div_up(m, (size_t)BT * DEPTH_PER)), dim3(BT), 0, s, S.depth.as<uint8_t>(), m,
+                       hist);
     RTG_HIP(hipMemcpyAsync(hh, hist, 256 * 4, hipMemcpyDeviceToHost, s));
     RTG_HIP(hipMemcpyAsync(hh + 256, deep, 4, hipMemcpyDeviceToHost, s));
     RTG_HIP(hipStreamSynchronize(s));
@@ -1080,8 +1208,8 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
       total += d < 255 ? hh[d] : 0u;
     }
     RTG_HIP(hipMemcpyAsync(hist + 256, off, 256 * 4, hipMemcpyHostToDevice, s));
-    hipLaunchKernelGGL(k_depth_scatter, dim3(div_up(m, BT)), dim3(BT), 0, s, S.depth.as<uint8_t>(), m, hist + 256,
-                       S.lvl_order.as<uint32_t>());
+    hipLaunchKernelGGL(k_depth_scatter, dim3(div_up(m, (size_t)BT * DEPTH_PER)), dim3(BT), 0, s,
+                       S.depth.as<uint8_t>(), m, hist + 256, S.lvl_order.as<uint32_t>());
     for (int d = 254; d >= 0; d--)
       if (hh[d])
         hipLaunchKernelGGL(k_refit_level, dim3(div_up(hh[d], BT)), dim3(BT), 0, s, ra,
@@ -1172,11 +1300,11 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   const uint32_t cgrid = std::min<uint32_t>(div_up(cap, BT), 2048u);
   auto enqueue_level = [&](uint32_t k) {
     const uint32_t o = k & 1u;
-    hipLaunchKernelGGL(k_zero_word, dim3(1), dim3(1), 0, s, &cnt->qcount[o]);
     ca.qin = o ? q1 : q0;
     ca.qout = o ? q0 : q1;
-    ca.nin = &cnt->qcount[o ^ 1u];
-    ca.qout_slot = o;
+    ca.nin = &cnt->qcount[(k + 2u) % 3u];
+    ca.qout_slot = k % 3u;
+    ca.qclear = &cnt->qcount[(k + 1u) % 3u];
     hipLaunchKernelGGL(k_collapse, dim3(cgrid), dim3(BT), 0, s, ca);
     RTG_HIP(hipGetLastError());
     RTG_HIP(hipMemcpyAsync(S.rb.slot((int)k), cnt, sizeof(BuildCounters), hipMemcpyDeviceToHost, s));
@@ -1193,7 +1321,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
     enqueue_level(k);  // speculative: without input it does nothing
     RTG_HIP(hipEventSynchronize(S.rb.ev[(k - 1) % LevelReadback::K]));
     memcpy(&hc, S.rb.slot((int)(k - 1)), sizeof(hc));
-    const uint32_t next = hc.qcount[(k - 1) & 1u];
+    const uint32_t next = hc.qcount[(k - 1) % 3u];
     level_sizes += " " + std::to_string(next);
     if (next == 0 || hc.error) break;
     if (next > cap) {

@@ -42,7 +42,7 @@ struct LevelReadback {
 };
 
 struct SahScratch {
-  DevBuf idx[2], blo[2], bhi[2], sid[2], segs[3][2], tiny, bins, splits, flags, scan, counters, roots;
+  DevBuf idx[2], blo[2], bhi[2], sid[2], segs[3][3], tiny, bins, splits, flags, scan, counters, roots;
   Scanner scanner;
   LevelReadback rb;
 };

@@ -17,7 +17,7 @@
 //
 // Segments (contiguous position ranges still to split) are split by size
 // class; the class fixes how a segment is split, whatever kernel does it:
-//   big   (> 4096): positions in chunks of 2048 bin into LDS, the chunk bins
+//   big   (> 4096): positions in chunks of 1024 bin into LDS, the chunk bins
 //                   merge into per-segment bins in HBM with atomics, one
 //                   workgroup per segment picks the split, and a global scan
 //                   of left flags gives every position its stable destination;
@@ -51,7 +51,7 @@ constexpr uint32_t MEDIAN_MAX = 64;      // largest median_below honoured
 constexpr uint32_t TINY_MAX = 16;        // segments up to this size are finished by one thread (exact SAH)
 constexpr uint32_t MID_MAX = 4096;
 constexpr uint32_t SMALL_MAX = 512;     // segments up to this size are split by one wave
-constexpr uint32_t CHUNK = 2048;        // positions per block of the big-segment binning
+constexpr uint32_t CHUNK = 1024;        // positions per block of the big-segment binning
 constexpr uint32_t MEDIAN_DEPTH = 48;   // from this binary depth on, splits are object medians
 constexpr int TBS = 256;
 
@@ -83,7 +83,8 @@ struct TinyItem {
 };
 
 struct Counters {
-  uint32_t nseg[2][3];  // segment counts by list parity: big, mid, small (a level reads one parity and fills the other)
+  uint32_t nseg[3][3];  // segment counts by list slot: big, mid, small (level L reads slot L % 3, fills (L + 1) % 3
+                        // and clears (L + 2) % 3 for the level after)
   uint32_t ntiny;       // tiny segments so far (all levels)
   uint32_t error;    // bit 0: a segment list overflowed, bit 1: bin counts disagree, bit 2: destination out of range,
                      // bit 3: median subtree stack overflow, bit 4: a tiny segment out of range,
@@ -164,6 +165,66 @@ __device__ inline void bin_prim(uint32_t* w, const Mapping& m, float4 l, float4 
       atomicMax(&w[wh(a, b, k)], oh[k]);
     }
   }
+}
+
+// Bins the references at positions [p, p + cnt), consecutive in the
+// Morton order: a run of references in one bin adds its count and box in
+// one set of atomics.
+__device__ inline void bin_run(uint32_t* w, const Mapping& m, const float4* __restrict__ blo,
+                               const float4* __restrict__ bhi, uint32_t p, uint32_t cnt) {
+  int cb[3] = {-1, -1, -1};
+  uint32_t cn[3] = {0u, 0u, 0u}, ql[3][3], qh[3][3];
+  auto flush = [&](int a) {
+    if (cn[a] == 0u) return;
+    atomicAdd(&w[wc(a, cb[a])], cn[a]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      atomicMin(&w[wl(a, cb[a], k)], ql[a][k]);
+      atomicMax(&w[wh(a, cb[a], k)], qh[a][k]);
+    }
+    cn[a] = 0u;
+  };
+  // loads in batches of four, issued before their use
+  constexpr uint32_t BATCH = 4;
+  for (uint32_t j0 = 0; j0 < cnt; j0 += BATCH) {
+    float4 bl[BATCH], bh[BATCH];
+#pragma unroll
+    for (uint32_t u = 0; u < BATCH; u++)
+      if (j0 + u < cnt) {
+        bl[u] = blo[p + j0 + u];
+        bh[u] = bhi[p + j0 + u];
+      }
+#pragma unroll
+    for (uint32_t u = 0; u < BATCH; u++) {
+      if (j0 + u >= cnt) break;
+      const float4 l = bl[u], h = bh[u];
+      const float c[3] = {l.x + h.x, l.y + h.y, l.z + h.z};
+      const uint32_t ol[3] = {f2o(l.x), f2o(l.y), f2o(l.z)};
+      const uint32_t oh[3] = {f2o(h.x), f2o(h.y), f2o(h.z)};
+#pragma unroll
+      for (int a = 0; a < 3; a++) {
+        const int b = bin_of(c[a], m.ofs[a], m.scale[a], m.nb);
+        if (b != cb[a] || cn[a] == 0u) {
+          flush(a);
+          cb[a] = b;
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            ql[a][k] = ol[k];
+            qh[a][k] = oh[k];
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < 3; k++) {
+            ql[a][k] = ol[k] < ql[a][k] ? ol[k] : ql[a][k];
+            qh[a][k] = oh[k] > qh[a][k] ? oh[k] : qh[a][k];
+          }
+        }
+        cn[a]++;
+      }
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 3; a++) flush(a);
 }
 
 // Box of bins [b0, b1) on axis a.
@@ -304,14 +365,83 @@ __device__ inline void group_sync() {
 }
 
 
-// Plain binned-SAH sweep of axis a by one half-wave (32 lanes; lane b holds
-// bin b): the least A_L*n_L + A_R*n_R over the "bin < k" candidates, first
-// minimum in k, with the float operations of a left-to-right scan (prefix
-// and suffix boxes are exact minima and maxima, so the costs are the scan's
-// bit for bit). cost is INFINITY when no k separates the references; every
-// lane of the half returns the result and the axis's reference count.
-__device__ inline void sweep_axis_half(const uint32_t* w, const Mapping& m, int a, int b, float& cost, int& kbest,
+// Moves within the 16-lane rows of a wave (DPP): lane i of a row gets
+// lane i - D's value (ROW_SHR + D), lane i + D's (ROW_SHL + D) or lane
+// (i - D) mod 16's (ROW_ROR + D); a lane without a source in its row keeps
+// its own value. Called with the whole wave active.
+constexpr int ROW_SHL = 0x100, ROW_SHR = 0x110, ROW_ROR = 0x120;
+template <int CTRL>
+__device__ inline uint32_t row_move(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_update_dpp((int)v, (int)v, CTRL, 0xf, 0xf, false);
+}
+template <int CTRL>
+__device__ inline float row_move(float v) {
+  return __uint_as_float(row_move<CTRL>(__float_as_uint(v)));
+}
+template <int CTRL>
+__device__ inline int row_move(int v) {
+  return (int)row_move<CTRL>((uint32_t)v);
+}
+
+// Plain binned-SAH sweep of axis a by one half-wave (32 lanes; lane b =
+// lane & 31 holds bin b): the least A_L*n_L + A_R*n_R over the "bin < k"
+// candidates, first minimum in k, with the float operations of a
+// left-to-right scan (prefix and suffix boxes are exact minima and maxima,
+// so the costs are the scan's bit for bit). cost is INFINITY when no k
+// separates the references; every lane of the half returns the result and
+// the axis's reference count. Called by a whole wave (both halves); the
+// scans move values within 16-lane rows (DPP) and across the half's two
+// rows by lane reads.
+__device__ inline float read_lane(float v, int l) { return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l)); }
+__device__ inline uint32_t read_lane(uint32_t v, int l) { return (uint32_t)__builtin_amdgcn_readlane((int)v, l); }
+__device__ inline int read_lane(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+template <int D>
+__device__ inline void sweep_scan_step(int r, uint32_t& nl, float llo[3], float lhi[3], uint32_t& nr, float rlo[3],
+                                       float rhi[3]) {
+  const uint32_t un = row_move<ROW_SHR + D>(nl), dn = row_move<ROW_SHL + D>(nr);
+  float ul[3], uh[3], dl[3], dh[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    ul[k] = row_move<ROW_SHR + D>(llo[k]);
+    uh[k] = row_move<ROW_SHR + D>(lhi[k]);
+    dl[k] = row_move<ROW_SHL + D>(rlo[k]);
+    dh[k] = row_move<ROW_SHL + D>(rhi[k]);
+  }
+  if (r >= D) {
+    nl += un;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      llo[k] = fminf(llo[k], ul[k]);
+      lhi[k] = fmaxf(lhi[k], uh[k]);
+    }
+  }
+  if (r + D < 16) {
+    nr += dn;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      rlo[k] = fminf(rlo[k], dl[k]);
+      rhi[k] = fmaxf(rhi[k], dh[k]);
+    }
+  }
+}
+
+template <int D>
+__device__ inline void sweep_min_step(int r, float& cst, int& kk, uint32_t& nlv) {
+  const float oc = row_move<ROW_SHR + D>(cst);
+  const int ok = row_move<ROW_SHR + D>(kk);
+  const uint32_t on = row_move<ROW_SHR + D>(nlv);
+  if (r >= D && (oc < cst || (oc == cst && ok < kk))) {
+    cst = oc;
+    kk = ok;
+    nlv = on;
+  }
+}
+
+__device__ inline void sweep_axis_half(const uint32_t* w, const Mapping& m, int a, int lane, float& cost, int& kbest,
                                        uint32_t& nleft, uint32_t& total) {
+  const int b = lane & 31, r = lane & 15;
+  const bool upper = lane >= 32;
   const uint32_t c = b < m.nb ? w[wc(a, b)] : 0u;
   float llo[3], lhi[3];
 #pragma unroll
@@ -321,41 +451,54 @@ __device__ inline void sweep_axis_half(const uint32_t* w, const Mapping& m, int 
   }
   float rlo[3] = {llo[0], llo[1], llo[2]}, rhi[3] = {lhi[0], lhi[1], lhi[2]};
   uint32_t nl = c, nr = c;
+  // prefix over the row's bins up to b, suffix from b
+  sweep_scan_step<1>(r, nl, llo, lhi, nr, rlo, rhi);
+  sweep_scan_step<2>(r, nl, llo, lhi, nr, rlo, rhi);
+  sweep_scan_step<4>(r, nl, llo, lhi, nr, rlo, rhi);
+  sweep_scan_step<8>(r, nl, llo, lhi, nr, rlo, rhi);
+  // the half's second row adds its first row's total to the prefix, the
+  // first row the second row's to the suffix
+  const uint32_t pn = upper ? read_lane(nl, 47) : read_lane(nl, 15);
+  const uint32_t sn = upper ? read_lane(nr, 48) : read_lane(nr, 16);
+  float pbl[3], pbh[3], sbl[3], sbh[3];
 #pragma unroll
-  for (int d = 1; d < 32; d <<= 1) {
-    const uint32_t un = __shfl_up(nl, d, 32), dn = __shfl_down(nr, d, 32);
-    float ul[3], uh[3], dl[3], dh[3];
+  for (int k = 0; k < 3; k++) {
+    pbl[k] = upper ? read_lane(llo[k], 47) : read_lane(llo[k], 15);
+    pbh[k] = upper ? read_lane(lhi[k], 47) : read_lane(lhi[k], 15);
+    sbl[k] = upper ? read_lane(rlo[k], 48) : read_lane(rlo[k], 16);
+    sbh[k] = upper ? read_lane(rhi[k], 48) : read_lane(rhi[k], 16);
+  }
+  if (b >= 16) {
+    nl += pn;
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      ul[k] = __shfl_up(llo[k], d, 32);
-      uh[k] = __shfl_up(lhi[k], d, 32);
-      dl[k] = __shfl_down(rlo[k], d, 32);
-      dh[k] = __shfl_down(rhi[k], d, 32);
+      llo[k] = fminf(llo[k], pbl[k]);
+      lhi[k] = fmaxf(lhi[k], pbh[k]);
     }
-    if (b >= d) {
-      nl += un;
+  } else {
+    nr += sn;
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        llo[k] = fminf(llo[k], ul[k]);
-        lhi[k] = fmaxf(lhi[k], uh[k]);
-      }
-    }
-    if (b + d < 32) {
-      nr += dn;
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        rlo[k] = fminf(rlo[k], dl[k]);
-        rhi[k] = fmaxf(rhi[k], dh[k]);
-      }
+    for (int k = 0; k < 3; k++) {
+      rlo[k] = fminf(rlo[k], sbl[k]);
+      rhi[k] = fmaxf(rhi[k], sbh[k]);
     }
   }
   // candidate k = b + 1: bins [0, b] left, the suffix from bin b + 1 right
-  const uint32_t nr1 = __shfl_down(nr, 1, 32);
+  // (for b = 15 that is the second row's first suffix, read above)
+  uint32_t nr1 = row_move<ROW_SHL + 1>(nr);
   float r1l[3], r1h[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    r1l[k] = __shfl_down(rlo[k], 1, 32);
-    r1h[k] = __shfl_down(rhi[k], 1, 32);
+    r1l[k] = row_move<ROW_SHL + 1>(rlo[k]);
+    r1h[k] = row_move<ROW_SHL + 1>(rhi[k]);
+  }
+  if (b == 15) {
+    nr1 = sn;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      r1l[k] = sbl[k];
+      r1h[k] = sbh[k];
+    }
   }
   float cst = INFINITY;
   if (m.scale[a] != 0.0f && b + 1 < m.nb && nl != 0u && nr1 != 0u)
@@ -363,21 +506,22 @@ __device__ inline void sweep_axis_half(const uint32_t* w, const Mapping& m, int 
           half_area(r1h[0] - r1l[0], r1h[1] - r1l[1], r1h[2] - r1l[2]) * (float)nr1;
   int kk = b + 1;
   uint32_t nlv = nl;
-  total = __shfl(nl, 31, 32);
-#pragma unroll
-  for (int d = 16; d >= 1; d >>= 1) {
-    const float oc = __shfl_xor(cst, d, 32);
-    const int ok = __shfl_xor(kk, d, 32);
-    const uint32_t on = __shfl_xor(nlv, d, 32);
-    if (oc < cst || (oc == cst && ok < kk)) {
-      cst = oc;
-      kk = ok;
-      nlv = on;
-    }
-  }
-  cost = cst;
-  kbest = kk;
-  nleft = nlv;
+  total = upper ? read_lane(nl, 63) : read_lane(nl, 31);
+  // least (cost, k) of each row reaches its last lane, then the two rows'
+  sweep_min_step<1>(r, cst, kk, nlv);
+  sweep_min_step<2>(r, cst, kk, nlv);
+  sweep_min_step<4>(r, cst, kk, nlv);
+  sweep_min_step<8>(r, cst, kk, nlv);
+  const float c0 = upper ? read_lane(cst, 47) : read_lane(cst, 15);
+  const int k0 = upper ? read_lane(kk, 47) : read_lane(kk, 15);
+  const uint32_t n0 = upper ? read_lane(nlv, 47) : read_lane(nlv, 15);
+  const float c1 = upper ? read_lane(cst, 63) : read_lane(cst, 31);
+  const int k1 = upper ? read_lane(kk, 63) : read_lane(kk, 31);
+  const uint32_t n1 = upper ? read_lane(nlv, 63) : read_lane(nlv, 31);
+  const bool second = c1 < c0 || (c1 == c0 && k1 < k0);
+  cost = second ? c1 : c0;
+  kbest = second ? k1 : k0;
+  nleft = second ? n1 : n0;
 }
 
 // Workgroup-level choice of the split of segment sg from its bins w (LDS or
@@ -466,16 +610,16 @@ __device__ inline Split choose_split(const uint32_t* w, const Mapping& m, const 
     group_sync<WAVE>();
   }
   if (!learned && tid < 64) {
-    // one half-wave per axis: x and y, then z
+    // one half-wave per axis: x and y, then z (the upper half sweeps z as
+    // well, for the whole-wave moves, and keeps nothing)
     const int half = tid >> 5, b = tid & 31;
     for (int pass = 0; pass < 2; pass++) {
       const int a = pass == 0 ? half : 2;
-      if (pass == 1 && half == 1) break;
       float c;
       int k;
       uint32_t nl, total;
-      sweep_axis_half(w, m, a, b, c, k, nl, total);
-      if (b == 0) {
+      sweep_axis_half(w, m, a, tid, c, k, nl, total);
+      if (b == 0 && !(pass == 1 && half == 1)) {
         if (total != sg.count) atomicOr(err, 2u);
         L.cost[a] = depth < MEDIAN_DEPTH && total == sg.count ? c : INFINITY;
         L.k[a] = k;
@@ -561,9 +705,6 @@ __global__ void k_init_gbins(uint32_t* g, const uint32_t* nseg) {
   g[i] = w < 3 * NB ? 0u : (w < 3 * NB + 9 * NB ? NIL : 0u);
 }
 
-__global__ void k_zero_counts(uint32_t* nseg) {
-  if (threadIdx.x < 3) nseg[threadIdx.x] = 0u;
-}
 
 // A chunk of CHUNK positions meets at most two big segments (each is longer
 // than the chunk): they are binned in LDS and merged into HBM; any other
@@ -710,12 +851,14 @@ __global__ __launch_bounds__(TBS) void k_big_split(const Seg* __restrict__ segs,
                                                    const uint32_t* __restrict__ gbins, Split* __restrict__ splits,
                                                    Lists nx, Tree t, LearnedCost lc) {
   __shared__ ChooseLds L;
+  __shared__ uint32_t w[BIN_WORDS];  // the segment's bins, read once from HBM
   const uint32_t i = blockIdx.x;
   if (i >= *nseg) return;
   const Seg sg = segs[i];
   const uint32_t depth = sg.meta >> 1;
   const Mapping m = make_mapping(sg);
-  const uint32_t* w = gbins + (size_t)i * BIN_WORDS;
+  for (int k = threadIdx.x; k < BIN_WORDS; k += TBS) w[k] = gbins[(size_t)i * BIN_WORDS + k];
+  __syncthreads();
   Split sp = choose_split<TBS, false, LEARNED>(w, m, sg, lc, L, &t.cnt->error, threadIdx.x);
   if (threadIdx.x != 0) return;
   // child centroid bounds: lo+hi of a primitive lies in [2 lo, 2 hi] of its
@@ -773,7 +916,10 @@ __global__ __launch_bounds__(TBS) void k_big_scatter(const uint32_t* __restrict_
   const uint32_t i = blockIdx.x * TBS + threadIdx.x;
   if (i >= n) return;
   const uint32_t s = sid[i];
-  if (s == NIL) return;
+  if (s == NIL) {  // not in a big segment: the position keeps its place
+    sid_o[i] = NIL;
+    return;
+  }
   const uint32_t first = segs[s].first;
   const Split& sp = splits[s];
   const uint32_t lr = scan[i] - scan[first];
@@ -825,80 +971,116 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
   const uint32_t end = sg.first + sg.count;
   const int per = PER == 1 ? 1 : (int)((sg.count + BLOCK - 1) / BLOCK);
   const uint32_t p0 = sg.first + (uint32_t)tid * (uint32_t)per;
-#pragma unroll 4
-  for (int j = 0; j < per; j++) {
-    const uint32_t p = p0 + j;
-    if (p < end) bin_prim(S.sb, m, blo[p], bhi[p]);
+  if constexpr (PER == 1) {
+    if (p0 < end) bin_prim(S.sb, m, blo[p0], bhi[p0]);
+  } else if (p0 < end) {
+    bin_run(S.sb, m, blo, bhi, p0, min((uint32_t)per, end - p0));
   }
   group_sync<WAVE>();
   const Split sp = choose_split<BLOCK, WAVE, LEARNED>(S.sb, m, sg, lc, S.L, &t.cnt->error, tid);
   // flags of this thread's positions, and the stable destinations
   uint32_t mask = 0u, nl = 0u;
-#pragma unroll 4
-  for (int j = 0; j < per; j++) {
-    const uint32_t p = p0 + j;
-    if (p < end && goes_left(sp, blo[p], bhi[p], p - sg.first)) {
-      mask |= 1u << j;
-      nl++;
+  constexpr int BATCH = PER < 4 ? PER : 4;
+  for (int j0 = 0; j0 < per; j0 += BATCH) {
+    float4 bl[BATCH], bh[BATCH];
+#pragma unroll
+    for (int u = 0; u < BATCH; u++) {
+      const uint32_t p = p0 + j0 + u;
+      if (j0 + u < per && p < end) {
+        bl[u] = blo[p];
+        bh[u] = bhi[p];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < BATCH; u++) {
+      const uint32_t p = p0 + j0 + u;
+      if (j0 + u < per && p < end && goes_left(sp, bl[u], bh[u], p - sg.first)) {
+        mask |= 1u << (j0 + u);
+        nl++;
+      }
     }
   }
-  uint32_t x = nl;
+  // the wave's exclusive prefix of nl (at most PER) and its total, bit by
+  // bit over ballots
+  constexpr int NL_BITS = 32 - __builtin_clz((unsigned)PER);
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  uint32_t x = 0u, wtotal = 0u;
 #pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const uint32_t y = __shfl_up(x, d, 64);
-    if (lane >= d) x += y;
+  for (int b = 0; b < NL_BITS; b++) {
+    const uint64_t m = __ballot((nl >> b) & 1u);
+    x += (uint32_t)__popcll(m & below) << b;
+    wtotal += (uint32_t)__popcll(m) << b;
   }
-  if (lane == 63) S.wsum[wave] = x;
+  if (lane == 0) S.wsum[wave] = wtotal;
   group_sync<WAVE>();
-  uint32_t lbase = x - nl;
+  uint32_t lbase = x;
   for (int w = 0; w < wave; w++) lbase += S.wsum[w];
   // centroid bounds of the left (lb) and right (rb) references: lo3, hi3
   float lb[6], rb[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) lb[k] = rb[k] = k < 3 ? INFINITY : -INFINITY;
   uint32_t lr = 0u;
-  for (int j = 0; j < per; j++) {
-    const uint32_t p = p0 + j;
-    if (p >= end) break;
-    const bool left = (mask >> j) & 1u;
-    const uint32_t rel = p - sg.first;
-    const uint32_t dst = left ? sg.first + lbase + lr : sg.first + sp.nleft + (rel - lbase - lr);
-    if (left) lr++;
-    if (dst >= end) {
-      atomicOr(&t.cnt->error, 4u);
-      continue;
-    }
-    const float4 l = blo[p], h = bhi[p];
-    const uint32_t v = idx[p];
-    idx_o[dst] = v;
-    blo_o[dst] = l;
-    bhi_o[dst] = h;
-    t.vals[dst] = v;
-    const float c[3] = {l.x + h.x, l.y + h.y, l.z + h.z};
-    if (left) {
+  for (int j0 = 0; j0 < per; j0 += BATCH) {
+    float4 bl[BATCH], bh[BATCH];
+    uint32_t bv[BATCH];
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        lb[k] = fminf(lb[k], c[k]);
-        lb[3 + k] = fmaxf(lb[3 + k], c[k]);
+    for (int u = 0; u < BATCH; u++) {  // loads in batches, issued before their use
+      const uint32_t p = p0 + j0 + u;
+      if (j0 + u < per && p < end) {
+        bl[u] = blo[p];
+        bh[u] = bhi[p];
+        bv[u] = idx[p];
       }
-    } else {
+    }
 #pragma unroll
-      for (int k = 0; k < 3; k++) {
-        rb[k] = fminf(rb[k], c[k]);
-        rb[3 + k] = fmaxf(rb[3 + k], c[k]);
+    for (int u = 0; u < BATCH; u++) {
+      const int j = j0 + u;
+      const uint32_t p = p0 + j;
+      if (j >= per || p >= end) break;
+      const bool left = (mask >> j) & 1u;
+      const uint32_t rel = p - sg.first;
+      const uint32_t dst = left ? sg.first + lbase + lr : sg.first + sp.nleft + (rel - lbase - lr);
+      if (left) lr++;
+      if (dst >= end) {
+        atomicOr(&t.cnt->error, 4u);
+        continue;
+      }
+      const float4 l = bl[u], h = bh[u];
+      const uint32_t v = bv[u];
+      idx_o[dst] = v;
+      blo_o[dst] = l;
+      bhi_o[dst] = h;
+      t.vals[dst] = v;
+      const float c[3] = {l.x + h.x, l.y + h.y, l.z + h.z};
+      if (left) {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          lb[k] = fminf(lb[k], c[k]);
+          lb[3 + k] = fmaxf(lb[3 + k], c[k]);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          rb[k] = fminf(rb[k], c[k]);
+          rb[3 + k] = fmaxf(rb[3 + k], c[k]);
+        }
       }
     }
   }
 #pragma unroll
   for (int k = 0; k < 12; k++) {
+    // the least (greatest) of each 16-lane row reaches the row's last lane
     float v = k < 6 ? lb[k] : rb[k - 6];
     const bool mx = (k % 6) >= 3;
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) {
-      const float y = __shfl_xor(v, d, 64);
-      v = mx ? fmaxf(v, y) : fminf(v, y);
-    }
-    if (lane == 0 && !__builtin_isinf(v)) {
+    float y = row_move<ROW_SHR + 1>(v);
+    v = mx ? fmaxf(v, y) : fminf(v, y);
+    y = row_move<ROW_SHR + 2>(v);
+    v = mx ? fmaxf(v, y) : fminf(v, y);
+    y = row_move<ROW_SHR + 4>(v);
+    v = mx ? fmaxf(v, y) : fminf(v, y);
+    y = row_move<ROW_SHR + 8>(v);
+    v = mx ? fmaxf(v, y) : fminf(v, y);
+    if ((lane & 15) == 15 && !__builtin_isinf(v)) {
       if (mx) atomicMax(&S.cb[k], f2o(v)); else atomicMin(&S.cb[k], f2o(v));
     }
   }
@@ -921,8 +1103,10 @@ __global__ __launch_bounds__(256) void k_seg(const Seg* __restrict__ segs, const
                                              const uint32_t* __restrict__ idx, const float4* __restrict__ blo,
                                              const float4* __restrict__ bhi, uint32_t* __restrict__ idx_o,
                                              float4* __restrict__ blo_o, float4* __restrict__ bhi_o, Lists nx,
-                                             Tree t, LearnedCost lc) {
+                                             Tree t, LearnedCost lc, uint32_t* __restrict__ clear) {
   __shared__ SegLds<256> S;
+  // the counts the next level fills (no kernel of this level uses them)
+  if (blockIdx.x == 0 && threadIdx.x < 3) clear[threadIdx.x] = 0u;
   if (blockIdx.x >= *nseg) return;
   const Seg sg = segs[blockIdx.x];
   float lcl[3], lch[3], rcl[3], rch[3];
@@ -1003,13 +1187,69 @@ struct Bufs {
 };
 
 struct TinyLds {
-  float k[3][TINY_MAX];
   float4 lo[TINY_MAX], hi[TINY_MAX];
   uint32_t id[TINY_MAX];
 };
 
 constexpr int TINY_GROUPS = 64 / (int)TINY_MAX;
 static_assert(TINY_GROUPS * TINY_MAX == 64, "tiny segments take 16-lane groups of a wave");
+
+// Stable ranks of a lane's keys among its node's lanes [nf, nf + nc) of
+// the row: r0 by (k0, position), r1 by (k1, k0, position), r2 by (k2, k1,
+// k0, position); f0 and f1 count the keys below on axis 0 and 1 with ties
+// broken by r2's order. The keys of lane (pos - R) mod 16 arrive by
+// rotating the row one lane per step (a chain, which keeps few registers
+// live); the lane itself counts for nothing.
+__device__ inline void tiny_ranks(float k0, float k1, float k2, uint32_t pos, uint32_t nf, uint32_t nc,
+                                  uint32_t& r0, uint32_t& r1, uint32_t& r2, uint32_t& f0, uint32_t& f1) {
+  float a0 = k0, a1 = k1, a2 = k2;
+#pragma unroll 1
+  for (uint32_t R = 1; R < TINY_MAX; R++) {
+    a0 = row_move<ROW_ROR + 1>(a0);
+    a1 = row_move<ROW_ROR + 1>(a1);
+    a2 = row_move<ROW_ROR + 1>(a2);
+    const uint32_t j = (pos - R) & (TINY_MAX - 1u);
+    // branch-free: bitwise operators on the comparisons
+    const bool in = j - nf < nc;
+    const bool lt0 = (a0 < k0) | ((a0 == k0) & (j < pos));
+    const bool lt1 = (a1 < k1) | ((a1 == k1) & lt0);
+    const bool lt2 = (a2 < k2) | ((a2 == k2) & lt1);
+    r0 += (uint32_t)(in & lt0);
+    r1 += (uint32_t)(in & lt1);
+    r2 += (uint32_t)(in & lt2);
+    f0 += (uint32_t)(in & ((a0 < k0) | ((a0 == k0) & lt2)));
+    f1 += (uint32_t)(in & ((a1 < k1) | ((a1 == k1) & lt2)));
+  }
+}
+
+// One step D of the prefix (pl, ph: lanes [nf, pos]) and suffix (ql, qh:
+// lanes [pos, nf + nc)) boxes of a node's lanes.
+template <int D>
+__device__ inline void tiny_scan_step(float pl[3], float ph[3], float ql[3], float qh[3], bool busy, uint32_t pos,
+                                      uint32_t nf, uint32_t nc) {
+  float ul[3], uh[3], dl[3], dh[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    ul[k] = row_move<ROW_SHR + D>(pl[k]);
+    uh[k] = row_move<ROW_SHR + D>(ph[k]);
+    dl[k] = row_move<ROW_SHL + D>(ql[k]);
+    dh[k] = row_move<ROW_SHL + D>(qh[k]);
+  }
+  if (busy && pos >= nf + (uint32_t)D) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      pl[k] = fminf(pl[k], ul[k]);
+      ph[k] = fmaxf(ph[k], uh[k]);
+    }
+  }
+  if (busy && pos + (uint32_t)D < nf + nc) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      ql[k] = fminf(ql[k], dl[k]);
+      qh[k] = fmaxf(qh[k], dh[k]);
+    }
+  }
+}
 
 __device__ inline bool tiny_better(float c, int a, uint32_t k, float bc, int ba, uint32_t bk) {
   return c < bc || (c == bc && (a < ba || (a == ba && k < bk)));
@@ -1068,27 +1308,12 @@ __device__ inline void tiny_groups(const TinyItem* q, int nq, const Bufs& B, con
       break;
     }
     const float k0 = l.x + h.x, k1 = l.y + h.y, k2 = l.z + h.z;
-    T.k[0][pos] = k0;
-    T.k[1][pos] = k1;
-    T.k[2][pos] = k2;
-    group_sync<true>();
     uint32_t r0 = 0u, r1 = 0u, r2 = 0u, f0 = 0u, f1 = 0u;
-    if (busy) {
-      for (uint32_t i = nf; i < nf + nc; i++) {
-        const float a0 = T.k[0][i], a1 = T.k[1][i], a2 = T.k[2][i];
-        const bool lt0 = a0 < k0 || (a0 == k0 && i < pos);
-        const bool lt1 = a1 < k1 || (a1 == k1 && lt0);
-        const bool lt2 = a2 < k2 || (a2 == k2 && lt1);
-        r0 += lt0 ? 1u : 0u;
-        r1 += lt1 ? 1u : 0u;
-        r2 += lt2 ? 1u : 0u;
-        f0 += (a0 < k0 || (a0 == k0 && lt2)) ? 1u : 0u;
-        f1 += (a1 < k1 || (a1 == k1 && lt2)) ? 1u : 0u;
-      }
-    }
+    tiny_ranks(k0, k1, k2, pos, nf, nc, r0, r1, r2, f0, f1);
     float bc = INFINITY;
     int ba = 0;
     uint32_t bk = 1u;
+#pragma unroll 1
     for (int a = 0; a < 3; a++) {
       const uint32_t r = a == 0 ? r0 : (a == 1 ? r1 : r2);
       group_sync<true>();
@@ -1105,36 +1330,16 @@ __device__ inline void tiny_groups(const TinyItem* q, int nq, const Bufs& B, con
       // prefix box over the node's sorted positions [nf, pos], suffix over [pos, nf + nc)
       float pl[3] = {sl.x, sl.y, sl.z}, ph[3] = {sh.x, sh.y, sh.z};
       float ql[3] = {sl.x, sl.y, sl.z}, qh[3] = {sh.x, sh.y, sh.z};
-#pragma unroll
-      for (int d = 1; d < (int)TINY_MAX; d <<= 1) {
-        float ul[3], uh[3], dl[3], dh[3];
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          ul[k] = __shfl_up(pl[k], d, TINY_MAX);
-          uh[k] = __shfl_up(ph[k], d, TINY_MAX);
-          dl[k] = __shfl_down(ql[k], d, TINY_MAX);
-          dh[k] = __shfl_down(qh[k], d, TINY_MAX);
-        }
-        if (busy && pos >= nf + (uint32_t)d) {
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            pl[k] = fminf(pl[k], ul[k]);
-            ph[k] = fmaxf(ph[k], uh[k]);
-          }
-        }
-        if (busy && pos + (uint32_t)d < nf + nc) {
-#pragma unroll
-          for (int k = 0; k < 3; k++) {
-            ql[k] = fminf(ql[k], dl[k]);
-            qh[k] = fmaxf(qh[k], dh[k]);
-          }
-        }
-      }
+      static_assert(TINY_MAX == 16, "the scans take 16-lane rows");
+      tiny_scan_step<1>(pl, ph, ql, qh, busy, pos, nf, nc);
+      tiny_scan_step<2>(pl, ph, ql, qh, busy, pos, nf, nc);
+      tiny_scan_step<4>(pl, ph, ql, qh, busy, pos, nf, nc);
+      tiny_scan_step<8>(pl, ph, ql, qh, busy, pos, nf, nc);
       float rl[3], rh[3];
 #pragma unroll
       for (int k = 0; k < 3; k++) {
-        rl[k] = __shfl_down(ql[k], 1, TINY_MAX);
-        rh[k] = __shfl_down(qh[k], 1, TINY_MAX);
+        rl[k] = row_move<ROW_SHL + 1>(ql[k]);
+        rh[k] = row_move<ROW_SHL + 1>(qh[k]);
       }
       // candidate: the first pos - nf + 1 sorted references left
       if (busy && pos + 1u < nf + nc) {
@@ -1149,17 +1354,17 @@ __device__ inline void tiny_groups(const TinyItem* q, int nq, const Bufs& B, con
       }
     }
     // least (cost, axis, position) of each node reaches the node's first lane
-#pragma unroll
-    for (int d = 1; d < (int)TINY_MAX; d <<= 1) {
-      const float oc = __shfl_down(bc, d, TINY_MAX);
-      const int oa = __shfl_down(ba, d, TINY_MAX);
-      const uint32_t ok = __shfl_down(bk, d, TINY_MAX);
-      if (busy && pos + (uint32_t)d < nf + nc && tiny_better(oc, oa, ok, bc, ba, bk)) {
+    auto reduce_step = [&](float oc, int oa, uint32_t ok, uint32_t d) {
+      if (busy && pos + d < nf + nc && tiny_better(oc, oa, ok, bc, ba, bk)) {
         bc = oc;
         ba = oa;
         bk = ok;
       }
-    }
+    };
+    reduce_step(row_move<ROW_SHL + 1>(bc), row_move<ROW_SHL + 1>(ba), row_move<ROW_SHL + 1>(bk), 1u);
+    reduce_step(row_move<ROW_SHL + 2>(bc), row_move<ROW_SHL + 2>(ba), row_move<ROW_SHL + 2>(bk), 2u);
+    reduce_step(row_move<ROW_SHL + 4>(bc), row_move<ROW_SHL + 4>(ba), row_move<ROW_SHL + 4>(bk), 4u);
+    reduce_step(row_move<ROW_SHL + 8>(bc), row_move<ROW_SHL + 8>(ba), row_move<ROW_SHL + 8>(bk), 8u);
     ba = __shfl(ba, (int)nf, TINY_MAX);
     bk = __shfl(bk, (int)nf, TINY_MAX);
     const uint32_t rf = ba == 0 ? f0 : (ba == 1 ? f1 : r2);
@@ -1245,8 +1450,9 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
     S.blo[p].reserve((size_t)n * 16);
     S.bhi[p].reserve((size_t)n * 16);
     S.sid[p].reserve((size_t)n * 4);
-    for (int c = 0; c < 3; c++) S.segs[c][p].reserve((size_t)cap[c] * sizeof(Seg));
   }
+  for (int c = 0; c < 3; c++)
+    for (int l = 0; l < 3; l++) S.segs[c][l].reserve((size_t)cap[c] * sizeof(Seg));
   S.tiny.reserve((size_t)tiny_cap * sizeof(TinyItem));
   S.bins.reserve((size_t)cap[0] * BIN_WORDS * 4);
   S.splits.reserve((size_t)cap[0] * sizeof(Split));
@@ -1308,33 +1514,37 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
   // work, and reads level l's counters (through a pinned slot) meanwhile.
   // big-segment kernels run until a level is known to have left no big
   // segment (children are smaller than their parents, so none comes back)
+  S.rb.ensure(sizeof(Counters));
   bool big_live = cur[0] > 0;
   const uint32_t small_grid = div_up(cap[2], SEGW_WAVES);
-  auto enqueue_level = [&](int in) {
-    const int o = in ^ 1;
+  // level L: references (and big-segment ids) in buffers of parity L & 1,
+  // written to the other parity; segment lists and counts in slot L % 3,
+  // children to slot (L + 1) % 3, while slot (L + 2) % 3, last read by
+  // level L - 1, is cleared for level L + 1
+  auto enqueue_level = [&](uint32_t L) {
+    const int in = (int)(L & 1u), o = in ^ 1;
+    const int li = (int)(L % 3u), lo = (int)((L + 1u) % 3u), lz = (int)((L + 2u) % 3u);
     Lists nx;
     for (int c = 0; c < 3; c++) {
-      nx.seg[c] = S.segs[c][o].as<Seg>();
+      nx.seg[c] = S.segs[c][lo].as<Seg>();
       nx.cap[c] = cap[c];
     }
-    nx.nseg = cnt->nseg[o];
+    nx.nseg = cnt->nseg[lo];
     nx.tiny = S.tiny.as<TinyItem>();
     nx.tiny_cap = tiny_cap;
     nx.par = (uint32_t)o;
     nx.fin_max = fin_max;
-    const Seg* bs = S.segs[0][in].as<Seg>();
-    hipLaunchKernelGGL(k_zero_counts, dim3(1), dim3(64), 0, s, cnt->nseg[o]);
+    const Seg* bs = S.segs[0][li].as<Seg>();
     if (big_live) {
       hipLaunchKernelGGL(k_init_gbins, dim3(div_up((size_t)cap[0] * BIN_WORDS, TBS)), dim3(TBS), 0, s,
-                         S.bins.as<uint32_t>(), &cnt->nseg[in][0]);
-      hipLaunchKernelGGL(k_fill_u32, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[o].as<uint32_t>(), (size_t)n, NIL);
+                         S.bins.as<uint32_t>(), &cnt->nseg[li][0]);
       hipLaunchKernelGGL(k_big_bins, dim3(div_up(n, CHUNK)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
                          S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs, S.bins.as<uint32_t>());
       if (lcost.P)
-        hipLaunchKernelGGL(k_big_split<true>, dim3(cap[0]), dim3(TBS), 0, s, bs, &cnt->nseg[in][0],
+        hipLaunchKernelGGL(k_big_split<true>, dim3(cap[0]), dim3(TBS), 0, s, bs, &cnt->nseg[li][0],
                            S.bins.as<uint32_t>(), S.splits.as<Split>(), nx, t, lcost);
       else
-        hipLaunchKernelGGL(k_big_split<false>, dim3(cap[0]), dim3(TBS), 0, s, bs, &cnt->nseg[in][0],
+        hipLaunchKernelGGL(k_big_split<false>, dim3(cap[0]), dim3(TBS), 0, s, bs, &cnt->nseg[li][0],
                            S.bins.as<uint32_t>(), S.splits.as<Split>(), nx, t, lcost);
       hipLaunchKernelGGL(k_big_flags, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
                          S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs, S.splits.as<Split>(),
@@ -1345,16 +1555,16 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
                          S.splits.as<Split>(), S.flags.as<uint32_t>(), S.scan.as<uint32_t>(), S.sid[o].as<uint32_t>(),
                          S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), vals, cnt);
     }
-    hipLaunchKernelGGL(lcost.P ? k_seg<true> : k_seg<false>, dim3(cap[1]), dim3(256), 0, s, S.segs[1][in].as<Seg>(),
-                       &cnt->nseg[in][1], S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
-                       S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t, lcost);
+    hipLaunchKernelGGL(lcost.P ? k_seg<true> : k_seg<false>, dim3(cap[1]), dim3(256), 0, s, S.segs[1][li].as<Seg>(),
+                       &cnt->nseg[li][1], S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
+                       S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t, lcost,
+                       cnt->nseg[lz]);
     hipLaunchKernelGGL(lcost.P ? k_seg_wave<true> : k_seg_wave<false>, dim3(small_grid), dim3(SEGW_TB), 0, s,
-                       S.segs[2][in].as<Seg>(), &cnt->nseg[in][2], S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(),
+                       S.segs[2][li].as<Seg>(), &cnt->nseg[li][2], S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(),
                        S.bhi[in].as<float4>(), S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(),
                        nx, t, lcost);
     RTG_HIP(hipGetLastError());
   };
-  S.rb.ensure(sizeof(Counters));
   auto readback = [&](uint32_t level) {
     RTG_HIP(hipMemcpyAsync(S.rb.slot((int)level), cnt, sizeof(Counters), hipMemcpyDeviceToHost, s));
     RTG_HIP(hipEventRecord(S.rb.ev[level % LevelReadback::K], s));
@@ -1368,7 +1578,7 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
         (void)hipStreamSynchronize(s);
         throw hip_error(hipErrorUnknown, "SAH build did not terminate");
       }
-      enqueue_level((int)(levels & 1u));  // speculative: without input it does nothing
+      enqueue_level(levels);  // speculative: without input it does nothing
       readback(levels);
       RTG_HIP(hipEventSynchronize(S.rb.ev[(levels - 1) % LevelReadback::K]));
       memcpy(&hc, S.rb.slot((int)(levels - 1)), sizeof(hc));
@@ -1376,7 +1586,7 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
         (void)hipStreamSynchronize(s);
         throw hip_error(hipErrorUnknown, "SAH build failed: error bits " + std::to_string(hc.error));
       }
-      const uint32_t* next = hc.nseg[levels & 1u];  // input of the level just queued
+      const uint32_t* next = hc.nseg[levels % 3u];  // input of the level just queued
       if (next[0] + next[1] + next[2] == 0) break;
       if (next[0] == 0) big_live = false;
     }
