@@ -738,9 +738,10 @@ __device__ inline void block_alloc(BuildCounters* cnt, uint32_t qslot, uint32_t 
   __syncthreads();  // wtot and bbase are reused by the next item
 }
 
-// One item (an 8-wide node to write) per thread; every thread of the
-// workgroup calls it (active: the thread has an item), for block_alloc.
-__device__ inline void collapse_item(const CollapseArgs& a, uint32_t t, bool active, CollapseStats& st) {
+// One item (an 8-wide node to write) per thread, for the levels with many
+// items (throughput); every thread of the workgroup calls it (active: the
+// thread has an item), for block_alloc.
+__device__ inline void collapse_item1(const CollapseArgs& a, uint32_t t, bool active, CollapseStats& st) {
   bool go = active;
   QItem item{};
   if (go) item = a.qin[t];
@@ -936,6 +937,238 @@ __device__ inline void collapse_item(const CollapseArgs& a, uint32_t t, bool act
   a.info[item.node] = inf;
 }
 
+// One item (an 8-wide node to write) per group of CG = 8 lanes, for the
+// levels with few items (latency), lane k of the group taking slot k of the
+// node: the group descends the binary tree
+// together (every lane the same way) to the node's children, lane k keeps
+// child k, and the lanes then load, count, quantise and write their own
+// child's slot, with the node's union box, counts and slot ranks formed by
+// reductions and scans over the group. Every thread of the workgroup calls
+// it (active: the group has an item), for block_alloc.
+constexpr uint32_t CG = 8;
+constexpr uint32_t COLLAPSE_GROUPED_MAX = 8192;  // items of a level up to which groups take them
+
+template <class T>
+__device__ inline T group_sum(T v) {
+#pragma unroll
+  for (int d = 1; d < (int)CG; d <<= 1) v += __shfl_xor(v, d, CG);
+  return v;
+}
+__device__ inline float group_min(float v) {
+#pragma unroll
+  for (int d = 1; d < (int)CG; d <<= 1) v = fminf(v, __shfl_xor(v, d, CG));
+  return v;
+}
+__device__ inline float group_max(float v) {
+#pragma unroll
+  for (int d = 1; d < (int)CG; d <<= 1) v = fmaxf(v, __shfl_xor(v, d, CG));
+  return v;
+}
+// exclusive prefix over the group's lanes
+__device__ inline uint32_t group_prefix(uint32_t v, uint32_t k) {
+  uint32_t x = v;
+#pragma unroll
+  for (int d = 1; d < (int)CG; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d, CG);
+    if (k >= (uint32_t)d) x += y;
+  }
+  return x - v;
+}
+
+__device__ inline void collapse_item(const CollapseArgs& a, uint32_t t, bool active, uint32_t k, CollapseStats& st) {
+  bool go = active;
+  QItem item{};
+  if (go) item = a.qin[t];
+  const float eps = o2f(a.cnt->maxabs) * BOX_EPS_REL;
+  if (go && item.bref == REF_EMPTY) {  // root of a scene without primitives: a node with no children
+    go = false;
+    if (k == 0) {
+      if (item.node >= a.n) {
+        atomicOr(&a.cnt->error, 2u);
+      } else {
+        QNode8 q;
+        q.px = q.py = q.pz = 0.0f;
+        q.ex = q.ey = q.ez = 127;
+        q.imask = 0;
+        q.child_base = q.tri_base = 0;
+#pragma unroll
+        for (int j = 0; j < 8; j++) {
+          q.meta[j] = 0;
+          q.qlox[j] = q.qloy[j] = q.qloz[j] = 255;
+          q.qhix[j] = q.qhiy[j] = q.qhiz[j] = 0;
+        }
+        a.nodes[item.node] = q;
+        NodeInfo inf;
+        inf.lo[0] = inf.lo[1] = inf.lo[2] = 0.0f;
+        inf.hi[0] = inf.hi[1] = inf.hi[2] = 0.0f;
+        inf.count = 0;
+        inf.depth = item.depth;
+        a.info[item.node] = inf;
+      }
+    }
+  }
+
+  // children: the optimal distribution of 8 slots below item.bref (the
+  // group's lanes take the same path); lane k keeps child k
+  uint32_t mine = 0u;
+  int n = 0;
+  if (!go) {
+  } else if (!is_inner(item.bref, a.isleaf)) {
+    mine = item.bref;  // whole (small) scene as one leaf
+    n = 1;
+  } else {
+    uint32_t st_ref[8], st_s[8];
+    int sp = 0;
+    {
+      const uint32_t j8 = (a.split8[8 * (size_t)item.bref + a.width - 1]) & 7u;
+      st_ref[0] = a.child[2 * item.bref + 1]; st_s[0] = a.width - j8;
+      st_ref[1] = a.child[2 * item.bref];     st_s[1] = j8;
+      sp = 2;
+    }
+    while (sp > 0 && n < 8) {
+      sp--;
+      uint32_t c = st_ref[sp], slots = st_s[sp];
+      if (!(c & REF_LEAF)) {
+        while (slots > 1 && (a.split8[8 * (size_t)c + slots - 1] & 0x80u)) slots--;
+      } else {
+        slots = 1;
+      }
+      if (slots <= 1) {
+        if ((uint32_t)n == k) mine = c;
+        n++;
+      } else {
+        const uint32_t jj = a.split8[8 * (size_t)c + slots - 1] & 7u;
+        if (sp + 2 > 8) {
+          if (k == 0) atomicOr(&a.cnt->error, 128u);
+          go = false;
+          n = 0;
+          break;
+        }
+        st_ref[sp] = a.child[2 * c + 1]; st_s[sp] = slots - jj; sp++;
+        st_ref[sp] = a.child[2 * c];     st_s[sp] = jj;          sp++;
+      }
+    }
+  }
+
+  // this lane's child: box, primitive range and kind
+  const bool has = go && k < (uint32_t)n;
+  const uint32_t* rng32 = reinterpret_cast<const uint32_t*>(a.range);
+  float4 l = make_float4(INFINITY, INFINITY, INFINITY, 0.0f), h = make_float4(-INFINITY, -INFINITY, -INFINITY, 0.0f);
+  uint32_t f = 0u, cnt = 0u;
+  bool inner = false;
+  if (has) {
+    if (mine & REF_LEAF) {
+      l = a.slo[mine & ~REF_LEAF];
+      h = a.shi[mine & ~REF_LEAF];
+      f = mine & ~REF_LEAF;
+      cnt = 1u;
+    } else {
+      l = a.nlo[mine];
+      h = a.nhi[mine];
+      f = rng32[2 * mine];
+      cnt = rng32[2 * mine + 1] - f + 1u;
+    }
+    inner = is_inner(mine, a.isleaf);
+  }
+  const float ul[3] = {group_min(l.x), group_min(l.y), group_min(l.z)};
+  const float uh[3] = {group_max(h.x), group_max(h.y), group_max(h.z)};
+  const uint32_t ci = inner ? 1u : 0u, ct = (has && !inner) ? cnt : 0u;
+  const uint32_t ni = group_sum(ci), nt = group_sum(ct);
+  const uint32_t ninner_prims = group_sum(inner ? cnt : 0u);
+  const uint32_t rank_i = group_prefix(ci, k), off_t = group_prefix(ct, k);
+  double sah_leaf = 0.0, sah_leaf_tri = 0.0;
+  if (has && !inner) {
+    const float A = box_half_area(l, h);
+    sah_leaf = (double)A * (double)((cnt + 3) / 4);
+    sah_leaf_tri = (double)A * (double)cnt;
+  }
+  sah_leaf = group_sum(sah_leaf);
+  sah_leaf_tri = group_sum(sah_leaf_tri);
+  // the children partition the parent's range: a second count of the leaf primitives
+  if (go && k == 0) {
+    const uint32_t b = item.bref;
+    const uint32_t total_prims = (b & REF_LEAF) ? 1u : (rng32[2 * b + 1] - rng32[2 * b] + 1u);
+    if (total_prims - ninner_prims != nt) atomicOr(&a.cnt->error, 64u);
+  }
+  // node, record and queue slots: one atomic per workgroup and counter (the
+  // group's first lane asks for the group)
+  uint32_t child_base, tri_base, qbase;
+  block_alloc(a.cnt, a.qout_slot, (go && k == 0) ? ni : 0u, (go && k == 0) ? nt : 0u, child_base, tri_base, qbase);
+  const int first_lane = (int)((threadIdx.x & 63u) & ~(CG - 1u));
+  child_base = (uint32_t)__shfl((int)child_base, first_lane, 64);
+  tri_base = (uint32_t)__shfl((int)tri_base, first_lane, 64);
+  qbase = (uint32_t)__shfl((int)qbase, first_lane, 64);
+  if (!go) return;
+  if ((uint64_t)child_base + ni > a.n || (uint64_t)tri_base + nt > a.tri_cap || (uint64_t)qbase + ni > a.n ||
+      item.node >= a.n) {
+    if (k == 0) atomicOr(&a.cnt->error, 2u);
+    return;
+  }
+  // a node pushes at most n-1 of its n children
+  const uint32_t need = item.need + (uint32_t)n - 1u;
+  if (k == 0) {
+    st.max_depth = max(st.max_depth, item.depth);
+    st.max_stack = max(st.max_stack, need);
+    st.sah_nodes += (double)half_area(uh[0] - ul[0], uh[1] - ul[1], uh[2] - ul[2]);
+    if (nt) {
+      st.sah_leaves += sah_leaf;
+      st.sah_leaves_tri += sah_leaf_tri;
+    }
+  }
+
+  // quantisation grid over the widened union box (every lane the same)
+  float p[3], sc[3];
+  uint32_t e[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    p[d] = ul[d] - eps;
+    e[d] = grid_exponent(p[d], uh[d] + eps);
+    sc[d] = __uint_as_float(e[d] << 23);
+  }
+  uint8_t* node = reinterpret_cast<uint8_t*>(a.nodes + item.node);
+  uint32_t meta = 0u, qlx = 255u, qhx = 0u, qly = 255u, qhy = 0u, qlz = 255u, qhz = 0u;
+  if (has) {
+    qlx = quant_lo(l.x - eps, p[0], sc[0]);
+    qhx = quant_hi(h.x + eps, p[0], sc[0]);
+    qly = quant_lo(l.y - eps, p[1], sc[1]);
+    qhy = quant_hi(h.y + eps, p[1], sc[1]);
+    qlz = quant_lo(l.z - eps, p[2], sc[2]);
+    qhz = quant_hi(h.z + eps, p[2], sc[2]);
+    if (inner) {
+      meta = 0x80u | rank_i;
+      a.qout[qbase + rank_i] = QItem{mine, child_base + rank_i, item.depth + 1, need};
+    } else {
+      meta = 0x80u | off_t;
+      // leaf records are written afterwards, one thread each (k_write_leaves)
+      for (uint32_t j = 0; j < cnt; j++)
+        a.leaf_src[tri_base + off_t + j] = (f + j) | ((j + 1 == cnt) ? LEAF_SRC_LAST : 0u);
+    }
+  }
+  node[offsetof(QNode8, meta) + k] = (uint8_t)meta;
+  node[offsetof(QNode8, qlox) + k] = (uint8_t)qlx;
+  node[offsetof(QNode8, qhix) + k] = (uint8_t)qhx;
+  node[offsetof(QNode8, qloy) + k] = (uint8_t)qly;
+  node[offsetof(QNode8, qhiy) + k] = (uint8_t)qhy;
+  node[offsetof(QNode8, qloz) + k] = (uint8_t)qlz;
+  node[offsetof(QNode8, qhiz) + k] = (uint8_t)qhz;
+  const uint32_t imask = (uint32_t)group_sum(inner ? (1u << k) : 0u);
+  if (k == 0) {
+    uint32_t* hdr = reinterpret_cast<uint32_t*>(node);
+    hdr[0] = __float_as_uint(p[0]);
+    hdr[1] = __float_as_uint(p[1]);
+    hdr[2] = __float_as_uint(p[2]);
+    hdr[3] = e[0] | (e[1] << 8) | (e[2] << 16) | (imask << 24);
+    hdr[4] = child_base;
+    hdr[5] = tri_base;
+    NodeInfo inf;
+    inf.lo[0] = ul[0]; inf.lo[1] = ul[1]; inf.lo[2] = ul[2];
+    inf.hi[0] = uh[0]; inf.hi[1] = uh[1]; inf.hi[2] = uh[2];
+    inf.count = ninner_prims + nt;
+    inf.depth = item.depth;
+    a.info[item.node] = inf;
+  }
+}
+
 // Leaf record r: the primitive at position leaf_src[r] (primref vals[pos],
 // whose .w fields hold its geometry and primitive index) as a TriRec
 // (v0, v0 - v1, v2 - v0; the three vertices for the robust test) or as the
@@ -988,11 +1221,23 @@ __global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
   if (threadIdx.x == 0) s_run = a.cnt->error ? 0u : 1u;
   __syncthreads();
   const uint32_t nin = *a.nin;
-  if (s_run && blockIdx.x * BT < nin) {
+  // few items: a group of CG lanes per item, which shortens each item's
+  // chain of work; many: a thread per item, fewer workgroup steps (each
+  // step's block_alloc atomics serialise on their addresses)
+  const bool grouped = nin <= COLLAPSE_GROUPED_MAX;
+  if (s_run && blockIdx.x * (grouped ? BT / CG : BT) < nin) {
     CollapseStats st;
-    // block-uniform loop: every thread takes part in each item's block_alloc
-    for (uint32_t base = blockIdx.x * BT; base < nin; base += gridDim.x * BT)
-      collapse_item(a, base + threadIdx.x, base + threadIdx.x < nin, st);
+    // block-uniform loops: every thread takes part in each item's block_alloc
+    if (grouped) {
+      constexpr uint32_t PER_BLOCK = BT / CG;
+      for (uint32_t base = blockIdx.x * PER_BLOCK; base < nin; base += gridDim.x * PER_BLOCK) {
+        const uint32_t t = base + threadIdx.x / CG;
+        collapse_item(a, t, t < nin, threadIdx.x % CG, st);
+      }
+    } else {
+      for (uint32_t base = blockIdx.x * BT; base < nin; base += gridDim.x * BT)
+        collapse_item1(a, base + threadIdx.x, base + threadIdx.x < nin, st);
+    }
     const uint32_t md = wave_umax(st.max_depth), ms = wave_umax(st.max_stack);
     const double sn = wave_sum(st.sah_nodes), sl = wave_sum(st.sah_leaves), slt = wave_sum(st.sah_leaves_tri);
     if ((threadIdx.x & 63u) == 0u) {
@@ -1297,6 +1542,7 @@ void build_bvh(const std::vector<GeomDesc>& geoms, const BuildParams& prm, Build
   // one launch per level of the 8-wide tree; the kernels read their queue
   // lengths on the device, so the host queues level k + 1 before it learns
   // (through a pinned slot) whether level k produced any items
+  // 2048 workgroups: a thread per item covers the large levels in one step
   const uint32_t cgrid = std::min<uint32_t>(div_up(cap, BT), 2048u);
   auto enqueue_level = [&](uint32_t k) {
     const uint32_t o = k & 1u;
