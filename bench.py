@@ -95,8 +95,11 @@ TRACE_KERNEL = "k_trace_p<false, true, false, 8, false, false>"
 
 def profiled_traffic():
     """HBM-side bytes per launch of the closest-hit kernel (FETCH_SIZE and
-    WRITE_SIZE scaled by a plain copy of known size) and its L2 hit rate,
-    from the committed rocprofv3 summary of this workload."""
+    WRITE_SIZE scaled by a plain copy of known size), its L2 hit rate and
+    what limits it (SQ counters: instructions per ray, active lanes per
+    VALU instruction), from the committed rocprofv3 summary of this
+    workload."""
+    traffic = hit = limiter = None
     try:
         for line in open(PROFILE):
             if TRACE_KERNEL in line and "GB read" in line:
@@ -104,10 +107,16 @@ def profiled_traffic():
                 rd = float(tail.split("GB read")[0])
                 wr = float(tail.split("GB read")[1].split("GB written")[0])
                 hit = float(tail.split("L2 hit")[1])
-                return (rd + wr) * 1e9, hit
+                traffic = (rd + wr) * 1e9
+            if line.strip().startswith("VALU instructions per ray"):
+                f = line.replace(",", " ").replace(";", " ").split()
+                limiter = {"kind": "VALU issue (SQ counters: VALU busy, divergent lanes)",
+                           "valu_instr_per_ray": float(f[4]), "salu_instr_per_ray": float(f[6]),
+                           "lanes_active_per_valu_instr": float(f[f.index("instruction") + 1]),
+                           "source": os.path.relpath(PROFILE, ROOT)}
     except (OSError, ValueError, IndexError):
         pass
-    return None, None
+    return traffic, hit, limiter
 
 
 def build_bytes_per_tri(n):
@@ -121,15 +130,17 @@ def build_bytes_per_tri(n):
       each big level (segments above 4096): id + box read to bin and
         again to flag, flags written and scanned, id + box + flag +
         scan read and id + box + value + segment id written            176
-      each subtree level: id + box read to bin and to partition,
-        id + box + value written                                       112
+      each mid or small level (segments of 17..4096): box read to bin
+        and to flag, id + box read and id + box + value written        140
+      tiny segments (2..16): id + box read, value + links written        56
       boxes in final order                                              68
       bottom-up boxes: box read and written per inner node              64
-      collapse: vertices read, 48 B triangle record written, 80 B node
-        per ~2.9 triangles                                             110
-    Big levels: ceil(log2(n / 4096)); subtree levels: log2(4096 / 16) = 8."""
+      collapse: a 128 B node slot per ~5.9 triangles                    22
+      leaf records: position written and read, value, primitive
+        reference, 3 indices and 3 vertices read, 48 B record written  140
+    Big levels: ceil(log2(n / 4096)); mid and small levels: log2(4096 / 16) = 8."""
     big = max(0, int(np.ceil(np.log2(max(1.0, n / 4096.0)))))
-    return 84 + 40 + 64 + 68 + 72 + 176 * big + 112 * 8 + 68 + 64 + 110, big
+    return 84 + 40 + 64 + 68 + 72 + 176 * big + 140 * 8 + 56 + 68 + 64 + 22 + 140, big
 
 
 def time_trace(L, dev, sc, d_rays, d_src, n, steps):
@@ -517,7 +528,7 @@ def main():
     nbar, tbar = float(cn.mean()), float(ct.mean())
     bytes_per_ray = 48 + 36 + nbar * node_b + tbar * tri_b
     achieved = bytes_per_ray * n / (ms_step / 1e3) / 1e9
-    traffic, l2hit = profiled_traffic()
+    traffic, l2hit, limiter = profiled_traffic()
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": round(traffic) if traffic and n == NUM_RAYS and args.tris == NUM_TRIS else None,
@@ -525,7 +536,8 @@ def main():
                 "l2_hit_rate": l2hit,
                 "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nbar, 2),
                 "tris_per_ray": round(tbar, 2),
-                "node_bytes_per_ray": round(nbar * node_b, 1)}
+                "node_bytes_per_ray": round(nbar * node_b, 1),
+                "limiter": limiter}
 
     b_tri, big_levels = build_bytes_per_tri(st["num_prims"])
     b_ach = b_tri * st["num_prims"] / (build_ms / 1e3) / 1e9
@@ -574,7 +586,7 @@ def main():
         "config": {"workload": "random triangle soup, 2^20 tris in [0,1]^3 (offsets +-0.005), 2^24 incoherent "
                                "rays, closest hit tnear=0 tfar=inf; occlusion pass tfar=0.1",
                    "tris": int(st["num_prims"]), "rays": n, "builder": "binned SAH + SAH-optimal 8-wide collapse",
-                   "bvh": "8-wide quantised nodes 80 B, triangles 48 B"},
+                   "bvh": "8-wide quantised nodes 80 B in 128 B slots, triangles 48 B"},
         "build_mtris_per_s": round(st["num_prims"] / (build_ms / 1e3) / 1e6, 2),
         "build_ms": round(build_ms, 3),
         "build_mtris_per_s_morton": round(st["num_prims"] / (build_ms_morton / 1e3) / 1e6, 2),

@@ -1661,7 +1661,7 @@ RTC_API void rtcgGetSceneStats(RTCScene scene, RTCGSceneStats* out) {
   out->num_prims = st.num_prims;
   out->num_nodes = st.num_nodes;
   out->max_depth = st.max_depth;
-  out->node_bytes = sizeof(rtg::QNode8);
+  out->node_bytes = (unsigned)rtg::QNODE_BYTES;
   out->tri_bytes = sizeof(rtg::TriRec);
   out->max_stack = st.max_stack;
   out->sah = st.sah;
@@ -1679,13 +1679,18 @@ RTC_API void rtcgReadBVH(RTCScene scene, void* nodes, void* tris) {
   d->activate();
   ThreadDev& t = d->td();
   const rtg::BuildStats& st = s->bvh->stats;
+  // nodes are handed out packed, QNODE_BYTES each (the device keeps each in
+  // a 128 B slot)
+  std::vector<rtg::QNode8> slots(nodes ? st.num_nodes : 0);
   if (nodes && st.num_nodes)
-    RTG_HIP(hipMemcpyAsync(nodes, s->bvh->nodes.ptr, (size_t)st.num_nodes * sizeof(rtg::QNode8), hipMemcpyDeviceToHost,
-                           t.stream));
+    RTG_HIP(hipMemcpyAsync(slots.data(), s->bvh->nodes.ptr, (size_t)st.num_nodes * sizeof(rtg::QNode8),
+                           hipMemcpyDeviceToHost, t.stream));
   if (tris && st.num_prims)
     RTG_HIP(hipMemcpyAsync(tris, s->bvh->tris.ptr, (size_t)st.num_prims * sizeof(rtg::TriRec), hipMemcpyDeviceToHost,
                            t.stream));
   RTG_HIP(hipStreamSynchronize(t.stream));
+  for (size_t i = 0; i < slots.size(); i++)
+    memcpy((char*)nodes + i * rtg::QNODE_BYTES, &slots[i], rtg::QNODE_BYTES);
   API_END(s)
 }
 

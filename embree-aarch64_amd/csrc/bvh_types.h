@@ -2,15 +2,19 @@
 // traversal kernels (host and device compile this header).
 //
 // Layout in HBM (one scene):
-//   nodes : QNode8[num_nodes], 80 B each, breadth-first; the root is node 0
-//           (node s for scene s of a batched build); the internal children
-//           of a node are contiguous from child_base.
+//   nodes : QNode8[num_nodes], 80 B of node in each 128 B slot (one slot
+//           per 128 B cache line, so a node visit never touches two lines),
+//           breadth-first; the root is node 0 (node s for scene s of a
+//           batched build); the internal children of a node are contiguous
+//           from child_base.
 //   tris  : TriRec[num_tris], 48 B each; the leaf children of a node are
 //           contiguous from tri_base; a leaf ends at the record whose
 //           TRI_LAST flag is set.
-// A ray visit reads one QNode8 per node and one TriRec per triangle test,
-// so the algorithmic bytes per ray are 48 (ray) + 36 (hit) + 80 n_node + 48 n_tri.
+// A ray visit reads the 80 B of one QNode8 per node and one TriRec per
+// triangle test, so the algorithmic bytes per ray are 48 (ray) + 36 (hit) +
+// 80 n_node + 48 n_tri.
 #pragma once
+#include <stddef.h>
 #include <stdint.h>
 
 namespace rtg {
@@ -22,7 +26,7 @@ namespace rtg {
 // conservative. Slot c is empty when meta[c] == 0; otherwise it is an
 // internal child (imask bit c) at child_base + (meta[c] & 0x7f) or a leaf
 // starting at tri_base + (meta[c] & 0x7f).
-struct alignas(16) QNode8 {
+struct alignas(128) QNode8 {
   float px, py, pz;
   uint8_t ex, ey, ez;
   uint8_t imask;
@@ -32,8 +36,10 @@ struct alignas(16) QNode8 {
   uint8_t qlox[8], qhix[8];
   uint8_t qloy[8], qhiy[8];
   uint8_t qloz[8], qhiz[8];
+  uint8_t pad[48];  // to the 128 B line
 };
-static_assert(sizeof(QNode8) == 80, "QNode8 must be 80 bytes");
+constexpr size_t QNODE_BYTES = 80;  // the part a visit reads
+static_assert(sizeof(QNode8) == 128 && offsetof(QNode8, pad) == QNODE_BYTES, "QNode8: 80 B in a 128 B slot");
 
 
 // Triangle record in Moeller-ready form, e1 = v0 - v1 and e2 = v2 - v0

@@ -73,7 +73,7 @@ struct RTCGSceneStats
   unsigned int num_prims;      /* valid triangles in the BVH */
   unsigned int num_nodes;      /* 8-wide nodes */
   unsigned int max_depth;      /* node levels */
-  unsigned int node_bytes;     /* bytes per node record */
+  unsigned int node_bytes;     /* bytes a visit reads of a node record (80; a node sits in a 128 B slot) */
   unsigned int tri_bytes;      /* bytes per triangle record */
   unsigned int max_stack;      /* traversal stack entries the deepest path can need */
   double sah;                  /* SAH statistic, leaf cost per block of 4 triangles */
@@ -82,8 +82,9 @@ struct RTCGSceneStats
 };
 RTC_API void rtcgGetSceneStats(RTCScene scene, struct RTCGSceneStats* stats);
 
-/* Copies the device BVH to host memory: nodes (80 B each) and triangle
- * records (48 B each); sizes from rtcgGetSceneStats. Either pointer may be NULL. */
+/* Copies the device BVH to host memory: nodes packed at node_bytes (80 B)
+ * each and triangle records (48 B each); sizes from rtcgGetSceneStats.
+ * Either pointer may be NULL. */
 RTC_API void rtcgReadBVH(RTCScene scene, void* nodes, void* tris);
 
 /* ---- workload generators (host memory, deterministic) ---------------- */
