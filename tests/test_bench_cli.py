@@ -20,6 +20,7 @@ def test_more_than_one_gpu_is_refused():
 def test_profiled_traffic_parses_the_committed_summary():
     sys.path.insert(0, ROOT)
     import bench
-    traffic, l2hit = bench.profiled_traffic()
+    traffic, l2hit, limiter = bench.profiled_traffic()
     assert traffic is not None and 1e10 < traffic < 2e11
     assert 0.0 < l2hit < 1.0
+    assert 100 < limiter["valu_instr_per_ray"] < 1000 and 1 < limiter["lanes_active_per_valu_instr"] <= 64
