@@ -239,6 +239,64 @@ def learned_vs_sah(grid, width, height, bounces, train_steps, steps):
             "build_ms_plain": round(build_plain, 2), "build_ms_learned": round(build_learned, 2)}
 
 
+def learned_point_source(steps):
+    """A ray distribution far from the uniform one the SAH assumes: 2^22 rays
+    from the point (0.5, 0.5, -2) into the central 0.3-wide cube of the
+    2^20-triangle soup. The split model (200 Adam steps, SAH slack 0.5) is
+    trained on the even-indexed rays; both trees are timed and counted on
+    the odd-indexed ones (held out)."""
+    L = R.lib()
+    v, ix = R.gen_soup(NUM_TRIS, seed=1, h=0.005)
+    rng = np.random.default_rng(7)
+    nq = 1 << 22
+    tgt = 0.35 + 0.3 * rng.random((nq, 3), dtype=np.float32)
+    org = np.array([0.5, 0.5, -2.0], np.float32)
+    d = tgt - org
+    Q = R.new_rayhits(nq)
+    Q["org_x"], Q["org_y"], Q["org_z"] = org
+    Q["dir_x"], Q["dir_y"], Q["dir_z"] = d.T
+    train, held = np.ascontiguousarray(Q[::2]), np.ascontiguousarray(Q[1::2])
+    dev = R.Device("builder=sah")
+    plain = R.Scene(dev, [(v, ix)]).commit()
+    n, nbytes = len(held), held.nbytes
+    d_src, d_rays, d_train = dev.malloc(nbytes), dev.malloc(nbytes), dev.malloc(train.nbytes)
+    dev.memcpy(d_src, held.ctypes.data, nbytes)
+    dev.memcpy(d_train, train.ctypes.data, train.nbytes)
+    model = R.SplitModel(dev, seed=1)
+    model.collect(plain, d_train, len(train))
+    loss = model.train(200, batch=65536, lr=3e-3, seed=1)
+    dev.free(d_train)
+    L.rtcgSetSplitModelSahSlack(model.h, ctypes.c_float(0.5))
+    learned = R.Scene(dev, [(v, ix)])
+    L.rtcgSetSceneSplitModel(learned.h, model.h)
+    learned.commit()
+    times = {"plain": [], "learned": []}
+    for name, sc in (("plain", plain), ("learned", learned)):
+        time_trace(L, dev, sc, d_rays, d_src, n, 2)
+    for _ in range(3):
+        for name, sc in (("plain", plain), ("learned", learned)):
+            times[name].append(time_trace(L, dev, sc, d_rays, d_src, n, steps))
+    res, outs = {}, {}
+    for name, sc in (("plain", plain), ("learned", learned)):
+        nb, tb, outs[name] = counted(L, dev, sc, d_rays, d_src, n, nbytes)
+        ms = float(np.median(times[name]))
+        res[name] = {"mrays_per_s": round(n / (ms / 1e3) / 1e6, 2), "nodes_per_ray": round(nb, 3),
+                     "tris_per_ray": round(tb, 3), "sah": round(sc.stats()["sah"], 3)}
+    same = bool(np.array_equal(outs["plain"].view(np.uint8), outs["learned"].view(np.uint8)))
+    for p_ in (d_src, d_rays):
+        dev.free(p_)
+    model.release()
+    plain.release()
+    learned.release()
+    dev.release()
+    return {"workload": "2^20-triangle soup, 2^22 rays from (0.5, 0.5, -2) into the central 0.3-wide cube; trained on "
+                        "the even-indexed, timed and counted on the odd-indexed (held out); 200 steps, SAH slack 0.5",
+            "plain_sah": res["plain"], "learned": res["learned"],
+            "node_ratio": round(res["learned"]["nodes_per_ray"] / res["plain"]["nodes_per_ray"], 4),
+            "speedup": round(res["learned"]["mrays_per_s"] / res["plain"]["mrays_per_s"], 4),
+            "hits_identical": same, "train_loss": round(loss, 4)}
+
+
 def batched_scenes(num_scenes, steps):
     """Batched scenes: num_scenes displaced UV spheres (3-octave value noise,
     amplitude 0.1) with triangle counts log-uniform in [1e3, 2e6] and ray
@@ -551,10 +609,12 @@ def main():
     sc.release()
     dev.release()
 
-    learned = None
+    learned = learned_ps = None
     if not args.no_learned:
         learned = learned_vs_sah(args.hf_grid, args.hf_width, args.hf_height, args.hf_depth, args.train_steps,
                                  max(3, args.steps // 2))
+        log("learned split cost, point source")
+        learned_ps = learned_point_source(max(3, args.steps // 2))
 
     batch = None
     if not args.no_batch:
@@ -601,6 +661,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "learned_vs_sah": learned,
+        "learned_point_source": learned_ps,
         "batched_scenes": batch,
         "instanced": inst,
     }

@@ -15,6 +15,7 @@
 #pragma once
 #include "bvh_types.h"
 #include "gpu_util.h"
+#include "sort.h"
 
 namespace rtg {
 
@@ -123,6 +124,9 @@ struct SplitModel {
   DevBuf params, m, v, grad, packed, loss;  // FP32 master, Adam moments, gradient, inference form
   DevBuf X, T, W;                           // training set: features, targets, sample weights
   DevBuf idx, visits, counts, scount;       // minibatch indices, visit logging scratch, appended-sample count
+  DevBuf part, lpart;                       // per-block gradient and loss partial sums
+  DevBuf skey, skey2, sval, xtmp, ttmp, wtmp;  // canonical ordering of appended samples
+  RadixSorter sorter;
   uint32_t nsamples = 0;
   uint32_t adam_t = 0;
   SceneNorm norm{};                         // of the scene the data came from
@@ -142,11 +146,16 @@ float mlp_gradient(SplitModel& m, const float* d_x, const float* d_t, const floa
 // MLP_MIN_VISITS rays: split_features of the child's box and count against
 // the parent's (info: NodeInfo per node) and the target
 // log2((v_C + 0.5) / (v_S + 0.5)) - log2(A_C / A_S + 2^-30), where v counts
-// the rays that entered a node.
+// the rays that entered a node. The appended samples are put in an order
+// fixed by their contents (a 64-bit hash of features and target), so that
+// the node numbering of the tree they came from, which the build does not
+// fix, does not change the minibatches training draws.
 constexpr uint32_t MLP_MIN_VISITS = 8;
 void mlp_append_samples(SplitModel& m, const QNode8* d_nodes, const NodeInfo* d_info, const uint32_t* d_visits,
                         uint32_t nnodes, const SceneNorm& sn, hipStream_t s);
 // Adam on random minibatches of the training set; returns the last loss.
+// The gradient sums each block's samples and then the blocks in index
+// order, so training is deterministic for a given training set and seed.
 float mlp_train(SplitModel& m, uint32_t steps, uint32_t batch, float lr, uint32_t seed, hipStream_t s);
 
 }  // namespace rtg

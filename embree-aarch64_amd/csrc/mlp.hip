@@ -68,7 +68,7 @@ __global__ void k_mlp_predict(const MlpPacked* __restrict__ P, const float* __re
 __global__ __launch_bounds__(WB) void k_mlp_grad(const MlpPacked* __restrict__ P, const float* __restrict__ X,
                                                  const float* __restrict__ T, const float* __restrict__ W,
                                                  const uint32_t* __restrict__ idx, uint32_t n, float scale,
-                                                 float* grad, float* loss) {
+                                                 float* __restrict__ part, float* __restrict__ lpart) {
   __shared__ float s_x[WB][MLP_IN];
   __shared__ float s_h1[WB][ROW], s_d1[WB][ROW], s_h2[WB][ROW], s_d2[WB][ROW];
   __shared__ float s_dy[WB];
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(WB) void k_mlp_grad(const MlpPacked* __restrict__ P
   float l = wt * e * e;
 #pragma unroll
   for (int d = 32; d >= 1; d >>= 1) l += __shfl_xor(l, d, 64);
-  if (lane == 0) atomicAdd(loss, l);
+  if (lane == 0) lpart[blockIdx.x] = l;
   __syncthreads();
   // parameter gradients summed over the block's samples
   for (int p = lane; p < MLP_NPARAM; p += WB) {
@@ -159,8 +159,62 @@ __global__ __launch_bounds__(WB) void k_mlp_grad(const MlpPacked* __restrict__ P
     } else {
       for (int q = 0; q < WB; q++) s += s_dy[q];
     }
-    if (s != 0.0f) atomicAdd(&grad[p], s);
+    part[(size_t)blockIdx.x * MLP_NPARAM + p] = s;
   }
+}
+
+// Sums the blocks' partial gradients (and losses) in block order.
+__global__ void k_mlp_reduce(const float* __restrict__ part, const float* __restrict__ lpart, uint32_t nblocks,
+                             float* __restrict__ grad, float* __restrict__ loss) {
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p < MLP_NPARAM) {
+    float s = 0.0f;
+    for (uint32_t b = 0; b < nblocks; b++) s += part[(size_t)b * MLP_NPARAM + p];
+    grad[p] = s;
+  } else if (p == MLP_NPARAM) {
+    float s = 0.0f;
+    for (uint32_t b = 0; b < nblocks; b++) s += lpart[b];
+    *loss = s;
+  }
+}
+
+// 64-bit content key of sample i (features then target, as bits): two
+// independent 32-bit mixes.
+__device__ inline uint32_t mix32(uint32_t h, uint32_t v) {
+  h ^= v * 0x9E3779B1u;
+  h = (h ^ (h >> 15)) * 0x85EBCA77u;
+  return h ^ (h >> 13);
+}
+__global__ void k_mlp_sample_keys(const float* __restrict__ X, const float* __restrict__ T, uint32_t n,
+                                  uint32_t* __restrict__ klo, uint32_t* __restrict__ khi, uint32_t* __restrict__ val) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t a = 0x243F6A88u, b = 0xB7E15162u;
+  for (int k = 0; k < MLP_IN; k++) {
+    const uint32_t v = __float_as_uint(X[(size_t)i * MLP_IN + k]);
+    a = mix32(a, v);
+    b = mix32(b, v ^ 0x5bd1e995u);
+  }
+  a = mix32(a, __float_as_uint(T[i]));
+  b = mix32(b, __float_as_uint(T[i]) ^ 0x5bd1e995u);
+  klo[i] = a;
+  khi[i] = b;
+  val[i] = i;
+}
+__global__ void k_gather_u32(const uint32_t* __restrict__ src, const uint32_t* __restrict__ idx, uint32_t n,
+                             uint32_t* __restrict__ dst) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+__global__ void k_mlp_permute(const float* __restrict__ X, const float* __restrict__ T, const float* __restrict__ W,
+                              const uint32_t* __restrict__ idx, uint32_t n, float* __restrict__ Xo,
+                              float* __restrict__ To, float* __restrict__ Wo) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t j = idx[i];
+  for (int k = 0; k < MLP_IN; k++) Xo[(size_t)i * MLP_IN + k] = X[(size_t)j * MLP_IN + k];
+  To[i] = T[j];
+  Wo[i] = W[j];
 }
 
 __global__ void k_mlp_adam(float* p, float* m, float* v, float* g, float lr, float bc1, float bc2, MlpPacked* P) {
@@ -263,15 +317,27 @@ void mlp_predict(SplitModel& m, const float* d_x, uint32_t n, float* d_y, hipStr
   RTG_HIP(hipGetLastError());
 }
 
+// Gradient and loss of n samples into m.grad and m.loss (block partial sums,
+// then their sum in block order).
+void grad_step(SplitModel& m, const float* d_x, const float* d_t, const float* d_w, const uint32_t* d_idx, uint32_t n,
+               hipStream_t s) {
+  const uint32_t nb = div_up(n, WB);
+  m.part.reserve((size_t)nb * MLP_NPARAM * 4);
+  m.lpart.reserve((size_t)nb * 4);
+  hipLaunchKernelGGL(k_mlp_grad, dim3(nb), dim3(WB), 0, s, m.packed.as<MlpPacked>(), d_x, d_t, d_w, d_idx, n,
+                     2.0f / (float)n, m.part.as<float>(), m.lpart.as<float>());
+  hipLaunchKernelGGL(k_mlp_reduce, dim3(div_up(MLP_NPARAM + 1, 256)), dim3(256), 0, s, m.part.as<float>(),
+                     m.lpart.as<float>(), nb, m.grad.as<float>(), m.loss.as<float>());
+  RTG_HIP(hipGetLastError());
+}
+
 float mlp_gradient(SplitModel& m, const float* d_x, const float* d_t, const float* d_w, const uint32_t* d_idx,
                    uint32_t n, hipStream_t s) {
   ensure(m);
   RTG_HIP(hipMemsetAsync(m.grad.ptr, 0, MLP_NPARAM * 4, s));
   RTG_HIP(hipMemsetAsync(m.loss.ptr, 0, 4, s));
   if (n == 0) return 0.0f;
-  hipLaunchKernelGGL(k_mlp_grad, dim3(div_up(n, WB)), dim3(WB), 0, s, m.packed.as<MlpPacked>(), d_x, d_t, d_w, d_idx, n,
-                     2.0f / (float)n, m.grad.as<float>(), m.loss.as<float>());
-  RTG_HIP(hipGetLastError());
+  grad_step(m, d_x, d_t, d_w, d_idx, n, s);
   float l = 0.0f;
   RTG_HIP(hipMemcpyAsync(&l, m.loss.ptr, 4, hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));
@@ -307,7 +373,34 @@ void mlp_append_samples(SplitModel& m, const QNode8* d_nodes, const NodeInfo* d_
   uint32_t added = 0;
   RTG_HIP(hipMemcpyAsync(&added, cnt, 4, hipMemcpyDeviceToHost, s));
   RTG_HIP(hipStreamSynchronize(s));
-  m.nsamples += added < room ? added : (uint32_t)room;
+  if (added > room) added = (uint32_t)room;
+  if (added > 1) {
+    // the new samples in the order of their 64-bit content keys (hi, lo)
+    float* X0 = m.X.as<float>() + (size_t)m.nsamples * MLP_IN;
+    float* T0 = m.T.as<float>() + m.nsamples;
+    float* W0 = m.W.as<float>() + m.nsamples;
+    m.skey.reserve((size_t)added * 4);
+    m.skey2.reserve((size_t)added * 4);
+    m.sval.reserve((size_t)added * 4);
+    m.xtmp.reserve((size_t)added * MLP_IN * 4);
+    m.ttmp.reserve((size_t)added * 4);
+    m.wtmp.reserve((size_t)added * 4);
+    uint32_t* klo = m.skey.as<uint32_t>();
+    uint32_t* khi = m.skey2.as<uint32_t>();
+    uint32_t* val = m.sval.as<uint32_t>();
+    hipLaunchKernelGGL(k_mlp_sample_keys, dim3(div_up(added, 256)), dim3(256), 0, s, X0, T0, added, klo, khi, val);
+    m.sorter.sort_pairs(klo, val, added, 0, 32, s);
+    // the high keys in the low keys' order, then a stable sort on them
+    hipLaunchKernelGGL(k_gather_u32, dim3(div_up(added, 256)), dim3(256), 0, s, khi, val, added, klo);
+    m.sorter.sort_pairs(klo, val, added, 0, 32, s);
+    hipLaunchKernelGGL(k_mlp_permute, dim3(div_up(added, 256)), dim3(256), 0, s, X0, T0, W0, val, added,
+                       m.xtmp.as<float>(), m.ttmp.as<float>(), m.wtmp.as<float>());
+    RTG_HIP(hipMemcpyAsync(X0, m.xtmp.ptr, (size_t)added * MLP_IN * 4, hipMemcpyDeviceToDevice, s));
+    RTG_HIP(hipMemcpyAsync(T0, m.ttmp.ptr, (size_t)added * 4, hipMemcpyDeviceToDevice, s));
+    RTG_HIP(hipMemcpyAsync(W0, m.wtmp.ptr, (size_t)added * 4, hipMemcpyDeviceToDevice, s));
+    RTG_HIP(hipGetLastError());
+  }
+  m.nsamples += added;
 }
 
 float mlp_train(SplitModel& m, uint32_t steps, uint32_t batch, float lr, uint32_t seed, hipStream_t s) {
@@ -319,10 +412,7 @@ float mlp_train(SplitModel& m, uint32_t steps, uint32_t batch, float lr, uint32_
   for (uint32_t it = 0; it < steps; it++) {
     hipLaunchKernelGGL(k_mlp_batch, dim3(div_up(batch, 256)), dim3(256), 0, s, m.idx.as<uint32_t>(), batch, m.nsamples,
                        seed * 7919u + it);
-    RTG_HIP(hipMemsetAsync(m.loss.ptr, 0, 4, s));
-    hipLaunchKernelGGL(k_mlp_grad, dim3(div_up(batch, WB)), dim3(WB), 0, s, m.packed.as<MlpPacked>(), m.X.as<float>(),
-                       m.T.as<float>(), m.W.as<float>(), m.idx.as<uint32_t>(), batch, 2.0f / (float)batch,
-                       m.grad.as<float>(), m.loss.as<float>());
+    grad_step(m, m.X.as<float>(), m.T.as<float>(), m.W.as<float>(), m.idx.as<uint32_t>(), batch, s);
     m.adam_t++;
     const float bc1 = 1.0f - powf(0.9f, (float)m.adam_t), bc2 = 1.0f - powf(0.999f, (float)m.adam_t);
     hipLaunchKernelGGL(k_mlp_adam, dim3(div_up(MLP_NPARAM, 256)), dim3(256), 0, s, m.params.as<float>(),

@@ -159,3 +159,72 @@ def test_sah_slack_bounds_the_tree(rtc, device):
         device.free(d)
         m.release()
         sc.release()
+
+
+def _counted_nodes(rtc, device, sc, rays):
+    """Mean node visits per ray (the library's counted traversal) and the
+    records it leaves."""
+    import ctypes  # noqa: F401
+    n = len(rays)
+    d_rays, d_cn, d_ct = device.malloc(rays.nbytes), device.malloc(4 * n), device.malloc(4 * n)
+    try:
+        device.memcpy(d_rays, rays.ctypes.data, rays.nbytes)
+        rtc.lib().rtcgTraceCounted(sc.h, d_rays, n, 80, 0, d_cn, d_ct)
+        device.check("rtcgTraceCounted")
+        cn = np.zeros(n, np.uint32)
+        device.memcpy(cn.ctypes.data, d_cn, 4 * n)
+        out = np.zeros(n, rays.dtype)
+        device.memcpy(out.ctypes.data, d_rays, rays.nbytes)
+        return float(cn.mean()), out
+    finally:
+        for p in (d_rays, d_cn, d_ct):
+            device.free(p)
+
+
+def test_held_out_gain_for_a_point_source(rtc, device, oracle):
+    """Where the rays are far from the uniform distribution the SAH assumes,
+    a model trained on half of them builds a tree that the other half (held
+    out) crosses with at least 10 % fewer node visits than the plain
+    binned-SAH tree, with the same records, equal to the oracle's on a
+    sample. Workload: the 2^20-triangle soup, 2^22 rays from the point
+    (0.5, 0.5, -2) into the central 0.3-wide cube; even rays train, odd
+    rays are timed. Training is deterministic (samples in content order,
+    gradients summed in a fixed order), so the ratio is reproducible: 0.826
+    measured with 200 Adam steps at SAH slack 0.5."""
+    import ctypes
+    v, ix = rtc.gen_soup(1 << 20, seed=1, h=0.005)
+    rng = np.random.default_rng(7)
+    n = 1 << 22
+    tgt = 0.35 + 0.3 * rng.random((n, 3), dtype=np.float32)
+    org = np.array([0.5, 0.5, -2.0], np.float32)
+    d = tgt - org
+    Q = rtc.new_rayhits(n)
+    Q["org_x"], Q["org_y"], Q["org_z"] = org
+    Q["dir_x"], Q["dir_y"], Q["dir_z"] = d.T
+    train, held = np.ascontiguousarray(Q[::2]), np.ascontiguousarray(Q[1::2])
+    plain = rtc.Scene(device, [(v, ix)]).commit()
+    m = rtc.SplitModel(device, seed=1)
+    d_train = device.malloc(train.nbytes)
+    try:
+        device.memcpy(d_train, train.ctypes.data, train.nbytes)
+        m.collect(plain, d_train, len(train))
+        m.train(200, batch=65536, lr=3e-3, seed=1)
+        rtc.lib().rtcgSetSplitModelSahSlack(m.h, ctypes.c_float(0.5))
+        learned = rtc.Scene(device, [(v, ix)])
+        rtc.lib().rtcgSetSceneSplitModel(learned.h, m.h)
+        learned.commit()
+        pn, pout = _counted_nodes(rtc, device, plain, held)
+        ln, lout = _counted_nodes(rtc, device, learned, held)
+        print(f"point source, held out: plain {pn:.3f} nodes/ray, learned {ln:.3f} ({ln / pn:.4f})")
+        assert ln <= 0.9 * pn, (ln, pn)
+        assert np.array_equal(lout.view(np.uint8), pout.view(np.uint8))
+        sample = np.ascontiguousarray(held[::512])
+        ref = sample.copy()
+        oracle.BVH([(v, ix)]).intersect(ref)
+        bad = mismatches(np.ascontiguousarray(lout[::512]), ref)
+        assert len(bad) == 0, bad[:5]
+        learned.release()
+    finally:
+        device.free(d_train)
+        m.release()
+        plain.release()
