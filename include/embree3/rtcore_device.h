@@ -20,7 +20,9 @@ typedef struct RTCDeviceTy* RTCDevice;
  * RTC_ERROR_INVALID_ARGUMENT, as the reference's factory does
  * (bvh8_factory.cpp:878); bvh_leaf / max_leaf, bvh_width, cost_node,
  * cost_tri, sah_block (leaves priced per block of that many triangles, as
- * the reference's sahBlockSize; tree shape); trace_stack, refill_min, refill_min_occ,
+ * the reference's sahBlockSize; tree shape); spatial_budget (extra triangle
+ * references spatial splits may add at RTC_BUILD_QUALITY_HIGH, as a fraction
+ * of the triangles; default 0.3, 0 turns them off); trace_stack, refill_min, refill_min_occ,
  * refill_min_occ_inst, blocks_per_cu, persistent, dual, ray_sort
  * (traversal launch settings, as rtcgAutotune writes them). The reference's
  * CPU keys (threads, isa, set_affinity, ...) and unknown keys are ignored as
