@@ -370,7 +370,7 @@ def batched_scenes(num_scenes, steps):
             "generation_s": round(gen_s, 1)}
 
 
-def instanced_scene(num_instances, width, height, steps, check_rays=2048):
+def instanced_scene(num_instances, width, height, steps, check_rays=2048, learn=True):
     """Two-level scene: five base meshes (two displaced spheres of about 1M and
     200k triangles, three heightfield patches of about 1M, 200k and 100k)
     committed as scenes, and num_instances instances of them with a uniform
@@ -438,6 +438,7 @@ def instanced_scene(num_instances, width, height, steps, check_rays=2048):
     I.occluded_bvh(sref)
     same_occ = bool(np.array_equal(sref["tfar"], shadow[spick]["tfar"]))
     check_s = time.perf_counter() - t2
+    learned = instanced_learned(dev, top, objs, which, T, rays, sh, steps) if learn else None
     for q in (d_src, d_rays, d_ssrc, d_sh):
         dev.free(q)
     top.release()
@@ -454,7 +455,79 @@ def instanced_scene(num_instances, width, height, steps, check_rays=2048):
             "occlusion_mrays_per_s": round(ns / (o_ms / 1e3) / 1e6, 2) if ns else None,
             "hit_rate": round(float(hit.mean()), 4), "shadowed": round(float((shadow["tfar"] < 0).mean()), 4) if ns else None,
             "oracle_check": {"rays": int(len(pick)), "shadow_rays": int(len(spick)), "closest_identical": same_ch,
-                             "occlusion_identical": same_occ, "seconds": round(check_s, 1)}}
+                             "occlusion_identical": same_occ, "seconds": round(check_s, 1)},
+            "learned_both_levels": learned}
+
+
+def instanced_learned(dev, top, objs, which, T, rays, shadow, steps, train_steps=200):
+    """The learned split cost at both levels of the two-level scene: one
+    model per base mesh trained on the visits of all its instances, one for
+    the top tree over the instances, all from the even-indexed camera rays;
+    the plain and the learned scenes are timed on the odd-indexed camera rays
+    and their shadow rays (held out)."""
+    L = R.lib()
+    train = np.ascontiguousarray(rays[::2])
+    held = np.ascontiguousarray(rays[1::2])
+    hsh = np.ascontiguousarray(shadow[1::2])
+    d_train = dev.malloc(train.nbytes)
+    dev.memcpy(d_train, train.ctypes.data, train.nbytes)
+    t0 = time.perf_counter()
+    models = []
+    for o in objs:
+        m = R.SplitModel(dev, seed=1)
+        m.collect_instanced(top, o, d_train, len(train))
+        m.train(train_steps, batch=65536, lr=3e-3, seed=1)
+        models.append(m)
+    mt = R.SplitModel(dev, seed=1)
+    mt.collect_instanced(top, None, d_train, len(train))
+    mt.train(train_steps, batch=65536, lr=3e-3, seed=1)
+    dev.free(d_train)
+    train_s = time.perf_counter() - t0
+
+    def timed(sc, recs, stride, occluded):
+        d_src, d_r = dev.malloc(recs.nbytes), dev.malloc(recs.nbytes)
+        dev.memcpy(d_src, recs.ctypes.data, recs.nbytes)
+        warm = np.zeros(2, np.float32)
+        L.rtcgTimeTrace(sc.h, d_r, d_src, len(recs), stride, occluded, 2, warm.ctypes.data)
+        ms = np.zeros(steps, np.float32)
+        L.rtcgTimeTrace(sc.h, d_r, d_src, len(recs), stride, occluded, steps, ms.ctypes.data)
+        dev.check("rtcgTimeTrace learned instanced")
+        out = np.zeros(len(recs), recs.dtype)
+        dev.memcpy(out.ctypes.data, d_r, recs.nbytes)
+        dev.free(d_src)
+        dev.free(d_r)
+        return len(recs) / (float(np.median(ms)) / 1e3) / 1e6, out
+
+    res = {}
+    res["plain"] = [timed(top, held, 80, 0), timed(top, hsh, 48, 1)]
+    # the object scenes rebuilt with their models (the plain top tree keeps
+    # the trees it was committed with), then a top scene over them with its own
+    t1 = time.perf_counter()
+    for o, m in zip(objs, models):
+        L.rtcgSetSceneSplitModel(o.h, m.h)
+        o.commit()
+    top2 = R.Scene(dev)
+    for k in range(len(which)):
+        top2.add_instance(objs[which[k]], T[k])
+    L.rtcgSetSceneSplitModel(top2.h, mt.h)
+    top2.commit()
+    rebuild_s = time.perf_counter() - t1
+    res["learned"] = [timed(top2, held, 80, 0), timed(top2, hsh, 48, 1)]
+    same = bool(np.array_equal(res["plain"][0][1].view(np.uint8), res["learned"][0][1].view(np.uint8)) and
+                np.array_equal(res["plain"][1][1]["tfar"], res["learned"][1][1]["tfar"]))
+    top2.release()
+    for o in objs:
+        L.rtcgSetSceneSplitModel(o.h, None)
+    for m in models + [mt]:
+        m.release()
+    out = {"workload": f"models trained on the {len(train)} even-indexed camera rays; timed on the {len(held)} "
+                       f"odd-indexed camera rays and their {len(hsh)} shadow rays (held out); {train_steps} steps",
+           "hits_identical": same, "train_s": round(train_s, 2), "rebuild_s": round(rebuild_s, 2)}
+    for name in ("plain", "learned"):
+        out[name] = {"closest_mrays_per_s": round(res[name][0][0], 2), "occlusion_mrays_per_s": round(res[name][1][0], 2)}
+    out["closest_speedup"] = round(res["learned"][0][0] / res["plain"][0][0], 4)
+    out["occlusion_speedup"] = round(res["learned"][1][0] / res["plain"][1][0], 4)
+    return out
 
 
 def main():
@@ -623,7 +696,8 @@ def main():
     inst = None
     if not args.no_instances:
         log("instanced scene")
-        inst = instanced_scene(args.instances, args.inst_width, args.inst_height, max(3, args.steps // 4))
+        inst = instanced_scene(args.instances, args.inst_width, args.inst_height, max(3, args.steps // 4),
+                               learn=not args.no_learned)
 
     cpu = None
     if not args.no_cpu:

@@ -267,6 +267,8 @@ struct Scene : RefObj {
   std::vector<rtg::InstRec> top_recs;  // the top tree's records and their world boxes (lo xyz, hi xyz), as built
   std::vector<float> top_boxes;
   uint32_t top_obj_stack = 0;          // deepest object tree's traversal stack
+  std::vector<rtg::DevScene> top_object_views;  // the object table as uploaded
+  std::vector<Scene*> top_object_scenes;        // the scene of each object table entry
   float bounds_lo[3] = {INFINITY, INFINITY, INFINITY};
   float bounds_hi[3] = {-INFINITY, -INFINITY, -INFINITY};
   explicit Scene(Device* d) : dev(d) { ref_inc(d); }
@@ -700,6 +702,8 @@ void build_top(Scene* sc, Device* d, ThreadDev& t) {
   sc->top_recs = std::move(recs);
   sc->top_boxes = std::move(boxes);
   sc->top_obj_stack = obj_stack;
+  sc->top_object_views = objects;
+  sc->top_object_scenes = object_scene;
   for (Scene* o : object_scene)
     if (o != sc) {
       ref_inc(o);
@@ -1962,6 +1966,132 @@ RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const v
   const rtg::SceneNorm sn = m->m.has_norm ? m->m.norm : scene_norm(s->bvh->slots[s->slot]);
   rtg::mlp_append_samples(m->m, bin.nodes.as<rtg::QNode8>(), bin.info.as<rtg::NodeInfo>(), m->m.visits.as<uint32_t>(),
                           nn, sn, t.stream);
+  m->m.norm = sn;
+  m->m.has_norm = true;
+  RTG_HIP(hipStreamSynchronize(t.stream));
+  check_trace_error(t);
+  API_END(m)
+}
+
+// Two-level scenes: samples for the top tree (object == NULL) or for one
+// instanced scene's own tree, from the visits of the two-level traversal.
+// The tree collected on is a width-2 build of the same records or triangles
+// (its parent/child pairs are the binary splits the builder prices), swapped
+// into the two-level view for the one traversal.
+RTC_API void rtcgSplitModelCollectInstanced(RTCGSplitModel model, RTCScene scene, RTCScene object, const void* d_rays,
+                                            unsigned int M, size_t byteStride) {
+  SplitModelH* m = SM(model);
+  Scene* s = SC(scene);
+  Scene* o = object ? SC(object) : nullptr;
+  API_BEGIN
+  VERIFY_HANDLE(model);
+  VERIFY_HANDLE(scene);
+  if (M && !d_rays) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid argument");
+  if (!s->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
+  if (!s->two_level) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene has no instances: use rtcgSplitModelCollect");
+  if (byteStride < sizeof(RTCRayHit) || byteStride % 4) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "invalid stride");
+  check_top(s);
+  uint32_t target = 0xffffffffu;
+  if (o) {
+    for (size_t k = 0; k < s->top_object_scenes.size(); k++)
+      if (s->top_object_scenes[k] == o) target = (uint32_t)k;
+    if (target == 0xffffffffu) RTC_THROW(RTC_ERROR_INVALID_ARGUMENT, "object is not instanced by the scene");
+  }
+  if (M == 0) return;
+  std::lock_guard<std::mutex> lk(m->mtx);
+  std::lock_guard<std::mutex> slk(s->mtx);
+  Device* d = s->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  rtg::LearnedCost none{};
+  rtg::DeviceBVH bin;
+  rtg::DevBuf dboxes, drecs, dobjects;
+  rtg::DevTop v = s->top_view;
+  rtg::SceneNorm sn;
+  if (!o) {
+    // width-2 top tree over the scene's instance records
+    const uint32_t n = (uint32_t)s->top_recs.size();
+    dboxes.reserve((size_t)n * 24);
+    drecs.reserve((size_t)n * sizeof(rtg::InstRec));
+    RTG_HIP(hipMemcpyAsync(dboxes.ptr, s->top_boxes.data(), (size_t)n * 24, hipMemcpyHostToDevice, t.stream));
+    RTG_HIP(hipMemcpyAsync(drecs.ptr, s->top_recs.data(), (size_t)n * sizeof(rtg::InstRec), hipMemcpyHostToDevice,
+                           t.stream));
+    rtg::GeomDesc gd{};
+    gd.vertices = (const char*)dboxes.ptr;
+    gd.indices = (const char*)drecs.ptr;
+    gd.vstride = 24;
+    gd.istride = sizeof(rtg::InstRec);
+    gd.num_vertices = n;
+    gd.num_tris = n;
+    gd.kind = rtg::GEOM_INSTANCES;
+    rtg::BuildParams p = scene_params(s, none);
+    p.learned = nullptr;
+    p.builder = rtg::BUILDER_SAH;
+    p.max_leaf = 1;
+    p.width = 2;
+    p.split_budget = 0.0f;
+    {
+      std::lock_guard<std::mutex> bl(d->build_mtx);
+      rtg::build_bvh({gd}, p, d->scratch, bin, t.stream);
+    }
+    v.nodes = bin.nodes.as<rtg::QNode8>();
+    v.insts = bin.insts.as<rtg::InstRec>();
+    v.num_nodes = bin.stats.num_nodes;
+    v.root = bin.slots.empty() ? 0u : bin.slots[0].root;
+    if (bin.stats.max_stack + s->top_obj_stack + 2 > rtg::TRACE_STACK)
+      RTC_THROW(RTC_ERROR_UNKNOWN, "two-level tree too deep for the traversal stack");
+    rtg::SceneSlot world;
+    for (int k = 0; k < 3; k++) {
+      world.bounds_lo[k] = s->bounds_lo[k];
+      world.bounds_hi[k] = s->bounds_hi[k];
+    }
+    sn = scene_norm(world);
+  } else {
+    // width-2 tree of the object's triangles in its slot of the object table
+    std::vector<rtg::GeomDesc> descs = o->descs;
+    uint32_t off = 0;
+    for (rtg::GeomDesc& g : descs) {
+      g.scene = 0;
+      g.prim_offset = off;
+      off += g.num_tris;
+    }
+    rtg::BuildParams p = scene_params(o, none);
+    p.learned = nullptr;
+    p.builder = rtg::BUILDER_SAH;
+    p.width = 2;
+    p.split_budget = 0.0f;
+    {
+      std::lock_guard<std::mutex> bl(d->build_mtx);
+      rtg::build_bvh(descs, p, d->scratch, bin, t.stream);
+    }
+    std::vector<rtg::DevScene> views = s->top_object_views;
+    views[target] = bin.view(0);
+    dobjects.reserve(views.size() * sizeof(rtg::DevScene));
+    RTG_HIP(hipMemcpyAsync(dobjects.ptr, views.data(), views.size() * sizeof(rtg::DevScene), hipMemcpyHostToDevice,
+                           t.stream));
+    v.objects = dobjects.as<rtg::DevScene>();
+    if (s->top.stats.max_stack + bin.stats.max_stack + 2 > rtg::TRACE_STACK)
+      RTC_THROW(RTC_ERROR_UNKNOWN, "two-level tree too deep for the traversal stack");
+    sn = scene_norm(o->bvh->slots[o->slot]);
+  }
+  const uint32_t nn = bin.stats.num_nodes;
+  if (nn == 0) return;
+  uint64_t obj_size = 0;
+  for (const rtg::DevScene& ov : s->top_object_views)
+    obj_size = std::max<uint64_t>(obj_size, (uint64_t)ov.num_nodes + ov.num_tris);
+  if (o) obj_size = std::max<uint64_t>(obj_size, (uint64_t)bin.stats.num_nodes + bin.stats.num_prims);
+  v.max_iter = (uint64_t)v.num_nodes + v.num_insts + (uint64_t)v.num_insts * (obj_size + 2) + 16;
+  rtg::DevBuf copy;
+  copy.reserve((size_t)M * byteStride);
+  RTG_HIP(hipMemcpyAsync(copy.ptr, d_rays, (size_t)M * byteStride, hipMemcpyDeviceToDevice, t.stream));
+  m->m.visits.reserve((size_t)nn * 4);
+  m->m.counts.reserve(256);
+  RTG_HIP(hipMemsetAsync(m->m.visits.ptr, 0, (size_t)nn * 4, t.stream));
+  uint32_t* vis = m->m.visits.as<uint32_t>();
+  rtg::launch_trace_inst_counted(v, copy.ptr, M, byteStride, t.errword.as<uint32_t>(), m->m.counts.as<uint32_t>(),
+                                 o ? nullptr : vis, o ? vis : nullptr, target, t.stream, d->trace_params());
+  if (m->m.has_norm) sn = m->m.norm;
+  rtg::mlp_append_samples(m->m, bin.nodes.as<rtg::QNode8>(), bin.info.as<rtg::NodeInfo>(), vis, nn, sn, t.stream);
   m->m.norm = sn;
   m->m.has_norm = true;
   RTG_HIP(hipStreamSynchronize(t.stream));

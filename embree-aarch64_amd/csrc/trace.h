@@ -75,10 +75,13 @@ void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t strid
                        uint32_t* err, hipStream_t s, const TraceParams& tp, RaySorter* sorter,
                        uint32_t* counter = nullptr);
 
-// Closest-hit two-level traversal that also counts, per node of the top
-// tree, the rays that enter it (top_visits, one word per top node).
+// Closest-hit two-level traversal that also counts the rays that enter each
+// node of the top tree (top_visits, one word per top node, may be null) and
+// of object target_obj's tree (obj_visits, one word per node of that object,
+// summed over all its instances; may be null).
 void launch_trace_inst_counted(const DevTop& top, void* rays, uint32_t M, uint64_t stride, uint32_t* err,
-                               uint32_t* counter, uint32_t* top_visits, hipStream_t s, const TraceParams& tp);
+                               uint32_t* counter, uint32_t* top_visits, uint32_t* obj_visits, uint32_t target_obj,
+                               hipStream_t s, const TraceParams& tp);
 
 // Scratch of the no-allocation queries: word 0 is the persistent kernel's
 // ray counter, word 1 the traversal error bits; one fixed array per GPU

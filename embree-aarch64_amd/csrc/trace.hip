@@ -976,7 +976,9 @@ template <bool OCCLUDED, bool ALIGNED, bool CNT = false>
 __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restrict__ rays, uint32_t M,
                                                       uint64_t stride, uint32_t ctx_inst, uint32_t* __restrict__ err,
                                                       uint32_t* __restrict__ next, int refill_min,
-                                                      uint32_t* __restrict__ top_visits = nullptr) {
+                                                      uint32_t* __restrict__ top_visits = nullptr,
+                                                      uint32_t* __restrict__ obj_visits = nullptr,
+                                                      uint32_t target_obj = 0xffffffffu) {
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   __shared__ uint64_t lds_stack[8 * TB];
@@ -998,6 +1000,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
   const QNode8* onodes = nullptr;
   const TriRec* otris = nullptr;
   uint32_t onn = 0, otn = 0;
+  uint32_t oidx = 0xffffffffu;  // the object the lane is in (counted variant)
   bool orobust = false;
   bool active = false, occluded = false, pool_empty = false;
   WavePool pool;
@@ -1056,7 +1059,13 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
         atomicOr(err, 4u);
         done = true;
       } else {
-        if (CNT && !in_obj) atomicAdd(top_visits + cur, 1u);
+        if (CNT) {
+          if (!in_obj) {
+            if (top_visits) atomicAdd(top_visits + cur, 1u);
+          } else if (obj_visits && oidx == target_obj) {
+            atomicAdd(obj_visits + cur, 1u);
+          }
+        }
         NodeHit h;
         intersect_node(nodes, cur, r, lim, h);
         if (OCCLUDED) {
@@ -1158,6 +1167,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
             }
             in_obj = true;
             inst = identity ? ctx_inst : w.x;
+            if (CNT) oidx = w.y;
             onodes = o.nodes;
             otris = o.tris;
             onn = o.num_nodes;
@@ -1391,7 +1401,8 @@ void launch_trace_inst(const DevTop& top, void* rays, uint32_t M, uint64_t strid
 }
 
 void launch_trace_inst_counted(const DevTop& top, void* rays, uint32_t M, uint64_t stride, uint32_t* err,
-                               uint32_t* counter, uint32_t* top_visits, hipStream_t s, const TraceParams& tp) {
+                               uint32_t* counter, uint32_t* top_visits, uint32_t* obj_visits, uint32_t target_obj,
+                               hipStream_t s, const TraceParams& tp) {
   if (M == 0 || top.num_nodes == 0) return;
   const bool aligned = ((uintptr_t)rays % 16 == 0) && (stride % 16 == 0);
   const uint32_t blocks = div_up(M, TB);
@@ -1402,10 +1413,10 @@ void launch_trace_inst_counted(const DevTop& top, void* rays, uint32_t M, uint64
   char* rp = (char*)rays;
   if (aligned)
     hipLaunchKernelGGL((k_trace_inst<false, true, true>), grid, block, 0, s, top, rp, M, stride, 0xffffffffu, err,
-                       counter, refill, top_visits);
+                       counter, refill, top_visits, obj_visits, target_obj);
   else
     hipLaunchKernelGGL((k_trace_inst<false, false, true>), grid, block, 0, s, top, rp, M, stride, 0xffffffffu, err,
-                       counter, refill, top_visits);
+                       counter, refill, top_visits, obj_visits, target_obj);
   RTG_HIP(hipGetLastError());
 }
 

@@ -225,6 +225,7 @@ SIGNATURES = {
     "rtcgSplitModelPredict": (None, [_vp, _vp, _u32, _vp]),
     "rtcgSplitModelGradient": (_f32, [_vp, _vp, _vp, _vp, _u32, _vp]),
     "rtcgSplitModelCollect": (None, [_vp, _vp, _vp, _u32, _sz]),
+    "rtcgSplitModelCollectInstanced": (None, [_vp, _vp, _vp, _vp, _u32, _sz]),
     "rtcgSplitModelSampleCount": (_u32, [_vp]),
     "rtcgSplitModelGetSamples": (None, [_vp, _vp, _vp]),
     "rtcgSplitModelClearSamples": (None, [_vp]),
@@ -604,6 +605,12 @@ class SplitModel:
     def collect(self, scene, d_rays, n, stride=80):
         lib().rtcgSplitModelCollect(self.h, scene.h, d_rays, n, stride)
         self.device.check("collect")
+
+    def collect_instanced(self, scene, obj, d_rays, n, stride=80):
+        """Samples for a two-level scene's top tree (obj None) or for the
+        tree of an instanced scene obj."""
+        lib().rtcgSplitModelCollectInstanced(self.h, scene.h, obj.h if obj is not None else None, d_rays, n, stride)
+        self.device.check("collect_instanced")
 
     def samples(self):
         n = int(lib().rtcgSplitModelSampleCount(self.h))

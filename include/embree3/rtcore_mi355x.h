@@ -234,6 +234,14 @@ RTC_API float rtcgSplitModelGradient(RTCGSplitModel model, const float* features
  * log2((v_C + 0.5) / (v_S + 0.5)) - log2(A_C / A_S). */
 RTC_API void rtcgSplitModelCollect(RTCGSplitModel model, RTCScene scene, const void* d_rays, unsigned int M,
                                    size_t byteStride);
+/* The same for a committed two-level scene (at least one instance): with
+ * object NULL, samples for the scene's top tree over its instances (the
+ * tree the scene's own split model prices); else for the tree of `object`,
+ * a scene the top tree instances, from the visits of all its instances in
+ * object space. The records are traced through the two-level scene with a
+ * width-2 build of that one tree in its place. Neither tree is changed. */
+RTC_API void rtcgSplitModelCollectInstanced(RTCGSplitModel model, RTCScene scene, RTCScene object,
+                                            const void* d_rays, unsigned int M, size_t byteStride);
 RTC_API unsigned int rtcgSplitModelSampleCount(RTCGSplitModel model);
 RTC_API void rtcgSplitModelGetSamples(RTCGSplitModel model, float* features, float* targets);
 RTC_API void rtcgSplitModelClearSamples(RTCGSplitModel model);

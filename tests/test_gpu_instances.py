@@ -264,3 +264,67 @@ def test_instances_host_stride_and_device_records(rtc, oracle, device):
     device.free(d)
     assert_same(out, exp, "device records")
     s.release()
+
+
+def test_learned_split_cost_at_both_levels(rtc, oracle, device):
+    """The split model of a two-level scene: samples collected for the top
+    tree (object None) and for each instanced scene's tree from the visits of
+    the two-level traversal (rtcgSplitModelCollectInstanced), one model per
+    tree trained, the objects and a new top scene built with them; the
+    records equal the oracle's, and samples for a scene the top tree does not
+    instance, or from a scene without instances, are refused."""
+    import ctypes
+    s = TwoLevel(rtc, oracle, device, n_inst=60)
+    rays = np.concatenate([aimed_rays(rtc, s.T, 6000, seed=21), rtc.gen_random_rays(2000, seed=22, lo=-6, hi=6)])
+    d = device.malloc(rays.nbytes)
+    models = []
+    try:
+        device.memcpy(d, rays.ctypes.data, rays.nbytes)
+        for o in s.objs:
+            m = rtc.SplitModel(device, seed=1)
+            m.collect_instanced(s.top, o, d, len(rays))
+            X, T = m.samples()
+            assert len(X) > 20 and np.isfinite(X).all() and np.isfinite(T).all()
+            # a child is entered only after its parent (the target's bound)
+            assert (T + 16 * X[:, 0]).max() <= 1e-4
+            m.train(100, batch=4096, lr=3e-3, seed=1)
+            models.append(m)
+        mt = rtc.SplitModel(device, seed=1)
+        mt.collect_instanced(s.top, None, d, len(rays))
+        assert mt.samples()[0].shape[0] > 10
+        mt.train(100, batch=4096, lr=3e-3, seed=1)
+        models.append(mt)
+        L = rtc.lib()
+        for o, m in zip(s.objs, models):
+            L.rtcgSetSceneSplitModel(o.h, m.h)
+            o.commit()
+        top2 = rtc.Scene(device, [])
+        for k, t in enumerate(s.T):
+            top2.add_instance(s.objs[k % 2], t)
+        L.rtcgSetSceneSplitModel(top2.h, mt.h)
+        top2.commit()
+        # top2 has no own triangles: compare with an oracle scene without them
+        O = oracle.Instances()
+        oa = O.add_object([rtc.gen_sphere(12)])
+        ob = O.add_object([rtc.gen_heightfield(14, seed=3), rtc.gen_soup(300, seed=4, h=0.08)])
+        for k, t in enumerate(s.T):
+            O.add_instance(oa if k % 2 == 0 else ob, t, k)
+        got = rays.copy()
+        top2.intersect1M(got)
+        exp = rays.copy()
+        O.intersect(exp)
+        assert_same(got, exp, "learned two-level closest hit")
+        other = rtc.Scene(device, [rtc.gen_sphere(8)]).commit()
+        with pytest.raises(RuntimeError):
+            models[0].collect_instanced(s.top, other, d, len(rays))
+        with pytest.raises(RuntimeError):
+            models[0].collect_instanced(other, None, d, len(rays))
+        other.release()
+        top2.release()
+        for o in s.objs:
+            L.rtcgSetSceneSplitModel(o.h, None)
+    finally:
+        device.free(d)
+        for m in models:
+            m.release()
+        s.release()
