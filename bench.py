@@ -90,7 +90,7 @@ def cpu_baseline(v, ix, rays, sample):
 
 
 PROFILE = os.path.join(ROOT, "profiles", "trace_build_c2_soup1M_16Mrays.txt")
-TRACE_KERNEL = "k_trace_p<false, true, false, 8, false, false>"
+TRACE_KERNEL = "k_trace_p<false, true, false, 12, false, false>"
 
 
 def profiled_traffic():

@@ -2503,7 +2503,7 @@ RTC_API float rtcgAutotune(RTCScene scene, const void* d_rays, unsigned int M, s
   }
   // traversal: LDS stack entries, refill threshold, resident blocks per CU
   rtg::TraceParams bestt = tp;
-  for (int slots : {0, 8, 16})
+  for (int slots : {0, 8, 12, 16})
     for (int refill : {8, 12, 16, 24, 32})
       for (int bpc : {4, 6, 8}) {
         rtg::TraceParams c = tp;

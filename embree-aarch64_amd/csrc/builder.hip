@@ -913,7 +913,7 @@ __device__ inline void collapse_item1(const CollapseArgs& a, uint32_t t, bool ac
         a.qout[qbase + rank_i] = QItem{c, child_base + rank_i, item.depth + 1, need};
         rank_i++;
       } else {
-        q.meta[k] = (uint8_t)(0x80u | off_t);
+        q.meta[k] = (uint8_t)(1u + off_t);
         // leaf records are written afterwards, one thread each (k_write_leaves)
         const uint32_t cntk = s_count[threadIdx.x][k];
         const uint32_t firstk = s_first[threadIdx.x][k];
@@ -1138,7 +1138,7 @@ __device__ inline void collapse_item(const CollapseArgs& a, uint32_t t, bool act
       meta = 0x80u | rank_i;
       a.qout[qbase + rank_i] = QItem{mine, child_base + rank_i, item.depth + 1, need};
     } else {
-      meta = 0x80u | off_t;
+      meta = 1u + off_t;
       // leaf records are written afterwards, one thread each (k_write_leaves)
       for (uint32_t j = 0; j < cnt; j++)
         a.leaf_src[tri_base + off_t + j] = (f + j) | ((j + 1 == cnt) ? LEAF_SRC_LAST : 0u);

@@ -24,8 +24,10 @@ namespace rtg {
 // and the builder checks with that exact expression that every decoded
 // child box contains the child's widened float box, so the slab test is
 // conservative. Slot c is empty when meta[c] == 0; otherwise it is an
-// internal child (imask bit c) at child_base + (meta[c] & 0x7f) or a leaf
-// starting at tri_base + (meta[c] & 0x7f).
+// internal child (imask bit c, meta[c] = 0x80 | rank) at child_base + rank
+// or a leaf (meta[c] = 1 + offset, at most 1 + 7 * 15) starting at
+// tri_base + offset. Bit 7 of meta repeats imask so that a traversal can
+// form a child's reference from its meta byte alone.
 struct alignas(128) QNode8 {
   float px, py, pz;
   uint8_t ex, ey, ez;

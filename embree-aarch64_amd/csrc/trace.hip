@@ -34,6 +34,9 @@ namespace rtg {
 
 constexpr int TB = 256;
 constexpr int STACK = (int)TRACE_STACK;  // entries per ray (the builder checks its trees against it)
+// A node step pushes up to 7 entries after checking only that the stack
+// holds at most STACK (fits()), so the arrays have room for 7 more.
+constexpr int STACK_CAP = STACK + 8;
 
 // Per-lane traversal stack: the top LDS_SLOTS entries live in LDS (slot-major,
 // so lanes touching the same slot hit distinct banks), older entries spill
@@ -62,36 +65,55 @@ __device__ inline float below_dist(uint32_t y) { return __uint_as_float(y << 16)
 
 template <int LDS_SLOTS>
 struct RingStack {
+  // A power-of-two ring finds entry i's slot as i & (LDS_SLOTS - 1); any
+  // other size keeps the slot of entry sp in `top`. When the ring is full,
+  // the oldest ring entry (lo = sp - LDS_SLOTS) sits in entry sp's slot.
+  static constexpr bool POW2 = (LDS_SLOTS & (LDS_SLOTS - 1)) == 0;
   lds_u64* lds;    // this lane's slot 0; slot k is lds + k * TB
-  uint64_t* spill; // STACK entries, private
+  uint64_t* spill; // STACK_CAP entries, private
   int sp, lo;      // [0, lo) spilled, [lo, sp) in the ring
+  int top;         // sp mod LDS_SLOTS (other sizes than powers of two)
+  __device__ inline int slot() const {
+    if constexpr (POW2) return sp & (LDS_SLOTS - 1);
+    else return top;
+  }
   __device__ inline void init(uint64_t* base, uint64_t* spill_mem) {
     lds = (lds_u64*)base;
     spill = spill_mem;
-    sp = lo = 0;
+    sp = lo = top = 0;
+  }
+  // Whether a node's entries (at most 7) may be put(): true while the stack
+  // holds at most STACK entries, which a tree the builder accepted never
+  // exceeds; past it the arrays' 7 spare entries keep a corrupt tree's
+  // pushes in bounds until the error stops the ray.
+  __device__ inline bool fits() const { return sp <= STACK; }
+  __device__ inline void put(uint2 e) {
+    if (sp - lo == LDS_SLOTS) {
+      spill[lo] = lds[slot() * TB];
+      lo++;
+    }
+    lds[slot() * TB] = pack_entry(e);
+    sp++;
+    if constexpr (!POW2) top = top == LDS_SLOTS - 1 ? 0 : top + 1;
   }
   __device__ inline bool push(uint2 e) {
     if (sp >= STACK) return false;
-    if (sp - lo == LDS_SLOTS) {
-      spill[lo] = lds[(lo & (LDS_SLOTS - 1)) * TB];
-      lo++;
-    }
-    lds[(sp & (LDS_SLOTS - 1)) * TB] = pack_entry(e);
-    sp++;
+    put(e);
     return true;
   }
   __device__ inline uint2 pop() {
     sp--;
+    if constexpr (!POW2) top = top == 0 ? LDS_SLOTS - 1 : top - 1;
     // an entry below the ring comes back through its ring slot, so that the
     // read below is always an LDS read
     if (sp < lo) {
       lo = sp;
-      lds[(sp & (LDS_SLOTS - 1)) * TB] = spill[sp];
+      lds[slot() * TB] = spill[sp];
     }
-    return unpack_entry(lds[(sp & (LDS_SLOTS - 1)) * TB]);
+    return unpack_entry(lds[slot() * TB]);
   }
   __device__ inline bool empty() const { return sp == 0; }
-  __device__ inline void clear() { sp = lo = 0; }
+  __device__ inline void clear() { sp = lo = top = 0; }
 };
 
 // LDS_SLOTS == 0: the whole stack is a private array.
@@ -108,6 +130,8 @@ struct RingStack<0> {
     spill[sp++] = pack_entry(e);
     return true;
   }
+  __device__ inline bool fits() const { return sp <= STACK; }
+  __device__ inline void put(uint2 e) { spill[sp++] = pack_entry(e); }
   __device__ inline uint2 pop() { return unpack_entry(spill[--sp]); }
   __device__ inline bool empty() const { return sp == 0; }
   __device__ inline void clear() { sp = 0; }
@@ -192,8 +216,7 @@ __device__ inline float byte_f(uint32_t w, int k) { return (float)((w >> (8 * k)
 struct NodeHit {
   uint32_t mask;
   uint32_t imask;
-  uint32_t child_base;
-  uint32_t tri_base;
+  uint32_t cb, tb;  // reference bases of meta_ref()
   uint32_t meta_lo, meta_hi;
   float tn[8];
   bool hit[8];
@@ -212,8 +235,11 @@ __device__ inline void intersect_node(const QNode8* __restrict__ nodes, uint32_t
   const float sy = __uint_as_float(((w0.w >> 8) & 0xffu) << 23);
   const float sz = __uint_as_float(((w0.w >> 16) & 0xffu) << 23);
   h.imask = w0.w >> 24;
-  h.child_base = w1.x;
-  h.tri_base = w1.y;
+  h.cb = w1.x - 0x80u;
+  h.tb = (w1.y | REF_LEAF) - 1u;
+  // Defined here, so that the compiler keeps the node's 16 B loads whole
+  // instead of loading these two words again where the pushes use them.
+  asm volatile("" : "+v"(h.cb), "+v"(h.tb));
   h.meta_lo = w1.z;
   h.meta_hi = w1.w;
   // near / far plane bytes per axis (lo plane is near for a positive direction)
@@ -258,32 +284,65 @@ __device__ inline void intersect_node(const QNode8* __restrict__ nodes, uint32_t
   h.mask = mask;
 }
 
+// A child's reference from its meta byte m (internal children 0x80 | rank,
+// leaves 1 + offset of the first record): m plus a base chosen by bit 7,
+// cb = child_base - 0x80 or tb = (tri_base | REF_LEAF) - 1.
+__device__ inline uint32_t meta_ref(uint32_t m, uint32_t cb, uint32_t tb) { return m + ((m & 0x80u) ? cb : tb); }
+
 __device__ inline uint32_t child_ref(const NodeHit& h, int c) {
-  const uint32_t mb = ((c < 4 ? h.meta_lo : h.meta_hi) >> (8 * (c & 3))) & 0x7fu;
-  return ((h.imask >> c) & 1u) ? (h.child_base + mb) : ((h.tri_base + mb) | REF_LEAF);
+  const uint32_t mb = ((c < 4 ? h.meta_lo : h.meta_hi) >> (8 * (c & 3))) & 0xffu;
+  return meta_ref(mb, h.cb, h.tb);
+}
+
+// Occlusion: pushes the hit children except the first in slot order, sets
+// `next` to that one and returns whether any child was hit.
+template <class Stack>
+__device__ inline bool push_any_order(const NodeHit& h, Stack& stack, uint32_t* err, uint32_t& next) {
+  if (!h.mask) return false;
+  const uint32_t cb = h.cb, tb = h.tb;
+  const uint64_t meta = (uint64_t)h.meta_lo | ((uint64_t)h.meta_hi << 32);
+  const int c0 = __builtin_ctz(h.mask);
+  uint32_t rem = h.mask & (h.mask - 1u);
+  if (stack.fits()) {
+    while (rem) {
+      const int c = __builtin_ctz(rem);
+      rem &= rem - 1u;
+      stack.put(make_uint2(meta_ref((uint32_t)(meta >> (8 * c)) & 0xffu, cb, tb), 0u));
+    }
+  } else {
+    atomicOr(err, 1u);
+  }
+  next = meta_ref((uint32_t)(meta >> (8 * c0)) & 0xffu, cb, tb);
+  return true;
 }
 
 // Closest hit: pushes the hit children except the nearest, farthest first,
 // each with its entry distance, sets `next` to the nearest child and returns
 // whether any child was hit.
-// A key is the entry distance's bit pattern with the slot in the low 3 bits,
-// or 0xffffffff for a child that is not hit. Entry distances are never below
-// the ray's widened tnear, so for tnear >= 0 the bit patterns order like the
-// distances (a negative distance, possible only with tnear < 0, orders as
-// farther than every positive one: a different visiting order, the same
-// result). The perturbation of at most 7 ulps changes only the order among
-// nearly equal children. A 19-comparator network (Batcher's odd-even merge
-// sort) orders the keys decreasingly without branches: children not hit
-// first, the nearest hit child last. The stored distance is the key's high
-// half (rounded down; -inf for a negative distance), never above the true
-// entry distance, so the pop test stays conservative.
+// A key is the entry distance's bit pattern with the child's meta byte in the
+// low 8 bits (distinct among a node's children), or 0xffffffff for a child
+// that is not hit. Entry distances are never below the ray's widened tnear,
+// so for tnear >= 0 the bit patterns order like the distances (a negative
+// distance, possible only with tnear < 0, orders as farther than every
+// positive one: a different visiting order, the same result). The
+// perturbation of the low 8 bits changes only the order among nearly equal
+// children. A 19-comparator network (Batcher's odd-even merge sort) orders
+// the keys decreasingly without branches: children not hit first, the
+// nearest hit child last. The stored distance is the key's high half
+// (rounded down; -inf for a negative distance), never above the true entry
+// distance, so the pop test stays conservative.
 template <class Stack>
 __device__ inline bool push_near_to_far(const NodeHit& h, Stack& stack, float& smin, uint32_t* err,
                                         uint32_t& next) {
   constexpr uint32_t NO_HIT = 0xffffffffu;
   uint32_t k[8];
+  // v_perm_b32 selector: byte 0 from the meta word (byte c & 3), bytes 1-3
+  // from the distance
 #pragma unroll
-  for (int c = 0; c < 8; c++) k[c] = h.hit[c] ? ((__float_as_uint(h.tn[c]) & ~7u) | (uint32_t)c) : NO_HIT;
+  for (int c = 0; c < 8; c++)
+    k[c] = h.hit[c] ? __builtin_amdgcn_perm(__float_as_uint(h.tn[c]), c < 4 ? h.meta_lo : h.meta_hi,
+                                            0x07060500u | (uint32_t)(c & 3))
+                    : NO_HIT;
   auto ce = [&](int i, int j) {
     const uint32_t hi = max(k[i], k[j]), lo = min(k[i], k[j]);
     k[i] = hi;
@@ -296,14 +355,22 @@ __device__ inline bool push_near_to_far(const NodeHit& h, Stack& stack, float& s
   ce(2, 4); ce(3, 5);
   ce(1, 2); ce(3, 4); ce(5, 6);
   if (k[7] == NO_HIT) return false;
+  const uint32_t cb = h.cb, tb = h.tb;
+  if (stack.fits()) {
 #pragma unroll
-  for (int j = 0; j < 7; j++) {
-    if (k[j] != NO_HIT) {
-      const uint32_t d16 = (k[j] & 0x80000000u) ? 0xff80u : (k[j] >> 16);
-      if (!stack.push(make_uint2(child_ref(h, (int)(k[j] & 7u)), pack_dist(d16, smin)))) atomicOr(err, 1u);
+    for (int j = 0; j < 7; j++) {
+      if (k[j] != NO_HIT) {
+        const uint32_t dk = ((int32_t)k[j] < 0) ? 0xff800000u : k[j];
+        // (dk's high half, smin's high half)
+        stack.put(make_uint2(meta_ref(k[j] & 0xffu, cb, tb), __builtin_amdgcn_perm(dk, __float_as_uint(smin), 0x07060302u)));
+        const float d = __uint_as_float(dk & 0xffff0000u);
+        smin = d < smin ? d : smin;
+      }
     }
+  } else {
+    atomicOr(err, 1u);
   }
-  next = child_ref(h, (int)(k[7] & 7u));
+  next = meta_ref(k[7] & 0xffu, cb, tb);
   return true;
 }
 
@@ -444,7 +511,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
   float lim = widen_up(tfar);
 
   __shared__ uint64_t lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
-  uint64_t spill_mem[STACK];
+  uint64_t spill_mem[STACK_CAP];
   RingStack<LDS_SLOTS> stack;
   stack.init(lds_stack + threadIdx.x, spill_mem);
   float smin = INFINITY;  // least entry distance on the stack (closest hit)
@@ -472,19 +539,7 @@ __global__ __launch_bounds__(TB) void k_trace(DevScene sc, char* __restrict__ ra
         nvis++;
         if (node_visits) atomicAdd(&node_visits[cur], 1u);
       }
-      if (OCCLUDED) {
-        if (h.mask) {
-          const int c0 = __builtin_ctz(h.mask);
-          uint32_t rem = h.mask & (h.mask - 1u);
-          while (rem) {
-            const int c = __builtin_ctz(rem);
-            rem &= rem - 1u;
-            if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
-          }
-          cur = child_ref(h, c0);
-          continue;
-        }
-      } else if (push_near_to_far(h, stack, smin, err, cur)) {
+      if (OCCLUDED ? push_any_order(h, stack, err, cur) : push_near_to_far(h, stack, smin, err, cur)) {
         continue;
       }
     } else {
@@ -576,7 +631,7 @@ constexpr int TRACE_MIN_BLOCKS = 6;
 // `items` (sorted by `first`); a lane finds its stream by binary search and
 // starts at that stream's scene root.
 template <bool OCCLUDED, bool ALIGNED, bool COUNT, int LDS_SLOTS, bool PL, bool BATCH = false>
-__global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
+__global__ __launch_bounds__(TB, LDS_SLOTS > 12 ? 5 : TRACE_MIN_BLOCKS) void k_trace_p(DevScene sc, char* __restrict__ rays, uint32_t M, uint64_t stride,
                                                 uint32_t inst_id, uint32_t* __restrict__ cnt_nodes,
                                                 uint32_t* __restrict__ cnt_tris, uint32_t* __restrict__ err,
                                                 const uint32_t* __restrict__ perm, uint32_t* __restrict__ next,
@@ -585,7 +640,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_tr
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   __shared__ uint64_t lds_stack[LDS_SLOTS > 0 ? LDS_SLOTS * TB : 1];
-  uint64_t spill_mem[STACK];
+  uint64_t spill_mem[STACK_CAP];
   RingStack<LDS_SLOTS> stack;
   stack.init(lds_stack + threadIdx.x, spill_mem);
   float smin = INFINITY;  // least entry distance on the stack (closest hit)
@@ -676,19 +731,7 @@ __global__ __launch_bounds__(TB, LDS_SLOTS > 8 ? 5 : TRACE_MIN_BLOCKS) void k_tr
           nvis++;
           if (node_visits) atomicAdd(&node_visits[cur], 1u);
         }
-        if (OCCLUDED) {
-          if (h.mask) {
-            const int c0 = __builtin_ctz(h.mask);
-            uint32_t rem = h.mask & (h.mask - 1u);
-            while (rem) {
-              const int c = __builtin_ctz(rem);
-              rem &= rem - 1u;
-              if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
-            }
-            cur = child_ref(h, c0);
-            continue;  // next step descends into cur
-          }
-        } else if (push_near_to_far(h, stack, smin, err, cur)) {
+        if (OCCLUDED ? push_any_order(h, stack, err, cur) : push_near_to_far(h, stack, smin, err, cur)) {
           continue;  // next step descends into cur
         }
       }
@@ -792,7 +835,7 @@ __global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, c
   const int lane = threadIdx.x & 63;
   const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   __shared__ uint64_t lds_stack[8 * TB];
-  uint64_t spill_mem[STACK];
+  uint64_t spill_mem[STACK_CAP];
   RingStack<8> stack;
   stack.init(lds_stack + threadIdx.x, spill_mem);
   float smin = INFINITY;  // least entry distance on the stack (closest hit)
@@ -869,20 +912,7 @@ __global__ __launch_bounds__(TB, TRACE_MIN_BLOCKS) void k_trace_d(DevScene sc, c
           nvis++;
           if (node_visits) atomicAdd(&node_visits[cur], 1u);
         }
-        if (OCCLUDED) {
-          if (h.mask) {
-            const int c0 = __builtin_ctz(h.mask);
-            uint32_t rem = h.mask & (h.mask - 1u);
-            while (rem) {
-              const int c = __builtin_ctz(rem);
-              rem &= rem - 1u;
-              if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
-            }
-            cur = child_ref(h, c0);
-          } else {
-            cur = NONE_REF;
-          }
-        } else if (!push_near_to_far(h, stack, smin, err, cur)) {
+        if (!(OCCLUDED ? push_any_order(h, stack, err, cur) : push_near_to_far(h, stack, smin, err, cur))) {
           cur = NONE_REF;
         }
       }
@@ -985,7 +1015,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
   // the world-space ray and its reciprocals while the lane is in a moved
   // object (restored from here on the way out, not reloaded and divided again)
   __shared__ float4 lds_world[3 * TB];
-  uint64_t spill_mem[STACK];
+  uint64_t spill_mem[STACK_CAP];
   RingStack<8> stack;
   stack.init(lds_stack + threadIdx.x, spill_mem);
   float smin = INFINITY;  // least entry distance on the stack (closest hit)
@@ -1068,19 +1098,7 @@ __global__ __launch_bounds__(TB, 5) void k_trace_inst(DevTop top, char* __restri
         }
         NodeHit h;
         intersect_node(nodes, cur, r, lim, h);
-        if (OCCLUDED) {
-          if (h.mask) {
-            const int c0 = __builtin_ctz(h.mask);
-            uint32_t rem = h.mask & (h.mask - 1u);
-            while (rem) {
-              const int c = __builtin_ctz(rem);
-              rem &= rem - 1u;
-              if (!stack.push(make_uint2(child_ref(h, c), 0u))) atomicOr(err, 1u);
-            }
-            cur = child_ref(h, c0);
-            continue;
-          }
-        } else if (push_near_to_far(h, stack, smin, err, cur)) {
+        if (OCCLUDED ? push_any_order(h, stack, err, cur) : push_near_to_far(h, stack, smin, err, cur)) {
           continue;
         }
       }
@@ -1267,7 +1285,7 @@ struct LaunchArgs {
 
 template <bool OCC, bool AL, bool CNT, int SL, bool PL>
 void launch_one(const LaunchArgs& a) {
-  if (a.persistent && a.dual && SL == 8)
+  if (a.persistent && a.dual)  // (its ring has 8 slots whatever SL is)
     hipLaunchKernelGGL((k_trace_d<OCC, AL, CNT, PL>), a.pgrid, dim3(TB), 0, a.s, a.sc, a.rays, a.M, a.stride,
                        a.inst_id, a.cnt_nodes, a.cnt_tris, a.err, a.perm, a.counter, a.refill, a.node_visits);
   else if (a.persistent)
@@ -1283,6 +1301,7 @@ template <bool OCC, bool AL, bool CNT, bool PL>
 void launch_slots(const LaunchArgs& a) {
   if (a.lds_slots <= 0) launch_one<OCC, AL, CNT, 0, PL>(a);
   else if (a.lds_slots <= 8) launch_one<OCC, AL, CNT, 8, PL>(a);
+  else if (a.lds_slots <= 12) launch_one<OCC, AL, CNT, 12, PL>(a);
   else launch_one<OCC, AL, CNT, 16, PL>(a);
 }
 
@@ -1341,7 +1360,7 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
 template <bool OCC, bool AL, bool PL>
 void launch_batch_one(const DevScene& sc, const BatchItem* items, uint32_t nitems, uint32_t M, uint32_t* err,
                       uint32_t* counter, int refill, dim3 pgrid, hipStream_t s) {
-  hipLaunchKernelGGL((k_trace_p<OCC, AL, false, 8, PL, true>), pgrid, dim3(TB), 0, s, sc, (char*)nullptr, M,
+  hipLaunchKernelGGL((k_trace_p<OCC, AL, false, 12, PL, true>), pgrid, dim3(TB), 0, s, sc, (char*)nullptr, M,
                      (uint64_t)0, 0xffffffffu, (uint32_t*)nullptr, (uint32_t*)nullptr, err,
                      (const uint32_t*)nullptr, counter, refill, (uint32_t*)nullptr, items, nitems);
 }

@@ -172,10 +172,11 @@ def check_bvh(nodes, tris, meshes):
                 continue
             lo = planes[n, :, 0, c] - 1e-6 * np.abs(planes[n, :, 0, c]) - 1e-30
             hi = planes[n, :, 1, c] + 1e-6 * np.abs(planes[n, :, 1, c]) + 1e-30
+            assert bool((imask[n] >> c) & 1) == bool(m & 0x80), (n, c)
             if (imask[n] >> c) & 1:
                 stack.append(int(base[n, 0]) + (m & 0x7F))
             else:
-                t = int(base[n, 1]) + (m & 0x7F)
+                t = int(base[n, 1]) + m - 1
                 while True:
                     seen[t] += 1
                     v0 = tf[t, 0:3]
