@@ -971,9 +971,21 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
   const uint32_t end = sg.first + sg.count;
   const int per = PER == 1 ? 1 : (int)((sg.count + BLOCK - 1) / BLOCK);
   const uint32_t p0 = sg.first + (uint32_t)tid * (uint32_t)per;
-  if constexpr (PER == 1) {
-    if (p0 < end) bin_prim(S.sb, m, blo[p0], bhi[p0]);
-  } else if (p0 < end) {
+  // One position per thread (a segment of at most BLOCK references): its
+  // reference is loaded once and kept in registers for the binning, the
+  // flag and the scatter, instead of three loads from global memory.
+  const bool single = PER == 1 || per == 1;
+  const bool have = p0 < end;
+  float4 l1 = make_float4(0.0f, 0.0f, 0.0f, 0.0f), h1 = l1;
+  uint32_t v1 = 0u;
+  if (single) {
+    if (have) {
+      l1 = blo[p0];
+      h1 = bhi[p0];
+      v1 = idx[p0];
+      bin_prim(S.sb, m, l1, h1);
+    }
+  } else if (have) {
     bin_run(S.sb, m, blo, bhi, p0, min((uint32_t)per, end - p0));
   }
   group_sync<WAVE>();
@@ -981,7 +993,12 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
   // flags of this thread's positions, and the stable destinations
   uint32_t mask = 0u, nl = 0u;
   constexpr int BATCH = PER < 4 ? PER : 4;
-  for (int j0 = 0; j0 < per; j0 += BATCH) {
+  if (single) {
+    if (have && goes_left(sp, l1, h1, p0 - sg.first)) {
+      mask = 1u;
+      nl = 1u;
+    }
+  } else for (int j0 = 0; j0 < per; j0 += BATCH) {
     float4 bl[BATCH], bh[BATCH];
 #pragma unroll
     for (int u = 0; u < BATCH; u++) {
@@ -1020,7 +1037,37 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
 #pragma unroll
   for (int k = 0; k < 6; k++) lb[k] = rb[k] = k < 3 ? INFINITY : -INFINITY;
   uint32_t lr = 0u;
-  for (int j0 = 0; j0 < per; j0 += BATCH) {
+  // the reference at position p (rel = p - sg.first) to its stable
+  // destination; centroid bounds of the side it joins
+  auto place = [&](uint32_t rel, bool left, float4 l, float4 h, uint32_t v) {
+    const uint32_t dst = left ? sg.first + lbase + lr : sg.first + sp.nleft + (rel - lbase - lr);
+    if (left) lr++;
+    if (dst >= end) {
+      atomicOr(&t.cnt->error, 4u);
+      return;
+    }
+    idx_o[dst] = v;
+    blo_o[dst] = l;
+    bhi_o[dst] = h;
+    t.vals[dst] = v;
+    const float c[3] = {l.x + h.x, l.y + h.y, l.z + h.z};
+    if (left) {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        lb[k] = fminf(lb[k], c[k]);
+        lb[3 + k] = fmaxf(lb[3 + k], c[k]);
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        rb[k] = fminf(rb[k], c[k]);
+        rb[3 + k] = fmaxf(rb[3 + k], c[k]);
+      }
+    }
+  };
+  if (single) {
+    if (have) place(p0 - sg.first, mask & 1u, l1, h1, v1);
+  } else for (int j0 = 0; j0 < per; j0 += BATCH) {
     float4 bl[BATCH], bh[BATCH];
     uint32_t bv[BATCH];
 #pragma unroll
@@ -1037,34 +1084,7 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
       const int j = j0 + u;
       const uint32_t p = p0 + j;
       if (j >= per || p >= end) break;
-      const bool left = (mask >> j) & 1u;
-      const uint32_t rel = p - sg.first;
-      const uint32_t dst = left ? sg.first + lbase + lr : sg.first + sp.nleft + (rel - lbase - lr);
-      if (left) lr++;
-      if (dst >= end) {
-        atomicOr(&t.cnt->error, 4u);
-        continue;
-      }
-      const float4 l = bl[u], h = bh[u];
-      const uint32_t v = bv[u];
-      idx_o[dst] = v;
-      blo_o[dst] = l;
-      bhi_o[dst] = h;
-      t.vals[dst] = v;
-      const float c[3] = {l.x + h.x, l.y + h.y, l.z + h.z};
-      if (left) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          lb[k] = fminf(lb[k], c[k]);
-          lb[3 + k] = fmaxf(lb[3 + k], c[k]);
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          rb[k] = fminf(rb[k], c[k]);
-          rb[3 + k] = fmaxf(rb[3 + k], c[k]);
-        }
-      }
+      place(p - sg.first, (mask >> j) & 1u, bl[u], bh[u], bv[u]);
     }
   }
 #pragma unroll
