@@ -1536,7 +1536,23 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
   // segment (children are smaller than their parents, so none comes back)
   S.rb.ensure(sizeof(Counters));
   bool big_live = cur[0] > 0;
-  const uint32_t small_grid = div_up(cap[2], SEGW_WAVES);
+  // Grids: a level's kernels read their counts on the device, so a grid
+  // need only cover an upper bound. The host knows the input counts of
+  // level kin_level (read back, or level 0's); a segment has at most two
+  // children, big ones only from big, mid ones from big or mid.
+  uint32_t kin[3] = {cur[0], cur[1], cur[2]};
+  uint32_t kin_level = 0;
+  auto grid_bound = [&](uint32_t L, uint32_t b[3]) {
+    uint64_t v[3] = {cap[0], cap[1], cap[2]};
+    if (kin_level == L) {
+      for (int c = 0; c < 3; c++) v[c] = kin[c];
+    } else if (kin_level + 1 == L) {
+      v[0] = 2ull * kin[0];
+      v[1] = 2ull * ((uint64_t)kin[0] + kin[1]);
+      v[2] = 2ull * ((uint64_t)kin[0] + kin[1] + kin[2]);
+    }
+    for (int c = 0; c < 3; c++) b[c] = (uint32_t)std::min<uint64_t>(v[c], cap[c]);
+  };
   // level L: references (and big-segment ids) in buffers of parity L & 1,
   // written to the other parity; segment lists and counts in slot L % 3,
   // children to slot (L + 1) % 3, while slot (L + 2) % 3, last read by
@@ -1555,16 +1571,18 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
     nx.par = (uint32_t)o;
     nx.fin_max = fin_max;
     const Seg* bs = S.segs[0][li].as<Seg>();
-    if (big_live) {
-      hipLaunchKernelGGL(k_init_gbins, dim3(div_up((size_t)cap[0] * BIN_WORDS, TBS)), dim3(TBS), 0, s,
+    uint32_t gb[3];
+    grid_bound(L, gb);
+    if (big_live && gb[0]) {
+      hipLaunchKernelGGL(k_init_gbins, dim3(div_up((size_t)gb[0] * BIN_WORDS, TBS)), dim3(TBS), 0, s,
                          S.bins.as<uint32_t>(), &cnt->nseg[li][0]);
       hipLaunchKernelGGL(k_big_bins, dim3(div_up(n, CHUNK)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
                          S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs, S.bins.as<uint32_t>());
       if (lcost.P)
-        hipLaunchKernelGGL(k_big_split<true>, dim3(cap[0]), dim3(TBS), 0, s, bs, &cnt->nseg[li][0],
+        hipLaunchKernelGGL(k_big_split<true>, dim3(gb[0]), dim3(TBS), 0, s, bs, &cnt->nseg[li][0],
                            S.bins.as<uint32_t>(), S.splits.as<Split>(), nx, t, lcost);
       else
-        hipLaunchKernelGGL(k_big_split<false>, dim3(cap[0]), dim3(TBS), 0, s, bs, &cnt->nseg[li][0],
+        hipLaunchKernelGGL(k_big_split<false>, dim3(gb[0]), dim3(TBS), 0, s, bs, &cnt->nseg[li][0],
                            S.bins.as<uint32_t>(), S.splits.as<Split>(), nx, t, lcost);
       hipLaunchKernelGGL(k_big_flags, dim3(div_up(n, TBS)), dim3(TBS), 0, s, S.sid[in].as<uint32_t>(),
                          S.blo[in].as<float4>(), S.bhi[in].as<float4>(), n, bs, S.splits.as<Split>(),
@@ -1575,11 +1593,13 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
                          S.splits.as<Split>(), S.flags.as<uint32_t>(), S.scan.as<uint32_t>(), S.sid[o].as<uint32_t>(),
                          S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), vals, cnt);
     }
-    hipLaunchKernelGGL(lcost.P ? k_seg<true> : k_seg<false>, dim3(cap[1]), dim3(256), 0, s, S.segs[1][li].as<Seg>(),
+    // (k_seg clears the next level's counts: it runs even with no mid segment)
+    hipLaunchKernelGGL(lcost.P ? k_seg<true> : k_seg<false>, dim3(std::max(gb[1], 1u)), dim3(256), 0, s, S.segs[1][li].as<Seg>(),
                        &cnt->nseg[li][1], S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(), S.bhi[in].as<float4>(),
                        S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(), nx, t, lcost,
                        cnt->nseg[lz]);
-    hipLaunchKernelGGL(lcost.P ? k_seg_wave<true> : k_seg_wave<false>, dim3(small_grid), dim3(SEGW_TB), 0, s,
+    if (gb[2])
+      hipLaunchKernelGGL(lcost.P ? k_seg_wave<true> : k_seg_wave<false>, dim3(div_up(gb[2], SEGW_WAVES)), dim3(SEGW_TB), 0, s,
                        S.segs[2][li].as<Seg>(), &cnt->nseg[li][2], S.idx[in].as<uint32_t>(), S.blo[in].as<float4>(),
                        S.bhi[in].as<float4>(), S.idx[o].as<uint32_t>(), S.blo[o].as<float4>(), S.bhi[o].as<float4>(),
                        nx, t, lcost);
@@ -1607,6 +1627,8 @@ void build_sah_tree(uint32_t n, uint32_t* vals, const float4* slo, const float4*
         throw hip_error(hipErrorUnknown, "SAH build failed: error bits " + std::to_string(hc.error));
       }
       const uint32_t* next = hc.nseg[levels % 3u];  // input of the level just queued
+      for (int c = 0; c < 3; c++) kin[c] = next[c];
+      kin_level = levels;
       if (next[0] + next[1] + next[2] == 0) break;
       if (next[0] == 0) big_live = false;
     }
