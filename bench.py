@@ -303,10 +303,11 @@ def batched_scenes(num_scenes, steps):
     streams log-uniform in [4e3, 1e6] from a radius-2 sphere towards the unit
     ball. The tree shape (node width, leaf size, collapse triangle cost) and
     the traversal launch (LDS stack entries, refill threshold, resident
-    blocks per CU) are autotuned first on the scene of median size with 2^20
-    rays drawn like the streams (a scene's own stream can be a few thousand
-    rays, too few to time a launch setting); then one rtcgCommitScenes builds
-    them all with the tuned
+    blocks per CU) are autotuned: the tree shape on the scene of median size
+    with 2^20 rays drawn like the streams (rtcgAutotune), then one
+    rtcgCommitScenes builds them all with it, and the launch settings are
+    tuned on the batched launch itself (rtcgAutotuneBatch, on a scratch copy
+    of the streams); then the tuned
     settings and one persistent launch, captured in a hipGraph, traces every
     stream."""
     L = R.lib()
@@ -339,6 +340,12 @@ def batched_scenes(num_scenes, steps):
     d_src, d_rays = dev.malloc(nbytes), dev.malloc(nbytes)
     dev.memcpy(d_src, allr.ctypes.data, nbytes)
     offs = np.concatenate([[0], np.cumsum([len(r) for r in rays])])
+    items = R.batch_items([(sc, d_src + 80 * int(offs[i]), len(r), 80) for i, (sc, r) in enumerate(zip(scs, rays))])
+    tb = time.perf_counter()
+    launch_tuned = ctypes.create_string_buffer(256)
+    launch_mrays = L.rtcgAutotuneBatch(items.ctypes.data, len(items), 0, launch_tuned, 256)
+    dev.check("rtcgAutotuneBatch")
+    tune_batch_s = time.perf_counter() - tb
     items = R.batch_items([(sc, d_rays + 80 * int(offs[i]), len(r), 80) for i, (sc, r) in enumerate(zip(scs, rays))])
     g = L.rtcgNewBatchGraph(items.ctypes.data, len(items), 0)
     dev.check("rtcgNewBatchGraph")
@@ -366,7 +373,10 @@ def batched_scenes(num_scenes, steps):
     return {"workload": f"{num_scenes} displaced spheres, {tri_total} triangles, {n} rays in {num_scenes} streams",
             "autotuned": {"config": tuned.value.decode(), "on_scene": med, "scene_tris": int(work[med][1]),
                           "tuning_rays": len(tune_rays), "mrays_per_s_on_scene": round(float(tuned_mrays), 2),
-                          "seconds": round(tune_s, 2)},
+                          "seconds": round(tune_s, 2),
+                          "launch_config": launch_tuned.value.decode(),
+                          "launch_mrays_per_s": round(float(launch_mrays), 2),
+                          "launch_seconds": round(tune_batch_s, 2)},
             "commit_ms_incl_upload": round(1e3 * commit_s, 2),
             "commit_mtris_per_s_incl_upload": round(tri_total / commit_s / 1e6, 2),
             "graph_ms": round(ms, 3), "mrays_per_s": round(n / (ms / 1e3) / 1e6, 2), "hit_rate": round(hit_rate, 4),

@@ -2326,6 +2326,90 @@ RTC_API void rtcgTraceBatch(const RTCGBatchItem* items, unsigned int numItems, i
   API_END(s0)
 }
 
+RTC_API float rtcgAutotuneBatch(const RTCGBatchItem* items, unsigned int numItems, int occluded, char* out,
+                                size_t outSize) {
+  Scene* s0 = (items && numItems) ? SC(items[0].scene) : nullptr;
+  API_BEGIN
+  PreparedBatch pb = prepare_batch(items, numItems, occluded != 0);
+  if (pb.items.empty()) return 0.0f;
+  Device* d = pb.first->dev;
+  d->activate();
+  ThreadDev& t = d->td();
+  // the items' rays packed into a pristine copy (src) and a work copy of the
+  // same layout, which each timed launch starts from (the caller's records
+  // are only read)
+  std::vector<rtg::BatchItem> wi = pb.items;
+  std::vector<size_t> off(wi.size());
+  size_t bytes = 0;
+  bool aligned = true;
+  for (size_t i = 0; i < wi.size(); i++) {
+    off[i] = bytes;
+    bytes += (size_t)wi[i].count * wi[i].stride;
+    aligned &= wi[i].stride % 16 == 0;
+  }
+  rtg::DevBuf src, work, table;
+  src.reserve(bytes);
+  work.reserve(bytes);
+  for (size_t i = 0; i < wi.size(); i++) {
+    RTG_HIP(hipMemcpyAsync(src.as<char>() + off[i], wi[i].rays, (size_t)wi[i].count * wi[i].stride,
+                           hipMemcpyDeviceToDevice, t.stream));
+    wi[i].rays = work.as<char>() + off[i];
+  }
+  table.reserve(wi.size() * sizeof(rtg::BatchItem));
+  RTG_HIP(hipMemcpyAsync(table.ptr, wi.data(), wi.size() * sizeof(rtg::BatchItem), hipMemcpyHostToDevice, t.stream));
+  t.batch_counter.reserve(256);
+  auto time_launch = [&](const rtg::TraceParams& c) {
+    float total = 0.0f;
+    const int reps = 2;
+    for (int i = -1; i < reps; i++) {  // one untimed warm-up
+      RTG_HIP(hipMemcpyAsync(work.ptr, src.ptr, bytes, hipMemcpyDeviceToDevice, t.stream));
+      RTG_HIP(hipEventRecord(t.ev0, t.stream));
+      rtg::launch_trace_batch(pb.sc, table.as<rtg::BatchItem>(), (uint32_t)wi.size(), pb.total, occluded != 0,
+                              aligned, t.errword.as<uint32_t>(), t.batch_counter.as<uint32_t>(), t.stream, c);
+      RTG_HIP(hipEventRecord(t.ev1, t.stream));
+      RTG_HIP(hipEventSynchronize(t.ev1));
+      float ms = 0.0f;
+      RTG_HIP(hipEventElapsedTime(&ms, t.ev0, t.ev1));
+      if (i >= 0) total += ms;
+    }
+    check_trace_error(t);
+    return total / (float)reps;
+  };
+  const rtg::TraceParams tp = d->trace_params();
+  rtg::TraceParams bestt = tp;
+  float best = INFINITY;
+  for (int slots : {8, 12})
+    for (int refill : {8, 12, 16, 24})
+      for (int bpc : {4, 6, 8}) {
+        rtg::TraceParams c = tp;
+        c.lds_slots = slots;
+        if (occluded) c.refill_min_occ = refill; else c.refill_min = refill;
+        c.blocks_per_cu = bpc;
+        const float ms = time_launch(c);
+        if (ms < best) {
+          best = ms;
+          bestt = c;
+        }
+      }
+  {
+    std::lock_guard<std::mutex> g(d->settings_mtx);
+    d->trace.lds_slots = bestt.lds_slots;
+    if (occluded) d->trace.refill_min_occ = bestt.refill_min_occ; else d->trace.refill_min = bestt.refill_min;
+    d->trace.blocks_per_cu = bestt.blocks_per_cu;
+  }
+  if (out && outSize) {
+    char buf[128];
+    snprintf(buf, sizeof(buf), "trace_stack=%d,%s=%d,blocks_per_cu=%d", bestt.lds_slots,
+             occluded ? "refill_min_occ" : "refill_min", occluded ? bestt.refill_min_occ : bestt.refill_min,
+             bestt.blocks_per_cu);
+    strncpy(out, buf, outSize - 1);
+    out[outSize - 1] = 0;
+  }
+  return (float)pb.total / (best * 1e-3f) / 1e6f;
+  API_END(s0)
+  return 0.0f;
+}
+
 RTC_API RTCGBatchGraph rtcgNewBatchGraph(const RTCGBatchItem* items, unsigned int numItems, int occluded) {
   Scene* s0 = (items && numItems) ? SC(items[0].scene) : nullptr;
   API_BEGIN

@@ -1357,12 +1357,18 @@ void launch_trace(const DevScene& sc, void* rays, uint32_t M, uint64_t stride, b
   RTG_HIP(hipGetLastError());
 }
 
+// (LDS stack rings of 8 or of 12 entries per lane: lds_slots up to 8, or more)
 template <bool OCC, bool AL, bool PL>
 void launch_batch_one(const DevScene& sc, const BatchItem* items, uint32_t nitems, uint32_t M, uint32_t* err,
-                      uint32_t* counter, int refill, dim3 pgrid, hipStream_t s) {
-  hipLaunchKernelGGL((k_trace_p<OCC, AL, false, 12, PL, true>), pgrid, dim3(TB), 0, s, sc, (char*)nullptr, M,
-                     (uint64_t)0, 0xffffffffu, (uint32_t*)nullptr, (uint32_t*)nullptr, err,
-                     (const uint32_t*)nullptr, counter, refill, (uint32_t*)nullptr, items, nitems);
+                      uint32_t* counter, int refill, dim3 pgrid, hipStream_t s, int lds_slots) {
+  if (lds_slots <= 8)
+    hipLaunchKernelGGL((k_trace_p<OCC, AL, false, 8, PL, true>), pgrid, dim3(TB), 0, s, sc, (char*)nullptr, M,
+                       (uint64_t)0, 0xffffffffu, (uint32_t*)nullptr, (uint32_t*)nullptr, err,
+                       (const uint32_t*)nullptr, counter, refill, (uint32_t*)nullptr, items, nitems);
+  else
+    hipLaunchKernelGGL((k_trace_p<OCC, AL, false, 12, PL, true>), pgrid, dim3(TB), 0, s, sc, (char*)nullptr, M,
+                       (uint64_t)0, 0xffffffffu, (uint32_t*)nullptr, (uint32_t*)nullptr, err,
+                       (const uint32_t*)nullptr, counter, refill, (uint32_t*)nullptr, items, nitems);
 }
 
 void launch_trace_batch(const DevScene& sc, const BatchItem* items, uint32_t nitems, uint32_t M, bool occluded,
@@ -1376,19 +1382,19 @@ void launch_trace_batch(const DevScene& sc, const BatchItem* items, uint32_t nit
   const dim3 pgrid(want < blocks ? want : blocks);
   if (sc.robust) {
     if (occluded) {
-      if (aligned) launch_batch_one<true, true, true>(sc, items, nitems, M, err, counter, refill, pgrid, s);
-      else launch_batch_one<true, false, true>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+      if (aligned) launch_batch_one<true, true, true>(sc, items, nitems, M, err, counter, refill, pgrid, s, tp.lds_slots);
+      else launch_batch_one<true, false, true>(sc, items, nitems, M, err, counter, refill, pgrid, s, tp.lds_slots);
     } else {
-      if (aligned) launch_batch_one<false, true, true>(sc, items, nitems, M, err, counter, refill, pgrid, s);
-      else launch_batch_one<false, false, true>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+      if (aligned) launch_batch_one<false, true, true>(sc, items, nitems, M, err, counter, refill, pgrid, s, tp.lds_slots);
+      else launch_batch_one<false, false, true>(sc, items, nitems, M, err, counter, refill, pgrid, s, tp.lds_slots);
     }
   } else {
     if (occluded) {
-      if (aligned) launch_batch_one<true, true, false>(sc, items, nitems, M, err, counter, refill, pgrid, s);
-      else launch_batch_one<true, false, false>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+      if (aligned) launch_batch_one<true, true, false>(sc, items, nitems, M, err, counter, refill, pgrid, s, tp.lds_slots);
+      else launch_batch_one<true, false, false>(sc, items, nitems, M, err, counter, refill, pgrid, s, tp.lds_slots);
     } else {
-      if (aligned) launch_batch_one<false, true, false>(sc, items, nitems, M, err, counter, refill, pgrid, s);
-      else launch_batch_one<false, false, false>(sc, items, nitems, M, err, counter, refill, pgrid, s);
+      if (aligned) launch_batch_one<false, true, false>(sc, items, nitems, M, err, counter, refill, pgrid, s, tp.lds_slots);
+      else launch_batch_one<false, false, false>(sc, items, nitems, M, err, counter, refill, pgrid, s, tp.lds_slots);
     }
   }
   RTG_HIP(hipGetLastError());

@@ -213,6 +213,7 @@ SIGNATURES = {
     "rtcOccludedNp": (None, [_vp, _vp, _vp, _u32]),
     "rtcgCommitScenes": (None, [_vp, _u32]),
     "rtcgTraceBatch": (None, [_vp, _u32, _i32]),
+    "rtcgAutotuneBatch": (_f32, [_vp, _u32, _i32, ctypes.c_char_p, _sz]),
     "rtcgNewBatchGraph": (_vp, [_vp, _u32, _i32]),
     "rtcgLaunchBatchGraph": (None, [_vp, _vp]),
     "rtcgReleaseBatchGraph": (None, [_vp]),

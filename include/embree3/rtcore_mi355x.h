@@ -141,7 +141,7 @@ RTC_API void rtcgGenSphereRays(unsigned int numRays, unsigned long long seed, fl
 /* No counterpart in the reference (its BVH and traversal settings are fixed
  * per ISA). Times traversal configurations on M device-resident sample rays
  * (closest hit, or occlusion records) of a committed scene and keeps the
- * fastest as the device's setting: LDS stack entries (0, 8, 16), refill
+ * fastest as the device's setting: LDS stack entries (0, 8, 12, 16), refill
  * threshold (8..32 idle lanes) and resident blocks per CU (4, 6, 8); with
  * rebuildTree != 0 first the tree shape, rebuilding the scene for each node
  * width (4, 8), leaf size (4..12) and collapse triangle cost, ending with the
@@ -175,6 +175,18 @@ struct RTCGBatchItem
  * atomic counter over the concatenated streams); records updated in place
  * as rtcIntersect1M/rtcOccluded1M would. Synchronous. */
 RTC_API void rtcgTraceBatch(const struct RTCGBatchItem* items, unsigned int numItems, int occluded);
+
+/* Launch settings for the batched traversal, tuned on the batch itself:
+ * times rtcgTraceBatch's launch over the streams (on a scratch copy: the
+ * records are only read) for each LDS stack ring (8 or 12 entries per
+ * lane), refill threshold (8..24 idle lanes) and resident blocks per CU (4,
+ * 6, 8), keeps the fastest as the setting of the scenes' device (batch
+ * graphs made afterwards capture it), writes it as an rtcNewDevice config
+ * string into out (outSize bytes) and returns the best Mrays/s. Needs twice
+ * the streams' bytes of scratch device memory. No counterpart in the
+ * reference. */
+RTC_API float rtcgAutotuneBatch(const struct RTCGBatchItem* items, unsigned int numItems, int occluded, char* out,
+                                size_t outSize);
 
 /* The same traversal captured once in a hipGraph (counter clear + launch;
  * the stream table stays on the device), for callers that replay it. The

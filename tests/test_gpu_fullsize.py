@@ -135,9 +135,10 @@ def test_c4_heightfield_full_size_plain_and_learned(rtc, oracle):
 def test_c3_batched_full_size(rtc, oracle):
     """256 displaced spheres (triangles log-uniform in [1e3, 2e6], 66.7 M in
     all) and 256 ray streams (log-uniform in [4e3, 1e6], 46.7 M rays): the
-    tree shape and launch settings autotuned on the median scene, one
-    rtcgCommitScenes, one persistent launch captured in a hipGraph and
-    replayed twice (identical records); every stream of 16 scenes, the
+    tree shape autotuned on the median scene, one rtcgCommitScenes, the
+    launch settings autotuned on the batch (which leaves the records as they
+    were), one persistent launch captured in a hipGraph and replayed twice
+    (identical records); every stream of 16 scenes, the
     smallest and the largest among them, checked against the oracle (at most
     32,768 rays per stream)."""
     work = rtc.batch_workload(256, seed=7)
@@ -165,6 +166,13 @@ def test_c3_batched_full_size(rtc, oracle):
     d = dev.malloc(allr.nbytes)
     try:
         items = rtc.batch_items([(sc, d + 80 * int(offs[i]), len(r), 80) for i, (sc, r) in enumerate(zip(scs, rays))])
+        dev.memcpy(d, allr.ctypes.data, allr.nbytes)
+        assert L.rtcgAutotuneBatch(items.ctypes.data, len(items), 0, buf, 256) > 0
+        dev.check("rtcgAutotuneBatch")
+        say(f"C3 launch tuned {buf.value.decode()}")
+        o = np.empty_like(allr)
+        dev.memcpy(o.ctypes.data, d, allr.nbytes)
+        assert o.tobytes() == allr.tobytes(), "rtcgAutotuneBatch changed the records"
         g = L.rtcgNewBatchGraph(items.ctypes.data, len(items), 0)
         dev.check("rtcgNewBatchGraph")
         got = []
