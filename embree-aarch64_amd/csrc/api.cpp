@@ -2332,6 +2332,9 @@ RTC_API float rtcgAutotuneBatch(const RTCGBatchItem* items, unsigned int numItem
   API_BEGIN
   PreparedBatch pb = prepare_batch(items, numItems, occluded != 0);
   if (pb.items.empty()) return 0.0f;
+  // the shared node and triangle arrays stay alive while the tuner traces
+  // them, whatever another thread commits meanwhile
+  const std::shared_ptr<rtg::DeviceBVH> keep = pb.first->bvh;
   Device* d = pb.first->dev;
   d->activate();
   ThreadDev& t = d->td();
