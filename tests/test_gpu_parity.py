@@ -231,6 +231,34 @@ def test_soup_parity_with_edge_rays(rtc, device, oracle, quality):
     sc.release()
 
 
+@pytest.mark.parametrize("robust", [False, True])
+def test_negative_tnear_parity(rtc, device, oracle, robust):
+    """Intervals that start behind the origin (tnear < 0), origins inside
+    the soup: hits at negative t count and the closest is the least t. The
+    boxes behind the origin have negative entry distances, whose child-order
+    keys sort as farthest and whose stack entries keep -inf as their bound
+    (never culled by the pop test). Closest and any hit equal the oracle's."""
+    v, ix = soup_case(rtc, 20000, 9, h=0.02)
+    sc = rtc.Scene(device, [(v, ix)], flags=ROBUST if robust else 0).commit()
+    rays = rtc.gen_random_rays(100000, seed=10, lo=0.2, hi=0.8)
+    rays["tnear"][0::3] = -0.3
+    rays["tnear"][1::3] = -3.0
+    rays["tfar"][1::6] = 0.05
+    rays["tfar"][2::7] = -0.1          # a wholly negative interval
+    out = trace(rtc, sc, rays)
+    ref = oracle_trace(oracle, [(v, ix)], rays, robust=robust)
+    assert_same(out, ref, f"negative tnear, robust={robust}")
+    hit = out["geomID"] != INV
+    assert (out["tfar"][hit] < 0).sum() > 1000, "hits behind the origin"
+    occ = np.zeros(len(rays), rtc.RTCRay)
+    for k in rtc.RTCRay.names:
+        occ[k] = rays[k]
+    assert_same(trace(rtc, sc, occ, occluded=True),
+                oracle_trace(oracle, [(v, ix)], occ, occluded=True, robust=robust),
+                f"negative tnear occlusion, robust={robust}")
+    sc.release()
+
+
 @pytest.fixture(scope="module")
 def sah_device(rtc):
     d = rtc.Device("builder=sah")
