@@ -388,13 +388,15 @@ def instanced_scene(num_instances, width, height, steps, check_rays=2048, learn=
     200k triangles, three heightfield patches of about 1M, 200k and 100k)
     committed as scenes, and num_instances instances of them with a uniform
     random rotation, a uniform scale in [0.5, 2] and a translation uniform in
-    [-50, 50]^3. Closest hit: width x height pinhole-camera rays from
-    (0, 20, -160) towards the origin; occlusion: one shadow ray per camera
-    ray (as many as the camera rays: 2^25 at 8192 x 4096) towards the point
-    light (0, 100, 0), from the hit point, or for a miss from the ray's point
-    nearest the origin. A subsample of both is checked against the oracle's
-    instance restatement over per-object BVHs (the full-size parity test
-    checks 64k of each)."""
+    [-50, 50]^3. Closest hit, half of the rays: width x height pinhole-camera
+    rays from (0, 20, -160) towards the origin, and as many cosine-hemisphere
+    bounce rays from the camera rays' hit points (each hit point in turn,
+    with a new seed per pass). Occlusion, the other half: one shadow ray per
+    closest-hit ray towards the point light (0, 100, 0), from the hit point,
+    or for a miss from the ray's point nearest the origin (2^26 rays in all
+    at 4096 x 4096). A subsample of each kind is checked against the
+    oracle's instance restatement over per-object BVHs (the full-size parity
+    test checks 64k of each)."""
     from oracle import oracle as O
     L = R.lib()
     dev = R.Device("")
@@ -410,74 +412,81 @@ def instanced_scene(num_instances, width, height, steps, check_rays=2048, learn=
     t1 = time.perf_counter()
     top.commit()
     commit_ms = 1e3 * (time.perf_counter() - t1)
-    rays = R.gen_camera(width, height, org=(0.0, 20.0, -160.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0)
-    n, nbytes = len(rays), rays.nbytes
-    d_src, d_rays = dev.malloc(nbytes), dev.malloc(nbytes)
-    dev.memcpy(d_src, rays.ctypes.data, nbytes)
-    warm = np.zeros(2, np.float32)
-    L.rtcgTimeTrace(top.h, d_rays, d_src, n, 80, 0, 2, warm.ctypes.data)
-    ms = np.zeros(steps, np.float32)
-    L.rtcgTimeTrace(top.h, d_rays, d_src, n, 80, 0, steps, ms.ctypes.data)
-    dev.check("rtcgTimeTrace instanced")
-    hits = np.zeros(n, R.RTCRayHit)
-    dev.memcpy(hits.ctypes.data, d_rays, nbytes)
+
+    def timed(recs, stride, occluded):
+        """Mean kernel ms over `steps` launches (after two untimed) and the
+        traced records."""
+        d_src, d_r = dev.malloc(max(1, recs.nbytes)), dev.malloc(max(1, recs.nbytes))
+        dev.memcpy(d_src, recs.ctypes.data, recs.nbytes)
+        warm = np.zeros(2, np.float32)
+        L.rtcgTimeTrace(top.h, d_r, d_src, len(recs), stride, occluded, 2, warm.ctypes.data)
+        ms = np.zeros(steps, np.float32)
+        L.rtcgTimeTrace(top.h, d_r, d_src, len(recs), stride, occluded, steps, ms.ctypes.data)
+        dev.check("rtcgTimeTrace instanced")
+        out = np.zeros(len(recs), recs.dtype)
+        dev.memcpy(out.ctypes.data, d_r, recs.nbytes)
+        dev.free(d_src)
+        dev.free(d_r)
+        return float(ms.mean()), out
+
+    cam = R.gen_camera(width, height, org=(0.0, 20.0, -160.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0)
+    c_ms, cam_hits = timed(cam, 80, 0)
+    bounce = R.gen_bounce_n(cam_hits, len(cam), seed=31)
+    b_ms, bounce_hits = timed(bounce, 80, 0)
+    rays = np.concatenate([cam, bounce])
+    hits = np.concatenate([cam_hits, bounce_hits])
+    n = len(rays)
     hit = hits["geomID"] != R.RTC_INVALID_GEOMETRY_ID
-    # one shadow ray per camera ray towards the light (from the hit point, or
-    # for a miss from the ray's point nearest the instance cloud's centre)
     sh = R.gen_shadow_rays(hits, light=(0.0, 100.0, 0.0), centre=(0.0, 0.0, 0.0))
     ns = len(sh)
-    d_ssrc, d_sh = dev.malloc(max(1, sh.nbytes)), dev.malloc(max(1, sh.nbytes))
-    dev.memcpy(d_ssrc, sh.ctypes.data, sh.nbytes)
-    L.rtcgTimeTrace(top.h, d_sh, d_ssrc, ns, 48, 1, 2, warm.ctypes.data)
-    oms = np.zeros(steps, np.float32)
-    L.rtcgTimeTrace(top.h, d_sh, d_ssrc, ns, 48, 1, steps, oms.ctypes.data)
-    dev.check("rtcgTimeTrace instanced occlusion")
-    shadow = np.zeros(ns, R.RTCRay)
-    dev.memcpy(shadow.ctypes.data, d_sh, sh.nbytes)
-    # oracle check on a subsample
+    o_ms, shadow = timed(sh, 48, 1)
+    # oracle check on a subsample of each kind
     t2 = time.perf_counter()
     I = O.Instances()
     oid = [I.add_object([m]) for m in meshes]
     for k in range(num_instances):
         I.add_instance(oid[which[k]], T[k], k)
     rng = np.random.default_rng(12)
-    pick = rng.choice(n, min(check_rays, n), replace=False)
-    ref = rays[pick].copy()
-    I.intersect_bvh(ref)
-    same_ch = bool(np.array_equal(ref.view(np.uint8).reshape(len(ref), -1),
-                                  hits[pick].view(np.uint8).reshape(len(ref), -1)))
-    spick = rng.choice(ns, min(check_rays, ns), replace=False) if ns else np.zeros(0, np.int64)
+    same = {}
+    for name, src, out in (("camera", cam, cam_hits), ("bounce", bounce, bounce_hits)):
+        pick = rng.choice(len(src), min(check_rays, len(src)), replace=False)
+        ref = src[pick].copy()
+        I.intersect_bvh(ref)
+        same[name] = bool(np.array_equal(ref.view(np.uint8).reshape(len(ref), -1),
+                                         out[pick].view(np.uint8).reshape(len(ref), -1)))
+    spick = rng.choice(ns, min(check_rays, ns), replace=False)
     sref = sh[spick].copy()
     I.occluded_bvh(sref)
-    same_occ = bool(np.array_equal(sref["tfar"], shadow[spick]["tfar"]))
+    same["shadow"] = bool(np.array_equal(sref["tfar"], shadow[spick]["tfar"]))
     check_s = time.perf_counter() - t2
     learned = instanced_learned(dev, top, objs, which, T, rays, sh, steps) if learn else None
-    for q in (d_src, d_rays, d_ssrc, d_sh):
-        dev.free(q)
     top.release()
     for o in objs:
         o.release()
     dev.release()
     base = [len(m[1]) for m in meshes]
-    c_ms, o_ms = float(ms.mean()), float(oms.mean())
     return {"workload": f"{num_instances} instances of 5 base meshes ({sum(base)} triangles: {base}), "
-                        f"{n} camera rays ({width}x{height}) closest hit + {ns} shadow rays",
+                        f"{len(cam)} camera rays ({width}x{height}) + {len(bounce)} cosine bounce rays closest hit, "
+                        f"{ns} shadow rays",
             "instanced_triangles": int(sum(base[w] for w in which)),
             "commit_top_ms": round(commit_ms, 2), "setup_s": round(setup_s, 1),
-            "closest_mrays_per_s": round(n / (c_ms / 1e3) / 1e6, 2), "closest_ms": round(c_ms, 3),
-            "occlusion_mrays_per_s": round(ns / (o_ms / 1e3) / 1e6, 2) if ns else None,
-            "hit_rate": round(float(hit.mean()), 4), "shadowed": round(float((shadow["tfar"] < 0).mean()), 4) if ns else None,
-            "oracle_check": {"rays": int(len(pick)), "shadow_rays": int(len(spick)), "closest_identical": same_ch,
-                             "occlusion_identical": same_occ, "seconds": round(check_s, 1)},
+            "closest_mrays_per_s": round(n / ((c_ms + b_ms) / 1e3) / 1e6, 2), "closest_ms": round(c_ms + b_ms, 3),
+            "camera_mrays_per_s": round(len(cam) / (c_ms / 1e3) / 1e6, 2),
+            "bounce_mrays_per_s": round(len(bounce) / (b_ms / 1e3) / 1e6, 2),
+            "occlusion_mrays_per_s": round(ns / (o_ms / 1e3) / 1e6, 2),
+            "hit_rate": round(float(hit.mean()), 4), "shadowed": round(float((shadow["tfar"] < 0).mean()), 4),
+            "oracle_check": {"rays_per_kind": int(check_rays), "camera_identical": same["camera"],
+                             "bounce_identical": same["bounce"], "shadow_identical": same["shadow"],
+                             "seconds": round(check_s, 1)},
             "learned_both_levels": learned}
 
 
 def instanced_learned(dev, top, objs, which, T, rays, shadow, steps, train_steps=200):
     """The learned split cost at both levels of the two-level scene: one
     model per base mesh trained on the visits of all its instances, one for
-    the top tree over the instances, all from the even-indexed camera rays;
-    the plain and the learned scenes are timed on the odd-indexed camera rays
-    and their shadow rays (held out)."""
+    the top tree over the instances, all from the even-indexed closest-hit
+    rays (camera and bounce); the plain and the learned scenes are timed on
+    the odd-indexed closest-hit rays and their shadow rays (held out)."""
     L = R.lib()
     train = np.ascontiguousarray(rays[::2])
     held = np.ascontiguousarray(rays[1::2])
@@ -533,8 +542,9 @@ def instanced_learned(dev, top, objs, which, T, rays, shadow, steps, train_steps
         L.rtcgSetSceneSplitModel(o.h, None)
     for m in models + [mt]:
         m.release()
-    out = {"workload": f"models trained on the {len(train)} even-indexed camera rays; timed on the {len(held)} "
-                       f"odd-indexed camera rays and their {len(hsh)} shadow rays (held out); {train_steps} steps",
+    out = {"workload": f"models trained on the {len(train)} even-indexed closest-hit rays (camera and bounce); "
+                       f"timed on the {len(held)} odd-indexed ones and their {len(hsh)} shadow rays (held out); "
+                       f"{train_steps} steps",
            "hits_identical": same, "train_s": round(train_s, 2), "rebuild_s": round(rebuild_s, 2)}
     for name in ("plain", "learned"):
         out[name] = {"closest_mrays_per_s": round(res[name][0][0], 2), "occlusion_mrays_per_s": round(res[name][1][0], 2)}
@@ -563,7 +573,7 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=256)
     ap.add_argument("--no-instances", action="store_true")
     ap.add_argument("--instances", type=int, default=10000)
-    ap.add_argument("--inst-width", type=int, default=8192)
+    ap.add_argument("--inst-width", type=int, default=4096)
     ap.add_argument("--inst-height", type=int, default=4096)
     args = ap.parse_args()
     if args.gpus != 1:

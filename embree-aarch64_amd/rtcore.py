@@ -346,6 +346,21 @@ def gen_bounce(hits, seed=3, epsilon=1e-4):
     return out[:k].copy()
 
 
+def gen_bounce_n(hits, n, seed=3, epsilon=1e-4):
+    """Exactly n cosine-hemisphere bounce rays from the hit points of `hits`:
+    one per hit in order, then again from the first hit with the next seed,
+    until n."""
+    parts, k, c = [], 0, 0
+    while k < n:
+        b = gen_bounce(hits, seed=seed + c, epsilon=epsilon)
+        if len(b) == 0:
+            raise ValueError("no hit to bounce from")
+        parts.append(b[: n - k])
+        k += len(parts[-1])
+        c += 1
+    return np.concatenate(parts)
+
+
 def gen_displaced_sphere(num_phi, seed=1, amplitude=0.1):
     nv = ctypes.c_uint(0)
     nt = lib().rtcgGenDisplacedSphere(num_phi, seed, amplitude, None, None, ctypes.byref(nv))

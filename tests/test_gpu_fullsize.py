@@ -44,9 +44,10 @@ def _trace_device(rtc, device, scene, rays, occluded=False):
 
 
 def test_c5_instanced_full_size(rtc, device, oracle):
-    """10,000 instances of five base meshes (2.5 M triangles, 5 G instanced),
-    8192 x 4096 camera rays closest hit and one shadow ray per camera ray
-    (2^25) towards the point light (0, 100, 0); 65,536 of each checked."""
+    """10,000 instances of five base meshes (2.5 M triangles, 5 G instanced);
+    closest hit: 4096 x 4096 camera rays and 2^24 cosine bounce rays from
+    their hit points; one shadow ray per closest-hit ray (2^25) towards the
+    point light (0, 100, 0): 2^26 rays; 65,536 of each kind checked."""
     meshes = rtc.instance_workload(seed=11)
     objs = [rtc.Scene(device, [m]).commit() for m in meshes]
     T = rtc.random_transforms(10000, seed=5, scale=(0.5, 2.0), extent=50.0)
@@ -55,24 +56,28 @@ def test_c5_instanced_full_size(rtc, device, oracle):
     for k in range(10000):
         assert top.add_instance(objs[which[k]], T[k]) == k
     top.commit()
-    rays = rtc.gen_camera(8192, 4096, org=(0.0, 20.0, -160.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0)
-    assert len(rays) == 1 << 25
-    hits = _trace_device(rtc, device, top, rays)
+    cam = rtc.gen_camera(4096, 4096, org=(0.0, 20.0, -160.0), target=(0.0, 0.0, 0.0), up=(0.0, 1.0, 0.0), fov=45.0)
+    assert len(cam) == 1 << 24
+    cam_hits = _trace_device(rtc, device, top, cam)
+    bounce = rtc.gen_bounce_n(cam_hits, 1 << 24, seed=31)
+    bounce_hits = _trace_device(rtc, device, top, bounce)
+    hits = np.concatenate([cam_hits, bounce_hits])
     sh = rtc.gen_shadow_rays(hits, light=(0.0, 100.0, 0.0), centre=(0.0, 0.0, 0.0))
     assert len(sh) == 1 << 25
     shadow = _trace_device(rtc, device, top, sh, occluded=True)
     hit_rate, shadowed = (hits["geomID"] != INV).mean(), (shadow["tfar"] < 0).mean()
-    say(f"C5: hit rate {hit_rate:.4f}, shadowed {shadowed:.4f}")
+    say(f"C5: hit rate {hit_rate:.4f} (camera {(cam_hits['geomID'] != INV).mean():.4f}), shadowed {shadowed:.4f}")
     assert 0.2 < hit_rate < 0.9 and 0.05 < shadowed < 0.95
     I = oracle.Instances()
     oid = [I.add_object([m]) for m in meshes]
     for k in range(10000):
         I.add_instance(oid[which[k]], T[k], k)
     rng = np.random.default_rng(501)
-    pick = np.sort(rng.choice(len(rays), 65536, replace=False))
-    ref = rays[pick].copy()
-    I.intersect_bvh(ref)
-    _same(hits[pick], ref, "C5 camera rays")
+    for name, src, out in (("camera", cam, cam_hits), ("bounce", bounce, bounce_hits)):
+        pick = np.sort(rng.choice(len(src), 65536, replace=False))
+        ref = src[pick].copy()
+        I.intersect_bvh(ref)
+        _same(out[pick], ref, f"C5 {name} rays")
     spick = np.sort(rng.choice(len(sh), 65536, replace=False))
     sref = sh[spick].copy()
     I.occluded_bvh(sref)
