@@ -54,6 +54,9 @@ struct ThreadDev {
   rtg::DevBuf staging;
   rtg::DevBuf errword;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // host records: two pinned chunks the queries of host memory move through
+  rtg::PinnedBuf stage[2];
+  hipEvent_t ev_stage[2] = {nullptr, nullptr};
   rtg::RaySorter sorter;
   rtg::DevBuf batch, batch_counter;  // batched traversal: stream table, work counter
 };
@@ -85,6 +88,8 @@ struct Device : RefObj {
       if (t->stream) (void)hipStreamDestroy(t->stream);
       if (t->ev0) (void)hipEventDestroy(t->ev0);
       if (t->ev1) (void)hipEventDestroy(t->ev1);
+      for (hipEvent_t e : t->ev_stage)
+        if (e) (void)hipEventDestroy(e);
       delete t;
     }
   }
@@ -97,6 +102,7 @@ struct Device : RefObj {
     RTG_HIP(hipStreamCreateWithFlags(&t->stream, hipStreamNonBlocking));
     RTG_HIP(hipEventCreate(&t->ev0));
     RTG_HIP(hipEventCreate(&t->ev1));
+    for (hipEvent_t& e : t->ev_stage) RTG_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     t->errword.reserve(256);
     RTG_HIP(hipMemsetAsync(t->errword.ptr, 0, 4, t->stream));
     threads[std::this_thread::get_id()] = t;
@@ -787,6 +793,35 @@ void check_trace_error(ThreadDev& t) {
 }
 
 // Stream query over host or device records (reference rtcore.cpp:595-623, 848-874).
+// Host records per chunk of a query in host memory (80 MB of RTCRayHit).
+constexpr size_t HOST_CHUNK_RAYS = size_t(1) << 20;
+
+// Copies n records of rec bytes from src (stride sstride) to dst (stride
+// dstride), on several host threads when there are many.
+void copy_records(void* dst, size_t dstride, const void* src, size_t sstride, size_t n, size_t rec) {
+  auto part = [&](size_t a, size_t b) {
+    if (dstride == rec && sstride == rec) {
+      memcpy((char*)dst + a * rec, (const char*)src + a * rec, (b - a) * rec);
+    } else {
+      for (size_t i = a; i < b; i++) memcpy((char*)dst + i * dstride, (const char*)src + i * sstride, rec);
+    }
+  };
+  const unsigned hw = std::thread::hardware_concurrency();
+  const size_t nt = std::min<size_t>({8, hw > 1 ? hw / 2 : 1, std::max<size_t>(1, n * rec / (4u << 20))});
+  if (nt <= 1) {
+    part(0, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const size_t per = (n + nt - 1) / nt;
+  for (size_t j = 1; j < nt; j++) {
+    const size_t a = std::min(n, j * per), b = std::min(n, (j + 1) * per);
+    if (a < b) th.emplace_back(part, a, b);
+  }
+  part(0, std::min(n, per));
+  for (auto& x : th) x.join();
+}
+
 void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigned M, size_t stride, bool occluded) {
   if (!sc->committed) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "scene not committed");
   if (ctx && ctx->filter) RTC_THROW(RTC_ERROR_INVALID_OPERATION, "filter functions are not supported");
@@ -804,26 +839,32 @@ void trace_stream(Scene* sc, const RTCIntersectContext* ctx, void* rays, unsigne
     check_trace_error(t);
     return;
   }
-  // host records: pack, upload, trace, download, unpack
+  // Host records move in chunks through two pinned buffers: host threads
+  // pack chunk k into one while the GPU uploads, traces and downloads chunk
+  // k - 1 through the other, and unpack a chunk once its download is done.
+  // (Pageable memory cannot be registered for DMA here, so every byte goes
+  // through a host copy; several threads copy faster than one.)
+  const size_t chunk = std::min<size_t>(M, HOST_CHUNK_RAYS);
+  const size_t nchunks = (M + chunk - 1) / chunk;
   t.staging.reserve((size_t)M * rec);
-  if (stride == rec) {
-    RTG_HIP(hipMemcpyAsync(t.staging.ptr, rays, (size_t)M * rec, hipMemcpyHostToDevice, t.stream));
-  } else {
-    std::vector<char> pack((size_t)M * rec);
-    for (size_t i = 0; i < M; i++) memcpy(&pack[i * rec], (const char*)rays + i * stride, rec);
-    RTG_HIP(hipMemcpyAsync(t.staging.ptr, pack.data(), pack.size(), hipMemcpyHostToDevice, t.stream));
-    RTG_HIP(hipStreamSynchronize(t.stream));
+  for (auto& b : t.stage) b.reserve(chunk * rec);
+  const rtg::TraceParams tp = d->trace_params();
+  auto unpack = [&](size_t k) {
+    const size_t first = k * chunk, n = std::min<size_t>(chunk, M - first);
+    RTG_HIP(hipEventSynchronize(t.ev_stage[k & 1]));
+    copy_records((char*)rays + first * stride, stride, t.stage[k & 1].ptr, rec, n, rec);
+  };
+  for (size_t k = 0; k < nchunks; k++) {
+    if (k >= 2) unpack(k - 2);  // frees this chunk's pinned buffer
+    const size_t first = k * chunk, n = std::min<size_t>(chunk, M - first);
+    char* dev = t.staging.as<char>() + first * rec;
+    copy_records(t.stage[k & 1].ptr, rec, (const char*)rays + first * stride, stride, n, rec);
+    RTG_HIP(hipMemcpyAsync(dev, t.stage[k & 1].ptr, n * rec, hipMemcpyHostToDevice, t.stream));
+    launch_scene(sc, t, dev, (unsigned)n, rec, occluded, inst, t.stream, tp);
+    RTG_HIP(hipMemcpyAsync(t.stage[k & 1].ptr, dev, n * rec, hipMemcpyDeviceToHost, t.stream));
+    RTG_HIP(hipEventRecord(t.ev_stage[k & 1], t.stream));
   }
-  launch_scene(sc, t, t.staging.ptr, M, rec, occluded, inst, t.stream, d->trace_params());
-  if (stride == rec) {
-    RTG_HIP(hipMemcpyAsync(rays, t.staging.ptr, (size_t)M * rec, hipMemcpyDeviceToHost, t.stream));
-    RTG_HIP(hipStreamSynchronize(t.stream));
-  } else {
-    std::vector<char> pack((size_t)M * rec);
-    RTG_HIP(hipMemcpyAsync(pack.data(), t.staging.ptr, pack.size(), hipMemcpyDeviceToHost, t.stream));
-    RTG_HIP(hipStreamSynchronize(t.stream));
-    for (size_t i = 0; i < M; i++) memcpy((char*)rays + i * stride, &pack[i * rec], rec);
-  }
+  for (size_t k = nchunks >= 2 ? nchunks - 2 : 0; k < nchunks; k++) unpack(k);
   check_trace_error(t);
 }
 
