@@ -814,11 +814,16 @@ void copy_records(void* dst, size_t dstride, const void* src, size_t sstride, si
   }
   std::vector<std::thread> th;
   const size_t per = (n + nt - 1) / nt;
-  for (size_t j = 1; j < nt; j++) {
-    const size_t a = std::min(n, j * per), b = std::min(n, (j + 1) * per);
-    if (a < b) th.emplace_back(part, a, b);
+  size_t j = 1;
+  try {
+    for (; j < nt; j++) {
+      const size_t a = std::min(n, j * per), b = std::min(n, (j + 1) * per);
+      if (a < b) th.emplace_back(part, a, b);
+    }
+  } catch (...) {  // no more threads: this one copies the parts not started
   }
   part(0, std::min(n, per));
+  for (; j < nt; j++) part(std::min(n, j * per), std::min(n, (j + 1) * per));
   for (auto& x : th) x.join();
 }
 
