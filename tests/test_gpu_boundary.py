@@ -102,6 +102,69 @@ def test_concurrent_rtcIntersect1M_from_8_threads(rtc, device, oracle, where):
     sc.release()
 
 
+def test_host_records_in_chunks(rtc, device, oracle):
+    """Host records pass through pinned chunks of 2^20 records: a query of
+    2^21 + 12,345 records (two full chunks and a partial one), contiguous and
+    with a 96 B stride, gives the same bytes as the same query on device
+    records (closest hit and occlusion); a sample equals the oracle."""
+    v, ix = rtc.gen_soup(200000, seed=331, h=0.01)
+    sc = rtc.Scene(device, [(v, ix)]).commit()
+    L = rtc.lib()
+    ctx = rtc.init_context()
+    n = (1 << 21) + 12345
+    rays = rtc.gen_random_rays(n, seed=332)
+    rays["tfar"][1::9] = 0.05
+    rays["tnear"][2::13] = 3.0
+    rays["tfar"][2::13] = 1.0  # tnear > tfar: untouched
+    host = rays.copy()
+    L.rtcIntersect1M(sc.h, ctypes.byref(ctx), host.ctypes.data, n, 80)
+    assert L.rtcGetDeviceError(device.h) == 0
+    d = device.malloc(rays.nbytes)
+    try:
+        device.memcpy(d, rays.ctypes.data, rays.nbytes)
+        L.rtcIntersect1M(sc.h, ctypes.byref(ctx), d, n, 80)
+        assert L.rtcGetDeviceError(device.h) == 0
+        dev_out = np.empty_like(rays)
+        device.memcpy(dev_out.ctypes.data, d, rays.nbytes)
+    finally:
+        device.free(d)
+    assert host.tobytes() == dev_out.tobytes(), "host and device records differ"
+    # strided host records: the padding stays as it was
+    padded = np.dtype({"names": list(rays.dtype.names) + ["pad"],
+                       "formats": [rays.dtype.fields[k][0] for k in rays.dtype.names] + ["<u4"],
+                       "offsets": [rays.dtype.fields[k][1] for k in rays.dtype.names] + [80], "itemsize": 96})
+    pr = np.zeros(n, padded)
+    for k in rays.dtype.names:
+        pr[k] = rays[k]
+    pr["pad"] = 0xDEADBEEF
+    L.rtcIntersect1M(sc.h, ctypes.byref(ctx), pr.ctypes.data, n, 96)
+    assert L.rtcGetDeviceError(device.h) == 0
+    got = np.zeros(n, rays.dtype)
+    for k in rays.dtype.names:
+        got[k] = pr[k]
+    assert got.tobytes() == host.tobytes(), "strided host records differ"
+    assert (pr["pad"] == 0xDEADBEEF).all()
+    # occlusion records
+    occ = np.zeros(n, rtc.RTCRay)
+    for k in rtc.RTCRay.names:
+        occ[k] = rays[k]
+    ho = occ.copy()
+    L.rtcOccluded1M(sc.h, ctypes.byref(ctx), ho.ctypes.data, n, 48)
+    assert L.rtcGetDeviceError(device.h) == 0
+    d = device.malloc(occ.nbytes)
+    try:
+        device.memcpy(d, occ.ctypes.data, occ.nbytes)
+        L.rtcOccluded1M(sc.h, ctypes.byref(ctx), d, n, 48)
+        do = np.empty_like(occ)
+        device.memcpy(do.ctypes.data, d, occ.nbytes)
+    finally:
+        device.free(d)
+    assert ho.tobytes() == do.tobytes(), "host and device occlusion records differ"
+    pick = np.sort(np.random.default_rng(333).choice(n, 20000, replace=False))
+    _same(host[pick], _oracle(oracle, [(v, ix)], rays[pick]), "host records in chunks")
+    sc.release()
+
+
 def _build_kind(rtc, cfg, quality=None):
     d = rtc.Device(cfg)
     v, ix = rtc.gen_soup(30000, seed=321, h=0.01)
