@@ -239,9 +239,9 @@ def learned_vs_sah(grid, width, height, bounces, train_steps, steps):
             "build_ms_plain": round(build_plain, 2), "build_ms_learned": round(build_learned, 2)}
 
 
-def learned_point_source(steps):
+def learned_point_source(steps, width=0.3):
     """A ray distribution far from the uniform one the SAH assumes: 2^22 rays
-    from the point (0.5, 0.5, -2) into the central 0.3-wide cube of the
+    from the point (0.5, 0.5, -2) into the central cube `width` wide of the
     2^20-triangle soup. The split model (200 Adam steps, SAH slack 0.5) is
     trained on the even-indexed rays; both trees are timed and counted on
     the odd-indexed ones (held out)."""
@@ -249,7 +249,7 @@ def learned_point_source(steps):
     v, ix = R.gen_soup(NUM_TRIS, seed=1, h=0.005)
     rng = np.random.default_rng(7)
     nq = 1 << 22
-    tgt = 0.35 + 0.3 * rng.random((nq, 3), dtype=np.float32)
+    tgt = (0.5 - width / 2) + width * rng.random((nq, 3), dtype=np.float32)
     org = np.array([0.5, 0.5, -2.0], np.float32)
     d = tgt - org
     Q = R.new_rayhits(nq)
@@ -289,8 +289,9 @@ def learned_point_source(steps):
     plain.release()
     learned.release()
     dev.release()
-    return {"workload": "2^20-triangle soup, 2^22 rays from (0.5, 0.5, -2) into the central 0.3-wide cube; trained on "
-                        "the even-indexed, timed and counted on the odd-indexed (held out); 200 steps, SAH slack 0.5",
+    return {"workload": f"2^20-triangle soup, 2^22 rays from (0.5, 0.5, -2) into the central {width}-wide cube; "
+                        "trained on the even-indexed, timed and counted on the odd-indexed (held out); 200 steps, "
+                        "SAH slack 0.5",
             "plain_sah": res["plain"], "learned": res["learned"],
             "node_ratio": round(res["learned"]["nodes_per_ray"] / res["plain"]["nodes_per_ray"], 4),
             "speedup": round(res["learned"]["mrays_per_s"] / res["plain"]["mrays_per_s"], 4),
@@ -705,12 +706,14 @@ def main():
     sc.release()
     dev.release()
 
-    learned = learned_ps = None
+    learned = learned_ps = learned_beam = None
     if not args.no_learned:
         learned = learned_vs_sah(args.hf_grid, args.hf_width, args.hf_height, args.hf_depth, args.train_steps,
                                  max(3, args.steps // 2))
         log("learned split cost, point source")
         learned_ps = learned_point_source(max(3, args.steps // 2))
+        log("learned split cost, narrow beam")
+        learned_beam = learned_point_source(max(3, args.steps // 2), width=0.05)
 
     batch = None
     if not args.no_batch:
@@ -759,6 +762,7 @@ def main():
         "cpu_baseline": cpu,
         "learned_vs_sah": learned,
         "learned_point_source": learned_ps,
+        "learned_narrow_beam": learned_beam,
         "batched_scenes": batch,
         "instanced": inst,
     }

@@ -181,21 +181,23 @@ def _counted_nodes(rtc, device, sc, rays):
             device.free(p)
 
 
-def test_held_out_gain_for_a_point_source(rtc, device, oracle):
+@pytest.mark.parametrize("width,bound", [(0.3, 0.9), (0.05, 0.75)])
+def test_held_out_gain_for_a_point_source(rtc, device, oracle, width, bound):
     """Where the rays are far from the uniform distribution the SAH assumes,
     a model trained on half of them builds a tree that the other half (held
-    out) crosses with at least 10 % fewer node visits than the plain
-    binned-SAH tree, with the same records, equal to the oracle's on a
-    sample. Workload: the 2^20-triangle soup, 2^22 rays from the point
-    (0.5, 0.5, -2) into the central 0.3-wide cube; even rays train, odd
-    rays are timed. Training is deterministic (samples in content order,
-    gradients summed in a fixed order), so the ratio is reproducible: 0.826
-    measured with 200 Adam steps at SAH slack 0.5."""
+    out) crosses with fewer node visits than the plain binned-SAH tree (at
+    most `bound` times as many), with the same records, equal to the
+    oracle's on a sample. Workload: the 2^20-triangle soup, 2^22 rays from
+    the point (0.5, 0.5, -2) into the central cube `width` wide; even rays
+    train, odd rays are counted. Training is deterministic (samples in
+    content order, gradients summed in a fixed order), so the ratio is
+    reproducible: 0.80 (width 0.3) and 0.69 (width 0.05) measured with 200
+    Adam steps at SAH slack 0.5."""
     import ctypes
     v, ix = rtc.gen_soup(1 << 20, seed=1, h=0.005)
     rng = np.random.default_rng(7)
     n = 1 << 22
-    tgt = 0.35 + 0.3 * rng.random((n, 3), dtype=np.float32)
+    tgt = (0.5 - width / 2) + width * rng.random((n, 3), dtype=np.float32)
     org = np.array([0.5, 0.5, -2.0], np.float32)
     d = tgt - org
     Q = rtc.new_rayhits(n)
@@ -215,8 +217,9 @@ def test_held_out_gain_for_a_point_source(rtc, device, oracle):
         learned.commit()
         pn, pout = _counted_nodes(rtc, device, plain, held)
         ln, lout = _counted_nodes(rtc, device, learned, held)
-        print(f"point source, held out: plain {pn:.3f} nodes/ray, learned {ln:.3f} ({ln / pn:.4f})")
-        assert ln <= 0.9 * pn, (ln, pn)
+        print(f"point source into a {width}-wide cube, held out: plain {pn:.3f} nodes/ray, learned {ln:.3f} "
+              f"({ln / pn:.4f})")
+        assert ln <= bound * pn, (ln, pn)
         assert np.array_equal(lout.view(np.uint8), pout.view(np.uint8))
         sample = np.ascontiguousarray(held[::512])
         ref = sample.copy()
