@@ -680,6 +680,9 @@ __device__ inline uint32_t quant_hi(float v, float p, float s) {
   return qi;
 }
 
+// Byte k of a node's eight split bytes loaded as one uint2.
+__device__ inline uint32_t split_byte(uint2 s8, uint32_t k) { return ((k < 4 ? s8.x : s8.y) >> (8 * (k & 3))) & 0xffu; }
+
 // Statistics of the nodes a thread wrote, reduced over the wave before
 // the global atomics (one per item would serialise on their addresses).
 struct CollapseStats {
@@ -791,23 +794,28 @@ __device__ inline void collapse_item1(const CollapseArgs& a, uint32_t t, bool ac
     while (sp > 0 && n < 8) {
       sp--;
       uint32_t c = st_ref[sp], slots = st_s[sp];
+      uint2 s8 = make_uint2(0u, 0u), cp = make_uint2(0u, 0u);
       if (!(c & REF_LEAF)) {
-        while (slots > 1 && (a.split8[8 * (size_t)c + slots - 1] & 0x80u)) slots--;
+        // the node's eight split bytes and its child pair: two 8 B loads
+        // issued together, not a byte load per slot count and then the pair
+        s8 = *reinterpret_cast<const uint2*>(a.split8 + 8 * (size_t)c);
+        cp = *reinterpret_cast<const uint2*>(a.child + 2 * (size_t)c);
+        while (slots > 1 && (split_byte(s8, slots - 1) & 0x80u)) slots--;
       } else {
         slots = 1;
       }
       if (slots <= 1) {
         ch[n++] = c;
       } else {
-        const uint32_t jj = a.split8[8 * (size_t)c + slots - 1] & 7u;
+        const uint32_t jj = split_byte(s8, slots - 1) & 7u;
         if (sp + 2 > 8) {
           atomicOr(&a.cnt->error, 128u);
           go = false;
           n = 0;
           break;
         }
-        st_ref[sp] = a.child[2 * c + 1]; st_s[sp] = slots - jj; sp++;
-        st_ref[sp] = a.child[2 * c];     st_s[sp] = jj;          sp++;
+        st_ref[sp] = cp.y; st_s[sp] = slots - jj; sp++;
+        st_ref[sp] = cp.x; st_s[sp] = jj;         sp++;
       }
     }
   }
@@ -818,6 +826,7 @@ __device__ inline void collapse_item1(const CollapseArgs& a, uint32_t t, bool ac
   const uint32_t* rng32 = reinterpret_cast<const uint32_t*>(a.range);
   float bl[8][3], bh[8][3];
   uint32_t ni = 0, nt = 0, ninner_prims = 0;
+  uint32_t innerm = 0;  // bit k: child k is an inner node (kept for the writes below)
   float ul[3] = {INFINITY, INFINITY, INFINITY}, uh[3] = {-INFINITY, -INFINITY, -INFINITY};
   double sah_leaf = 0.0, sah_leaf_tri = 0.0;
 #pragma unroll
@@ -847,6 +856,7 @@ __device__ inline void collapse_item1(const CollapseArgs& a, uint32_t t, bool ac
         uh[d] = fmaxf(uh[d], bh[k][d]);
       }
       if (is_inner(c, a.isleaf)) {
+        innerm |= 1u << k;
         ni++;
         ninner_prims += cnt;
       } else {
@@ -907,7 +917,7 @@ __device__ inline void collapse_item1(const CollapseArgs& a, uint32_t t, bool ac
       q.qlox[k] = (uint8_t)qlx; q.qhix[k] = (uint8_t)qhx;
       q.qloy[k] = (uint8_t)qly; q.qhiy[k] = (uint8_t)qhy;
       q.qloz[k] = (uint8_t)qlz; q.qhiz[k] = (uint8_t)qhz;
-      if (is_inner(c, a.isleaf)) {
+      if ((innerm >> k) & 1u) {
         imask |= 1u << k;
         q.meta[k] = (uint8_t)(0x80u | rank_i);
         a.qout[qbase + rank_i] = QItem{c, child_base + rank_i, item.depth + 1, need};
@@ -1028,8 +1038,13 @@ __device__ inline void collapse_item(const CollapseArgs& a, uint32_t t, bool act
     while (sp > 0 && n < 8) {
       sp--;
       uint32_t c = st_ref[sp], slots = st_s[sp];
+      uint2 s8 = make_uint2(0u, 0u), cp = make_uint2(0u, 0u);
       if (!(c & REF_LEAF)) {
-        while (slots > 1 && (a.split8[8 * (size_t)c + slots - 1] & 0x80u)) slots--;
+        // the node's eight split bytes and its child pair: two 8 B loads
+        // issued together, not a byte load per slot count and then the pair
+        s8 = *reinterpret_cast<const uint2*>(a.split8 + 8 * (size_t)c);
+        cp = *reinterpret_cast<const uint2*>(a.child + 2 * (size_t)c);
+        while (slots > 1 && (split_byte(s8, slots - 1) & 0x80u)) slots--;
       } else {
         slots = 1;
       }
@@ -1037,15 +1052,15 @@ __device__ inline void collapse_item(const CollapseArgs& a, uint32_t t, bool act
         if ((uint32_t)n == k) mine = c;
         n++;
       } else {
-        const uint32_t jj = a.split8[8 * (size_t)c + slots - 1] & 7u;
+        const uint32_t jj = split_byte(s8, slots - 1) & 7u;
         if (sp + 2 > 8) {
           if (k == 0) atomicOr(&a.cnt->error, 128u);
           go = false;
           n = 0;
           break;
         }
-        st_ref[sp] = a.child[2 * c + 1]; st_s[sp] = slots - jj; sp++;
-        st_ref[sp] = a.child[2 * c];     st_s[sp] = jj;          sp++;
+        st_ref[sp] = cp.y; st_s[sp] = slots - jj; sp++;
+        st_ref[sp] = cp.x; st_s[sp] = jj;         sp++;
       }
     }
   }
