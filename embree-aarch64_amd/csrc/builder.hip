@@ -1229,7 +1229,7 @@ __global__ __launch_bounds__(BT) void k_write_leaves(const uint32_t* __restrict_
 // One level of the collapse: the grid strides over the level's queue, whose
 // length it reads on the device. A tree that failed its check (error bits
 // set) is not followed.
-__global__ __launch_bounds__(BT) void k_collapse(CollapseArgs a) {
+__global__ __launch_bounds__(BT, 3) void k_collapse(CollapseArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *a.qclear = 0u;
   // the workgroup decides together (block_alloc synchronises it)
   __shared__ uint32_t s_run;

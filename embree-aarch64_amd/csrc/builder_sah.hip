@@ -847,7 +847,7 @@ __global__ __launch_bounds__(TBS) void k_big_bins(const uint32_t* __restrict__ s
 }
 
 template <bool LEARNED>
-__global__ __launch_bounds__(TBS) void k_big_split(const Seg* __restrict__ segs, const uint32_t* __restrict__ nseg,
+__global__ __launch_bounds__(TBS, 4) void k_big_split(const Seg* __restrict__ segs, const uint32_t* __restrict__ nseg,
                                                    const uint32_t* __restrict__ gbins, Split* __restrict__ splits,
                                                    Lists nx, Tree t, LearnedCost lc) {
   __shared__ ChooseLds L;
@@ -1119,7 +1119,7 @@ __device__ inline Split split_segment(const Seg& sg, const uint32_t* __restrict_
 // One workgroup per mid segment of a level (between SMALL_MAX and MID_MAX
 // references).
 template <bool LEARNED>
-__global__ __launch_bounds__(256) void k_seg(const Seg* __restrict__ segs, const uint32_t* __restrict__ nseg,
+__global__ __launch_bounds__(256, 4) void k_seg(const Seg* __restrict__ segs, const uint32_t* __restrict__ nseg,
                                              const uint32_t* __restrict__ idx, const float4* __restrict__ blo,
                                              const float4* __restrict__ bhi, uint32_t* __restrict__ idx_o,
                                              float4* __restrict__ blo_o, float4* __restrict__ bhi_o, Lists nx,
